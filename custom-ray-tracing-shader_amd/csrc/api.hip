@@ -1,0 +1,633 @@
+// api.hip - the plugin's C ABI: session state, validation, the GPU frame driver and the
+// CPU backends.  Entry points keep the reference plugin's names, argument meaning and
+// 1 = success / 0 = failure contract (Native/renderer.cu:309-458).
+#include <chrono>
+#include <cstring>
+#include <string>
+#include <vector>
+#include <dlfcn.h>
+#include <omp.h>
+#include "kernels.h"
+#include "traverse.h"
+#include "bvh_wide.h"
+
+using namespace rt;
+
+namespace {
+
+struct HostSum { f3 rad, alb, nrm; float covered, depth; };
+
+struct Session {
+    std::string hostError;
+    hipError_t error = hipSuccess;
+    // scene (colours linearised), light CDF, BVH
+    std::vector<Triangle> tris;
+    std::vector<Light> lights;
+    float lightArea = 0;
+    std::vector<RtBvhNode> binNodes;
+    std::vector<int> binOrder;
+    int bvhDepth = 0;
+    float buildMs = 0;
+    int width = 0, height = 0;
+    int bvhWidth = 4;            // option, used at the next RtInitialize
+    int builtWidth = 4;
+    int blocksPerCU = 4;         // persistent traversal workgroups per CU
+    std::vector<unsigned char> hostNodes;
+    std::vector<TriRecord> hostRecs;
+    std::vector<HostSum> hostSum;
+    // device
+    Triangle* dTris = nullptr;
+    Light* dLights = nullptr;
+    void* dNodes = nullptr;
+    TriRecord* dRecs = nullptr;
+    float4 *dSumRad = nullptr, *dSumAlb = nullptr, *dSumNrm = nullptr;
+    uchar4* dPixels = nullptr;
+    float4 *dThr = nullptr, *dRad = nullptr, *dAlb = nullptr, *dNrm = nullptr, *dNee = nullptr;
+    float4 *dRayA = nullptr, *dRayB = nullptr, *dShadow = nullptr;
+    int4* dHits = nullptr;
+    int* dCounters = nullptr;     // kMaxRaysPerPixel chunks x kCounterStride
+    int* dSpill = nullptr;
+    int* dOverflow = nullptr;
+    unsigned long long* dStats = nullptr;
+    long long slotCap = 0;
+    int spillEntries = 0, traceBlocks = 0, traceLanes = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // progressive state (Native/renderer.cu:65-68)
+    int cpuThreads = 0, lastBackend = -1;
+    unsigned samples = 0, version = 0;
+    CameraData lastCamera{};
+    RenderSettings lastSettings{};
+    bool ready = false;
+    unsigned long long lastClosest = 0, lastShadow = 0;
+    // measurement hooks
+    bool stageTiming = false, traceStats = false;
+    std::vector<hipEvent_t> eventPool;
+    std::vector<std::pair<int, int>> timedLaunches;   // (stage, first event index)
+    float stageMs[RT_STAGE_COUNT] = {};
+    int stageLaunches[RT_STAGE_COUNT] = {};
+    unsigned long long traceCounts[4] = {};
+    bool hasFrame = false;
+};
+
+// Brackets one launch with pooled events when stage timing is on.
+struct StageScope {
+    Session* s;
+    int stage, ev;
+    StageScope(Session* s_, int stage_) : s(s_), stage(stage_), ev(-1) {
+        if (!s->stageTiming) return;
+        const int used = (int)s->timedLaunches.size() * 2;
+        if (used + 2 > (int)s->eventPool.size()) return;
+        ev = used;
+        (void)hipEventRecord(s->eventPool[ev], s->stream);
+    }
+    ~StageScope() {
+        if (ev < 0) return;
+        (void)hipEventRecord(s->eventPool[ev + 1], s->stream);
+        s->timedLaunches.emplace_back(stage, ev);
+    }
+};
+
+template <class T> bool devAlloc(Session* s, T*& p, size_t count) {
+    s->error = hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T) > 0 ? count * sizeof(T) : 16);
+    if (s->error != hipSuccess) { p = nullptr; return false; }
+    return true;
+}
+template <class T> bool upload(Session* s, T*& p, const T* host, size_t count) {
+    if (!devAlloc(s, p, count)) return false;
+    if (count == 0) return true;
+    s->error = hipMemcpy(p, host, count * sizeof(T), hipMemcpyHostToDevice);
+    return s->error == hipSuccess;
+}
+template <class T> void devFree(T*& p) { if (p) (void)hipFree(p); p = nullptr; }
+
+void freeDevice(Session* s) {
+    devFree(s->dTris); devFree(s->dLights); devFree(s->dNodes); devFree(s->dRecs);
+    devFree(s->dSumRad); devFree(s->dSumAlb); devFree(s->dSumNrm); devFree(s->dPixels);
+    devFree(s->dThr); devFree(s->dRad); devFree(s->dAlb); devFree(s->dNrm); devFree(s->dNee);
+    devFree(s->dRayA); devFree(s->dRayB); devFree(s->dShadow); devFree(s->dHits);
+    devFree(s->dCounters); devFree(s->dSpill); devFree(s->dOverflow); devFree(s->dStats);
+    for (hipEvent_t e : s->eventPool)
+        if (e) (void)hipEventDestroy(e);
+    s->eventPool.clear();
+    if (s->ev0) (void)hipEventDestroy(s->ev0);
+    if (s->ev1) (void)hipEventDestroy(s->ev1);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+    s->ev0 = s->ev1 = nullptr; s->stream = nullptr;
+}
+
+SceneView deviceScene(const Session* s) {
+    SceneView v; v.tris = s->dTris; v.lights = s->dLights; v.lightCount = (int)s->lights.size(); v.lightArea = s->lightArea;
+    return v;
+}
+
+// Builds the binary BVH and its W-wide collapse, then uploads nodes and records.
+bool buildAccel(Session* s) {
+    auto t0 = std::chrono::steady_clock::now();
+    HostBinaryBvh b;
+    b.bins = 32; b.maxLeaf = 4;
+    b.build(s->tris.data(), (int)s->tris.size());
+    s->binNodes = std::move(b.nodes); s->binOrder = std::move(b.order); s->bvhDepth = b.depth;
+    s->builtWidth = s->bvhWidth;
+    auto collapse = [&](auto tag) {
+        constexpr int W = decltype(tag)::value;
+        WideBuilder<W> wb;
+        wb.build(s->binNodes.data(), s->binOrder.data(), s->tris.data());
+        s->hostNodes.resize(wb.nodes.size() * sizeof(WideNode<W>));
+        std::memcpy(s->hostNodes.data(), wb.nodes.data(), s->hostNodes.size());
+        s->hostRecs = std::move(wb.recs);
+    };
+    if (s->builtWidth == 2) collapse(std::integral_constant<int, 2>());
+    else if (s->builtWidth == 8) collapse(std::integral_constant<int, 8>());
+    else collapse(std::integral_constant<int, 4>());
+    s->buildMs = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    unsigned char* dn = nullptr;
+    if (!upload(s, dn, s->hostNodes.data(), s->hostNodes.size())) return false;
+    s->dNodes = dn;
+    return upload(s, s->dRecs, s->hostRecs.data(), s->hostRecs.size());
+}
+
+TraceArgs baseTraceArgs(Session* s) {
+    TraceArgs a{};
+    a.nodes = s->dNodes; a.recs = s->dRecs;
+    a.spill = s->dSpill; a.spillEntries = s->spillEntries; a.totalLanes = s->traceLanes;
+    a.overflow = s->dOverflow;
+    return a;
+}
+
+// One GPU RtRender: chunks of R sample groups; see kernels.hip for the stage order.
+bool renderGpu(Session* s, const CameraData& cam, const RenderSettings& rs, float* ms) {
+    const int npix = s->width * s->height;
+    const int rpp = rs.raysPerPixel;
+    int chunkR = (int)std::min<long long>(rpp, s->slotCap / npix);
+    if (chunkR < 1) chunkR = 1;
+    hipStream_t st = s->stream;
+    s->timedLaunches.clear();
+    if (s->stageTiming && s->eventPool.empty()) {
+        s->eventPool.resize(4096);
+        for (auto& e : s->eventPool)
+            if ((s->error = hipEventCreate(&e)) != hipSuccess) return false;
+    }
+    if (s->traceStats && (s->error = hipMemsetAsync(s->dStats, 0, 4 * sizeof(unsigned long long), st)) != hipSuccess)
+        return false;
+    if ((s->error = hipEventRecord(s->ev0, st)) != hipSuccess) return false;
+    int chunk = 0;
+    for (int r0 = 0; r0 < rpp; r0 += chunkR, chunk++) {
+        const int R = std::min(chunkR, rpp - r0);
+        int* ctr = s->dCounters + (size_t)chunk * kCounterStride;
+        FrameArgs f{};
+        f.cam = cam; f.rs = rs; f.sc = deviceScene(s);
+        f.w = s->width; f.h = s->height; f.npix = npix; f.R = R; f.sample0 = s->samples + (unsigned)r0;
+        f.thr = s->dThr; f.rad = s->dRad; f.alb = s->dAlb; f.nrm = s->dNrm; f.nee = s->dNee;
+        f.shadow = s->dShadow; f.hits = s->dHits; f.counters = ctr;
+        {
+            StageScope sc(s, RT_STAGE_RESET);
+            if ((s->error = launchResetCounters(ctr, R * npix, st)) != hipSuccess) return false;
+        }
+        f.rayOut = s->dRayA;
+        {
+            StageScope sc(s, RT_STAGE_RAYGEN);
+            if ((s->error = launchRaygen(f, st)) != hipSuccess) return false;
+        }
+        for (int b = 0; b < rs.maxBounces; b++) {
+            float4* in = (b & 1) ? s->dRayB : s->dRayA;
+            float4* out = (b & 1) ? s->dRayA : s->dRayB;
+            TraceArgs ta = baseTraceArgs(s);
+            ta.rays = in; ta.countPtr = ctr + kCntRays + b; ta.work = ctr + kWorkRays + b; ta.hits = s->dHits;
+            ta.stats = s->dStats;
+            {
+                StageScope sc(s, RT_STAGE_CLOSEST);
+                if ((s->error = launchTrace(s->builtWidth, false, s->traceStats, s->traceBlocks, ta, st)) != hipSuccess)
+                    return false;
+            }
+            f.bounce = b; f.rayIn = in; f.rayOut = out;
+            {
+                StageScope sc(s, RT_STAGE_SHADE);
+                if ((s->error = launchShade(f, st)) != hipSuccess) return false;
+            }
+            if (!s->lights.empty()) {
+                TraceArgs sa = baseTraceArgs(s);
+                sa.rays = s->dShadow; sa.countPtr = ctr + kCntShadow + b; sa.work = ctr + kWorkShadow + b;
+                sa.rad = s->dRad; sa.contrib = s->dNee; sa.stats = s->dStats + 2;
+                StageScope sc(s, RT_STAGE_SHADOW);
+                if ((s->error = launchTrace(s->builtWidth, true, s->traceStats, s->traceBlocks, sa, st)) != hipSuccess)
+                    return false;
+            }
+        }
+        AccumArgs a{};
+        a.thr = s->dThr; a.rad = s->dRad; a.alb = s->dAlb; a.nrm = s->dNrm;
+        a.sumRad = s->dSumRad; a.sumAlb = s->dSumAlb; a.sumNrm = s->dSumNrm; a.pixels = s->dPixels;
+        a.npix = npix; a.R = R;
+        a.startFromZero = (s->samples == 0 && r0 == 0) ? 1 : 0;
+        a.writeDisplay = (r0 + R >= rpp) ? 1 : 0;
+        a.invSamples = 1.0f / (float)(s->samples + (unsigned)rpp);
+        a.exposure = rs.exposure;
+        StageScope sc(s, RT_STAGE_ACCUMULATE);
+        if ((s->error = launchAccumulate(a, st)) != hipSuccess) return false;
+    }
+    if ((s->error = hipEventRecord(s->ev1, st)) != hipSuccess) return false;
+    if ((s->error = hipEventSynchronize(s->ev1)) != hipSuccess) return false;
+    if ((s->error = hipEventElapsedTime(ms, s->ev0, s->ev1)) != hipSuccess) return false;
+    std::vector<int> ctrs((size_t)chunk * kCounterStride);
+    int overflow = 0;
+    if ((s->error = hipMemcpy(ctrs.data(), s->dCounters, ctrs.size() * sizeof(int), hipMemcpyDeviceToHost)) != hipSuccess) return false;
+    if ((s->error = hipMemcpy(&overflow, s->dOverflow, sizeof(int), hipMemcpyDeviceToHost)) != hipSuccess) return false;
+    if (overflow) { s->hostError = "traversal stack exceeded its spill ring"; return false; }
+    unsigned long long closest = 0, shadow = 0;
+    for (int c = 0; c < chunk; c++)
+        for (int b = 0; b < rs.maxBounces; b++) {
+            closest += (unsigned)ctrs[(size_t)c * kCounterStride + kCntRays + b];
+            shadow += (unsigned)ctrs[(size_t)c * kCounterStride + kCntShadow + b];
+        }
+    s->lastClosest = closest; s->lastShadow = shadow;
+    for (int k = 0; k < RT_STAGE_COUNT; k++) { s->stageMs[k] = 0; s->stageLaunches[k] = 0; }
+    for (const auto& tl : s->timedLaunches) {
+        float ms2 = 0;
+        if ((s->error = hipEventElapsedTime(&ms2, s->eventPool[tl.second], s->eventPool[tl.second + 1])) != hipSuccess)
+            return false;
+        s->stageMs[tl.first] += ms2;
+        s->stageLaunches[tl.first]++;
+    }
+    if (s->traceStats &&
+        (s->error = hipMemcpy(s->traceCounts, s->dStats, sizeof(s->traceCounts), hipMemcpyDeviceToHost)) != hipSuccess)
+        return false;
+    s->hasFrame = true;
+    return true;
+}
+
+bool validRender(Session* s, const CameraData* camera, const RenderSettings* settings) {
+    return camera && settings && s->ready && settings->raysPerPixel >= 1 && settings->raysPerPixel <= kMaxRaysPerPixel &&
+           settings->maxBounces >= 1 && settings->maxBounces <= kMaxBounces && settings->exposure > 0 &&
+           settings->aperture >= 0 && settings->focusDistance > 0 && settings->skyStrength >= 0;
+}
+
+// Native/renderer.cu:289-302 on the host, over the same wide BVH as the GPU.
+template <int W>
+void renderCpuRows(Session* s, const CameraData& cam, const RenderSettings& rs, unsigned char* out, bool multi,
+                   unsigned long long& closest, unsigned long long& shadow) {
+    const WideNode<W>* nodes = reinterpret_cast<const WideNode<W>*>(s->hostNodes.data());
+    const TriRecord* recs = s->hostRecs.data();
+    SceneView sc; sc.tris = s->tris.data(); sc.lights = s->lights.data(); sc.lightCount = (int)s->lights.size(); sc.lightArea = s->lightArea;
+    const int w = s->width, h = s->height;
+    const unsigned* dirs = hostSobolDirections();
+    const unsigned sample = s->samples;
+    unsigned long long cc = 0, ss = 0;
+    #pragma omp parallel if (multi) reduction(+ : cc, ss)
+    {
+        #pragma omp single
+        s->cpuThreads = omp_get_num_threads();
+        #pragma omp for schedule(static)
+        for (int y = 0; y < h; y++)
+            for (int x = 0; x < w; x++) {
+                const size_t i = (size_t)y * w + x;
+                HostSum tot{};
+                if (sample) tot = s->hostSum[i];
+                for (int r = 0; r < rs.raysPerPixel; r++) {
+                    const unsigned smp = sample + (unsigned)r, pix = (unsigned)(y * w + x);
+                    unsigned base = 0;
+                    auto rng = [&](int k) { return sobolSample(dirs, pix, smp, rs.seed, base + (unsigned)k); };
+                    f3 o, d;
+                    cameraRay(cam, rs, x, y, w, h, rng, o, d);
+                    PathState ps; ps.thr = F3(1, 1, 1); ps.prevPdf = 0; ps.rad = F3(0, 0, 0);
+                    FirstHit fh; fh.covered = false; fh.albedo = fh.normal = F3(0, 0, 0); fh.depth = 0;
+                    for (int b = 0; b < rs.maxBounces; b++) {
+                        base = 4 + 8 * (unsigned)b;
+                        RtHit hit = traverseHost<W, false>(nodes, recs, o, d, 1e30f);
+                        cc++;
+                        ShadeResult res;
+                        shadeBounce(sc, rs, b, rng, o, d, hit, ps, &fh, res);
+                        if (res.shadow) {
+                            ss++;
+                            if (traverseHost<W, true>(nodes, recs, res.nextO, res.sdir, res.slimit).index < 0)
+                                ps.rad = ps.rad + res.contrib;
+                        }
+                        if (!res.cont) break;
+                        o = res.nextO; d = res.nextD;
+                    }
+                    tot.rad = tot.rad + ps.rad;
+                    tot.alb = tot.alb + fh.albedo;
+                    tot.nrm = tot.nrm + fh.normal;
+                    tot.covered = tot.covered + (fh.covered ? 1.0f : 0.0f);
+                    tot.depth = tot.depth + fh.depth;
+                }
+                s->hostSum[i] = tot;
+                const float inv = 1.0f / (float)(sample + (unsigned)rs.raysPerPixel);
+                unsigned char* px = out + i * 4;
+                px[0] = displayByte(tot.rad.x * inv, rs.exposure);
+                px[1] = displayByte(tot.rad.y * inv, rs.exposure);
+                px[2] = displayByte(tot.rad.z * inv, rs.exposure);
+                px[3] = 255;
+            }
+    }
+    closest = cc; shadow = ss;
+}
+
+}  // namespace
+
+// ======================================================================== C ABI
+extern "C" {
+
+void* RtCreate(void) { return new Session(); }
+
+void RtReset(void* p) {
+    Session* s = static_cast<Session*>(p);
+    if (s) { s->samples = 0; s->version++; }
+}
+unsigned RtAccumulationVersion(void* p) { Session* s = static_cast<Session*>(p); return s ? s->version : 0; }
+unsigned RtSampleCount(void* p) { Session* s = static_cast<Session*>(p); return s ? s->samples : 0; }
+float RtEmbreeBuildMilliseconds(void* p) { (void)p; return 0; }
+int RtCpuThreadCount(void* p) { Session* s = static_cast<Session*>(p); return s ? s->cpuThreads : 0; }
+
+void RtBvhInfo(void* p, int* nodes, int* depth, float* buildMs) {
+    Session* s = static_cast<Session*>(p);
+    *nodes = s ? (int)s->binNodes.size() : 0;
+    *depth = s ? s->bvhDepth : 0;
+    *buildMs = s ? s->buildMs : 0;
+}
+
+const char* RtLastError(void* p) {
+    Session* s = static_cast<Session*>(p);
+    if (s && !s->hostError.empty()) return s->hostError.c_str();
+    return hipGetErrorString(s ? s->error : hipErrorInvalidValue);
+}
+
+void RtShutdown(void* p) {
+    Session* s = static_cast<Session*>(p);
+    if (!s) return;
+    s->hostError.clear();
+    freeDevice(s);
+    s->ready = false;
+    s->tris.clear(); s->lights.clear(); s->binNodes.clear(); s->binOrder.clear();
+    s->hostNodes.clear(); s->hostRecs.clear(); s->hostSum.clear();
+    s->width = s->height = 0; s->samples = 0; s->lightArea = 0; s->buildMs = 0; s->bvhDepth = 0; s->lastBackend = -1;
+    s->hasFrame = false;
+}
+
+void RtDestroy(void* p) {
+    Session* s = static_cast<Session*>(p);
+    if (s) { RtShutdown(s); delete s; }
+}
+
+int RtInitialize(void* p, const Triangle* triangles, int count, int w, int h) {
+    Session* s = static_cast<Session*>(p);
+    if (!s) return 0;
+    RtShutdown(s);
+    if (!triangles || count <= 0 || w <= 0 || h <= 0 || w > 4096 || h > 4096) { s->error = hipErrorInvalidValue; return 0; }
+    int device = 0, cus = 0;
+    if ((s->error = hipGetDevice(&device)) != hipSuccess) return 0;
+    if ((s->error = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device)) != hipSuccess) return 0;
+    s->tris.assign(triangles, triangles + count);
+    s->width = w; s->height = h;
+    s->hostSum.assign((size_t)w * h, HostSum{});
+    // Native/renderer.cu:345-353: linear colours, emitter CDF by FP32 running area sum.
+    for (int i = 0; i < count; i++) {
+        Triangle& t = s->tris[i];
+        t.color = RtVec3{srgbToLinear(t.color.x), srgbToLinear(t.color.y), srgbToLinear(t.color.z)};
+        t.specularColor = RtVec3{srgbToLinear(t.specularColor.x), srgbToLinear(t.specularColor.y), srgbToLinear(t.specularColor.z)};
+        if (t.emission > 0) {
+            f3 an = crossv(V(t.edge1), V(t.edge2));
+            s->lightArea += .5f * sqrtf(dotv(an, an));
+            s->lights.push_back(Light{i, s->lightArea});
+        }
+    }
+    if ((s->error = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking)) != hipSuccess) return 0;
+    if ((s->error = hipEventCreate(&s->ev0)) != hipSuccess) return 0;
+    if ((s->error = hipEventCreate(&s->ev1)) != hipSuccess) return 0;
+    if (!buildAccel(s)) return 0;
+    if (!upload(s, s->dTris, s->tris.data(), s->tris.size())) return 0;
+    if (!upload(s, s->dLights, s->lights.data(), s->lights.size())) return 0;
+    const size_t npix = (size_t)w * h;
+    s->slotCap = std::max<long long>((long long)npix, std::min<long long>((long long)npix * kMaxRaysPerPixel, 1LL << 24));
+    const size_t slots = (size_t)s->slotCap;
+    if (!devAlloc(s, s->dSumRad, npix) || !devAlloc(s, s->dSumAlb, npix) || !devAlloc(s, s->dSumNrm, npix) ||
+        !devAlloc(s, s->dPixels, npix) || !devAlloc(s, s->dThr, slots) || !devAlloc(s, s->dRad, slots) ||
+        !devAlloc(s, s->dAlb, slots) || !devAlloc(s, s->dNrm, slots) || !devAlloc(s, s->dNee, slots) ||
+        !devAlloc(s, s->dRayA, 2 * slots) || !devAlloc(s, s->dRayB, 2 * slots) || !devAlloc(s, s->dShadow, 2 * slots) ||
+        !devAlloc(s, s->dHits, slots) || !devAlloc(s, s->dCounters, (size_t)kMaxRaysPerPixel * kCounterStride) ||
+        !devAlloc(s, s->dOverflow, 1) || !devAlloc(s, s->dStats, 4))
+        return 0;
+    s->traceBlocks = cus * s->blocksPerCU;
+    s->traceLanes = s->traceBlocks * kBlock;
+    s->spillEntries = 36 * (s->builtWidth - 1);
+    if (!devAlloc(s, s->dSpill, (size_t)s->spillEntries * s->traceLanes * 2)) return 0;
+    if ((s->error = hipMemset(s->dOverflow, 0, sizeof(int))) != hipSuccess) return 0;
+    if ((s->error = hipMemset(s->dSumRad, 0, npix * sizeof(float4))) != hipSuccess) return 0;
+    s->ready = true;
+    return 1;
+}
+
+int RtRender(void* p, const CameraData* camera, const RenderSettings* settings, unsigned char* output, int outputBytes,
+             int backend, float* computeMs) {
+    Session* s = static_cast<Session*>(p);
+    if (!s) return 0;
+    if (!camera || !settings || !output || !computeMs || !s->ready || backend < 0 || backend > 3 || outputBytes < 0 ||
+        (size_t)outputBytes < (size_t)s->width * s->height * 4 || settings->raysPerPixel < 1 ||
+        settings->raysPerPixel > kMaxRaysPerPixel || settings->maxBounces < 1 || settings->maxBounces > kMaxBounces ||
+        !(settings->exposure > 0) || !(settings->aperture >= 0) || !(settings->focusDistance > 0) ||
+        !(settings->skyStrength >= 0)) {
+        s->error = hipErrorInvalidValue;
+        return 0;
+    }
+    s->hostError.clear();
+    if (backend == RT_BACKEND_EMBREE) {
+        // Backend 3 wraps Embree 3 (libembree3.so.3), which is not part of this build's
+        // environment; fail like the reference does when the library cannot be loaded.
+        s->hostError = "Embree backend unavailable: libembree3.so.3 is not installed";
+        return 0;
+    }
+    if (s->lastBackend != backend || std::memcmp(camera, &s->lastCamera, sizeof(CameraData)) ||
+        std::memcmp(settings, &s->lastSettings, sizeof(RenderSettings)))
+        RtReset(s);
+    s->lastBackend = backend; s->lastCamera = *camera; s->lastSettings = *settings; s->cpuThreads = 0;
+    if (backend == RT_BACKEND_GPU) {
+        if (!renderGpu(s, *camera, *settings, computeMs)) return 0;
+        s->error = hipMemcpy(output, s->dPixels, (size_t)s->width * s->height * 4, hipMemcpyDeviceToHost);
+        if (s->error != hipSuccess) return 0;
+    } else {
+        auto t0 = std::chrono::steady_clock::now();
+        const bool multi = backend == RT_BACKEND_CPU_MULTI;
+        unsigned long long c = 0, sh = 0;
+        if (s->builtWidth == 2) renderCpuRows<2>(s, *camera, *settings, output, multi, c, sh);
+        else if (s->builtWidth == 8) renderCpuRows<8>(s, *camera, *settings, output, multi, c, sh);
+        else renderCpuRows<4>(s, *camera, *settings, output, multi, c, sh);
+        s->lastClosest = c; s->lastShadow = sh;
+        *computeMs = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+    s->samples += (unsigned)settings->raysPerPixel;
+    s->error = hipSuccess;
+    return 1;
+}
+
+int RtRenderResident(void* p, const CameraData* camera, const RenderSettings* settings, float* computeMs) {
+    Session* s = static_cast<Session*>(p);
+    if (!s) return 0;
+    if (!computeMs || !validRender(s, camera, settings)) { s->error = hipErrorInvalidValue; return 0; }
+    s->hostError.clear();
+    if (s->lastBackend != RT_BACKEND_GPU || std::memcmp(camera, &s->lastCamera, sizeof(CameraData)) ||
+        std::memcmp(settings, &s->lastSettings, sizeof(RenderSettings)))
+        RtReset(s);
+    s->lastBackend = RT_BACKEND_GPU; s->lastCamera = *camera; s->lastSettings = *settings; s->cpuThreads = 0;
+    if (!renderGpu(s, *camera, *settings, computeMs)) return 0;
+    s->samples += (unsigned)settings->raysPerPixel;
+    s->error = hipSuccess;
+    return 1;
+}
+
+int RtCopyOutput(void* p, unsigned char* output, int outputBytes) {
+    Session* s = static_cast<Session*>(p);
+    if (!s) return 0;
+    if (!s->ready || !s->hasFrame || !output || outputBytes < 0 || (size_t)outputBytes < (size_t)s->width * s->height * 4) {
+        s->error = hipErrorInvalidValue;
+        return 0;
+    }
+    s->error = hipMemcpy(output, s->dPixels, (size_t)s->width * s->height * 4, hipMemcpyDeviceToHost);
+    return s->error == hipSuccess ? 1 : 0;
+}
+
+int RtStageTimes(void* p, float* milliseconds, int* launches, int count) {
+    Session* s = static_cast<Session*>(p);
+    if (!s || count < 0) return 0;
+    for (int k = 0; k < count && k < RT_STAGE_COUNT; k++) {
+        if (milliseconds) milliseconds[k] = s->stageMs[k];
+        if (launches) launches[k] = s->stageLaunches[k];
+    }
+    return 1;
+}
+
+int RtTraceStats(void* p, unsigned long long* counts4) {
+    Session* s = static_cast<Session*>(p);
+    if (!s || !counts4) return 0;
+    for (int k = 0; k < 4; k++) counts4[k] = s->traceCounts[k];
+    return 1;
+}
+
+int RtDenoise(void* p, unsigned char* output, int outputBytes, float* milliseconds) {
+    Session* s = static_cast<Session*>(p);
+    if (!s) return 0;
+    if (!s->ready || !s->samples || !output || !milliseconds || outputBytes < 0 ||
+        (size_t)outputBytes < (size_t)s->width * s->height * 4) {
+        s->error = hipErrorInvalidValue;
+        return 0;
+    }
+    s->hostError = "denoiser weights not loaded";
+    return 0;
+}
+
+// ------------------------------------------------------------------ extensions
+int RtTraceRays(void* p, const RtRay* rays, int count, int anyHit, RtHit* hits) {
+    Session* s = static_cast<Session*>(p);
+    if (!s) return 0;
+    if (!s->ready || !rays || !hits || count < 0) { s->error = hipErrorInvalidValue; return 0; }
+    if (count == 0) return 1;
+    float4* dr = nullptr; int4* dh = nullptr; int* work = nullptr;
+    bool ok = upload(s, dr, reinterpret_cast<const float4*>(rays), (size_t)count * 2) && devAlloc(s, dh, (size_t)count) &&
+              devAlloc(s, work, 1);
+    if (ok) ok = (s->error = hipMemsetAsync(work, 0, sizeof(int), s->stream)) == hipSuccess;
+    if (ok) {
+        TraceArgs a = baseTraceArgs(s);
+        a.rays = dr; a.count = count; a.work = work; a.hits = dh;
+        ok = (s->error = launchTrace(s->builtWidth, anyHit != 0, false, s->traceBlocks, a, s->stream)) == hipSuccess;
+    }
+    if (ok) ok = (s->error = hipStreamSynchronize(s->stream)) == hipSuccess;
+    if (ok) ok = (s->error = hipMemcpy(hits, dh, (size_t)count * sizeof(RtHit), hipMemcpyDeviceToHost)) == hipSuccess;
+    int overflow = 0;
+    if (ok) ok = (s->error = hipMemcpy(&overflow, s->dOverflow, sizeof(int), hipMemcpyDeviceToHost)) == hipSuccess;
+    if (ok && overflow) { s->hostError = "traversal stack exceeded its spill ring"; ok = false; }
+    devFree(dr); devFree(dh); devFree(work);
+    return ok ? 1 : 0;
+}
+
+int RtSobolBatch(const unsigned* pixels, const unsigned* samples, int count, unsigned seed, int dimensions, float* out) {
+    if (!pixels || !samples || !out || count < 0 || dimensions < 1 || dimensions > kSobolDims) return 0;
+    if (count == 0) return 1;
+    unsigned *dp = nullptr, *ds = nullptr;
+    float* dout = nullptr;
+    bool ok = hipMalloc(&dp, (size_t)count * 4) == hipSuccess && hipMalloc(&ds, (size_t)count * 4) == hipSuccess &&
+              hipMalloc(&dout, (size_t)count * dimensions * 4) == hipSuccess;
+    if (ok) ok = hipMemcpy(dp, pixels, (size_t)count * 4, hipMemcpyHostToDevice) == hipSuccess &&
+                 hipMemcpy(ds, samples, (size_t)count * 4, hipMemcpyHostToDevice) == hipSuccess;
+    if (ok) ok = launchSobolBatch(dp, ds, count, seed, dimensions, dout, nullptr) == hipSuccess;
+    if (ok) ok = hipMemcpy(out, dout, (size_t)count * dimensions * 4, hipMemcpyDeviceToHost) == hipSuccess;
+    if (dp) (void)hipFree(dp);
+    if (ds) (void)hipFree(ds);
+    if (dout) (void)hipFree(dout);
+    return ok ? 1 : 0;
+}
+
+int RtBvhExport(void* p, RtBvhNode* nodes, int nodeCapacity, int* order, int orderCapacity) {
+    Session* s = static_cast<Session*>(p);
+    if (!s || !s->ready) return 0;
+    if (nodes && nodeCapacity > 0)
+        std::memcpy(nodes, s->binNodes.data(), sizeof(RtBvhNode) * std::min<size_t>(nodeCapacity, s->binNodes.size()));
+    if (order && orderCapacity > 0)
+        std::memcpy(order, s->binOrder.data(), sizeof(int) * std::min<size_t>(orderCapacity, s->binOrder.size()));
+    return (int)s->binNodes.size();
+}
+
+int RtReadAccumulation(void* p, float* out, long long floatCount) {
+    Session* s = static_cast<Session*>(p);
+    if (!s || !s->ready || !out) return 0;
+    const size_t npix = (size_t)s->width * s->height;
+    if (floatCount < (long long)(npix * 10)) { s->error = hipErrorInvalidValue; return 0; }
+    if (s->lastBackend == RT_BACKEND_GPU) {
+        std::vector<float4> r(npix), a(npix), n(npix);
+        if ((s->error = hipMemcpy(r.data(), s->dSumRad, npix * 16, hipMemcpyDeviceToHost)) != hipSuccess ||
+            (s->error = hipMemcpy(a.data(), s->dSumAlb, npix * 16, hipMemcpyDeviceToHost)) != hipSuccess ||
+            (s->error = hipMemcpy(n.data(), s->dSumNrm, npix * 16, hipMemcpyDeviceToHost)) != hipSuccess)
+            return 0;
+        for (size_t i = 0; i < npix; i++) {
+            float* o = out + i * 10;
+            o[0] = r[i].x; o[1] = r[i].y; o[2] = r[i].z; o[3] = a[i].x; o[4] = a[i].y; o[5] = a[i].z;
+            o[6] = n[i].x; o[7] = n[i].y; o[8] = n[i].z; o[9] = r[i].w;
+        }
+    } else {
+        for (size_t i = 0; i < npix; i++) {
+            const HostSum& h = s->hostSum[i];
+            float* o = out + i * 10;
+            o[0] = h.rad.x; o[1] = h.rad.y; o[2] = h.rad.z; o[3] = h.alb.x; o[4] = h.alb.y; o[5] = h.alb.z;
+            o[6] = h.nrm.x; o[7] = h.nrm.y; o[8] = h.nrm.z; o[9] = h.covered;
+        }
+    }
+    return 1;
+}
+
+int RtRayCounts(void* p, unsigned long long* closest, unsigned long long* shadow) {
+    Session* s = static_cast<Session*>(p);
+    if (!s || !closest || !shadow) return 0;
+    *closest = s->lastClosest; *shadow = s->lastShadow;
+    return 1;
+}
+
+int RtSetOption(void* p, const char* key, int value) {
+    Session* s = static_cast<Session*>(p);
+    if (!s || !key) return 0;
+    if (!std::strcmp(key, "bvh_width")) {
+        if (value != 2 && value != 4 && value != 8) return 0;
+        s->bvhWidth = value; return 1;
+    }
+    if (!std::strcmp(key, "blocks_per_cu")) {
+        if (value < 1 || value > 64) return 0;
+        s->blocksPerCU = value; return 1;
+    }
+    if (!std::strcmp(key, "stage_timing")) {
+        if (value != 0 && value != 1) return 0;
+        s->stageTiming = value != 0; return 1;
+    }
+    if (!std::strcmp(key, "trace_stats")) {
+        if (value != 0 && value != 1) return 0;
+        s->traceStats = value != 0; return 1;
+    }
+    return 0;
+}
+
+int RtGetOption(void* p, const char* key) {
+    Session* s = static_cast<Session*>(p);
+    if (!s || !key) return -1;
+    if (!std::strcmp(key, "bvh_width")) return s->ready ? s->builtWidth : s->bvhWidth;
+    if (!std::strcmp(key, "blocks_per_cu")) return s->blocksPerCU;
+    if (!std::strcmp(key, "stage_timing")) return s->stageTiming ? 1 : 0;
+    if (!std::strcmp(key, "trace_stats")) return s->traceStats ? 1 : 0;
+    return -1;
+}
+
+}  // extern "C"

@@ -1,0 +1,65 @@
+// kernels.h - launch interface of the gfx950 kernels (defined in kernels.hip).
+#pragma once
+#include "rt_core.h"
+
+namespace rt {
+
+constexpr int kBlock = 256;       // threads per workgroup for every kernel here
+constexpr int kShortStack = 8;    // LDS stack entries per traversal lane
+constexpr int kCounterStride = 4 * (kMaxBounces + 1);   // ints per chunk of frame counters
+// Frame counter layout within one chunk: [0,33) closest-ray queue sizes per bounce,
+// [33,66) shadow queue sizes, [66,99) closest work cursors, [99,132) shadow work cursors.
+constexpr int kCntRays = 0, kCntShadow = kMaxBounces + 1, kWorkRays = 2 * (kMaxBounces + 1),
+              kWorkShadow = 3 * (kMaxBounces + 1);
+
+struct TraceArgs {
+    const float4* rays;      // 2 float4 per ray: (origin, limit), (direction, path slot bits)
+    const int* countPtr;     // device-resident ray count, or null to use count
+    int count;
+    int* work;               // work cursor, starts at 0
+    const void* nodes;       // WideNode<W>[]
+    const TriRecord* recs;
+    int4* hits;              // closest / query mode: RtHit per ray
+    float4* rad;             // shadow mode: radiance += contrib when unoccluded
+    const float4* contrib;
+    int* spill;              // global spill ring: 2 ints per entry, entry-major
+    int spillEntries;        // entries per lane beyond the LDS part
+    int totalLanes;          // launched lanes (spill stride)
+    unsigned long long* stats;   // optional: [0] node visits, [1] triangle tests
+    int* overflow;           // set to 1 if a lane's stack exceeded the spill ring
+};
+
+struct FrameArgs {
+    CameraData cam;
+    RenderSettings rs;
+    SceneView sc;
+    int w, h, npix, R;
+    unsigned sample0;        // sample index of slot group r = 0
+    float4 *thr, *rad, *alb, *nrm, *nee;
+    float4 *rayIn, *rayOut, *shadow;
+    int4* hits;
+    int* counters;           // this chunk's counters
+    int bounce;
+};
+
+struct AccumArgs {
+    float4 *thr, *rad, *alb, *nrm;
+    float4 *sumRad, *sumAlb, *sumNrm;
+    uchar4* pixels;
+    int npix, R;
+    int startFromZero;       // first chunk of a call with no prior samples
+    int writeDisplay;        // last chunk: tone-map into pixels
+    float invSamples, exposure;
+};
+
+// Persistent-lane launch of the traversal kernel for width W (2, 4 or 8).
+hipError_t launchTrace(int width, bool anyHit, bool stats, int blocks, const TraceArgs& a, hipStream_t s);
+hipError_t launchResetCounters(int* counters, int initialRays, hipStream_t s);
+hipError_t launchRaygen(const FrameArgs& f, hipStream_t s);
+hipError_t launchShade(const FrameArgs& f, hipStream_t s);
+hipError_t launchAccumulate(const AccumArgs& a, hipStream_t s);
+hipError_t launchSobolBatch(const unsigned* pixels, const unsigned* samples, int count, unsigned seed, int dims,
+                            float* out, hipStream_t s);
+const unsigned* hostSobolDirections();
+
+}  // namespace rt

@@ -1,0 +1,286 @@
+// rt_core.h - arithmetic shared by the gfx950 kernels and the host (CPU backend) code.
+//
+// Every function here is written so that each float operation is the same IEEE operation,
+// in the same order, as the reference integrator (Native/renderer.cu:74-302); the library
+// is compiled with -ffp-contract=off (the reference uses --fmad=false, Native/build.sh:11),
+// correctly rounded fp32 division/sqrt and denormals kept, so the GPU and the CPU paths
+// produce identical hit records and differ only where libm transcendentals (powf, expf,
+// sinf, cosf) differ by ulps.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cfloat>
+#include <cmath>
+#include <cstdint>
+#include "../../include/rt_abi.h"
+
+#define RT_HD __host__ __device__ __forceinline__
+
+namespace rt {
+
+constexpr float kPi = 3.14159265358979323846f;
+constexpr int kSobolDims = 4 + 8 * 32;   // Native/sampling.cuh:6
+constexpr int kMaxBounces = 32;           // RtRender bound (Native/renderer.cu:379)
+constexpr int kMaxRaysPerPixel = 16;      // RtRender bound (Native/renderer.cu:379)
+constexpr int kBvhDepthLimit = 32;        // forced-leaf depth (Native/bvh.cuh:9,64)
+
+struct f3 { float x, y, z; };
+RT_HD f3 F3(float x, float y, float z) { f3 r; r.x = x; r.y = y; r.z = z; return r; }
+RT_HD f3 V(const RtVec3& a) { return F3(a.x, a.y, a.z); }
+RT_HD f3 operator+(f3 a, f3 b) { return F3(a.x + b.x, a.y + b.y, a.z + b.z); }
+RT_HD f3 operator-(f3 a, f3 b) { return F3(a.x - b.x, a.y - b.y, a.z - b.z); }
+RT_HD f3 operator*(f3 a, float s) { return F3(a.x * s, a.y * s, a.z * s); }
+RT_HD f3 mulv(f3 a, f3 b) { return F3(a.x * b.x, a.y * b.y, a.z * b.z); }
+RT_HD float dotv(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+RT_HD f3 crossv(f3 a, f3 b) { return F3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
+// Native/renderer.cu:82
+RT_HD f3 unitv(f3 a) { return a * (1 / sqrtf(fmaxf(dotv(a, a), 1e-20f))); }
+
+// ------------------------------------------------------------------ Sobol sampler
+// Native/sampling.cuh:21-25 (integer hash) and :26-35 (digital-shifted Gray-code Sobol).
+RT_HD unsigned hash32(unsigned x) {
+    x ^= x >> 16; x *= 0x7feb352du;
+    x ^= x >> 15; x *= 0x846ca68bu;
+    return x ^ (x >> 16);
+}
+// Per-(seed, dimension) part of the digital shift; uniform across a launch.
+RT_HD unsigned dimShift(unsigned seed, unsigned dim) { return hash32(seed + dim * 0x9e3779b9u); }
+// XOR of the direction numbers picked by gray(sample); uniform across pixels.
+RT_HD unsigned grayTerm(const unsigned* dirs, unsigned dim, unsigned sample) {
+    unsigned acc = 0, g = sample ^ (sample >> 1);
+    for (unsigned k = 0; g != 0; ++k, g >>= 1)
+        if (g & 1u) acc ^= dirs[dim * 32 + k];
+    return acc;
+}
+// XOR is associative, so hash(pixel ^ shift) ^ term equals the reference's incremental XOR.
+RT_HD float sobolFromParts(unsigned pixel, unsigned shift, unsigned term) {
+    return (float)((hash32(pixel ^ shift) ^ term) >> 8) * (1.0f / 16777216.0f);
+}
+RT_HD float sobolSample(const unsigned* dirs, unsigned pixel, unsigned sample, unsigned seed, unsigned dim) {
+    return sobolFromParts(pixel, dimShift(seed, dim), grayTerm(dirs, dim, sample));
+}
+
+// ------------------------------------------------------------------ geometry
+// Native/renderer.cu:99-112: Moller-Trumbore with the reference's acceptance rules.
+RT_HD float intersectTri(f3 o, f3 d, f3 v0, f3 e1, f3 e2, bool oneSided, float& u, float& v) {
+    f3 p = crossv(d, e2);
+    float det = dotv(e1, p);
+    if (fabsf(det) < 1e-7f || (oneSided && det < 0)) return -1;
+    float inv = 1 / det;
+    f3 s = o - v0;
+    u = dotv(s, p) * inv;
+    if (u < 0 || u > 1) return -1;
+    f3 q = crossv(s, e1);
+    v = dotv(d, q) * inv;
+    if (v < 0 || u + v > 1) return -1;
+    float t = dotv(e2, q) * inv;
+    return t > 1e-4f ? t : -1;
+}
+// Native/renderer.cu:142-143: components below FLT_MIN become a containment test.
+RT_HD f3 safeInverse(f3 d) {
+    return F3(fabsf(d.x) < FLT_MIN ? 0 : 1 / d.x, fabsf(d.y) < FLT_MIN ? 0 : 1 / d.y, fabsf(d.z) < FLT_MIN ? 0 : 1 / d.z);
+}
+// Slab interval of one axis; an inverse of 0 turns into [-inf, inf] inside the slab and an
+// empty interval outside it (Native/renderer.cu:120).
+RT_HD void slabAxis(float o, float inv, float lo, float hi, float& tmin, float& tmax) {
+    float a = inv != 0 ? (lo - o) * inv : (o < lo ? INFINITY : -INFINITY);
+    float b = inv != 0 ? (hi - o) * inv : (o > hi ? -INFINITY : INFINITY);
+    tmin = fminf(a, b); tmax = fmaxf(a, b);
+}
+// Entry distance of the box for a ray limited to [0, limit], or +inf on a miss.
+RT_HD float boxEntry(f3 o, f3 inv, float lx, float ly, float lz, float hx, float hy, float hz, float limit) {
+    float ax, bx, ay, by, az, bz;
+    slabAxis(o.x, inv.x, lx, hx, ax, bx);
+    slabAxis(o.y, inv.y, ly, hy, ay, by);
+    slabAxis(o.z, inv.z, lz, hz, az, bz);
+    float nearT = fmaxf(fmaxf(ax, ay), fmaxf(az, 0.0f));
+    float farT = fminf(fminf(bx, by), fminf(bz, limit));
+    return nearT <= farT ? nearT : INFINITY;
+}
+
+// ------------------------------------------------------------------ wide BVH layout
+// A W-wide node is W*32 bytes (64 B for W=2), 64-byte aligned: six W-float planes of child
+// bounds, then W child references and W triangle counts.  Child reference >= 0 is an inner
+// node index; < 0 is a leaf whose first triangle record is ~ref; kEmptyChild is an unused
+// slot (its bounds are +inf so every slab test misses).
+constexpr int kEmptyChild = INT32_MIN;
+template <int W> struct alignas(64) WideNode {
+    float lox[W], hix[W], loy[W], hiy[W], loz[W], hiz[W];
+    int child[W];
+    int count[W];
+};
+static_assert(sizeof(WideNode<2>) == 64 && sizeof(WideNode<4>) == 128 && sizeof(WideNode<8>) == 256, "node sizes");
+
+// Intersection record, 48 bytes: origin/edge1/edge2 exactly as the caller passed them (so the
+// Moller-Trumbore arithmetic is the reference's), original triangle index and flags.
+constexpr int kTriOneSided = 1, kTriLastInLeaf = 2;
+struct alignas(16) TriRecord {
+    float v0x, v0y, v0z; int index;
+    float e1x, e1y, e1z; int flags;
+    float e2x, e2y, e2z; int pad;
+};
+static_assert(sizeof(TriRecord) == 48, "tri record");
+
+// ------------------------------------------------------------------ integrator pieces
+struct Light { int triangle; float cumulativeArea; };   // Native/renderer.cu:32
+
+struct SceneView {
+    const Triangle* tris;   // colours already linear
+    const Light* lights;
+    int lightCount;
+    float lightArea;
+};
+
+// Native/renderer.cu:84,86-90
+RT_HD float srgbToLinear(float c) { return c <= .04045f ? c / 12.92f : powf((c + .055f) / 1.055f, 2.4f); }
+RT_HD unsigned char displayByte(float c, float exposure) {
+    c = 1 - expf(-fmaxf(c, 0) * exposure);
+    c = c <= .0031308f ? 12.92f * c : 1.055f * powf(c, 1 / 2.4f) - .055f;
+    return (unsigned char)(255 * fminf(c, 1) + .5f);
+}
+// Native/renderer.cu:91-95
+RT_HD f3 aroundAxis(f3 axis, float cosine, float angle) {
+    f3 helper = fabsf(axis.y) < .999f ? F3(0, 1, 0) : F3(1, 0, 0);
+    f3 tangent = unitv(crossv(helper, axis));
+    float sine = sqrtf(fmaxf(0, 1 - cosine * cosine));
+    return unitv(axis * cosine + (tangent * (sine * cosf(angle)) + crossv(axis, tangent) * (sine * sinf(angle))));
+}
+RT_HD float powerHeuristic(float a, float b) { return a * a / fmaxf(a * a + b * b, 1e-30f); }
+// Native/renderer.cu:181-192
+RT_HD float glossExponent(const Triangle& t) { float r = fmaxf(1 - t.smoothness, .015f); return fmaxf(0, 2 / (r * r) - 2); }
+RT_HD f3 evalBrdf(const Triangle& t, f3 n, f3 refl, f3 dir, float& pdf) {
+    float c = fmaxf(0, dotv(n, dir)), p = t.specularProbability;
+    pdf = (1 - p) * c / kPi;
+    f3 val = V(t.color) * ((1 - p) / kPi);
+    if (p > 0 && t.smoothness < .999f) {
+        float e = glossExponent(t), lobe = powf(fmaxf(0, dotv(refl, dir)), e);
+        pdf += p * (e + 1) * lobe / (2 * kPi);
+        val = val + V(t.specularColor) * (p * (e + 2) * lobe / (2 * kPi));
+    }
+    return val;
+}
+// Native/renderer.cu:193-196
+RT_HD f3 skyRadiance(f3 d, float strength) {
+    float t = fminf(1, fmaxf(0, d.y));
+    return (F3(.65f, .72f, .82f) * (1 - t) + F3(.15f, .32f, .65f) * t) * strength;
+}
+
+// Native/renderer.cu:198-210: primary ray from camera dimensions 0..3.
+template <class Rng>
+RT_HD void cameraRay(const CameraData& cam, const RenderSettings& rs, int x, int y, int w, int h, const Rng& rng, f3& o, f3& d) {
+    float px = 2 * (x + rng(0)) / w - 1, py = 2 * (y + rng(1)) / h - 1;
+    f3 right = V(cam.right), up = V(cam.up), fwd = V(cam.forward);
+    o = V(cam.origin);
+    d = unitv(fwd + (right * (px * cam.aspect * cam.tanHalfFov) + up * (py * cam.tanHalfFov)));
+    if (cam.orthographic) {
+        o = o + (right * (px * cam.aspect * cam.orthographicSize) + up * (py * cam.orthographicSize));
+        d = fwd;
+    } else if (rs.aperture > 0) {
+        float ang = 2 * kPi * rng(2), rad = .5f * rs.aperture * sqrtf(rng(3));
+        f3 lens = right * (rad * cosf(ang)) + up * (rad * sinf(ang));
+        f3 focus = o + d * (rs.focusDistance / dotv(d, fwd));
+        o = o + lens;
+        d = unitv(focus - o);
+    }
+}
+
+struct PathState {
+    f3 thr; float prevPdf;
+    f3 rad;
+};
+struct FirstHit { f3 albedo; f3 normal; float depth; bool covered; };
+struct ShadeResult {
+    bool cont;        // a next ray (nextO, nextD) follows
+    bool shadow;      // a shadow query (o=nextO, dir=sdir, limit) guards contribution
+    f3 nextO, nextD;
+    f3 sdir; float slimit;
+    f3 contrib;       // added to the path radiance when the shadow query misses
+};
+
+// One bounce of Native/renderer.cu:215-286 after the closest-hit query.  rng(k) returns the
+// sample of dimension 4 + 8*bounce + k.  The next-event contribution is returned instead of
+// added, and the caller adds it when the shadow query reports no occluder; radiance terms
+// are still added in the reference's order because the next bounce starts after that.
+template <class Rng>
+RT_HD void shadeBounce(const SceneView& sc, const RenderSettings& rs, int bounce, const Rng& rng,
+                       f3 o, f3 d, const RtHit& hit, PathState& ps, FirstHit* first, ShadeResult& res) {
+    res.cont = false; res.shadow = false;
+    if (hit.index < 0) {
+        if (bounce > 0) ps.rad = ps.rad + mulv(ps.thr, skyRadiance(d, rs.skyStrength));
+        return;
+    }
+    const Triangle& t = sc.tris[hit.index];
+    f3 g = unitv(crossv(V(t.edge1), V(t.edge2)));
+    if (dotv(g, d) > 0) g = g * -1;
+    if (bounce == 0) {
+        first->covered = true;
+        first->albedo = t.emission > 0 ? F3(1, 1, 1)
+                                       : V(t.color) * (1 - t.specularProbability) + V(t.specularColor) * t.specularProbability;
+        first->normal = g;
+        first->depth = hit.distance;
+    }
+    if (t.emission > 0) {
+        float lp = sc.lightArea > 0 ? hit.distance * hit.distance / (sc.lightArea * fmaxf(1e-6f, -dotv(g, d))) : 0;
+        float w = ps.prevPdf > 0 ? powerHeuristic(ps.prevPdf, lp) : 1;
+        ps.rad = ps.rad + mulv(ps.thr, V(t.color)) * (t.emission * w);
+        return;
+    }
+    f3 n = unitv(V(t.normal0) * (1 - hit.u - hit.v) + (V(t.normal1) * hit.u + V(t.normal2) * hit.v));
+    if (dotv(n, g) < 0) n = n * -1;
+    if (dotv(n, d) >= 0) n = g;
+    if (bounce == 0) first->normal = n;
+    f3 p = o + d * hit.distance;
+    f3 refl = d - n * (2 * dotv(d, n));
+    f3 no = p + g * .0005f;
+    res.nextO = no;
+    if (sc.lightCount > 0) {
+        float sel = rng(0) * sc.lightArea;
+        // First light whose cumulative area exceeds sel: the reference's linear scan
+        // (Native/renderer.cu:250) written as a binary search with the same result.
+        int lo = 0, hi = sc.lightCount - 1;
+        while (lo < hi) {
+            int mid = (lo + hi) >> 1;
+            if (sel >= sc.lights[mid].cumulativeArea) lo = mid + 1; else hi = mid;
+        }
+        const Triangle& L = sc.tris[sc.lights[lo].triangle];
+        float r = sqrtf(rng(1)), vv = rng(2);
+        f3 target = V(L.origin) + (V(L.edge1) * (r * (1 - vv)) + V(L.edge2) * (r * vv));
+        f3 off = target - no;
+        float dist2 = dotv(off, off), dist = sqrtf(dist2);
+        f3 in = off * (1 / dist), ln = unitv(crossv(V(L.edge1), V(L.edge2)));
+        float lc = -dotv(ln, in);
+        if (!L.oneSided) lc = fabsf(lc);
+        float sc2 = fmaxf(0, dotv(n, in));
+        if (lc > 0 && sc2 > 0 && dotv(g, in) > 0) {
+            float pdf, lp = dist2 / (sc.lightArea * lc);
+            f3 f = evalBrdf(t, n, refl, in, pdf);
+            float w = bounce + 1 == rs.maxBounces ? 1 : powerHeuristic(lp, pdf);
+            res.shadow = true;
+            res.sdir = in;
+            res.slimit = dist - .001f;
+            res.contrib = mulv(mulv(ps.thr, f), V(L.color)) * (L.emission * sc2 * w / lp);
+        }
+    }
+    if (bounce + 1 == rs.maxBounces) return;
+    bool spec = rng(3) < t.specularProbability;
+    f3 nd;
+    if (spec && t.smoothness >= .999f) {
+        nd = refl; ps.thr = mulv(ps.thr, V(t.specularColor)); ps.prevPdf = 0;
+    } else {
+        float cosine = spec ? powf(rng(4), 1 / (glossExponent(t) + 1)) : sqrtf(rng(4));
+        nd = aroundAxis(spec ? refl : n, cosine, 2 * kPi * rng(5));
+        if (dotv(g, nd) <= 0 || dotv(n, nd) <= 0) return;
+        float pdf;
+        f3 f = evalBrdf(t, n, refl, nd, pdf);
+        if (pdf < 1e-12f) return;
+        ps.thr = mulv(ps.thr, f * (dotv(n, nd) / pdf)); ps.prevPdf = pdf;
+    }
+    if (bounce >= 3) {
+        float surv = fminf(.95f, fmaxf(ps.thr.x, fmaxf(ps.thr.y, ps.thr.z)));
+        if (rng(6) >= surv) return;
+        ps.thr = ps.thr * (1 / surv);
+    }
+    res.cont = true;
+    res.nextD = nd;
+}
+
+}  // namespace rt

@@ -1,0 +1,146 @@
+/*
+ * rt_abi.h - the plain C ABI of libCudaRayTracer.so, the MI355X (gfx950) path tracer.
+ *
+ * The first part is the drop-in boundary: the same 13 exports, argument meaning, struct
+ * layouts and 1 = success / 0 = failure contract that the reference plugin gives Unity
+ * (P/Invoke, UnityRayTracer/Assets/RayTracing/RayTraceSession.cs:36-63) and the ctypes tools
+ * (Tools/native_api.py:28-43).  Each declaration names the reference interface it replaces.
+ *
+ * The second part holds additive extension exports (ray-query batches, Sobol batches, BVH
+ * dumps, HDR readback, ray counters, tuning knobs).  They have no counterpart in the
+ * reference; the reference's own C++ tests reached the same internals by including
+ * Native/renderer.cu directly (Tools/verify_bvh.cu:3,69-85).
+ *
+ * All structs are packed to 4 bytes and sequential, exactly as the reference's
+ * (Native/renderer.cu:11-31, Tools/native_api.py:6-26).  Colours arrive sRGB-encoded.
+ * Output images are RGBA8, row 0 = bottom row, alpha = 255.
+ */
+#ifndef RT_ABI_H
+#define RT_ABI_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct RtVec3 { float x, y, z; } RtVec3;
+
+/* Native/renderer.cu:11-18; 116 bytes.  vertices = origin, origin+edge1, origin+edge2. */
+typedef struct Triangle {
+    RtVec3 origin, edge1, edge2, color;  /* color: sRGB-encoded albedo */
+    float emission;
+    int oneSided;
+    RtVec3 normal0, normal1, normal2;    /* per-vertex shading normals */
+    int mesh;                            /* carried, unused by the tracer */
+    RtVec3 specularColor;                /* sRGB-encoded */
+    float smoothness, specularProbability;
+} Triangle;
+
+/* Native/renderer.cu:21-24; 64 bytes. */
+typedef struct CameraData {
+    RtVec3 origin, right, up, forward;
+    float tanHalfFov, aspect, orthographicSize;
+    int orthographic;
+} CameraData;
+
+/* Native/renderer.cu:25-30; 28 bytes. */
+typedef struct RenderSettings {
+    int maxBounces;
+    float exposure, aperture, focusDistance, skyStrength;
+    unsigned seed;
+    int raysPerPixel;
+} RenderSettings;
+
+#ifdef __cplusplus
+static_assert(sizeof(Triangle) == 116 && sizeof(CameraData) == 64 && sizeof(RenderSettings) == 28,
+              "ABI struct sizes must match the reference plugin");
+#endif
+
+/* Backends accepted by RtRender (UnityRayTracer/Assets/RayTracing/CudaRenderer.cs:9-15). */
+enum { RT_BACKEND_CPU_SINGLE = 0, RT_BACKEND_CPU_MULTI = 1, RT_BACKEND_GPU = 2, RT_BACKEND_EMBREE = 3 };
+
+/* ---------------------------------------------------------------------------------------
+ * Part 1: the reference's 13 exports (Native/renderer.cu:309-458)
+ * ------------------------------------------------------------------------------------- */
+
+/* Native/renderer.cu:309 - new, empty session. */
+void* RtCreate(void);
+/* Native/renderer.cu:332 - RtShutdown then delete. */
+void RtDestroy(void* session);
+/* Native/renderer.cu:319-331 - free every host/device buffer, keep the object. */
+void RtShutdown(void* session);
+/* Native/renderer.cu:310 - samples = 0, accumulation version + 1. */
+void RtReset(void* session);
+/* Native/renderer.cu:311 */
+unsigned RtAccumulationVersion(void* session);
+/* Native/renderer.cu:312 */
+unsigned RtSampleCount(void* session);
+/* Native/renderer.cu:313 - backend 3 scene build time; 0 when Embree is absent. */
+float RtEmbreeBuildMilliseconds(void* session);
+/* Native/renderer.cu:314 - 1 / OpenMP threads / 0 / Embree threads for backends 0/1/2/3. */
+int RtCpuThreadCount(void* session);
+/* Native/renderer.cu:315-317 - binary BVH node count, depth and GPU build milliseconds. */
+void RtBvhInfo(void* session, int* nodes, int* depth, float* buildMs);
+/* Native/renderer.cu:318 - host error text, else the HIP runtime's error string. */
+const char* RtLastError(void* session);
+/* Native/renderer.cu:340-373 - copy, linearise colours, build light CDF and BVH, upload. */
+int RtInitialize(void* session, const Triangle* triangles, int count, int width, int height);
+/* Native/renderer.cu:374-418 - raysPerPixel progressive samples per pixel, RGBA8 out.
+ * computeMs = device time of the frame's kernels (backend 2) or CPU render loop time. */
+int RtRender(void* session, const CameraData* camera, const RenderSettings* settings,
+             unsigned char* output, int outputBytes, int backend, float* computeMs);
+/* Native/renderer.cu:436-458 - learned denoiser over the accumulated HDR image and its
+ * albedo/normal/depth guides; never feeds back into the accumulation. */
+int RtDenoise(void* session, unsigned char* output, int outputBytes, float* milliseconds);
+
+/* ---------------------------------------------------------------------------------------
+ * Part 2: extension exports (no counterpart in the reference)
+ * ------------------------------------------------------------------------------------- */
+
+/* One ray query: 32 bytes. Hit accepted when 1e-4 < t < limit (Native/renderer.cu:111,153). */
+typedef struct RtRay { float origin[3]; float limit; float direction[3]; int flags; } RtRay;
+/* One hit record: 16 bytes, the reference's Hit (Native/renderer.cu:43); index -1 = miss. */
+typedef struct RtHit { int index; float distance, u, v; } RtHit;
+/* Reference BvhNode layout (Native/bvh.cuh:7): inner nodes have count 0, first = left child. */
+typedef struct RtBvhNode { float low[3]; int first; float high[3]; int count; } RtBvhNode;
+
+/* Closest-hit (anyHit = 0) or any-hit queries against the session's GPU BVH on the GPU. */
+int RtTraceRays(void* session, const RtRay* rays, int count, int anyHit, RtHit* hits);
+/* Scrambled Sobol values computed on the GPU: out[i*dimensions + d] = sample(pixels[i],
+ * samples[i], seed, d) for d < dimensions <= 260 (Native/sampling.cuh:26-35). */
+int RtSobolBatch(const unsigned* pixels, const unsigned* samples, int count, unsigned seed,
+                 int dimensions, float* out);
+/* Copy the binary GPU BVH back in the reference's node layout, plus the primitive order.
+ * Returns the node count (0 on failure); copies min(capacity, size) entries. */
+int RtBvhExport(void* session, RtBvhNode* nodes, int nodeCapacity, int* order, int orderCapacity);
+/* Accumulated HDR sums: per pixel 10 floats = radiance xyz, albedo xyz, normal xyz, covered. */
+int RtReadAccumulation(void* session, float* out, long long floatCount);
+/* Ray queries issued by the last RtRender (closest-hit and shadow). */
+int RtRayCounts(void* session, unsigned long long* closest, unsigned long long* shadow);
+/* Tuning knobs: "bvh_width" (2, 4 or 8) and "blocks_per_cu" (persistent traversal
+ * workgroups per compute unit, 1..64) take effect at the next RtInitialize; "stage_timing"
+ * and "trace_stats" (0/1) at the next render.  Returns 1 if the key is known and the value
+ * valid.  RtGetOption returns -1 for unknown keys. */
+int RtSetOption(void* session, const char* key, int value);
+int RtGetOption(void* session, const char* key);
+
+/* RtRender on the GPU (backend 2) without the RGBA readback: the frame stays in device
+ * memory for GPU-resident consumers; RtCopyOutput copies the last frame to the host. */
+int RtRenderResident(void* session, const CameraData* camera, const RenderSettings* settings,
+                     float* computeMs);
+int RtCopyOutput(void* session, unsigned char* output, int outputBytes);
+
+/* Measurement hooks.  With option "stage_timing" = 1 each GPU kernel launch of RtRender is
+ * bracketed by HIP events on the session's stream; RtStageTimes returns, per stage
+ * (0 reset, 1 raygen, 2 closest-hit traversal, 3 shade, 4 shadow traversal, 5 accumulate),
+ * the summed milliseconds and launch counts of the last frame.  With option "trace_stats"
+ * = 1 the traversal kernels also count node visits and triangle tests; RtTraceStats returns
+ * {closest nodes, closest triangles, shadow nodes, shadow triangles} of the last frame. */
+enum { RT_STAGE_RESET = 0, RT_STAGE_RAYGEN, RT_STAGE_CLOSEST, RT_STAGE_SHADE, RT_STAGE_SHADOW, RT_STAGE_ACCUMULATE,
+       RT_STAGE_COUNT };
+int RtStageTimes(void* session, float* milliseconds, int* launches, int count);
+int RtTraceStats(void* session, unsigned long long* counts4);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RT_ABI_H */

@@ -1,0 +1,220 @@
+"""GPU images through RtRender against the oracle at identical sample indices, and the
+reference's end-to-end ABI known answers (Tools/verify_native.py) on backends 0, 1, 2."""
+import ctypes as c
+import math
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from helpers import cam_record, image_agreement, settings_record, suzanne, vn_quad
+from native_api import Camera, Session, Settings, Vector
+
+pytestmark = pytest.mark.gpu
+
+
+def oracle_frames(tris, cam, st, w, h, calls):
+    sc = oracle.OracleScene(tris)
+    total, sample = None, 0
+    for _ in range(calls):
+        rgba, total, rays = sc.render(cam_record(cam), settings_record(st), w, h, sample, total)
+        sample += st.raysPerPixel
+    return rgba, total, rays
+
+
+def hdr_mean_relative_error(gpu, ref):
+    """Mean over pixels of |gpu - ref| / max(|ref|, 1e-2) on the accumulated radiance."""
+    g, r = gpu[..., :3].astype(np.float64), ref[..., :3].astype(np.float64)
+    return float(np.mean(np.abs(g - r) / np.maximum(np.abs(r), 1e-2)))
+
+
+@pytest.mark.parametrize("calls", [1, 4])
+def test_suzanne_gpu_matches_oracle_and_golden(lib, calls):
+    tris, cam, st, w, h, golden = suzanne()
+    st.seed, st.raysPerPixel = 42, 16
+    with Session(lib) as s:
+        s.initialize(tris, w, h)
+        for _ in range(calls):
+            img, _ = s.render(cam, st, 2)
+        acc = s.accumulation()
+    ref_img, ref_acc, _ = oracle_frames(tris, cam, st, w, h, calls)
+    assert hdr_mean_relative_error(acc, ref_acc) <= 1e-3
+    mae, over1, _ = image_agreement(img, ref_img)
+    assert mae <= 0.5 and over1 <= 0.01
+    assert np.array_equal(acc[..., 9], ref_acc[..., 9])          # covered counts
+    mae, over1, _ = image_agreement(img, golden["sobol16" if calls == 1 else "sobol64"])
+    assert mae <= 0.5 and over1 <= 0.01
+
+
+def test_cornell_gpu_matches_oracle(lib):
+    from scenes import cornell, cornell_camera, make_settings
+    w = h = 128
+    tris, cam, st = cornell(), cornell_camera(w, h), make_settings(4, rpp=16)
+    with Session(lib) as s:
+        s.initialize(tris, w, h)
+        img, _ = s.render(cam, st, 2)
+        acc = s.accumulation()
+        closest, shadow = s.ray_counts()
+    ref_img, ref_acc, (rc, rs) = oracle_frames(tris, cam, st, w, h, 1)
+    assert hdr_mean_relative_error(acc, ref_acc) <= 1e-3
+    assert image_agreement(img, ref_img)[0] <= 0.5
+    assert abs(closest - rc) <= 1e-3 * rc and abs(shadow - rs) <= 1e-3 * rs
+
+
+# ---------------------------------------------------------------- Tools/verify_native.py
+LIT = dict(maxBounces=8, exposure=1, aperture=0, focusDistance=3, skyStrength=0, seed=42, raysPerPixel=1)
+
+
+def lit(**kw):
+    d = dict(LIT, **kw)
+    return Settings(d["maxBounces"], d["exposure"], d["aperture"], d["focusDistance"], d["skyStrength"], d["seed"],
+                    d["raysPerPixel"])
+
+
+class Agree:
+    """verify_native.py:73-95: backends 0, 1 and 2 agree within 1/255; backend 3 (Embree)
+    is reported unavailable instead of rendering."""
+
+    def __init__(self, lib):
+        self.lib = lib
+        self.s = Session(lib)
+
+    def batch(self, cam, st, backend, samples):
+        self.lib.RtReset(self.s.handle)
+        for _ in range(samples):
+            img, ms = self.s.render(cam, st, backend)
+            assert ms >= 0 and math.isfinite(ms)
+        assert self.lib.RtSampleCount(self.s.handle) == samples * st.raysPerPixel
+        return img
+
+    def __call__(self, cam, st, samples=4):
+        cpu = self.batch(cam, st, 0, samples)
+        assert self.lib.RtCpuThreadCount(self.s.handle) == 1
+        multi = self.batch(cam, st, 1, samples)
+        if len(os.sched_getaffinity(0)) > 1:
+            assert self.lib.RtCpuThreadCount(self.s.handle) > 1
+        gpu = self.batch(cam, st, 2, samples)
+        assert self.lib.RtCpuThreadCount(self.s.handle) == 0
+        assert np.abs(cpu.astype(int) - multi).max() <= 1
+        assert np.abs(cpu.astype(int) - gpu).max() <= 1
+        return gpu
+
+
+def centre(img):
+    h, w = img.shape[:2]
+    return tuple(int(x) for x in img[h // 2, w // 2])
+
+
+def test_verify_native_known_answers(lib):
+    agree = Agree(lib)
+    s = agree.s
+    try:
+        for w, h in [(37, 25), (17, 19)]:
+            cam = Camera(Vector(0, 0, -2), Vector(1, 0, 0), Vector(0, 1, 0), Vector(0, 0, 1), .5, w / h, 1, 0)
+            for sided, inside, expected in [(0, False, (208, 0, 0, 255)), (1, False, (0, 0, 0, 255)),
+                                            (1, True, (208, 0, 0, 255))]:
+                red = vn_quad(-1, 1, -1, 1, 0, color=(1, 0, 0), emission=1)
+                red["oneSided"] = sided
+                cam.origin.z = 2 if inside else -2
+                cam.forward.z = -1 if inside else 1
+                s.initialize(red, w, h)
+                img = agree(cam, lit())
+                assert centre(img) == expected
+                assert (img[..., 3] == 255).all()
+        # independent sessions (verify_native.py:135-143)
+        s.initialize(red, 17, 19)
+        reference, _ = s.render(cam, lit())
+        with Session(lib) as other:
+            other.initialize(vn_quad(-1, 1, -1, 1, 0, color=(0, 1, 0), emission=1), 7, 9)
+            ocam = Camera(Vector(0, 0, -2), Vector(1, 0, 0), Vector(0, 1, 0), Vector(0, 0, 1), .5, 7 / 9, 1, 0)
+            assert centre(other.render(ocam, lit())[0]) == (0, 208, 0, 255)
+            lib.RtShutdown(other.handle)
+        assert np.array_equal(s.render(cam, lit())[0], reference)
+
+        w, h = 33, 25
+        cam = Camera(Vector(0, 0, -3), Vector(1, 0, 0), Vector(0, 1, 0), Vector(0, 0, 1), .3, w / h, 1, 0)
+        floor = vn_quad(-2, 2, -2, 2, 0)
+        light = vn_quad(.8, 1.2, -.2, .2, -2, emission=30)
+        blocker = vn_quad(.25, .75, -.5, .5, -1)
+        s.initialize(np.concatenate([floor, light]), w, h)
+        unblocked = agree(cam, lit(), 16)
+        s.initialize(np.concatenate([floor, light, blocker]), w, h)
+        blocked = agree(cam, lit(), 16)
+        assert centre(unblocked)[0] > centre(blocked)[0] + 20
+
+        def accumulate(samples, st):
+            lib.RtReset(s.handle)
+            for _ in range(samples):
+                img, _ = s.render(cam, st)
+            return img
+        rm = lambda a, b: math.sqrt(np.mean((a[..., :3].astype(float) - b[..., :3]) ** 2))
+        one, sixteen, ref = accumulate(1, lit()), accumulate(16, lit()), accumulate(256, lit())
+        assert rm(sixteen, ref) < rm(one, ref) * .6
+
+        mirror = vn_quad(-2, 2, -2, 2, 0)
+        mirror["specularProbability"], mirror["smoothness"] = 1, 1
+        behind = vn_quad(-1, 1, -1, 1, -4, color=(0, 1, 0), emission=2)
+        s.initialize(np.concatenate([mirror, behind]), w, h)
+        img = agree(cam, lit())
+        assert centre(img)[1] > 230 and centre(img)[0] == 0
+
+        s.initialize(floor, w, h)
+        assert centre(agree(cam, lit())) == (0, 0, 0, 255)
+        sky = lit(skyStrength=1)
+        assert centre(agree(cam, sky, 16))[0] > 100
+        cam.origin.x = 20
+        empty = agree(cam, sky)
+        assert (empty[..., :3] == 0).all() and (empty[..., 3] == 255).all()
+        cam.origin.x = 0
+        separate = accumulate(12, sky)
+        for rays in (2, 4, 6):
+            batched = agree(cam, lit(skyStrength=1, raysPerPixel=rays), 12 // rays)
+            assert np.abs(separate.astype(int) - batched).max() <= 1
+            assert lib.RtSampleCount(s.handle) == 12
+        cam.orthographic = 1
+        agree(cam, sky)
+        cam.orthographic = 0
+        agree(cam, lit(skyStrength=1, aperture=.2))
+
+        # reset / auto-reset (verify_native.py:191-205)
+        st = lit(skyStrength=1, aperture=.2)
+        lib.RtReset(s.handle)
+        a, _ = s.render(cam, st)
+        s.render(cam, st)
+        assert lib.RtSampleCount(s.handle) == 2
+        lib.RtReset(s.handle)
+        assert np.array_equal(s.render(cam, st)[0], a) and lib.RtSampleCount(s.handle) == 1
+        cam.origin.x += .1
+        s.render(cam, st)
+        assert lib.RtSampleCount(s.handle) == 1
+        st.exposure = 2
+        s.render(cam, st)
+        assert lib.RtSampleCount(s.handle) == 1
+        s.render(cam, st, 1)
+        assert lib.RtSampleCount(s.handle) == 1
+
+        # invalid arguments (verify_native.py:206-213)
+        out = (c.c_ubyte * (w * h * 4))()
+        for backend in (-1, 4):
+            assert not lib.RtRender(s.handle, c.byref(cam), c.byref(st), out, len(out), backend, c.byref(c.c_float()))
+        assert not lib.RtRender(s.handle, c.byref(cam), c.byref(st), out, 4, 2, c.byref(c.c_float()))
+        assert not lib.RtRender(s.handle, c.byref(cam), c.byref(st), out, len(out), 3, c.byref(c.c_float()))
+        assert b"Embree" in lib.RtLastError(s.handle)
+        s.render(cam, st)
+        lib.RtShutdown(s.handle)
+        assert not lib.RtRender(s.handle, c.byref(cam), c.byref(st), out, len(out), 2, c.byref(c.c_float()))
+    finally:
+        s.close()
+
+
+def test_resident_render_matches_rtrender(lib):
+    from scenes import cornell, cornell_camera, make_settings
+    tris, cam, st = cornell(), cornell_camera(64, 48), make_settings(4, rpp=4)
+    with Session(lib) as a, Session(lib) as b:
+        a.initialize(tris, 64, 48)
+        b.initialize(tris, 64, 48)
+        img, _ = a.render(cam, st)
+        ms = b.render_resident(cam, st)
+        assert ms > 0 and np.array_equal(b.copy_output(), img)
+        assert lib.RtSampleCount(b.handle) == 4
