@@ -1,0 +1,195 @@
+"""Mrays/s of the MI355X path tracer on one GPU, with roofline and CPU-baseline figures.
+
+Workload (default): the 1M-triangle random soup at 1920x1080, 3 bounces, 1 progressive
+sample per step (no emitters, sky strength 1, so next-event estimation is off), BVH built
+once at RtInitialize.  A step is one RtRenderResident call (raygen, traversal, shading and
+accumulation of one frame, output left in HBM).  value = (closest-hit + shadow ray queries)
+/ wall time of the timed steps.
+
+After the timed region, two instrumented frames measure (a) per-stage kernel time with HIP
+events on the session's stream and (b) node visits / triangle tests per ray, from which the
+closest-hit traversal kernel's algorithmic bytes per launch are formed:
+    bytes = 48 B per ray (32 B ray in + 16 B hit out) + node visits * node bytes
+            + triangle tests * 48 B
+roofline.achieved = bytes per launch / mean launch time.
+
+The CPU baseline is the oracle (a plain C++ restatement of the reference's CPU path,
+OpenMP over image rows like Native/renderer.cu:405-416) on the host's cores, over a
+bounded block of rows of the same frame.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+for p in (str(ROOT), str(ROOT / "custom-ray-tracing-shader_amd")):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+HBM_PEAK_GBS = 8000.0
+NODE_BYTES = {2: 64, 4: 128, 8: 256}
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def workload(name, tris_count):
+    import scenes
+    if name == "soup":
+        w, h = 1920, 1080
+        tris = scenes.triangle_soup(tris_count, seed=1)
+        return (f"random triangle soup {len(tris)} tris, 1920x1080, 3 bounces, 1 spp/step, sky 1",
+                tris, scenes.soup_camera(w, h), scenes.make_settings(3, rpp=1, sky=1.0), w, h)
+    if name == "icosphere":
+        w, h = 1920, 1080
+        tris = scenes.displaced_icosphere(subdiv=9, seed=7)
+        return (f"displaced icosphere + room {len(tris)} tris, 1920x1080, 8 bounces + RR, 1 spp/step",
+                tris, scenes.icosphere_camera(w, h), scenes.make_settings(8, rpp=1, sky=0.0), w, h)
+    if name == "cornell":
+        w = h = 256
+        return ("Cornell box 36 tris, 256x256, 4 bounces, 16 spp/step", scenes.cornell(), scenes.cornell_camera(w, h),
+                scenes.make_settings(4, rpp=16), w, h)
+    raise SystemExit(f"unknown workload {name}")
+
+
+def cpu_baseline(tris, cam, st, w, h, budget_s):
+    import oracle
+    scene = oracle.OracleScene(tris)
+    cam_rec = np.frombuffer(bytes(cam), dtype=oracle.CAMERA).copy()
+    st_rec = np.frombuffer(bytes(st), dtype=oracle.SETTINGS).copy()
+    threads = len(os.sched_getaffinity(0))
+    total = np.zeros((h, w, 10), dtype=np.float32)
+    y0 = h // 2
+    rows = 4
+    t0 = time.perf_counter()
+    _, _, (c0, s0) = scene.render(cam_rec, st_rec, w, h, 0, total, threads, rows=(y0, y0 + rows))
+    dt = max(time.perf_counter() - t0, 1e-3)
+    rows = int(min(h - y0, max(rows, rows * budget_s / dt)))
+    t0 = time.perf_counter()
+    _, _, (c1, s1) = scene.render(cam_rec, st_rec, w, h, 0, total, threads, rows=(y0, y0 + rows))
+    dt = time.perf_counter() - t0
+    return {"value": (c1 + s1) / dt / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "sample": f"oracle (C++ restatement of the reference CPU path, 12-bin SAH BVH, OpenMP rows) on "
+                      f"{rows} rows x {w} px of the same frame, 1 spp: {c1 + s1} ray queries in {dt:.2f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="soup", choices=("soup", "icosphere", "cornell"))
+    ap.add_argument("--tris", type=int, default=1_000_000)
+    ap.add_argument("--bvh-width", type=int, default=4)
+    ap.add_argument("--blocks-per-cu", type=int, default=4)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (paths are independent per pixel, "
+              f"so a multi-GPU run would shard image rows; see the design notes)", file=sys.stderr)
+        return 2
+
+    import native_api
+    desc, tris, cam, st, w, h = workload(args.workload, args.tris)
+    log(f"workload: {desc}")
+    s = native_api.Session()
+    s.set_option("bvh_width", args.bvh_width)
+    s.set_option("blocks_per_cu", args.blocks_per_cu)
+    t0 = time.perf_counter()
+    s.initialize(tris, w, h)
+    init_s = time.perf_counter() - t0
+    nodes, depth, build_ms = s.bvh_info()
+    log(f"initialize {init_s:.2f} s (BVH {nodes} binary nodes, depth {depth}, build {build_ms:.1f} ms)")
+
+    for _ in range(args.warmup):
+        s.render_resident(cam, st)
+    log("warmup done")
+    rays = 0
+    device_ms = 0.0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        device_ms += s.render_resident(cam, st)
+        c, sh = s.ray_counts()
+        rays += c + sh
+    wall = time.perf_counter() - t0
+    ms_per_step = wall * 1e3 / args.steps
+    value = rays / wall / 1e6
+    log(f"timed: {args.steps} steps, {ms_per_step:.3f} ms/step wall, {device_ms / args.steps:.3f} ms/step device, "
+        f"{value:.1f} Mrays/s")
+
+    # Instrumented frames (outside the timed region).
+    s.set_option("stage_timing", 1)
+    stage_ms = {}
+    for _ in range(3):
+        s.render_resident(cam, st)
+        for k, (ms, n) in s.stage_times().items():
+            a = stage_ms.setdefault(k, [0.0, 0])
+            a[0] += ms
+            a[1] += n
+    s.set_option("stage_timing", 0)
+    s.set_option("trace_stats", 1)
+    s.render_resident(cam, st)
+    stats = s.trace_stats()
+    closest_rays, shadow_rays = s.ray_counts()
+    s.set_option("trace_stats", 0)
+    width = native_api.load_library().RtGetOption(s.handle, b"bvh_width")
+    node_b = NODE_BYTES[width]
+    launches = stage_ms["closest"][1] / 3
+    closest_ms = stage_ms["closest"][0] / 3
+    bytes_frame = 48 * closest_rays + stats["closest_nodes"] * node_b + stats["closest_tris"] * 48
+    bytes_launch = bytes_frame / launches
+    mean_launch_s = closest_ms / 1e3 / launches
+    achieved = bytes_launch / mean_launch_s / 1e9
+    log(f"stages (ms/frame): " + ", ".join(f"{k} {v[0] / 3:.3f} ({v[1] // 3} launches)" for k, v in stage_ms.items()))
+    log(f"closest: {closest_rays} rays, {stats['closest_nodes'] / max(closest_rays, 1):.1f} nodes/ray, "
+        f"{stats['closest_tris'] / max(closest_rays, 1):.1f} tris/ray, {achieved:.0f} GB/s algorithmic")
+
+    baseline = None
+    if not args.no_cpu_baseline:
+        log("cpu baseline ...")
+        baseline = cpu_baseline(tris, cam, st, w, h, args.cpu_seconds)
+        log(f"cpu baseline {baseline['value']:.2f} Mrays/s on {baseline['cores']} threads")
+
+    traffic = None
+    tf = ROOT / "profiles" / f"traffic_{args.workload}.json"
+    if tf.exists():
+        traffic = json.loads(tf.read_text()).get("hbm_bytes_per_launch")
+
+    line = {
+        "metric": "Mrays/s on 1 MI355X",
+        "value": round(value, 2),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded scene generator)",
+        "config": {"workload": desc, "triangles": int(len(tris)), "width": w, "height": h,
+                   "bvh_width": width, "rays_per_step": int(rays / args.steps),
+                   "device_ms_per_step": round(device_ms / args.steps, 4), "bvh_build_ms": round(build_ms, 1)},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "kernel": f"traceKernel<{width},closest>", "mean_launch_ms": round(mean_launch_s * 1e3, 4),
+                     "launches_per_step": launches, "algorithmic_bytes_per_launch": int(bytes_launch),
+                     "nodes_per_ray": round(stats["closest_nodes"] / max(closest_rays, 1), 2),
+                     "tris_per_ray": round(stats["closest_tris"] / max(closest_rays, 1), 2)},
+        "cpu_baseline": baseline,
+    }
+    s.close()
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())
