@@ -32,6 +32,13 @@ for p in (str(ROOT), str(ROOT / "custom-ray-tracing-shader_amd")):
         sys.path.insert(0, p)
 
 HBM_PEAK_GBS = 8000.0
+# Launch configurations chosen by autotune.py on an MI355X for each workload (bench.py
+# --autotune repeats the search); the BVH width also fixes the node size below.
+TUNED = {
+    "soup": {"bvh_width": 4, "blocks_per_cu": 8, "short_stack": 8, "refill": 0},
+    "icosphere": {"bvh_width": 4, "blocks_per_cu": 4, "short_stack": 16, "refill": 0},
+    "cornell": {"bvh_width": 4, "blocks_per_cu": 8, "short_stack": 8, "refill": 0},
+}
 NODE_BYTES = {2: 64, 4: 128, 8: 256}
 
 
@@ -59,24 +66,30 @@ def workload(name, tris_count):
 
 
 def cpu_baseline(tris, cam, st, w, h, budget_s):
+    """Oracle on all host cores over whole-frame row blocks until about budget_s of work."""
     import oracle
     scene = oracle.OracleScene(tris)
     cam_rec = np.frombuffer(bytes(cam), dtype=oracle.CAMERA).copy()
     st_rec = np.frombuffer(bytes(st), dtype=oracle.SETTINGS).copy()
     threads = len(os.sched_getaffinity(0))
     total = np.zeros((h, w, 10), dtype=np.float32)
-    y0 = h // 2
-    rows = 4
-    t0 = time.perf_counter()
-    _, _, (c0, s0) = scene.render(cam_rec, st_rec, w, h, 0, total, threads, rows=(y0, y0 + rows))
-    dt = max(time.perf_counter() - t0, 1e-3)
-    rows = int(min(h - y0, max(rows, rows * budget_s / dt)))
-    t0 = time.perf_counter()
-    _, _, (c1, s1) = scene.render(cam_rec, st_rec, w, h, 0, total, threads, rows=(y0, y0 + rows))
-    dt = time.perf_counter() - t0
-    return {"value": (c1 + s1) / dt / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
-            "sample": f"oracle (C++ restatement of the reference CPU path, 12-bin SAH BVH, OpenMP rows) on "
-                      f"{rows} rows x {w} px of the same frame, 1 spp: {c1 + s1} ray queries in {dt:.2f} s"}
+    rays, rows, sample, elapsed = 0, 0, 0, 0.0
+    block = max(1, h // 16)
+    y = 0
+    while elapsed < budget_s:
+        y1 = min(h, y + block)
+        t0 = time.perf_counter()
+        _, _, (c, sh) = scene.render(cam_rec, st_rec, w, h, sample, total, threads, rows=(y, y1))
+        elapsed += time.perf_counter() - t0
+        rays += c + sh
+        rows += y1 - y
+        y = y1
+        if y >= h:
+            y, sample = 0, sample + int(st_rec["raysPerPixel"][0])
+    return {"value": rays / elapsed / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "sample": f"oracle (C++ restatement of the reference CPU path, 12-bin SAH BVH, OpenMP over rows, "
+                      f"{threads} threads) on {rows} rows x {w} px of the same frame and settings: "
+                      f"{rays} ray queries in {elapsed:.1f} s"}
 
 
 def main():
@@ -86,11 +99,26 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="soup", choices=("soup", "icosphere", "cornell"))
     ap.add_argument("--tris", type=int, default=1_000_000)
-    ap.add_argument("--bvh-width", type=int, default=4)
-    ap.add_argument("--blocks-per-cu", type=int, default=4)
+    ap.add_argument("--bvh-width", type=int)
+    ap.add_argument("--blocks-per-cu", type=int)
+    ap.add_argument("--short-stack", type=int)
+    ap.add_argument("--refill", type=int)
+    ap.add_argument("--autotune", action="store_true",
+                    help="search the launch configuration now instead of using the recorded tuned one")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
+                    help="only run the plain copy kernel over BYTES (HBM counter calibration) and exit")
     args = ap.parse_args()
+    if args.copy_probe:
+        import ctypes
+        import native_api
+        ms = ctypes.c_float()
+        if not native_api.load_library().RtBandwidthProbe(args.copy_probe, 5, ctypes.byref(ms)):
+            raise SystemExit("RtBandwidthProbe failed")
+        gbs = 2 * args.copy_probe / (ms.value / 1e3) / 1e9
+        print(json.dumps({"copy_bytes": args.copy_probe, "mean_ms": ms.value, "read_plus_write_GBs": gbs}))
+        return 0
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (paths are independent per pixel, "
               f"so a multi-GPU run would shard image rows; see the design notes)", file=sys.stderr)
@@ -99,9 +127,20 @@ def main():
     import native_api
     desc, tris, cam, st, w, h = workload(args.workload, args.tris)
     log(f"workload: {desc}")
+    cfg = dict(TUNED[args.workload])
+    for k in cfg:
+        if getattr(args, k) is not None:
+            cfg[k] = getattr(args, k)
+    if args.autotune:
+        import autotune
+        t0 = time.perf_counter()
+        best, trials = autotune.autotune(tris, cam, st, w, h, use_cache=False)
+        cfg = {k: best[k] for k in cfg}
+        log(f"autotune: {len(trials)} configurations in {time.perf_counter() - t0:.1f} s, best {best}")
     s = native_api.Session()
-    s.set_option("bvh_width", args.bvh_width)
-    s.set_option("blocks_per_cu", args.blocks_per_cu)
+    s.set_option("bvh_width", cfg["bvh_width"])
+    for k in ("blocks_per_cu", "short_stack", "refill"):
+        s.set_option(k, cfg[k])
     t0 = time.perf_counter()
     s.initialize(tris, w, h)
     init_s = time.perf_counter() - t0
@@ -176,7 +215,8 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded scene generator)",
         "config": {"workload": desc, "triangles": int(len(tris)), "width": w, "height": h,
-                   "bvh_width": width, "rays_per_step": int(rays / args.steps),
+                   "bvh_width": width, "launch": cfg, "autotuned_now": args.autotune,
+                   "rays_per_step": int(rays / args.steps),
                    "device_ms_per_step": round(device_ms / args.steps, 4), "bvh_build_ms": round(build_ms, 1)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,

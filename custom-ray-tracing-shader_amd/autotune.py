@@ -1,0 +1,91 @@
+"""Autotuner for the traversal launch: BVH width, persistent workgroups per CU, LDS
+short-stack depth and per-lane ray refill.
+
+For a scene, camera and settings it initialises one session per BVH width, times a few
+device-resident frames for every (blocks_per_cu, short_stack, refill) combination, and keeps
+the configuration with the highest ray throughput.  Choices persist in a JSON cache keyed by
+scene size class and resolution ($RT_AUTOTUNE_CACHE, default ~/.cache/rt_amd/autotune.json),
+so later sessions of a similar scene reuse them without searching again.
+"""
+import json
+import math
+import os
+import time
+from pathlib import Path
+
+from native_api import Session
+
+WIDTHS = (2, 4, 8)
+BLOCKS = (4, 6, 8, 12)
+STACKS = (8, 16)
+REFILL = (0, 1)
+
+
+def cache_path():
+    return Path(os.environ.get("RT_AUTOTUNE_CACHE", Path.home() / ".cache" / "rt_amd" / "autotune.json"))
+
+
+def scene_key(n_tris, width, height, max_bounces):
+    return f"tris2^{int(round(math.log2(max(n_tris, 1))))}-px{width}x{height}-b{max_bounces}"
+
+
+def load_cache():
+    p = cache_path()
+    try:
+        return json.loads(p.read_text())
+    except (OSError, ValueError):
+        return {}
+
+
+def store_cache(cache):
+    p = cache_path()
+    try:
+        p.parent.mkdir(parents=True, exist_ok=True)
+        p.write_text(json.dumps(cache, indent=1, sort_keys=True))
+    except OSError:
+        pass
+
+
+def apply(session, cfg):
+    for key in ("blocks_per_cu", "short_stack", "refill"):
+        if not session.set_option(key, cfg[key]):
+            raise ValueError(f"option {key}={cfg[key]} rejected")
+
+
+def measure(session, camera, settings, frames):
+    session.render_resident(camera, settings)          # warm (and reset accumulation)
+    rays, t0 = 0, time.perf_counter()
+    for _ in range(frames):
+        session.render_resident(camera, settings)
+        c, s = session.ray_counts()
+        rays += c + s
+    return rays / (time.perf_counter() - t0) / 1e6
+
+
+def autotune(triangles, camera, settings, width, height, frames=3, widths=WIDTHS, use_cache=True, lib=None, log=None):
+    """Returns (best config dict, list of all measurements)."""
+    key = scene_key(len(triangles), width, height, settings.maxBounces)
+    cache = load_cache() if use_cache else {}
+    if use_cache and key in cache:
+        return cache[key], []
+    results = []
+    best = None
+    for w in widths:
+        with Session(lib) as s:
+            s.set_option("bvh_width", w)
+            s.initialize(triangles, width, height)
+            for b in BLOCKS:
+                for st in STACKS:
+                    for r in REFILL:
+                        cfg = {"bvh_width": w, "blocks_per_cu": b, "short_stack": st, "refill": r}
+                        apply(s, cfg)
+                        cfg["mrays_per_s"] = round(measure(s, camera, settings, frames), 2)
+                        results.append(cfg)
+                        if log:
+                            log(f"autotune {cfg}")
+                        if best is None or cfg["mrays_per_s"] > best["mrays_per_s"]:
+                            best = dict(cfg)
+    if use_cache:
+        cache[key] = best
+        store_cache(cache)
+    return best, results

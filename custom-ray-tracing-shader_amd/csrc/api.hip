@@ -9,7 +9,7 @@
 #include <omp.h>
 #include "kernels.h"
 #include "traverse.h"
-#include "bvh_wide.h"
+#include "bvh_gpu.h"
 
 using namespace rt;
 
@@ -24,22 +24,28 @@ struct Session {
     std::vector<Triangle> tris;
     std::vector<Light> lights;
     float lightArea = 0;
-    std::vector<RtBvhNode> binNodes;
+    std::vector<RtBvhNode> binNodes;   // host copies, fetched on demand
     std::vector<int> binOrder;
+    int binNodeCount = 0, wideNodeCount = 0;
     int bvhDepth = 0;
     float buildMs = 0;
     int width = 0, height = 0;
     int bvhWidth = 4;            // option, used at the next RtInitialize
+    int buildBins = 32;          // option: SAH bins per axis of the GPU build
     int builtWidth = 4;
-    int blocksPerCU = 4;         // persistent traversal workgroups per CU
+    int blocksPerCU = 8;         // persistent traversal workgroups per CU
+    int shortStack = 8;          // LDS stack entries per traversal lane (8 or 16)
+    int refill = 1;              // per-lane ray refill in the traversal kernel
     std::vector<unsigned char> hostNodes;
     std::vector<TriRecord> hostRecs;
     std::vector<HostSum> hostSum;
     // device
     Triangle* dTris = nullptr;
     Light* dLights = nullptr;
-    void* dNodes = nullptr;
+    void* dNodes = nullptr;          // WideNode<builtWidth>[]
     TriRecord* dRecs = nullptr;
+    RtBvhNode* dBinNodes = nullptr;  // binary tree of the GPU build
+    int* dOrder = nullptr;
     float4 *dSumRad = nullptr, *dSumAlb = nullptr, *dSumNrm = nullptr;
     uchar4* dPixels = nullptr;
     float4 *dThr = nullptr, *dRad = nullptr, *dAlb = nullptr, *dNrm = nullptr, *dNee = nullptr;
@@ -50,7 +56,8 @@ struct Session {
     int* dOverflow = nullptr;
     unsigned long long* dStats = nullptr;
     long long slotCap = 0;
-    int spillEntries = 0, traceBlocks = 0, traceLanes = 0;
+    int spillEntries = 0, traceBlocks = 0, traceLanes = 0, cus = 0;
+    size_t spillLanes = 0;           // lanes the spill ring is sized for
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     // progressive state (Native/renderer.cu:65-68)
@@ -103,6 +110,7 @@ template <class T> void devFree(T*& p) { if (p) (void)hipFree(p); p = nullptr; }
 
 void freeDevice(Session* s) {
     devFree(s->dTris); devFree(s->dLights); devFree(s->dNodes); devFree(s->dRecs);
+    devFree(s->dBinNodes); devFree(s->dOrder);
     devFree(s->dSumRad); devFree(s->dSumAlb); devFree(s->dSumNrm); devFree(s->dPixels);
     devFree(s->dThr); devFree(s->dRad); devFree(s->dAlb); devFree(s->dNrm); devFree(s->dNee);
     devFree(s->dRayA); devFree(s->dRayB); devFree(s->dShadow); devFree(s->dHits);
@@ -121,30 +129,64 @@ SceneView deviceScene(const Session* s) {
     return v;
 }
 
-// Builds the binary BVH and its W-wide collapse, then uploads nodes and records.
+// GPU binned-SAH build (32 bins, leaf <= 4) and collapse to the session's wide layout.
 bool buildAccel(Session* s) {
     auto t0 = std::chrono::steady_clock::now();
-    HostBinaryBvh b;
-    b.bins = 32; b.maxLeaf = 4;
-    b.build(s->tris.data(), (int)s->tris.size());
-    s->binNodes = std::move(b.nodes); s->binOrder = std::move(b.order); s->bvhDepth = b.depth;
+    GpuBuildParams prm;
+    prm.bins = s->buildBins; prm.maxLeaf = 4; prm.width = s->bvhWidth;
+    GpuBuildResult r;
+    if ((s->error = buildBvhGpu(s->dTris, (int)s->tris.size(), prm, r, s->stream)) != hipSuccess) {
+        s->hostError = std::string("GPU BVH build failed: ") + hipGetErrorString(s->error);
+        return false;
+    }
+    s->dBinNodes = r.binNodes; s->dOrder = r.order; s->dRecs = r.recs; s->dNodes = r.wideNodes;
+    s->binNodeCount = r.binNodeCount; s->wideNodeCount = r.wideCount; s->bvhDepth = r.depth;
     s->builtWidth = s->bvhWidth;
-    auto collapse = [&](auto tag) {
-        constexpr int W = decltype(tag)::value;
-        WideBuilder<W> wb;
-        wb.build(s->binNodes.data(), s->binOrder.data(), s->tris.data());
-        s->hostNodes.resize(wb.nodes.size() * sizeof(WideNode<W>));
-        std::memcpy(s->hostNodes.data(), wb.nodes.data(), s->hostNodes.size());
-        s->hostRecs = std::move(wb.recs);
-    };
-    if (s->builtWidth == 2) collapse(std::integral_constant<int, 2>());
-    else if (s->builtWidth == 8) collapse(std::integral_constant<int, 8>());
-    else collapse(std::integral_constant<int, 4>());
     s->buildMs = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
-    unsigned char* dn = nullptr;
-    if (!upload(s, dn, s->hostNodes.data(), s->hostNodes.size())) return false;
-    s->dNodes = dn;
-    return upload(s, s->dRecs, s->hostRecs.data(), s->hostRecs.size());
+    return true;
+}
+
+// Host copies of the wide nodes and records for the CPU backends (same tree as the GPU).
+bool fetchHostAccel(Session* s) {
+    if (!s->hostRecs.empty()) return true;
+    const size_t nodeBytes = (size_t)s->wideNodeCount * 32 * s->builtWidth;
+    s->hostNodes.resize(nodeBytes);
+    s->hostRecs.resize(s->tris.size());
+    if ((s->error = hipMemcpy(s->hostNodes.data(), s->dNodes, nodeBytes, hipMemcpyDeviceToHost)) != hipSuccess ||
+        (s->error = hipMemcpy(s->hostRecs.data(), s->dRecs, s->hostRecs.size() * sizeof(TriRecord),
+                              hipMemcpyDeviceToHost)) != hipSuccess) {
+        s->hostRecs.clear();
+        return false;
+    }
+    return true;
+}
+
+bool fetchBinaryTree(Session* s) {
+    if (!s->binNodes.empty()) return true;
+    s->binNodes.resize(s->binNodeCount);
+    s->binOrder.resize(s->tris.size());
+    if ((s->error = hipMemcpy(s->binNodes.data(), s->dBinNodes, s->binNodes.size() * sizeof(RtBvhNode),
+                              hipMemcpyDeviceToHost)) != hipSuccess ||
+        (s->error = hipMemcpy(s->binOrder.data(), s->dOrder, s->binOrder.size() * sizeof(int), hipMemcpyDeviceToHost)) !=
+            hipSuccess) {
+        s->binNodes.clear();
+        return false;
+    }
+    return true;
+}
+
+// Sizes the persistent traversal grid from the current options; grows the spill ring.
+bool prepareTraceGrid(Session* s) {
+    s->traceBlocks = s->cus * s->blocksPerCU;
+    s->traceLanes = s->traceBlocks * kBlock;
+    s->spillEntries = 36 * (s->builtWidth - 1);   // >= (W-1) * (wide depth <= 33) entries
+    if ((size_t)s->traceLanes > s->spillLanes) {
+        devFree(s->dSpill);
+        s->spillLanes = 0;
+        if (!devAlloc(s, s->dSpill, (size_t)s->spillEntries * s->traceLanes * 2)) return false;
+        s->spillLanes = (size_t)s->traceLanes;
+    }
+    return true;
 }
 
 TraceArgs baseTraceArgs(Session* s) {
@@ -162,6 +204,7 @@ bool renderGpu(Session* s, const CameraData& cam, const RenderSettings& rs, floa
     int chunkR = (int)std::min<long long>(rpp, s->slotCap / npix);
     if (chunkR < 1) chunkR = 1;
     hipStream_t st = s->stream;
+    if (!prepareTraceGrid(s)) return false;
     s->timedLaunches.clear();
     if (s->stageTiming && s->eventPool.empty()) {
         s->eventPool.resize(4096);
@@ -197,7 +240,7 @@ bool renderGpu(Session* s, const CameraData& cam, const RenderSettings& rs, floa
             ta.stats = s->dStats;
             {
                 StageScope sc(s, RT_STAGE_CLOSEST);
-                if ((s->error = launchTrace(s->builtWidth, false, s->traceStats, s->traceBlocks, ta, st)) != hipSuccess)
+                if ((s->error = launchTrace(s->builtWidth, false, s->traceStats, s->traceBlocks, s->shortStack, s->refill != 0, ta, st)) != hipSuccess)
                     return false;
             }
             f.bounce = b; f.rayIn = in; f.rayOut = out;
@@ -210,7 +253,7 @@ bool renderGpu(Session* s, const CameraData& cam, const RenderSettings& rs, floa
                 sa.rays = s->dShadow; sa.countPtr = ctr + kCntShadow + b; sa.work = ctr + kWorkShadow + b;
                 sa.rad = s->dRad; sa.contrib = s->dNee; sa.stats = s->dStats + 2;
                 StageScope sc(s, RT_STAGE_SHADOW);
-                if ((s->error = launchTrace(s->builtWidth, true, s->traceStats, s->traceBlocks, sa, st)) != hipSuccess)
+                if ((s->error = launchTrace(s->builtWidth, true, s->traceStats, s->traceBlocks, s->shortStack, s->refill != 0, sa, st)) != hipSuccess)
                     return false;
             }
         }
@@ -340,7 +383,7 @@ int RtCpuThreadCount(void* p) { Session* s = static_cast<Session*>(p); return s 
 
 void RtBvhInfo(void* p, int* nodes, int* depth, float* buildMs) {
     Session* s = static_cast<Session*>(p);
-    *nodes = s ? (int)s->binNodes.size() : 0;
+    *nodes = s ? s->binNodeCount : 0;
     *depth = s ? s->bvhDepth : 0;
     *buildMs = s ? s->buildMs : 0;
 }
@@ -359,6 +402,7 @@ void RtShutdown(void* p) {
     s->ready = false;
     s->tris.clear(); s->lights.clear(); s->binNodes.clear(); s->binOrder.clear();
     s->hostNodes.clear(); s->hostRecs.clear(); s->hostSum.clear();
+    s->binNodeCount = s->wideNodeCount = 0;
     s->width = s->height = 0; s->samples = 0; s->lightArea = 0; s->buildMs = 0; s->bvhDepth = 0; s->lastBackend = -1;
     s->hasFrame = false;
 }
@@ -393,8 +437,8 @@ int RtInitialize(void* p, const Triangle* triangles, int count, int w, int h) {
     if ((s->error = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking)) != hipSuccess) return 0;
     if ((s->error = hipEventCreate(&s->ev0)) != hipSuccess) return 0;
     if ((s->error = hipEventCreate(&s->ev1)) != hipSuccess) return 0;
-    if (!buildAccel(s)) return 0;
     if (!upload(s, s->dTris, s->tris.data(), s->tris.size())) return 0;
+    if (!buildAccel(s)) return 0;
     if (!upload(s, s->dLights, s->lights.data(), s->lights.size())) return 0;
     const size_t npix = (size_t)w * h;
     s->slotCap = std::max<long long>((long long)npix, std::min<long long>((long long)npix * kMaxRaysPerPixel, 1LL << 24));
@@ -406,10 +450,9 @@ int RtInitialize(void* p, const Triangle* triangles, int count, int w, int h) {
         !devAlloc(s, s->dHits, slots) || !devAlloc(s, s->dCounters, (size_t)kMaxRaysPerPixel * kCounterStride) ||
         !devAlloc(s, s->dOverflow, 1) || !devAlloc(s, s->dStats, 4))
         return 0;
-    s->traceBlocks = cus * s->blocksPerCU;
-    s->traceLanes = s->traceBlocks * kBlock;
-    s->spillEntries = 36 * (s->builtWidth - 1);
-    if (!devAlloc(s, s->dSpill, (size_t)s->spillEntries * s->traceLanes * 2)) return 0;
+    s->cus = cus;
+    s->spillLanes = 0;
+    if (!prepareTraceGrid(s)) return 0;
     if ((s->error = hipMemset(s->dOverflow, 0, sizeof(int))) != hipSuccess) return 0;
     if ((s->error = hipMemset(s->dSumRad, 0, npix * sizeof(float4))) != hipSuccess) return 0;
     s->ready = true;
@@ -444,6 +487,7 @@ int RtRender(void* p, const CameraData* camera, const RenderSettings* settings, 
         s->error = hipMemcpy(output, s->dPixels, (size_t)s->width * s->height * 4, hipMemcpyDeviceToHost);
         if (s->error != hipSuccess) return 0;
     } else {
+        if (!fetchHostAccel(s)) return 0;
         auto t0 = std::chrono::steady_clock::now();
         const bool multi = backend == RT_BACKEND_CPU_MULTI;
         unsigned long long c = 0, sh = 0;
@@ -501,6 +545,30 @@ int RtTraceStats(void* p, unsigned long long* counts4) {
     return 1;
 }
 
+int RtBandwidthProbe(long long bytes, int iterations, float* milliseconds) {
+    if (bytes < 16 || iterations < 1 || !milliseconds) return 0;
+    const long long n = bytes / 16;
+    float4 *a = nullptr, *b = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int device = 0, cus = 0;
+    bool ok = hipGetDevice(&device) == hipSuccess &&
+              hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess &&
+              hipMalloc(&a, n * 16) == hipSuccess && hipMalloc(&b, n * 16) == hipSuccess &&
+              hipMemset(a, 0, n * 16) == hipSuccess && hipEventCreate(&e0) == hipSuccess &&
+              hipEventCreate(&e1) == hipSuccess;
+    if (ok) ok = launchCopy(a, b, n, cus * 8, nullptr) == hipSuccess && hipDeviceSynchronize() == hipSuccess;
+    if (ok) ok = hipEventRecord(e0, nullptr) == hipSuccess;
+    for (int k = 0; ok && k < iterations; k++) ok = launchCopy(a, b, n, cus * 8, nullptr) == hipSuccess;
+    if (ok) ok = hipEventRecord(e1, nullptr) == hipSuccess && hipEventSynchronize(e1) == hipSuccess &&
+                 hipEventElapsedTime(milliseconds, e0, e1) == hipSuccess;
+    if (ok) *milliseconds /= iterations;
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (a) (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    return ok ? 1 : 0;
+}
+
 int RtDenoise(void* p, unsigned char* output, int outputBytes, float* milliseconds) {
     Session* s = static_cast<Session*>(p);
     if (!s) return 0;
@@ -519,6 +587,7 @@ int RtTraceRays(void* p, const RtRay* rays, int count, int anyHit, RtHit* hits) 
     if (!s) return 0;
     if (!s->ready || !rays || !hits || count < 0) { s->error = hipErrorInvalidValue; return 0; }
     if (count == 0) return 1;
+    if (!prepareTraceGrid(s)) return 0;
     float4* dr = nullptr; int4* dh = nullptr; int* work = nullptr;
     bool ok = upload(s, dr, reinterpret_cast<const float4*>(rays), (size_t)count * 2) && devAlloc(s, dh, (size_t)count) &&
               devAlloc(s, work, 1);
@@ -526,7 +595,7 @@ int RtTraceRays(void* p, const RtRay* rays, int count, int anyHit, RtHit* hits) 
     if (ok) {
         TraceArgs a = baseTraceArgs(s);
         a.rays = dr; a.count = count; a.work = work; a.hits = dh;
-        ok = (s->error = launchTrace(s->builtWidth, anyHit != 0, false, s->traceBlocks, a, s->stream)) == hipSuccess;
+        ok = (s->error = launchTrace(s->builtWidth, anyHit != 0, false, s->traceBlocks, s->shortStack, s->refill != 0, a, s->stream)) == hipSuccess;
     }
     if (ok) ok = (s->error = hipStreamSynchronize(s->stream)) == hipSuccess;
     if (ok) ok = (s->error = hipMemcpy(hits, dh, (size_t)count * sizeof(RtHit), hipMemcpyDeviceToHost)) == hipSuccess;
@@ -557,11 +626,12 @@ int RtSobolBatch(const unsigned* pixels, const unsigned* samples, int count, uns
 int RtBvhExport(void* p, RtBvhNode* nodes, int nodeCapacity, int* order, int orderCapacity) {
     Session* s = static_cast<Session*>(p);
     if (!s || !s->ready) return 0;
+    if ((nodes || order) && !fetchBinaryTree(s)) return 0;
     if (nodes && nodeCapacity > 0)
         std::memcpy(nodes, s->binNodes.data(), sizeof(RtBvhNode) * std::min<size_t>(nodeCapacity, s->binNodes.size()));
     if (order && orderCapacity > 0)
         std::memcpy(order, s->binOrder.data(), sizeof(int) * std::min<size_t>(orderCapacity, s->binOrder.size()));
-    return (int)s->binNodes.size();
+    return s->binNodeCount;
 }
 
 int RtReadAccumulation(void* p, float* out, long long floatCount) {
@@ -605,9 +675,21 @@ int RtSetOption(void* p, const char* key, int value) {
         if (value != 2 && value != 4 && value != 8) return 0;
         s->bvhWidth = value; return 1;
     }
+    if (!std::strcmp(key, "build_bins")) {
+        if (value < 2 || value > 32) return 0;
+        s->buildBins = value; return 1;
+    }
     if (!std::strcmp(key, "blocks_per_cu")) {
-        if (value < 1 || value > 64) return 0;
+        if (value < 1 || value > 32) return 0;
         s->blocksPerCU = value; return 1;
+    }
+    if (!std::strcmp(key, "short_stack")) {
+        if (value != 8 && value != 16) return 0;
+        s->shortStack = value; return 1;
+    }
+    if (!std::strcmp(key, "refill")) {
+        if (value != 0 && value != 1) return 0;
+        s->refill = value; return 1;
     }
     if (!std::strcmp(key, "stage_timing")) {
         if (value != 0 && value != 1) return 0;
@@ -625,6 +707,9 @@ int RtGetOption(void* p, const char* key) {
     if (!s || !key) return -1;
     if (!std::strcmp(key, "bvh_width")) return s->ready ? s->builtWidth : s->bvhWidth;
     if (!std::strcmp(key, "blocks_per_cu")) return s->blocksPerCU;
+    if (!std::strcmp(key, "build_bins")) return s->buildBins;
+    if (!std::strcmp(key, "short_stack")) return s->shortStack;
+    if (!std::strcmp(key, "refill")) return s->refill;
     if (!std::strcmp(key, "stage_timing")) return s->stageTiming ? 1 : 0;
     if (!std::strcmp(key, "trace_stats")) return s->traceStats ? 1 : 0;
     return -1;

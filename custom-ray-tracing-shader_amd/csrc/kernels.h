@@ -5,7 +5,6 @@
 namespace rt {
 
 constexpr int kBlock = 256;       // threads per workgroup for every kernel here
-constexpr int kShortStack = 8;    // LDS stack entries per traversal lane
 constexpr int kCounterStride = 4 * (kMaxBounces + 1);   // ints per chunk of frame counters
 // Frame counter layout within one chunk: [0,33) closest-ray queue sizes per bounce,
 // [33,66) shadow queue sizes, [66,99) closest work cursors, [99,132) shadow work cursors.
@@ -52,14 +51,17 @@ struct AccumArgs {
     float invSamples, exposure;
 };
 
-// Persistent-lane launch of the traversal kernel for width W (2, 4 or 8).
-hipError_t launchTrace(int width, bool anyHit, bool stats, int blocks, const TraceArgs& a, hipStream_t s);
+// Persistent-lane launch of the traversal kernel: width W (2, 4 or 8), LDS short-stack
+// entries per lane (8 or 16), per-lane refill on or off.
+hipError_t launchTrace(int width, bool anyHit, bool stats, int blocks, int stack, bool refill, const TraceArgs& a,
+                       hipStream_t s);
 hipError_t launchResetCounters(int* counters, int initialRays, hipStream_t s);
 hipError_t launchRaygen(const FrameArgs& f, hipStream_t s);
 hipError_t launchShade(const FrameArgs& f, hipStream_t s);
 hipError_t launchAccumulate(const AccumArgs& a, hipStream_t s);
 hipError_t launchSobolBatch(const unsigned* pixels, const unsigned* samples, int count, unsigned seed, int dims,
                             float* out, hipStream_t s);
+hipError_t launchCopy(const float4* in, float4* out, long long n, int blocks, hipStream_t s);
 const unsigned* hostSobolDirections();
 
 }  // namespace rt

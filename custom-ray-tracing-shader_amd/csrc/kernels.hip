@@ -29,106 +29,160 @@ __device__ __forceinline__ void loadNode(const WideNode<W>* nodes, int ref, Wide
     n = u.node;
 }
 
-template <int W, bool ANY, bool STATS>
+// Per-lane traversal state, kept in registers across refill rounds.
+struct LaneRay {
+    f3 o, d, inv;
+    float best, bu, bv;
+    int bestIdx, sp, ref;
+    int index;      // queue position of the ray, -1 when the lane is idle
+    int slot;       // path slot (shadow mode)
+};
+
+template <int S>
+struct LdsStack {
+    int* ref;
+    float* dist;
+    int2* spill;
+    int spillEntries, lanes, gl, tid;
+    __device__ __forceinline__ void push(int& sp, int r, float dd, bool& overflow) {
+        const int slot = sp & (S - 1);
+        if (sp >= S) {
+            const int e = sp - S;
+            if (e < spillEntries)
+                spill[(size_t)e * lanes + gl] = make_int2(ref[slot * kBlock + tid], __float_as_int(dist[slot * kBlock + tid]));
+            else
+                overflow = true;
+        }
+        ref[slot * kBlock + tid] = r;
+        dist[slot * kBlock + tid] = dd;
+        sp++;
+    }
+    __device__ __forceinline__ void pop(int& sp, int& r, float& dd) {
+        sp--;
+        const int slot = sp & (S - 1);
+        r = ref[slot * kBlock + tid];
+        dd = dist[slot * kBlock + tid];
+        if (sp >= S) {
+            const int e = sp - S;
+            if (e < spillEntries) {
+                const int2 back = spill[(size_t)e * lanes + gl];
+                ref[slot * kBlock + tid] = back.x;
+                dist[slot * kBlock + tid] = __int_as_float(back.y);
+            }
+        }
+    }
+};
+
+// One traversal step: visit an inner node (push far children, descend to the nearest) or
+// test a leaf, then pop until a deferred entry survives the current best distance.
+// Returns true when the ray is finished.
+template <int W, bool ANY, bool STATS, int S>
+__device__ __forceinline__ bool traceStep(LaneRay& L, const WideNode<W>* nodes, const float4* recs, LdsStack<S>& st,
+                                          bool& overflow, unsigned long long& nv, unsigned long long& tt) {
+    if (L.ref >= 0) {
+        WideNode<W> n;
+        loadNode<W>(nodes, L.ref, n);
+        if (STATS) nv++;
+        float dist[W]; int cr[W];
+#pragma unroll
+        for (int c = 0; c < W; c++) {
+            dist[c] = boxEntry(L.o, L.inv, n.lox[c], n.loy[c], n.loz[c], n.hix[c], n.hiy[c], n.hiz[c], L.best);
+            cr[c] = n.child[c];
+        }
+        sortChildren<W>(dist, cr);
+#pragma unroll
+        for (int c = W - 1; c >= 1; c--)
+            if (dist[c] != INFINITY) st.push(L.sp, cr[c], dist[c], overflow);
+        if (dist[0] != INFINITY) { L.ref = cr[0]; return false; }
+    } else {
+        for (int k = ~L.ref;; k++) {
+            const float4 A = recs[3 * k], B = recs[3 * k + 1], C = recs[3 * k + 2];
+            const int flags = __float_as_int(B.w), index = __float_as_int(A.w);
+            float u = 0, v = 0;
+            const float t = intersectTri(L.o, L.d, F3(A.x, A.y, A.z), F3(B.x, B.y, B.z), F3(C.x, C.y, C.z),
+                                         (flags & kTriOneSided) != 0, u, v);
+            if (STATS) tt++;
+            if (t > 0 && (t < L.best || (t == L.best && index < L.bestIdx))) {
+                L.best = t; L.bestIdx = index; L.bu = u; L.bv = v;
+                if (ANY) return true;
+            }
+            if (flags & kTriLastInLeaf) break;
+        }
+    }
+    while (L.sp > 0) {
+        int r; float dd;
+        st.pop(L.sp, r, dd);
+        if (dd <= L.best) { L.ref = r; return false; }
+    }
+    return true;
+}
+
+template <bool ANY>
+__device__ __forceinline__ void finishRay(const TraceArgs& a, const LaneRay& L) {
+    if (a.hits) {
+        a.hits[L.index] = make_int4(L.bestIdx, __float_as_int(L.best), __float_as_int(L.bu), __float_as_int(L.bv));
+    } else if (L.bestIdx < 0) {
+        // Shadow query found no occluder: add the stored next-event term.
+        float4 acc = a.rad[L.slot];
+        const float4 c = a.contrib[L.slot];
+        acc.x = acc.x + c.x; acc.y = acc.y + c.y; acc.z = acc.z + c.z;
+        a.rad[L.slot] = acc;
+    }
+}
+
+__device__ __forceinline__ void startRay(const TraceArgs& a, int i, LaneRay& L) {
+    const float4 r0 = a.rays[2 * i], r1 = a.rays[2 * i + 1];
+    L.o = F3(r0.x, r0.y, r0.z);
+    L.d = F3(r1.x, r1.y, r1.z);
+    L.inv = safeInverse(L.d);
+    L.best = r0.w; L.bu = 0; L.bv = 0;
+    L.bestIdx = -1; L.sp = 0; L.ref = 0;
+    L.index = i;
+    L.slot = __float_as_int(r1.w);
+}
+
+// Persistent traversal.  Each wave takes rays from the shared cursor; with REFILL, lanes
+// whose ray finished take new rays as soon as a quarter of the wave is idle, instead of
+// waiting for the wave's slowest ray.
+template <int W, bool ANY, bool STATS, int S, bool REFILL>
 __global__ __launch_bounds__(kBlock) void traceKernel(TraceArgs a) {
-    // Short stack: kShortStack (ref, entry distance) pairs per lane in LDS, entry-major so
-    // the 64 lanes of a wave touch 64 consecutive words; deeper entries go to a global ring.
-    __shared__ int sRef[kShortStack * kBlock];
-    __shared__ float sDist[kShortStack * kBlock];
+    __shared__ int sRef[S * kBlock];
+    __shared__ float sDist[S * kBlock];
     const int tid = threadIdx.x, lane = tid & 63;
-    const int gl = blockIdx.x * kBlock + tid;
     const int total = a.countPtr ? *a.countPtr : a.count;
     const WideNode<W>* nodes = static_cast<const WideNode<W>*>(a.nodes);
     const float4* recs = reinterpret_cast<const float4*>(a.recs);
-    int2* spill = reinterpret_cast<int2*>(a.spill);
+    LdsStack<S> st{sRef, sDist, reinterpret_cast<int2*>(a.spill), a.spillEntries, a.totalLanes,
+                   (int)(blockIdx.x * kBlock + tid), tid};
     unsigned long long nv = 0, tt = 0;
     bool overflow = false;
+    LaneRay L;
+    L.index = -1;
+    bool exhausted = false;   // wave-uniform: the cursor has passed the end
     for (;;) {
-        int base = 0;
-        if (lane == 0) base = atomicAdd(a.work, 64);
-        base = __shfl(base, 0, 64);
-        if (base >= total) break;
-        const int i = base + lane;
-        if (i >= total) continue;
-        const float4 r0 = a.rays[2 * i], r1 = a.rays[2 * i + 1];
-        const f3 o = F3(r0.x, r0.y, r0.z), d = F3(r1.x, r1.y, r1.z);
-        const f3 inv = safeInverse(d);
-        float best = r0.w, bu = 0, bv = 0;
-        int bestIdx = -1;
-        int sp = 0, ref = 0;
-        bool done = false;
-        for (;;) {
-            if (ref >= 0) {
-                WideNode<W> n;
-                loadNode<W>(nodes, ref, n);
-                if (STATS) nv++;
-                float dist[W]; int cr[W];
-#pragma unroll
-                for (int c = 0; c < W; c++) {
-                    dist[c] = boxEntry(o, inv, n.lox[c], n.loy[c], n.loz[c], n.hix[c], n.hiy[c], n.hiz[c], best);
-                    cr[c] = n.child[c];
-                }
-                sortChildren<W>(dist, cr);
-#pragma unroll
-                for (int c = W - 1; c >= 1; c--) {
-                    if (dist[c] != INFINITY) {
-                        const int slot = sp & (kShortStack - 1);
-                        if (sp >= kShortStack) {
-                            const int e = sp - kShortStack;
-                            if (e < a.spillEntries)
-                                spill[(size_t)e * a.totalLanes + gl] = make_int2(sRef[slot * kBlock + tid], __float_as_int(sDist[slot * kBlock + tid]));
-                            else
-                                overflow = true;
-                        }
-                        sRef[slot * kBlock + tid] = cr[c];
-                        sDist[slot * kBlock + tid] = dist[c];
-                        sp++;
-                    }
-                }
-                if (dist[0] != INFINITY) { ref = cr[0]; continue; }
-            } else {
-                for (int k = ~ref;; k++) {
-                    const float4 A = recs[3 * k], B = recs[3 * k + 1], C = recs[3 * k + 2];
-                    const int flags = __float_as_int(B.w), index = __float_as_int(A.w);
-                    float u = 0, v = 0;
-                    const float t = intersectTri(o, d, F3(A.x, A.y, A.z), F3(B.x, B.y, B.z), F3(C.x, C.y, C.z),
-                                                 (flags & kTriOneSided) != 0, u, v);
-                    if (STATS) tt++;
-                    if (t > 0 && (t < best || (t == best && index < bestIdx))) {
-                        best = t; bestIdx = index; bu = u; bv = v;
-                        if (ANY) { done = true; break; }
-                    }
-                    if (flags & kTriLastInLeaf) break;
-                }
-                if (done) break;
+        const unsigned long long idle = __ballot(L.index < 0);
+        if (idle && !exhausted) {
+            const int leader = __ffsll((long long)idle) - 1;
+            int base = 0;
+            if (lane == leader) base = atomicAdd(a.work, __popcll(idle));
+            base = __shfl(base, leader, 64);
+            if (L.index < 0) {
+                const int rank = (int)__builtin_amdgcn_mbcnt_hi((unsigned)(idle >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)idle, 0u));
+                const int i = base + rank;
+                if (i < total) startRay(a, i, L);
             }
-            bool found = false;
-            while (sp > 0) {
-                sp--;
-                const int slot = sp & (kShortStack - 1);
-                const int eref = sRef[slot * kBlock + tid];
-                const float edist = sDist[slot * kBlock + tid];
-                if (sp >= kShortStack) {
-                    const int e = sp - kShortStack;
-                    if (e < a.spillEntries) {
-                        const int2 back = spill[(size_t)e * a.totalLanes + gl];
-                        sRef[slot * kBlock + tid] = back.x;
-                        sDist[slot * kBlock + tid] = __int_as_float(back.y);
-                    }
-                }
-                if (edist <= best) { ref = eref; found = true; break; }
-            }
-            if (!found) break;
+            exhausted = base + __popcll(idle) >= total;
         }
-        if (a.hits) {
-            a.hits[i] = make_int4(bestIdx, __float_as_int(best), __float_as_int(bu), __float_as_int(bv));
-        } else if (bestIdx < 0) {
-            // Shadow query found no occluder: add the stored next-event term.
-            const int slot = __float_as_int(r1.w);
-            float4 acc = a.rad[slot];
-            const float4 c = a.contrib[slot];
-            acc.x = acc.x + c.x; acc.y = acc.y + c.y; acc.z = acc.z + c.z;
-            a.rad[slot] = acc;
+        if (__ballot(L.index >= 0) == 0) break;
+        if (L.index >= 0) {
+            for (;;) {
+                if (traceStep<W, ANY, STATS, S>(L, nodes, recs, st, overflow, nv, tt)) {
+                    finishRay<ANY>(a, L);
+                    L.index = -1;
+                    break;
+                }
+                if (REFILL && !exhausted && __popcll(__ballot(true)) <= 48) break;
+            }
         }
     }
     if (STATS) {
@@ -138,23 +192,37 @@ __global__ __launch_bounds__(kBlock) void traceKernel(TraceArgs a) {
     if (overflow) atomicOr(a.overflow, 1);
 }
 
-template <int W>
-static hipError_t launchTraceW(bool anyHit, bool stats, int blocks, const TraceArgs& a, hipStream_t s) {
+template <int W, int S, bool R>
+static void launchTraceWS(bool anyHit, bool stats, int blocks, const TraceArgs& a, hipStream_t s) {
     dim3 grid(blocks), block(kBlock);
     if (anyHit) {
-        if (stats) hipLaunchKernelGGL((traceKernel<W, true, true>), grid, block, 0, s, a);
-        else hipLaunchKernelGGL((traceKernel<W, true, false>), grid, block, 0, s, a);
+        if (stats) hipLaunchKernelGGL((traceKernel<W, true, true, S, R>), grid, block, 0, s, a);
+        else hipLaunchKernelGGL((traceKernel<W, true, false, S, R>), grid, block, 0, s, a);
     } else {
-        if (stats) hipLaunchKernelGGL((traceKernel<W, false, true>), grid, block, 0, s, a);
-        else hipLaunchKernelGGL((traceKernel<W, false, false>), grid, block, 0, s, a);
+        if (stats) hipLaunchKernelGGL((traceKernel<W, false, true, S, R>), grid, block, 0, s, a);
+        else hipLaunchKernelGGL((traceKernel<W, false, false, S, R>), grid, block, 0, s, a);
+    }
+}
+template <int W>
+static hipError_t launchTraceW(bool anyHit, bool stats, int blocks, int stack, bool refill, const TraceArgs& a,
+                               hipStream_t s) {
+    if (stack == 8) {
+        if (refill) launchTraceWS<W, 8, true>(anyHit, stats, blocks, a, s);
+        else launchTraceWS<W, 8, false>(anyHit, stats, blocks, a, s);
+    } else if (stack == 16) {
+        if (refill) launchTraceWS<W, 16, true>(anyHit, stats, blocks, a, s);
+        else launchTraceWS<W, 16, false>(anyHit, stats, blocks, a, s);
+    } else {
+        return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
-hipError_t launchTrace(int width, bool anyHit, bool stats, int blocks, const TraceArgs& a, hipStream_t s) {
+hipError_t launchTrace(int width, bool anyHit, bool stats, int blocks, int stack, bool refill, const TraceArgs& a,
+                       hipStream_t s) {
     switch (width) {
-        case 2: return launchTraceW<2>(anyHit, stats, blocks, a, s);
-        case 4: return launchTraceW<4>(anyHit, stats, blocks, a, s);
-        case 8: return launchTraceW<8>(anyHit, stats, blocks, a, s);
+        case 2: return launchTraceW<2>(anyHit, stats, blocks, stack, refill, a, s);
+        case 4: return launchTraceW<4>(anyHit, stats, blocks, stack, refill, a, s);
+        case 8: return launchTraceW<8>(anyHit, stats, blocks, stack, refill, a, s);
         default: return hipErrorInvalidValue;
     }
 }
@@ -295,6 +363,16 @@ __global__ __launch_bounds__(kBlock) void accumulateKernel(AccumArgs a) {
 }
 hipError_t launchAccumulate(const AccumArgs& a, hipStream_t s) {
     hipLaunchKernelGGL(accumulateKernel, dim3((a.npix + kBlock - 1) / kBlock), dim3(kBlock), 0, s, a);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ bandwidth probe
+__global__ __launch_bounds__(kBlock) void copyKernel(const float4* __restrict__ in, float4* __restrict__ out, long long n) {
+    for (long long i = blockIdx.x * (long long)kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock)
+        out[i] = in[i];
+}
+hipError_t launchCopy(const float4* in, float4* out, long long n, int blocks, hipStream_t s) {
+    hipLaunchKernelGGL(copyKernel, dim3(blocks), dim3(kBlock), 0, s, in, out, n);
     return hipGetLastError();
 }
 

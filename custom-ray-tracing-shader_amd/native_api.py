@@ -106,6 +106,7 @@ def load_library(path=None):
     lib.RtCopyOutput.argtypes = [vp, c.POINTER(c.c_ubyte), c.c_int]
     lib.RtStageTimes.argtypes = [vp, c.POINTER(c.c_float), c.POINTER(c.c_int), c.c_int]
     lib.RtTraceStats.argtypes = [vp, c.POINTER(c.c_ulonglong)]
+    lib.RtBandwidthProbe.argtypes = [c.c_longlong, c.c_int, c.POINTER(c.c_float)]
     return lib
 
 

@@ -116,9 +116,11 @@ int RtBvhExport(void* session, RtBvhNode* nodes, int nodeCapacity, int* order, i
 int RtReadAccumulation(void* session, float* out, long long floatCount);
 /* Ray queries issued by the last RtRender (closest-hit and shadow). */
 int RtRayCounts(void* session, unsigned long long* closest, unsigned long long* shadow);
-/* Tuning knobs: "bvh_width" (2, 4 or 8) and "blocks_per_cu" (persistent traversal
- * workgroups per compute unit, 1..64) take effect at the next RtInitialize; "stage_timing"
- * and "trace_stats" (0/1) at the next render.  Returns 1 if the key is known and the value
+/* Tuning knobs: "bvh_width" (2, 4 or 8) and "build_bins" (SAH bins per axis of the GPU
+ * build, 2..32, default 32) take effect at the next RtInitialize; "blocks_per_cu"
+ * (persistent traversal workgroups per compute unit, 1..32), "short_stack" (LDS stack
+ * entries per traversal lane, 8 or 16), "refill" (per-lane ray refill, 0/1),
+ * "stage_timing" and "trace_stats" (0/1) at the next render or query.  Returns 1 if the key is known and the value
  * valid.  RtGetOption returns -1 for unknown keys. */
 int RtSetOption(void* session, const char* key, int value);
 int RtGetOption(void* session, const char* key);
@@ -139,6 +141,9 @@ enum { RT_STAGE_RESET = 0, RT_STAGE_RAYGEN, RT_STAGE_CLOSEST, RT_STAGE_SHADE, RT
        RT_STAGE_COUNT };
 int RtStageTimes(void* session, float* milliseconds, int* launches, int count);
 int RtTraceStats(void* session, unsigned long long* counts4);
+/* Plain device-to-device copy of `bytes` (float4 grid-stride kernel), `iterations` times;
+ * milliseconds = mean kernel time.  Calibrates HBM byte counters against a known size. */
+int RtBandwidthProbe(long long bytes, int iterations, float* milliseconds);
 
 #ifdef __cplusplus
 }
