@@ -246,7 +246,7 @@ bool renderGpu(Session* s, const CameraData& cam, const RenderSettings& rs, floa
             f.bounce = b; f.rayIn = in; f.rayOut = out;
             {
                 StageScope sc(s, RT_STAGE_SHADE);
-                if ((s->error = launchShade(f, st)) != hipSuccess) return false;
+                if ((s->error = launchShade(f, s->cus * 8, st)) != hipSuccess) return false;
             }
             if (!s->lights.empty()) {
                 TraceArgs sa = baseTraceArgs(s);

@@ -279,28 +279,52 @@ __global__ void sahKernel(LevelArgs a) {
     a.split[i] = si;
 }
 
-// Scatter each split task's positions to its left / right child ranges.
+// Scatter each split task's positions to its left / right child ranges.  When a whole wave
+// works on one task (the common case near the root), each side takes one atomic per wave.
 __global__ __launch_bounds__(kB) void partitionKernel(const int* order, int* orderOut, const int* posTask,
                                                       int* posTaskOut, const float4* cen, const Task* tasks,
                                                       const SplitInfo* split, const int* box, int* cursors,
                                                       int nbins, int n, int* error) {
     const int j = blockIdx.x * kB + threadIdx.x;
-    if (j >= n) return;
-    const int t = posTask[j];
+    const int lane = threadIdx.x & 63;
+    const int t = j < n ? posTask[j] : -2;
     const SplitInfo si = t >= 0 ? split[t] : SplitInfo{-1, -1, 0, -1};
-    if (t < 0 || si.axis < 0) {
+    const bool moving = t >= 0 && si.axis >= 0;
+    if (j < n && !moving) {
         orderOut[j] = order[j];
         posTaskOut[j] = -1;
-        return;
     }
-    const int p = order[j];
-    const float4 c = cen[p];
-    const float cc = si.axis == 0 ? c.x : (si.axis == 1 ? c.y : c.z);
-    const int* bx = box + (size_t)t * 12;
-    const float lo = ordVal(bx[6 + si.axis]);
-    const float scale = nbins / (ordVal(bx[9 + si.axis]) - lo);
-    const int right = binIndex(cc, lo, scale, nbins) <= si.bin ? 0 : 1;
-    const int k = atomicAdd(&cursors[2 * t + right], 1);
+    int right = 0, p = 0;
+    if (moving) {
+        p = order[j];
+        const float4 c = cen[p];
+        const float cc = si.axis == 0 ? c.x : (si.axis == 1 ? c.y : c.z);
+        const int* bx = box + (size_t)t * 12;
+        const float lo = ordVal(bx[6 + si.axis]);
+        const float scale = nbins / (ordVal(bx[9 + si.axis]) - lo);
+        right = binIndex(cc, lo, scale, nbins) <= si.bin ? 0 : 1;
+    }
+    // Wave-uniform task?  (all lanes of the wave reach this point)
+    const int t0 = __shfl(t, 0, 64);
+    const unsigned long long all = __ballot(true);
+    const bool uniform = __ballot(moving && t == t0) == all;
+    int k;
+    if (uniform) {
+        const unsigned long long mr = __ballot(right == 1), ml = all & ~mr;
+        int baseL = 0, baseR = 0;
+        if (lane == 0) {
+            if (ml) baseL = atomicAdd(&cursors[2 * t0], __popcll(ml));
+            if (mr) baseR = atomicAdd(&cursors[2 * t0 + 1], __popcll(mr));
+        }
+        baseL = __shfl(baseL, 0, 64);
+        baseR = __shfl(baseR, 0, 64);
+        const unsigned long long mine = right ? mr : ml;
+        k = (right ? baseR : baseL) +
+            (int)__builtin_amdgcn_mbcnt_hi((unsigned)(mine >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mine, 0u));
+    } else {
+        k = moving ? atomicAdd(&cursors[2 * t + right], 1) : 0;
+    }
+    if (!moving) return;
     const int limit = right ? tasks[t].count - si.leftCount : si.leftCount;
     if (k >= limit) { atomicOr(error, 1); return; }   // bins and partition disagree: refuse
     const int dst = tasks[t].first + (right ? si.leftCount : 0) + k;

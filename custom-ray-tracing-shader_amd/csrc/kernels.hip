@@ -289,53 +289,57 @@ __device__ __forceinline__ void appendRay(bool pred, int* counter, float4* queue
     }
 }
 
+// Grid-stride over the bounce's queue, so the grid is sized to the GPU, not to the largest
+// possible queue; every lane of a wave reaches the appends in each round.
 __global__ __launch_bounds__(kBlock) void shadeKernel(FrameArgs f) {
     __shared__ unsigned sShift[8], sTerm[kMaxRaysPerPixel * 8];
     loadSobolTerms<8>(sShift, sTerm, f.rs.seed, 4 + 8 * f.bounce, f.sample0, f.R);
     const int count = f.counters[kCntRays + f.bounce];
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (blockIdx.x * kBlock >= count) return;   // whole block idle (uniform exit)
-    bool wantShadow = false, wantNext = false;
-    float4 sa = make_float4(0, 0, 0, 0), sb = sa, na = sa, nb = sa;
-    if (i < count) {
-        const float4 r0 = f.rayIn[2 * i], r1 = f.rayIn[2 * i + 1];
-        const int slot = __float_as_int(r1.w);
-        const int r = slot / f.npix;
-        const unsigned pixel = (unsigned)(slot - r * f.npix);
-        const int4 hv = f.hits[i];
-        RtHit hit;
-        hit.index = hv.x; hit.distance = __int_as_float(hv.y); hit.u = __int_as_float(hv.z); hit.v = __int_as_float(hv.w);
-        const float4 t4 = f.thr[slot], q4 = f.rad[slot];
-        PathState ps;
-        ps.thr = F3(t4.x, t4.y, t4.z); ps.prevPdf = t4.w; ps.rad = F3(q4.x, q4.y, q4.z);
-        FirstHit fh; fh.covered = false;
-        auto rng = [&](int k) { return sobolFromParts(pixel, sShift[k], sTerm[r * 8 + k]); };
-        ShadeResult res;
-        shadeBounce(f.sc, f.rs, f.bounce, rng, F3(r0.x, r0.y, r0.z), F3(r1.x, r1.y, r1.z), hit, ps, &fh, res);
-        f.thr[slot] = make_float4(ps.thr.x, ps.thr.y, ps.thr.z, ps.prevPdf);
-        f.rad[slot] = make_float4(ps.rad.x, ps.rad.y, ps.rad.z, q4.w);
-        if (fh.covered) {
-            f.alb[slot] = make_float4(fh.albedo.x, fh.albedo.y, fh.albedo.z, 1.0f);
-            f.nrm[slot] = make_float4(fh.normal.x, fh.normal.y, fh.normal.z, fh.depth);
+    for (int base = blockIdx.x * kBlock; base < count; base += gridDim.x * kBlock) {
+        const int i = base + threadIdx.x;
+        bool wantShadow = false, wantNext = false;
+        float4 sa = make_float4(0, 0, 0, 0), sb = sa, na = sa, nb = sa;
+        if (i < count) {
+            const float4 r0 = f.rayIn[2 * i], r1 = f.rayIn[2 * i + 1];
+            const int slot = __float_as_int(r1.w);
+            const int r = slot / f.npix;
+            const unsigned pixel = (unsigned)(slot - r * f.npix);
+            const int4 hv = f.hits[i];
+            RtHit hit;
+            hit.index = hv.x; hit.distance = __int_as_float(hv.y); hit.u = __int_as_float(hv.z); hit.v = __int_as_float(hv.w);
+            const float4 t4 = f.thr[slot], q4 = f.rad[slot];
+            PathState ps;
+            ps.thr = F3(t4.x, t4.y, t4.z); ps.prevPdf = t4.w; ps.rad = F3(q4.x, q4.y, q4.z);
+            FirstHit fh; fh.covered = false;
+            auto rng = [&](int k) { return sobolFromParts(pixel, sShift[k], sTerm[r * 8 + k]); };
+            ShadeResult res;
+            shadeBounce(f.sc, f.rs, f.bounce, rng, F3(r0.x, r0.y, r0.z), F3(r1.x, r1.y, r1.z), hit, ps, &fh, res);
+            f.thr[slot] = make_float4(ps.thr.x, ps.thr.y, ps.thr.z, ps.prevPdf);
+            f.rad[slot] = make_float4(ps.rad.x, ps.rad.y, ps.rad.z, q4.w);
+            if (fh.covered) {
+                f.alb[slot] = make_float4(fh.albedo.x, fh.albedo.y, fh.albedo.z, 1.0f);
+                f.nrm[slot] = make_float4(fh.normal.x, fh.normal.y, fh.normal.z, fh.depth);
+            }
+            if (res.shadow) {
+                wantShadow = true;
+                f.nee[slot] = make_float4(res.contrib.x, res.contrib.y, res.contrib.z, 0);
+                sa = make_float4(res.nextO.x, res.nextO.y, res.nextO.z, res.slimit);
+                sb = make_float4(res.sdir.x, res.sdir.y, res.sdir.z, __int_as_float(slot));
+            }
+            if (res.cont) {
+                wantNext = true;
+                na = make_float4(res.nextO.x, res.nextO.y, res.nextO.z, 1e30f);
+                nb = make_float4(res.nextD.x, res.nextD.y, res.nextD.z, __int_as_float(slot));
+            }
         }
-        if (res.shadow) {
-            wantShadow = true;
-            f.nee[slot] = make_float4(res.contrib.x, res.contrib.y, res.contrib.z, 0);
-            sa = make_float4(res.nextO.x, res.nextO.y, res.nextO.z, res.slimit);
-            sb = make_float4(res.sdir.x, res.sdir.y, res.sdir.z, __int_as_float(slot));
-        }
-        if (res.cont) {
-            wantNext = true;
-            na = make_float4(res.nextO.x, res.nextO.y, res.nextO.z, 1e30f);
-            nb = make_float4(res.nextD.x, res.nextD.y, res.nextD.z, __int_as_float(slot));
-        }
+        appendRay(wantShadow, &f.counters[kCntShadow + f.bounce], f.shadow, sa, sb);
+        appendRay(wantNext, &f.counters[kCntRays + f.bounce + 1], f.rayOut, na, nb);
     }
-    appendRay(wantShadow, &f.counters[kCntShadow + f.bounce], f.shadow, sa, sb);
-    appendRay(wantNext, &f.counters[kCntRays + f.bounce + 1], f.rayOut, na, nb);
 }
-hipError_t launchShade(const FrameArgs& f, hipStream_t s) {
+hipError_t launchShade(const FrameArgs& f, int blocks, hipStream_t s) {
     const int n = f.R * f.npix;
-    hipLaunchKernelGGL(shadeKernel, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, f);
+    const int need = (n + kBlock - 1) / kBlock;
+    hipLaunchKernelGGL(shadeKernel, dim3(need < blocks ? need : blocks), dim3(kBlock), 0, s, f);
     return hipGetLastError();
 }
 

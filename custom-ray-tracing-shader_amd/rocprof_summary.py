@@ -1,0 +1,116 @@
+"""Summarise rocprofv3 output of a bench.py run into a short Markdown table plus a JSON
+with the traversal kernel's HBM bytes per launch (read by bench.py as roofline.traffic).
+
+Inputs (directories written by rocprofv3 -d ... --output-format csv):
+  --trace DIR        --kernel-trace --stats run            (*_kernel_stats.csv)
+  --fetch DIR        --pmc FETCH_SIZE run of the workload   (*_counter_collection.csv)
+  --write DIR        --pmc WRITE_SIZE run of the workload
+  --tcc DIR          --pmc TCC_HIT_sum TCC_MISS_sum run     (optional)
+  --copy-fetch DIR / --copy-write DIR   the same counters over bench.py --copy-probe BYTES
+  --copy-bytes BYTES                    size the probe copied (reads BYTES, writes BYTES)
+FETCH_SIZE and WRITE_SIZE are in KiB per dispatch; each is multiplied by
+(expected KiB of the copy) / (counter KiB of the copy) before use.
+"""
+import argparse
+import csv
+import glob
+import json
+import os
+from collections import defaultdict
+
+
+def one_csv(directory, suffix):
+    files = sorted(glob.glob(os.path.join(directory, "**", f"*{suffix}"), recursive=True))
+    if not files:
+        raise SystemExit(f"no *{suffix} under {directory}")
+    return list(csv.DictReader(open(files[0])))
+
+
+def counter_means(directory):
+    """{kernel name: {counter: mean value per dispatch}}"""
+    acc = defaultdict(lambda: defaultdict(list))
+    for r in one_csv(directory, "counter_collection.csv"):
+        acc[r["Kernel_Name"]][r["Counter_Name"]].append(float(r["Counter_Value"]))
+    return {k: {c: sum(v) / len(v) for c, v in d.items()} for k, d in acc.items()}
+
+
+def short(name, n=72):
+    name = name.replace("rt::(anonymous namespace)::", "").replace("rt::", "")
+    return name if len(name) <= n else name[:n - 3] + "..."
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--trace", required=True)
+    ap.add_argument("--fetch", required=True)
+    ap.add_argument("--write", required=True)
+    ap.add_argument("--tcc")
+    ap.add_argument("--copy-fetch", required=True)
+    ap.add_argument("--copy-write", required=True)
+    ap.add_argument("--copy-bytes", type=int, required=True)
+    ap.add_argument("--kernel", required=True, help="substring naming the dominant kernel")
+    ap.add_argument("--title", required=True)
+    ap.add_argument("--commands", required=True, help="file holding the commands that made the inputs")
+    ap.add_argument("--bench-json", help="bench.py output line of the same workload (algorithmic bytes)")
+    ap.add_argument("--md", required=True)
+    ap.add_argument("--json", required=True)
+    args = ap.parse_args()
+
+    stats = one_csv(args.trace, "kernel_stats.csv")
+    fetch, write = counter_means(args.fetch), counter_means(args.write)
+    cf, cw = counter_means(args.copy_fetch), counter_means(args.copy_write)
+    copy_name = next(k for k in cf if "copyKernel" in k)
+    expected_kib = args.copy_bytes / 1024
+    fetch_scale = expected_kib / cf[copy_name]["FETCH_SIZE"]
+    write_scale = expected_kib / cw[copy_name]["WRITE_SIZE"]
+    tcc = counter_means(args.tcc) if args.tcc else {}
+
+    lines = [f"# {args.title}", "", "Commands (run from the repository root on one MI355X):", "", "```",
+             open(args.commands).read().strip(), "```", "",
+             f"HBM counter calibration with a plain {args.copy_bytes / 2**30:.2f} GiB device copy: "
+             f"FETCH_SIZE x {fetch_scale:.3f}, WRITE_SIZE x {write_scale:.3f}.", "",
+             "| kernel | calls | mean us | % time | HBM read MB/launch | HBM write MB/launch | L2 hit % |",
+             "|---|---:|---:|---:|---:|---:|---:|"]
+    dominant = None
+    for r in sorted(stats, key=lambda r: -float(r["Percentage"]))[:14]:
+        name = r["Name"]
+        f = fetch.get(name, {}).get("FETCH_SIZE")
+        w = write.get(name, {}).get("WRITE_SIZE")
+        t = tcc.get(name, {})
+        hit = 100 * t["TCC_HIT_sum"] / max(t["TCC_HIT_sum"] + t["TCC_MISS_sum"], 1) if t else None
+        rd = f * fetch_scale * 1024 / 1e6 if f is not None else None
+        wr = w * write_scale * 1024 / 1e6 if w is not None else None
+        fmt = lambda v, p=1: "—" if v is None else f"{v:.{p}f}"
+        lines.append(f"| `{short(name)}` | {r['Calls']} | {float(r['AverageNs']) / 1e3:.1f} | {float(r['Percentage']):.1f} "
+                     f"| {fmt(rd)} | {fmt(wr)} | {fmt(hit)} |")
+        if dominant is None and args.kernel in name:
+            dominant = {"kernel": name, "mean_ms": float(r["AverageNs"]) / 1e6, "calls": int(r["Calls"]),
+                        "hbm_read_bytes_per_launch": rd * 1e6 if rd is not None else None,
+                        "hbm_write_bytes_per_launch": wr * 1e6 if wr is not None else None,
+                        "l2_hit_percent": hit}
+    if dominant is None:
+        raise SystemExit(f"kernel {args.kernel!r} not in the trace")
+    traffic = (dominant["hbm_read_bytes_per_launch"] or 0) + (dominant["hbm_write_bytes_per_launch"] or 0)
+    dominant["hbm_bytes_per_launch"] = traffic
+    dominant["fetch_scale"], dominant["write_scale"] = fetch_scale, write_scale
+    lines += ["", f"Dominant kernel `{short(dominant['kernel'], 120)}`: mean {dominant['mean_ms'] * 1e3:.1f} us over "
+              f"{dominant['calls']} launches; HBM traffic {traffic / 1e6:.1f} MB per launch "
+              f"({traffic / dominant['mean_ms'] / 1e6:.0f} GB/s of HBM)."]
+    if args.bench_json:
+        b = json.loads(open(args.bench_json).read().strip().splitlines()[-1])
+        rf = b["roofline"]
+        alg = rf["algorithmic_bytes_per_launch"]
+        lines += [f"bench.py measured {rf['mean_launch_ms'] * 1e3:.1f} us per launch with HIP events; algorithmic "
+                  f"bytes {alg / 1e6:.1f} MB per launch ({rf['nodes_per_ray']} nodes and {rf['tris_per_ray']} "
+                  f"triangles per ray), i.e. {alg / max(traffic, 1):.1f}x the HBM traffic: the rest is served "
+                  f"by L2 / MALL."]
+        dominant["algorithmic_bytes_per_launch"] = alg
+    with open(args.md, "w") as fh:
+        fh.write("\n".join(lines) + "\n")
+    with open(args.json, "w") as fh:
+        json.dump(dominant, fh, indent=1)
+    print("\n".join(lines))
+
+
+if __name__ == "__main__":
+    main()
