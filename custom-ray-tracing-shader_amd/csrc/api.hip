@@ -2,6 +2,7 @@
 // CPU backends.  Entry points keep the reference plugin's names, argument meaning and
 // 1 = success / 0 = failure contract (Native/renderer.cu:309-458).
 #include <chrono>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -10,12 +11,13 @@
 #include "kernels.h"
 #include "traverse.h"
 #include "bvh_gpu.h"
+#include "denoiser.h"
 
 using namespace rt;
 
 namespace {
 
-struct HostSum { f3 rad, alb, nrm; float covered, depth; };
+struct HostSum { f3 rad, alb, nrm; float covered, depth, lum2; };
 
 struct Session {
     std::string hostError;
@@ -75,6 +77,15 @@ struct Session {
     int stageLaunches[RT_STAGE_COUNT] = {};
     unsigned long long traceCounts[4] = {};
     bool hasFrame = false;
+    // denoiser
+    DenoiserNet* denoiser = nullptr;   // borrowed (RtSetDenoiser) or owned
+    bool ownDenoiser = false;
+    int denoiseFallback = 0;           // 0 per-tile check, 1 always plain accumulation, 2 never
+    float* dDenoiseIn = nullptr;       // [hp][wp][10]
+    float* dDenoiseOut = nullptr;      // [hp][wp][4]
+    int* dTileStats = nullptr;         // [0] tiles, [1] rejected
+    int lastTiles = 0, lastRejected = 0;
+    float lastInferMs = 0;
 };
 
 // Brackets one launch with pooled events when stage timing is on.
@@ -115,6 +126,7 @@ void freeDevice(Session* s) {
     devFree(s->dThr); devFree(s->dRad); devFree(s->dAlb); devFree(s->dNrm); devFree(s->dNee);
     devFree(s->dRayA); devFree(s->dRayB); devFree(s->dShadow); devFree(s->dHits);
     devFree(s->dCounters); devFree(s->dSpill); devFree(s->dOverflow); devFree(s->dStats);
+    devFree(s->dDenoiseIn); devFree(s->dDenoiseOut); devFree(s->dTileStats);
     for (hipEvent_t e : s->eventPool)
         if (e) (void)hipEventDestroy(e);
     s->eventPool.clear();
@@ -352,6 +364,8 @@ void renderCpuRows(Session* s, const CameraData& cam, const RenderSettings& rs, 
                     tot.nrm = tot.nrm + fh.normal;
                     tot.covered = tot.covered + (fh.covered ? 1.0f : 0.0f);
                     tot.depth = tot.depth + fh.depth;
+                    const float lum = 0.2126f * ps.rad.x + 0.7152f * ps.rad.y + 0.0722f * ps.rad.z;
+                    tot.lum2 = tot.lum2 + lum * lum;
                 }
                 s->hostSum[i] = tot;
                 const float inv = 1.0f / (float)(sample + (unsigned)rs.raysPerPixel);
@@ -399,6 +413,9 @@ void RtShutdown(void* p) {
     if (!s) return;
     s->hostError.clear();
     freeDevice(s);
+    if (s->ownDenoiser) denoiserDestroy(s->denoiser);
+    if (s->ownDenoiser) s->denoiser = nullptr;
+    s->ownDenoiser = false;
     s->ready = false;
     s->tris.clear(); s->lights.clear(); s->binNodes.clear(); s->binOrder.clear();
     s->hostNodes.clear(); s->hostRecs.clear(); s->hostSum.clear();
@@ -569,6 +586,52 @@ int RtBandwidthProbe(long long bytes, int iterations, float* milliseconds) {
     return ok ? 1 : 0;
 }
 
+// Default weights: $RT_DENOISER_WEIGHTS, else denoiser_weights.bin next to this library.
+static std::string defaultWeightsPath() {
+    if (const char* env = std::getenv("RT_DENOISER_WEIGHTS")) return env;
+    Dl_info info;
+    if (dladdr(reinterpret_cast<void*>(&RtCreate), &info) && info.dli_fname) {
+        std::string so = info.dli_fname;
+        const size_t slash = so.rfind('/');
+        return (slash == std::string::npos ? std::string(".") : so.substr(0, slash)) + "/denoiser_weights.bin";
+    }
+    return "denoiser_weights.bin";
+}
+
+static bool ensureDenoiser(Session* s) {
+    if (s->denoiser) return true;
+    DenoiserNet* net = denoiserCreate();
+    if (!net) { s->hostError = "denoiser: out of device memory"; return false; }
+    if (!denoiserLoad(net, defaultWeightsPath().c_str(), s->hostError)) { denoiserDestroy(net); return false; }
+    s->denoiser = net;
+    s->ownDenoiser = true;
+    return true;
+}
+
+// Uploads the CPU backends' accumulation so the GPU stages see the same sums.
+static bool uploadHostSums(Session* s) {
+    const size_t npix = (size_t)s->width * s->height;
+    std::vector<float4> r(npix), a(npix), n(npix);
+    for (size_t i = 0; i < npix; i++) {
+        const HostSum& h = s->hostSum[i];
+        r[i] = make_float4(h.rad.x, h.rad.y, h.rad.z, h.covered);
+        a[i] = make_float4(h.alb.x, h.alb.y, h.alb.z, h.lum2);
+        n[i] = make_float4(h.nrm.x, h.nrm.y, h.nrm.z, h.depth);
+    }
+    return (s->error = hipMemcpy(s->dSumRad, r.data(), npix * 16, hipMemcpyHostToDevice)) == hipSuccess &&
+           (s->error = hipMemcpy(s->dSumAlb, a.data(), npix * 16, hipMemcpyHostToDevice)) == hipSuccess &&
+           (s->error = hipMemcpy(s->dSumNrm, n.data(), npix * 16, hipMemcpyHostToDevice)) == hipSuccess;
+}
+
+static bool denoiseBuffers(Session* s, int& hp, int& wp) {
+    hp = (s->height + 15) / 16 * 16;
+    wp = (s->width + 15) / 16 * 16;
+    if (!s->dDenoiseIn && !devAlloc(s, s->dDenoiseIn, (size_t)hp * wp * kDenoiseInputChannels)) return false;
+    if (!s->dDenoiseOut && !devAlloc(s, s->dDenoiseOut, (size_t)hp * wp * 4)) return false;
+    if (!s->dTileStats && !devAlloc(s, s->dTileStats, 2)) return false;
+    return true;
+}
+
 int RtDenoise(void* p, unsigned char* output, int outputBytes, float* milliseconds) {
     Session* s = static_cast<Session*>(p);
     if (!s) return 0;
@@ -577,8 +640,32 @@ int RtDenoise(void* p, unsigned char* output, int outputBytes, float* millisecon
         s->error = hipErrorInvalidValue;
         return 0;
     }
-    s->hostError = "denoiser weights not loaded";
-    return 0;
+    auto t0 = std::chrono::steady_clock::now();
+    s->hostError.clear();
+    if (!ensureDenoiser(s)) return 0;
+    if (s->lastBackend != RT_BACKEND_GPU && !uploadHostSums(s)) return 0;
+    int hp, wp;
+    if (!denoiseBuffers(s, hp, wp)) return 0;
+    hipStream_t st = s->stream;
+    DenoiseArgs a{};
+    a.sumRad = s->dSumRad; a.sumAlb = s->dSumAlb; a.sumNrm = s->dSumNrm;
+    a.w = s->width; a.h = s->height; a.wp = wp; a.hp = hp; a.samples = s->samples;
+    a.planes = s->dDenoiseIn; a.denoised = s->dDenoiseOut; a.pixels = s->dPixels; a.tileStats = s->dTileStats;
+    a.exposure = s->lastSettings.exposure; a.fallback = s->denoiseFallback;
+    if ((s->error = launchDenoisePrepare(a, st)) != hipSuccess) return 0;
+    if ((s->error = denoiserInferDevice(s->denoiser, s->dDenoiseIn, 1, hp, wp, s->dDenoiseOut, st)) != hipSuccess) return 0;
+    s->lastInferMs = denoiserLastForwardMs(s->denoiser);
+    if ((s->error = hipMemsetAsync(s->dTileStats, 0, 2 * sizeof(int), st)) != hipSuccess) return 0;
+    if ((s->error = launchDenoisePresent(a, st)) != hipSuccess) return 0;
+    if ((s->error = hipStreamSynchronize(st)) != hipSuccess) return 0;
+    int stats[2] = {0, 0};
+    if ((s->error = hipMemcpy(stats, s->dTileStats, sizeof(stats), hipMemcpyDeviceToHost)) != hipSuccess) return 0;
+    s->lastTiles = stats[0]; s->lastRejected = stats[1];
+    s->error = hipMemcpy(output, s->dPixels, (size_t)s->width * s->height * 4, hipMemcpyDeviceToHost);
+    if (s->error != hipSuccess) return 0;
+    *milliseconds = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    s->error = hipSuccess;
+    return 1;
 }
 
 // ------------------------------------------------------------------ extensions
@@ -668,6 +755,102 @@ int RtRayCounts(void* p, unsigned long long* closest, unsigned long long* shadow
     return 1;
 }
 
+int RtSetDenoiser(void* p, void* net) {
+    Session* s = static_cast<Session*>(p);
+    if (!s) return 0;
+    if (s->ownDenoiser) denoiserDestroy(s->denoiser);
+    s->denoiser = static_cast<DenoiserNet*>(net);
+    s->ownDenoiser = false;
+    return 1;
+}
+
+int RtDenoiseStats(void* p, int* tiles, int* rejected, float* inferMs) {
+    Session* s = static_cast<Session*>(p);
+    if (!s) return 0;
+    if (tiles) *tiles = s->lastTiles;
+    if (rejected) *rejected = s->lastRejected;
+    if (inferMs) *inferMs = s->lastInferMs;
+    return 1;
+}
+
+int RtReadDenoiserInputs(void* p, float* out, long long floatCount) {
+    Session* s = static_cast<Session*>(p);
+    if (!s || !s->ready || !s->samples || !out) return 0;
+    const size_t npix = (size_t)s->width * s->height;
+    if (floatCount < (long long)(npix * kDenoiseInputChannels)) { s->error = hipErrorInvalidValue; return 0; }
+    if (s->lastBackend != RT_BACKEND_GPU && !uploadHostSums(s)) return 0;
+    int hp, wp;
+    if (!denoiseBuffers(s, hp, wp)) return 0;
+    DenoiseArgs a{};
+    a.sumRad = s->dSumRad; a.sumAlb = s->dSumAlb; a.sumNrm = s->dSumNrm;
+    a.w = s->width; a.h = s->height; a.wp = wp; a.hp = hp; a.samples = s->samples; a.planes = s->dDenoiseIn;
+    if ((s->error = launchDenoisePrepare(a, s->stream)) != hipSuccess) return 0;
+    if ((s->error = hipStreamSynchronize(s->stream)) != hipSuccess) return 0;
+    std::vector<float> planes((size_t)hp * wp * kDenoiseInputChannels);
+    if ((s->error = hipMemcpy(planes.data(), s->dDenoiseIn, planes.size() * 4, hipMemcpyDeviceToHost)) != hipSuccess)
+        return 0;
+    for (int y = 0; y < s->height; y++)
+        std::memcpy(out + (size_t)y * s->width * kDenoiseInputChannels, planes.data() + (size_t)y * wp * kDenoiseInputChannels,
+                    (size_t)s->width * kDenoiseInputChannels * 4);
+    return 1;
+}
+
+// ------------------------------------------------------------------ denoiser network
+void* RtDenoiserCreate(void) { return denoiserCreate(); }
+void RtDenoiserDestroy(void* net) { denoiserDestroy(static_cast<DenoiserNet*>(net)); }
+long long RtDenoiserParamCount(void* net) { return denoiserParamCount(static_cast<DenoiserNet*>(net)); }
+int RtDenoiserInit(void* net, unsigned seed) { return net && denoiserInit(static_cast<DenoiserNet*>(net), seed) == hipSuccess; }
+int RtDenoiserGetParams(void* net, float* params) {
+    return net && params && denoiserGetParams(static_cast<DenoiserNet*>(net), params) == hipSuccess;
+}
+int RtDenoiserSetParams(void* net, const float* params) {
+    return net && params && denoiserSetParams(static_cast<DenoiserNet*>(net), params) == hipSuccess;
+}
+int RtDenoiserSave(void* net, const char* path) {
+    std::string err;
+    return net && path && denoiserSave(static_cast<DenoiserNet*>(net), path, err);
+}
+int RtDenoiserLoad(void* net, const char* path) {
+    std::string err;
+    return net && path && denoiserLoad(static_cast<DenoiserNet*>(net), path, err);
+}
+int RtDenoiserInfer(void* net, const float* input, int n, int h, int w, float* output) {
+    if (!net || !input || !output || n < 1 || h < 16 || w < 16 || h % 16 || w % 16) return 0;
+    return denoiserInferHost(static_cast<DenoiserNet*>(net), input, n, h, w, output) == hipSuccess;
+}
+int RtDenoiserGradients(void* net, const float* input, const float* target, int n, int h, int w, float* loss,
+                        float* grads) {
+    if (!net || !input || !target || !loss || !grads || n < 1 || h != w || h < 16 || h % 16) return 0;
+    return denoiserGradients(static_cast<DenoiserNet*>(net), input, target, n, h, w, loss, grads) == hipSuccess;
+}
+int RtDenoiserSetDataset(void* net, const float* inputs, const float* targets, int count, int h, int w) {
+    if (!net || !inputs || !targets || count < 1 || h < 16 || w < 16) return 0;
+    return denoiserSetDataset(static_cast<DenoiserNet*>(net), inputs, targets, count, h, w) == hipSuccess;
+}
+int RtDenoiserTrain(void* net, int steps, int batch, int crop, float learningRate, unsigned seed, float* losses) {
+    if (!net || steps < 1 || batch < 1 || crop < 16 || crop % 16 || !(learningRate > 0)) return 0;
+    return denoiserTrain(static_cast<DenoiserNet*>(net), steps, batch, crop, learningRate, seed, losses) == hipSuccess;
+}
+int RtDenoiserTimings(void* net, float* forwardMs, float* trainStepMs) {
+    if (!net) return 0;
+    if (forwardMs) *forwardMs = denoiserLastForwardMs(static_cast<DenoiserNet*>(net));
+    if (trainStepMs) *trainStepMs = denoiserLastTrainStepMs(static_cast<DenoiserNet*>(net));
+    return 1;
+}
+double RtDenoiserForwardFlops(int n, int h, int w) { return denoiserForwardFlops(n, h, w); }
+int RtConv3x3Forward(const float* in, int n, int h, int w, int cin, const float* weight, const float* bias, int cout,
+                     int relu, float* out) {
+    if (!in || !weight || !out || n < 1 || h < 1 || w < 1 || cin < 1 || cout < 1 || cin > 256 || cout > 256) return 0;
+    return convForwardHost(in, n, h, w, cin, weight, bias, cout, relu, out) == hipSuccess;
+}
+int RtConv3x3Backward(const float* in, int n, int h, int w, int cin, const float* weight, const float* gradOut, int cout,
+                      float* gradIn, float* gradWeight, float* gradBias) {
+    if (!in || !weight || !gradOut || !gradIn || !gradWeight || !gradBias || n < 1 || h < 1 || w < 1 || cin < 1 ||
+        cout < 1 || cin > 256 || cout > 256)
+        return 0;
+    return convBackwardHost(in, n, h, w, cin, weight, gradOut, cout, gradIn, gradWeight, gradBias) == hipSuccess;
+}
+
 int RtSetOption(void* p, const char* key, int value) {
     Session* s = static_cast<Session*>(p);
     if (!s || !key) return 0;
@@ -682,6 +865,10 @@ int RtSetOption(void* p, const char* key, int value) {
     if (!std::strcmp(key, "blocks_per_cu")) {
         if (value < 1 || value > 32) return 0;
         s->blocksPerCU = value; return 1;
+    }
+    if (!std::strcmp(key, "denoise_fallback")) {
+        if (value < 0 || value > 2) return 0;
+        s->denoiseFallback = value; return 1;
     }
     if (!std::strcmp(key, "short_stack")) {
         if (value != 8 && value != 16) return 0;
@@ -709,6 +896,7 @@ int RtGetOption(void* p, const char* key) {
     if (!std::strcmp(key, "blocks_per_cu")) return s->blocksPerCU;
     if (!std::strcmp(key, "build_bins")) return s->buildBins;
     if (!std::strcmp(key, "short_stack")) return s->shortStack;
+    if (!std::strcmp(key, "denoise_fallback")) return s->denoiseFallback;
     if (!std::strcmp(key, "refill")) return s->refill;
     if (!std::strcmp(key, "stage_timing")) return s->stageTiming ? 1 : 0;
     if (!std::strcmp(key, "trace_stats")) return s->traceStats ? 1 : 0;

@@ -1,0 +1,1002 @@
+// denoiser.hip - U-Net denoiser trained and run by hand-written gfx950 BF16 MFMA kernels.
+//
+// Network (all convolutions 3x3, stride 1, zero padding, ReLU except the last):
+//   level 0 (H x W)      e0a 10->32, e0b 32->32           ... d0a 96->32, d0b 32->32, out 32->3
+//   level 1 (H/2)        e1a 32->64, e1b 64->64           ... d1a 128->64, d1b 64->64
+//   level 2 (H/4)        e2a 64->64, e2b 64->64           ... d2a 160->64, d2b 64->64
+//   level 3 (H/8)        e3a 64->96, e3b 96->96           ... d3a 224->96, d3b 96->96
+//   level 4 (H/16)       ba 96->128, bb 128->128
+// 2x2 average pooling down, nearest upsampling up, skip connections by concatenation.
+// Input features: log(1+radiance) 3, albedo 3, normal 3, log(1+depth) 1 (zero-padded to 32
+// channels).  The output is added to log(1+noisy radiance) (residual) and mapped back with
+// exp(x)-1; training minimises the L1 distance in log space to a high-sample target.
+//
+// Tensors are NHWC BF16.  A concatenation is a buffer holding [upsampled | skip] channels:
+// producers write their slice through a (pointer, channel stride, channel offset) view, so no
+// copies happen.  Convolutions are implicit GEMMs on v_mfma_f32_16x16x32_bf16 with FP32
+// accumulation:
+//   forward      M = pixels, N = output channels, K = 9 taps x input channels
+//   input grad   the same kernel over the output gradient with the weights transposed and
+//                the taps mirrored
+//   weight grad  M = output channels, N = input channels, K = pixels; operands are staged in
+//                LDS transposed so each lane reads 8 consecutive pixels
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <random>
+#include "denoiser.h"
+
+namespace rt {
+
+namespace {
+
+typedef __bf16 bf16;
+typedef bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kB = 256;
+
+struct View {
+    bf16* p;
+    int cs;   // channels per pixel in the buffer
+    int co;   // channel offset of the view
+    int c;    // channels of the view
+};
+
+__device__ __forceinline__ bf16x8 zero8() {
+    bf16x8 z;
+#pragma unroll
+    for (int e = 0; e < 8; e++) z[e] = (bf16)0.0f;
+    return z;
+}
+
+// ------------------------------------------------------------------ convolution kernels
+struct ConvArgs {
+    View in;
+    const bf16* wt;       // [9][coutPad][cinPad]
+    const float* bias;    // [coutPad] or null
+    View out;             // optional BF16 output (p may be null)
+    float* outF;          // optional FP32 output, [pixel][outFStride], channels < outFStride
+    int outFStride;
+    int n, h, w;
+    int cinPad, coutPad;
+    int relu, accumulate;
+};
+
+template <int NT>
+__global__ __launch_bounds__(kB) void convFwdKernel(ConvArgs a) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int g = lane >> 4, r = lane & 15;
+    const int npix = a.n * a.h * a.w;
+    const int base = blockIdx.x * kB + wave * 64;
+    const int cob = blockIdx.y * 16 * NT;
+    int py[4], px[4], pn[4];
+    bool pv[4];
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+        const int p = base + t * 16 + r;
+        pv[t] = p < npix;
+        const int pp = pv[t] ? p : 0;
+        px[t] = pp % a.w;
+        const int q = pp / a.w;
+        py[t] = q % a.h;
+        pn[t] = q / a.h;
+    }
+    f32x4 acc[4][NT];
+#pragma unroll
+    for (int t = 0; t < 4; t++)
+#pragma unroll
+        for (int j = 0; j < NT; j++) acc[t][j] = f32x4{0, 0, 0, 0};
+    for (int tap = 0; tap < 9; tap++) {
+        const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+        const bf16* src[4];
+        bool ok[4];
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            const int yy = py[t] + dy, xx = px[t] + dx;
+            ok[t] = pv[t] && yy >= 0 && yy < a.h && xx >= 0 && xx < a.w;
+            const int q = ok[t] ? (pn[t] * a.h + yy) * a.w + xx : 0;
+            src[t] = a.in.p + (size_t)q * a.in.cs + a.in.co + 8 * g;
+        }
+        const bf16* wrow = a.wt + ((size_t)tap * a.coutPad + cob + r) * a.cinPad + 8 * g;
+        for (int ci0 = 0; ci0 < a.cinPad; ci0 += 32) {
+            bf16x8 av[4];
+#pragma unroll
+            for (int t = 0; t < 4; t++) av[t] = ok[t] ? *reinterpret_cast<const bf16x8*>(src[t] + ci0) : zero8();
+#pragma unroll
+            for (int j = 0; j < NT; j++) {
+                const bf16x8 bv = *reinterpret_cast<const bf16x8*>(wrow + (size_t)16 * j * a.cinPad + ci0);
+#pragma unroll
+                for (int t = 0; t < 4; t++) acc[t][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[t], bv, acc[t][j], 0, 0, 0);
+            }
+        }
+    }
+    // D layout of 16x16 MFMA: lane (g, r) holds rows 4g..4g+3 (pixels), column r (channel).
+#pragma unroll
+    for (int t = 0; t < 4; t++)
+#pragma unroll
+        for (int j = 0; j < NT; j++)
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int p = base + t * 16 + 4 * g + i;
+                const int c = cob + 16 * j + r;
+                if (p >= npix) continue;
+                float v = acc[t][j][i] + (a.bias ? a.bias[c] : 0.0f);
+                if (a.relu) v = fmaxf(v, 0.0f);
+                if (a.out.p && c < a.out.c) {
+                    bf16* dst = a.out.p + (size_t)p * a.out.cs + a.out.co + c;
+                    if (a.accumulate) v += (float)*dst;
+                    *dst = (bf16)v;
+                }
+                if (a.outF && c < a.outFStride) a.outF[(size_t)p * a.outFStride + c] = v;
+            }
+}
+
+struct BwdWArgs {
+    View dy;       // gradient w.r.t. the pre-activation output, coutPad channels
+    View in;       // layer input, cinPad channels
+    float* part;   // [parts][9][coutPad][cinPad], accumulated with atomics
+    int n, h, w;
+    int cinPad, coutPad;
+    int pixPerPart;
+};
+
+template <int TCO, int TCI>
+__global__ __launch_bounds__(kB) void convBwdWKernel(BwdWArgs a) {
+    __shared__ __attribute__((aligned(16))) bf16 sA[4][16 * TCO * 32];
+    __shared__ __attribute__((aligned(16))) bf16 sB[4][16 * TCI * 32];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int g = lane >> 4, r = lane & 15;
+    const int tap = blockIdx.y, dy = tap / 3 - 1, dx = tap % 3 - 1;
+    const int ncib = a.cinPad / (16 * TCI);
+    const int cob = (blockIdx.z / ncib) * 16 * TCO, cib = (blockIdx.z % ncib) * 16 * TCI;
+    const int npix = a.n * a.h * a.w;
+    const int begin = blockIdx.x * a.pixPerPart;
+    const int end = min(npix, begin + a.pixPerPart);
+    const int steps = (a.pixPerPart + 127) / 128;   // each step: 4 waves x 32 pixels
+    f32x4 acc[TCO][TCI];
+#pragma unroll
+    for (int i = 0; i < TCO; i++)
+#pragma unroll
+        for (int j = 0; j < TCI; j++) acc[i][j] = f32x4{0, 0, 0, 0};
+    for (int s = 0; s < steps; s++) {
+        const int p0 = begin + (s * 4 + wave) * 32;
+        for (int idx = lane; idx < 32 * 2 * TCO; idx += 64) {
+            const int pxl = idx / (2 * TCO), cg = idx % (2 * TCO);
+            const int p = p0 + pxl;
+            bf16x8 v = zero8();
+            if (p < end) v = *reinterpret_cast<const bf16x8*>(a.dy.p + (size_t)p * a.dy.cs + a.dy.co + cob + 8 * cg);
+#pragma unroll
+            for (int e = 0; e < 8; e++) sA[wave][(8 * cg + e) * 32 + pxl] = v[e];
+        }
+        for (int idx = lane; idx < 32 * 2 * TCI; idx += 64) {
+            const int pxl = idx / (2 * TCI), cg = idx % (2 * TCI);
+            const int p = p0 + pxl;
+            bf16x8 v = zero8();
+            if (p < end) {
+                const int x = p % a.w, q = p / a.w, y = q % a.h, nn = q / a.h;
+                const int yy = y + dy, xx = x + dx;
+                if (yy >= 0 && yy < a.h && xx >= 0 && xx < a.w) {
+                    const int qq = (nn * a.h + yy) * a.w + xx;
+                    v = *reinterpret_cast<const bf16x8*>(a.in.p + (size_t)qq * a.in.cs + a.in.co + cib + 8 * cg);
+                }
+            }
+#pragma unroll
+            for (int e = 0; e < 8; e++) sB[wave][(8 * cg + e) * 32 + pxl] = v[e];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < TCO; i++) {
+            const bf16x8 av = *reinterpret_cast<const bf16x8*>(&sA[wave][(16 * i + r) * 32 + 8 * g]);
+#pragma unroll
+            for (int j = 0; j < TCI; j++) {
+                const bf16x8 bv = *reinterpret_cast<const bf16x8*>(&sB[wave][(16 * j + r) * 32 + 8 * g]);
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc[i][j], 0, 0, 0);
+            }
+        }
+        __syncthreads();
+    }
+    float* part = a.part + (size_t)(blockIdx.x * 9 + tap) * a.coutPad * a.cinPad;
+#pragma unroll
+    for (int i = 0; i < TCO; i++)
+#pragma unroll
+        for (int j = 0; j < TCI; j++)
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const int co = cob + 16 * i + 4 * g + e, ci = cib + 16 * j + r;
+                atomicAdd(&part[(size_t)co * a.cinPad + ci], acc[i][j][e]);
+            }
+}
+
+__global__ void reducePartsKernel(const float* part, int parts, long long size, float* out) {
+    for (long long i = blockIdx.x * (long long)kB + threadIdx.x; i < size; i += (long long)gridDim.x * kB) {
+        float s = 0;
+        for (int k = 0; k < parts; k++) s += part[(size_t)k * size + i];
+        out[i] = s;
+    }
+}
+
+// dBias[c] += sum over a part of the pixels of dy[p][c].
+__global__ void biasGradKernel(View dy, int npix, int channels, int pixPerPart, float* db) {
+    const int c = threadIdx.x % channels;
+    const int lanesPer = kB / channels;
+    const int k = threadIdx.x / channels;
+    if (k >= lanesPer) return;
+    const int begin = blockIdx.x * pixPerPart, end = min(npix, begin + pixPerPart);
+    float s = 0;
+    for (int p = begin + k; p < end; p += lanesPer) s += (float)dy.p[(size_t)p * dy.cs + dy.co + c];
+    atomicAdd(&db[c], s);
+}
+
+// ------------------------------------------------------------------ elementwise kernels
+__global__ void reluMaskKernel(View g, View act, bf16* out, int npix, int channels) {
+    const long long i = blockIdx.x * (long long)kB + threadIdx.x;
+    if (i >= (long long)npix * channels) return;
+    const int p = (int)(i / channels), c = (int)(i % channels);
+    const float a = (float)act.p[(size_t)p * act.cs + act.co + c];
+    out[i] = a > 0.0f ? g.p[(size_t)p * g.cs + g.co + c] : (bf16)0.0f;
+}
+
+__global__ void avgPoolKernel(View in, View out, int n, int h, int w, int channels) {
+    const int ho = h / 2, wo = w / 2;
+    const long long i = blockIdx.x * (long long)kB + threadIdx.x;
+    if (i >= (long long)n * ho * wo * channels) return;
+    const int c = (int)(i % channels);
+    const long long po = i / channels;
+    const int x = (int)(po % wo), y = (int)((po / wo) % ho), nn = (int)(po / ((long long)wo * ho));
+    float s = 0;
+    for (int a = 0; a < 2; a++)
+        for (int b = 0; b < 2; b++) {
+            const size_t pi = ((size_t)nn * h + 2 * y + a) * w + 2 * x + b;
+            s += (float)in.p[pi * in.cs + in.co + c];
+        }
+    out.p[(size_t)po * out.cs + out.co + c] = (bf16)(0.25f * s);
+}
+
+// Adds the pooling gradient to the input-side gradient view.
+__global__ void avgPoolBackKernel(View gout, View gin, int n, int h, int w, int channels) {
+    const long long i = blockIdx.x * (long long)kB + threadIdx.x;
+    if (i >= (long long)n * h * w * channels) return;
+    const int c = (int)(i % channels);
+    const long long p = i / channels;
+    const int x = (int)(p % w), y = (int)((p / w) % h), nn = (int)(p / ((long long)w * h));
+    const size_t po = ((size_t)nn * (h / 2) + y / 2) * (w / 2) + x / 2;
+    bf16* dst = gin.p + (size_t)p * gin.cs + gin.co + c;
+    *dst = (bf16)((float)*dst + 0.25f * (float)gout.p[po * gout.cs + gout.co + c]);
+}
+
+__global__ void upsampleKernel(View in, View out, int n, int h, int w, int channels) {   // h, w: output size
+    const long long i = blockIdx.x * (long long)kB + threadIdx.x;
+    if (i >= (long long)n * h * w * channels) return;
+    const int c = (int)(i % channels);
+    const long long p = i / channels;
+    const int x = (int)(p % w), y = (int)((p / w) % h), nn = (int)(p / ((long long)w * h));
+    const size_t pi = ((size_t)nn * (h / 2) + y / 2) * (w / 2) + x / 2;
+    out.p[(size_t)p * out.cs + out.co + c] = in.p[pi * in.cs + in.co + c];
+}
+
+__global__ void upsampleBackKernel(View gout, View gin, int n, int h, int w, int channels) {   // h, w: input size
+    const long long i = blockIdx.x * (long long)kB + threadIdx.x;
+    if (i >= (long long)n * h * w * channels) return;
+    const int c = (int)(i % channels);
+    const long long p = i / channels;
+    const int x = (int)(p % w), y = (int)((p / w) % h), nn = (int)(p / ((long long)w * h));
+    float s = 0;
+    for (int a = 0; a < 2; a++)
+        for (int b = 0; b < 2; b++) {
+            const size_t po = ((size_t)nn * 2 * h + 2 * y + a) * (2 * w) + 2 * x + b;
+            s += (float)gout.p[po * gout.cs + gout.co + c];
+        }
+    gin.p[(size_t)p * gin.cs + gin.co + c] = (bf16)s;
+}
+
+// Raw planes [p][10] = radiance, albedo, normal, depth -> BF16 features [p][32] and the
+// log radiance the residual is added to.
+__device__ __forceinline__ void writeFeatures(const float* raw, bf16* x0, float* noisyLog, size_t p) {
+    float f[kDenoiseInputChannels];
+    for (int c = 0; c < 3; c++) f[c] = log1pf(fmaxf(raw[c], 0.0f));
+    for (int c = 3; c < 9; c++) f[c] = raw[c];
+    f[9] = log1pf(fmaxf(raw[9], 0.0f));
+    bf16* dst = x0 + p * 32;
+    for (int c = 0; c < 32; c++) dst[c] = (bf16)(c < kDenoiseInputChannels ? f[c] : 0.0f);
+    for (int c = 0; c < 3; c++) noisyLog[p * 4 + c] = f[c];
+    noisyLog[p * 4 + 3] = 0.0f;
+}
+
+__global__ void featuresKernel(const float* raw, int npix, bf16* x0, float* noisyLog) {
+    const int p = blockIdx.x * kB + threadIdx.x;
+    if (p >= npix) return;
+    writeFeatures(raw + (size_t)p * kDenoiseInputChannels, x0, noisyLog, p);
+}
+
+struct Crop { int image, ox, oy, flip; };
+
+__global__ void cropKernel(const float* dsIn, const float* dsTgt, int dsH, int dsW, const Crop* crops, int crop,
+                           int batch, bf16* x0, float* noisyLog, float* targetLog) {
+    const long long i = blockIdx.x * (long long)kB + threadIdx.x;
+    if (i >= (long long)batch * crop * crop) return;
+    const int x = (int)(i % crop), y = (int)((i / crop) % crop), b = (int)(i / ((long long)crop * crop));
+    const Crop cr = crops[b];
+    const int sx = cr.ox + ((cr.flip & 1) ? crop - 1 - x : x);
+    const int sy = cr.oy + ((cr.flip & 2) ? crop - 1 - y : y);
+    const size_t sp = ((size_t)cr.image * dsH + sy) * dsW + sx;
+    writeFeatures(dsIn + sp * kDenoiseInputChannels, x0, noisyLog, (size_t)i);
+    for (int c = 0; c < 3; c++) targetLog[i * 4 + c] = log1pf(fmaxf(dsTgt[sp * 3 + c], 0.0f));
+    targetLog[i * 4 + 3] = 0.0f;
+}
+
+// L1 in log space on the residual prediction; writes the output gradient (BF16, 32 ch).
+__global__ void lossKernel(const float* out, const float* noisyLog, const float* targetLog, int npix, bf16* gout,
+                           float* loss) {
+    __shared__ float red[kB];
+    const int p = blockIdx.x * kB + threadIdx.x;
+    float l = 0;
+    if (p < npix) {
+        const float scale = 1.0f / (3.0f * npix);
+        for (int c = 0; c < 32; c++) {
+            float gr = 0;
+            if (c < 3) {
+                const float d = noisyLog[(size_t)p * 4 + c] + out[(size_t)p * 4 + c] - targetLog[(size_t)p * 4 + c];
+                l += fabsf(d);
+                gr = d > 0 ? scale : (d < 0 ? -scale : 0.0f);
+            }
+            gout[(size_t)p * 32 + c] = (bf16)gr;
+        }
+    }
+    red[threadIdx.x] = l;
+    __syncthreads();
+    for (int s = kB / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicAdd(loss, red[0] / (3.0f * npix));
+}
+
+__global__ void finishOutputKernel(const float* out, const float* noisyLog, int npix, float* radiance) {
+    const int p = blockIdx.x * kB + threadIdx.x;
+    if (p >= npix) return;
+    for (int c = 0; c < 3; c++) radiance[(size_t)p * 4 + c] = fmaxf(expm1f(noisyLog[(size_t)p * 4 + c] + out[(size_t)p * 4 + c]), 0.0f);
+    radiance[(size_t)p * 4 + 3] = 0.0f;
+}
+
+__global__ void adamKernel(float* w, const float* g, float* m, float* v, long long count, float lr, float b1, float b2,
+                           float eps, float c1, float c2) {
+    const long long i = blockIdx.x * (long long)kB + threadIdx.x;
+    if (i >= count) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1 - b1) * gi, vi = b2 * v[i] + (1 - b2) * gi * gi;
+    m[i] = mi; v[i] = vi;
+    w[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+// FP32 master [9][co][ci] -> BF16 forward copy (same layout) and BF16 input-gradient copy
+// [9][ci][co] with mirrored taps.
+__global__ void packWeightsKernel(const float* w, int coPad, int ciPad, bf16* wf, bf16* wb) {
+    const long long i = blockIdx.x * (long long)kB + threadIdx.x;
+    if (i >= 9LL * coPad * ciPad) return;
+    const int ci = (int)(i % ciPad), co = (int)((i / ciPad) % coPad), tap = (int)(i / ((long long)ciPad * coPad));
+    const bf16 v = (bf16)w[i];
+    wf[i] = v;
+    wb[((size_t)(8 - tap) * ciPad + ci) * coPad + co] = v;
+}
+
+inline int blocks(long long n) { return (int)((n + kB - 1) / kB); }
+
+}  // namespace
+
+// ====================================================================== network
+struct DnLayer {
+    int cin, cout, cinPad, coutPad;
+    bool relu;
+    long long wOff, bOff;   // offsets in the FP32 parameter vector
+    long long bfOff;        // offset of this layer's BF16 copies (forward then input-gradient)
+};
+
+struct DenoiserNet {
+    std::vector<DnLayer> layers;
+    long long nParams = 0, nBf = 0;
+    float *params = nullptr, *grads = nullptr, *adamM = nullptr, *adamV = nullptr;
+    bf16* wbf = nullptr;
+    int adamStep = 0;
+    // activations for the current (n, h, w)
+    int n = 0, h = 0, w = 0;
+    std::vector<void*> bufs;
+    bf16 *x0 = nullptr, *e0a, *D0, *p0, *e1a, *D1, *p1, *e2a, *D2, *p2, *e3a, *D3, *p3, *ba, *bb, *d3a, *d3b, *d2a, *d2b,
+         *d1a, *d1b, *d0a, *d0b;
+    bf16 *gD0, *gD1, *gD2, *gD3, *ge0a, *gp0, *ge1a, *gp1, *ge2a, *gp2, *ge3a, *gp3, *gba, *gbb, *gd3a, *gd3b, *gd2a,
+         *gd2b, *gd1a, *gd1b, *gd0a, *gd0b, *gout, *dpre;
+    float *outF = nullptr, *noisyLog = nullptr, *targetLog = nullptr, *loss = nullptr;
+    float* part = nullptr;
+    size_t partCap = 0;
+    // dataset
+    float *dsIn = nullptr, *dsTgt = nullptr;
+    int dsN = 0, dsH = 0, dsW = 0;
+    Crop* dCrops = nullptr;
+    int cropCap = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    float lastFwdMs = 0, lastTrainMs = 0;
+};
+
+namespace {
+
+struct LayerSpec { int cin, cout; bool relu; };
+// e0a e0b e1a e1b e2a e2b e3a e3b ba bb d3a d3b d2a d2b d1a d1b d0a d0b out
+const LayerSpec kSpec[] = {{10, 32, true},  {32, 32, true},   {32, 64, true},  {64, 64, true}, {64, 64, true},
+                           {64, 64, true},  {64, 96, true},   {96, 96, true},  {96, 128, true}, {128, 128, true},
+                           {224, 96, true}, {96, 96, true},   {160, 64, true}, {64, 64, true}, {128, 64, true},
+                           {64, 64, true},  {96, 32, true},   {32, 32, true},  {32, 3, false}};
+constexpr int kLayers = sizeof(kSpec) / sizeof(kSpec[0]);
+enum { E0A, E0B, E1A, E1B, E2A, E2B, E3A, E3B, BA, BB, D3A, D3B, D2A, D2B, D1A, D1B, D0A, D0B, OUT };
+
+inline int pad32(int c) { return (c + 31) / 32 * 32; }
+
+View V(bf16* p, int cs, int co = 0, int c = -1) { return View{p, cs, co, c < 0 ? cs : c}; }
+
+bf16* wfOf(DenoiserNet* net, int l) { return net->wbf + net->layers[l].bfOff; }
+bf16* wbOf(DenoiserNet* net, int l) {
+    const DnLayer& L = net->layers[l];
+    return net->wbf + L.bfOff + 9LL * L.coutPad * L.cinPad;
+}
+
+hipError_t launchConvFwd(const ConvArgs& a, hipStream_t s) {
+    const int npix = a.n * a.h * a.w;
+    const int tiles = a.coutPad / 16;
+    if (a.cinPad % 32 || a.coutPad % 32 || a.in.c != a.cinPad) return hipErrorInvalidValue;
+    if (tiles % 4 == 0) hipLaunchKernelGGL(convFwdKernel<4>, dim3(blocks(npix), tiles / 4), dim3(kB), 0, s, a);
+    else hipLaunchKernelGGL(convFwdKernel<2>, dim3(blocks(npix), tiles / 2), dim3(kB), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t ensurePart(DenoiserNet* net, size_t floats) {
+    if (floats <= net->partCap) return hipSuccess;
+    if (net->part) (void)hipFree(net->part);
+    net->part = nullptr;
+    net->partCap = floats;
+    return hipMalloc(&net->part, floats * sizeof(float));
+}
+
+// Weight and bias gradients of one layer from the pre-activation gradient dy.
+hipError_t convWeightGrad(DenoiserNet* net, const View& dy, const View& in, int n, int h, int w, int cinPad,
+                          int coutPad, float* dW, float* dB, hipStream_t s) {
+    const int npix = n * h * w;
+    int parts = 64;
+    int per = (npix + parts - 1) / parts;
+    per = (per + 127) / 128 * 128;
+    parts = (npix + per - 1) / per;
+    const size_t size = 9ULL * coutPad * cinPad;
+    hipError_t e;
+    if ((e = ensurePart(net, size * parts)) != hipSuccess) return e;
+    if ((e = hipMemsetAsync(net->part, 0, size * parts * sizeof(float), s)) != hipSuccess) return e;
+    BwdWArgs a{dy, in, net->part, n, h, w, cinPad, coutPad, per};
+    const int tco = (coutPad / 16) % 4 == 0 ? 4 : 2, tci = (cinPad / 16) % 4 == 0 ? 4 : 2;
+    dim3 grid(parts, 9, (coutPad / (16 * tco)) * (cinPad / (16 * tci)));
+    if (tco == 4 && tci == 4) hipLaunchKernelGGL((convBwdWKernel<4, 4>), grid, dim3(kB), 0, s, a);
+    else if (tco == 4) hipLaunchKernelGGL((convBwdWKernel<4, 2>), grid, dim3(kB), 0, s, a);
+    else if (tci == 4) hipLaunchKernelGGL((convBwdWKernel<2, 4>), grid, dim3(kB), 0, s, a);
+    else hipLaunchKernelGGL((convBwdWKernel<2, 2>), grid, dim3(kB), 0, s, a);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    hipLaunchKernelGGL(reducePartsKernel, dim3(std::min<long long>(blocks((long long)size), 4096)), dim3(kB), 0, s,
+                       net->part, parts, (long long)size, dW);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if ((e = hipMemsetAsync(dB, 0, coutPad * sizeof(float), s)) != hipSuccess) return e;
+    hipLaunchKernelGGL(biasGradKernel, dim3(parts), dim3(kB), 0, s, dy, npix, coutPad, per, dB);
+    return hipGetLastError();
+}
+
+void freeActivations(DenoiserNet* net) {
+    for (void* p : net->bufs) (void)hipFree(p);
+    net->bufs.clear();
+    net->x0 = nullptr;
+    net->n = net->h = net->w = 0;
+}
+
+hipError_t ensureActivations(DenoiserNet* net, int n, int h, int w, bool training) {
+    if (net->x0 && net->n == n && net->h == h && net->w == w && (!training || net->gout)) return hipSuccess;
+    freeActivations(net);
+    net->gout = nullptr;
+    hipError_t err = hipSuccess;
+    const size_t P[5] = {(size_t)n * h * w, (size_t)n * (h / 2) * (w / 2), (size_t)n * (h / 4) * (w / 4),
+                         (size_t)n * (h / 8) * (w / 8), (size_t)n * (h / 16) * (w / 16)};
+    auto alloc = [&](bf16*& p, int level, int ch) {
+        if (err != hipSuccess) return;
+        void* q = nullptr;
+        err = hipMalloc(&q, P[level] * ch * sizeof(bf16));
+        if (err == hipSuccess) { net->bufs.push_back(q); p = static_cast<bf16*>(q); }
+    };
+    auto allocF = [&](float*& p, size_t count) {
+        if (err != hipSuccess) return;
+        void* q = nullptr;
+        err = hipMalloc(&q, count * sizeof(float));
+        if (err == hipSuccess) { net->bufs.push_back(q); p = static_cast<float*>(q); }
+    };
+    alloc(net->x0, 0, 32); alloc(net->e0a, 0, 32); alloc(net->D0, 0, 96); alloc(net->p0, 1, 32);
+    alloc(net->e1a, 1, 64); alloc(net->D1, 1, 128); alloc(net->p1, 2, 64); alloc(net->e2a, 2, 64);
+    alloc(net->D2, 2, 160); alloc(net->p2, 3, 64); alloc(net->e3a, 3, 96); alloc(net->D3, 3, 224);
+    alloc(net->p3, 4, 96); alloc(net->ba, 4, 128); alloc(net->bb, 4, 128); alloc(net->d3a, 3, 96);
+    alloc(net->d3b, 3, 96); alloc(net->d2a, 2, 64); alloc(net->d2b, 2, 64); alloc(net->d1a, 1, 64);
+    alloc(net->d1b, 1, 64); alloc(net->d0a, 0, 32); alloc(net->d0b, 0, 32);
+    allocF(net->outF, P[0] * 4); allocF(net->noisyLog, P[0] * 4); allocF(net->targetLog, P[0] * 4);
+    allocF(net->loss, 4);
+    if (training) {
+        alloc(net->gD0, 0, 96); alloc(net->gD1, 1, 128); alloc(net->gD2, 2, 160); alloc(net->gD3, 3, 224);
+        alloc(net->ge0a, 0, 32); alloc(net->gp0, 1, 32); alloc(net->ge1a, 1, 64); alloc(net->gp1, 2, 64);
+        alloc(net->ge2a, 2, 64); alloc(net->gp2, 3, 64); alloc(net->ge3a, 3, 96); alloc(net->gp3, 4, 96);
+        alloc(net->gba, 4, 128); alloc(net->gbb, 4, 128); alloc(net->gd3a, 3, 96); alloc(net->gd3b, 3, 96);
+        alloc(net->gd2a, 2, 64); alloc(net->gd2b, 2, 64); alloc(net->gd1a, 1, 64); alloc(net->gd1b, 1, 64);
+        alloc(net->gd0a, 0, 32); alloc(net->gd0b, 0, 32); alloc(net->dpre, 0, 32);
+        bf16* go = nullptr;
+        alloc(go, 0, 32);
+        if (err == hipSuccess) net->gout = go;
+    }
+    if (err != hipSuccess) { freeActivations(net); return err; }
+    net->n = n; net->h = h; net->w = w;
+    return hipSuccess;
+}
+
+hipError_t conv(DenoiserNet* net, int l, const View& in, const View& out, int n, int h, int w, hipStream_t s,
+                float* outF = nullptr) {
+    const DnLayer& L = net->layers[l];
+    ConvArgs a{};
+    a.in = in; a.wt = wfOf(net, l); a.bias = net->params + L.bOff; a.out = out;
+    a.outF = outF; a.outFStride = 4;
+    a.n = n; a.h = h; a.w = w; a.cinPad = L.cinPad; a.coutPad = L.coutPad; a.relu = L.relu; a.accumulate = 0;
+    return launchConvFwd(a, s);
+}
+
+hipError_t pool(const View& in, const View& out, int n, int h, int w, int c, hipStream_t s) {
+    hipLaunchKernelGGL(avgPoolKernel, dim3(blocks((long long)n * (h / 2) * (w / 2) * c)), dim3(kB), 0, s, in, out, n, h, w, c);
+    return hipGetLastError();
+}
+hipError_t upsample(const View& in, const View& out, int n, int h, int w, int c, hipStream_t s) {
+    hipLaunchKernelGGL(upsampleKernel, dim3(blocks((long long)n * h * w * c)), dim3(kB), 0, s, in, out, n, h, w, c);
+    return hipGetLastError();
+}
+
+#define DN_TRY(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) return e_; } while (0)
+
+// Forward pass from net->x0 to net->outF.
+hipError_t forward(DenoiserNet* net, hipStream_t s) {
+    const int n = net->n, h0 = net->h, w0 = net->w;
+    const int h1 = h0 / 2, w1 = w0 / 2, h2 = h0 / 4, w2 = w0 / 4, h3 = h0 / 8, w3 = w0 / 8, h4 = h0 / 16, w4 = w0 / 16;
+    DN_TRY(conv(net, E0A, V(net->x0, 32), V(net->e0a, 32), n, h0, w0, s));
+    DN_TRY(conv(net, E0B, V(net->e0a, 32), V(net->D0, 96, 64, 32), n, h0, w0, s));
+    DN_TRY(pool(V(net->D0, 96, 64, 32), V(net->p0, 32), n, h0, w0, 32, s));
+    DN_TRY(conv(net, E1A, V(net->p0, 32), V(net->e1a, 64), n, h1, w1, s));
+    DN_TRY(conv(net, E1B, V(net->e1a, 64), V(net->D1, 128, 64, 64), n, h1, w1, s));
+    DN_TRY(pool(V(net->D1, 128, 64, 64), V(net->p1, 64), n, h1, w1, 64, s));
+    DN_TRY(conv(net, E2A, V(net->p1, 64), V(net->e2a, 64), n, h2, w2, s));
+    DN_TRY(conv(net, E2B, V(net->e2a, 64), V(net->D2, 160, 96, 64), n, h2, w2, s));
+    DN_TRY(pool(V(net->D2, 160, 96, 64), V(net->p2, 64), n, h2, w2, 64, s));
+    DN_TRY(conv(net, E3A, V(net->p2, 64), V(net->e3a, 96), n, h3, w3, s));
+    DN_TRY(conv(net, E3B, V(net->e3a, 96), V(net->D3, 224, 128, 96), n, h3, w3, s));
+    DN_TRY(pool(V(net->D3, 224, 128, 96), V(net->p3, 96), n, h3, w3, 96, s));
+    DN_TRY(conv(net, BA, V(net->p3, 96), V(net->ba, 128), n, h4, w4, s));
+    DN_TRY(conv(net, BB, V(net->ba, 128), V(net->bb, 128), n, h4, w4, s));
+    DN_TRY(upsample(V(net->bb, 128), V(net->D3, 224, 0, 128), n, h3, w3, 128, s));
+    DN_TRY(conv(net, D3A, V(net->D3, 224), V(net->d3a, 96), n, h3, w3, s));
+    DN_TRY(conv(net, D3B, V(net->d3a, 96), V(net->d3b, 96), n, h3, w3, s));
+    DN_TRY(upsample(V(net->d3b, 96), V(net->D2, 160, 0, 96), n, h2, w2, 96, s));
+    DN_TRY(conv(net, D2A, V(net->D2, 160), V(net->d2a, 64), n, h2, w2, s));
+    DN_TRY(conv(net, D2B, V(net->d2a, 64), V(net->d2b, 64), n, h2, w2, s));
+    DN_TRY(upsample(V(net->d2b, 64), V(net->D1, 128, 0, 64), n, h1, w1, 64, s));
+    DN_TRY(conv(net, D1A, V(net->D1, 128), V(net->d1a, 64), n, h1, w1, s));
+    DN_TRY(conv(net, D1B, V(net->d1a, 64), V(net->d1b, 64), n, h1, w1, s));
+    DN_TRY(upsample(V(net->d1b, 64), V(net->D0, 96, 0, 64), n, h0, w0, 64, s));
+    DN_TRY(conv(net, D0A, V(net->D0, 96), V(net->d0a, 32), n, h0, w0, s));
+    DN_TRY(conv(net, D0B, V(net->d0a, 32), V(net->d0b, 32), n, h0, w0, s));
+    DN_TRY(conv(net, OUT, V(net->d0b, 32), View{nullptr, 32, 0, 32}, n, h0, w0, s, net->outF));
+    return hipSuccess;
+}
+
+// Backward through one convolution: weight/bias gradients and (if gin.p) the input
+// gradient, set or accumulated.  g is the gradient w.r.t. the layer's (post-ReLU) output.
+hipError_t convBack(DenoiserNet* net, int l, const View& g, const View& act, const View& in, const View& gin,
+                    bool accumulate, int n, int h, int w, hipStream_t s) {
+    const DnLayer& L = net->layers[l];
+    const int npix = n * h * w;
+    View dy = g;
+    if (L.relu) {
+        hipLaunchKernelGGL(reluMaskKernel, dim3(blocks((long long)npix * L.coutPad)), dim3(kB), 0, s, g, act, net->dpre,
+                           npix, L.coutPad);
+        DN_TRY(hipGetLastError());
+        dy = V(net->dpre, L.coutPad);
+    }
+    DN_TRY(convWeightGrad(net, dy, in, n, h, w, L.cinPad, L.coutPad, net->grads + L.wOff, net->grads + L.bOff, s));
+    if (gin.p) {
+        ConvArgs a{};
+        a.in = dy; a.wt = wbOf(net, l); a.bias = nullptr; a.out = gin; a.outF = nullptr; a.outFStride = 0;
+        a.n = n; a.h = h; a.w = w; a.cinPad = L.coutPad; a.coutPad = L.cinPad; a.relu = 0; a.accumulate = accumulate;
+        DN_TRY(launchConvFwd(a, s));
+    }
+    return hipSuccess;
+}
+
+hipError_t backward(DenoiserNet* net, hipStream_t s) {
+    const int n = net->n, h0 = net->h, w0 = net->w;
+    const int h1 = h0 / 2, w1 = w0 / 2, h2 = h0 / 4, w2 = w0 / 4, h3 = h0 / 8, w3 = w0 / 8, h4 = h0 / 16, w4 = w0 / 16;
+    const View none{nullptr, 0, 0, 0};
+    DN_TRY(convBack(net, OUT, V(net->gout, 32), none, V(net->d0b, 32), V(net->gd0b, 32), false, n, h0, w0, s));
+    DN_TRY(convBack(net, D0B, V(net->gd0b, 32), V(net->d0b, 32), V(net->d0a, 32), V(net->gd0a, 32), false, n, h0, w0, s));
+    DN_TRY(convBack(net, D0A, V(net->gd0a, 32), V(net->d0a, 32), V(net->D0, 96), V(net->gD0, 96), false, n, h0, w0, s));
+    hipLaunchKernelGGL(upsampleBackKernel, dim3(blocks((long long)n * h1 * w1 * 64)), dim3(kB), 0, s, V(net->gD0, 96, 0, 64),
+                       V(net->gd1b, 64), n, h1, w1, 64);
+    DN_TRY(convBack(net, D1B, V(net->gd1b, 64), V(net->d1b, 64), V(net->d1a, 64), V(net->gd1a, 64), false, n, h1, w1, s));
+    DN_TRY(convBack(net, D1A, V(net->gd1a, 64), V(net->d1a, 64), V(net->D1, 128), V(net->gD1, 128), false, n, h1, w1, s));
+    hipLaunchKernelGGL(upsampleBackKernel, dim3(blocks((long long)n * h2 * w2 * 64)), dim3(kB), 0, s, V(net->gD1, 128, 0, 64),
+                       V(net->gd2b, 64), n, h2, w2, 64);
+    DN_TRY(convBack(net, D2B, V(net->gd2b, 64), V(net->d2b, 64), V(net->d2a, 64), V(net->gd2a, 64), false, n, h2, w2, s));
+    DN_TRY(convBack(net, D2A, V(net->gd2a, 64), V(net->d2a, 64), V(net->D2, 160), V(net->gD2, 160), false, n, h2, w2, s));
+    hipLaunchKernelGGL(upsampleBackKernel, dim3(blocks((long long)n * h3 * w3 * 96)), dim3(kB), 0, s, V(net->gD2, 160, 0, 96),
+                       V(net->gd3b, 96), n, h3, w3, 96);
+    DN_TRY(convBack(net, D3B, V(net->gd3b, 96), V(net->d3b, 96), V(net->d3a, 96), V(net->gd3a, 96), false, n, h3, w3, s));
+    DN_TRY(convBack(net, D3A, V(net->gd3a, 96), V(net->d3a, 96), V(net->D3, 224), V(net->gD3, 224), false, n, h3, w3, s));
+    hipLaunchKernelGGL(upsampleBackKernel, dim3(blocks((long long)n * h4 * w4 * 128)), dim3(kB), 0, s, V(net->gD3, 224, 0, 128),
+                       V(net->gbb, 128), n, h4, w4, 128);
+    DN_TRY(convBack(net, BB, V(net->gbb, 128), V(net->bb, 128), V(net->ba, 128), V(net->gba, 128), false, n, h4, w4, s));
+    DN_TRY(convBack(net, BA, V(net->gba, 128), V(net->ba, 128), V(net->p3, 96), V(net->gp3, 96), false, n, h4, w4, s));
+    hipLaunchKernelGGL(avgPoolBackKernel, dim3(blocks((long long)n * h3 * w3 * 96)), dim3(kB), 0, s, V(net->gp3, 96),
+                       V(net->gD3, 224, 128, 96), n, h3, w3, 96);
+    DN_TRY(convBack(net, E3B, V(net->gD3, 224, 128, 96), V(net->D3, 224, 128, 96), V(net->e3a, 96), V(net->ge3a, 96), false, n, h3, w3, s));
+    DN_TRY(convBack(net, E3A, V(net->ge3a, 96), V(net->e3a, 96), V(net->p2, 64), V(net->gp2, 64), false, n, h3, w3, s));
+    hipLaunchKernelGGL(avgPoolBackKernel, dim3(blocks((long long)n * h2 * w2 * 64)), dim3(kB), 0, s, V(net->gp2, 64),
+                       V(net->gD2, 160, 96, 64), n, h2, w2, 64);
+    DN_TRY(convBack(net, E2B, V(net->gD2, 160, 96, 64), V(net->D2, 160, 96, 64), V(net->e2a, 64), V(net->ge2a, 64), false, n, h2, w2, s));
+    DN_TRY(convBack(net, E2A, V(net->ge2a, 64), V(net->e2a, 64), V(net->p1, 64), V(net->gp1, 64), false, n, h2, w2, s));
+    hipLaunchKernelGGL(avgPoolBackKernel, dim3(blocks((long long)n * h1 * w1 * 64)), dim3(kB), 0, s, V(net->gp1, 64),
+                       V(net->gD1, 128, 64, 64), n, h1, w1, 64);
+    DN_TRY(convBack(net, E1B, V(net->gD1, 128, 64, 64), V(net->D1, 128, 64, 64), V(net->e1a, 64), V(net->ge1a, 64), false, n, h1, w1, s));
+    DN_TRY(convBack(net, E1A, V(net->ge1a, 64), V(net->e1a, 64), V(net->p0, 32), V(net->gp0, 32), false, n, h1, w1, s));
+    hipLaunchKernelGGL(avgPoolBackKernel, dim3(blocks((long long)n * h0 * w0 * 32)), dim3(kB), 0, s, V(net->gp0, 32),
+                       V(net->gD0, 96, 64, 32), n, h0, w0, 32);
+    DN_TRY(convBack(net, E0B, V(net->gD0, 96, 64, 32), V(net->D0, 96, 64, 32), V(net->e0a, 32), V(net->ge0a, 32), false, n, h0, w0, s));
+    DN_TRY(convBack(net, E0A, V(net->ge0a, 32), V(net->e0a, 32), V(net->x0, 32), none, false, n, h0, w0, s));
+    return hipGetLastError();
+}
+
+hipError_t packAll(DenoiserNet* net, hipStream_t s) {
+    for (int l = 0; l < kLayers; l++) {
+        const DnLayer& L = net->layers[l];
+        hipLaunchKernelGGL(packWeightsKernel, dim3(blocks(9LL * L.coutPad * L.cinPad)), dim3(kB), 0, s, net->params + L.wOff,
+                           L.coutPad, L.cinPad, wfOf(net, l), wbOf(net, l));
+        DN_TRY(hipGetLastError());
+    }
+    return hipSuccess;
+}
+
+}  // namespace
+
+// ====================================================================== public functions
+DenoiserNet* denoiserCreate() {
+    DenoiserNet* net = new DenoiserNet();
+    long long off = 0, bfOff = 0;
+    for (const LayerSpec& sp : kSpec) {
+        DnLayer L;
+        L.cin = sp.cin; L.cout = sp.cout; L.cinPad = pad32(sp.cin); L.coutPad = pad32(sp.cout); L.relu = sp.relu;
+        L.wOff = off; off += 9LL * L.coutPad * L.cinPad;
+        L.bOff = off; off += L.coutPad;
+        L.bfOff = bfOff; bfOff += 2 * 9LL * L.coutPad * L.cinPad;
+        net->layers.push_back(L);
+    }
+    net->nParams = off; net->nBf = bfOff;
+    if (hipStreamCreateWithFlags(&net->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&net->ev0) != hipSuccess || hipEventCreate(&net->ev1) != hipSuccess ||
+        hipMalloc(&net->params, off * 4) != hipSuccess || hipMalloc(&net->grads, off * 4) != hipSuccess ||
+        hipMalloc(&net->adamM, off * 4) != hipSuccess || hipMalloc(&net->adamV, off * 4) != hipSuccess ||
+        hipMalloc(&net->wbf, bfOff * sizeof(bf16)) != hipSuccess) {
+        denoiserDestroy(net);
+        return nullptr;
+    }
+    return net;
+}
+
+void denoiserDestroy(DenoiserNet* net) {
+    if (!net) return;
+    freeActivations(net);
+    for (void* p : {(void*)net->params, (void*)net->grads, (void*)net->adamM, (void*)net->adamV, (void*)net->wbf,
+                    (void*)net->part, (void*)net->dsIn, (void*)net->dsTgt, (void*)net->dCrops})
+        if (p) (void)hipFree(p);
+    if (net->ev0) (void)hipEventDestroy(net->ev0);
+    if (net->ev1) (void)hipEventDestroy(net->ev1);
+    if (net->stream) (void)hipStreamDestroy(net->stream);
+    delete net;
+}
+
+long long denoiserParamCount(const DenoiserNet* net) { return net ? net->nParams : 0; }
+
+// He-normal weights (over the real fan-in), zero biases; the output layer starts small so
+// the initial network is close to the identity (residual formulation).
+hipError_t denoiserInit(DenoiserNet* net, unsigned seed) {
+    std::vector<float> host(net->nParams, 0.0f);
+    std::mt19937 rng(seed);
+    for (int l = 0; l < kLayers; l++) {
+        const DnLayer& L = net->layers[l];
+        std::normal_distribution<float> nd(0.0f, std::sqrt(2.0f / (9.0f * L.cin)) * (l == OUT ? 0.1f : 1.0f));
+        for (int tap = 0; tap < 9; tap++)
+            for (int co = 0; co < L.cout; co++)
+                for (int ci = 0; ci < L.cin; ci++) host[L.wOff + ((size_t)tap * L.coutPad + co) * L.cinPad + ci] = nd(rng);
+    }
+    return denoiserSetParams(net, host.data());
+}
+
+hipError_t denoiserGetParams(DenoiserNet* net, float* host) {
+    return hipMemcpy(host, net->params, net->nParams * 4, hipMemcpyDeviceToHost);
+}
+
+hipError_t denoiserSetParams(DenoiserNet* net, const float* host) {
+    DN_TRY(hipMemcpy(net->params, host, net->nParams * 4, hipMemcpyHostToDevice));
+    DN_TRY(hipMemset(net->adamM, 0, net->nParams * 4));
+    DN_TRY(hipMemset(net->adamV, 0, net->nParams * 4));
+    net->adamStep = 0;
+    DN_TRY(packAll(net, net->stream));
+    return hipStreamSynchronize(net->stream);
+}
+
+bool denoiserSave(DenoiserNet* net, const char* path, std::string& error) {
+    std::vector<float> host(net->nParams);
+    if (denoiserGetParams(net, host.data()) != hipSuccess) { error = "cannot read denoiser weights"; return false; }
+    FILE* f = std::fopen(path, "wb");
+    if (!f) { error = std::string("cannot write ") + path; return false; }
+    const unsigned hdr[4] = {kWeightFileMagic, kWeightFileVersion, (unsigned)kLayers, (unsigned)net->nParams};
+    bool ok = std::fwrite(hdr, sizeof(hdr), 1, f) == 1 && std::fwrite(host.data(), 4, host.size(), f) == host.size();
+    ok = (std::fclose(f) == 0) && ok;
+    if (!ok) error = std::string("short write to ") + path;
+    return ok;
+}
+
+bool denoiserLoad(DenoiserNet* net, const char* path, std::string& error) {
+    FILE* f = std::fopen(path, "rb");
+    if (!f) { error = std::string("denoiser weights not found: ") + path; return false; }
+    unsigned hdr[4] = {};
+    std::vector<float> host(net->nParams);
+    bool ok = std::fread(hdr, sizeof(hdr), 1, f) == 1;
+    if (ok && (hdr[0] != kWeightFileMagic || hdr[1] != kWeightFileVersion || hdr[2] != (unsigned)kLayers ||
+               hdr[3] != (unsigned)net->nParams)) {
+        error = std::string("denoiser weight file has another format: ") + path;
+        std::fclose(f);
+        return false;
+    }
+    ok = ok && std::fread(host.data(), 4, host.size(), f) == host.size();
+    std::fclose(f);
+    if (!ok) { error = std::string("truncated denoiser weight file: ") + path; return false; }
+    if (denoiserSetParams(net, host.data()) != hipSuccess) { error = "cannot upload denoiser weights"; return false; }
+    return true;
+}
+
+hipError_t denoiserInferDevice(DenoiserNet* net, const float* dInput, int n, int h, int w, float* dOutput, hipStream_t s) {
+    if (h % 16 || w % 16 || n < 1) return hipErrorInvalidValue;
+    DN_TRY(ensureActivations(net, n, h, w, false));
+    const int npix = n * h * w;
+    hipLaunchKernelGGL(featuresKernel, dim3(blocks(npix)), dim3(kB), 0, s, dInput, npix, net->x0, net->noisyLog);
+    DN_TRY(hipGetLastError());
+    DN_TRY(hipEventRecord(net->ev0, s));
+    DN_TRY(forward(net, s));
+    DN_TRY(hipEventRecord(net->ev1, s));
+    hipLaunchKernelGGL(finishOutputKernel, dim3(blocks(npix)), dim3(kB), 0, s, net->outF, net->noisyLog, npix, dOutput);
+    DN_TRY(hipGetLastError());
+    DN_TRY(hipEventSynchronize(net->ev1));
+    return hipEventElapsedTime(&net->lastFwdMs, net->ev0, net->ev1);
+}
+
+hipError_t denoiserInferHost(DenoiserNet* net, const float* input, int n, int h, int w, float* output) {
+    const size_t npix = (size_t)n * h * w;
+    float *di = nullptr, *dout = nullptr;
+    hipError_t e = hipMalloc(&di, npix * kDenoiseInputChannels * 4);
+    if (e == hipSuccess) e = hipMalloc(&dout, npix * 16);
+    if (e == hipSuccess) e = hipMemcpy(di, input, npix * kDenoiseInputChannels * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = denoiserInferDevice(net, di, n, h, w, dout, net->stream);
+    if (e == hipSuccess) {
+        std::vector<float> o(npix * 4);
+        e = hipMemcpy(o.data(), dout, npix * 16, hipMemcpyDeviceToHost);
+        for (size_t p = 0; p < npix && e == hipSuccess; p++)
+            for (int c = 0; c < 3; c++) output[p * 3 + c] = o[p * 4 + c];
+    }
+    if (di) (void)hipFree(di);
+    if (dout) (void)hipFree(dout);
+    return e;
+}
+
+hipError_t denoiserGradients(DenoiserNet* net, const float* input, const float* target, int n, int h, int w, float* loss,
+                             float* grads) {
+    if (h % 16 || w % 16 || n < 1) return hipErrorInvalidValue;
+    hipStream_t s = net->stream;
+    DN_TRY(ensureActivations(net, n, h, w, true));
+    const size_t npix = (size_t)n * h * w;
+    // Reuse the dataset path: the images are the dataset and each "crop" is a whole image.
+    DN_TRY(denoiserSetDataset(net, input, target, n, h, w));
+    std::vector<Crop> crops(n);
+    for (int b = 0; b < n; b++) crops[b] = Crop{b, 0, 0, 0};
+    if (net->cropCap < n) {
+        if (net->dCrops) (void)hipFree(net->dCrops);
+        net->dCrops = nullptr;
+        DN_TRY(hipMalloc(&net->dCrops, n * sizeof(Crop)));
+        net->cropCap = n;
+    }
+    if (h != w) return hipErrorInvalidValue;   // square images only (crop geometry)
+    DN_TRY(hipMemcpy(net->dCrops, crops.data(), n * sizeof(Crop), hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(cropKernel, dim3(blocks((long long)npix)), dim3(kB), 0, s, net->dsIn, net->dsTgt, h, w, net->dCrops, h, n,
+                       net->x0, net->noisyLog, net->targetLog);
+    DN_TRY(hipGetLastError());
+    DN_TRY(forward(net, s));
+    DN_TRY(hipMemsetAsync(net->loss, 0, 4, s));
+    hipLaunchKernelGGL(lossKernel, dim3(blocks((long long)npix)), dim3(kB), 0, s, net->outF, net->noisyLog, net->targetLog,
+                       (int)npix, net->gout, net->loss);
+    DN_TRY(hipGetLastError());
+    DN_TRY(backward(net, s));
+    DN_TRY(hipStreamSynchronize(s));
+    DN_TRY(hipMemcpy(loss, net->loss, 4, hipMemcpyDeviceToHost));
+    return hipMemcpy(grads, net->grads, net->nParams * 4, hipMemcpyDeviceToHost);
+}
+
+hipError_t denoiserSetDataset(DenoiserNet* net, const float* inputs, const float* targets, int count, int h, int w) {
+    if (net->dsIn) (void)hipFree(net->dsIn);
+    if (net->dsTgt) (void)hipFree(net->dsTgt);
+    net->dsIn = net->dsTgt = nullptr;
+    const size_t npix = (size_t)count * h * w;
+    DN_TRY(hipMalloc(&net->dsIn, npix * kDenoiseInputChannels * 4));
+    DN_TRY(hipMalloc(&net->dsTgt, npix * 3 * 4));
+    DN_TRY(hipMemcpy(net->dsIn, inputs, npix * kDenoiseInputChannels * 4, hipMemcpyHostToDevice));
+    DN_TRY(hipMemcpy(net->dsTgt, targets, npix * 3 * 4, hipMemcpyHostToDevice));
+    net->dsN = count; net->dsH = h; net->dsW = w;
+    return hipSuccess;
+}
+
+hipError_t denoiserTrain(DenoiserNet* net, int steps, int batch, int crop, float lr, unsigned seed, float* losses) {
+    if (!net->dsIn || crop % 16 || crop > net->dsH || crop > net->dsW || batch < 1 || steps < 1) return hipErrorInvalidValue;
+    hipStream_t s = net->stream;
+    DN_TRY(ensureActivations(net, batch, crop, crop, true));
+    std::mt19937 rng(seed);
+    std::vector<Crop> crops((size_t)steps * batch);
+    for (auto& c : crops) {
+        c.image = (int)(rng() % net->dsN);
+        c.ox = (int)(rng() % (net->dsW - crop + 1));
+        c.oy = (int)(rng() % (net->dsH - crop + 1));
+        c.flip = (int)(rng() % 4);
+    }
+    if (net->cropCap < (int)crops.size()) {
+        if (net->dCrops) (void)hipFree(net->dCrops);
+        net->dCrops = nullptr;
+        DN_TRY(hipMalloc(&net->dCrops, crops.size() * sizeof(Crop)));
+        net->cropCap = (int)crops.size();
+    }
+    DN_TRY(hipMemcpy(net->dCrops, crops.data(), crops.size() * sizeof(Crop), hipMemcpyHostToDevice));
+    float* dLoss = nullptr;
+    DN_TRY(hipMalloc(&dLoss, steps * sizeof(float)));
+    DN_TRY(hipMemsetAsync(dLoss, 0, steps * sizeof(float), s));
+    const int npix = batch * crop * crop;
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    hipError_t err = hipSuccess;
+    for (int it = 0; it < steps && err == hipSuccess; it++) {
+        if (it == steps - 1) err = hipEventRecord(net->ev0, s);
+        hipLaunchKernelGGL(cropKernel, dim3(blocks((long long)npix)), dim3(kB), 0, s, net->dsIn, net->dsTgt, net->dsH, net->dsW,
+                           net->dCrops + (size_t)it * batch, crop, batch, net->x0, net->noisyLog, net->targetLog);
+        if (err == hipSuccess) err = forward(net, s);
+        hipLaunchKernelGGL(lossKernel, dim3(blocks((long long)npix)), dim3(kB), 0, s, net->outF, net->noisyLog,
+                           net->targetLog, npix, net->gout, dLoss + it);
+        if (err == hipSuccess) err = backward(net, s);
+        if (it == steps - 1 && err == hipSuccess) err = hipEventRecord(net->ev1, s);
+        net->adamStep++;
+        const float c1 = 1 - std::pow(b1, (float)net->adamStep), c2 = 1 - std::pow(b2, (float)net->adamStep);
+        hipLaunchKernelGGL(adamKernel, dim3(blocks(net->nParams)), dim3(kB), 0, s, net->params, net->grads, net->adamM,
+                           net->adamV, net->nParams, lr, b1, b2, eps, c1, c2);
+        if (err == hipSuccess) err = hipGetLastError();
+        if (err == hipSuccess) err = packAll(net, s);
+    }
+    if (err == hipSuccess) err = hipStreamSynchronize(s);
+    if (err == hipSuccess) err = hipEventElapsedTime(&net->lastTrainMs, net->ev0, net->ev1);
+    if (err == hipSuccess && losses) err = hipMemcpy(losses, dLoss, steps * sizeof(float), hipMemcpyDeviceToHost);
+    (void)hipFree(dLoss);
+    return err;
+}
+
+double denoiserForwardFlops(int n, int h, int w) {
+    const int level[kLayers] = {0, 0, 1, 1, 2, 2, 3, 3, 4, 4, 3, 3, 2, 2, 1, 1, 0, 0, 0};
+    double f = 0;
+    for (int l = 0; l < kLayers; l++) {
+        const double pix = (double)n * (h >> level[l]) * (w >> level[l]);
+        f += 2.0 * 9.0 * pad32(kSpec[l].cin) * pad32(kSpec[l].cout) * pix;
+    }
+    return f;
+}
+float denoiserLastForwardMs(const DenoiserNet* net) { return net ? net->lastFwdMs : 0; }
+float denoiserLastTrainStepMs(const DenoiserNet* net) { return net ? net->lastTrainMs : 0; }
+
+// ---------------------------------------------------------------- single-layer checks
+hipError_t convForwardHost(const float* in, int n, int h, int w, int cin, const float* weight, const float* bias, int cout,
+                           int relu, float* out) {
+    const int ciPad = pad32(cin), coPad = pad32(cout);
+    const size_t npix = (size_t)n * h * w;
+    std::vector<bf16> hin(npix * ciPad, (bf16)0.0f), hw(9ULL * coPad * ciPad, (bf16)0.0f);
+    std::vector<float> hb(coPad, 0.0f);
+    for (size_t p = 0; p < npix; p++)
+        for (int c = 0; c < cin; c++) hin[p * ciPad + c] = (bf16)in[p * cin + c];
+    for (int t = 0; t < 9; t++)
+        for (int co = 0; co < cout; co++)
+            for (int ci = 0; ci < cin; ci++) hw[((size_t)t * coPad + co) * ciPad + ci] = (bf16)weight[((size_t)t * cout + co) * cin + ci];
+    for (int co = 0; co < cout; co++) hb[co] = bias ? bias[co] : 0.0f;
+    bf16 *din = nullptr, *dw = nullptr;
+    float *db = nullptr, *dout = nullptr;
+    hipError_t e = hipMalloc(&din, hin.size() * 2);
+    if (e == hipSuccess) e = hipMalloc(&dw, hw.size() * 2);
+    if (e == hipSuccess) e = hipMalloc(&db, coPad * 4);
+    if (e == hipSuccess) e = hipMalloc(&dout, npix * coPad * 4);
+    if (e == hipSuccess) e = hipMemcpy(din, hin.data(), hin.size() * 2, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dw, hw.data(), hw.size() * 2, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(db, hb.data(), coPad * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        ConvArgs a{};
+        a.in = View{din, ciPad, 0, ciPad}; a.wt = dw; a.bias = db; a.out = View{nullptr, coPad, 0, coPad};
+        a.outF = dout; a.outFStride = coPad; a.n = n; a.h = h; a.w = w; a.cinPad = ciPad; a.coutPad = coPad;
+        a.relu = relu; a.accumulate = 0;
+        e = launchConvFwd(a, nullptr);
+    }
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e == hipSuccess) {
+        std::vector<float> o(npix * coPad);
+        e = hipMemcpy(o.data(), dout, o.size() * 4, hipMemcpyDeviceToHost);
+        for (size_t p = 0; p < npix && e == hipSuccess; p++)
+            for (int c = 0; c < cout; c++) out[p * cout + c] = o[p * coPad + c];
+    }
+    for (void* p : {(void*)din, (void*)dw, (void*)db, (void*)dout})
+        if (p) (void)hipFree(p);
+    return e;
+}
+
+hipError_t convBackwardHost(const float* in, int n, int h, int w, int cin, const float* weight, const float* gradOut,
+                            int cout, float* gradIn, float* gradWeight, float* gradBias) {
+    DenoiserNet* net = denoiserCreate();   // only for its scratch buffers
+    if (!net) return hipErrorOutOfMemory;
+    const int ciPad = pad32(cin), coPad = pad32(cout);
+    const size_t npix = (size_t)n * h * w;
+    std::vector<bf16> hin(npix * ciPad, (bf16)0.0f), hg(npix * coPad, (bf16)0.0f), hwb(9ULL * ciPad * coPad, (bf16)0.0f);
+    for (size_t p = 0; p < npix; p++) {
+        for (int c = 0; c < cin; c++) hin[p * ciPad + c] = (bf16)in[p * cin + c];
+        for (int c = 0; c < cout; c++) hg[p * coPad + c] = (bf16)gradOut[p * cout + c];
+    }
+    for (int t = 0; t < 9; t++)
+        for (int co = 0; co < cout; co++)
+            for (int ci = 0; ci < cin; ci++)
+                hwb[((size_t)(8 - t) * ciPad + ci) * coPad + co] = (bf16)weight[((size_t)t * cout + co) * cin + ci];
+    bf16 *din = nullptr, *dg = nullptr, *dwb = nullptr;
+    float *dgi = nullptr, *dgw = nullptr, *dgb = nullptr;
+    hipError_t e = hipMalloc(&din, hin.size() * 2);
+    if (e == hipSuccess) e = hipMalloc(&dg, hg.size() * 2);
+    if (e == hipSuccess) e = hipMalloc(&dwb, hwb.size() * 2);
+    if (e == hipSuccess) e = hipMalloc(&dgi, npix * ciPad * 4);
+    if (e == hipSuccess) e = hipMalloc(&dgw, 9ULL * coPad * ciPad * 4);
+    if (e == hipSuccess) e = hipMalloc(&dgb, coPad * 4);
+    if (e == hipSuccess) e = hipMemcpy(din, hin.data(), hin.size() * 2, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dg, hg.data(), hg.size() * 2, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dwb, hwb.data(), hwb.size() * 2, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        ConvArgs a{};
+        a.in = View{dg, coPad, 0, coPad}; a.wt = dwb; a.bias = nullptr; a.out = View{nullptr, ciPad, 0, ciPad};
+        a.outF = dgi; a.outFStride = ciPad; a.n = n; a.h = h; a.w = w; a.cinPad = coPad; a.coutPad = ciPad;
+        a.relu = 0; a.accumulate = 0;
+        e = launchConvFwd(a, nullptr);
+    }
+    if (e == hipSuccess)
+        e = convWeightGrad(net, View{dg, coPad, 0, coPad}, View{din, ciPad, 0, ciPad}, n, h, w, ciPad, coPad, dgw, dgb, nullptr);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e == hipSuccess) {
+        std::vector<float> gi(npix * ciPad), gw(9ULL * coPad * ciPad), gb(coPad);
+        e = hipMemcpy(gi.data(), dgi, gi.size() * 4, hipMemcpyDeviceToHost);
+        if (e == hipSuccess) e = hipMemcpy(gw.data(), dgw, gw.size() * 4, hipMemcpyDeviceToHost);
+        if (e == hipSuccess) e = hipMemcpy(gb.data(), dgb, gb.size() * 4, hipMemcpyDeviceToHost);
+        if (e == hipSuccess) {
+            for (size_t p = 0; p < npix; p++)
+                for (int c = 0; c < cin; c++) gradIn[p * cin + c] = gi[p * ciPad + c];
+            for (int t = 0; t < 9; t++)
+                for (int co = 0; co < cout; co++)
+                    for (int ci = 0; ci < cin; ci++)
+                        gradWeight[((size_t)t * cout + co) * cin + ci] = gw[((size_t)t * coPad + co) * ciPad + ci];
+            for (int co = 0; co < cout; co++) gradBias[co] = gb[co];
+        }
+    }
+    for (void* p : {(void*)din, (void*)dg, (void*)dwb, (void*)dgi, (void*)dgw, (void*)dgb})
+        if (p) (void)hipFree(p);
+    denoiserDestroy(net);
+    return e;
+}
+
+}  // namespace rt

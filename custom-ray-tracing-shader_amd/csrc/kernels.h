@@ -51,6 +51,20 @@ struct AccumArgs {
     float invSamples, exposure;
 };
 
+struct DenoiseArgs {
+    const float4 *sumRad, *sumAlb, *sumNrm;   // radiance+covered, albedo+lum^2, normal+depth
+    int w, h, wp, hp;                          // image and padded (multiple of 16) sizes
+    unsigned samples;
+    float* planes;                             // [hp][wp][10] network input
+    const float* denoised;                     // [hp][wp][4] network output radiance
+    uchar4* pixels;
+    int* tileStats;                            // [0] tiles, [1] tiles that fell back
+    float exposure;
+    int fallback;                              // 0 per-tile check, 1 always fall back, 2 never
+};
+hipError_t launchDenoisePrepare(const DenoiseArgs& a, hipStream_t s);
+hipError_t launchDenoisePresent(const DenoiseArgs& a, hipStream_t s);
+
 // Persistent-lane launch of the traversal kernel: width W (2, 4 or 8), LDS short-stack
 // entries per lane (8 or 16), per-lane refill on or off.
 hipError_t launchTrace(int width, bool anyHit, bool stats, int blocks, int stack, bool refill, const TraceArgs& a,

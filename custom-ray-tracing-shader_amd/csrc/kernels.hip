@@ -357,6 +357,8 @@ __global__ __launch_bounds__(kBlock) void accumulateKernel(AccumArgs a) {
         sn.x = sn.x + pn.x; sn.y = sn.y + pn.y; sn.z = sn.z + pn.z;
         sr.w = sr.w + pa.w;   // covered count
         sn.w = sn.w + pn.w;   // first-hit depth sum
+        const float lum = 0.2126f * pr.x + 0.7152f * pr.y + 0.0722f * pr.z;
+        sa.w = sa.w + lum * lum;   // sum of squared path luminance (variance estimate)
     }
     a.sumRad[i] = sr; a.sumAlb[i] = sa; a.sumNrm[i] = sn;
     if (a.writeDisplay) {
@@ -367,6 +369,92 @@ __global__ __launch_bounds__(kBlock) void accumulateKernel(AccumArgs a) {
 }
 hipError_t launchAccumulate(const AccumArgs& a, hipStream_t s) {
     hipLaunchKernelGGL(accumulateKernel, dim3((a.npix + kBlock - 1) / kBlock), dim3(kBlock), 0, s, a);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ denoiser stages
+// Native/renderer.cu:422-428 (HDR colour and albedo averages, unit normal sum) plus the
+// mean first-hit depth; the image is edge-extended to the padded size.
+__global__ __launch_bounds__(kBlock) void denoisePrepareKernel(DenoiseArgs a) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= a.wp * a.hp) return;
+    const int x = min(i % a.wp, a.w - 1), y = min(i / a.wp, a.h - 1);
+    const int src = y * a.w + x;
+    const float inv = 1.0f / a.samples;
+    const float4 r = a.sumRad[src], al = a.sumAlb[src], n = a.sumNrm[src];
+    const f3 nu = unitv(F3(n.x, n.y, n.z));
+    float* o = a.planes + (size_t)i * 10;
+    o[0] = r.x * inv; o[1] = r.y * inv; o[2] = r.z * inv;
+    o[3] = al.x * inv; o[4] = al.y * inv; o[5] = al.z * inv;
+    o[6] = nu.x; o[7] = nu.y; o[8] = nu.z;
+    o[9] = r.w > 0 ? n.w / r.w : 0.0f;
+}
+hipError_t launchDenoisePrepare(const DenoiseArgs& a, hipStream_t s) {
+    hipLaunchKernelGGL(denoisePrepareKernel, dim3((a.wp * a.hp + kBlock - 1) / kBlock), dim3(kBlock), 0, s, a);
+    return hipGetLastError();
+}
+
+// One workgroup per 16x16 tile: accept the network's tile when its mean luminance stays
+// within 4 standard errors (plus 10%) of the accumulated mean, else show plain
+// accumulation; then mask pixels with no primary hit and tone-map
+// (Native/renderer.cu:429-435).
+__global__ __launch_bounds__(kBlock) void denoisePresentKernel(DenoiseArgs a) {
+    __shared__ float sNoisy[kBlock], sVar[kBlock], sDen[kBlock];
+    __shared__ int sBad[kBlock];
+    __shared__ int sReject;
+    const int lx = threadIdx.x & 15, ly = threadIdx.x >> 4;
+    const int x = blockIdx.x * 16 + lx, y = blockIdx.y * 16 + ly;
+    const bool valid = x < a.w && y < a.h;
+    const float inv = 1.0f / a.samples;
+    f3 noisy = F3(0, 0, 0), den = F3(0, 0, 0);
+    float covered = 0;
+    if (valid) {
+        const int i = y * a.w + x;
+        const float4 r = a.sumRad[i], al = a.sumAlb[i];
+        noisy = F3(r.x * inv, r.y * inv, r.z * inv);
+        covered = r.w;
+        const float* d = a.denoised + ((size_t)y * a.wp + x) * 4;
+        den = F3(d[0], d[1], d[2]);
+        const float mu = 0.2126f * noisy.x + 0.7152f * noisy.y + 0.0722f * noisy.z;
+        sNoisy[threadIdx.x] = mu;
+        sVar[threadIdx.x] = fmaxf(al.w * inv - mu * mu, 0.0f) * inv;
+        sDen[threadIdx.x] = 0.2126f * den.x + 0.7152f * den.y + 0.0722f * den.z;
+        sBad[threadIdx.x] = !(isfinite(den.x) && isfinite(den.y) && isfinite(den.z));
+    } else {
+        sNoisy[threadIdx.x] = sVar[threadIdx.x] = sDen[threadIdx.x] = 0;
+        sBad[threadIdx.x] = 0;
+    }
+    __syncthreads();
+    for (int st = kBlock / 2; st > 0; st >>= 1) {
+        if (threadIdx.x < st) {
+            sNoisy[threadIdx.x] += sNoisy[threadIdx.x + st];
+            sVar[threadIdx.x] += sVar[threadIdx.x + st];
+            sDen[threadIdx.x] += sDen[threadIdx.x + st];
+            sBad[threadIdx.x] |= sBad[threadIdx.x + st];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const int nx = min(16, a.w - blockIdx.x * 16), ny = min(16, a.h - blockIdx.y * 16);
+        const float cnt = (float)(nx * ny);
+        const float mn = sNoisy[0] / cnt, md = sDen[0] / cnt, se = sqrtf(sVar[0]) / cnt;
+        int reject = 0;
+        if (a.fallback == 1) reject = 1;
+        else if (a.fallback == 0) reject = sBad[0] || fabsf(md - mn) > 4.0f * se + 0.1f * mn + 1e-4f;
+        sReject = reject;
+        atomicAdd(&a.tileStats[0], 1);
+        if (reject) atomicAdd(&a.tileStats[1], 1);
+    }
+    __syncthreads();
+    if (!valid) return;
+    f3 c = sReject ? noisy : den;
+    if (!(isfinite(c.x) && isfinite(c.y) && isfinite(c.z))) c = noisy;
+    if (covered == 0) c = F3(0, 0, 0);
+    a.pixels[y * a.w + x] = make_uchar4(displayByte(c.x, a.exposure), displayByte(c.y, a.exposure),
+                                        displayByte(c.z, a.exposure), 255);
+}
+hipError_t launchDenoisePresent(const DenoiseArgs& a, hipStream_t s) {
+    hipLaunchKernelGGL(denoisePresentKernel, dim3((a.w + 15) / 16, (a.h + 15) / 16), dim3(kBlock), 0, s, a);
     return hipGetLastError();
 }
 

@@ -107,7 +107,108 @@ def load_library(path=None):
     lib.RtStageTimes.argtypes = [vp, c.POINTER(c.c_float), c.POINTER(c.c_int), c.c_int]
     lib.RtTraceStats.argtypes = [vp, c.POINTER(c.c_ulonglong)]
     lib.RtBandwidthProbe.argtypes = [c.c_longlong, c.c_int, c.POINTER(c.c_float)]
+    fp = c.POINTER(c.c_float)
+    lib.RtSetDenoiser.argtypes = [vp, vp]
+    lib.RtDenoiseStats.argtypes = [vp, c.POINTER(c.c_int), c.POINTER(c.c_int), fp]
+    lib.RtReadDenoiserInputs.argtypes = [vp, fp, c.c_longlong]
+    lib.RtDenoiserCreate.restype = vp
+    lib.RtDenoiserCreate.argtypes = []
+    lib.RtDenoiserDestroy.argtypes = [vp]
+    lib.RtDenoiserDestroy.restype = None
+    lib.RtDenoiserParamCount.argtypes = [vp]
+    lib.RtDenoiserParamCount.restype = c.c_longlong
+    lib.RtDenoiserInit.argtypes = [vp, c.c_uint]
+    lib.RtDenoiserGetParams.argtypes = [vp, fp]
+    lib.RtDenoiserSetParams.argtypes = [vp, fp]
+    lib.RtDenoiserSave.argtypes = [vp, c.c_char_p]
+    lib.RtDenoiserLoad.argtypes = [vp, c.c_char_p]
+    lib.RtDenoiserInfer.argtypes = [vp, fp, c.c_int, c.c_int, c.c_int, fp]
+    lib.RtDenoiserGradients.argtypes = [vp, fp, fp, c.c_int, c.c_int, c.c_int, fp, fp]
+    lib.RtDenoiserSetDataset.argtypes = [vp, fp, fp, c.c_int, c.c_int, c.c_int]
+    lib.RtDenoiserTrain.argtypes = [vp, c.c_int, c.c_int, c.c_int, c.c_float, c.c_uint, fp]
+    lib.RtDenoiserTimings.argtypes = [vp, fp, fp]
+    lib.RtDenoiserForwardFlops.argtypes = [c.c_int, c.c_int, c.c_int]
+    lib.RtDenoiserForwardFlops.restype = c.c_double
+    lib.RtConv3x3Forward.argtypes = [fp, c.c_int, c.c_int, c.c_int, c.c_int, fp, fp, c.c_int, c.c_int, fp]
+    lib.RtConv3x3Backward.argtypes = [fp, c.c_int, c.c_int, c.c_int, c.c_int, fp, fp, c.c_int, fp, fp, fp]
     return lib
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+class Denoiser:
+    """The learned denoiser network (RtDenoiser* exports)."""
+
+    def __init__(self, lib=None, seed=None, weights=None):
+        self.lib = lib or load_library()
+        self.handle = self.lib.RtDenoiserCreate()
+        if not self.handle:
+            raise RuntimeError("RtDenoiserCreate failed")
+        self.count = self.lib.RtDenoiserParamCount(self.handle)
+        if weights is not None:
+            if not self.lib.RtDenoiserLoad(self.handle, str(weights).encode()):
+                raise RuntimeError(f"cannot load {weights}")
+        elif seed is not None:
+            assert self.lib.RtDenoiserInit(self.handle, seed)
+
+    def close(self):
+        if self.handle:
+            self.lib.RtDenoiserDestroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def params(self):
+        out = np.zeros(self.count, np.float32)
+        assert self.lib.RtDenoiserGetParams(self.handle, as_pointer(out, c.c_float))
+        return out
+
+    @params.setter
+    def params(self, value):
+        value = _f32(value)
+        assert value.size == self.count and self.lib.RtDenoiserSetParams(self.handle, as_pointer(value, c.c_float))
+
+    def save(self, path):
+        assert self.lib.RtDenoiserSave(self.handle, str(path).encode())
+
+    def infer(self, raw):
+        raw = _f32(raw)
+        n, h, w, _ = raw.shape
+        out = np.zeros((n, h, w, 3), np.float32)
+        assert self.lib.RtDenoiserInfer(self.handle, as_pointer(raw, c.c_float), n, h, w, as_pointer(out, c.c_float))
+        return out
+
+    def gradients(self, raw, target):
+        raw, target = _f32(raw), _f32(target)
+        n, h, w, _ = raw.shape
+        loss = c.c_float()
+        grads = np.zeros(self.count, np.float32)
+        assert self.lib.RtDenoiserGradients(self.handle, as_pointer(raw, c.c_float), as_pointer(target, c.c_float), n, h,
+                                            w, c.byref(loss), as_pointer(grads, c.c_float))
+        return loss.value, grads
+
+    def set_dataset(self, inputs, targets):
+        inputs, targets = _f32(inputs), _f32(targets)
+        n, h, w, _ = inputs.shape
+        assert self.lib.RtDenoiserSetDataset(self.handle, as_pointer(inputs, c.c_float), as_pointer(targets, c.c_float),
+                                             n, h, w)
+
+    def train(self, steps, batch=8, crop=128, lr=2e-4, seed=1):
+        losses = np.zeros(steps, np.float32)
+        assert self.lib.RtDenoiserTrain(self.handle, steps, batch, crop, lr, seed, as_pointer(losses, c.c_float))
+        return losses
+
+    def timings(self):
+        f, t = c.c_float(), c.c_float()
+        self.lib.RtDenoiserTimings(self.handle, c.byref(f), c.byref(t))
+        return f.value, t.value
 
 
 STAGES = ("reset", "raygen", "closest", "shade", "shadow", "accumulate")
@@ -222,6 +323,22 @@ class Session:
         order = np.zeros(n, dtype=np.int32)
         self.lib.RtBvhExport(self.handle, None, 0, as_pointer(order, c.c_int), n)
         return order
+
+    def denoise(self):
+        out = (c.c_ubyte * (self.width * self.height * 4))()
+        ms = c.c_float()
+        self.check(self.lib.RtDenoise(self.handle, out, len(out), c.byref(ms)))
+        return np.frombuffer(bytes(out), dtype=np.uint8).reshape(self.height, self.width, 4), ms.value
+
+    def denoise_stats(self):
+        t, r, ms = c.c_int(), c.c_int(), c.c_float()
+        self.lib.RtDenoiseStats(self.handle, c.byref(t), c.byref(r), c.byref(ms))
+        return t.value, r.value, ms.value
+
+    def denoiser_inputs(self):
+        out = np.zeros(self.width * self.height * 10, np.float32)
+        self.check(self.lib.RtReadDenoiserInputs(self.handle, as_pointer(out, c.c_float), out.size))
+        return out.reshape(self.height, self.width, 10)
 
     def bvh_info(self):
         nodes, depth, ms = c.c_int(), c.c_int(), c.c_float()

@@ -1,0 +1,175 @@
+"""Train the learned denoiser on images this path tracer renders from seeded scenes.
+
+Dataset: for each seeded scene (displaced icosphere in a lit room, triangle soup under sky
+light, or a Cornell-box variant), a 4-spp render supplies the network input planes
+(RtReadDenoiserInputs: radiance, albedo, normal, depth) and an independently seeded
+high-sample render supplies the target radiance.  Training runs entirely on the GPU
+(RtDenoiserTrain: random crops + flips, Adam, L1 in log space).  The trained FP32 weights are
+written with a versioned header to denoiser_weights.bin next to the library, where RtDenoise
+finds them.
+
+    python train_denoiser.py --scenes 96 --target-spp 1024 --steps 3000 --out denoiser_weights.bin
+"""
+import argparse
+import json
+import math
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+if str(HERE) not in sys.path:
+    sys.path.insert(0, str(HERE))
+
+import scenes  # noqa: E402
+from native_api import Denoiser, Session, load_library  # noqa: E402
+
+
+def display(hdr, exposure=1.0):
+    """Native/renderer.cu:86-90 in float64: exposure tone map, sRGB encode, 8-bit."""
+    c = 1 - np.exp(-np.maximum(hdr, 0) * exposure)
+    c = np.where(c <= 0.0031308, 12.92 * c, 1.055 * np.power(c, 1 / 2.4) - 0.055)
+    return (255 * np.minimum(c, 1) + 0.5).astype(np.uint8)
+
+
+def psnr(a, b):
+    """PSNR of 8-bit display RGB (Tools/verify_quality.py:53-54 RMSE, as 20 log10(255/RMSE))."""
+    d = a[..., :3].astype(np.float64) - b[..., :3].astype(np.float64)
+    rmse = math.sqrt(float(np.mean(d * d)))
+    return 99.0 if rmse == 0 else 20 * math.log10(255 / rmse)
+
+
+def render_mean(session, camera, settings, spp, seed):
+    """Mean HDR radiance [h][w][3] of spp samples with the given seed (16 per call)."""
+    lib = session.lib
+    lib.RtReset(session.handle)
+    settings.seed = seed
+    done = 0
+    while done < spp:
+        settings.raysPerPixel = min(16, spp - done)
+        session.render(camera, settings)
+        done += settings.raysPerPixel
+    acc = session.accumulation()
+    return acc[..., 0:3] / spp
+
+
+def random_scene(seed):
+    """(triangles, camera factory, settings) of one seeded training scene."""
+    rng = np.random.default_rng(seed)
+    kind = seed % 3
+    if kind == 0:
+        tris = scenes.displaced_icosphere(subdiv=int(rng.integers(5, 8)), seed=int(seed), amplitude=float(rng.uniform(0.02, 0.1)),
+                                          metal_probability=float(rng.uniform(0, 0.5)))
+        tris["emission"] = np.where(tris["emission"] > 0, rng.uniform(4, 20), 0).astype(np.float32)
+        ang = rng.uniform(-0.8, 0.8)
+        dist = rng.uniform(2.4, 3.8)
+        eye = (math.sin(ang) * dist, rng.uniform(0.9, 2.4), -math.cos(ang) * dist)
+        cam = lambda w, h: scenes.make_camera(eye, (0, 1.25, 0), fov_deg=float(rng.uniform(35, 60)), aspect=w / h)
+        st = scenes.make_settings(8, sky=0.0)
+    elif kind == 1:
+        tris = scenes.triangle_soup(int(rng.integers(20000, 120000)), seed=int(seed))
+        eye = (rng.uniform(0.2, 0.8), rng.uniform(0.2, 0.8), rng.uniform(-1.4, -0.8))
+        cam = lambda w, h: scenes.make_camera(eye, (0.5, 0.5, 0.5), fov_deg=float(rng.uniform(40, 65)), aspect=w / h)
+        st = scenes.make_settings(3, sky=float(rng.uniform(0.5, 2.0)))
+    else:
+        tris = scenes.cornell()
+        wall = rng.uniform(0.1, 0.9, size=3)
+        tris["color"]["x"][6:8], tris["color"]["y"][6:8], tris["color"]["z"][6:8] = scenes.srgb_encode(wall)
+        tris["emission"][10:12] = rng.uniform(8, 25)
+        box = tris[12:].copy()
+        for f in ("origin",):
+            for a in "xz":
+                box[f][a] += rng.uniform(-0.15, 0.15)
+        tris = np.concatenate([tris[:12], box])
+        eye = (rng.uniform(-0.4, 0.4), rng.uniform(0.7, 1.3), -3.4)
+        cam = lambda w, h: scenes.make_camera(eye, (0, 1, 1), fov_deg=40, aspect=w / h)
+        st = scenes.make_settings(int(rng.integers(4, 9)))
+    return tris, cam, st
+
+
+def heldout_scene(seed):
+    tris, cam, st = random_scene(seed)
+    return cam(256, 256), st, tris
+
+
+def build_dataset(lib, count, size, input_spp, target_spp, first_seed, log):
+    inputs = np.zeros((count, size, size, 10), np.float32)
+    targets = np.zeros((count, size, size, 3), np.float32)
+    t0 = time.perf_counter()
+    for i in range(count):
+        tris, camf, st = random_scene(first_seed + i)
+        cam = camf(size, size)
+        with Session(lib) as s:
+            s.initialize(tris, size, size)
+            targets[i] = render_mean(s, cam, st, target_spp, seed=1000003 + i)
+            render_mean(s, cam, st, input_spp, seed=42 + i)
+            inputs[i] = s.denoiser_inputs()
+        if log and (i + 1) % 16 == 0:
+            log(f"dataset {i + 1}/{count} ({time.perf_counter() - t0:.1f} s)")
+    return inputs, targets
+
+
+def evaluate(net, inputs, targets):
+    """Mean display-space PSNR of plain 4-spp and of the network, vs the targets."""
+    out = net.infer(inputs)
+    p_in = np.mean([psnr(display(inputs[i, ..., 0:3]), display(targets[i])) for i in range(len(inputs))])
+    p_out = np.mean([psnr(display(out[i]), display(targets[i])) for i in range(len(inputs))])
+    return float(p_in), float(p_out)
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--scenes", type=int, default=96)
+    ap.add_argument("--heldout", type=int, default=8)
+    ap.add_argument("--size", type=int, default=256)
+    ap.add_argument("--input-spp", type=int, default=4)
+    ap.add_argument("--target-spp", type=int, default=1024)
+    ap.add_argument("--steps", type=int, default=3000)
+    ap.add_argument("--batch", type=int, default=8)
+    ap.add_argument("--crop", type=int, default=128)
+    ap.add_argument("--lr", type=float, default=3e-4)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--out", default=str(HERE / "denoiser_weights.bin"))
+    ap.add_argument("--report", default=None, help="write a JSON summary here")
+    args = ap.parse_args()
+    log = lambda m: print(m, file=sys.stderr, flush=True)
+    lib = load_library()
+    t0 = time.perf_counter()
+    inputs, targets = build_dataset(lib, args.scenes, args.size, args.input_spp, args.target_spp, 100, log)
+    hin, htg = build_dataset(lib, args.heldout, args.size, args.input_spp, args.target_spp, 100000, log)
+    data_s = time.perf_counter() - t0
+    net = Denoiser(lib, seed=args.seed)
+    net.set_dataset(inputs, targets)
+    before = evaluate(net, hin, htg)
+    log(f"held-out PSNR before training: plain {before[0]:.2f} dB, network {before[1]:.2f} dB")
+    losses = []
+    t0 = time.perf_counter()
+    chunks = 10
+    for k in range(chunks):
+        lr = args.lr * 0.5 * (1 + math.cos(math.pi * k / chunks))   # cosine decay in chunks
+        losses.append(net.train(args.steps // chunks, args.batch, args.crop, lr, args.seed + k))
+        log(f"steps {(k + 1) * (args.steps // chunks)}: lr {lr:.2e}, loss {losses[-1][-50:].mean():.4f}")
+    train_s = time.perf_counter() - t0
+    fwd_ms, step_ms = net.timings()
+    after = evaluate(net, hin, htg)
+    log(f"held-out PSNR after training: plain {after[0]:.2f} dB, network {after[1]:.2f} dB")
+    net.save(args.out)
+    flops = 3 * lib.RtDenoiserForwardFlops(args.batch, args.crop, args.crop)
+    report = {"scenes": args.scenes, "heldout": args.heldout, "size": args.size, "input_spp": args.input_spp,
+              "target_spp": args.target_spp, "steps": args.steps, "batch": args.batch, "crop": args.crop,
+              "dataset_seconds": round(data_s, 1), "train_seconds": round(train_s, 1),
+              "train_step_ms": round(step_ms, 3),
+              "train_step_mfma_utilization": round(flops / (step_ms / 1e3) / 2.5e15, 4) if step_ms > 0 else None,
+              "loss_first": float(np.concatenate(losses)[:20].mean()), "loss_last": float(np.concatenate(losses)[-100:].mean()),
+              "heldout_psnr_plain_db": round(after[0], 2), "heldout_psnr_network_db": round(after[1], 2),
+              "heldout_psnr_untrained_db": round(before[1], 2), "weights": str(args.out)}
+    print(json.dumps(report), flush=True)
+    if args.report:
+        Path(args.report).write_text(json.dumps(report, indent=1))
+    net.close()
+
+
+if __name__ == "__main__":
+    main()

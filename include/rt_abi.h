@@ -145,6 +145,45 @@ int RtTraceStats(void* session, unsigned long long* counts4);
  * milliseconds = mean kernel time.  Calibrates HBM byte counters against a known size. */
 int RtBandwidthProbe(long long bytes, int iterations, float* milliseconds);
 
+/* Learned denoiser used by RtDenoise.  A session loads $RT_DENOISER_WEIGHTS or
+ * denoiser_weights.bin next to the library on first use, unless RtSetDenoiser attached a
+ * network (the session borrows it; pass null to return to the default weights).  Option
+ * "denoise_fallback": 0 per-16x16-tile variance check (default), 1 always show plain
+ * accumulation, 2 never fall back.  RtDenoiseStats reports the last call's tile count,
+ * tiles that fell back and network milliseconds.  RtReadDenoiserInputs returns the
+ * h*w*10 network input planes (radiance, albedo, normal averages, mean first-hit depth). */
+int RtSetDenoiser(void* session, void* denoiser);
+int RtDenoiseStats(void* session, int* tiles, int* rejected, float* inferMs);
+int RtReadDenoiserInputs(void* session, float* out, long long floatCount);
+
+/* The network: 5-level U-Net of 3x3 convolutions (BF16 MFMA, FP32 accumulation).
+ * Parameters are FP32, per layer [9][coutPad][cinPad] weights then [coutPad] biases.
+ * Images are [n][h][w][10] raw planes in, [n][h][w][3] radiance out; h and w multiples of 16
+ * (RtDenoiserGradients also needs h == w).  Weight files carry a versioned header. */
+void* RtDenoiserCreate(void);
+void RtDenoiserDestroy(void* denoiser);
+long long RtDenoiserParamCount(void* denoiser);
+int RtDenoiserInit(void* denoiser, unsigned seed);
+int RtDenoiserGetParams(void* denoiser, float* params);
+int RtDenoiserSetParams(void* denoiser, const float* params);
+int RtDenoiserSave(void* denoiser, const char* path);
+int RtDenoiserLoad(void* denoiser, const char* path);
+int RtDenoiserInfer(void* denoiser, const float* input, int n, int h, int w, float* output);
+int RtDenoiserGradients(void* denoiser, const float* input, const float* target, int n, int h, int w, float* loss,
+                        float* grads);
+int RtDenoiserSetDataset(void* denoiser, const float* inputs, const float* targets, int count, int h, int w);
+/* steps Adam steps on random crop x crop windows (flips as augmentation) of the dataset;
+ * losses[steps] receives each step's L1 loss in log space. */
+int RtDenoiserTrain(void* denoiser, int steps, int batch, int crop, float learningRate, unsigned seed, float* losses);
+int RtDenoiserTimings(void* denoiser, float* forwardMs, float* trainStepMs);
+double RtDenoiserForwardFlops(int n, int h, int w);
+/* One 3x3 convolution (stride 1, zero padding, NHWC, weights [9][cout][cin]) and its
+ * gradients on the GPU, inputs rounded to BF16: the kernels the network is built from. */
+int RtConv3x3Forward(const float* in, int n, int h, int w, int cin, const float* weight, const float* bias, int cout,
+                     int relu, float* out);
+int RtConv3x3Backward(const float* in, int n, int h, int w, int cin, const float* weight, const float* gradOut,
+                      int cout, float* gradIn, float* gradWeight, float* gradBias);
+
 #ifdef __cplusplus
 }
 #endif
