@@ -92,6 +92,81 @@ def cpu_baseline(tris, cam, st, w, h, budget_s):
                       f"{rays} ray queries in {elapsed:.1f} s"}
 
 
+def psnr_study(args):
+    """Time to equal PSNR: learned denoise at 4 spp vs plain accumulation, against a
+    high-sample render (seed 20261023), 8-bit display PSNR as Tools/verify_quality.py:53-54.
+    Wall times include each RtRender's readback (Tools/verify_quality.py:72-78) and the
+    denoiser's transfers."""
+    import ctypes
+    import native_api
+    import train_denoiser as td
+    desc, tris, cam, st, w, h = workload("icosphere", 0)
+    lib = native_api.load_library()
+    s = native_api.Session(lib)
+    for k, v in TUNED["icosphere"].items():
+        s.set_option(k, v)
+    s.initialize(tris, w, h)
+    log(f"psnr study: {desc}; reference {args.reference_spp} spp")
+    t0 = time.perf_counter()
+    ref = td.render_mean(s, cam, st, args.reference_spp, seed=20261023)
+    ref_s = time.perf_counter() - t0
+    ref_img = td.display(ref, st.exposure)
+
+    def plain(spp, seed=42):
+        lib.RtReset(s.handle)
+        st.seed = seed
+        done, t = 0, time.perf_counter()
+        while done < spp:
+            st.raysPerPixel = min(16, spp - done)
+            img, _ = s.render(cam, st)
+            done += st.raysPerPixel
+        return img, time.perf_counter() - t
+
+    ladder = []
+    for spp in (4, 8, 16, 32, 64, 128, 256, 512):
+        img, sec = plain(spp)
+        ladder.append({"spp": spp, "seconds": round(sec, 4), "psnr_db": round(td.psnr(img, ref_img), 3)})
+        log(f"plain {spp:4d} spp: {sec * 1e3:8.1f} ms, PSNR {ladder[-1]['psnr_db']:.2f} dB")
+    plain(4)          # warm the denoiser path once (weights load, buffers)
+    s.denoise()
+    denoised = []
+    for _ in range(3):
+        t = time.perf_counter()
+        plain_img, _ = plain(4)
+        img, _ = s.denoise()
+        denoised.append((time.perf_counter() - t, img))
+    sec, img = min(denoised, key=lambda x: x[0])
+    tiles, rejected, infer_ms = s.denoise_stats()
+    dn_psnr = td.psnr(img, ref_img)
+    hp, wp = (h + 15) // 16 * 16, (w + 15) // 16 * 16
+    flops = lib.RtDenoiserForwardFlops(1, hp, wp)
+    # equal-PSNR point on the plain ladder (linear in log2 spp, times interpolated alike)
+    eq = None
+    for a, b in zip(ladder, ladder[1:]):
+        if a["psnr_db"] <= dn_psnr <= b["psnr_db"]:
+            f = (dn_psnr - a["psnr_db"]) / max(b["psnr_db"] - a["psnr_db"], 1e-9)
+            eq = {"spp": round(2 ** (np.log2(a["spp"]) + f * (np.log2(b["spp"]) - np.log2(a["spp"]))), 1),
+                  "seconds": a["seconds"] + f * (b["seconds"] - a["seconds"])}
+            break
+    if eq is None and dn_psnr > ladder[-1]["psnr_db"]:
+        eq = {"spp": f">{ladder[-1]['spp']}", "seconds": ladder[-1]["seconds"]}
+    p256 = next(x for x in ladder if x["spp"] == 256)
+    result = {"workload": desc, "reference_spp": args.reference_spp, "reference_seconds": round(ref_s, 2),
+              "plain": ladder, "denoised_4spp": {"seconds": round(sec, 4), "psnr_db": round(dn_psnr, 3),
+                                                 "plain_4spp_psnr_db": round(td.psnr(plain_img, ref_img), 3),
+                                                 "tiles": tiles, "tiles_fallback": rejected,
+                                                 "network_ms": round(infer_ms, 3),
+                                                 "network_gflop": round(flops / 1e9, 1),
+                                                 "mfma_utilization": round(flops / (infer_ms / 1e3) / 2.5e15, 4)},
+              "equal_psnr_plain": eq,
+              "time_to_equal_psnr_speedup": round(eq["seconds"] / sec, 2) if eq else None,
+              "vs_256spp": {"psnr_gap_db": round(dn_psnr - p256["psnr_db"], 3),
+                            "time_ratio": round(p256["seconds"] / sec, 2)}}
+    s.close()
+    print(json.dumps(result), flush=True)
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -107,6 +182,9 @@ def main():
                     help="search the launch configuration now instead of using the recorded tuned one")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--psnr-study", action="store_true",
+                    help="time-to-equal-PSNR of the learned denoiser vs plain accumulation on the icosphere")
+    ap.add_argument("--reference-spp", type=int, default=4096)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run the plain copy kernel over BYTES (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -119,6 +197,8 @@ def main():
         gbs = 2 * args.copy_probe / (ms.value / 1e3) / 1e9
         print(json.dumps({"copy_bytes": args.copy_probe, "mean_ms": ms.value, "read_plus_write_GBs": gbs}))
         return 0
+    if args.psnr_study:
+        return psnr_study(args)
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (paths are independent per pixel, "
               f"so a multi-GPU run would shard image rows; see the design notes)", file=sys.stderr)
@@ -200,6 +280,15 @@ def main():
     tf = ROOT / "profiles" / f"traffic_{args.workload}.json"
     if tf.exists():
         traffic = json.loads(tf.read_text()).get("hbm_bytes_per_launch")
+    denoise = None
+    pf = ROOT / "profiles" / "denoise_psnr_icosphere.json"
+    if pf.exists():
+        d = json.loads(pf.read_text())
+        denoise = {"source": "profiles/denoise_psnr_icosphere.json (bench.py --psnr-study)",
+                   "time_to_equal_psnr_speedup": d.get("time_to_equal_psnr_speedup"),
+                   "denoised_4spp_psnr_db": d["denoised_4spp"]["psnr_db"],
+                   "plain_256spp_psnr_db": next(x["psnr_db"] for x in d["plain"] if x["spp"] == 256),
+                   "equal_psnr_plain_spp": d["equal_psnr_plain"]["spp"] if d.get("equal_psnr_plain") else None}
 
     line = {
         "metric": "Mrays/s on 1 MI355X",
@@ -225,6 +314,7 @@ def main():
                      "nodes_per_ray": round(stats["closest_nodes"] / max(closest_rays, 1), 2),
                      "tris_per_ray": round(stats["closest_tris"] / max(closest_rays, 1), 2)},
         "cpu_baseline": baseline,
+        "time_to_equal_psnr": denoise,
     }
     s.close()
     print(json.dumps(line), flush=True)

@@ -9,7 +9,7 @@ namespace rt {
 
 constexpr int kDenoiseInputChannels = 10;   // log radiance 3, albedo 3, normal 3, log depth 1
 constexpr unsigned kWeightFileMagic = 0x4E445452u;   // "RTDN"
-constexpr unsigned kWeightFileVersion = 1;
+constexpr unsigned kWeightFileVersion = 2;   // 2: albedo-demodulated radiance features
 
 struct DenoiserNet;
 

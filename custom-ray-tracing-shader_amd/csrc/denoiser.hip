@@ -7,9 +7,11 @@
 //   level 3 (H/8)        e3a 64->96, e3b 96->96           ... d3a 224->96, d3b 96->96
 //   level 4 (H/16)       ba 96->128, bb 128->128
 // 2x2 average pooling down, nearest upsampling up, skip connections by concatenation.
-// Input features: log(1+radiance) 3, albedo 3, normal 3, log(1+depth) 1 (zero-padded to 32
-// channels).  The output is added to log(1+noisy radiance) (residual) and mapped back with
-// exp(x)-1; training minimises the L1 distance in log space to a high-sample target.
+// Input features: log(1 + radiance / (albedo + e)) 3 (albedo-demodulated), albedo 3, normal 3,
+// log(1+depth) 1, zero-padded to 32 channels (e = 0.01).  The output is added to the
+// demodulated log radiance (residual), mapped back with exp(x)-1 and re-modulated by
+// (albedo + e); training minimises the L1 distance in that demodulated log space to the
+// high-sample target.
 //
 // Tensors are NHWC BF16.  A concatenation is a buffer holding [upsampled | skip] channels:
 // producers write their slice through a (pointer, channel stride, channel offset) view, so no
@@ -290,11 +292,18 @@ __global__ void upsampleBackKernel(View gout, View gin, int n, int h, int w, int
     gin.p[(size_t)p * gin.cs + gin.co + c] = (bf16)s;
 }
 
-// Raw planes [p][10] = radiance, albedo, normal, depth -> BF16 features [p][32] and the
-// log radiance the residual is added to.
-__device__ __forceinline__ void writeFeatures(const float* raw, bf16* x0, float* noisyLog, size_t p) {
+constexpr float kDemodEps = 0.01f;
+
+// Raw planes [p][10] = radiance, albedo, normal, depth -> BF16 features [p][32], the
+// demodulated log radiance the residual is added to, and the modulation factors.
+__device__ __forceinline__ void writeFeatures(const float* raw, bf16* x0, float* noisyLog, float* demod, size_t p) {
     float f[kDenoiseInputChannels];
-    for (int c = 0; c < 3; c++) f[c] = log1pf(fmaxf(raw[c], 0.0f));
+    for (int c = 0; c < 3; c++) {
+        const float m = fmaxf(raw[3 + c], 0.0f) + kDemodEps;
+        f[c] = log1pf(fmaxf(raw[c], 0.0f) / m);
+        demod[p * 4 + c] = m;
+    }
+    demod[p * 4 + 3] = 0.0f;
     for (int c = 3; c < 9; c++) f[c] = raw[c];
     f[9] = log1pf(fmaxf(raw[9], 0.0f));
     bf16* dst = x0 + p * 32;
@@ -303,16 +312,16 @@ __device__ __forceinline__ void writeFeatures(const float* raw, bf16* x0, float*
     noisyLog[p * 4 + 3] = 0.0f;
 }
 
-__global__ void featuresKernel(const float* raw, int npix, bf16* x0, float* noisyLog) {
+__global__ void featuresKernel(const float* raw, int npix, bf16* x0, float* noisyLog, float* demod) {
     const int p = blockIdx.x * kB + threadIdx.x;
     if (p >= npix) return;
-    writeFeatures(raw + (size_t)p * kDenoiseInputChannels, x0, noisyLog, p);
+    writeFeatures(raw + (size_t)p * kDenoiseInputChannels, x0, noisyLog, demod, p);
 }
 
 struct Crop { int image, ox, oy, flip; };
 
 __global__ void cropKernel(const float* dsIn, const float* dsTgt, int dsH, int dsW, const Crop* crops, int crop,
-                           int batch, bf16* x0, float* noisyLog, float* targetLog) {
+                           int batch, bf16* x0, float* noisyLog, float* demod, float* targetLog) {
     const long long i = blockIdx.x * (long long)kB + threadIdx.x;
     if (i >= (long long)batch * crop * crop) return;
     const int x = (int)(i % crop), y = (int)((i / crop) % crop), b = (int)(i / ((long long)crop * crop));
@@ -320,8 +329,8 @@ __global__ void cropKernel(const float* dsIn, const float* dsTgt, int dsH, int d
     const int sx = cr.ox + ((cr.flip & 1) ? crop - 1 - x : x);
     const int sy = cr.oy + ((cr.flip & 2) ? crop - 1 - y : y);
     const size_t sp = ((size_t)cr.image * dsH + sy) * dsW + sx;
-    writeFeatures(dsIn + sp * kDenoiseInputChannels, x0, noisyLog, (size_t)i);
-    for (int c = 0; c < 3; c++) targetLog[i * 4 + c] = log1pf(fmaxf(dsTgt[sp * 3 + c], 0.0f));
+    writeFeatures(dsIn + sp * kDenoiseInputChannels, x0, noisyLog, demod, (size_t)i);
+    for (int c = 0; c < 3; c++) targetLog[i * 4 + c] = log1pf(fmaxf(dsTgt[sp * 3 + c], 0.0f) / demod[i * 4 + c]);
     targetLog[i * 4 + 3] = 0.0f;
 }
 
@@ -352,10 +361,13 @@ __global__ void lossKernel(const float* out, const float* noisyLog, const float*
     if (threadIdx.x == 0) atomicAdd(loss, red[0] / (3.0f * npix));
 }
 
-__global__ void finishOutputKernel(const float* out, const float* noisyLog, int npix, float* radiance) {
+__global__ void finishOutputKernel(const float* out, const float* noisyLog, const float* demod, int npix,
+                                   float* radiance) {
     const int p = blockIdx.x * kB + threadIdx.x;
     if (p >= npix) return;
-    for (int c = 0; c < 3; c++) radiance[(size_t)p * 4 + c] = fmaxf(expm1f(noisyLog[(size_t)p * 4 + c] + out[(size_t)p * 4 + c]), 0.0f);
+    for (int c = 0; c < 3; c++)
+        radiance[(size_t)p * 4 + c] =
+            fmaxf(expm1f(noisyLog[(size_t)p * 4 + c] + out[(size_t)p * 4 + c]), 0.0f) * demod[(size_t)p * 4 + c];
     radiance[(size_t)p * 4 + 3] = 0.0f;
 }
 
@@ -405,7 +417,7 @@ struct DenoiserNet {
          *d1a, *d1b, *d0a, *d0b;
     bf16 *gD0, *gD1, *gD2, *gD3, *ge0a, *gp0, *ge1a, *gp1, *ge2a, *gp2, *ge3a, *gp3, *gba, *gbb, *gd3a, *gd3b, *gd2a,
          *gd2b, *gd1a, *gd1b, *gd0a, *gd0b, *gout, *dpre;
-    float *outF = nullptr, *noisyLog = nullptr, *targetLog = nullptr, *loss = nullptr;
+    float *outF = nullptr, *noisyLog = nullptr, *demod = nullptr, *targetLog = nullptr, *loss = nullptr;
     float* part = nullptr;
     size_t partCap = 0;
     // dataset
@@ -517,6 +529,7 @@ hipError_t ensureActivations(DenoiserNet* net, int n, int h, int w, bool trainin
     alloc(net->d3b, 3, 96); alloc(net->d2a, 2, 64); alloc(net->d2b, 2, 64); alloc(net->d1a, 1, 64);
     alloc(net->d1b, 1, 64); alloc(net->d0a, 0, 32); alloc(net->d0b, 0, 32);
     allocF(net->outF, P[0] * 4); allocF(net->noisyLog, P[0] * 4); allocF(net->targetLog, P[0] * 4);
+    allocF(net->demod, P[0] * 4);
     allocF(net->loss, 4);
     if (training) {
         alloc(net->gD0, 0, 96); alloc(net->gD1, 1, 128); alloc(net->gD2, 2, 160); alloc(net->gD3, 3, 224);
@@ -767,12 +780,14 @@ hipError_t denoiserInferDevice(DenoiserNet* net, const float* dInput, int n, int
     if (h % 16 || w % 16 || n < 1) return hipErrorInvalidValue;
     DN_TRY(ensureActivations(net, n, h, w, false));
     const int npix = n * h * w;
-    hipLaunchKernelGGL(featuresKernel, dim3(blocks(npix)), dim3(kB), 0, s, dInput, npix, net->x0, net->noisyLog);
+    hipLaunchKernelGGL(featuresKernel, dim3(blocks(npix)), dim3(kB), 0, s, dInput, npix, net->x0, net->noisyLog,
+                       net->demod);
     DN_TRY(hipGetLastError());
     DN_TRY(hipEventRecord(net->ev0, s));
     DN_TRY(forward(net, s));
     DN_TRY(hipEventRecord(net->ev1, s));
-    hipLaunchKernelGGL(finishOutputKernel, dim3(blocks(npix)), dim3(kB), 0, s, net->outF, net->noisyLog, npix, dOutput);
+    hipLaunchKernelGGL(finishOutputKernel, dim3(blocks(npix)), dim3(kB), 0, s, net->outF, net->noisyLog, net->demod, npix,
+                       dOutput);
     DN_TRY(hipGetLastError());
     DN_TRY(hipEventSynchronize(net->ev1));
     return hipEventElapsedTime(&net->lastFwdMs, net->ev0, net->ev1);
@@ -815,7 +830,7 @@ hipError_t denoiserGradients(DenoiserNet* net, const float* input, const float* 
     if (h != w) return hipErrorInvalidValue;   // square images only (crop geometry)
     DN_TRY(hipMemcpy(net->dCrops, crops.data(), n * sizeof(Crop), hipMemcpyHostToDevice));
     hipLaunchKernelGGL(cropKernel, dim3(blocks((long long)npix)), dim3(kB), 0, s, net->dsIn, net->dsTgt, h, w, net->dCrops, h, n,
-                       net->x0, net->noisyLog, net->targetLog);
+                       net->x0, net->noisyLog, net->demod, net->targetLog);
     DN_TRY(hipGetLastError());
     DN_TRY(forward(net, s));
     DN_TRY(hipMemsetAsync(net->loss, 0, 4, s));
@@ -869,7 +884,8 @@ hipError_t denoiserTrain(DenoiserNet* net, int steps, int batch, int crop, float
     for (int it = 0; it < steps && err == hipSuccess; it++) {
         if (it == steps - 1) err = hipEventRecord(net->ev0, s);
         hipLaunchKernelGGL(cropKernel, dim3(blocks((long long)npix)), dim3(kB), 0, s, net->dsIn, net->dsTgt, net->dsH, net->dsW,
-                           net->dCrops + (size_t)it * batch, crop, batch, net->x0, net->noisyLog, net->targetLog);
+                           net->dCrops + (size_t)it * batch, crop, batch, net->x0, net->noisyLog, net->demod,
+                           net->targetLog);
         if (err == hipSuccess) err = forward(net, s);
         hipLaunchKernelGGL(lossKernel, dim3(blocks((long long)npix)), dim3(kB), 0, s, net->outF, net->noisyLog,
                            net->targetLog, npix, net->gout, dLoss + it);

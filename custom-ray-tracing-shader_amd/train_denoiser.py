@@ -60,7 +60,7 @@ def random_scene(seed):
     rng = np.random.default_rng(seed)
     kind = seed % 3
     if kind == 0:
-        tris = scenes.displaced_icosphere(subdiv=int(rng.integers(5, 8)), seed=int(seed), amplitude=float(rng.uniform(0.02, 0.1)),
+        tris = scenes.displaced_icosphere(subdiv=int(rng.integers(5, 9)), seed=int(seed), amplitude=float(rng.uniform(0.02, 0.1)),
                                           metal_probability=float(rng.uniform(0, 0.5)))
         tris["emission"] = np.where(tris["emission"] > 0, rng.uniform(4, 20), 0).astype(np.float32)
         ang = rng.uniform(-0.8, 0.8)

@@ -2,8 +2,9 @@
 
 Restates the network of custom-ray-tracing-shader_amd/csrc/denoiser.hip (layer list,
 padded parameter layout [9][coutPad][cinPad] + bias[coutPad] per layer, 2x2 average
-pooling, nearest upsampling, concatenation order [upsampled | skip], log-space residual
-output and L1 loss) in double precision, with a hand-written backward pass.
+pooling, nearest upsampling, concatenation order [upsampled | skip], albedo-demodulated
+log-space residual output and L1 loss) in double precision, with a hand-written backward
+pass.
 
 Pinned by: conv2d forward against scipy.signal.correlate (an independent library routine),
 every backward (layer and whole network) against central finite differences in FP64
@@ -91,20 +92,25 @@ def upsample_backward(g):
     return g.reshape(n, h // 2, 2, w // 2, 2, c).sum(axis=(2, 4))
 
 
+DEMOD_EPS = 0.01
+
+
 def features(raw):
-    """raw [n,h,w,10] (radiance, albedo, normal, depth) -> features [n,h,w,32], log radiance."""
+    """raw [n,h,w,10] (radiance, albedo, normal, depth) -> features [n,h,w,32], demodulated
+    log radiance, modulation factors (albedo + eps)."""
     f = np.zeros(raw.shape[:3] + (32,))
-    f[..., 0:3] = np.log1p(np.maximum(raw[..., 0:3], 0))
+    m = np.maximum(raw[..., 3:6], 0) + DEMOD_EPS
+    f[..., 0:3] = np.log1p(np.maximum(raw[..., 0:3], 0) / m)
     f[..., 3:9] = raw[..., 3:9]
     f[..., 9] = np.log1p(np.maximum(raw[..., 9], 0))
-    return f, f[..., 0:3].copy()
+    return f, f[..., 0:3].copy(), m
 
 
 # ------------------------------------------------------------------ network
 def forward(params, raw):
     L = unpack(params)
-    x0, noisy_log = features(raw.astype(np.float64))
-    cache = {"x0": x0}
+    x0, noisy_log, demod = features(raw.astype(np.float64))
+    cache = {"x0": x0, "demod": demod}
 
     def c(name, x):
         w, b, relu, _, _ = L[name]
@@ -131,14 +137,14 @@ def forward(params, raw):
 
 def infer(params, raw):
     out, cache = forward(params, raw)
-    return np.maximum(np.expm1(cache["noisy_log"] + out), 0)
+    return np.maximum(np.expm1(cache["noisy_log"] + out), 0) * cache["demod"]
 
 
 def loss_and_grads(params, raw, target):
     """L1 loss in log space (mean over pixels and RGB) and its gradient w.r.t. params."""
     L = unpack(params)
     out, cache = forward(params, raw)
-    tgt_log = np.log1p(np.maximum(target.astype(np.float64), 0))
+    tgt_log = np.log1p(np.maximum(target.astype(np.float64), 0) / cache["demod"])
     d = cache["noisy_log"] + out - tgt_log
     count = d.size
     loss = np.abs(d).sum() / count
