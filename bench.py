@@ -38,7 +38,60 @@ TUNED = {
     "soup": {"bvh_width": 4, "blocks_per_cu": 8, "short_stack": 8, "refill": 0},
     "icosphere": {"bvh_width": 4, "blocks_per_cu": 4, "short_stack": 16, "refill": 0},
     "cornell": {"bvh_width": 4, "blocks_per_cu": 8, "short_stack": 8, "refill": 0},
+    "batch": {"bvh_width": 4, "blocks_per_cu": 8, "short_stack": 8, "refill": 0},
 }
+
+
+def batch_views(count, seed=2024):
+    """scenes from the soup / displaced-sphere generators with triangle counts
+    log-uniform in [1e3, 2e6], image sides log-uniform in [256, 1024], 2..8 bounces."""
+    import scenes
+    rng = np.random.default_rng(seed)
+    out = []
+    for v in range(count):
+        n = int(round(10 ** rng.uniform(3, np.log10(2e6))))
+        w, h = (int(round(2 ** rng.uniform(8, 10))) for _ in range(2))
+        bounces = int(rng.integers(2, 9))
+        if v % 2 == 0:
+            tris = scenes.triangle_soup(n, seed=seed + v)
+            cam, st = scenes.soup_camera(w, h), scenes.make_settings(bounces, sky=1.0, seed=seed + v)
+        else:
+            sub = int(np.clip(round(np.log(max(n - 20, 20) / 20) / np.log(4)), 1, 8))
+            tris = scenes.displaced_icosphere(subdiv=sub, seed=seed + v)
+            cam, st = scenes.icosphere_camera(w, h), scenes.make_settings(bounces, seed=seed + v)
+        out.append((tris, w, h, cam, st))
+    return out
+
+
+def bench_batch(args):
+    import native_api
+    views = batch_views(args.batch_views)
+    cfg = TUNED["batch"]
+    b = native_api.Batch(**cfg)
+    for t, w, h, cam, st in views:
+        b.add_view(t, w, h, cam, st)
+    build_ms = b.build()
+    pixels = sum(w * h for _, w, h, _, _ in views)
+    tris = sum(len(t) for t, *_ in views)
+    log(f"batch: {len(views)} views, {tris} triangles, {pixels} pixels; build {build_ms:.0f} ms")
+    b.render(args.warmup)
+    rays, wall = 0, 0.0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        b.render(1)
+        c, sh = b.ray_counts()
+        rays += c + sh
+    wall = time.perf_counter() - t0
+    b.close()
+    line = {"metric": "Mrays/s on 1 MI355X (batched views)", "value": round(rays / wall / 1e6, 2), "unit": "Mrays/s",
+            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(wall * 1e3 / args.steps, 4),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+            "data": "synthetic (seeded scene generators)",
+            "config": {"workload": f"{len(views)} batched views (soup / displaced sphere), one persistent launch per "
+                                   f"stage, hipGraph per frame", "views": len(views), "triangles": tris,
+                       "pixels": pixels, "launch": cfg, "build_ms": round(build_ms, 1)}}
+    print(json.dumps(line), flush=True)
+    return 0
 NODE_BYTES = {2: 64, 4: 128, 8: 256}
 
 
@@ -172,7 +225,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="soup", choices=("soup", "icosphere", "cornell"))
+    ap.add_argument("--workload", default="soup", choices=("soup", "icosphere", "cornell", "batch"))
+    ap.add_argument("--batch-views", type=int, default=64)
     ap.add_argument("--tris", type=int, default=1_000_000)
     ap.add_argument("--bvh-width", type=int)
     ap.add_argument("--blocks-per-cu", type=int)
@@ -199,6 +253,8 @@ def main():
         return 0
     if args.psnr_study:
         return psnr_study(args)
+    if args.workload == "batch":
+        return bench_batch(args)
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (paths are independent per pixel, "
               f"so a multi-GPU run would shard image rows; see the design notes)", file=sys.stderr)

@@ -1,0 +1,391 @@
+// batch.hip - many scenes and cameras rendered together: one launch per stage over a
+// shared path queue, replayed as a captured hipGraph per frame.
+//
+// Every scene keeps its own GPU-built BVH (same build as a session's), rebased into global
+// node / record / triangle arrays; a path slot's view is found from the slot, and the view
+// gives the camera, settings, lights and root node.  Per-view sample counters live in device
+// memory and advance on the device, so the captured frame replays unchanged.
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <string>
+#include <vector>
+#include "bvh_gpu.h"
+#include "kernels.h"
+
+using namespace rt;
+
+namespace {
+
+template <int W>
+__global__ void rebaseNodesKernel(WideNode<W>* nodes, int count, int nodeOffset, int recordOffset) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= count) return;
+    WideNode<W>& n = nodes[i];
+    for (int c = 0; c < W; c++) {
+        const int r = n.child[c];
+        if (r == kEmptyChild) continue;
+        n.child[c] = r >= 0 ? r + nodeOffset : ~(~r + recordOffset);
+    }
+}
+
+__global__ void rebaseRecordsKernel(TriRecord* recs, int count, int triangleOffset) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < count) recs[i].index += triangleOffset;
+}
+
+struct PendingView {
+    std::vector<Triangle> tris;
+    int w, h;
+    CameraData cam;
+    RenderSettings rs;
+};
+
+struct Batch {
+    std::vector<PendingView> pending;
+    std::string error;
+    int width = 4, blocksPerCU = 8, shortStack = 8, refill = 0, useGraph = 1;
+    bool built = false;
+    int cus = 0, views = 0, totalSlots = 0, maxBounces = 0;
+    long long totalTris = 0;
+    std::vector<int> viewBase;        // views + 1
+    std::vector<int> pixelBase;       // offsets of each view's sums / pixels
+    // device
+    Triangle* dTris = nullptr;
+    Light* dLights = nullptr;
+    void* dNodes = nullptr;
+    TriRecord* dRecs = nullptr;
+    BatchView* dViews = nullptr;
+    int *dViewBase = nullptr, *dRoots = nullptr;
+    unsigned* dSamples = nullptr;
+    float4 *dSumRad = nullptr, *dSumAlb = nullptr, *dSumNrm = nullptr;
+    uchar4* dPixels = nullptr;
+    float4 *dThr = nullptr, *dRad = nullptr, *dAlb = nullptr, *dNrm = nullptr, *dNee = nullptr;
+    float4 *dRayA = nullptr, *dRayB = nullptr, *dShadow = nullptr;
+    int4* dHits = nullptr;
+    int *dCounters = nullptr, *dSpill = nullptr, *dOverflow = nullptr;
+    int spillEntries = 0, traceBlocks = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    float buildMs = 0;
+    unsigned long long lastClosest = 0, lastShadow = 0;
+};
+
+template <class T> bool dalloc(Batch* b, T*& p, size_t count) {
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&p), std::max<size_t>(count * sizeof(T), 16));
+    if (e != hipSuccess) { p = nullptr; b->error = std::string("device allocation: ") + hipGetErrorString(e); return false; }
+    return true;
+}
+template <class T> void dfree(T*& p) { if (p) (void)hipFree(p); p = nullptr; }
+
+void release(Batch* b) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr; b->graph = nullptr;
+    dfree(b->dTris); dfree(b->dLights); dfree(b->dNodes); dfree(b->dRecs); dfree(b->dViews); dfree(b->dViewBase);
+    dfree(b->dRoots); dfree(b->dSamples); dfree(b->dSumRad); dfree(b->dSumAlb); dfree(b->dSumNrm); dfree(b->dPixels);
+    dfree(b->dThr); dfree(b->dRad); dfree(b->dAlb); dfree(b->dNrm); dfree(b->dNee); dfree(b->dRayA); dfree(b->dRayB);
+    dfree(b->dShadow); dfree(b->dHits); dfree(b->dCounters); dfree(b->dSpill); dfree(b->dOverflow);
+    if (b->ev0) (void)hipEventDestroy(b->ev0);
+    if (b->ev1) (void)hipEventDestroy(b->ev1);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    b->ev0 = b->ev1 = nullptr; b->stream = nullptr;
+    b->built = false;
+}
+
+#define B_TRY(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) { b->error = std::string(#x) + ": " + hipGetErrorString(e_); return false; } } while (0)
+
+BatchArgs batchArgs(Batch* b) {
+    BatchArgs a{};
+    a.views = b->dViews; a.viewBase = b->dViewBase; a.viewCount = b->views; a.totalSlots = b->totalSlots;
+    a.samples = b->dSamples;
+    a.thr = b->dThr; a.rad = b->dRad; a.alb = b->dAlb; a.nrm = b->dNrm; a.nee = b->dNee;
+    a.shadow = b->dShadow; a.hits = b->dHits; a.counters = b->dCounters;
+    return a;
+}
+
+// The launches of one frame (recorded into the graph, or issued directly).
+bool enqueueFrame(Batch* b) {
+    hipStream_t st = b->stream;
+    BatchArgs a = batchArgs(b);
+    B_TRY(launchResetCounters(b->dCounters, b->totalSlots, st));
+    a.rayOut = b->dRayA;
+    B_TRY(launchBatchRaygen(a, st));
+    for (int bounce = 0; bounce < b->maxBounces; bounce++) {
+        float4* in = (bounce & 1) ? b->dRayB : b->dRayA;
+        float4* out = (bounce & 1) ? b->dRayA : b->dRayB;
+        TraceArgs ta{};
+        ta.nodes = b->dNodes; ta.recs = b->dRecs; ta.spill = b->dSpill; ta.spillEntries = b->spillEntries;
+        ta.totalLanes = b->traceBlocks * kBlock; ta.overflow = b->dOverflow;
+        ta.roots = b->dRoots; ta.viewBase = b->dViewBase; ta.views = b->views;
+        TraceArgs sa = ta;
+        ta.rays = in; ta.countPtr = b->dCounters + kCntRays + bounce; ta.work = b->dCounters + kWorkRays + bounce;
+        ta.hits = b->dHits;
+        B_TRY(launchTrace(b->width, false, false, b->traceBlocks, b->shortStack, b->refill != 0, ta, st));
+        a.bounce = bounce; a.rayIn = in; a.rayOut = out;
+        B_TRY(launchBatchShade(a, b->cus * 8, st));
+        sa.rays = b->dShadow; sa.countPtr = b->dCounters + kCntShadow + bounce; sa.work = b->dCounters + kWorkShadow + bounce;
+        sa.rad = b->dRad; sa.contrib = b->dNee;
+        B_TRY(launchTrace(b->width, true, false, b->traceBlocks, b->shortStack, b->refill != 0, sa, st));
+    }
+    B_TRY(launchBatchAccumulate(a, st));
+    B_TRY(launchBatchAdvance(a, st));
+    return true;
+}
+
+bool build(Batch* b) {
+    if (b->pending.empty()) { b->error = "batch has no views"; return false; }
+    auto t0 = std::chrono::steady_clock::now();
+    int device = 0;
+    B_TRY(hipGetDevice(&device));
+    B_TRY(hipDeviceGetAttribute(&b->cus, hipDeviceAttributeMultiprocessorCount, device));
+    B_TRY(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    B_TRY(hipEventCreate(&b->ev0));
+    B_TRY(hipEventCreate(&b->ev1));
+    b->views = (int)b->pending.size();
+    // Host preprocessing per scene, as RtInitialize (Native/renderer.cu:345-353).
+    std::vector<Triangle> tris;
+    std::vector<Light> lights;
+    std::vector<int> triOff, lightOff, lightCount;
+    std::vector<float> lightArea;
+    for (const PendingView& pv : b->pending) {
+        triOff.push_back((int)tris.size());
+        lightOff.push_back((int)lights.size());
+        float area = 0;
+        for (size_t i = 0; i < pv.tris.size(); i++) {
+            Triangle t = pv.tris[i];
+            t.color = RtVec3{srgbToLinear(t.color.x), srgbToLinear(t.color.y), srgbToLinear(t.color.z)};
+            t.specularColor = RtVec3{srgbToLinear(t.specularColor.x), srgbToLinear(t.specularColor.y), srgbToLinear(t.specularColor.z)};
+            if (t.emission > 0) {
+                f3 an = crossv(V(t.edge1), V(t.edge2));
+                area += .5f * sqrtf(dotv(an, an));
+                lights.push_back(Light{(int)tris.size(), area});
+            }
+            tris.push_back(t);
+        }
+        lightCount.push_back((int)lights.size() - lightOff.back());
+        lightArea.push_back(area);
+    }
+    b->totalTris = (long long)tris.size();
+    if (!dalloc(b, b->dTris, tris.size()) || !dalloc(b, b->dLights, lights.size())) return false;
+    B_TRY(hipMemcpy(b->dTris, tris.data(), tris.size() * sizeof(Triangle), hipMemcpyHostToDevice));
+    if (!lights.empty()) B_TRY(hipMemcpy(b->dLights, lights.data(), lights.size() * sizeof(Light), hipMemcpyHostToDevice));
+    // Per-scene GPU builds, then one global node / record array.
+    std::vector<GpuBuildResult> res(b->views);
+    GpuBuildParams prm;
+    prm.width = b->width;
+    long long nodeTotal = 0;
+    for (int v = 0; v < b->views; v++) {
+        B_TRY(buildBvhGpu(b->dTris + triOff[v], (int)b->pending[v].tris.size(), prm, res[v], b->stream));
+        nodeTotal += res[v].wideCount;
+    }
+    const size_t nodeBytes = (size_t)32 * b->width;
+    unsigned char* nodes = nullptr;
+    if (!dalloc(b, nodes, (size_t)nodeTotal * nodeBytes) || !dalloc(b, b->dRecs, tris.size())) return false;
+    b->dNodes = nodes;
+    std::vector<int> roots(b->views);
+    long long nodeOff = 0;
+    for (int v = 0; v < b->views; v++) {
+        const int nv = res[v].wideCount, nt = (int)b->pending[v].tris.size();
+        if (b->width == 2) hipLaunchKernelGGL(rebaseNodesKernel<2>, dim3((nv + kBlock - 1) / kBlock), dim3(kBlock), 0, b->stream, (WideNode<2>*)res[v].wideNodes, nv, (int)nodeOff, triOff[v]);
+        else if (b->width == 8) hipLaunchKernelGGL(rebaseNodesKernel<8>, dim3((nv + kBlock - 1) / kBlock), dim3(kBlock), 0, b->stream, (WideNode<8>*)res[v].wideNodes, nv, (int)nodeOff, triOff[v]);
+        else hipLaunchKernelGGL(rebaseNodesKernel<4>, dim3((nv + kBlock - 1) / kBlock), dim3(kBlock), 0, b->stream, (WideNode<4>*)res[v].wideNodes, nv, (int)nodeOff, triOff[v]);
+        hipLaunchKernelGGL(rebaseRecordsKernel, dim3((nt + kBlock - 1) / kBlock), dim3(kBlock), 0, b->stream, res[v].recs, nt, triOff[v]);
+        B_TRY(hipGetLastError());
+        B_TRY(hipMemcpyAsync(nodes + nodeOff * nodeBytes, res[v].wideNodes, (size_t)nv * nodeBytes, hipMemcpyDeviceToDevice, b->stream));
+        B_TRY(hipMemcpyAsync(b->dRecs + triOff[v], res[v].recs, (size_t)nt * sizeof(TriRecord), hipMemcpyDeviceToDevice, b->stream));
+        roots[v] = (int)nodeOff;
+        nodeOff += nv;
+    }
+    B_TRY(hipStreamSynchronize(b->stream));
+    for (GpuBuildResult& r : res) {
+        dfree(r.binNodes); dfree(r.order); dfree(r.recs);
+        if (r.wideNodes) (void)hipFree(r.wideNodes);
+        r.wideNodes = nullptr;
+    }
+    // Views, slots and accumulation buffers.
+    b->viewBase.assign(1, 0);
+    long long slots = 0;
+    for (const PendingView& pv : b->pending) {
+        slots += (long long)pv.w * pv.h;
+        b->viewBase.push_back((int)slots);
+        b->maxBounces = std::max(b->maxBounces, pv.rs.maxBounces);
+    }
+    if (slots > (1LL << 28)) { b->error = "batch exceeds 2^28 pixels"; return false; }
+    b->totalSlots = (int)slots;
+    const size_t n = (size_t)slots;
+    if (!dalloc(b, b->dSumRad, n) || !dalloc(b, b->dSumAlb, n) || !dalloc(b, b->dSumNrm, n) || !dalloc(b, b->dPixels, n) ||
+        !dalloc(b, b->dThr, n) || !dalloc(b, b->dRad, n) || !dalloc(b, b->dAlb, n) || !dalloc(b, b->dNrm, n) ||
+        !dalloc(b, b->dNee, n) || !dalloc(b, b->dRayA, 2 * n) || !dalloc(b, b->dRayB, 2 * n) ||
+        !dalloc(b, b->dShadow, 2 * n) || !dalloc(b, b->dHits, n) || !dalloc(b, b->dCounters, (size_t)kCounterStride) ||
+        !dalloc(b, b->dOverflow, 1) || !dalloc(b, b->dViews, b->views) || !dalloc(b, b->dViewBase, b->views + 1) ||
+        !dalloc(b, b->dRoots, b->views) || !dalloc(b, b->dSamples, b->views))
+        return false;
+    std::vector<BatchView> hv(b->views);
+    for (int v = 0; v < b->views; v++) {
+        const PendingView& pv = b->pending[v];
+        BatchView& V = hv[v];
+        V.cam = pv.cam; V.rs = pv.rs; V.rs.raysPerPixel = 1;
+        V.sc.tris = b->dTris; V.sc.lights = b->dLights + lightOff[v]; V.sc.lightCount = lightCount[v];
+        V.sc.lightArea = lightArea[v];
+        V.w = pv.w; V.h = pv.h; V.slotBase = b->viewBase[v];
+        V.sumRad = b->dSumRad + b->viewBase[v]; V.sumAlb = b->dSumAlb + b->viewBase[v];
+        V.sumNrm = b->dSumNrm + b->viewBase[v]; V.pixels = b->dPixels + b->viewBase[v];
+    }
+    B_TRY(hipMemcpy(b->dViews, hv.data(), hv.size() * sizeof(BatchView), hipMemcpyHostToDevice));
+    B_TRY(hipMemcpy(b->dViewBase, b->viewBase.data(), b->viewBase.size() * sizeof(int), hipMemcpyHostToDevice));
+    B_TRY(hipMemcpy(b->dRoots, roots.data(), roots.size() * sizeof(int), hipMemcpyHostToDevice));
+    B_TRY(hipMemset(b->dSamples, 0, b->views * sizeof(unsigned)));
+    B_TRY(hipMemset(b->dOverflow, 0, sizeof(int)));
+    b->traceBlocks = b->cus * b->blocksPerCU;
+    b->spillEntries = 36 * (b->width - 1);
+    if (!dalloc(b, b->dSpill, (size_t)b->spillEntries * b->traceBlocks * kBlock * 2)) return false;
+    if (b->useGraph) {
+        B_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        const bool ok = enqueueFrame(b);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(b->stream, &g);
+        if (!ok) { if (g) (void)hipGraphDestroy(g); return false; }
+        B_TRY(e);
+        b->graph = g;
+        B_TRY(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    }
+    b->buildMs = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    b->built = true;
+    return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+void* RtBatchCreate(void) { return new Batch(); }
+
+void RtBatchDestroy(void* p) {
+    Batch* b = static_cast<Batch*>(p);
+    if (!b) return;
+    release(b);
+    delete b;
+}
+
+const char* RtBatchLastError(void* p) {
+    Batch* b = static_cast<Batch*>(p);
+    return b ? b->error.c_str() : "invalid argument";
+}
+
+int RtBatchAddView(void* p, const Triangle* triangles, int count, int width, int height, const CameraData* camera,
+                   const RenderSettings* settings) {
+    Batch* b = static_cast<Batch*>(p);
+    if (!b) return 0;
+    if (b->built) { b->error = "batch already built"; return 0; }
+    if (!triangles || count <= 0 || width <= 0 || height <= 0 || width > 4096 || height > 4096 || !camera || !settings ||
+        settings->maxBounces < 1 || settings->maxBounces > kMaxBounces || !(settings->exposure > 0) ||
+        !(settings->aperture >= 0) || !(settings->focusDistance > 0) || !(settings->skyStrength >= 0)) {
+        b->error = "invalid view";
+        return 0;
+    }
+    PendingView pv;
+    pv.tris.assign(triangles, triangles + count);
+    pv.w = width; pv.h = height; pv.cam = *camera; pv.rs = *settings;
+    b->pending.push_back(std::move(pv));
+    return (int)b->pending.size();
+}
+
+int RtBatchSetOption(void* p, const char* key, int value) {
+    Batch* b = static_cast<Batch*>(p);
+    if (!b || !key || b->built) return 0;
+    if (!std::strcmp(key, "bvh_width") && (value == 2 || value == 4 || value == 8)) { b->width = value; return 1; }
+    if (!std::strcmp(key, "blocks_per_cu") && value >= 1 && value <= 32) { b->blocksPerCU = value; return 1; }
+    if (!std::strcmp(key, "short_stack") && (value == 8 || value == 16)) { b->shortStack = value; return 1; }
+    if (!std::strcmp(key, "refill") && (value == 0 || value == 1)) { b->refill = value; return 1; }
+    if (!std::strcmp(key, "use_graph") && (value == 0 || value == 1)) { b->useGraph = value; return 1; }
+    return 0;
+}
+
+int RtBatchBuild(void* p, float* buildMs) {
+    Batch* b = static_cast<Batch*>(p);
+    if (!b) return 0;
+    if (b->built) return 1;
+    if (!build(b)) { release(b); return 0; }
+    if (buildMs) *buildMs = b->buildMs;
+    return 1;
+}
+
+int RtBatchRender(void* p, int frames, float* deviceMs) {
+    Batch* b = static_cast<Batch*>(p);
+    if (!b || !b->built || frames < 1 || !deviceMs) return 0;
+    B_TRY(hipEventRecord(b->ev0, b->stream));
+    for (int f = 0; f < frames; f++) {
+        if (b->exec) B_TRY(hipGraphLaunch(b->exec, b->stream));
+        else if (!enqueueFrame(b)) return 0;
+    }
+    B_TRY(hipEventRecord(b->ev1, b->stream));
+    B_TRY(hipEventSynchronize(b->ev1));
+    B_TRY(hipEventElapsedTime(deviceMs, b->ev0, b->ev1));
+    int ctr[kCounterStride];
+    int overflow = 0;
+    B_TRY(hipMemcpy(ctr, b->dCounters, sizeof(ctr), hipMemcpyDeviceToHost));
+    B_TRY(hipMemcpy(&overflow, b->dOverflow, sizeof(int), hipMemcpyDeviceToHost));
+    if (overflow) { b->error = "traversal stack exceeded its spill ring"; return 0; }
+    b->lastClosest = b->lastShadow = 0;
+    for (int k = 0; k < b->maxBounces; k++) {
+        b->lastClosest += (unsigned)ctr[kCntRays + k];
+        b->lastShadow += (unsigned)ctr[kCntShadow + k];
+    }
+    return 1;
+}
+
+int RtBatchReset(void* p) {
+    Batch* b = static_cast<Batch*>(p);
+    if (!b || !b->built) return 0;
+    B_TRY(hipMemsetAsync(b->dSamples, 0, b->views * sizeof(unsigned), b->stream));
+    B_TRY(hipStreamSynchronize(b->stream));
+    return 1;
+}
+
+int RtBatchRayCounts(void* p, unsigned long long* closest, unsigned long long* shadow) {
+    Batch* b = static_cast<Batch*>(p);
+    if (!b || !closest || !shadow) return 0;
+    *closest = b->lastClosest; *shadow = b->lastShadow;
+    return 1;
+}
+
+int RtBatchViewInfo(void* p, int view, int* width, int* height, unsigned* samples) {
+    Batch* b = static_cast<Batch*>(p);
+    if (!b || view < 0 || view >= (int)b->pending.size()) return 0;
+    if (width) *width = b->pending[view].w;
+    if (height) *height = b->pending[view].h;
+    if (samples) {
+        *samples = 0;
+        if (b->built) B_TRY(hipMemcpy(samples, b->dSamples + view, sizeof(unsigned), hipMemcpyDeviceToHost));
+    }
+    return 1;
+}
+
+int RtBatchReadView(void* p, int view, unsigned char* rgba, int bytes, float* accumulation, long long floatCount) {
+    Batch* b = static_cast<Batch*>(p);
+    if (!b || !b->built || view < 0 || view >= b->views) return 0;
+    const size_t npix = (size_t)b->pending[view].w * b->pending[view].h;
+    const int base = b->viewBase[view];
+    if (rgba) {
+        if (bytes < 0 || (size_t)bytes < npix * 4) { b->error = "output buffer too small"; return 0; }
+        B_TRY(hipMemcpy(rgba, b->dPixels + base, npix * 4, hipMemcpyDeviceToHost));
+    }
+    if (accumulation) {
+        if (floatCount < (long long)(npix * 10)) { b->error = "accumulation buffer too small"; return 0; }
+        std::vector<float4> r(npix), a(npix), n(npix);
+        B_TRY(hipMemcpy(r.data(), b->dSumRad + base, npix * 16, hipMemcpyDeviceToHost));
+        B_TRY(hipMemcpy(a.data(), b->dSumAlb + base, npix * 16, hipMemcpyDeviceToHost));
+        B_TRY(hipMemcpy(n.data(), b->dSumNrm + base, npix * 16, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < npix; i++) {
+            float* o = accumulation + i * 10;
+            o[0] = r[i].x; o[1] = r[i].y; o[2] = r[i].z; o[3] = a[i].x; o[4] = a[i].y; o[5] = a[i].z;
+            o[6] = n[i].x; o[7] = n[i].y; o[8] = n[i].z; o[9] = r[i].w;
+        }
+    }
+    return 1;
+}
+
+}  // extern "C"

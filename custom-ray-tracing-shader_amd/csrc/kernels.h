@@ -26,6 +26,9 @@ struct TraceArgs {
     int totalLanes;          // launched lanes (spill stride)
     unsigned long long* stats;   // optional: [0] node visits, [1] triangle tests
     int* overflow;           // set to 1 if a lane's stack exceeded the spill ring
+    const int* roots;        // batched views: root node per view (null: root 0)
+    const int* viewBase;     // batched views: first path slot per view, views + 1 entries
+    int views;
 };
 
 struct FrameArgs {
@@ -75,6 +78,31 @@ hipError_t launchShade(const FrameArgs& f, int blocks, hipStream_t s);
 hipError_t launchAccumulate(const AccumArgs& a, hipStream_t s);
 hipError_t launchSobolBatch(const unsigned* pixels, const unsigned* samples, int count, unsigned seed, int dims,
                             float* out, hipStream_t s);
+// ---- batched views: many scenes and cameras in one launch per stage ------------------
+struct BatchView {
+    CameraData cam;
+    RenderSettings rs;         // raysPerPixel is 1: one sample per view per frame
+    SceneView sc;              // triangles of the whole batch, this view's lights
+    int w, h, slotBase;
+    float4 *sumRad, *sumAlb, *sumNrm;
+    uchar4* pixels;
+};
+struct BatchArgs {
+    const BatchView* views;
+    const int* viewBase;       // views + 1 entries
+    int viewCount, totalSlots;
+    unsigned* samples;         // per view samples so far (advanced on the device each frame)
+    float4 *thr, *rad, *alb, *nrm, *nee;
+    float4 *rayIn, *rayOut, *shadow;
+    int4* hits;
+    int* counters;
+    int bounce;
+};
+hipError_t launchBatchRaygen(const BatchArgs& b, hipStream_t s);
+hipError_t launchBatchShade(const BatchArgs& b, int blocks, hipStream_t s);
+hipError_t launchBatchAccumulate(const BatchArgs& b, hipStream_t s);
+hipError_t launchBatchAdvance(const BatchArgs& b, hipStream_t s);
+
 hipError_t launchCopy(const float4* in, float4* out, long long n, int blocks, hipStream_t s);
 const unsigned* hostSobolDirections();
 

@@ -130,15 +130,26 @@ __device__ __forceinline__ void finishRay(const TraceArgs& a, const LaneRay& L) 
     }
 }
 
+// View owning a path slot: last v with viewBase[v] <= slot.
+__device__ __forceinline__ int findView(const int* viewBase, int views, int slot) {
+    int lo = 0, hi = views - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (viewBase[mid] <= slot) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
 __device__ __forceinline__ void startRay(const TraceArgs& a, int i, LaneRay& L) {
     const float4 r0 = a.rays[2 * i], r1 = a.rays[2 * i + 1];
     L.o = F3(r0.x, r0.y, r0.z);
     L.d = F3(r1.x, r1.y, r1.z);
     L.inv = safeInverse(L.d);
     L.best = r0.w; L.bu = 0; L.bv = 0;
-    L.bestIdx = -1; L.sp = 0; L.ref = 0;
+    L.bestIdx = -1; L.sp = 0;
     L.index = i;
     L.slot = __float_as_int(r1.w);
+    L.ref = a.roots ? a.roots[findView(a.viewBase, a.views, L.slot)] : 0;
 }
 
 // Persistent traversal.  Each wave takes rays from the shared cursor; with REFILL, lanes
@@ -369,6 +380,121 @@ __global__ __launch_bounds__(kBlock) void accumulateKernel(AccumArgs a) {
 }
 hipError_t launchAccumulate(const AccumArgs& a, hipStream_t s) {
     hipLaunchKernelGGL(accumulateKernel, dim3((a.npix + kBlock - 1) / kBlock), dim3(kBlock), 0, s, a);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ batched views
+// The same stages as the single-view frame, with each path slot's view looked up from its
+// slot; Sobol values are formed per lane (views have their own seeds and sample counts).
+__global__ __launch_bounds__(kBlock) void batchRaygenKernel(BatchArgs b) {
+    const int slot = blockIdx.x * kBlock + threadIdx.x;
+    if (slot >= b.totalSlots) return;
+    const int v = findView(b.viewBase, b.viewCount, slot);
+    const BatchView& V = b.views[v];
+    const unsigned pixel = (unsigned)(slot - V.slotBase);
+    const int x = (int)(pixel % (unsigned)V.w), y = (int)(pixel / (unsigned)V.w);
+    const unsigned sample = b.samples[v];
+    auto rng = [&](int k) { return sobolSample(gSobolDirs, pixel, sample, V.rs.seed, (unsigned)k); };
+    f3 o, d;
+    cameraRay(V.cam, V.rs, x, y, V.w, V.h, rng, o, d);
+    b.thr[slot] = make_float4(1, 1, 1, 0);
+    b.rad[slot] = make_float4(0, 0, 0, 0);
+    b.alb[slot] = make_float4(0, 0, 0, 0);
+    b.nrm[slot] = make_float4(0, 0, 0, 0);
+    b.rayOut[2 * slot] = make_float4(o.x, o.y, o.z, 1e30f);
+    b.rayOut[2 * slot + 1] = make_float4(d.x, d.y, d.z, __int_as_float(slot));
+}
+hipError_t launchBatchRaygen(const BatchArgs& b, hipStream_t s) {
+    hipLaunchKernelGGL(batchRaygenKernel, dim3((b.totalSlots + kBlock - 1) / kBlock), dim3(kBlock), 0, s, b);
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(kBlock) void batchShadeKernel(BatchArgs b) {
+    const int count = b.counters[kCntRays + b.bounce];
+    for (int base = blockIdx.x * kBlock; base < count; base += gridDim.x * kBlock) {
+        const int i = base + threadIdx.x;
+        bool wantShadow = false, wantNext = false;
+        float4 sa = make_float4(0, 0, 0, 0), sb = sa, na = sa, nb = sa;
+        if (i < count) {
+            const float4 r0 = b.rayIn[2 * i], r1 = b.rayIn[2 * i + 1];
+            const int slot = __float_as_int(r1.w);
+            const int v = findView(b.viewBase, b.viewCount, slot);
+            const BatchView& V = b.views[v];
+            const unsigned pixel = (unsigned)(slot - V.slotBase);
+            const unsigned sample = b.samples[v];
+            const unsigned dimBase = 4 + 8 * (unsigned)b.bounce;
+            const int4 hv = b.hits[i];
+            RtHit hit;
+            hit.index = hv.x; hit.distance = __int_as_float(hv.y); hit.u = __int_as_float(hv.z); hit.v = __int_as_float(hv.w);
+            const float4 t4 = b.thr[slot], q4 = b.rad[slot];
+            PathState ps;
+            ps.thr = F3(t4.x, t4.y, t4.z); ps.prevPdf = t4.w; ps.rad = F3(q4.x, q4.y, q4.z);
+            FirstHit fh; fh.covered = false;
+            auto rng = [&](int k) { return sobolSample(gSobolDirs, pixel, sample, V.rs.seed, dimBase + (unsigned)k); };
+            ShadeResult res;
+            shadeBounce(V.sc, V.rs, b.bounce, rng, F3(r0.x, r0.y, r0.z), F3(r1.x, r1.y, r1.z), hit, ps, &fh, res);
+            b.thr[slot] = make_float4(ps.thr.x, ps.thr.y, ps.thr.z, ps.prevPdf);
+            b.rad[slot] = make_float4(ps.rad.x, ps.rad.y, ps.rad.z, q4.w);
+            if (fh.covered) {
+                b.alb[slot] = make_float4(fh.albedo.x, fh.albedo.y, fh.albedo.z, 1.0f);
+                b.nrm[slot] = make_float4(fh.normal.x, fh.normal.y, fh.normal.z, fh.depth);
+            }
+            if (res.shadow) {
+                wantShadow = true;
+                b.nee[slot] = make_float4(res.contrib.x, res.contrib.y, res.contrib.z, 0);
+                sa = make_float4(res.nextO.x, res.nextO.y, res.nextO.z, res.slimit);
+                sb = make_float4(res.sdir.x, res.sdir.y, res.sdir.z, __int_as_float(slot));
+            }
+            if (res.cont) {
+                wantNext = true;
+                na = make_float4(res.nextO.x, res.nextO.y, res.nextO.z, 1e30f);
+                nb = make_float4(res.nextD.x, res.nextD.y, res.nextD.z, __int_as_float(slot));
+            }
+        }
+        appendRay(wantShadow, &b.counters[kCntShadow + b.bounce], b.shadow, sa, sb);
+        appendRay(wantNext, &b.counters[kCntRays + b.bounce + 1], b.rayOut, na, nb);
+    }
+}
+hipError_t launchBatchShade(const BatchArgs& b, int blocks, hipStream_t s) {
+    const int need = (b.totalSlots + kBlock - 1) / kBlock;
+    hipLaunchKernelGGL(batchShadeKernel, dim3(need < blocks ? need : blocks), dim3(kBlock), 0, s, b);
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(kBlock) void batchAccumulateKernel(BatchArgs b) {
+    const int slot = blockIdx.x * kBlock + threadIdx.x;
+    if (slot >= b.totalSlots) return;
+    const int v = findView(b.viewBase, b.viewCount, slot);
+    const BatchView& V = b.views[v];
+    const int i = slot - V.slotBase;
+    const unsigned sample = b.samples[v];
+    float4 sr = make_float4(0, 0, 0, 0), sa = sr, sn = sr;
+    if (sample) { sr = V.sumRad[i]; sa = V.sumAlb[i]; sn = V.sumNrm[i]; }
+    const float4 pr = b.rad[slot], pa = b.alb[slot], pn = b.nrm[slot];
+    sr.x = sr.x + pr.x; sr.y = sr.y + pr.y; sr.z = sr.z + pr.z;
+    sa.x = sa.x + pa.x; sa.y = sa.y + pa.y; sa.z = sa.z + pa.z;
+    sn.x = sn.x + pn.x; sn.y = sn.y + pn.y; sn.z = sn.z + pn.z;
+    sr.w = sr.w + pa.w;
+    sn.w = sn.w + pn.w;
+    const float lum = 0.2126f * pr.x + 0.7152f * pr.y + 0.0722f * pr.z;
+    sa.w = sa.w + lum * lum;
+    V.sumRad[i] = sr; V.sumAlb[i] = sa; V.sumNrm[i] = sn;
+    const float inv = 1.0f / (float)(sample + 1u);
+    const float e = V.rs.exposure;
+    V.pixels[i] = make_uchar4(displayByte(sr.x * inv, e), displayByte(sr.y * inv, e), displayByte(sr.z * inv, e), 255);
+}
+hipError_t launchBatchAccumulate(const BatchArgs& b, hipStream_t s) {
+    hipLaunchKernelGGL(batchAccumulateKernel, dim3((b.totalSlots + kBlock - 1) / kBlock), dim3(kBlock), 0, s, b);
+    return hipGetLastError();
+}
+
+__global__ void batchAdvanceKernel(unsigned* samples, int views) {
+    const int v = blockIdx.x * kBlock + threadIdx.x;
+    if (v < views) samples[v] += 1u;
+}
+hipError_t launchBatchAdvance(const BatchArgs& b, hipStream_t s) {
+    hipLaunchKernelGGL(batchAdvanceKernel, dim3((b.viewCount + kBlock - 1) / kBlock), dim3(kBlock), 0, s, b.samples,
+                       b.viewCount);
     return hipGetLastError();
 }
 

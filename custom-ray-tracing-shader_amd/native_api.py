@@ -131,7 +131,79 @@ def load_library(path=None):
     lib.RtDenoiserForwardFlops.restype = c.c_double
     lib.RtConv3x3Forward.argtypes = [fp, c.c_int, c.c_int, c.c_int, c.c_int, fp, fp, c.c_int, c.c_int, fp]
     lib.RtConv3x3Backward.argtypes = [fp, c.c_int, c.c_int, c.c_int, c.c_int, fp, fp, c.c_int, fp, fp, fp]
+    lib.RtBatchCreate.restype = vp
+    lib.RtBatchCreate.argtypes = []
+    lib.RtBatchDestroy.argtypes = [vp]
+    lib.RtBatchDestroy.restype = None
+    lib.RtBatchLastError.argtypes = [vp]
+    lib.RtBatchLastError.restype = c.c_char_p
+    lib.RtBatchAddView.argtypes = [vp, c.POINTER(Triangle), c.c_int, c.c_int, c.c_int, c.POINTER(Camera),
+                                   c.POINTER(Settings)]
+    lib.RtBatchSetOption.argtypes = [vp, c.c_char_p, c.c_int]
+    lib.RtBatchBuild.argtypes = [vp, fp]
+    lib.RtBatchRender.argtypes = [vp, c.c_int, fp]
+    lib.RtBatchReset.argtypes = [vp]
+    lib.RtBatchRayCounts.argtypes = [vp, c.POINTER(c.c_ulonglong), c.POINTER(c.c_ulonglong)]
+    lib.RtBatchViewInfo.argtypes = [vp, c.c_int, c.POINTER(c.c_int), c.POINTER(c.c_int), c.POINTER(c.c_uint)]
+    lib.RtBatchReadView.argtypes = [vp, c.c_int, c.POINTER(c.c_ubyte), c.c_int, fp, c.c_longlong]
     return lib
+
+
+class Batch:
+    """Batched views rendered together (RtBatch* exports)."""
+
+    def __init__(self, lib=None, **options):
+        self.lib = lib or load_library()
+        self.handle = self.lib.RtBatchCreate()
+        self.sizes = []
+        for k, v in options.items():
+            if not self.lib.RtBatchSetOption(self.handle, k.encode(), int(v)):
+                raise ValueError(f"batch option {k}={v} rejected")
+
+    def close(self):
+        if self.handle:
+            self.lib.RtBatchDestroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def check(self, ok):
+        if not ok:
+            raise RuntimeError(self.lib.RtBatchLastError(self.handle).decode())
+
+    def add_view(self, triangles, width, height, camera, settings):
+        self.check(self.lib.RtBatchAddView(self.handle, triangle_pointer(triangles), len(triangles), width, height,
+                                           c.byref(camera), c.byref(settings)))
+        self.sizes.append((width, height))
+
+    def build(self):
+        ms = c.c_float()
+        self.check(self.lib.RtBatchBuild(self.handle, c.byref(ms)))
+        return ms.value
+
+    def render(self, frames=1):
+        ms = c.c_float()
+        self.check(self.lib.RtBatchRender(self.handle, frames, c.byref(ms)))
+        return ms.value
+
+    def reset(self):
+        self.check(self.lib.RtBatchReset(self.handle))
+
+    def ray_counts(self):
+        a, b = c.c_ulonglong(), c.c_ulonglong()
+        self.check(self.lib.RtBatchRayCounts(self.handle, c.byref(a), c.byref(b)))
+        return a.value, b.value
+
+    def read(self, view):
+        w, h = self.sizes[view]
+        rgba = (c.c_ubyte * (w * h * 4))()
+        acc = np.zeros(w * h * 10, np.float32)
+        self.check(self.lib.RtBatchReadView(self.handle, view, rgba, len(rgba), as_pointer(acc, c.c_float), acc.size))
+        return (np.frombuffer(bytes(rgba), dtype=np.uint8).reshape(h, w, 4), acc.reshape(h, w, 10))
 
 
 def _f32(a):

@@ -184,6 +184,25 @@ int RtConv3x3Forward(const float* in, int n, int h, int w, int cin, const float*
 int RtConv3x3Backward(const float* in, int n, int h, int w, int cin, const float* weight, const float* gradOut,
                       int cout, float* gradIn, float* gradWeight, float* gradBias);
 
+/* Batched views: many scenes and cameras rendered together, one launch per stage over a
+ * shared path queue, each frame replayed as a captured hipGraph ("use_graph", default 1).
+ * Each view takes one progressive sample per frame (raysPerPixel is ignored) and equals a
+ * session rendering its scene with raysPerPixel 1.  Options ("bvh_width", "blocks_per_cu",
+ * "short_stack", "refill", "use_graph") must be set before RtBatchBuild.  RtBatchReadView
+ * copies a view's RGBA8 image and/or its 10-float-per-pixel accumulation. */
+void* RtBatchCreate(void);
+void RtBatchDestroy(void* batch);
+const char* RtBatchLastError(void* batch);
+int RtBatchAddView(void* batch, const Triangle* triangles, int count, int width, int height,
+                   const CameraData* camera, const RenderSettings* settings);
+int RtBatchSetOption(void* batch, const char* key, int value);
+int RtBatchBuild(void* batch, float* buildMs);
+int RtBatchRender(void* batch, int frames, float* deviceMs);
+int RtBatchReset(void* batch);
+int RtBatchRayCounts(void* batch, unsigned long long* closest, unsigned long long* shadow);
+int RtBatchViewInfo(void* batch, int view, int* width, int* height, unsigned* samples);
+int RtBatchReadView(void* batch, int view, unsigned char* rgba, int bytes, float* accumulation, long long floatCount);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,70 @@
+"""Batched views: every view of a batch equals a session rendering that scene alone with
+one sample per call (same seed and sample indices), bit for bit, with and without the
+captured graph; sample counters advance and reset."""
+import numpy as np
+import pytest
+
+import scenes
+from native_api import Batch, Session
+
+pytestmark = pytest.mark.gpu
+
+
+def views():
+    out = []
+    t = scenes.cornell()
+    out.append((t, 64, 48, scenes.cornell_camera(64, 48), scenes.make_settings(4, seed=7)))
+    t = scenes.triangle_soup(20000, seed=5)
+    out.append((t, 80, 60, scenes.soup_camera(80, 60), scenes.make_settings(3, sky=1.0, seed=42)))
+    t = scenes.displaced_icosphere(subdiv=4, seed=3)
+    out.append((t, 96, 64, scenes.icosphere_camera(96, 64), scenes.make_settings(8, seed=1234)))
+    t = scenes.cornell()
+    out.append((t, 33, 17, scenes.cornell_camera(33, 17), scenes.make_settings(2, seed=9, aperture=0.05, focus=3.0)))
+    return out
+
+
+@pytest.mark.parametrize("use_graph", [1, 0])
+@pytest.mark.parametrize("width", [4, 8])
+def test_batch_views_equal_single_sessions(lib, use_graph, width):
+    vs = views()
+    frames = 3
+    with Batch(lib, use_graph=use_graph, bvh_width=width) as b:
+        for t, w, h, cam, st in vs:
+            b.add_view(t, w, h, cam, st)
+        b.build()
+        b.render(frames)
+        closest, shadow = b.ray_counts()
+        got = [b.read(v) for v in range(len(vs))]
+    expect_closest = expect_shadow = 0
+    for v, (t, w, h, cam, st) in enumerate(vs):
+        with Session(lib) as s:
+            s.set_option("bvh_width", width)
+            s.initialize(t, w, h)
+            st.raysPerPixel = 1
+            for _ in range(frames):
+                img, _ = s.render(cam, st)
+            acc = s.accumulation()
+            c, sh = s.ray_counts()
+            expect_closest += c
+            expect_shadow += sh
+        assert np.array_equal(got[v][0], img), v
+        assert np.array_equal(got[v][1], acc), v
+    assert (closest, shadow) == (expect_closest, expect_shadow)
+
+
+def test_batch_reset_and_counts(lib):
+    vs = views()[:2]
+    with Batch(lib) as b:
+        for t, w, h, cam, st in vs:
+            b.add_view(t, w, h, cam, st)
+        b.build()
+        b.render(1)
+        first = b.read(0)[0]
+        b.render(2)
+        import ctypes as c
+        n = c.c_uint()
+        assert lib.RtBatchViewInfo(b.handle, 1, None, None, c.byref(n)) and n.value == 3
+        b.reset()
+        b.render(1)
+        assert np.array_equal(b.read(0)[0], first)
+        assert not lib.RtBatchAddView(b.handle, None, 0, 1, 1, None, None)
