@@ -34,10 +34,12 @@ for p in (str(ROOT), str(ROOT / "custom-ray-tracing-shader_amd")):
 HBM_PEAK_GBS = 8000.0
 # Launch configurations chosen by autotune.py on an MI355X for each workload (bench.py
 # --autotune repeats the search); the BVH width also fixes the node size below.
+# The soup keeps the reference builder's leaf <= 4 rule (its configuration names it); the
+# icosphere uses the autotuner's leaf size.
 TUNED = {
-    "soup": {"bvh_width": 4, "blocks_per_cu": 8, "short_stack": 8, "refill": 0},
-    "icosphere": {"bvh_width": 4, "blocks_per_cu": 4, "short_stack": 16, "refill": 0},
-    "cornell": {"bvh_width": 4, "blocks_per_cu": 8, "short_stack": 8, "refill": 0},
+    "soup": {"bvh_width": 4, "build_max_leaf": 4, "blocks_per_cu": 8, "short_stack": 8, "refill": 0},
+    "icosphere": {"bvh_width": 4, "build_max_leaf": 2, "blocks_per_cu": 4, "short_stack": 8, "refill": 0},
+    "cornell": {"bvh_width": 4, "build_max_leaf": 4, "blocks_per_cu": 8, "short_stack": 8, "refill": 0},
     "batch": {"bvh_width": 4, "blocks_per_cu": 8, "short_stack": 8, "refill": 0},
 }
 
@@ -157,7 +159,7 @@ def psnr_study(args):
     lib = native_api.load_library()
     s = native_api.Session(lib)
     for k, v in TUNED["icosphere"].items():
-        s.set_option(k, v)
+        s.set_option(k, v)   # before initialize: width and leaf size shape the build
     s.initialize(tris, w, h)
     log(f"psnr study: {desc}; reference {args.reference_spp} spp")
     t0 = time.perf_counter()
@@ -232,6 +234,7 @@ def main():
     ap.add_argument("--blocks-per-cu", type=int)
     ap.add_argument("--short-stack", type=int)
     ap.add_argument("--refill", type=int)
+    ap.add_argument("--build-max-leaf", type=int)
     ap.add_argument("--autotune", action="store_true",
                     help="search the launch configuration now instead of using the recorded tuned one")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
@@ -270,13 +273,14 @@ def main():
     if args.autotune:
         import autotune
         t0 = time.perf_counter()
-        best, trials = autotune.autotune(tris, cam, st, w, h, use_cache=False)
+        best, trials = autotune.autotune(tris, cam, st, w, h, use_cache=False,
+                                         max_leaf=(4,) if args.workload == "soup" else (2, 3, 4))
         cfg = {k: best[k] for k in cfg}
         log(f"autotune: {len(trials)} configurations in {time.perf_counter() - t0:.1f} s, best {best}")
     s = native_api.Session()
-    s.set_option("bvh_width", cfg["bvh_width"])
-    for k in ("blocks_per_cu", "short_stack", "refill"):
-        s.set_option(k, cfg[k])
+    for k in ("bvh_width", "build_max_leaf", "blocks_per_cu", "short_stack", "refill"):
+        if k in cfg:
+            s.set_option(k, cfg[k])
     t0 = time.perf_counter()
     s.initialize(tris, w, h)
     init_s = time.perf_counter() - t0

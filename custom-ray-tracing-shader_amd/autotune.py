@@ -16,6 +16,7 @@ from pathlib import Path
 from native_api import Session
 
 WIDTHS = (2, 4, 8)
+MAX_LEAF = (4,)
 BLOCKS = (4, 6, 8, 12)
 STACKS = (8, 16)
 REFILL = (0, 1)
@@ -48,6 +49,8 @@ def store_cache(cache):
 
 def apply(session, cfg):
     for key in ("blocks_per_cu", "short_stack", "refill"):
+        if key not in cfg:
+            continue
         if not session.set_option(key, cfg[key]):
             raise ValueError(f"option {key}={cfg[key]} rejected")
 
@@ -62,7 +65,8 @@ def measure(session, camera, settings, frames):
     return rays / (time.perf_counter() - t0) / 1e6
 
 
-def autotune(triangles, camera, settings, width, height, frames=3, widths=WIDTHS, use_cache=True, lib=None, log=None):
+def autotune(triangles, camera, settings, width, height, frames=3, widths=WIDTHS, use_cache=True, lib=None, log=None,
+             max_leaf=MAX_LEAF):
     """Returns (best config dict, list of all measurements)."""
     key = scene_key(len(triangles), width, height, settings.maxBounces)
     cache = load_cache() if use_cache else {}
@@ -70,14 +74,16 @@ def autotune(triangles, camera, settings, width, height, frames=3, widths=WIDTHS
         return cache[key], []
     results = []
     best = None
-    for w in widths:
+    configs = [(w, ml) for w in widths for ml in max_leaf]
+    for w, ml in configs:
         with Session(lib) as s:
             s.set_option("bvh_width", w)
+            s.set_option("build_max_leaf", ml)
             s.initialize(triangles, width, height)
             for b in BLOCKS:
                 for st in STACKS:
                     for r in REFILL:
-                        cfg = {"bvh_width": w, "blocks_per_cu": b, "short_stack": st, "refill": r}
+                        cfg = {"bvh_width": w, "build_max_leaf": ml, "blocks_per_cu": b, "short_stack": st, "refill": r}
                         apply(s, cfg)
                         cfg["mrays_per_s"] = round(measure(s, camera, settings, frames), 2)
                         results.append(cfg)

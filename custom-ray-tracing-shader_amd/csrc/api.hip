@@ -34,10 +34,13 @@ struct Session {
     int width = 0, height = 0;
     int bvhWidth = 4;            // option, used at the next RtInitialize
     int buildBins = 32;          // option: SAH bins per axis of the GPU build
+    int buildMaxLeaf = 4;        // option: leaf size limit of the GPU build
     int builtWidth = 4;
     int blocksPerCU = 8;         // persistent traversal workgroups per CU
     int shortStack = 8;          // LDS stack entries per traversal lane (8 or 16)
-    int refill = 1;              // per-lane ray refill in the traversal kernel
+    int refill = 0;              // per-lane ray refill in the traversal kernel
+    int tiledPrimary = 1;        // primary rays queued in 8x8 pixel tiles
+    int nontemporal = 1;         // rays / hits bypass cache retention in traversal
     std::vector<unsigned char> hostNodes;
     std::vector<TriRecord> hostRecs;
     std::vector<HostSum> hostSum;
@@ -145,7 +148,7 @@ SceneView deviceScene(const Session* s) {
 bool buildAccel(Session* s) {
     auto t0 = std::chrono::steady_clock::now();
     GpuBuildParams prm;
-    prm.bins = s->buildBins; prm.maxLeaf = 4; prm.width = s->bvhWidth;
+    prm.bins = s->buildBins; prm.maxLeaf = s->buildMaxLeaf; prm.width = s->bvhWidth;
     GpuBuildResult r;
     if ((s->error = buildBvhGpu(s->dTris, (int)s->tris.size(), prm, r, s->stream)) != hipSuccess) {
         s->hostError = std::string("GPU BVH build failed: ") + hipGetErrorString(s->error);
@@ -206,6 +209,7 @@ TraceArgs baseTraceArgs(Session* s) {
     a.nodes = s->dNodes; a.recs = s->dRecs;
     a.spill = s->dSpill; a.spillEntries = s->spillEntries; a.totalLanes = s->traceLanes;
     a.overflow = s->dOverflow;
+    a.nontemporal = s->nontemporal;
     return a;
 }
 
@@ -234,7 +238,7 @@ bool renderGpu(Session* s, const CameraData& cam, const RenderSettings& rs, floa
         f.cam = cam; f.rs = rs; f.sc = deviceScene(s);
         f.w = s->width; f.h = s->height; f.npix = npix; f.R = R; f.sample0 = s->samples + (unsigned)r0;
         f.thr = s->dThr; f.rad = s->dRad; f.alb = s->dAlb; f.nrm = s->dNrm; f.nee = s->dNee;
-        f.shadow = s->dShadow; f.hits = s->dHits; f.counters = ctr;
+        f.shadow = s->dShadow; f.hits = s->dHits; f.counters = ctr; f.tiled = s->tiledPrimary;
         {
             StageScope sc(s, RT_STAGE_RESET);
             if ((s->error = launchResetCounters(ctr, R * npix, st)) != hipSuccess) return false;
@@ -862,6 +866,10 @@ int RtSetOption(void* p, const char* key, int value) {
         if (value < 2 || value > 32) return 0;
         s->buildBins = value; return 1;
     }
+    if (!std::strcmp(key, "build_max_leaf")) {
+        if (value < 1 || value > 16) return 0;
+        s->buildMaxLeaf = value; return 1;
+    }
     if (!std::strcmp(key, "blocks_per_cu")) {
         if (value < 1 || value > 32) return 0;
         s->blocksPerCU = value; return 1;
@@ -877,6 +885,14 @@ int RtSetOption(void* p, const char* key, int value) {
     if (!std::strcmp(key, "refill")) {
         if (value != 0 && value != 1) return 0;
         s->refill = value; return 1;
+    }
+    if (!std::strcmp(key, "tiled_primary")) {
+        if (value != 0 && value != 1) return 0;
+        s->tiledPrimary = value; return 1;
+    }
+    if (!std::strcmp(key, "nontemporal")) {
+        if (value != 0 && value != 1) return 0;
+        s->nontemporal = value; return 1;
     }
     if (!std::strcmp(key, "stage_timing")) {
         if (value != 0 && value != 1) return 0;
@@ -895,9 +911,12 @@ int RtGetOption(void* p, const char* key) {
     if (!std::strcmp(key, "bvh_width")) return s->ready ? s->builtWidth : s->bvhWidth;
     if (!std::strcmp(key, "blocks_per_cu")) return s->blocksPerCU;
     if (!std::strcmp(key, "build_bins")) return s->buildBins;
+    if (!std::strcmp(key, "build_max_leaf")) return s->buildMaxLeaf;
     if (!std::strcmp(key, "short_stack")) return s->shortStack;
     if (!std::strcmp(key, "denoise_fallback")) return s->denoiseFallback;
     if (!std::strcmp(key, "refill")) return s->refill;
+    if (!std::strcmp(key, "tiled_primary")) return s->tiledPrimary;
+    if (!std::strcmp(key, "nontemporal")) return s->nontemporal;
     if (!std::strcmp(key, "stage_timing")) return s->stageTiming ? 1 : 0;
     if (!std::strcmp(key, "trace_stats")) return s->traceStats ? 1 : 0;
     return -1;

@@ -119,7 +119,7 @@ bool enqueueFrame(Batch* b) {
         TraceArgs ta{};
         ta.nodes = b->dNodes; ta.recs = b->dRecs; ta.spill = b->dSpill; ta.spillEntries = b->spillEntries;
         ta.totalLanes = b->traceBlocks * kBlock; ta.overflow = b->dOverflow;
-        ta.roots = b->dRoots; ta.viewBase = b->dViewBase; ta.views = b->views;
+        ta.roots = b->dRoots; ta.viewBase = b->dViewBase; ta.views = b->views; ta.nontemporal = 1;
         TraceArgs sa = ta;
         ta.rays = in; ta.countPtr = b->dCounters + kCntRays + bounce; ta.work = b->dCounters + kWorkRays + bounce;
         ta.hits = b->dHits;

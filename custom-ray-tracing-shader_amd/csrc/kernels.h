@@ -26,6 +26,7 @@ struct TraceArgs {
     int totalLanes;          // launched lanes (spill stride)
     unsigned long long* stats;   // optional: [0] node visits, [1] triangle tests
     int* overflow;           // set to 1 if a lane's stack exceeded the spill ring
+    int nontemporal;         // 1: stream rays in and hits out past the caches
     const int* roots;        // batched views: root node per view (null: root 0)
     const int* viewBase;     // batched views: first path slot per view, views + 1 entries
     int views;
@@ -42,6 +43,7 @@ struct FrameArgs {
     int4* hits;
     int* counters;           // this chunk's counters
     int bounce;
+    int tiled;               // 1: primary rays queued in 8x8 pixel tiles
 };
 
 struct AccumArgs {

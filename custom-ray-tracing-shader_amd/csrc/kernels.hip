@@ -120,7 +120,13 @@ __device__ __forceinline__ bool traceStep(LaneRay& L, const WideNode<W>* nodes, 
 template <bool ANY>
 __device__ __forceinline__ void finishRay(const TraceArgs& a, const LaneRay& L) {
     if (a.hits) {
-        a.hits[L.index] = make_int4(L.bestIdx, __float_as_int(L.best), __float_as_int(L.bu), __float_as_int(L.bv));
+        if (a.nontemporal) {
+            typedef int v4i __attribute__((ext_vector_type(4)));
+            const v4i hv = {L.bestIdx, __float_as_int(L.best), __float_as_int(L.bu), __float_as_int(L.bv)};
+            __builtin_nontemporal_store(hv, reinterpret_cast<v4i*>(a.hits + L.index));
+        } else {
+            a.hits[L.index] = make_int4(L.bestIdx, __float_as_int(L.best), __float_as_int(L.bu), __float_as_int(L.bv));
+        }
     } else if (L.bestIdx < 0) {
         // Shadow query found no occluder: add the stored next-event term.
         float4 acc = a.rad[L.slot];
@@ -141,7 +147,18 @@ __device__ __forceinline__ int findView(const int* viewBase, int views, int slot
 }
 
 __device__ __forceinline__ void startRay(const TraceArgs& a, int i, LaneRay& L) {
-    const float4 r0 = a.rays[2 * i], r1 = a.rays[2 * i + 1];
+    // Rays are read once: keep them from displacing BVH nodes and records in the caches.
+    float4 r0, r1;
+    if (a.nontemporal) {
+        typedef float v4f __attribute__((ext_vector_type(4)));
+        const v4f* rp = reinterpret_cast<const v4f*>(a.rays + 2 * i);
+        const v4f q0 = __builtin_nontemporal_load(rp), q1 = __builtin_nontemporal_load(rp + 1);
+        r0 = make_float4(q0.x, q0.y, q0.z, q0.w);
+        r1 = make_float4(q1.x, q1.y, q1.z, q1.w);
+    } else {
+        r0 = a.rays[2 * i];
+        r1 = a.rays[2 * i + 1];
+    }
     L.o = F3(r0.x, r0.y, r0.z);
     L.d = F3(r1.x, r1.y, r1.z);
     L.inv = safeInverse(L.d);
@@ -260,13 +277,30 @@ __device__ __forceinline__ void loadSobolTerms(unsigned* sShift, unsigned* sTerm
     __syncthreads();
 }
 
+// Bijection on [0, w*h): queue position -> pixel, visiting 8x8 tiles first (each wave of
+// primary rays covers one compact tile), then the right and bottom remainder strips.
+__device__ __forceinline__ int tiledPixel(int q, int w, int h) {
+    const int tw = w / 8, th = h / 8;
+    const int nA = tw * th * 64;
+    if (q < nA) {
+        const int t = q >> 6, k = q & 63;
+        return ((t / tw) * 8 + (k >> 3)) * w + (t % tw) * 8 + (k & 7);
+    }
+    q -= nA;
+    const int rw = w - tw * 8, nR = rw * th * 8;
+    if (q < nR) return (q / rw) * w + tw * 8 + q % rw;
+    q -= nR;
+    return (th * 8 + q / w) * w + q % w;
+}
+
 __global__ __launch_bounds__(kBlock) void raygenKernel(FrameArgs f) {
     __shared__ unsigned sShift[4], sTerm[kMaxRaysPerPixel * 4];
     loadSobolTerms<4>(sShift, sTerm, f.rs.seed, 0, f.sample0, f.R);
-    const int slot = blockIdx.x * kBlock + threadIdx.x;
-    if (slot >= f.R * f.npix) return;
-    const int r = slot / f.npix;
-    const unsigned pixel = (unsigned)(slot - r * f.npix);
+    const int q = blockIdx.x * kBlock + threadIdx.x;   // queue position
+    if (q >= f.R * f.npix) return;
+    const int r = q / f.npix;
+    const unsigned pixel = (unsigned)(f.tiled ? tiledPixel(q - r * f.npix, f.w, f.h) : q - r * f.npix);
+    const int slot = r * f.npix + (int)pixel;
     const int x = (int)(pixel % (unsigned)f.w), y = (int)(pixel / (unsigned)f.w);
     auto rng = [&](int k) { return sobolFromParts(pixel, sShift[k], sTerm[r * 4 + k]); };
     f3 o, d;
@@ -275,8 +309,8 @@ __global__ __launch_bounds__(kBlock) void raygenKernel(FrameArgs f) {
     f.rad[slot] = make_float4(0, 0, 0, 0);
     f.alb[slot] = make_float4(0, 0, 0, 0);
     f.nrm[slot] = make_float4(0, 0, 0, 0);
-    f.rayOut[2 * slot] = make_float4(o.x, o.y, o.z, 1e30f);
-    f.rayOut[2 * slot + 1] = make_float4(d.x, d.y, d.z, __int_as_float(slot));
+    f.rayOut[2 * q] = make_float4(o.x, o.y, o.z, 1e30f);
+    f.rayOut[2 * q + 1] = make_float4(d.x, d.y, d.z, __int_as_float(slot));
 }
 hipError_t launchRaygen(const FrameArgs& f, hipStream_t s) {
     const int n = f.R * f.npix;
@@ -387,11 +421,12 @@ hipError_t launchAccumulate(const AccumArgs& a, hipStream_t s) {
 // The same stages as the single-view frame, with each path slot's view looked up from its
 // slot; Sobol values are formed per lane (views have their own seeds and sample counts).
 __global__ __launch_bounds__(kBlock) void batchRaygenKernel(BatchArgs b) {
-    const int slot = blockIdx.x * kBlock + threadIdx.x;
-    if (slot >= b.totalSlots) return;
-    const int v = findView(b.viewBase, b.viewCount, slot);
+    const int q = blockIdx.x * kBlock + threadIdx.x;   // queue position
+    if (q >= b.totalSlots) return;
+    const int v = findView(b.viewBase, b.viewCount, q);
     const BatchView& V = b.views[v];
-    const unsigned pixel = (unsigned)(slot - V.slotBase);
+    const unsigned pixel = (unsigned)tiledPixel(q - V.slotBase, V.w, V.h);
+    const int slot = V.slotBase + (int)pixel;
     const int x = (int)(pixel % (unsigned)V.w), y = (int)(pixel / (unsigned)V.w);
     const unsigned sample = b.samples[v];
     auto rng = [&](int k) { return sobolSample(gSobolDirs, pixel, sample, V.rs.seed, (unsigned)k); };
@@ -401,8 +436,8 @@ __global__ __launch_bounds__(kBlock) void batchRaygenKernel(BatchArgs b) {
     b.rad[slot] = make_float4(0, 0, 0, 0);
     b.alb[slot] = make_float4(0, 0, 0, 0);
     b.nrm[slot] = make_float4(0, 0, 0, 0);
-    b.rayOut[2 * slot] = make_float4(o.x, o.y, o.z, 1e30f);
-    b.rayOut[2 * slot + 1] = make_float4(d.x, d.y, d.z, __int_as_float(slot));
+    b.rayOut[2 * q] = make_float4(o.x, o.y, o.z, 1e30f);
+    b.rayOut[2 * q + 1] = make_float4(d.x, d.y, d.z, __int_as_float(slot));
 }
 hipError_t launchBatchRaygen(const BatchArgs& b, hipStream_t s) {
     hipLaunchKernelGGL(batchRaygenKernel, dim3((b.totalSlots + kBlock - 1) / kBlock), dim3(kBlock), 0, s, b);

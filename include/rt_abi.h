@@ -116,8 +116,9 @@ int RtBvhExport(void* session, RtBvhNode* nodes, int nodeCapacity, int* order, i
 int RtReadAccumulation(void* session, float* out, long long floatCount);
 /* Ray queries issued by the last RtRender (closest-hit and shadow). */
 int RtRayCounts(void* session, unsigned long long* closest, unsigned long long* shadow);
-/* Tuning knobs: "bvh_width" (2, 4 or 8) and "build_bins" (SAH bins per axis of the GPU
- * build, 2..32, default 32) take effect at the next RtInitialize; "blocks_per_cu"
+/* Tuning knobs: "bvh_width" (2, 4 or 8), "build_bins" (SAH bins per axis of the GPU
+ * build, 2..32, default 32) and "build_max_leaf" (leaf size limit, 1..16, default 4 as the
+ * reference builder) take effect at the next RtInitialize; "blocks_per_cu"
  * (persistent traversal workgroups per compute unit, 1..32), "short_stack" (LDS stack
  * entries per traversal lane, 8 or 16), "refill" (per-lane ray refill, 0/1),
  * "stage_timing" and "trace_stats" (0/1) at the next render or query.  Returns 1 if the key is known and the value
