@@ -37,15 +37,16 @@ HBM_PEAK_GBS = 8000.0
 # The soup keeps the reference builder's leaf <= 4 rule (its configuration names it); the
 # icosphere uses the autotuner's leaf size.
 TUNED = {
-    "soup": {"bvh_width": 4, "build_max_leaf": 4, "blocks_per_cu": 8, "short_stack": 8, "refill": 0},
-    "icosphere": {"bvh_width": 4, "build_max_leaf": 2, "blocks_per_cu": 4, "short_stack": 8, "refill": 0},
-    "cornell": {"bvh_width": 4, "build_max_leaf": 4, "blocks_per_cu": 8, "short_stack": 8, "refill": 0},
-    "batch": {"bvh_width": 4, "blocks_per_cu": 8, "short_stack": 8, "refill": 0},
+    "soup": {"bvh_width": 4, "build_max_leaf": 4, "blocks_per_cu": 6, "short_stack": 8, "while_while": 1, "refill": 1},
+    "icosphere": {"bvh_width": 4, "build_max_leaf": 2, "blocks_per_cu": 4, "short_stack": 8, "while_while": 0,
+                  "refill": 0},
+    "cornell": {"bvh_width": 4, "build_max_leaf": 4, "blocks_per_cu": 6, "short_stack": 8, "while_while": 1, "refill": 1},
+    "batch": {"bvh_width": 4, "blocks_per_cu": 6, "short_stack": 8, "while_while": 1, "refill": 1},
 }
 
 
 def batch_views(count, seed=2024):
-    """scenes from the soup / displaced-sphere generators with triangle counts
+    """Batched views: scenes from the soup / displaced-sphere generators with triangle counts
     log-uniform in [1e3, 2e6], image sides log-uniform in [256, 1024], 2..8 bounces."""
     import scenes
     rng = np.random.default_rng(seed)
@@ -94,7 +95,6 @@ def bench_batch(args):
                        "pixels": pixels, "launch": cfg, "build_ms": round(build_ms, 1)}}
     print(json.dumps(line), flush=True)
     return 0
-NODE_BYTES = {2: 64, 4: 128, 8: 256}
 
 
 def log(msg):
@@ -235,6 +235,8 @@ def main():
     ap.add_argument("--short-stack", type=int)
     ap.add_argument("--refill", type=int)
     ap.add_argument("--build-max-leaf", type=int)
+    ap.add_argument("--bvh-quantized", type=int, choices=(0, 1))
+    ap.add_argument("--while-while", type=int, choices=(0, 1))
     ap.add_argument("--autotune", action="store_true",
                     help="search the launch configuration now instead of using the recorded tuned one")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
@@ -245,6 +247,10 @@ def main():
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run the plain copy kernel over BYTES (HBM counter calibration) and exit")
     args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (paths are independent per pixel, "
+              f"so a multi-GPU run would shard image rows; see the design notes)", file=sys.stderr)
+        return 2
     if args.copy_probe:
         import ctypes
         import native_api
@@ -258,11 +264,6 @@ def main():
         return psnr_study(args)
     if args.workload == "batch":
         return bench_batch(args)
-    if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (paths are independent per pixel, "
-              f"so a multi-GPU run would shard image rows; see the design notes)", file=sys.stderr)
-        return 2
-
     import native_api
     desc, tris, cam, st, w, h = workload(args.workload, args.tris)
     log(f"workload: {desc}")
@@ -278,7 +279,7 @@ def main():
         cfg = {k: best[k] for k in cfg}
         log(f"autotune: {len(trials)} configurations in {time.perf_counter() - t0:.1f} s, best {best}")
     s = native_api.Session()
-    for k in ("bvh_width", "build_max_leaf", "blocks_per_cu", "short_stack", "refill"):
+    for k in ("bvh_width", "bvh_quantized", "build_max_leaf", "blocks_per_cu", "short_stack", "refill", "while_while"):
         if k in cfg:
             s.set_option(k, cfg[k])
     t0 = time.perf_counter()
@@ -318,8 +319,9 @@ def main():
     stats = s.trace_stats()
     closest_rays, shadow_rays = s.ray_counts()
     s.set_option("trace_stats", 0)
-    width = native_api.load_library().RtGetOption(s.handle, b"bvh_width")
-    node_b = NODE_BYTES[width]
+    width = s.lib.RtGetOption(s.handle, b"bvh_width")
+    quantized = s.lib.RtGetOption(s.handle, b"bvh_quantized")
+    node_b = s.lib.RtGetOption(s.handle, b"node_bytes")
     launches = stage_ms["closest"][1] / 3
     closest_ms = stage_ms["closest"][0] / 3
     bytes_frame = 48 * closest_rays + stats["closest_nodes"] * node_b + stats["closest_tris"] * 48
@@ -369,7 +371,7 @@ def main():
                    "device_ms_per_step": round(device_ms / args.steps, 4), "bvh_build_ms": round(build_ms, 1)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": f"traceKernel<{width},closest>", "mean_launch_ms": round(mean_launch_s * 1e3, 4),
+                     "kernel": f"traceKernel<{'QNode' if quantized else 'WideNode'}<{width}>,closest>", "node_bytes": node_b, "mean_launch_ms": round(mean_launch_s * 1e3, 4),
                      "launches_per_step": launches, "algorithmic_bytes_per_launch": int(bytes_launch),
                      "nodes_per_ray": round(stats["closest_nodes"] / max(closest_rays, 1), 2),
                      "tris_per_ray": round(stats["closest_tris"] / max(closest_rays, 1), 2)},

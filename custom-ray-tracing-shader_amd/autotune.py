@@ -1,9 +1,10 @@
-"""Autotuner for the traversal launch: BVH width, persistent workgroups per CU, LDS
-short-stack depth and per-lane ray refill.
+"""Autotuner for the traversal launch: BVH width and leaf size, persistent workgroups per
+CU, LDS short-stack depth, and the traversal loop (speculative while-while with per-lane ray
+refill, or one step per iteration).
 
-For a scene, camera and settings it initialises one session per BVH width, times a few
-device-resident frames for every (blocks_per_cu, short_stack, refill) combination, and keeps
-the configuration with the highest ray throughput.  Choices persist in a JSON cache keyed by
+For a scene, camera and settings it initialises one session per (BVH width, leaf size),
+times a few device-resident frames for every (blocks_per_cu, short_stack, loop) combination,
+and keeps the configuration with the highest ray throughput.  Choices persist in a JSON cache keyed by
 scene size class and resolution ($RT_AUTOTUNE_CACHE, default ~/.cache/rt_amd/autotune.json),
 so later sessions of a similar scene reuse them without searching again.
 """
@@ -15,11 +16,11 @@ from pathlib import Path
 
 from native_api import Session
 
-WIDTHS = (2, 4, 8)
+WIDTHS = (4, 8)
 MAX_LEAF = (4,)
-BLOCKS = (4, 6, 8, 12)
+BLOCKS = (4, 6, 8)
 STACKS = (8, 16)
-REFILL = (0, 1)
+LOOPS = ((1, 1), (1, 0), (0, 0))   # (while_while, refill)
 
 
 def cache_path():
@@ -48,7 +49,7 @@ def store_cache(cache):
 
 
 def apply(session, cfg):
-    for key in ("blocks_per_cu", "short_stack", "refill"):
+    for key in ("blocks_per_cu", "short_stack", "refill", "while_while"):
         if key not in cfg:
             continue
         if not session.set_option(key, cfg[key]):
@@ -82,8 +83,9 @@ def autotune(triangles, camera, settings, width, height, frames=3, widths=WIDTHS
             s.initialize(triangles, width, height)
             for b in BLOCKS:
                 for st in STACKS:
-                    for r in REFILL:
-                        cfg = {"bvh_width": w, "build_max_leaf": ml, "blocks_per_cu": b, "short_stack": st, "refill": r}
+                    for ww, r in LOOPS:
+                        cfg = {"bvh_width": w, "build_max_leaf": ml, "blocks_per_cu": b, "short_stack": st,
+                               "while_while": ww, "refill": r}
                         apply(s, cfg)
                         cfg["mrays_per_s"] = round(measure(s, camera, settings, frames), 2)
                         results.append(cfg)

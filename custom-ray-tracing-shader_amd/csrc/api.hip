@@ -36,9 +36,12 @@ struct Session {
     int buildBins = 32;          // option: SAH bins per axis of the GPU build
     int buildMaxLeaf = 4;        // option: leaf size limit of the GPU build
     int builtWidth = 4;
+    int bvhQuantized = 0;        // option: quantized (QNode) wide nodes at the next RtInitialize
+    int builtQuantized = 0;
     int blocksPerCU = 8;         // persistent traversal workgroups per CU
     int shortStack = 8;          // LDS stack entries per traversal lane (8 or 16)
-    int refill = 0;              // per-lane ray refill in the traversal kernel
+    int refill = 1;              // per-lane ray refill in the traversal kernel
+    int whileWhile = 1;          // speculative while-while traversal loop
     int tiledPrimary = 1;        // primary rays queued in 8x8 pixel tiles
     int nontemporal = 1;         // rays / hits bypass cache retention in traversal
     std::vector<unsigned char> hostNodes;
@@ -47,7 +50,7 @@ struct Session {
     // device
     Triangle* dTris = nullptr;
     Light* dLights = nullptr;
-    void* dNodes = nullptr;          // WideNode<builtWidth>[]
+    void* dNodes = nullptr;          // WideNode<builtWidth>[] or QNode<builtWidth>[]
     TriRecord* dRecs = nullptr;
     RtBvhNode* dBinNodes = nullptr;  // binary tree of the GPU build
     int* dOrder = nullptr;
@@ -148,7 +151,7 @@ SceneView deviceScene(const Session* s) {
 bool buildAccel(Session* s) {
     auto t0 = std::chrono::steady_clock::now();
     GpuBuildParams prm;
-    prm.bins = s->buildBins; prm.maxLeaf = s->buildMaxLeaf; prm.width = s->bvhWidth;
+    prm.bins = s->buildBins; prm.maxLeaf = s->buildMaxLeaf; prm.width = s->bvhWidth; prm.quantized = s->bvhQuantized;
     GpuBuildResult r;
     if ((s->error = buildBvhGpu(s->dTris, (int)s->tris.size(), prm, r, s->stream)) != hipSuccess) {
         s->hostError = std::string("GPU BVH build failed: ") + hipGetErrorString(s->error);
@@ -157,6 +160,7 @@ bool buildAccel(Session* s) {
     s->dBinNodes = r.binNodes; s->dOrder = r.order; s->dRecs = r.recs; s->dNodes = r.wideNodes;
     s->binNodeCount = r.binNodeCount; s->wideNodeCount = r.wideCount; s->bvhDepth = r.depth;
     s->builtWidth = s->bvhWidth;
+    s->builtQuantized = s->bvhQuantized;
     s->buildMs = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return true;
 }
@@ -164,7 +168,7 @@ bool buildAccel(Session* s) {
 // Host copies of the wide nodes and records for the CPU backends (same tree as the GPU).
 bool fetchHostAccel(Session* s) {
     if (!s->hostRecs.empty()) return true;
-    const size_t nodeBytes = (size_t)s->wideNodeCount * 32 * s->builtWidth;
+    const size_t nodeBytes = (size_t)s->wideNodeCount * wideNodeBytes(s->builtWidth, s->builtQuantized);
     s->hostNodes.resize(nodeBytes);
     s->hostRecs.resize(s->tris.size());
     if ((s->error = hipMemcpy(s->hostNodes.data(), s->dNodes, nodeBytes, hipMemcpyDeviceToHost)) != hipSuccess ||
@@ -188,6 +192,13 @@ bool fetchBinaryTree(Session* s) {
         return false;
     }
     return true;
+}
+
+TraceLaunch traceLaunch(const Session* s) {
+    TraceLaunch l;
+    l.width = s->builtWidth; l.quantized = s->builtQuantized != 0; l.stack = s->shortStack;
+    l.refill = s->refill != 0; l.whileWhile = s->whileWhile != 0;
+    return l;
 }
 
 // Sizes the persistent traversal grid from the current options; grows the spill ring.
@@ -256,7 +267,7 @@ bool renderGpu(Session* s, const CameraData& cam, const RenderSettings& rs, floa
             ta.stats = s->dStats;
             {
                 StageScope sc(s, RT_STAGE_CLOSEST);
-                if ((s->error = launchTrace(s->builtWidth, false, s->traceStats, s->traceBlocks, s->shortStack, s->refill != 0, ta, st)) != hipSuccess)
+                if ((s->error = launchTrace(traceLaunch(s), false, s->traceStats, s->traceBlocks, ta, st)) != hipSuccess)
                     return false;
             }
             f.bounce = b; f.rayIn = in; f.rayOut = out;
@@ -269,7 +280,7 @@ bool renderGpu(Session* s, const CameraData& cam, const RenderSettings& rs, floa
                 sa.rays = s->dShadow; sa.countPtr = ctr + kCntShadow + b; sa.work = ctr + kWorkShadow + b;
                 sa.rad = s->dRad; sa.contrib = s->dNee; sa.stats = s->dStats + 2;
                 StageScope sc(s, RT_STAGE_SHADOW);
-                if ((s->error = launchTrace(s->builtWidth, true, s->traceStats, s->traceBlocks, s->shortStack, s->refill != 0, sa, st)) != hipSuccess)
+                if ((s->error = launchTrace(traceLaunch(s), true, s->traceStats, s->traceBlocks, sa, st)) != hipSuccess)
                     return false;
             }
         }
@@ -321,10 +332,10 @@ bool validRender(Session* s, const CameraData* camera, const RenderSettings* set
 }
 
 // Native/renderer.cu:289-302 on the host, over the same wide BVH as the GPU.
-template <int W>
+template <class N>
 void renderCpuRows(Session* s, const CameraData& cam, const RenderSettings& rs, unsigned char* out, bool multi,
                    unsigned long long& closest, unsigned long long& shadow) {
-    const WideNode<W>* nodes = reinterpret_cast<const WideNode<W>*>(s->hostNodes.data());
+    const N* nodes = reinterpret_cast<const N*>(s->hostNodes.data());
     const TriRecord* recs = s->hostRecs.data();
     SceneView sc; sc.tris = s->tris.data(); sc.lights = s->lights.data(); sc.lightCount = (int)s->lights.size(); sc.lightArea = s->lightArea;
     const int w = s->width, h = s->height;
@@ -351,13 +362,13 @@ void renderCpuRows(Session* s, const CameraData& cam, const RenderSettings& rs, 
                     FirstHit fh; fh.covered = false; fh.albedo = fh.normal = F3(0, 0, 0); fh.depth = 0;
                     for (int b = 0; b < rs.maxBounces; b++) {
                         base = 4 + 8 * (unsigned)b;
-                        RtHit hit = traverseHost<W, false>(nodes, recs, o, d, 1e30f);
+                        RtHit hit = traverseHost<N, false>(nodes, recs, o, d, 1e30f);
                         cc++;
                         ShadeResult res;
                         shadeBounce(sc, rs, b, rng, o, d, hit, ps, &fh, res);
                         if (res.shadow) {
                             ss++;
-                            if (traverseHost<W, true>(nodes, recs, res.nextO, res.sdir, res.slimit).index < 0)
+                            if (traverseHost<N, true>(nodes, recs, res.nextO, res.sdir, res.slimit).index < 0)
                                 ps.rad = ps.rad + res.contrib;
                         }
                         if (!res.cont) break;
@@ -512,9 +523,15 @@ int RtRender(void* p, const CameraData* camera, const RenderSettings* settings, 
         auto t0 = std::chrono::steady_clock::now();
         const bool multi = backend == RT_BACKEND_CPU_MULTI;
         unsigned long long c = 0, sh = 0;
-        if (s->builtWidth == 2) renderCpuRows<2>(s, *camera, *settings, output, multi, c, sh);
-        else if (s->builtWidth == 8) renderCpuRows<8>(s, *camera, *settings, output, multi, c, sh);
-        else renderCpuRows<4>(s, *camera, *settings, output, multi, c, sh);
+        if (s->builtQuantized) {
+            if (s->builtWidth == 2) renderCpuRows<QNode<2>>(s, *camera, *settings, output, multi, c, sh);
+            else if (s->builtWidth == 8) renderCpuRows<QNode<8>>(s, *camera, *settings, output, multi, c, sh);
+            else renderCpuRows<QNode<4>>(s, *camera, *settings, output, multi, c, sh);
+        } else {
+            if (s->builtWidth == 2) renderCpuRows<WideNode<2>>(s, *camera, *settings, output, multi, c, sh);
+            else if (s->builtWidth == 8) renderCpuRows<WideNode<8>>(s, *camera, *settings, output, multi, c, sh);
+            else renderCpuRows<WideNode<4>>(s, *camera, *settings, output, multi, c, sh);
+        }
         s->lastClosest = c; s->lastShadow = sh;
         *computeMs = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
     }
@@ -686,7 +703,7 @@ int RtTraceRays(void* p, const RtRay* rays, int count, int anyHit, RtHit* hits) 
     if (ok) {
         TraceArgs a = baseTraceArgs(s);
         a.rays = dr; a.count = count; a.work = work; a.hits = dh;
-        ok = (s->error = launchTrace(s->builtWidth, anyHit != 0, false, s->traceBlocks, s->shortStack, s->refill != 0, a, s->stream)) == hipSuccess;
+        ok = (s->error = launchTrace(traceLaunch(s), anyHit != 0, false, s->traceBlocks, a, s->stream)) == hipSuccess;
     }
     if (ok) ok = (s->error = hipStreamSynchronize(s->stream)) == hipSuccess;
     if (ok) ok = (s->error = hipMemcpy(hits, dh, (size_t)count * sizeof(RtHit), hipMemcpyDeviceToHost)) == hipSuccess;
@@ -862,6 +879,10 @@ int RtSetOption(void* p, const char* key, int value) {
         if (value != 2 && value != 4 && value != 8) return 0;
         s->bvhWidth = value; return 1;
     }
+    if (!std::strcmp(key, "bvh_quantized")) {
+        if (value != 0 && value != 1) return 0;
+        s->bvhQuantized = value; return 1;
+    }
     if (!std::strcmp(key, "build_bins")) {
         if (value < 2 || value > 32) return 0;
         s->buildBins = value; return 1;
@@ -886,6 +907,10 @@ int RtSetOption(void* p, const char* key, int value) {
         if (value != 0 && value != 1) return 0;
         s->refill = value; return 1;
     }
+    if (!std::strcmp(key, "while_while")) {
+        if (value != 0 && value != 1) return 0;
+        s->whileWhile = value; return 1;
+    }
     if (!std::strcmp(key, "tiled_primary")) {
         if (value != 0 && value != 1) return 0;
         s->tiledPrimary = value; return 1;
@@ -909,12 +934,16 @@ int RtGetOption(void* p, const char* key) {
     Session* s = static_cast<Session*>(p);
     if (!s || !key) return -1;
     if (!std::strcmp(key, "bvh_width")) return s->ready ? s->builtWidth : s->bvhWidth;
+    if (!std::strcmp(key, "bvh_quantized")) return s->ready ? s->builtQuantized : s->bvhQuantized;
+    if (!std::strcmp(key, "node_bytes")) return (int)wideNodeBytes(s->ready ? s->builtWidth : s->bvhWidth,
+                                                                    s->ready ? s->builtQuantized : s->bvhQuantized);
     if (!std::strcmp(key, "blocks_per_cu")) return s->blocksPerCU;
     if (!std::strcmp(key, "build_bins")) return s->buildBins;
     if (!std::strcmp(key, "build_max_leaf")) return s->buildMaxLeaf;
     if (!std::strcmp(key, "short_stack")) return s->shortStack;
     if (!std::strcmp(key, "denoise_fallback")) return s->denoiseFallback;
     if (!std::strcmp(key, "refill")) return s->refill;
+    if (!std::strcmp(key, "while_while")) return s->whileWhile;
     if (!std::strcmp(key, "tiled_primary")) return s->tiledPrimary;
     if (!std::strcmp(key, "nontemporal")) return s->nontemporal;
     if (!std::strcmp(key, "stage_timing")) return s->stageTiming ? 1 : 0;

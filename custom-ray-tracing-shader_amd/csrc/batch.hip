@@ -17,16 +17,25 @@ using namespace rt;
 
 namespace {
 
-template <int W>
-__global__ void rebaseNodesKernel(WideNode<W>* nodes, int count, int nodeOffset, int recordOffset) {
+// Moves one view's wide nodes into the global arrays: inner references by the node offset,
+// leaf references by the record offset (the first record sits above the 3-bit count code).
+template <class N>
+__global__ void rebaseNodesKernel(N* nodes, int count, int nodeOffset, int recordOffset) {
+    constexpr int W = NodeTraits<N>::kWidth;
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= count) return;
-    WideNode<W>& n = nodes[i];
+    N& n = nodes[i];
     for (int c = 0; c < W; c++) {
         const int r = n.child[c];
         if (r == kEmptyChild) continue;
-        n.child[c] = r >= 0 ? r + nodeOffset : ~(~r + recordOffset);
+        n.child[c] = r >= 0 ? r + nodeOffset : ~(~r + (recordOffset << 3));
     }
+}
+template <class N>
+void launchRebase(const GpuBuildResult& r, int nodeOffset, int recordOffset, hipStream_t s) {
+    const int nv = r.wideCount;
+    hipLaunchKernelGGL(rebaseNodesKernel<N>, dim3((nv + kBlock - 1) / kBlock), dim3(kBlock), 0, s, (N*)r.wideNodes, nv,
+                       nodeOffset, recordOffset);
 }
 
 __global__ void rebaseRecordsKernel(TriRecord* recs, int count, int triangleOffset) {
@@ -44,7 +53,7 @@ struct PendingView {
 struct Batch {
     std::vector<PendingView> pending;
     std::string error;
-    int width = 4, blocksPerCU = 8, shortStack = 8, refill = 0, useGraph = 1;
+    int width = 4, quantized = 0, blocksPerCU = 8, shortStack = 8, refill = 1, whileWhile = 1, useGraph = 1;
     bool built = false;
     int cus = 0, views = 0, totalSlots = 0, maxBounces = 0;
     long long totalTris = 0;
@@ -107,6 +116,13 @@ BatchArgs batchArgs(Batch* b) {
 }
 
 // The launches of one frame (recorded into the graph, or issued directly).
+TraceLaunch launchOf(const Batch* b) {
+    TraceLaunch l;
+    l.width = b->width; l.quantized = b->quantized != 0; l.stack = b->shortStack;
+    l.refill = b->refill != 0; l.whileWhile = b->whileWhile != 0;
+    return l;
+}
+
 bool enqueueFrame(Batch* b) {
     hipStream_t st = b->stream;
     BatchArgs a = batchArgs(b);
@@ -123,12 +139,12 @@ bool enqueueFrame(Batch* b) {
         TraceArgs sa = ta;
         ta.rays = in; ta.countPtr = b->dCounters + kCntRays + bounce; ta.work = b->dCounters + kWorkRays + bounce;
         ta.hits = b->dHits;
-        B_TRY(launchTrace(b->width, false, false, b->traceBlocks, b->shortStack, b->refill != 0, ta, st));
+        B_TRY(launchTrace(launchOf(b), false, false, b->traceBlocks, ta, st));
         a.bounce = bounce; a.rayIn = in; a.rayOut = out;
         B_TRY(launchBatchShade(a, b->cus * 8, st));
         sa.rays = b->dShadow; sa.countPtr = b->dCounters + kCntShadow + bounce; sa.work = b->dCounters + kWorkShadow + bounce;
         sa.rad = b->dRad; sa.contrib = b->dNee;
-        B_TRY(launchTrace(b->width, true, false, b->traceBlocks, b->shortStack, b->refill != 0, sa, st));
+        B_TRY(launchTrace(launchOf(b), true, false, b->traceBlocks, sa, st));
     }
     B_TRY(launchBatchAccumulate(a, st));
     B_TRY(launchBatchAdvance(a, st));
@@ -176,22 +192,35 @@ bool build(Batch* b) {
     std::vector<GpuBuildResult> res(b->views);
     GpuBuildParams prm;
     prm.width = b->width;
+    prm.quantized = b->quantized;
+    if ((long long)tris.size() >= kLeafFirstLimit) {
+        b->error = "batch holds too many triangles for leaf references";
+        return false;
+    }
     long long nodeTotal = 0;
     for (int v = 0; v < b->views; v++) {
         B_TRY(buildBvhGpu(b->dTris + triOff[v], (int)b->pending[v].tris.size(), prm, res[v], b->stream));
         nodeTotal += res[v].wideCount;
     }
-    const size_t nodeBytes = (size_t)32 * b->width;
+    const size_t nodeBytes = wideNodeBytes(b->width, b->quantized);
     unsigned char* nodes = nullptr;
-    if (!dalloc(b, nodes, (size_t)nodeTotal * nodeBytes) || !dalloc(b, b->dRecs, tris.size())) return false;
+    if (!dalloc(b, nodes, (size_t)nodeTotal * nodeBytes) || !dalloc(b, b->dRecs, tris.size() + kRecordPad)) return false;
+    B_TRY(hipMemsetAsync(b->dRecs + tris.size(), 0, kRecordPad * sizeof(TriRecord), b->stream));
     b->dNodes = nodes;
     std::vector<int> roots(b->views);
     long long nodeOff = 0;
     for (int v = 0; v < b->views; v++) {
         const int nv = res[v].wideCount, nt = (int)b->pending[v].tris.size();
-        if (b->width == 2) hipLaunchKernelGGL(rebaseNodesKernel<2>, dim3((nv + kBlock - 1) / kBlock), dim3(kBlock), 0, b->stream, (WideNode<2>*)res[v].wideNodes, nv, (int)nodeOff, triOff[v]);
-        else if (b->width == 8) hipLaunchKernelGGL(rebaseNodesKernel<8>, dim3((nv + kBlock - 1) / kBlock), dim3(kBlock), 0, b->stream, (WideNode<8>*)res[v].wideNodes, nv, (int)nodeOff, triOff[v]);
-        else hipLaunchKernelGGL(rebaseNodesKernel<4>, dim3((nv + kBlock - 1) / kBlock), dim3(kBlock), 0, b->stream, (WideNode<4>*)res[v].wideNodes, nv, (int)nodeOff, triOff[v]);
+        const int no = (int)nodeOff, ro = triOff[v];
+        if (b->quantized) {
+            if (b->width == 2) launchRebase<QNode<2>>(res[v], no, ro, b->stream);
+            else if (b->width == 8) launchRebase<QNode<8>>(res[v], no, ro, b->stream);
+            else launchRebase<QNode<4>>(res[v], no, ro, b->stream);
+        } else {
+            if (b->width == 2) launchRebase<WideNode<2>>(res[v], no, ro, b->stream);
+            else if (b->width == 8) launchRebase<WideNode<8>>(res[v], no, ro, b->stream);
+            else launchRebase<WideNode<4>>(res[v], no, ro, b->stream);
+        }
         hipLaunchKernelGGL(rebaseRecordsKernel, dim3((nt + kBlock - 1) / kBlock), dim3(kBlock), 0, b->stream, res[v].recs, nt, triOff[v]);
         B_TRY(hipGetLastError());
         B_TRY(hipMemcpyAsync(nodes + nodeOff * nodeBytes, res[v].wideNodes, (size_t)nv * nodeBytes, hipMemcpyDeviceToDevice, b->stream));
@@ -297,9 +326,11 @@ int RtBatchSetOption(void* p, const char* key, int value) {
     Batch* b = static_cast<Batch*>(p);
     if (!b || !key || b->built) return 0;
     if (!std::strcmp(key, "bvh_width") && (value == 2 || value == 4 || value == 8)) { b->width = value; return 1; }
+    if (!std::strcmp(key, "bvh_quantized") && (value == 0 || value == 1)) { b->quantized = value; return 1; }
     if (!std::strcmp(key, "blocks_per_cu") && value >= 1 && value <= 32) { b->blocksPerCU = value; return 1; }
     if (!std::strcmp(key, "short_stack") && (value == 8 || value == 16)) { b->shortStack = value; return 1; }
     if (!std::strcmp(key, "refill") && (value == 0 || value == 1)) { b->refill = value; return 1; }
+    if (!std::strcmp(key, "while_while") && (value == 0 || value == 1)) { b->whileWhile = value; return 1; }
     if (!std::strcmp(key, "use_graph") && (value == 0 || value == 1)) { b->useGraph = value; return 1; }
     return 0;
 }

@@ -358,10 +358,35 @@ __device__ __forceinline__ float nodeHalfArea(const RtBvhNode& n) {
     return halfAreaOf(n.low[0], n.low[1], n.low[2], n.high[0], n.high[1], n.high[2]);
 }
 
-// Greedy collapse (largest-area inner child opened first), as WideBuilder in bvh_wide.h.
+// Writes one wide node from its nk children (binary node boxes and wide references).
 template <int W>
-__global__ void collapseKernel(const WideTask* tasks, int taskCount, const RtBvhNode* bin, WideNode<W>* out,
-                               int* counters, WideTask* next) {
+__device__ __forceinline__ void writeWide(WideNode<W>& node, const RtBvhNode* kids, const int* refs, int nk) {
+    for (int c = 0; c < W; c++) {
+        node.lox[c] = node.loy[c] = node.loz[c] = INFINITY;
+        node.hix[c] = node.hiy[c] = node.hiz[c] = INFINITY;
+        node.child[c] = kEmptyChild; node.count[c] = 0;
+    }
+    for (int c = 0; c < nk; c++) {
+        const RtBvhNode& k = kids[c];
+        node.lox[c] = k.low[0]; node.loy[c] = k.low[1]; node.loz[c] = k.low[2];
+        node.hix[c] = k.high[0]; node.hiy[c] = k.high[1]; node.hiz[c] = k.high[2];
+        node.child[c] = refs[c]; node.count[c] = k.count;
+    }
+}
+template <int W>
+__device__ __forceinline__ void writeWide(QNode<W>& node, const RtBvhNode* kids, const int* refs, int nk) {
+    float lo[W][3], hi[W][3];
+    for (int c = 0; c < nk; c++)
+        for (int a = 0; a < 3; a++) { lo[c][a] = kids[c].low[a]; hi[c][a] = kids[c].high[a]; }
+    quantizeNode<W>(node, lo, hi, refs, nk);
+}
+
+// Greedy collapse (largest-area inner child opened first) of the binary tree into N nodes
+// (WideNode<W> or QNode<W>); leaves keep their binary-tree record ranges.
+template <class N>
+__global__ void collapseKernel(const WideTask* tasks, int taskCount, const RtBvhNode* bin, N* out, int* counters,
+                               WideTask* next) {
+    constexpr int W = NodeTraits<N>::kWidth;
     const int i = blockIdx.x * kB + threadIdx.x;
     if (i >= taskCount) return;
     const WideTask t = tasks[i];
@@ -383,24 +408,20 @@ __global__ void collapseKernel(const WideTask* tasks, int taskCount, const RtBvh
         kids[pick] = f;
         kids[nk++] = f + 1;
     }
-    WideNode<W> node;
-    for (int c = 0; c < W; c++) {
-        node.lox[c] = node.loy[c] = node.loz[c] = INFINITY;
-        node.hix[c] = node.hiy[c] = node.hiz[c] = INFINITY;
-        node.child[c] = kEmptyChild; node.count[c] = 0;
-    }
+    RtBvhNode kn[W];
+    int refs[W];
     for (int c = 0; c < nk; c++) {
-        const RtBvhNode k = bin[kids[c]];
-        node.lox[c] = k.low[0]; node.loy[c] = k.low[1]; node.loz[c] = k.low[2];
-        node.hix[c] = k.high[0]; node.hiy[c] = k.high[1]; node.hiz[c] = k.high[2];
-        if (k.count > 0) {
-            node.child[c] = ~k.first; node.count[c] = k.count;
+        kn[c] = bin[kids[c]];
+        if (kn[c].count > 0) {
+            refs[c] = makeLeafRef(kn[c].first, kn[c].count);
         } else {
             const int idx = atomicAdd(&counters[0], 1);
-            node.child[c] = idx;
+            refs[c] = idx;
             next[atomicAdd(&counters[1], 1)] = WideTask{kids[c], idx, t.depth + 1};
         }
     }
+    N node;
+    writeWide(node, kn, refs, nk);
     out[t.out] = node;
 }
 
@@ -418,18 +439,18 @@ template <class T> hipError_t grow(T*& p, size_t& cap, size_t need) {
 
 }  // namespace
 
-template <int W>
+template <class N>
 static hipError_t collapseW(const RtBvhNode* dBin, int binCount, void** wideOut, int* wideCount, int* wideDepth,
                             hipStream_t s) {
     hipError_t err = hipSuccess;
-    WideNode<W>* out = nullptr;
+    N* out = nullptr;
     WideTask *ta = nullptr, *tb = nullptr;
     int* ctr = nullptr;
     int h[3] = {1, 1, 0};   // wide node count, next task count, max depth
     int taskCount = 1;
     const WideTask root{0, 0, 0};
     // At most one wide node per binary inner node, plus the root.
-    RT_TRY(hipMalloc(&out, (size_t)(binCount + 1) * sizeof(WideNode<W>)));
+    RT_TRY(hipMalloc(&out, (size_t)(binCount + 1) * sizeof(N)));
     RT_TRY(hipMalloc(&ta, (size_t)(binCount + 1) * sizeof(WideTask)));
     RT_TRY(hipMalloc(&tb, (size_t)(binCount + 1) * sizeof(WideTask)));
     RT_TRY(hipMalloc(&ctr, 3 * sizeof(int)));
@@ -438,7 +459,7 @@ static hipError_t collapseW(const RtBvhNode* dBin, int binCount, void** wideOut,
     RT_TRY(hipMemcpy(ctr, h, sizeof(h), hipMemcpyHostToDevice));
     while (taskCount > 0) {
         RT_TRY(hipMemsetAsync(ctr + 1, 0, sizeof(int), s));
-        hipLaunchKernelGGL((collapseKernel<W>), dim3(blocksFor(taskCount)), dim3(kB), 0, s, ta, taskCount, dBin, out,
+        hipLaunchKernelGGL((collapseKernel<N>), dim3(blocksFor(taskCount)), dim3(kB), 0, s, ta, taskCount, dBin, out,
                            ctr, tb);
         RT_TRY(hipGetLastError());
         RT_TRY(hipMemcpyAsync(h, ctr, sizeof(h), hipMemcpyDeviceToHost, s));
@@ -469,7 +490,7 @@ hipError_t buildBvhGpu(const Triangle* dTris, int n, const GpuBuildParams& prm, 
     size_t taskCap = 0, nextCap = 0, splitCap = 0, boxCap = 0, binCap = 0, curCap = 0;
     int taskCount = 1, binSlots = 0;
     int h[5] = {1, 0, 0, 0, 0};   // nodes, next tasks, next bin slots, max depth, partition error
-    if (nbins < 2 || nbins > kMaxBins || n < 1) return hipErrorInvalidValue;
+    if (nbins < 2 || nbins > kMaxBins || n < 1 || n >= kLeafFirstLimit) return hipErrorInvalidValue;
     RT_TRY(hipMalloc(&plo, (size_t)n * 16));
     RT_TRY(hipMalloc(&phi, (size_t)n * 16));
     RT_TRY(hipMalloc(&cen, (size_t)n * 16));
@@ -534,13 +555,22 @@ hipError_t buildBvhGpu(const Triangle* dTris, int n, const GpuBuildParams& prm, 
     RT_TRY(hipMemsetAsync(lastFlag, 0, (size_t)n * 4, s));
     hipLaunchKernelGGL(leafMarkKernel, dim3(blocksFor(res.binNodeCount)), dim3(kB), 0, s, nodes, res.binNodeCount, lastFlag);
     RT_TRY(hipGetLastError());
-    RT_TRY(hipMalloc(&res.recs, (size_t)n * sizeof(TriRecord)));
+    RT_TRY(hipMalloc(&res.recs, (size_t)(n + kRecordPad) * sizeof(TriRecord)));
+    RT_TRY(hipMemsetAsync(res.recs + n, 0, kRecordPad * sizeof(TriRecord), s));
     hipLaunchKernelGGL(recordKernel, dim3(blocksFor(n)), dim3(kB), 0, s, dTris, order, lastFlag, res.recs, n);
     RT_TRY(hipGetLastError());
-    switch (prm.width) {
-        case 2: RT_TRY(collapseW<2>(nodes, res.binNodeCount, &res.wideNodes, &res.wideCount, &res.wideDepth, s)); break;
-        case 8: RT_TRY(collapseW<8>(nodes, res.binNodeCount, &res.wideNodes, &res.wideCount, &res.wideDepth, s)); break;
-        default: RT_TRY(collapseW<4>(nodes, res.binNodeCount, &res.wideNodes, &res.wideCount, &res.wideDepth, s)); break;
+    if (prm.quantized) {
+        switch (prm.width) {
+            case 2: RT_TRY(collapseW<QNode<2>>(nodes, res.binNodeCount, &res.wideNodes, &res.wideCount, &res.wideDepth, s)); break;
+            case 8: RT_TRY(collapseW<QNode<8>>(nodes, res.binNodeCount, &res.wideNodes, &res.wideCount, &res.wideDepth, s)); break;
+            default: RT_TRY(collapseW<QNode<4>>(nodes, res.binNodeCount, &res.wideNodes, &res.wideCount, &res.wideDepth, s)); break;
+        }
+    } else {
+        switch (prm.width) {
+            case 2: RT_TRY(collapseW<WideNode<2>>(nodes, res.binNodeCount, &res.wideNodes, &res.wideCount, &res.wideDepth, s)); break;
+            case 8: RT_TRY(collapseW<WideNode<8>>(nodes, res.binNodeCount, &res.wideNodes, &res.wideCount, &res.wideDepth, s)); break;
+            default: RT_TRY(collapseW<WideNode<4>>(nodes, res.binNodeCount, &res.wideNodes, &res.wideCount, &res.wideDepth, s)); break;
+        }
     }
     RT_TRY(hipStreamSynchronize(s));
     res.binNodes = nodes; nodes = nullptr;

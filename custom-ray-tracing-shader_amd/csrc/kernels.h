@@ -16,7 +16,7 @@ struct TraceArgs {
     const int* countPtr;     // device-resident ray count, or null to use count
     int count;
     int* work;               // work cursor, starts at 0
-    const void* nodes;       // WideNode<W>[]
+    const void* nodes;       // WideNode<W>[] or QNode<W>[]
     const TriRecord* recs;
     int4* hits;              // closest / query mode: RtHit per ray
     float4* rad;             // shadow mode: radiance += contrib when unoccluded
@@ -70,10 +70,15 @@ struct DenoiseArgs {
 hipError_t launchDenoisePrepare(const DenoiseArgs& a, hipStream_t s);
 hipError_t launchDenoisePresent(const DenoiseArgs& a, hipStream_t s);
 
-// Persistent-lane launch of the traversal kernel: width W (2, 4 or 8), LDS short-stack
-// entries per lane (8 or 16), per-lane refill on or off.
-hipError_t launchTrace(int width, bool anyHit, bool stats, int blocks, int stack, bool refill, const TraceArgs& a,
-                       hipStream_t s);
+// Variant of the persistent traversal kernel.
+struct TraceLaunch {
+    int width = 4;          // children per node: 2, 4 or 8
+    bool quantized = false; // QNode (8-bit child bounds) instead of WideNode
+    int stack = 8;          // LDS short-stack entries per lane: 8 or 16
+    bool refill = false;    // lanes take new rays while the wave still traces others
+    bool whileWhile = true; // speculative while-while loop (inner-node and leaf phases)
+};
+hipError_t launchTrace(const TraceLaunch& l, bool anyHit, bool stats, int blocks, const TraceArgs& a, hipStream_t s);
 hipError_t launchResetCounters(int* counters, int initialRays, hipStream_t s);
 hipError_t launchRaygen(const FrameArgs& f, hipStream_t s);
 hipError_t launchShade(const FrameArgs& f, int blocks, hipStream_t s);

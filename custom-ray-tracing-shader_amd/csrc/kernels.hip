@@ -20,20 +20,30 @@ static const unsigned kSobolDirsHost[kSobolDims * 32] = {
 const unsigned* hostSobolDirections() { return kSobolDirsHost; }
 
 // ------------------------------------------------------------------ traversal
-template <int W>
-__device__ __forceinline__ void loadNode(const WideNode<W>* nodes, int ref, WideNode<W>& n) {
-    union { float4 v[2 * W]; WideNode<W> node; } u;
+// Leaf records fetched together before testing (one memory round trip per batch instead of
+// one per triangle).
+#ifndef RT_LEAF_BATCH
+#define RT_LEAF_BATCH 2
+#endif
+constexpr int kLeafBatch = RT_LEAF_BATCH;
+
+template <class N>
+__device__ __forceinline__ void loadNode(const N* nodes, int ref, N& n) {
+    constexpr int V = NodeTraits<N>::kVectors;
+    union { float4 v[sizeof(N) / 16]; N node; } u;
     const float4* p = reinterpret_cast<const float4*>(nodes + ref);
 #pragma unroll
-    for (int j = 0; j < 2 * W; j++) u.v[j] = p[j];
+    for (int j = 0; j < V; j++) u.v[j] = p[j];
     n = u.node;
 }
 
 // Per-lane traversal state, kept in registers across refill rounds.
 struct LaneRay {
     f3 o, d, inv;
+    bool nz;        // no zero component in inv: slab tests take boxEntryNZ
     float best, bu, bv;
     int bestIdx, sp, ref;
+    int leaf;       // postponed leaf reference (< 0), or >= 0 for none
     int index;      // queue position of the ray, -1 when the lane is idle
     int slot;       // path slot (shadow mode)
 };
@@ -76,18 +86,24 @@ struct LdsStack {
 // One traversal step: visit an inner node (push far children, descend to the nearest) or
 // test a leaf, then pop until a deferred entry survives the current best distance.
 // Returns true when the ray is finished.
-template <int W, bool ANY, bool STATS, int S>
-__device__ __forceinline__ bool traceStep(LaneRay& L, const WideNode<W>* nodes, const float4* recs, LdsStack<S>& st,
+template <class N, bool ANY, bool STATS, int S>
+__device__ __forceinline__ bool traceStep(LaneRay& L, const N* nodes, const float4* recs, LdsStack<S>& st,
                                           bool& overflow, unsigned long long& nv, unsigned long long& tt) {
+    constexpr int W = NodeTraits<N>::kWidth;
     if (L.ref >= 0) {
-        WideNode<W> n;
-        loadNode<W>(nodes, L.ref, n);
         if (STATS) nv++;
+        // Each path loads the node itself, so no node register lives across both.
         float dist[W]; int cr[W];
+        if (L.nz) {
+            N n;
+            loadNode<N>(nodes, L.ref, n);
 #pragma unroll
-        for (int c = 0; c < W; c++) {
-            dist[c] = boxEntry(L.o, L.inv, n.lox[c], n.loy[c], n.loz[c], n.hix[c], n.hiy[c], n.hiz[c], L.best);
-            cr[c] = n.child[c];
+            for (int c = 0; c < W; c++) { dist[c] = n.entryNZ(c, L.o, L.inv, L.best); cr[c] = n.child[c]; }
+        } else {
+            N n;
+            loadNode<N>(nodes, L.ref, n);
+#pragma unroll
+            for (int c = 0; c < W; c++) { dist[c] = n.entry(c, L.o, L.inv, L.best); cr[c] = n.child[c]; }
         }
         sortChildren<W>(dist, cr);
 #pragma unroll
@@ -95,18 +111,38 @@ __device__ __forceinline__ bool traceStep(LaneRay& L, const WideNode<W>* nodes, 
             if (dist[c] != INFINITY) st.push(L.sp, cr[c], dist[c], overflow);
         if (dist[0] != INFINITY) { L.ref = cr[0]; return false; }
     } else {
-        for (int k = ~L.ref;; k++) {
-            const float4 A = recs[3 * k], B = recs[3 * k + 1], C = recs[3 * k + 2];
-            const int flags = __float_as_int(B.w), index = __float_as_int(A.w);
-            float u = 0, v = 0;
-            const float t = intersectTri(L.o, L.d, F3(A.x, A.y, A.z), F3(B.x, B.y, B.z), F3(C.x, C.y, C.z),
-                                         (flags & kTriOneSided) != 0, u, v);
-            if (STATS) tt++;
-            if (t > 0 && (t < L.best || (t == L.best && index < L.bestIdx))) {
-                L.best = t; L.bestIdx = index; L.bu = u; L.bv = v;
-                if (ANY) return true;
+        // Records k.. of the leaf, kLeafBatch at a time; the reference count bounds the first
+        // batches exactly, larger leaves continue until the last-in-leaf flag.
+        int k = leafFirst(L.ref), known = leafBatch(L.ref);
+        for (;;) {
+            const int m = known < kLeafBatch ? known : kLeafBatch;
+            // Unconditional loads (slots past m repeat the last record) so the whole batch is
+            // in flight at once.
+            float4 A[kLeafBatch], B[kLeafBatch], C[kLeafBatch];
+#pragma unroll
+            for (int j = 0; j < kLeafBatch; j++) {
+                const int r = 3 * (k + (j < m ? j : m - 1));
+                A[j] = recs[r]; B[j] = recs[r + 1]; C[j] = recs[r + 2];
             }
-            if (flags & kTriLastInLeaf) break;
+            bool last = false;
+#pragma unroll
+            for (int j = 0; j < kLeafBatch; j++) {
+                if (j < m && !last) {
+                    const int flags = __float_as_int(B[j].w), index = __float_as_int(A[j].w);
+                    float u = 0, v = 0;
+                    const float t = intersectTri(L.o, L.d, F3(A[j].x, A[j].y, A[j].z), F3(B[j].x, B[j].y, B[j].z),
+                                                 F3(C[j].x, C[j].y, C[j].z), (flags & kTriOneSided) != 0, u, v);
+                    if (STATS) tt++;
+                    if (t > 0 && (t < L.best || (t == L.best && index < L.bestIdx))) {
+                        L.best = t; L.bestIdx = index; L.bu = u; L.bv = v;
+                        if (ANY) return true;
+                    }
+                    last = (flags & kTriLastInLeaf) != 0;
+                }
+            }
+            if (last) break;
+            k += m;
+            known = known > m ? known - m : kLeafBatch;
         }
     }
     while (L.sp > 0) {
@@ -115,6 +151,80 @@ __device__ __forceinline__ bool traceStep(LaneRay& L, const WideNode<W>* nodes, 
         if (dd <= L.best) { L.ref = r; return false; }
     }
     return true;
+}
+
+// ---- speculative while-while traversal (inner-node and leaf phases run separately, so the
+// lanes of a wave execute the same phase; a lane that reaches its first leaf postpones it and
+// keeps descending until every lane of the wave holds a leaf)
+constexpr int kDone = INT32_MAX;   // reference after the last deferred entry
+
+// Next deferred entry that can still hold a hit nearer than best, or kDone.
+template <int S>
+__device__ __forceinline__ int popNext(LaneRay& L, LdsStack<S>& st) {
+    while (L.sp > 0) {
+        int r; float dd;
+        st.pop(L.sp, r, dd);
+        if (dd <= L.best) return r;
+    }
+    return kDone;
+}
+
+template <class N, bool STATS, int S>
+__device__ __forceinline__ void visitNode(LaneRay& L, const N* nodes, LdsStack<S>& st, bool& overflow,
+                                          unsigned long long& nv) {
+    constexpr int W = NodeTraits<N>::kWidth;
+    if (STATS) nv++;
+    float dist[W]; int cr[W];
+    N n;
+    loadNode<N>(nodes, L.ref, n);
+    if (L.nz) {
+#pragma unroll
+        for (int c = 0; c < W; c++) dist[c] = n.entryNZ(c, L.o, L.inv, L.best);
+    } else {
+#pragma unroll
+        for (int c = 0; c < W; c++) dist[c] = n.entry(c, L.o, L.inv, L.best);
+    }
+#pragma unroll
+    for (int c = 0; c < W; c++) cr[c] = n.child[c];
+    sortChildren<W>(dist, cr);
+#pragma unroll
+    for (int c = W - 1; c >= 1; c--)
+        if (dist[c] != INFINITY) st.push(L.sp, cr[c], dist[c], overflow);
+    L.ref = dist[0] != INFINITY ? cr[0] : popNext<S>(L, st);
+}
+
+// Tests the triangles of leaf reference ref; true when an any-hit query is answered.
+template <bool ANY, bool STATS>
+__device__ __forceinline__ bool testLeaf(LaneRay& L, int ref, const float4* recs, unsigned long long& tt) {
+    int k = leafFirst(ref), known = leafBatch(ref);
+    for (;;) {
+        const int m = known < kLeafBatch ? known : kLeafBatch;
+        float4 A[kLeafBatch], B[kLeafBatch], C[kLeafBatch];
+#pragma unroll
+        for (int j = 0; j < kLeafBatch; j++) {
+            const int r = 3 * (k + (j < m ? j : m - 1));
+            A[j] = recs[r]; B[j] = recs[r + 1]; C[j] = recs[r + 2];
+        }
+        bool last = false;
+#pragma unroll
+        for (int j = 0; j < kLeafBatch; j++) {
+            if (j < m && !last) {
+                const int flags = __float_as_int(B[j].w), index = __float_as_int(A[j].w);
+                float u = 0, v = 0;
+                const float t = intersectTri(L.o, L.d, F3(A[j].x, A[j].y, A[j].z), F3(B[j].x, B[j].y, B[j].z),
+                                             F3(C[j].x, C[j].y, C[j].z), (flags & kTriOneSided) != 0, u, v);
+                if (STATS) tt++;
+                if (t > 0 && (t < L.best || (t == L.best && index < L.bestIdx))) {
+                    L.best = t; L.bestIdx = index; L.bu = u; L.bv = v;
+                    if (ANY) return true;
+                }
+                last = (flags & kTriLastInLeaf) != 0;
+            }
+        }
+        if (last) return false;
+        k += m;
+        known = known > m ? known - m : kLeafBatch;
+    }
 }
 
 template <bool ANY>
@@ -162,23 +272,31 @@ __device__ __forceinline__ void startRay(const TraceArgs& a, int i, LaneRay& L) 
     L.o = F3(r0.x, r0.y, r0.z);
     L.d = F3(r1.x, r1.y, r1.z);
     L.inv = safeInverse(L.d);
+    L.nz = L.inv.x != 0 && L.inv.y != 0 && L.inv.z != 0;
     L.best = r0.w; L.bu = 0; L.bv = 0;
     L.bestIdx = -1; L.sp = 0;
     L.index = i;
     L.slot = __float_as_int(r1.w);
     L.ref = a.roots ? a.roots[findView(a.viewBase, a.views, L.slot)] : 0;
+    L.leaf = 0;
 }
 
 // Persistent traversal.  Each wave takes rays from the shared cursor; with REFILL, lanes
 // whose ray finished take new rays as soon as a quarter of the wave is idle, instead of
 // waiting for the wave's slowest ray.
-template <int W, bool ANY, bool STATS, int S, bool REFILL>
-__global__ __launch_bounds__(kBlock) void traceKernel(TraceArgs a) {
+#ifndef RT_FETCH_THRESHOLD
+#define RT_FETCH_THRESHOLD 40
+#endif
+#ifndef RT_TRACE_MIN_WAVES
+#define RT_TRACE_MIN_WAVES 1
+#endif
+template <class N, bool ANY, bool STATS, int S, bool REFILL, bool WW>
+__global__ __launch_bounds__(kBlock, RT_TRACE_MIN_WAVES) void traceKernel(TraceArgs a) {
     __shared__ int sRef[S * kBlock];
     __shared__ float sDist[S * kBlock];
     const int tid = threadIdx.x, lane = tid & 63;
     const int total = a.countPtr ? *a.countPtr : a.count;
-    const WideNode<W>* nodes = static_cast<const WideNode<W>*>(a.nodes);
+    const N* nodes = static_cast<const N*>(a.nodes);
     const float4* recs = reinterpret_cast<const float4*>(a.recs);
     LdsStack<S> st{sRef, sDist, reinterpret_cast<int2*>(a.spill), a.spillEntries, a.totalLanes,
                    (int)(blockIdx.x * kBlock + tid), tid};
@@ -202,14 +320,37 @@ __global__ __launch_bounds__(kBlock) void traceKernel(TraceArgs a) {
             exhausted = base + __popcll(idle) >= total;
         }
         if (__ballot(L.index >= 0) == 0) break;
-        if (L.index >= 0) {
+        if (WW && L.index >= 0) {
             for (;;) {
-                if (traceStep<W, ANY, STATS, S>(L, nodes, recs, st, overflow, nv, tt)) {
+                bool searching = true;
+                while ((unsigned)L.ref < (unsigned)kDone) {
+                    visitNode<N, STATS, S>(L, nodes, st, overflow, nv);
+                    if (L.ref < 0 && L.leaf >= 0) { searching = false; L.leaf = L.ref; L.ref = popNext<S>(L, st); }
+                    if (!__any(searching)) break;
+                }
+                bool answered = false;
+                while (L.leaf < 0) {
+                    if (testLeaf<ANY, STATS>(L, L.leaf, recs, tt)) { answered = true; break; }
+                    L.leaf = L.ref;
+                    if (L.ref < 0) L.ref = popNext<S>(L, st);
+                }
+                if (answered) { L.ref = kDone; L.leaf = 0; }
+                if (L.ref == kDone && L.leaf >= 0) {
                     finishRay<ANY>(a, L);
                     L.index = -1;
                     break;
                 }
-                if (REFILL && !exhausted && __popcll(__ballot(true)) <= 48) break;
+                if (REFILL && !exhausted && __popcll(__ballot(true)) < RT_FETCH_THRESHOLD) break;
+            }
+        }
+        if (!WW && L.index >= 0) {
+            for (;;) {
+                if (traceStep<N, ANY, STATS, S>(L, nodes, recs, st, overflow, nv, tt)) {
+                    finishRay<ANY>(a, L);
+                    L.index = -1;
+                    break;
+                }
+                if (REFILL && !exhausted && __popcll(__ballot(true)) < RT_FETCH_THRESHOLD) break;
             }
         }
     }
@@ -220,37 +361,47 @@ __global__ __launch_bounds__(kBlock) void traceKernel(TraceArgs a) {
     if (overflow) atomicOr(a.overflow, 1);
 }
 
-template <int W, int S, bool R>
-static void launchTraceWS(bool anyHit, bool stats, int blocks, const TraceArgs& a, hipStream_t s) {
+template <class N, int S, bool R, bool WW>
+static void launchTraceNS(bool anyHit, bool stats, int blocks, const TraceArgs& a, hipStream_t s) {
     dim3 grid(blocks), block(kBlock);
     if (anyHit) {
-        if (stats) hipLaunchKernelGGL((traceKernel<W, true, true, S, R>), grid, block, 0, s, a);
-        else hipLaunchKernelGGL((traceKernel<W, true, false, S, R>), grid, block, 0, s, a);
+        if (stats) hipLaunchKernelGGL((traceKernel<N, true, true, S, R, WW>), grid, block, 0, s, a);
+        else hipLaunchKernelGGL((traceKernel<N, true, false, S, R, WW>), grid, block, 0, s, a);
     } else {
-        if (stats) hipLaunchKernelGGL((traceKernel<W, false, true, S, R>), grid, block, 0, s, a);
-        else hipLaunchKernelGGL((traceKernel<W, false, false, S, R>), grid, block, 0, s, a);
+        if (stats) hipLaunchKernelGGL((traceKernel<N, false, true, S, R, WW>), grid, block, 0, s, a);
+        else hipLaunchKernelGGL((traceKernel<N, false, false, S, R, WW>), grid, block, 0, s, a);
     }
 }
-template <int W>
-static hipError_t launchTraceW(bool anyHit, bool stats, int blocks, int stack, bool refill, const TraceArgs& a,
+template <class N>
+static hipError_t launchTraceN(bool anyHit, bool stats, int blocks, const TraceLaunch& l, const TraceArgs& a,
                                hipStream_t s) {
-    if (stack == 8) {
-        if (refill) launchTraceWS<W, 8, true>(anyHit, stats, blocks, a, s);
-        else launchTraceWS<W, 8, false>(anyHit, stats, blocks, a, s);
-    } else if (stack == 16) {
-        if (refill) launchTraceWS<W, 16, true>(anyHit, stats, blocks, a, s);
-        else launchTraceWS<W, 16, false>(anyHit, stats, blocks, a, s);
-    } else {
-        return hipErrorInvalidValue;
+    const int key = (l.stack == 16 ? 4 : l.stack == 8 ? 0 : -100) + (l.refill ? 2 : 0) + (l.whileWhile ? 1 : 0);
+    switch (key) {
+        case 0: launchTraceNS<N, 8, false, false>(anyHit, stats, blocks, a, s); break;
+        case 1: launchTraceNS<N, 8, false, true>(anyHit, stats, blocks, a, s); break;
+        case 2: launchTraceNS<N, 8, true, false>(anyHit, stats, blocks, a, s); break;
+        case 3: launchTraceNS<N, 8, true, true>(anyHit, stats, blocks, a, s); break;
+        case 4: launchTraceNS<N, 16, false, false>(anyHit, stats, blocks, a, s); break;
+        case 5: launchTraceNS<N, 16, false, true>(anyHit, stats, blocks, a, s); break;
+        case 6: launchTraceNS<N, 16, true, false>(anyHit, stats, blocks, a, s); break;
+        case 7: launchTraceNS<N, 16, true, true>(anyHit, stats, blocks, a, s); break;
+        default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
-hipError_t launchTrace(int width, bool anyHit, bool stats, int blocks, int stack, bool refill, const TraceArgs& a,
-                       hipStream_t s) {
-    switch (width) {
-        case 2: return launchTraceW<2>(anyHit, stats, blocks, stack, refill, a, s);
-        case 4: return launchTraceW<4>(anyHit, stats, blocks, stack, refill, a, s);
-        case 8: return launchTraceW<8>(anyHit, stats, blocks, stack, refill, a, s);
+hipError_t launchTrace(const TraceLaunch& l, bool anyHit, bool stats, int blocks, const TraceArgs& a, hipStream_t s) {
+    if (l.quantized) {
+        switch (l.width) {
+            case 2: return launchTraceN<QNode<2>>(anyHit, stats, blocks, l, a, s);
+            case 4: return launchTraceN<QNode<4>>(anyHit, stats, blocks, l, a, s);
+            case 8: return launchTraceN<QNode<8>>(anyHit, stats, blocks, l, a, s);
+            default: return hipErrorInvalidValue;
+        }
+    }
+    switch (l.width) {
+        case 2: return launchTraceN<WideNode<2>>(anyHit, stats, blocks, l, a, s);
+        case 4: return launchTraceN<WideNode<4>>(anyHit, stats, blocks, l, a, s);
+        case 8: return launchTraceN<WideNode<8>>(anyHit, stats, blocks, l, a, s);
         default: return hipErrorInvalidValue;
     }
 }

@@ -96,19 +96,117 @@ RT_HD float boxEntry(f3 o, f3 inv, float lx, float ly, float lz, float hx, float
     float farT = fminf(fminf(bx, by), fminf(bz, limit));
     return nearT <= farT ? nearT : INFINITY;
 }
+// boxEntry for a ray whose inverse direction has no zero component: the slab selects drop
+// out and the remaining operations are exactly boxEntry's.
+RT_HD float boxEntryNZ(f3 o, f3 inv, float lx, float ly, float lz, float hx, float hy, float hz, float limit) {
+    const float ax = (lx - o.x) * inv.x, bx = (hx - o.x) * inv.x;
+    const float ay = (ly - o.y) * inv.y, by = (hy - o.y) * inv.y;
+    const float az = (lz - o.z) * inv.z, bz = (hz - o.z) * inv.z;
+    float nearT = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fmaxf(fminf(az, bz), 0.0f));
+    float farT = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fminf(fmaxf(az, bz), limit));
+    return nearT <= farT ? nearT : INFINITY;
+}
 
 // ------------------------------------------------------------------ wide BVH layout
-// A W-wide node is W*32 bytes (64 B for W=2), 64-byte aligned: six W-float planes of child
-// bounds, then W child references and W triangle counts.  Child reference >= 0 is an inner
-// node index; < 0 is a leaf whose first triangle record is ~ref; kEmptyChild is an unused
-// slot (its bounds are +inf so every slab test misses).
+// Child references: >= 0 is an inner node index; < 0 is a leaf, ~ref = first record << 3 |
+// code, where code = count - 1 for leaves of 1..7 triangles and 7 for larger ones (whose end
+// is found from the last-in-leaf record flag).  kEmptyChild marks an unused slot.
 constexpr int kEmptyChild = INT32_MIN;
+constexpr int kLeafFirstLimit = 1 << 28;   // records addressable by a leaf reference
+RT_HD int makeLeafRef(int first, int count) { return ~((first << 3) | (count < 8 ? count - 1 : 7)); }
+RT_HD int leafFirst(int ref) { return (~ref) >> 3; }
+// Records certainly in the leaf (8 means "8 or more").
+RT_HD int leafBatch(int ref) { return ((~ref) & 7) + 1; }
+
+RT_HD float bitsToFloat(unsigned u) { union { unsigned u; float f; } c; c.u = u; return c.f; }
+
+// Full-precision W-wide node, W*32 bytes (64 B for W=2), 64-byte aligned: six W-float planes
+// of child bounds, then W child references and W triangle counts.  Unused slots have +inf
+// bounds, so every slab test misses them.
 template <int W> struct alignas(64) WideNode {
     float lox[W], hix[W], loy[W], hiy[W], loz[W], hiz[W];
     int child[W];
     int count[W];
+    RT_HD float entry(int c, f3 o, f3 inv, float limit) const {
+        return boxEntry(o, inv, lox[c], loy[c], loz[c], hix[c], hiy[c], hiz[c], limit);
+    }
+    RT_HD float entryNZ(int c, f3 o, f3 inv, float limit) const {
+        return boxEntryNZ(o, inv, lox[c], loy[c], loz[c], hix[c], hiy[c], hiz[c], limit);
+    }
 };
 static_assert(sizeof(WideNode<2>) == 64 && sizeof(WideNode<4>) == 128 && sizeof(WideNode<8>) == 256, "node sizes");
+
+// Quantized W-wide node: child bounds as 8-bit offsets from the node's own box corner in
+// per-axis power-of-two steps, rounded outwards, so every decoded child box contains the
+// exact one (the build checks each with the decode arithmetic below).  A looser box can only
+// add node visits, never drop a triangle whose hit could win, so hit records are unchanged.
+// 56 B (W=4) or 96 B (W=8) are read per visit instead of 128 B or 256 B.
+template <int W> struct alignas(64) QNode {
+    float px, py, pz;        // corner: minimum of the child boxes
+    unsigned steps;          // biased FP32 exponents of the x, y, z steps in bytes 0, 1, 2
+    int child[W];
+    unsigned char qlox[W], qhix[W], qloy[W], qhiy[W], qloz[W], qhiz[W];
+    RT_HD float step(int axis) const { return bitsToFloat(((steps >> (8 * axis)) & 255u) << 23); }
+    RT_HD float entry(int c, f3 o, f3 inv, float limit) const {
+        if (child[c] == kEmptyChild) return INFINITY;
+        const float sx = step(0), sy = step(1), sz = step(2);
+        return boxEntry(o, inv, px + (float)qlox[c] * sx, py + (float)qloy[c] * sy, pz + (float)qloz[c] * sz,
+                        px + (float)qhix[c] * sx, py + (float)qhiy[c] * sy, pz + (float)qhiz[c] * sz, limit);
+    }
+    RT_HD float entryNZ(int c, f3 o, f3 inv, float limit) const {
+        if (child[c] == kEmptyChild) return INFINITY;
+        const float sx = step(0), sy = step(1), sz = step(2);
+        return boxEntryNZ(o, inv, px + (float)qlox[c] * sx, py + (float)qloy[c] * sy, pz + (float)qloz[c] * sz,
+                          px + (float)qhix[c] * sx, py + (float)qhiy[c] * sy, pz + (float)qhiz[c] * sz, limit);
+    }
+};
+static_assert(sizeof(QNode<2>) == 64 && sizeof(QNode<4>) == 64 && sizeof(QNode<8>) == 128, "quantized node sizes");
+
+// Bytes of a node a traversal step reads (16-byte vectors; unused tail padding skipped).
+template <class N> struct NodeTraits;
+template <int W> struct NodeTraits<WideNode<W>> { static constexpr int kWidth = W, kVectors = 2 * W; };
+template <int W> struct NodeTraits<QNode<W>> { static constexpr int kWidth = W, kVectors = (16 + 10 * W + 15) / 16; };
+
+// Quantizes child boxes lo/hi[c][axis] (c < nk) and references into a QNode.  Exponents
+// are clamped to [27, 254] (steps 2^-100 .. 2^127), where q * step is exact for q <= 255.
+RT_HD float quantDecode(float corner, float step, int q) { return corner + (float)q * step; }
+template <int W>
+RT_HD void quantizeNode(QNode<W>& n, const float (*lo)[3], const float (*hi)[3], const int* refs, int nk) {
+    float corner[3], step[3];
+    unsigned steps = 0;
+    for (int a = 0; a < 3; a++) {
+        float l = INFINITY, h = -INFINITY;
+        for (int c = 0; c < nk; c++) { l = fminf(l, lo[c][a]); h = fmaxf(h, hi[c][a]); }
+        if (!(l <= h)) { l = 0; h = 0; }
+        const float ext = (h - l) / 255.0f;
+        union { float f; unsigned u; } e; e.f = ext;
+        int be = (int)((e.u >> 23) & 255u) + 1;
+        be = be < 27 ? 27 : (be > 254 ? 254 : be);
+        while (be > 27 && quantDecode(l, bitsToFloat((unsigned)(be - 1) << 23), 255) >= h) be--;
+        while (be < 254 && !(quantDecode(l, bitsToFloat((unsigned)be << 23), 255) >= h)) be++;
+        corner[a] = l;
+        step[a] = bitsToFloat((unsigned)be << 23);
+        steps |= (unsigned)be << (8 * a);
+    }
+    n.px = corner[0]; n.py = corner[1]; n.pz = corner[2]; n.steps = steps;
+    unsigned char* qlo[3] = {n.qlox, n.qloy, n.qloz};
+    unsigned char* qhi[3] = {n.qhix, n.qhiy, n.qhiz};
+    for (int c = 0; c < W; c++) {
+        n.child[c] = c < nk ? refs[c] : kEmptyChild;
+        for (int a = 0; a < 3; a++) {
+            if (c >= nk) { qlo[a][c] = 255; qhi[a][c] = 0; continue; }
+            const float l = corner[a], s = step[a];
+            int ql = (int)fminf(fmaxf(floorf((lo[c][a] - l) / s), 0.0f), 255.0f);
+            while (ql > 0 && quantDecode(l, s, ql) > lo[c][a]) ql--;
+            while (ql < 255 && quantDecode(l, s, ql + 1) <= lo[c][a]) ql++;
+            int qh = (int)fminf(fmaxf(ceilf((hi[c][a] - l) / s), 0.0f), 255.0f);
+            while (qh < 255 && quantDecode(l, s, qh) < hi[c][a]) qh++;
+            while (qh > 0 && quantDecode(l, s, qh - 1) >= hi[c][a]) qh--;
+            qlo[a][c] = (unsigned char)ql;
+            qhi[a][c] = (unsigned char)qh;
+        }
+    }
+}
 
 // Intersection record, 48 bytes: origin/edge1/edge2 exactly as the caller passed them (so the
 // Moller-Trumbore arithmetic is the reference's), original triangle index and flags.

@@ -1,4 +1,5 @@
-// traverse.h - W-wide BVH traversal over 64-byte aligned nodes, for the gfx950 kernels
+// traverse.h - W-wide BVH traversal over 64-byte aligned nodes (full precision or
+// quantized), for the gfx950 kernels
 // (LDS short stack with a global spill ring) and for the host backends (array stack).
 //
 // Result semantics follow Native/renderer.cu:136-178: the accepted hit is the smallest t in
@@ -46,8 +47,10 @@ RT_HD bool leafTest(const TriRecord* recs, int first, f3 o, f3 d, float& best, i
 }
 
 // Host traversal with a plain array stack; the same visiting rules as the device kernel.
-template <int W, bool ANY>
-inline RtHit traverseHost(const WideNode<W>* nodes, const TriRecord* recs, f3 o, f3 d, float limit) {
+// N is WideNode<W> or QNode<W>.
+template <class N, bool ANY>
+inline RtHit traverseHost(const N* nodes, const TriRecord* recs, f3 o, f3 d, float limit) {
+    constexpr int W = NodeTraits<N>::kWidth;
     f3 inv = safeInverse(d);
     float best = limit, bu = 0, bv = 0;
     int bestIdx = -1;
@@ -57,10 +60,10 @@ inline RtHit traverseHost(const WideNode<W>* nodes, const TriRecord* recs, f3 o,
     unsigned long long tt = 0;
     for (;;) {
         if (ref >= 0) {
-            const WideNode<W>& n = nodes[ref];
+            const N& n = nodes[ref];
             float dist[W]; int cr[W];
             for (int c = 0; c < W; c++) {
-                dist[c] = boxEntry(o, inv, n.lox[c], n.loy[c], n.loz[c], n.hix[c], n.hiy[c], n.hiz[c], best);
+                dist[c] = n.entry(c, o, inv, best);
                 cr[c] = n.child[c];
             }
             sortChildren<W>(dist, cr);
@@ -68,7 +71,7 @@ inline RtHit traverseHost(const WideNode<W>* nodes, const TriRecord* recs, f3 o,
                 if (dist[c] != INFINITY) { sref[sp] = cr[c]; sdist[sp] = dist[c]; sp++; }
             if (dist[0] != INFINITY) { ref = cr[0]; continue; }
         } else {
-            if (leafTest<ANY>(recs, ~ref, o, d, best, bestIdx, bu, bv, tt)) break;
+            if (leafTest<ANY>(recs, leafFirst(ref), o, d, best, bestIdx, bu, bv, tt)) break;
         }
         bool found = false;
         while (sp > 0) {

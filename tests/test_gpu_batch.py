@@ -23,12 +23,11 @@ def views():
     return out
 
 
-@pytest.mark.parametrize("use_graph", [1, 0])
-@pytest.mark.parametrize("width", [4, 8])
-def test_batch_views_equal_single_sessions(lib, use_graph, width):
+@pytest.mark.parametrize("use_graph,width,quantized", [(1, 4, 0), (0, 4, 0), (1, 8, 0), (0, 8, 0), (1, 4, 1), (1, 8, 1)])
+def test_batch_views_equal_single_sessions(lib, use_graph, width, quantized):
     vs = views()
     frames = 3
-    with Batch(lib, use_graph=use_graph, bvh_width=width) as b:
+    with Batch(lib, use_graph=use_graph, bvh_width=width, bvh_quantized=quantized) as b:
         for t, w, h, cam, st in vs:
             b.add_view(t, w, h, cam, st)
         b.build()
@@ -39,6 +38,7 @@ def test_batch_views_equal_single_sessions(lib, use_graph, width):
     for v, (t, w, h, cam, st) in enumerate(vs):
         with Session(lib) as s:
             s.set_option("bvh_width", width)
+            s.set_option("bvh_quantized", quantized)
             s.initialize(t, w, h)
             st.raysPerPixel = 1
             for _ in range(frames):

@@ -218,3 +218,30 @@ def test_resident_render_matches_rtrender(lib):
         ms = b.render_resident(cam, st)
         assert ms > 0 and np.array_equal(b.copy_output(), img)
         assert lib.RtSampleCount(b.handle) == 4
+
+
+@pytest.mark.parametrize("width", [4, 8])
+def test_quantized_nodes_leave_images_unchanged(lib, width):
+    """Quantized child boxes only enlarge boxes, so every path makes the same hits: the
+    accumulation, the image and the ray counts equal those of full-precision nodes bit for
+    bit, on the GPU and on the CPU backend (which traverses the same node array)."""
+    from scenes import displaced_icosphere, icosphere_camera, make_settings
+    tris = displaced_icosphere(subdiv=5, seed=3)
+    cam, st = icosphere_camera(80, 60), make_settings(8, rpp=2, seed=5)
+    out = {}
+    for q in (0, 1):
+        with Session(lib) as s:
+            s.set_option("bvh_width", width)
+            s.set_option("bvh_quantized", q)
+            s.initialize(tris, 80, 60)
+            assert lib.RtGetOption(s.handle, b"bvh_quantized") == q
+            for _ in range(2):
+                img, _ = s.render(cam, st, 2)
+            acc = s.accumulation()
+            counts = s.ray_counts()
+            cpu, _ = s.render(cam, st, 1)
+            out[q] = (img, acc, counts, cpu)
+    assert np.array_equal(out[0][0], out[1][0])
+    assert np.array_equal(out[0][1].view(np.uint32), out[1][1].view(np.uint32))
+    assert out[0][2] == out[1][2]
+    assert np.array_equal(out[0][3], out[1][3])

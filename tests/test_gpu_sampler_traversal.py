@@ -41,13 +41,15 @@ def compare_hits(gpu, ref, any_hit):
         assert np.array_equal(gpu[k][hit].view(np.uint32), ref[k][hit].view(np.uint32)), k
 
 
+@pytest.mark.parametrize("quantized", [0, 1])
 @pytest.mark.parametrize("width", [2, 4, 8])
 @pytest.mark.parametrize("name", ["edges", "scattered", "overlap", "unbalanced"])
-def test_verify_bvh_scenes(lib, vbvh, width, name):
+def test_verify_bvh_scenes(lib, vbvh, width, name, quantized):
     rays, scenes = vbvh
     tris = scenes[name]
     with Session(lib) as s:
         assert s.set_option("bvh_width", width)
+        assert s.set_option("bvh_quantized", quantized)
         s.initialize(tris, 1, 1)
         nodes = s.bvh()
         check_layout(nodes, s.bvh_order(len(tris)), tris)
@@ -65,13 +67,15 @@ def soup_rays(n, seed):
     return rays
 
 
+@pytest.mark.parametrize("quantized", [0, 1])
 @pytest.mark.parametrize("width", [2, 4, 8])
-def test_soup_queries_vs_oracle_and_fp64(lib, width):
+def test_soup_queries_vs_oracle_and_fp64(lib, width, quantized):
     from scenes import triangle_soup
     tris = triangle_soup(20000, seed=11)
     rays = soup_rays(20000, 5)
     with Session(lib) as s:
         s.set_option("bvh_width", width)
+        s.set_option("bvh_quantized", quantized)
         s.initialize(tris, 1, 1)
         ref_bvh = oracle.Bvh(tris, bins=12)
         for any_hit in (False, True):
@@ -87,6 +91,41 @@ def test_soup_queries_vs_oracle_and_fp64(lib, width):
         # Index disagreements with FP64 only where FP64 distances are within FP32 resolution.
         diff = (gpu["index"] != idx64) & (gpu["index"] >= 0) & (idx64 >= 0)
         assert diff.mean() < 1e-3
+
+
+@pytest.mark.parametrize("stack", [8, 16])
+@pytest.mark.parametrize("while_while,refill", [(0, 0), (0, 1), (1, 0), (1, 1)])
+def test_traversal_loop_variants_match_brute_force(lib, vbvh, while_while, refill, stack):
+    """Every persistent-loop variant (step loop or speculative while-while, with or without
+    per-lane refill, 8- or 16-entry LDS stack) returns the brute-force hit records."""
+    from scenes import triangle_soup
+    rays, scenes = vbvh
+    cases = [(scenes["overlap"], rays), (scenes["unbalanced"], rays), (triangle_soup(5000, seed=3), soup_rays(6000, 2))]
+    for tris, r in cases:
+        with Session(lib) as s:
+            for k, v in (("while_while", while_while), ("refill", refill), ("short_stack", stack)):
+                assert s.set_option(k, v)
+            s.initialize(tris, 1, 1)
+            for any_hit in (False, True):
+                compare_hits(s.trace(r, any_hit), oracle.brute32(tris, r, any_hit), any_hit)
+
+
+@pytest.mark.parametrize("quantized", [0, 1])
+@pytest.mark.parametrize("max_leaf", [7, 16])
+def test_large_leaves_match_brute_force(lib, max_leaf, quantized):
+    """Leaves of 5..16 triangles: records fetched in batches past the count held in the
+    leaf reference (7 = "8 or more") still stop exactly at the last-in-leaf flag."""
+    from scenes import triangle_soup
+    tris = triangle_soup(3000, seed=4)
+    with Session(lib) as s:
+        assert s.set_option("build_max_leaf", max_leaf)
+        assert s.set_option("bvh_quantized", quantized)
+        s.initialize(tris, 1, 1)
+        sizes = s.bvh()["count"]
+        assert sizes.max() > 8 or max_leaf < 8
+        r = soup_rays(4000, 9)
+        for any_hit in (False, True):
+            compare_hits(s.trace(r, any_hit), oracle.brute32(tris, r, any_hit), any_hit)
 
 
 def test_gpu_bvh_sah_cost_vs_reference_build(lib):
