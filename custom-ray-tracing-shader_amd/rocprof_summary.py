@@ -6,6 +6,8 @@ Inputs (directories written by rocprofv3 -d ... --output-format csv):
   --fetch DIR        --pmc FETCH_SIZE run of the workload   (*_counter_collection.csv)
   --write DIR        --pmc WRITE_SIZE run of the workload
   --tcc DIR          --pmc TCC_HIT_sum TCC_MISS_sum run     (optional)
+  --sq DIR           --pmc SQ_WAVES SQ_INSTS_VALU SQ_INSTS_VMEM_RD SQ_INSTS_LDS SQ_ACTIVE_INST_VALU
+                     SQ_THREAD_CYCLES_VALU run (optional): instruction mix and VALU lane use
   --copy-fetch DIR / --copy-write DIR   the same counters over bench.py --copy-probe BYTES
   --copy-bytes BYTES                    size the probe copied (reads BYTES, writes BYTES)
 FETCH_SIZE and WRITE_SIZE are in KiB per dispatch; each is multiplied by
@@ -45,6 +47,7 @@ def main():
     ap.add_argument("--fetch", required=True)
     ap.add_argument("--write", required=True)
     ap.add_argument("--tcc")
+    ap.add_argument("--sq")
     ap.add_argument("--copy-fetch", required=True)
     ap.add_argument("--copy-write", required=True)
     ap.add_argument("--copy-bytes", type=int, required=True)
@@ -96,6 +99,18 @@ def main():
     lines += ["", f"Dominant kernel `{short(dominant['kernel'], 120)}`: mean {dominant['mean_ms'] * 1e3:.1f} us over "
               f"{dominant['calls']} launches; HBM traffic {traffic / 1e6:.1f} MB per launch "
               f"({traffic / dominant['mean_ms'] / 1e6:.0f} GB/s of HBM)."]
+    if args.sq:
+        sq = counter_means(args.sq).get(dominant["kernel"])
+        if sq:
+            waves = max(sq["SQ_WAVES"], 1)
+            util = sq["SQ_THREAD_CYCLES_VALU"] / max(sq["SQ_ACTIVE_INST_VALU"] * 64, 1)
+            dominant.update({"valu_insts_per_wave": sq["SQ_INSTS_VALU"] / waves,
+                             "vmem_reads_per_wave": sq["SQ_INSTS_VMEM_RD"] / waves,
+                             "lds_insts_per_wave": sq["SQ_INSTS_LDS"] / waves,
+                             "valu_lane_utilization": util})
+            lines += [f"Instruction mix per wave: {sq['SQ_INSTS_VALU'] / waves:.0f} VALU, "
+                      f"{sq['SQ_INSTS_VMEM_RD'] / waves:.0f} vector-memory reads, {sq['SQ_INSTS_LDS'] / waves:.0f} LDS; "
+                      f"VALU lane utilization (SQ_THREAD_CYCLES_VALU / (64 x SQ_ACTIVE_INST_VALU)) {100 * util:.1f}%."]
     if args.bench_json:
         b = json.loads(open(args.bench_json).read().strip().splitlines()[-1])
         rf = b["roofline"]
