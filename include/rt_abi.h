@@ -116,13 +116,16 @@ int RtBvhExport(void* session, RtBvhNode* nodes, int nodeCapacity, int* order, i
 int RtReadAccumulation(void* session, float* out, long long floatCount);
 /* Ray queries issued by the last RtRender (closest-hit and shadow). */
 int RtRayCounts(void* session, unsigned long long* closest, unsigned long long* shadow);
-/* Tuning knobs: "bvh_width" (2, 4 or 8), "build_bins" (SAH bins per axis of the GPU
- * build, 2..32, default 32) and "build_max_leaf" (leaf size limit, 1..16, default 4 as the
- * reference builder) take effect at the next RtInitialize; "blocks_per_cu"
- * (persistent traversal workgroups per compute unit, 1..32), "short_stack" (LDS stack
- * entries per traversal lane, 8 or 16), "refill" (per-lane ray refill, 0/1),
- * "stage_timing" and "trace_stats" (0/1) at the next render or query.  Returns 1 if the key is known and the value
- * valid.  RtGetOption returns -1 for unknown keys. */
+/* Tuning knobs: "bvh_width" (2, 4 or 8), "bvh_quantized" (0/1: 8-bit child bounds, rounded
+ * outwards, so hits are unchanged), "build_bins" (SAH bins per axis of the GPU build, 2..32,
+ * default 32) and "build_max_leaf" (leaf size limit, 1..16, default 4 as the reference
+ * builder) take effect at the next RtInitialize; "blocks_per_cu" (persistent traversal
+ * workgroups per compute unit, 1..32), "short_stack" (LDS stack entries per traversal lane,
+ * 8 or 16), "while_while" (0/1: speculative while-while traversal loop, default 1),
+ * "refill" (0/1: lanes take new rays while the wave traces others, default 1),
+ * "tiled_primary", "nontemporal", "stage_timing", "trace_stats" and "denoise_fallback" at the
+ * next render or query.  Returns 1 if the key is known and the value valid.  RtGetOption
+ * returns -1 for unknown keys; it also reports "node_bytes", the size of one wide node. */
 int RtSetOption(void* session, const char* key, int value);
 int RtGetOption(void* session, const char* key);
 
@@ -188,8 +191,9 @@ int RtConv3x3Backward(const float* in, int n, int h, int w, int cin, const float
 /* Batched views: many scenes and cameras rendered together, one launch per stage over a
  * shared path queue, each frame replayed as a captured hipGraph ("use_graph", default 1).
  * Each view takes one progressive sample per frame (raysPerPixel is ignored) and equals a
- * session rendering its scene with raysPerPixel 1.  Options ("bvh_width", "blocks_per_cu",
- * "short_stack", "refill", "use_graph") must be set before RtBatchBuild.  RtBatchReadView
+ * session rendering its scene with raysPerPixel 1.  Options ("bvh_width", "bvh_quantized",
+ * "blocks_per_cu", "short_stack", "while_while", "refill", "use_graph") must be set before
+ * RtBatchBuild.  RtBatchReadView
  * copies a view's RGBA8 image and/or its 10-float-per-pixel accumulation. */
 void* RtBatchCreate(void);
 void RtBatchDestroy(void* batch);
