@@ -852,6 +852,9 @@ int RtDenoiserTrain(void* net, int steps, int batch, int crop, float learningRat
     if (!net || steps < 1 || batch < 1 || crop < 16 || crop % 16 || !(learningRate > 0)) return 0;
     return denoiserTrain(static_cast<DenoiserNet*>(net), steps, batch, crop, learningRate, seed, losses) == hipSuccess;
 }
+int RtDenoiserSetDisplayLoss(void* net, float weight) {
+    return denoiserSetDisplayWeight(static_cast<DenoiserNet*>(net), weight) ? 1 : 0;
+}
 int RtDenoiserTimings(void* net, float* forwardMs, float* trainStepMs) {
     if (!net) return 0;
     if (forwardMs) *forwardMs = denoiserLastForwardMs(static_cast<DenoiserNet*>(net));

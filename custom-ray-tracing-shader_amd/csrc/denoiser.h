@@ -34,6 +34,8 @@ hipError_t denoiserSetDataset(DenoiserNet* net, const float* inputs, const float
 hipError_t denoiserTrain(DenoiserNet* net, int steps, int batch, int crop, float lr, unsigned seed, float* losses);
 // Analytic multiply-add FLOPs of one forward pass over n x h x w pixels (h, w multiples of 16).
 double denoiserForwardFlops(int n, int h, int w);
+// Weight of the display-space L1 term of the training loss (0 = log-space L1 only).
+bool denoiserSetDisplayWeight(DenoiserNet* net, float weight);
 // Device time of the last forward and of the last training step's forward+backward.
 float denoiserLastForwardMs(const DenoiserNet* net);
 float denoiserLastTrainStepMs(const DenoiserNet* net);

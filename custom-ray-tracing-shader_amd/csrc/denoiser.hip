@@ -334,9 +334,22 @@ __global__ void cropKernel(const float* dsIn, const float* dsTgt, int dsH, int d
     targetLog[i * 4 + 3] = 0.0f;
 }
 
-// L1 in log space on the residual prediction; writes the output gradient (BF16, 32 ch).
-__global__ void lossKernel(const float* out, const float* noisyLog, const float* targetLog, int npix, bf16* gout,
-                           float* loss) {
+// Display transform of Native/renderer.cu:86-90 at exposure 1 without the 8-bit rounding:
+// t = 1 - exp(-L), then the sRGB curve; and its derivative with respect to L.
+__device__ __forceinline__ float displayValue(float L, float& slope) {
+    const float e = expf(-L), t = 1.0f - e;
+    if (t <= 0.0031308f) { slope = 12.92f * e; return 12.92f * t; }
+    const float pw = powf(t, 1.0f / 2.4f);
+    slope = 1.055f / 2.4f * pw / t * e;
+    return 1.055f * pw - 0.055f;
+}
+
+// Loss on the residual prediction y = noisyLog + out (demodulated log radiance):
+// mean over pixels and RGB of |y - targetLog| + displayWeight * |D(L(y)) - D(L(target))|,
+// where L(y) = max(expm1(y), 0) * demod is the radiance the network outputs and D the
+// display transform.  Writes the gradient w.r.t. out (BF16, 32 channels).
+__global__ void lossKernel(const float* out, const float* noisyLog, const float* targetLog, const float* demod,
+                           float displayWeight, int npix, bf16* gout, float* loss) {
     __shared__ float red[kB];
     const int p = blockIdx.x * kB + threadIdx.x;
     float l = 0;
@@ -345,9 +358,21 @@ __global__ void lossKernel(const float* out, const float* noisyLog, const float*
         for (int c = 0; c < 32; c++) {
             float gr = 0;
             if (c < 3) {
-                const float d = noisyLog[(size_t)p * 4 + c] + out[(size_t)p * 4 + c] - targetLog[(size_t)p * 4 + c];
+                const float y = noisyLog[(size_t)p * 4 + c] + out[(size_t)p * 4 + c];
+                const float tl = targetLog[(size_t)p * 4 + c];
+                const float d = y - tl;
                 l += fabsf(d);
                 gr = d > 0 ? scale : (d < 0 ? -scale : 0.0f);
+                if (displayWeight > 0) {
+                    const float m = demod[(size_t)p * 4 + c];
+                    const float em = expm1f(y);
+                    float sp, st;
+                    const float dp = displayValue(fmaxf(em, 0.0f) * m, sp);
+                    const float dt = displayValue(fmaxf(expm1f(tl), 0.0f) * m, st);
+                    const float dd = dp - dt;
+                    l += displayWeight * fabsf(dd);
+                    if (em > 0 && dd != 0) gr += (dd > 0 ? scale : -scale) * displayWeight * sp * (em + 1.0f) * m;
+                }
             }
             gout[(size_t)p * 32 + c] = (bf16)gr;
         }
@@ -428,6 +453,7 @@ struct DenoiserNet {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     float lastFwdMs = 0, lastTrainMs = 0;
+    float displayWeight = 0;   // weight of the display-space L1 term of the training loss
 };
 
 namespace {
@@ -835,7 +861,7 @@ hipError_t denoiserGradients(DenoiserNet* net, const float* input, const float* 
     DN_TRY(forward(net, s));
     DN_TRY(hipMemsetAsync(net->loss, 0, 4, s));
     hipLaunchKernelGGL(lossKernel, dim3(blocks((long long)npix)), dim3(kB), 0, s, net->outF, net->noisyLog, net->targetLog,
-                       (int)npix, net->gout, net->loss);
+                       net->demod, net->displayWeight, (int)npix, net->gout, net->loss);
     DN_TRY(hipGetLastError());
     DN_TRY(backward(net, s));
     DN_TRY(hipStreamSynchronize(s));
@@ -888,7 +914,7 @@ hipError_t denoiserTrain(DenoiserNet* net, int steps, int batch, int crop, float
                            net->targetLog);
         if (err == hipSuccess) err = forward(net, s);
         hipLaunchKernelGGL(lossKernel, dim3(blocks((long long)npix)), dim3(kB), 0, s, net->outF, net->noisyLog,
-                           net->targetLog, npix, net->gout, dLoss + it);
+                           net->targetLog, net->demod, net->displayWeight, npix, net->gout, dLoss + it);
         if (err == hipSuccess) err = backward(net, s);
         if (it == steps - 1 && err == hipSuccess) err = hipEventRecord(net->ev1, s);
         net->adamStep++;
@@ -915,6 +941,11 @@ double denoiserForwardFlops(int n, int h, int w) {
     return f;
 }
 float denoiserLastForwardMs(const DenoiserNet* net) { return net ? net->lastFwdMs : 0; }
+bool denoiserSetDisplayWeight(DenoiserNet* net, float weight) {
+    if (!net || !(weight >= 0) || weight > 1e6f) return false;
+    net->displayWeight = weight;
+    return true;
+}
 float denoiserLastTrainStepMs(const DenoiserNet* net) { return net ? net->lastTrainMs : 0; }
 
 // ---------------------------------------------------------------- single-layer checks

@@ -127,6 +127,7 @@ def load_library(path=None):
     lib.RtDenoiserSetDataset.argtypes = [vp, fp, fp, c.c_int, c.c_int, c.c_int]
     lib.RtDenoiserTrain.argtypes = [vp, c.c_int, c.c_int, c.c_int, c.c_float, c.c_uint, fp]
     lib.RtDenoiserTimings.argtypes = [vp, fp, fp]
+    lib.RtDenoiserSetDisplayLoss.argtypes = [vp, c.c_float]
     lib.RtDenoiserForwardFlops.argtypes = [c.c_int, c.c_int, c.c_int]
     lib.RtDenoiserForwardFlops.restype = c.c_double
     lib.RtConv3x3Forward.argtypes = [fp, c.c_int, c.c_int, c.c_int, c.c_int, fp, fp, c.c_int, c.c_int, fp]
@@ -281,6 +282,10 @@ class Denoiser:
         f, t = c.c_float(), c.c_float()
         self.lib.RtDenoiserTimings(self.handle, c.byref(f), c.byref(t))
         return f.value, t.value
+
+    def set_display_loss(self, weight):
+        if not self.lib.RtDenoiserSetDisplayLoss(self.handle, float(weight)):
+            raise ValueError(f"display loss weight {weight} rejected")
 
 
 STAGES = ("reset", "raygen", "closest", "shade", "shadow", "accumulate")

@@ -4,7 +4,8 @@ Dataset: for each seeded scene (displaced icosphere in a lit room, triangle soup
 light, or a Cornell-box variant), a 4-spp render supplies the network input planes
 (RtReadDenoiserInputs: radiance, albedo, normal, depth) and an independently seeded
 high-sample render supplies the target radiance.  Training runs entirely on the GPU
-(RtDenoiserTrain: random crops + flips, Adam, L1 in log space).  The trained FP32 weights are
+(RtDenoiserTrain: random crops + flips, Adam, L1 in log space, optionally plus an L1 term on
+the display-transformed radiance).  The trained FP32 weights are
 written with a versioned header to denoiser_weights.bin next to the library, where RtDenoise
 finds them.
 
@@ -130,6 +131,8 @@ def main():
     ap.add_argument("--batch", type=int, default=8)
     ap.add_argument("--crop", type=int, default=128)
     ap.add_argument("--lr", type=float, default=3e-4)
+    ap.add_argument("--display-weight", type=float, default=0.0,
+                    help="weight of the display-space L1 term added to the log-space L1 loss")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--out", default=str(HERE / "denoiser_weights.bin"))
     ap.add_argument("--report", default=None, help="write a JSON summary here")
@@ -141,6 +144,7 @@ def main():
     hin, htg = build_dataset(lib, args.heldout, args.size, args.input_spp, args.target_spp, 100000, log)
     data_s = time.perf_counter() - t0
     net = Denoiser(lib, seed=args.seed)
+    net.set_display_loss(args.display_weight)
     net.set_dataset(inputs, targets)
     before = evaluate(net, hin, htg)
     log(f"held-out PSNR before training: plain {before[0]:.2f} dB, network {before[1]:.2f} dB")
@@ -159,6 +163,7 @@ def main():
     flops = 3 * lib.RtDenoiserForwardFlops(args.batch, args.crop, args.crop)
     report = {"scenes": args.scenes, "heldout": args.heldout, "size": args.size, "input_spp": args.input_spp,
               "target_spp": args.target_spp, "steps": args.steps, "batch": args.batch, "crop": args.crop,
+              "display_weight": args.display_weight,
               "dataset_seconds": round(data_s, 1), "train_seconds": round(train_s, 1),
               "train_step_ms": round(step_ms, 3),
               "train_step_mfma_utilization": round(flops / (step_ms / 1e3) / 2.5e15, 4) if step_ms > 0 else None,

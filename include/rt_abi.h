@@ -180,6 +180,10 @@ int RtDenoiserSetDataset(void* denoiser, const float* inputs, const float* targe
  * losses[steps] receives each step's L1 loss in log space. */
 int RtDenoiserTrain(void* denoiser, int steps, int batch, int crop, float learningRate, unsigned seed, float* losses);
 int RtDenoiserTimings(void* denoiser, float* forwardMs, float* trainStepMs);
+/* Adds weight * |D(out) - D(target)| to the per-channel training loss (RtDenoiserTrain and
+ * RtDenoiserGradients), D being the display transform of Native/renderer.cu:86-90 at
+ * exposure 1 before 8-bit rounding; 0 (the default) trains on the log-space L1 alone. */
+int RtDenoiserSetDisplayLoss(void* denoiser, float weight);
 double RtDenoiserForwardFlops(int n, int h, int w);
 /* One 3x3 convolution (stride 1, zero padding, NHWC, weights [9][cout][cin]) and its
  * gradients on the GPU, inputs rounded to BF16: the kernels the network is built from. */

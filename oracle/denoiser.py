@@ -140,14 +140,38 @@ def infer(params, raw):
     return np.maximum(np.expm1(cache["noisy_log"] + out), 0) * cache["demod"]
 
 
-def loss_and_grads(params, raw, target):
-    """L1 loss in log space (mean over pixels and RGB) and its gradient w.r.t. params."""
+def display_value(L):
+    """Display transform of Native/renderer.cu:86-90 at exposure 1 before 8-bit rounding,
+    and its derivative w.r.t. L."""
+    e = np.exp(-L)
+    t = 1 - e
+    lin = t <= 0.0031308
+    tp = np.where(lin, 1.0, t)
+    value = np.where(lin, 12.92 * t, 1.055 * tp ** (1 / 2.4) - 0.055)
+    slope = np.where(lin, 12.92 * e, 1.055 / 2.4 * tp ** (1 / 2.4 - 1) * e)
+    return value, slope
+
+
+def loss_and_grads(params, raw, target, display_weight=0.0):
+    """Mean over pixels and RGB of |y - log target| + display_weight * |D(L(y)) - D(target)|
+    (y the demodulated log prediction, L(y) = max(expm1(y), 0) * demod, D the display
+    transform) and its gradient w.r.t. params."""
     L = unpack(params)
     out, cache = forward(params, raw)
     tgt_log = np.log1p(np.maximum(target.astype(np.float64), 0) / cache["demod"])
-    d = cache["noisy_log"] + out - tgt_log
+    y = cache["noisy_log"] + out
+    d = y - tgt_log
     count = d.size
     loss = np.abs(d).sum() / count
+    g_y = np.sign(d) / count
+    if display_weight > 0:
+        m = cache["demod"]
+        em = np.expm1(y)
+        dp, sp = display_value(np.maximum(em, 0) * m)
+        dt, _ = display_value(np.maximum(np.expm1(tgt_log), 0) * m)
+        dd = dp - dt
+        loss += display_weight * np.abs(dd).sum() / count
+        g_y = g_y + np.where(em > 0, np.sign(dd) * display_weight * sp * (em + 1) * m, 0.0) / count
     grads = np.zeros(params.size)
     lay = {row[0]: row for row in layout()[0]}
 
@@ -162,7 +186,7 @@ def loss_and_grads(params, raw, target):
         return gx
 
     g_out = np.zeros(out.shape[:3] + (32,))
-    g_out[..., :3] = np.sign(d) / count
+    g_out[..., :3] = g_y
     g = back("d0a", back("d0b", back("out", g_out)))             # g w.r.t. concat [up(d1) | e0]
     g_e0 = g[..., 64:96]
     g = back("d1a", back("d1b", upsample_backward(g[..., :64])))

@@ -58,15 +58,17 @@ def test_conv_backward_vs_fp64(lib, cin, cout, h, w):
     assert np.abs(gb - rb).max() <= 1e-5 * np.abs(rb).max() + 1e-6
 
 
-def test_network_gradients_vs_fp64_oracle(lib):
+@pytest.mark.parametrize("display_weight", [0.0, 4.0])
+def test_network_gradients_vs_fp64_oracle(lib, display_weight):
     rng = np.random.default_rng(7)
     raw = np.abs(rng.normal(size=(2, 32, 32, 10))).astype(np.float32)
     raw[..., 6:9] = rng.normal(size=(2, 32, 32, 3))
     target = (np.abs(rng.normal(size=(2, 32, 32, 3))) * 4 + 2).astype(np.float32)
     with Denoiser(lib, seed=11) as net:
+        net.set_display_loss(display_weight)
         params = net.params
         loss, grads = net.gradients(raw, target)
-    ref_loss, ref_grads = odn.loss_and_grads(odn.bf16_round(params).astype(np.float64), raw, target)
+    ref_loss, ref_grads = odn.loss_and_grads(odn.bf16_round(params).astype(np.float64), raw, target, display_weight)
     assert loss == pytest.approx(ref_loss, rel=2e-2)
     lay, _ = odn.layout()
     for name, cin, cout, ci, co, relu, w_off, b_off in lay:

@@ -2,6 +2,8 @@
 finite differences (layer and whole network)."""
 import numpy as np
 import pytest
+
+import oracle
 from scipy.signal import correlate
 
 from oracle import denoiser as dn
@@ -52,12 +54,25 @@ def test_pool_upsample_adjoints():
     assert (dn.upsample(y) * G).sum() == pytest.approx((y * dn.upsample_backward(G)).sum())
 
 
-def test_network_gradient_finite_differences():
+def test_display_value_closed_form_and_slope():
+    """D(1) is the byte-208 case of Tools/verify_native.py:126 before rounding; the slope
+    matches central differences on both sides of the sRGB knee."""
+    v, _ = dn.display_value(np.array([1.0]))
+    assert int(255 * min(v[0], 1) + 0.5) == 208
+    assert oracle.display(1.0, 1.0) == 208
+    for L in (1e-4, 2e-3, 0.05, 0.7, 3.0):
+        eps = 1e-7 * max(L, 1e-3)
+        fd = (dn.display_value(np.array([L + eps]))[0] - dn.display_value(np.array([L - eps]))[0]) / (2 * eps)
+        assert dn.display_value(np.array([L]))[1][0] == pytest.approx(fd[0], rel=1e-5)
+
+
+@pytest.mark.parametrize("display_weight", [0.0, 2.0])
+def test_network_gradient_finite_differences(display_weight):
     rng = np.random.default_rng(3)
     params = dn.init_params(4).astype(np.float64)
     raw = np.abs(rng.normal(size=(1, 16, 16, 10)))
     target = np.abs(rng.normal(size=(1, 16, 16, 3))) * 5 + 3     # far from the prediction: stable signs
-    loss, grads = dn.loss_and_grads(params, raw, target)
+    loss, grads = dn.loss_and_grads(params, raw, target, display_weight)
     lay, _ = dn.layout()
     eps = 1e-5
     checked = 0
@@ -67,7 +82,8 @@ def test_network_gradient_finite_differences():
                 continue
             pp, pm = params.copy(), params.copy()
             pp[k] += eps; pm[k] -= eps
-            fd = (dn.loss_and_grads(pp, raw, target)[0] - dn.loss_and_grads(pm, raw, target)[0]) / (2 * eps)
+            fd = (dn.loss_and_grads(pp, raw, target, display_weight)[0]
+                  - dn.loss_and_grads(pm, raw, target, display_weight)[0]) / (2 * eps)
             assert grads[k] == pytest.approx(fd, rel=1e-4, abs=1e-9), (name, k)
             checked += 1
     assert checked >= 30
