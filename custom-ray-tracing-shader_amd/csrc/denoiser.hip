@@ -134,6 +134,76 @@ __global__ __launch_bounds__(kB) void convFwdKernel(ConvArgs a) {
             }
 }
 
+// The same convolution for layers with 32 or 64 output channels on large images: a block
+// computes a 4-row x 64-column output tile with all output channels (one row per wave).
+// Each 32-channel slice of the input tile plus its one-pixel halo (6 x 66 pixels) is staged
+// in LDS once and read there by all nine taps, so each input pixel leaves L2 once per
+// block instead of once per tap.  (Staging the weights in LDS as well measured no faster:
+// the larger LDS footprint halves the resident blocks.)
+constexpr int kTileW = 64, kTileH = 4, kHaloW = kTileW + 2, kHaloH = kTileH + 2;
+
+template <int NT>
+__global__ __launch_bounds__(kB) void convFwdTileKernel(ConvArgs a) {
+    __shared__ __attribute__((aligned(16))) bf16 tile[kHaloH * kHaloW * 32];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int g = lane >> 4, r = lane & 15;
+    const int x0 = blockIdx.x * kTileW, y0 = blockIdx.y * kTileH, nn = blockIdx.z;
+    f32x4 acc[4][NT];
+#pragma unroll
+    for (int t = 0; t < 4; t++)
+#pragma unroll
+        for (int j = 0; j < NT; j++) acc[t][j] = f32x4{0, 0, 0, 0};
+    for (int ci0 = 0; ci0 < a.cinPad; ci0 += 32) {
+        __syncthreads();   // the previous slice's reads are done
+        for (int v = threadIdx.x; v < kHaloH * kHaloW * 4; v += kB) {
+            const int px = v >> 2, part = v & 3;
+            const int yy = y0 + px / kHaloW - 1, xx = x0 + px % kHaloW - 1;
+            bf16x8 val = zero8();
+            if (yy >= 0 && yy < a.h && xx >= 0 && xx < a.w)
+                val = *reinterpret_cast<const bf16x8*>(a.in.p + ((size_t)(nn * a.h + yy) * a.w + xx) * a.in.cs + a.in.co +
+                                                       ci0 + 8 * part);
+            *reinterpret_cast<bf16x8*>(tile + px * 32 + 8 * part) = val;
+        }
+        __syncthreads();
+#pragma unroll 1
+        for (int tap = 0; tap < 9; tap++) {
+            const int dy = tap / 3, dx = tap % 3;
+            bf16x8 av[4];
+#pragma unroll
+            for (int t = 0; t < 4; t++)
+                av[t] = *reinterpret_cast<const bf16x8*>(tile + ((wave + dy) * kHaloW + 16 * t + r + dx) * 32 + 8 * g);
+            const bf16* wrow = a.wt + ((size_t)tap * a.coutPad + r) * a.cinPad + ci0 + 8 * g;
+#pragma unroll
+            for (int j = 0; j < NT; j++) {
+                const bf16x8 bv = *reinterpret_cast<const bf16x8*>(wrow + (size_t)16 * j * a.cinPad);
+#pragma unroll
+                for (int t = 0; t < 4; t++) acc[t][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[t], bv, acc[t][j], 0, 0, 0);
+            }
+        }
+    }
+    const int y = y0 + wave;
+    if (y >= a.h) return;
+#pragma unroll
+    for (int t = 0; t < 4; t++)
+#pragma unroll
+        for (int j = 0; j < NT; j++)
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int x = x0 + 16 * t + 4 * g + i;
+                if (x >= a.w) continue;
+                const size_t p = (size_t)(nn * a.h + y) * a.w + x;
+                const int c = 16 * j + r;
+                float v = acc[t][j][i] + (a.bias ? a.bias[c] : 0.0f);
+                if (a.relu) v = fmaxf(v, 0.0f);
+                if (a.out.p && c < a.out.c) {
+                    bf16* dst = a.out.p + p * a.out.cs + a.out.co + c;
+                    if (a.accumulate) v += (float)*dst;
+                    *dst = (bf16)v;
+                }
+                if (a.outF && c < a.outFStride) a.outF[p * a.outFStride + c] = v;
+            }
+}
+
 struct BwdWArgs {
     View dy;       // gradient w.r.t. the pre-activation output, coutPad channels
     View in;       // layer input, cinPad channels
@@ -239,57 +309,79 @@ __global__ void reluMaskKernel(View g, View act, bf16* out, int npix, int channe
     out[i] = a > 0.0f ? g.p[(size_t)p * g.cs + g.co + c] : (bf16)0.0f;
 }
 
+// Pooling and upsampling move 8 channels (16 bytes) per thread: every view's channel
+// stride, offset and width are multiples of 32, so each 8-channel group is 16-byte aligned.
+__device__ __forceinline__ bf16x8 load8(const View& v, size_t pixel, int c) {
+    return *reinterpret_cast<const bf16x8*>(v.p + pixel * v.cs + v.co + c);
+}
+__device__ __forceinline__ void store8(const View& v, size_t pixel, int c, bf16x8 x) {
+    *reinterpret_cast<bf16x8*>(v.p + pixel * v.cs + v.co + c) = x;
+}
+
 __global__ void avgPoolKernel(View in, View out, int n, int h, int w, int channels) {
-    const int ho = h / 2, wo = w / 2;
+    const int ho = h / 2, wo = w / 2, groups = channels / 8;
     const long long i = blockIdx.x * (long long)kB + threadIdx.x;
-    if (i >= (long long)n * ho * wo * channels) return;
-    const int c = (int)(i % channels);
-    const long long po = i / channels;
+    if (i >= (long long)n * ho * wo * groups) return;
+    const int c = 8 * (int)(i % groups);
+    const long long po = i / groups;
     const int x = (int)(po % wo), y = (int)((po / wo) % ho), nn = (int)(po / ((long long)wo * ho));
-    float s = 0;
+    float s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (int a = 0; a < 2; a++)
         for (int b = 0; b < 2; b++) {
-            const size_t pi = ((size_t)nn * h + 2 * y + a) * w + 2 * x + b;
-            s += (float)in.p[pi * in.cs + in.co + c];
+            const bf16x8 v = load8(in, ((size_t)nn * h + 2 * y + a) * w + 2 * x + b, c);
+#pragma unroll
+            for (int e = 0; e < 8; e++) s[e] += (float)v[e];
         }
-    out.p[(size_t)po * out.cs + out.co + c] = (bf16)(0.25f * s);
+    bf16x8 o;
+#pragma unroll
+    for (int e = 0; e < 8; e++) o[e] = (bf16)(0.25f * s[e]);
+    store8(out, (size_t)po, c, o);
 }
 
 // Adds the pooling gradient to the input-side gradient view.
 __global__ void avgPoolBackKernel(View gout, View gin, int n, int h, int w, int channels) {
+    const int groups = channels / 8;
     const long long i = blockIdx.x * (long long)kB + threadIdx.x;
-    if (i >= (long long)n * h * w * channels) return;
-    const int c = (int)(i % channels);
-    const long long p = i / channels;
+    if (i >= (long long)n * h * w * groups) return;
+    const int c = 8 * (int)(i % groups);
+    const long long p = i / groups;
     const int x = (int)(p % w), y = (int)((p / w) % h), nn = (int)(p / ((long long)w * h));
     const size_t po = ((size_t)nn * (h / 2) + y / 2) * (w / 2) + x / 2;
-    bf16* dst = gin.p + (size_t)p * gin.cs + gin.co + c;
-    *dst = (bf16)((float)*dst + 0.25f * (float)gout.p[po * gout.cs + gout.co + c]);
+    const bf16x8 g = load8(gout, po, c);
+    bf16x8 d = load8(gin, (size_t)p, c);
+#pragma unroll
+    for (int e = 0; e < 8; e++) d[e] = (bf16)((float)d[e] + 0.25f * (float)g[e]);
+    store8(gin, (size_t)p, c, d);
 }
 
 __global__ void upsampleKernel(View in, View out, int n, int h, int w, int channels) {   // h, w: output size
+    const int groups = channels / 8;
     const long long i = blockIdx.x * (long long)kB + threadIdx.x;
-    if (i >= (long long)n * h * w * channels) return;
-    const int c = (int)(i % channels);
-    const long long p = i / channels;
+    if (i >= (long long)n * h * w * groups) return;
+    const int c = 8 * (int)(i % groups);
+    const long long p = i / groups;
     const int x = (int)(p % w), y = (int)((p / w) % h), nn = (int)(p / ((long long)w * h));
-    const size_t pi = ((size_t)nn * (h / 2) + y / 2) * (w / 2) + x / 2;
-    out.p[(size_t)p * out.cs + out.co + c] = in.p[pi * in.cs + in.co + c];
+    store8(out, (size_t)p, c, load8(in, ((size_t)nn * (h / 2) + y / 2) * (w / 2) + x / 2, c));
 }
 
 __global__ void upsampleBackKernel(View gout, View gin, int n, int h, int w, int channels) {   // h, w: input size
+    const int groups = channels / 8;
     const long long i = blockIdx.x * (long long)kB + threadIdx.x;
-    if (i >= (long long)n * h * w * channels) return;
-    const int c = (int)(i % channels);
-    const long long p = i / channels;
+    if (i >= (long long)n * h * w * groups) return;
+    const int c = 8 * (int)(i % groups);
+    const long long p = i / groups;
     const int x = (int)(p % w), y = (int)((p / w) % h), nn = (int)(p / ((long long)w * h));
-    float s = 0;
+    float s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (int a = 0; a < 2; a++)
         for (int b = 0; b < 2; b++) {
-            const size_t po = ((size_t)nn * 2 * h + 2 * y + a) * (2 * w) + 2 * x + b;
-            s += (float)gout.p[po * gout.cs + gout.co + c];
+            const bf16x8 v = load8(gout, ((size_t)nn * 2 * h + 2 * y + a) * (2 * w) + 2 * x + b, c);
+#pragma unroll
+            for (int e = 0; e < 8; e++) s[e] += (float)v[e];
         }
-    gin.p[(size_t)p * gin.cs + gin.co + c] = (bf16)s;
+    bf16x8 o;
+#pragma unroll
+    for (int e = 0; e < 8; e++) o[e] = (bf16)s[e];
+    store8(gin, (size_t)p, c, o);
 }
 
 constexpr float kDemodEps = 0.01f;
@@ -481,6 +573,12 @@ hipError_t launchConvFwd(const ConvArgs& a, hipStream_t s) {
     const int npix = a.n * a.h * a.w;
     const int tiles = a.coutPad / 16;
     if (a.cinPad % 32 || a.coutPad % 32 || a.in.c != a.cinPad) return hipErrorInvalidValue;
+    if ((a.coutPad == 32 || a.coutPad == 64) && (long long)a.w * a.h >= 16384) {
+        const dim3 grid((a.w + kTileW - 1) / kTileW, (a.h + kTileH - 1) / kTileH, a.n);
+        if (a.coutPad == 32) hipLaunchKernelGGL(convFwdTileKernel<2>, grid, dim3(kB), 0, s, a);
+        else hipLaunchKernelGGL(convFwdTileKernel<4>, grid, dim3(kB), 0, s, a);
+        return hipGetLastError();
+    }
     if (tiles % 4 == 0) hipLaunchKernelGGL(convFwdKernel<4>, dim3(blocks(npix), tiles / 4), dim3(kB), 0, s, a);
     else hipLaunchKernelGGL(convFwdKernel<2>, dim3(blocks(npix), tiles / 2), dim3(kB), 0, s, a);
     return hipGetLastError();
@@ -584,11 +682,11 @@ hipError_t conv(DenoiserNet* net, int l, const View& in, const View& out, int n,
 }
 
 hipError_t pool(const View& in, const View& out, int n, int h, int w, int c, hipStream_t s) {
-    hipLaunchKernelGGL(avgPoolKernel, dim3(blocks((long long)n * (h / 2) * (w / 2) * c)), dim3(kB), 0, s, in, out, n, h, w, c);
+    hipLaunchKernelGGL(avgPoolKernel, dim3(blocks((long long)n * (h / 2) * (w / 2) * (c / 8))), dim3(kB), 0, s, in, out, n, h, w, c);
     return hipGetLastError();
 }
 hipError_t upsample(const View& in, const View& out, int n, int h, int w, int c, hipStream_t s) {
-    hipLaunchKernelGGL(upsampleKernel, dim3(blocks((long long)n * h * w * c)), dim3(kB), 0, s, in, out, n, h, w, c);
+    hipLaunchKernelGGL(upsampleKernel, dim3(blocks((long long)n * h * w * (c / 8))), dim3(kB), 0, s, in, out, n, h, w, c);
     return hipGetLastError();
 }
 
@@ -658,35 +756,35 @@ hipError_t backward(DenoiserNet* net, hipStream_t s) {
     DN_TRY(convBack(net, OUT, V(net->gout, 32), none, V(net->d0b, 32), V(net->gd0b, 32), false, n, h0, w0, s));
     DN_TRY(convBack(net, D0B, V(net->gd0b, 32), V(net->d0b, 32), V(net->d0a, 32), V(net->gd0a, 32), false, n, h0, w0, s));
     DN_TRY(convBack(net, D0A, V(net->gd0a, 32), V(net->d0a, 32), V(net->D0, 96), V(net->gD0, 96), false, n, h0, w0, s));
-    hipLaunchKernelGGL(upsampleBackKernel, dim3(blocks((long long)n * h1 * w1 * 64)), dim3(kB), 0, s, V(net->gD0, 96, 0, 64),
+    hipLaunchKernelGGL(upsampleBackKernel, dim3(blocks((long long)n * h1 * w1 * 8)), dim3(kB), 0, s, V(net->gD0, 96, 0, 64),
                        V(net->gd1b, 64), n, h1, w1, 64);
     DN_TRY(convBack(net, D1B, V(net->gd1b, 64), V(net->d1b, 64), V(net->d1a, 64), V(net->gd1a, 64), false, n, h1, w1, s));
     DN_TRY(convBack(net, D1A, V(net->gd1a, 64), V(net->d1a, 64), V(net->D1, 128), V(net->gD1, 128), false, n, h1, w1, s));
-    hipLaunchKernelGGL(upsampleBackKernel, dim3(blocks((long long)n * h2 * w2 * 64)), dim3(kB), 0, s, V(net->gD1, 128, 0, 64),
+    hipLaunchKernelGGL(upsampleBackKernel, dim3(blocks((long long)n * h2 * w2 * 8)), dim3(kB), 0, s, V(net->gD1, 128, 0, 64),
                        V(net->gd2b, 64), n, h2, w2, 64);
     DN_TRY(convBack(net, D2B, V(net->gd2b, 64), V(net->d2b, 64), V(net->d2a, 64), V(net->gd2a, 64), false, n, h2, w2, s));
     DN_TRY(convBack(net, D2A, V(net->gd2a, 64), V(net->d2a, 64), V(net->D2, 160), V(net->gD2, 160), false, n, h2, w2, s));
-    hipLaunchKernelGGL(upsampleBackKernel, dim3(blocks((long long)n * h3 * w3 * 96)), dim3(kB), 0, s, V(net->gD2, 160, 0, 96),
+    hipLaunchKernelGGL(upsampleBackKernel, dim3(blocks((long long)n * h3 * w3 * 12)), dim3(kB), 0, s, V(net->gD2, 160, 0, 96),
                        V(net->gd3b, 96), n, h3, w3, 96);
     DN_TRY(convBack(net, D3B, V(net->gd3b, 96), V(net->d3b, 96), V(net->d3a, 96), V(net->gd3a, 96), false, n, h3, w3, s));
     DN_TRY(convBack(net, D3A, V(net->gd3a, 96), V(net->d3a, 96), V(net->D3, 224), V(net->gD3, 224), false, n, h3, w3, s));
-    hipLaunchKernelGGL(upsampleBackKernel, dim3(blocks((long long)n * h4 * w4 * 128)), dim3(kB), 0, s, V(net->gD3, 224, 0, 128),
+    hipLaunchKernelGGL(upsampleBackKernel, dim3(blocks((long long)n * h4 * w4 * 16)), dim3(kB), 0, s, V(net->gD3, 224, 0, 128),
                        V(net->gbb, 128), n, h4, w4, 128);
     DN_TRY(convBack(net, BB, V(net->gbb, 128), V(net->bb, 128), V(net->ba, 128), V(net->gba, 128), false, n, h4, w4, s));
     DN_TRY(convBack(net, BA, V(net->gba, 128), V(net->ba, 128), V(net->p3, 96), V(net->gp3, 96), false, n, h4, w4, s));
-    hipLaunchKernelGGL(avgPoolBackKernel, dim3(blocks((long long)n * h3 * w3 * 96)), dim3(kB), 0, s, V(net->gp3, 96),
+    hipLaunchKernelGGL(avgPoolBackKernel, dim3(blocks((long long)n * h3 * w3 * 12)), dim3(kB), 0, s, V(net->gp3, 96),
                        V(net->gD3, 224, 128, 96), n, h3, w3, 96);
     DN_TRY(convBack(net, E3B, V(net->gD3, 224, 128, 96), V(net->D3, 224, 128, 96), V(net->e3a, 96), V(net->ge3a, 96), false, n, h3, w3, s));
     DN_TRY(convBack(net, E3A, V(net->ge3a, 96), V(net->e3a, 96), V(net->p2, 64), V(net->gp2, 64), false, n, h3, w3, s));
-    hipLaunchKernelGGL(avgPoolBackKernel, dim3(blocks((long long)n * h2 * w2 * 64)), dim3(kB), 0, s, V(net->gp2, 64),
+    hipLaunchKernelGGL(avgPoolBackKernel, dim3(blocks((long long)n * h2 * w2 * 8)), dim3(kB), 0, s, V(net->gp2, 64),
                        V(net->gD2, 160, 96, 64), n, h2, w2, 64);
     DN_TRY(convBack(net, E2B, V(net->gD2, 160, 96, 64), V(net->D2, 160, 96, 64), V(net->e2a, 64), V(net->ge2a, 64), false, n, h2, w2, s));
     DN_TRY(convBack(net, E2A, V(net->ge2a, 64), V(net->e2a, 64), V(net->p1, 64), V(net->gp1, 64), false, n, h2, w2, s));
-    hipLaunchKernelGGL(avgPoolBackKernel, dim3(blocks((long long)n * h1 * w1 * 64)), dim3(kB), 0, s, V(net->gp1, 64),
+    hipLaunchKernelGGL(avgPoolBackKernel, dim3(blocks((long long)n * h1 * w1 * 8)), dim3(kB), 0, s, V(net->gp1, 64),
                        V(net->gD1, 128, 64, 64), n, h1, w1, 64);
     DN_TRY(convBack(net, E1B, V(net->gD1, 128, 64, 64), V(net->D1, 128, 64, 64), V(net->e1a, 64), V(net->ge1a, 64), false, n, h1, w1, s));
     DN_TRY(convBack(net, E1A, V(net->ge1a, 64), V(net->e1a, 64), V(net->p0, 32), V(net->gp0, 32), false, n, h1, w1, s));
-    hipLaunchKernelGGL(avgPoolBackKernel, dim3(blocks((long long)n * h0 * w0 * 32)), dim3(kB), 0, s, V(net->gp0, 32),
+    hipLaunchKernelGGL(avgPoolBackKernel, dim3(blocks((long long)n * h0 * w0 * 4)), dim3(kB), 0, s, V(net->gp0, 32),
                        V(net->gD0, 96, 64, 32), n, h0, w0, 32);
     DN_TRY(convBack(net, E0B, V(net->gD0, 96, 64, 32), V(net->D0, 96, 64, 32), V(net->e0a, 32), V(net->ge0a, 32), false, n, h0, w0, s));
     DN_TRY(convBack(net, E0A, V(net->ge0a, 32), V(net->e0a, 32), V(net->x0, 32), none, false, n, h0, w0, s));

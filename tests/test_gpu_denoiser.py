@@ -23,8 +23,11 @@ def conv_forward(lib, x, w, b, relu):
     return out
 
 
+# The last three shapes are large enough for the LDS-tiled kernel (32/64 output channels,
+# >= 16384 pixels per image), with partial 4 x 64 tiles and one or several input slices.
 @pytest.mark.parametrize("cin,cout,h,w,relu", [(10, 32, 16, 16, 1), (32, 64, 5, 7, 0), (224, 96, 8, 12, 1),
-                                               (32, 3, 9, 4, 0), (64, 128, 6, 6, 1), (160, 64, 4, 8, 0)])
+                                               (32, 3, 9, 4, 0), (64, 128, 6, 6, 1), (160, 64, 4, 8, 0),
+                                               (10, 32, 130, 150, 1), (96, 64, 136, 128, 0), (32, 3, 129, 131, 0)])
 def test_conv_forward_vs_fp64(lib, cin, cout, h, w, relu):
     rng = np.random.default_rng(cin * 1000 + cout)
     x = odn.bf16_round(rng.normal(size=(2, h, w, cin)).astype(np.float32))
@@ -39,7 +42,8 @@ def test_conv_forward_vs_fp64(lib, cin, cout, h, w, relu):
     assert np.abs(got - ref).max() <= 1e-6 * scale + 1e-6
 
 
-@pytest.mark.parametrize("cin,cout,h,w", [(32, 32, 16, 16), (224, 96, 8, 8), (64, 128, 12, 10), (32, 3, 16, 16)])
+@pytest.mark.parametrize("cin,cout,h,w", [(32, 32, 16, 16), (224, 96, 8, 8), (64, 128, 12, 10), (32, 3, 16, 16),
+                                          (64, 32, 130, 132)])
 def test_conv_backward_vs_fp64(lib, cin, cout, h, w):
     rng = np.random.default_rng(cin + cout)
     n = 2
