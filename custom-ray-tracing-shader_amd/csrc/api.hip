@@ -49,7 +49,6 @@ struct Session {
     std::vector<HostSum> hostSum;
     // device
     Triangle* dTris = nullptr;
-    Surface* dSurf = nullptr;        // shading records of dTris
     Light* dLights = nullptr;
     void* dNodes = nullptr;          // WideNode<builtWidth>[] or QNode<builtWidth>[]
     TriRecord* dRecs = nullptr;
@@ -127,7 +126,7 @@ template <class T> bool upload(Session* s, T*& p, const T* host, size_t count) {
 template <class T> void devFree(T*& p) { if (p) (void)hipFree(p); p = nullptr; }
 
 void freeDevice(Session* s) {
-    devFree(s->dTris); devFree(s->dSurf); devFree(s->dLights); devFree(s->dNodes); devFree(s->dRecs);
+    devFree(s->dTris); devFree(s->dLights); devFree(s->dNodes); devFree(s->dRecs);
     devFree(s->dBinNodes); devFree(s->dOrder);
     devFree(s->dSumRad); devFree(s->dSumAlb); devFree(s->dSumNrm); devFree(s->dPixels);
     devFree(s->dThr); devFree(s->dRad); devFree(s->dAlb); devFree(s->dNrm); devFree(s->dNee);
@@ -145,7 +144,6 @@ void freeDevice(Session* s) {
 
 SceneView deviceScene(const Session* s) {
     SceneView v; v.tris = s->dTris; v.lights = s->dLights; v.lightCount = (int)s->lights.size(); v.lightArea = s->lightArea;
-    v.surf = s->dSurf;
     return v;
 }
 
@@ -340,7 +338,6 @@ void renderCpuRows(Session* s, const CameraData& cam, const RenderSettings& rs, 
     const N* nodes = reinterpret_cast<const N*>(s->hostNodes.data());
     const TriRecord* recs = s->hostRecs.data();
     SceneView sc; sc.tris = s->tris.data(); sc.lights = s->lights.data(); sc.lightCount = (int)s->lights.size(); sc.lightArea = s->lightArea;
-    sc.surf = nullptr;   // the host derives shading terms per hit with the same arithmetic
     const int w = s->width, h = s->height;
     const unsigned* dirs = hostSobolDirections();
     const unsigned sample = s->samples;
@@ -473,8 +470,6 @@ int RtInitialize(void* p, const Triangle* triangles, int count, int w, int h) {
     if ((s->error = hipEventCreate(&s->ev0)) != hipSuccess) return 0;
     if ((s->error = hipEventCreate(&s->ev1)) != hipSuccess) return 0;
     if (!upload(s, s->dTris, s->tris.data(), s->tris.size())) return 0;
-    if (!devAlloc(s, s->dSurf, s->tris.size())) return 0;
-    if ((s->error = launchSurfaces(s->dTris, count, s->dSurf, s->stream)) != hipSuccess) return 0;
     if (!buildAccel(s)) return 0;
     if (!upload(s, s->dLights, s->lights.data(), s->lights.size())) return 0;
     const size_t npix = (size_t)w * h;

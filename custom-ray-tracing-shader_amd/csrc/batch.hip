@@ -61,7 +61,6 @@ struct Batch {
     std::vector<int> pixelBase;       // offsets of each view's sums / pixels
     // device
     Triangle* dTris = nullptr;
-    Surface* dSurf = nullptr;
     Light* dLights = nullptr;
     void* dNodes = nullptr;
     TriRecord* dRecs = nullptr;
@@ -94,7 +93,7 @@ void release(Batch* b) {
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
     b->exec = nullptr; b->graph = nullptr;
-    dfree(b->dTris); dfree(b->dSurf); dfree(b->dLights); dfree(b->dNodes); dfree(b->dRecs); dfree(b->dViews); dfree(b->dViewBase);
+    dfree(b->dTris); dfree(b->dLights); dfree(b->dNodes); dfree(b->dRecs); dfree(b->dViews); dfree(b->dViewBase);
     dfree(b->dRoots); dfree(b->dSamples); dfree(b->dSumRad); dfree(b->dSumAlb); dfree(b->dSumNrm); dfree(b->dPixels);
     dfree(b->dThr); dfree(b->dRad); dfree(b->dAlb); dfree(b->dNrm); dfree(b->dNee); dfree(b->dRayA); dfree(b->dRayB);
     dfree(b->dShadow); dfree(b->dHits); dfree(b->dCounters); dfree(b->dSpill); dfree(b->dOverflow);
@@ -188,8 +187,6 @@ bool build(Batch* b) {
     b->totalTris = (long long)tris.size();
     if (!dalloc(b, b->dTris, tris.size()) || !dalloc(b, b->dLights, lights.size())) return false;
     B_TRY(hipMemcpy(b->dTris, tris.data(), tris.size() * sizeof(Triangle), hipMemcpyHostToDevice));
-    if (!dalloc(b, b->dSurf, tris.size())) return false;
-    B_TRY(launchSurfaces(b->dTris, (int)tris.size(), b->dSurf, b->stream));
     if (!lights.empty()) B_TRY(hipMemcpy(b->dLights, lights.data(), lights.size() * sizeof(Light), hipMemcpyHostToDevice));
     // Per-scene GPU builds, then one global node / record array.
     std::vector<GpuBuildResult> res(b->views);
@@ -261,7 +258,6 @@ bool build(Batch* b) {
         BatchView& V = hv[v];
         V.cam = pv.cam; V.rs = pv.rs; V.rs.raysPerPixel = 1;
         V.sc.tris = b->dTris; V.sc.lights = b->dLights + lightOff[v]; V.sc.lightCount = lightCount[v];
-        V.sc.surf = b->dSurf;
         V.sc.lightArea = lightArea[v];
         V.w = pv.w; V.h = pv.h; V.slotBase = b->viewBase[v];
         V.sumRad = b->dSumRad + b->viewBase[v]; V.sumAlb = b->dSumAlb + b->viewBase[v];

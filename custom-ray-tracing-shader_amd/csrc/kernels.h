@@ -80,8 +80,6 @@ struct TraceLaunch {
 };
 hipError_t launchTrace(const TraceLaunch& l, bool anyHit, bool stats, int blocks, const TraceArgs& a, hipStream_t s);
 hipError_t launchResetCounters(int* counters, int initialRays, hipStream_t s);
-// Per-triangle shading records (surfaceOf) for count triangles.
-hipError_t launchSurfaces(const Triangle* tris, int count, Surface* out, hipStream_t s);
 hipError_t launchRaygen(const FrameArgs& f, hipStream_t s);
 hipError_t launchShade(const FrameArgs& f, int blocks, hipStream_t s);
 hipError_t launchAccumulate(const AccumArgs& a, hipStream_t s);

@@ -407,16 +407,6 @@ hipError_t launchTrace(const TraceLaunch& l, bool anyHit, bool stats, int blocks
 }
 
 // ------------------------------------------------------------------ frame kernels
-__global__ void surfaceKernel(const Triangle* tris, int count, Surface* out) {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i < count) out[i] = surfaceOf(tris[i]);
-}
-hipError_t launchSurfaces(const Triangle* tris, int count, Surface* out, hipStream_t s) {
-    if (count <= 0) return hipSuccess;
-    hipLaunchKernelGGL(surfaceKernel, dim3((count + kBlock - 1) / kBlock), dim3(kBlock), 0, s, tris, count, out);
-    return hipGetLastError();
-}
-
 __global__ void resetCountersKernel(int* c, int initialRays) {
     for (int k = threadIdx.x; k < kCounterStride; k += blockDim.x) c[k] = 0;
     __syncthreads();

@@ -221,26 +221,11 @@ static_assert(sizeof(TriRecord) == 48, "tri record");
 // ------------------------------------------------------------------ integrator pieces
 struct Light { int triangle; float cumulativeArea; };   // Native/renderer.cu:32
 
-// Shading record of one triangle, 96 bytes (six 16-byte vectors): what the integrator
-// reads at a hit, with the per-triangle terms (geometric normal, gloss exponent and its
-// sampling exponent) computed once by surfaceOf, with the same operations the integrator
-// would apply per hit.
-struct alignas(16) Surface {
-    float gx, gy, gz, emission;         // unitv(cross(edge1, edge2)), not flipped
-    float cr, cg, cb, specularProbability;
-    float sr, sg, sb, smoothness;
-    float n0x, n0y, n0z, gloss;         // glossExponent
-    float n1x, n1y, n1z, sampleExp;     // 1 / (gloss + 1)
-    float n2x, n2y, n2z, pad;
-};
-static_assert(sizeof(Surface) == 96, "surface record");
-
 struct SceneView {
     const Triangle* tris;   // colours already linear
     const Light* lights;
     int lightCount;
     float lightArea;
-    const Surface* surf;    // per-triangle shading records (GPU), or null: derive from tris
 };
 
 // Native/renderer.cu:84,86-90
@@ -259,31 +244,15 @@ RT_HD f3 aroundAxis(f3 axis, float cosine, float angle) {
 }
 RT_HD float powerHeuristic(float a, float b) { return a * a / fmaxf(a * a + b * b, 1e-30f); }
 // Native/renderer.cu:181-192
-RT_HD float glossExponent(float smoothness) { float r = fmaxf(1 - smoothness, .015f); return fmaxf(0, 2 / (r * r) - 2); }
-
-RT_HD Surface surfaceOf(const Triangle& t) {
-    Surface s;
-    const f3 g = unitv(crossv(V(t.edge1), V(t.edge2)));
-    s.gx = g.x; s.gy = g.y; s.gz = g.z; s.emission = t.emission;
-    s.cr = t.color.x; s.cg = t.color.y; s.cb = t.color.z; s.specularProbability = t.specularProbability;
-    s.sr = t.specularColor.x; s.sg = t.specularColor.y; s.sb = t.specularColor.z; s.smoothness = t.smoothness;
-    s.gloss = glossExponent(t.smoothness);
-    s.sampleExp = 1 / (s.gloss + 1);
-    s.n0x = t.normal0.x; s.n0y = t.normal0.y; s.n0z = t.normal0.z;
-    s.n1x = t.normal1.x; s.n1y = t.normal1.y; s.n1z = t.normal1.z;
-    s.n2x = t.normal2.x; s.n2y = t.normal2.y; s.n2z = t.normal2.z; s.pad = 0;
-    return s;
-}
-RT_HD Surface surfaceAt(const SceneView& sc, int i) { return sc.surf ? sc.surf[i] : surfaceOf(sc.tris[i]); }
-
-RT_HD f3 evalBrdf(const Surface& t, f3 n, f3 refl, f3 dir, float& pdf) {
+RT_HD float glossExponent(const Triangle& t) { float r = fmaxf(1 - t.smoothness, .015f); return fmaxf(0, 2 / (r * r) - 2); }
+RT_HD f3 evalBrdf(const Triangle& t, f3 n, f3 refl, f3 dir, float& pdf) {
     float c = fmaxf(0, dotv(n, dir)), p = t.specularProbability;
     pdf = (1 - p) * c / kPi;
-    f3 val = F3(t.cr, t.cg, t.cb) * ((1 - p) / kPi);
+    f3 val = V(t.color) * ((1 - p) / kPi);
     if (p > 0 && t.smoothness < .999f) {
-        float e = t.gloss, lobe = powf(fmaxf(0, dotv(refl, dir)), e);
+        float e = glossExponent(t), lobe = powf(fmaxf(0, dotv(refl, dir)), e);
         pdf += p * (e + 1) * lobe / (2 * kPi);
-        val = val + F3(t.sr, t.sg, t.sb) * (p * (e + 2) * lobe / (2 * kPi));
+        val = val + V(t.specularColor) * (p * (e + 2) * lobe / (2 * kPi));
     }
     return val;
 }
@@ -337,24 +306,23 @@ RT_HD void shadeBounce(const SceneView& sc, const RenderSettings& rs, int bounce
         if (bounce > 0) ps.rad = ps.rad + mulv(ps.thr, skyRadiance(d, rs.skyStrength));
         return;
     }
-    const Surface t = surfaceAt(sc, hit.index);
-    f3 g = F3(t.gx, t.gy, t.gz);
+    const Triangle& t = sc.tris[hit.index];
+    f3 g = unitv(crossv(V(t.edge1), V(t.edge2)));
     if (dotv(g, d) > 0) g = g * -1;
-    const f3 color = F3(t.cr, t.cg, t.cb), specColor = F3(t.sr, t.sg, t.sb);
     if (bounce == 0) {
         first->covered = true;
         first->albedo = t.emission > 0 ? F3(1, 1, 1)
-                                       : color * (1 - t.specularProbability) + specColor * t.specularProbability;
+                                       : V(t.color) * (1 - t.specularProbability) + V(t.specularColor) * t.specularProbability;
         first->normal = g;
         first->depth = hit.distance;
     }
     if (t.emission > 0) {
         float lp = sc.lightArea > 0 ? hit.distance * hit.distance / (sc.lightArea * fmaxf(1e-6f, -dotv(g, d))) : 0;
         float w = ps.prevPdf > 0 ? powerHeuristic(ps.prevPdf, lp) : 1;
-        ps.rad = ps.rad + mulv(ps.thr, color) * (t.emission * w);
+        ps.rad = ps.rad + mulv(ps.thr, V(t.color)) * (t.emission * w);
         return;
     }
-    f3 n = unitv(F3(t.n0x, t.n0y, t.n0z) * (1 - hit.u - hit.v) + (F3(t.n1x, t.n1y, t.n1z) * hit.u + F3(t.n2x, t.n2y, t.n2z) * hit.v));
+    f3 n = unitv(V(t.normal0) * (1 - hit.u - hit.v) + (V(t.normal1) * hit.u + V(t.normal2) * hit.v));
     if (dotv(n, g) < 0) n = n * -1;
     if (dotv(n, d) >= 0) n = g;
     if (bounce == 0) first->normal = n;
@@ -371,14 +339,12 @@ RT_HD void shadeBounce(const SceneView& sc, const RenderSettings& rs, int bounce
             int mid = (lo + hi) >> 1;
             if (sel >= sc.lights[mid].cumulativeArea) lo = mid + 1; else hi = mid;
         }
-        const int li = sc.lights[lo].triangle;
-        const Triangle& L = sc.tris[li];
+        const Triangle& L = sc.tris[sc.lights[lo].triangle];
         float r = sqrtf(rng(1)), vv = rng(2);
         f3 target = V(L.origin) + (V(L.edge1) * (r * (1 - vv)) + V(L.edge2) * (r * vv));
         f3 off = target - no;
         float dist2 = dotv(off, off), dist = sqrtf(dist2);
-        const Surface ls = surfaceAt(sc, li);
-        f3 in = off * (1 / dist), ln = F3(ls.gx, ls.gy, ls.gz);
+        f3 in = off * (1 / dist), ln = unitv(crossv(V(L.edge1), V(L.edge2)));
         float lc = -dotv(ln, in);
         if (!L.oneSided) lc = fabsf(lc);
         float sc2 = fmaxf(0, dotv(n, in));
@@ -389,16 +355,16 @@ RT_HD void shadeBounce(const SceneView& sc, const RenderSettings& rs, int bounce
             res.shadow = true;
             res.sdir = in;
             res.slimit = dist - .001f;
-            res.contrib = mulv(mulv(ps.thr, f), F3(ls.cr, ls.cg, ls.cb)) * (ls.emission * sc2 * w / lp);
+            res.contrib = mulv(mulv(ps.thr, f), V(L.color)) * (L.emission * sc2 * w / lp);
         }
     }
     if (bounce + 1 == rs.maxBounces) return;
     bool spec = rng(3) < t.specularProbability;
     f3 nd;
     if (spec && t.smoothness >= .999f) {
-        nd = refl; ps.thr = mulv(ps.thr, specColor); ps.prevPdf = 0;
+        nd = refl; ps.thr = mulv(ps.thr, V(t.specularColor)); ps.prevPdf = 0;
     } else {
-        float cosine = spec ? powf(rng(4), t.sampleExp) : sqrtf(rng(4));
+        float cosine = spec ? powf(rng(4), 1 / (glossExponent(t) + 1)) : sqrtf(rng(4));
         nd = aroundAxis(spec ? refl : n, cosine, 2 * kPi * rng(5));
         if (dotv(g, nd) <= 0 || dotv(n, nd) <= 0) return;
         float pdf;
