@@ -273,7 +273,7 @@ bool renderGpu(Session* s, const CameraData& cam, const RenderSettings& rs, floa
             f.bounce = b; f.rayIn = in; f.rayOut = out;
             {
                 StageScope sc(s, RT_STAGE_SHADE);
-                if ((s->error = launchShade(f, s->cus * 8, st)) != hipSuccess) return false;
+                if ((s->error = launchShade(f, s->cus * shadeBlocksPerCU(), st)) != hipSuccess) return false;
             }
             if (!s->lights.empty()) {
                 TraceArgs sa = baseTraceArgs(s);

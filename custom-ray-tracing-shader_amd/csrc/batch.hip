@@ -141,7 +141,7 @@ bool enqueueFrame(Batch* b) {
         ta.hits = b->dHits;
         B_TRY(launchTrace(launchOf(b), false, false, b->traceBlocks, ta, st));
         a.bounce = bounce; a.rayIn = in; a.rayOut = out;
-        B_TRY(launchBatchShade(a, b->cus * 8, st));
+        B_TRY(launchBatchShade(a, b->cus * shadeBlocksPerCU(), st));
         sa.rays = b->dShadow; sa.countPtr = b->dCounters + kCntShadow + bounce; sa.work = b->dCounters + kWorkShadow + bounce;
         sa.rad = b->dRad; sa.contrib = b->dNee;
         B_TRY(launchTrace(launchOf(b), true, false, b->traceBlocks, sa, st));

@@ -82,6 +82,7 @@ hipError_t launchTrace(const TraceLaunch& l, bool anyHit, bool stats, int blocks
 hipError_t launchResetCounters(int* counters, int initialRays, hipStream_t s);
 hipError_t launchRaygen(const FrameArgs& f, hipStream_t s);
 hipError_t launchShade(const FrameArgs& f, int blocks, hipStream_t s);
+int shadeBlocksPerCU();   // resident shade workgroups per compute unit
 hipError_t launchAccumulate(const AccumArgs& a, hipStream_t s);
 hipError_t launchSobolBatch(const unsigned* pixels, const unsigned* samples, int count, unsigned seed, int dims,
                             float* out, hipStream_t s);

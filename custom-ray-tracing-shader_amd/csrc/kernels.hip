@@ -486,8 +486,10 @@ __device__ __forceinline__ void appendRay(bool pred, int* counter, float4* queue
 }
 
 // Grid-stride over the bounce's queue, so the grid is sized to the GPU, not to the largest
-// possible queue; every lane of a wave reaches the appends in each round.
-__global__ __launch_bounds__(kBlock) void shadeKernel(FrameArgs f) {
+// possible queue; every lane of a wave reaches the appends in each round.  Capped at 96
+// VGPRs so five workgroups fit on a compute unit without spilling.
+constexpr int kShadeBlocksPerCU = 5;
+__global__ __launch_bounds__(kBlock, kShadeBlocksPerCU) void shadeKernel(FrameArgs f) {
     __shared__ unsigned sShift[8], sTerm[kMaxRaysPerPixel * 8];
     loadSobolTerms<8>(sShift, sTerm, f.rs.seed, 4 + 8 * f.bounce, f.sample0, f.R);
     const int count = f.counters[kCntRays + f.bounce];
@@ -532,6 +534,7 @@ __global__ __launch_bounds__(kBlock) void shadeKernel(FrameArgs f) {
         appendRay(wantNext, &f.counters[kCntRays + f.bounce + 1], f.rayOut, na, nb);
     }
 }
+int shadeBlocksPerCU() { return kShadeBlocksPerCU; }
 hipError_t launchShade(const FrameArgs& f, int blocks, hipStream_t s) {
     const int n = f.R * f.npix;
     const int need = (n + kBlock - 1) / kBlock;
@@ -595,7 +598,7 @@ hipError_t launchBatchRaygen(const BatchArgs& b, hipStream_t s) {
     return hipGetLastError();
 }
 
-__global__ __launch_bounds__(kBlock) void batchShadeKernel(BatchArgs b) {
+__global__ __launch_bounds__(kBlock, kShadeBlocksPerCU) void batchShadeKernel(BatchArgs b) {
     const int count = b.counters[kCntRays + b.bounce];
     for (int base = blockIdx.x * kBlock; base < count; base += gridDim.x * kBlock) {
         const int i = base + threadIdx.x;
