@@ -12,6 +12,7 @@
 #include "traverse.h"
 #include "bvh_gpu.h"
 #include "denoiser.h"
+#include "embree_cpu.h"
 
 using namespace rt;
 
@@ -24,6 +25,7 @@ struct Session {
     hipError_t error = hipSuccess;
     // scene (colours linearised), light CDF, BVH
     std::vector<Triangle> tris;
+    EmbreeScene embree;          // backend 3, built on its first render
     std::vector<Light> lights;
     float lightArea = 0;
     std::vector<RtBvhNode> binNodes;   // host copies, fetched on demand
@@ -332,11 +334,11 @@ bool validRender(Session* s, const CameraData* camera, const RenderSettings* set
 }
 
 // Native/renderer.cu:289-302 on the host, over the same wide BVH as the GPU.
-template <class N>
+// trace(o, d, limit, anyHit) answers the queries: the session's BVH (backends 0, 1) or
+// Embree (backend 3).
+template <class Tracer>
 void renderCpuRows(Session* s, const CameraData& cam, const RenderSettings& rs, unsigned char* out, bool multi,
-                   unsigned long long& closest, unsigned long long& shadow) {
-    const N* nodes = reinterpret_cast<const N*>(s->hostNodes.data());
-    const TriRecord* recs = s->hostRecs.data();
+                   const Tracer& trace, unsigned long long& closest, unsigned long long& shadow) {
     SceneView sc; sc.tris = s->tris.data(); sc.lights = s->lights.data(); sc.lightCount = (int)s->lights.size(); sc.lightArea = s->lightArea;
     const int w = s->width, h = s->height;
     const unsigned* dirs = hostSobolDirections();
@@ -362,13 +364,13 @@ void renderCpuRows(Session* s, const CameraData& cam, const RenderSettings& rs, 
                     FirstHit fh; fh.covered = false; fh.albedo = fh.normal = F3(0, 0, 0); fh.depth = 0;
                     for (int b = 0; b < rs.maxBounces; b++) {
                         base = 4 + 8 * (unsigned)b;
-                        RtHit hit = traverseHost<N, false>(nodes, recs, o, d, 1e30f);
+                        RtHit hit = trace(o, d, 1e30f, false);
                         cc++;
                         ShadeResult res;
                         shadeBounce(sc, rs, b, rng, o, d, hit, ps, &fh, res);
                         if (res.shadow) {
                             ss++;
-                            if (traverseHost<N, true>(nodes, recs, res.nextO, res.sdir, res.slimit).index < 0)
+                            if (trace(res.nextO, res.sdir, res.slimit, true).index < 0)
                                 ps.rad = ps.rad + res.contrib;
                         }
                         if (!res.cont) break;
@@ -394,6 +396,17 @@ void renderCpuRows(Session* s, const CameraData& cam, const RenderSettings& rs, 
     closest = cc; shadow = ss;
 }
 
+template <class N>
+void renderBvhRows(Session* s, const CameraData& cam, const RenderSettings& rs, unsigned char* out, bool multi,
+                   unsigned long long& closest, unsigned long long& shadow) {
+    const N* nodes = reinterpret_cast<const N*>(s->hostNodes.data());
+    const TriRecord* recs = s->hostRecs.data();
+    auto trace = [nodes, recs](f3 o, f3 d, float limit, bool any) {
+        return any ? traverseHost<N, true>(nodes, recs, o, d, limit) : traverseHost<N, false>(nodes, recs, o, d, limit);
+    };
+    renderCpuRows(s, cam, rs, out, multi, trace, closest, shadow);
+}
+
 }  // namespace
 
 // ======================================================================== C ABI
@@ -407,7 +420,7 @@ void RtReset(void* p) {
 }
 unsigned RtAccumulationVersion(void* p) { Session* s = static_cast<Session*>(p); return s ? s->version : 0; }
 unsigned RtSampleCount(void* p) { Session* s = static_cast<Session*>(p); return s ? s->samples : 0; }
-float RtEmbreeBuildMilliseconds(void* p) { (void)p; return 0; }
+float RtEmbreeBuildMilliseconds(void* p) { Session* s = static_cast<Session*>(p); return s ? s->embree.buildMs : 0; }
 int RtCpuThreadCount(void* p) { Session* s = static_cast<Session*>(p); return s ? s->cpuThreads : 0; }
 
 void RtBvhInfo(void* p, int* nodes, int* depth, float* buildMs) {
@@ -427,6 +440,7 @@ void RtShutdown(void* p) {
     Session* s = static_cast<Session*>(p);
     if (!s) return;
     s->hostError.clear();
+    s->embree.release();   // before the triangles its filter reads go away
     freeDevice(s);
     if (s->ownDenoiser) denoiserDestroy(s->denoiser);
     if (s->ownDenoiser) s->denoiser = nullptr;
@@ -504,11 +518,10 @@ int RtRender(void* p, const CameraData* camera, const RenderSettings* settings, 
         return 0;
     }
     s->hostError.clear();
-    if (backend == RT_BACKEND_EMBREE) {
-        // Backend 3 wraps Embree 3 (libembree3.so.3), which is not part of this build's
-        // environment; fail like the reference does when the library cannot be loaded.
-        s->hostError = "Embree backend unavailable: libembree3.so.3 is not installed";
-        return 0;
+    if (backend == RT_BACKEND_EMBREE && !s->embree.scene) {
+        // Built on first use, as the reference's backend 3 (Native/renderer.cu:391-395); fails
+        // with the loader's message when libembree3.so.3 is not available.
+        if (!s->embree.build(s->tris, s->hostError)) { s->embree.release(); return 0; }
     }
     if (s->lastBackend != backend || std::memcmp(camera, &s->lastCamera, sizeof(CameraData)) ||
         std::memcmp(settings, &s->lastSettings, sizeof(RenderSettings)))
@@ -518,19 +531,27 @@ int RtRender(void* p, const CameraData* camera, const RenderSettings* settings, 
         if (!renderGpu(s, *camera, *settings, computeMs)) return 0;
         s->error = hipMemcpy(output, s->dPixels, (size_t)s->width * s->height * 4, hipMemcpyDeviceToHost);
         if (s->error != hipSuccess) return 0;
+    } else if (backend == RT_BACKEND_EMBREE) {
+        auto t0 = std::chrono::steady_clock::now();
+        const EmbreeScene* em = &s->embree;
+        auto trace = [em](f3 o, f3 d, float limit, bool any) { return em->query(o, d, limit, any); };
+        unsigned long long c = 0, sh = 0;
+        renderCpuRows(s, *camera, *settings, output, true, trace, c, sh);
+        s->lastClosest = c; s->lastShadow = sh;
+        *computeMs = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
     } else {
         if (!fetchHostAccel(s)) return 0;
         auto t0 = std::chrono::steady_clock::now();
         const bool multi = backend == RT_BACKEND_CPU_MULTI;
         unsigned long long c = 0, sh = 0;
         if (s->builtQuantized) {
-            if (s->builtWidth == 2) renderCpuRows<QNode<2>>(s, *camera, *settings, output, multi, c, sh);
-            else if (s->builtWidth == 8) renderCpuRows<QNode<8>>(s, *camera, *settings, output, multi, c, sh);
-            else renderCpuRows<QNode<4>>(s, *camera, *settings, output, multi, c, sh);
+            if (s->builtWidth == 2) renderBvhRows<QNode<2>>(s, *camera, *settings, output, multi, c, sh);
+            else if (s->builtWidth == 8) renderBvhRows<QNode<8>>(s, *camera, *settings, output, multi, c, sh);
+            else renderBvhRows<QNode<4>>(s, *camera, *settings, output, multi, c, sh);
         } else {
-            if (s->builtWidth == 2) renderCpuRows<WideNode<2>>(s, *camera, *settings, output, multi, c, sh);
-            else if (s->builtWidth == 8) renderCpuRows<WideNode<8>>(s, *camera, *settings, output, multi, c, sh);
-            else renderCpuRows<WideNode<4>>(s, *camera, *settings, output, multi, c, sh);
+            if (s->builtWidth == 2) renderBvhRows<WideNode<2>>(s, *camera, *settings, output, multi, c, sh);
+            else if (s->builtWidth == 8) renderBvhRows<WideNode<8>>(s, *camera, *settings, output, multi, c, sh);
+            else renderBvhRows<WideNode<4>>(s, *camera, *settings, output, multi, c, sh);
         }
         s->lastClosest = c; s->lastShadow = sh;
         *computeMs = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();

@@ -62,6 +62,50 @@ def test_cornell_gpu_matches_oracle(lib):
     assert abs(closest - rc) <= 1e-3 * rc and abs(shadow - rs) <= 1e-3 * rs
 
 
+def _glossy_cornell(smoothness):
+    from scenes import cornell, srgb_encode
+    tris = cornell()
+    boxes = tris[12:]
+    boxes["specularProbability"] = 1.0
+    boxes["smoothness"] = smoothness
+    boxes["specularColor"]["x"], boxes["specularColor"]["y"], boxes["specularColor"]["z"] = srgb_encode((0.9, 0.8, 0.6))
+    tris[12:] = boxes
+    return tris
+
+
+@pytest.mark.parametrize("case", ["orthographic", "thin_lens", "sky_exposure", "glossy", "mirror", "deep"])
+def test_cornell_variants_match_oracle(lib, case):
+    """Camera models, sky and exposure, glossy and mirror materials and a long path limit,
+    each at identical sample indices against the oracle (Native/renderer.cu:181-288)."""
+    from scenes import cornell, make_camera, make_settings
+    w, h = 96, 72
+    tris = cornell()
+    cam = make_camera((0, 1, -3.4), (0, 1, 1), fov_deg=40, aspect=w / h)
+    st = make_settings(4, rpp=8)
+    if case == "orthographic":
+        cam = make_camera((0, 1, -3.4), (0, 1, 1), aspect=w / h, orthographic=1, ortho_size=1.2)
+    elif case == "thin_lens":
+        st = make_settings(4, rpp=8, aperture=0.08, focus=3.0)
+    elif case == "sky_exposure":
+        st = make_settings(5, rpp=8, sky=0.6, exposure=2.5)
+    elif case == "glossy":
+        tris = _glossy_cornell(0.7)
+    elif case == "mirror":
+        tris = _glossy_cornell(1.0)
+    elif case == "deep":
+        st = make_settings(24, rpp=4, seed=99)
+    with Session(lib) as s:
+        s.initialize(tris, w, h)
+        img, _ = s.render(cam, st, 2)
+        acc = s.accumulation()
+        closest, shadow = s.ray_counts()
+    ref_img, ref_acc, (rc, rs) = oracle_frames(tris, cam, st, w, h, 1)
+    assert hdr_mean_relative_error(acc, ref_acc) <= 1e-3
+    mae, over1, _ = image_agreement(img, ref_img)
+    assert mae <= 0.5 and over1 <= 0.01
+    assert abs(closest - rc) <= 1e-3 * rc and abs(shadow - rs) <= 1e-3 * max(rs, 1)
+
+
 # ---------------------------------------------------------------- Tools/verify_native.py
 LIT = dict(maxBounces=8, exposure=1, aperture=0, focusDistance=3, skyStrength=0, seed=42, raysPerPixel=1)
 
