@@ -83,6 +83,50 @@ struct LdsStack {
     }
 };
 
+// Entry distances and references of a node's children.
+template <class N>
+__device__ __forceinline__ void childEntries(const N* nodes, const LaneRay& L, float* dist, int* cr) {
+    constexpr int W = NodeTraits<N>::kWidth;
+    N n;
+    loadNode<N>(nodes, L.ref, n);
+    if (L.nz) {
+#pragma unroll
+        for (int c = 0; c < W; c++) dist[c] = n.entryNZ(c, L.o, L.inv, L.best);
+    } else {
+#pragma unroll
+        for (int c = 0; c < W; c++) dist[c] = n.entry(c, L.o, L.inv, L.best);
+    }
+#pragma unroll
+    for (int c = 0; c < W; c++) cr[c] = n.child[c];
+}
+// The 256-byte node in two halves of four children: the second half's loads are issued
+// after the first half's slab tests (they hit in L1: both halves span the same two cache
+// lines), so only half the node's registers are live at once.
+template <>
+__device__ __forceinline__ void childEntries<WideNode<8>>(const WideNode<8>* nodes, const LaneRay& L, float* dist,
+                                                         int* cr) {
+    const float4* p = reinterpret_cast<const float4*>(nodes + L.ref);
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+        const float4 lx = p[h], hx = p[2 + h], ly = p[4 + h], hy = p[6 + h], lz = p[8 + h], hz = p[10 + h];
+        const int4 ch = reinterpret_cast<const int4*>(p)[12 + h];
+        const float lxa[4] = {lx.x, lx.y, lx.z, lx.w}, hxa[4] = {hx.x, hx.y, hx.z, hx.w};
+        const float lya[4] = {ly.x, ly.y, ly.z, ly.w}, hya[4] = {hy.x, hy.y, hy.z, hy.w};
+        const float lza[4] = {lz.x, lz.y, lz.z, lz.w}, hza[4] = {hz.x, hz.y, hz.z, hz.w};
+        if (L.nz) {
+#pragma unroll
+            for (int c = 0; c < 4; c++)
+                dist[4 * h + c] = boxEntryNZ(L.o, L.inv, lxa[c], lya[c], lza[c], hxa[c], hya[c], hza[c], L.best);
+        } else {
+#pragma unroll
+            for (int c = 0; c < 4; c++)
+                dist[4 * h + c] = boxEntry(L.o, L.inv, lxa[c], lya[c], lza[c], hxa[c], hya[c], hza[c], L.best);
+        }
+        cr[4 * h] = ch.x; cr[4 * h + 1] = ch.y; cr[4 * h + 2] = ch.z; cr[4 * h + 3] = ch.w;
+        __builtin_amdgcn_sched_barrier(0);
+    }
+}
+
 // One traversal step: visit an inner node (push far children, descend to the nearest) or
 // test a leaf, then pop until a deferred entry survives the current best distance.
 // Returns true when the ray is finished.
@@ -92,19 +136,8 @@ __device__ __forceinline__ bool traceStep(LaneRay& L, const N* nodes, const floa
     constexpr int W = NodeTraits<N>::kWidth;
     if (L.ref >= 0) {
         if (STATS) nv++;
-        // Each path loads the node itself, so no node register lives across both.
         float dist[W]; int cr[W];
-        if (L.nz) {
-            N n;
-            loadNode<N>(nodes, L.ref, n);
-#pragma unroll
-            for (int c = 0; c < W; c++) { dist[c] = n.entryNZ(c, L.o, L.inv, L.best); cr[c] = n.child[c]; }
-        } else {
-            N n;
-            loadNode<N>(nodes, L.ref, n);
-#pragma unroll
-            for (int c = 0; c < W; c++) { dist[c] = n.entry(c, L.o, L.inv, L.best); cr[c] = n.child[c]; }
-        }
+        childEntries<N>(nodes, L, dist, cr);
         sortChildren<W>(dist, cr);
 #pragma unroll
         for (int c = W - 1; c >= 1; c--)
@@ -175,17 +208,7 @@ __device__ __forceinline__ void visitNode(LaneRay& L, const N* nodes, LdsStack<S
     constexpr int W = NodeTraits<N>::kWidth;
     if (STATS) nv++;
     float dist[W]; int cr[W];
-    N n;
-    loadNode<N>(nodes, L.ref, n);
-    if (L.nz) {
-#pragma unroll
-        for (int c = 0; c < W; c++) dist[c] = n.entryNZ(c, L.o, L.inv, L.best);
-    } else {
-#pragma unroll
-        for (int c = 0; c < W; c++) dist[c] = n.entry(c, L.o, L.inv, L.best);
-    }
-#pragma unroll
-    for (int c = 0; c < W; c++) cr[c] = n.child[c];
+    childEntries<N>(nodes, L, dist, cr);
     sortChildren<W>(dist, cr);
 #pragma unroll
     for (int c = W - 1; c >= 1; c--)
