@@ -193,6 +193,20 @@ def psnr_study(args):
     sec, img = min(denoised, key=lambda x: x[0])
     tiles, rejected, infer_ms = s.denoise_stats()
     dn_psnr = td.psnr(img, ref_img)
+    # Denoised frames at more samples: the smallest count whose denoised PSNR reaches plain
+    # accumulation at 256 spp, and how much sooner it gets there.
+    dn_ladder = []
+    for spp in (8, 16, 32, 64):
+        best = None
+        for _ in range(2):
+            t = time.perf_counter()
+            plain(spp)
+            im, _ = s.denoise()
+            el = time.perf_counter() - t
+            best = el if best is None else min(best, el)
+        dn_ladder.append({"spp": spp, "seconds": round(best, 4), "psnr_db": round(td.psnr(im, ref_img), 3),
+                          "tiles_fallback": s.denoise_stats()[1]})
+        log(f"denoised {spp:3d} spp: {best * 1e3:8.1f} ms, PSNR {dn_ladder[-1]['psnr_db']:.2f} dB")
     hp, wp = (h + 15) // 16 * 16, (w + 15) // 16 * 16
     flops = lib.RtDenoiserForwardFlops(1, hp, wp)
     # equal-PSNR point on the plain ladder (linear in log2 spp, times interpolated alike)
@@ -216,7 +230,11 @@ def psnr_study(args):
               "equal_psnr_plain": eq,
               "time_to_equal_psnr_speedup": round(eq["seconds"] / sec, 2) if eq else None,
               "vs_256spp": {"psnr_gap_db": round(dn_psnr - p256["psnr_db"], 3),
-                            "time_ratio": round(p256["seconds"] / sec, 2)}}
+                            "time_ratio": round(p256["seconds"] / sec, 2)},
+              "denoised_ladder": dn_ladder}
+    match = next((d for d in dn_ladder if d["psnr_db"] >= p256["psnr_db"]), None)
+    result["match_256spp"] = ({"denoised_spp": match["spp"], "seconds": match["seconds"],
+                               "time_ratio": round(p256["seconds"] / match["seconds"], 2)} if match else None)
     s.close()
     print(json.dumps(result), flush=True)
     return 0
@@ -234,6 +252,7 @@ def main():
     ap.add_argument("--blocks-per-cu", type=int)
     ap.add_argument("--short-stack", type=int)
     ap.add_argument("--refill", type=int)
+    ap.add_argument("--refill-below", type=int)
     ap.add_argument("--build-max-leaf", type=int)
     ap.add_argument("--bvh-quantized", type=int, choices=(0, 1))
     ap.add_argument("--while-while", type=int, choices=(0, 1))
@@ -268,7 +287,8 @@ def main():
     desc, tris, cam, st, w, h = workload(args.workload, args.tris)
     log(f"workload: {desc}")
     cfg = dict(TUNED[args.workload])
-    for k in cfg:
+    for k in ("bvh_width", "bvh_quantized", "build_max_leaf", "blocks_per_cu", "short_stack", "refill", "refill_below",
+              "while_while"):
         if getattr(args, k) is not None:
             cfg[k] = getattr(args, k)
     if args.autotune:
@@ -279,7 +299,8 @@ def main():
         cfg = {k: best[k] for k in cfg}
         log(f"autotune: {len(trials)} configurations in {time.perf_counter() - t0:.1f} s, best {best}")
     s = native_api.Session()
-    for k in ("bvh_width", "bvh_quantized", "build_max_leaf", "blocks_per_cu", "short_stack", "refill", "while_while"):
+    for k in ("bvh_width", "bvh_quantized", "build_max_leaf", "blocks_per_cu", "short_stack", "refill", "refill_below",
+              "while_while"):
         if k in cfg:
             s.set_option(k, cfg[k])
     t0 = time.perf_counter()

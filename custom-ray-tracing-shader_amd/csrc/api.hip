@@ -43,6 +43,7 @@ struct Session {
     int blocksPerCU = 8;         // persistent traversal workgroups per CU
     int shortStack = 8;          // LDS stack entries per traversal lane (8 or 16)
     int refill = 1;              // per-lane ray refill in the traversal kernel
+    int refillBelow = 40;        // busy lanes below which a refilling wave fetches rays
     int whileWhile = 1;          // speculative while-while traversal loop
     int tiledPrimary = 1;        // primary rays queued in 8x8 pixel tiles
     int nontemporal = 1;         // rays / hits bypass cache retention in traversal
@@ -199,7 +200,7 @@ bool fetchBinaryTree(Session* s) {
 TraceLaunch traceLaunch(const Session* s) {
     TraceLaunch l;
     l.width = s->builtWidth; l.quantized = s->builtQuantized != 0; l.stack = s->shortStack;
-    l.refill = s->refill != 0; l.whileWhile = s->whileWhile != 0;
+    l.refill = s->refill != 0; l.whileWhile = s->whileWhile != 0; l.refillBelow = s->refillBelow;
     return l;
 }
 
@@ -714,7 +715,8 @@ int RtDenoise(void* p, unsigned char* output, int outputBytes, float* millisecon
 int RtTraceRays(void* p, const RtRay* rays, int count, int anyHit, RtHit* hits) {
     Session* s = static_cast<Session*>(p);
     if (!s) return 0;
-    if (!s->ready || !rays || !hits || count < 0) { s->error = hipErrorInvalidValue; return 0; }
+    // The traversal's int ray cursor runs past the count by up to 64 rays per wave.
+    if (!s->ready || !rays || !hits || count < 0 || count > (1 << 30)) { s->error = hipErrorInvalidValue; return 0; }
     if (count == 0) return 1;
     if (!prepareTraceGrid(s)) return 0;
     float4* dr = nullptr; int4* dh = nullptr; int* work = nullptr;
@@ -931,6 +933,10 @@ int RtSetOption(void* p, const char* key, int value) {
         if (value != 0 && value != 1) return 0;
         s->refill = value; return 1;
     }
+    if (!std::strcmp(key, "refill_below")) {
+        if (value < 1 || value > 64) return 0;
+        s->refillBelow = value; return 1;
+    }
     if (!std::strcmp(key, "while_while")) {
         if (value != 0 && value != 1) return 0;
         s->whileWhile = value; return 1;
@@ -967,6 +973,7 @@ int RtGetOption(void* p, const char* key) {
     if (!std::strcmp(key, "short_stack")) return s->shortStack;
     if (!std::strcmp(key, "denoise_fallback")) return s->denoiseFallback;
     if (!std::strcmp(key, "refill")) return s->refill;
+    if (!std::strcmp(key, "refill_below")) return s->refillBelow;
     if (!std::strcmp(key, "while_while")) return s->whileWhile;
     if (!std::strcmp(key, "tiled_primary")) return s->tiledPrimary;
     if (!std::strcmp(key, "nontemporal")) return s->nontemporal;

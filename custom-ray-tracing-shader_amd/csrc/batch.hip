@@ -53,7 +53,8 @@ struct PendingView {
 struct Batch {
     std::vector<PendingView> pending;
     std::string error;
-    int width = 4, quantized = 0, blocksPerCU = 8, shortStack = 8, refill = 1, whileWhile = 1, useGraph = 1;
+    int width = 4, quantized = 0, blocksPerCU = 8, shortStack = 8, refill = 1, refillBelow = 40, whileWhile = 1,
+        useGraph = 1;
     bool built = false;
     int cus = 0, views = 0, totalSlots = 0, maxBounces = 0;
     long long totalTris = 0;
@@ -119,7 +120,7 @@ BatchArgs batchArgs(Batch* b) {
 TraceLaunch launchOf(const Batch* b) {
     TraceLaunch l;
     l.width = b->width; l.quantized = b->quantized != 0; l.stack = b->shortStack;
-    l.refill = b->refill != 0; l.whileWhile = b->whileWhile != 0;
+    l.refill = b->refill != 0; l.whileWhile = b->whileWhile != 0; l.refillBelow = b->refillBelow;
     return l;
 }
 
@@ -330,6 +331,7 @@ int RtBatchSetOption(void* p, const char* key, int value) {
     if (!std::strcmp(key, "blocks_per_cu") && value >= 1 && value <= 32) { b->blocksPerCU = value; return 1; }
     if (!std::strcmp(key, "short_stack") && (value == 8 || value == 16)) { b->shortStack = value; return 1; }
     if (!std::strcmp(key, "refill") && (value == 0 || value == 1)) { b->refill = value; return 1; }
+    if (!std::strcmp(key, "refill_below") && value >= 1 && value <= 64) { b->refillBelow = value; return 1; }
     if (!std::strcmp(key, "while_while") && (value == 0 || value == 1)) { b->whileWhile = value; return 1; }
     if (!std::strcmp(key, "use_graph") && (value == 0 || value == 1)) { b->useGraph = value; return 1; }
     return 0;

@@ -207,7 +207,7 @@ __global__ __launch_bounds__(kB) void convFwdTileKernel(ConvArgs a) {
 struct BwdWArgs {
     View dy;       // gradient w.r.t. the pre-activation output, coutPad channels
     View in;       // layer input, cinPad channels
-    float* part;   // [parts][9][coutPad][cinPad], accumulated with atomics
+    float* part;   // [9][coutPad][cinPad] weight gradient, accumulated with atomics
     int n, h, w;
     int cinPad, coutPad;
     int pixPerPart;
@@ -268,7 +268,7 @@ __global__ __launch_bounds__(kB) void convBwdWKernel(BwdWArgs a) {
         }
         __syncthreads();
     }
-    float* part = a.part + (size_t)(blockIdx.x * 9 + tap) * a.coutPad * a.cinPad;
+    float* part = a.part + (size_t)tap * a.coutPad * a.cinPad;   // the layer's gradient, zeroed before backward
 #pragma unroll
     for (int i = 0; i < TCO; i++)
 #pragma unroll
@@ -280,24 +280,24 @@ __global__ __launch_bounds__(kB) void convBwdWKernel(BwdWArgs a) {
             }
 }
 
-__global__ void reducePartsKernel(const float* part, int parts, long long size, float* out) {
-    for (long long i = blockIdx.x * (long long)kB + threadIdx.x; i < size; i += (long long)gridDim.x * kB) {
-        float s = 0;
-        for (int k = 0; k < parts; k++) s += part[(size_t)k * size + i];
-        out[i] = s;
-    }
-}
-
-// dBias[c] += sum over a part of the pixels of dy[p][c].
-__global__ void biasGradKernel(View dy, int npix, int channels, int pixPerPart, float* db) {
+// dBias[c] += sum over a part of the pixels of dy[p][c] (channels <= kB); the block's lanes
+// are combined in LDS so each block adds once per channel.
+__global__ __launch_bounds__(kB) void biasGradKernel(View dy, int npix, int channels, int pixPerPart, float* db) {
+    __shared__ float red[kB];
     const int c = threadIdx.x % channels;
     const int lanesPer = kB / channels;
     const int k = threadIdx.x / channels;
-    if (k >= lanesPer) return;
     const int begin = blockIdx.x * pixPerPart, end = min(npix, begin + pixPerPart);
     float s = 0;
-    for (int p = begin + k; p < end; p += lanesPer) s += (float)dy.p[(size_t)p * dy.cs + dy.co + c];
-    atomicAdd(&db[c], s);
+    if (k < lanesPer)
+        for (int p = begin + k; p < end; p += lanesPer) s += (float)dy.p[(size_t)p * dy.cs + dy.co + c];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    if (threadIdx.x < channels) {
+        float t = 0;
+        for (int j = 0; j < lanesPer; j++) t += red[j * channels + threadIdx.x];
+        atomicAdd(&db[threadIdx.x], t);
+    }
 }
 
 // ------------------------------------------------------------------ elementwise kernels
@@ -535,8 +535,6 @@ struct DenoiserNet {
     bf16 *gD0, *gD1, *gD2, *gD3, *ge0a, *gp0, *ge1a, *gp1, *ge2a, *gp2, *ge3a, *gp3, *gba, *gbb, *gd3a, *gd3b, *gd2a,
          *gd2b, *gd1a, *gd1b, *gd0a, *gd0b, *gout, *dpre;
     float *outF = nullptr, *noisyLog = nullptr, *demod = nullptr, *targetLog = nullptr, *loss = nullptr;
-    float* part = nullptr;
-    size_t partCap = 0;
     // dataset
     float *dsIn = nullptr, *dsTgt = nullptr;
     int dsN = 0, dsH = 0, dsW = 0;
@@ -584,39 +582,29 @@ hipError_t launchConvFwd(const ConvArgs& a, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t ensurePart(DenoiserNet* net, size_t floats) {
-    if (floats <= net->partCap) return hipSuccess;
-    if (net->part) (void)hipFree(net->part);
-    net->part = nullptr;
-    net->partCap = floats;
-    return hipMalloc(&net->part, floats * sizeof(float));
-}
-
-// Weight and bias gradients of one layer from the pre-activation gradient dy.
+// Weight and bias gradients of one layer from the pre-activation gradient dy, added with
+// atomics into dW and dB (zeroed once per backward pass).
 hipError_t convWeightGrad(DenoiserNet* net, const View& dy, const View& in, int n, int h, int w, int cinPad,
                           int coutPad, float* dW, float* dB, hipStream_t s) {
+    (void)net;
     const int npix = n * h * w;
     int parts = 64;
     int per = (npix + parts - 1) / parts;
     per = (per + 127) / 128 * 128;
     parts = (npix + per - 1) / per;
-    const size_t size = 9ULL * coutPad * cinPad;
-    hipError_t e;
-    if ((e = ensurePart(net, size * parts)) != hipSuccess) return e;
-    if ((e = hipMemsetAsync(net->part, 0, size * parts * sizeof(float), s)) != hipSuccess) return e;
-    BwdWArgs a{dy, in, net->part, n, h, w, cinPad, coutPad, per};
+    BwdWArgs a{dy, in, dW, n, h, w, cinPad, coutPad, per};
     const int tco = (coutPad / 16) % 4 == 0 ? 4 : 2, tci = (cinPad / 16) % 4 == 0 ? 4 : 2;
     dim3 grid(parts, 9, (coutPad / (16 * tco)) * (cinPad / (16 * tci)));
     if (tco == 4 && tci == 4) hipLaunchKernelGGL((convBwdWKernel<4, 4>), grid, dim3(kB), 0, s, a);
     else if (tco == 4) hipLaunchKernelGGL((convBwdWKernel<4, 2>), grid, dim3(kB), 0, s, a);
     else if (tci == 4) hipLaunchKernelGGL((convBwdWKernel<2, 4>), grid, dim3(kB), 0, s, a);
     else hipLaunchKernelGGL((convBwdWKernel<2, 2>), grid, dim3(kB), 0, s, a);
+    hipError_t e;
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    hipLaunchKernelGGL(reducePartsKernel, dim3(std::min<long long>(blocks((long long)size), 4096)), dim3(kB), 0, s,
-                       net->part, parts, (long long)size, dW);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-    if ((e = hipMemsetAsync(dB, 0, coutPad * sizeof(float), s)) != hipSuccess) return e;
-    hipLaunchKernelGGL(biasGradKernel, dim3(parts), dim3(kB), 0, s, dy, npix, coutPad, per, dB);
+    // Bias: many short pixel ranges so the reduction spreads over the whole GPU.
+    const int biasBlocks = std::max(1, std::min(2048, npix / 64));
+    const int biasPer = (npix + biasBlocks - 1) / biasBlocks;
+    hipLaunchKernelGGL(biasGradKernel, dim3(biasBlocks), dim3(kB), 0, s, dy, npix, coutPad, biasPer, dB);
     return hipGetLastError();
 }
 
@@ -751,6 +739,7 @@ hipError_t convBack(DenoiserNet* net, int l, const View& g, const View& act, con
 
 hipError_t backward(DenoiserNet* net, hipStream_t s) {
     const int n = net->n, h0 = net->h, w0 = net->w;
+    DN_TRY(hipMemsetAsync(net->grads, 0, net->nParams * sizeof(float), s));
     const int h1 = h0 / 2, w1 = w0 / 2, h2 = h0 / 4, w2 = w0 / 4, h3 = h0 / 8, w3 = w0 / 8, h4 = h0 / 16, w4 = w0 / 16;
     const View none{nullptr, 0, 0, 0};
     DN_TRY(convBack(net, OUT, V(net->gout, 32), none, V(net->d0b, 32), V(net->gd0b, 32), false, n, h0, w0, s));
@@ -831,7 +820,7 @@ void denoiserDestroy(DenoiserNet* net) {
     if (!net) return;
     freeActivations(net);
     for (void* p : {(void*)net->params, (void*)net->grads, (void*)net->adamM, (void*)net->adamV, (void*)net->wbf,
-                    (void*)net->part, (void*)net->dsIn, (void*)net->dsTgt, (void*)net->dCrops})
+                    (void*)net->dsIn, (void*)net->dsTgt, (void*)net->dCrops})
         if (p) (void)hipFree(p);
     if (net->ev0) (void)hipEventDestroy(net->ev0);
     if (net->ev1) (void)hipEventDestroy(net->ev1);
@@ -1120,8 +1109,11 @@ hipError_t convBackwardHost(const float* in, int n, int h, int w, int cin, const
         a.relu = 0; a.accumulate = 0;
         e = launchConvFwd(a, nullptr);
     }
+    if (e == hipSuccess) e = hipMemset(dgw, 0, 9ULL * coPad * ciPad * 4);
+    if (e == hipSuccess) e = hipMemset(dgb, 0, coPad * 4);
     if (e == hipSuccess)
-        e = convWeightGrad(net, View{dg, coPad, 0, coPad}, View{din, ciPad, 0, ciPad}, n, h, w, ciPad, coPad, dgw, dgb, nullptr);
+        e = convWeightGrad(net, View{dg, coPad, 0, coPad}, View{din, ciPad, 0, ciPad}, n, h, w, ciPad, coPad, dgw, dgb,
+                           nullptr);
     if (e == hipSuccess) e = hipDeviceSynchronize();
     if (e == hipSuccess) {
         std::vector<float> gi(npix * ciPad), gw(9ULL * coPad * ciPad), gb(coPad);

@@ -30,6 +30,7 @@ struct TraceArgs {
     const int* roots;        // batched views: root node per view (null: root 0)
     const int* viewBase;     // batched views: first path slot per view, views + 1 entries
     int views;
+    int refillBelow;         // set by launchTrace from TraceLaunch::refillBelow
 };
 
 struct FrameArgs {
@@ -76,6 +77,7 @@ struct TraceLaunch {
     bool quantized = false; // QNode (8-bit child bounds) instead of WideNode
     int stack = 8;          // LDS short-stack entries per lane: 8 or 16
     bool refill = false;    // lanes take new rays while the wave still traces others
+    int refillBelow = 40;   // with refill: a wave fetches when fewer lanes than this are busy
     bool whileWhile = true; // speculative while-while loop (inner-node and leaf phases)
 };
 hipError_t launchTrace(const TraceLaunch& l, bool anyHit, bool stats, int blocks, const TraceArgs& a, hipStream_t s);

@@ -40,7 +40,8 @@ __device__ __forceinline__ void loadNode(const N* nodes, int ref, N& n) {
 // Per-lane traversal state, kept in registers across refill rounds.
 struct LaneRay {
     f3 o, d, inv;
-    bool nz;        // no zero component in inv: slab tests take boxEntryNZ
+    f3 oi;          // o * inv
+    bool nz;        // no zero component in inv: slab tests take boxEntryFma
     float best, bu, bv;
     int bestIdx, sp, ref;
     int leaf;       // postponed leaf reference (< 0), or >= 0 for none
@@ -91,7 +92,7 @@ __device__ __forceinline__ void childEntries(const N* nodes, const LaneRay& L, f
     loadNode<N>(nodes, L.ref, n);
     if (L.nz) {
 #pragma unroll
-        for (int c = 0; c < W; c++) dist[c] = n.entryNZ(c, L.o, L.inv, L.best);
+        for (int c = 0; c < W; c++) dist[c] = n.entryFma(c, L.oi, L.inv, L.best);
     } else {
 #pragma unroll
         for (int c = 0; c < W; c++) dist[c] = n.entry(c, L.o, L.inv, L.best);
@@ -116,7 +117,7 @@ __device__ __forceinline__ void childEntries<WideNode<8>>(const WideNode<8>* nod
         if (L.nz) {
 #pragma unroll
             for (int c = 0; c < 4; c++)
-                dist[4 * h + c] = boxEntryNZ(L.o, L.inv, lxa[c], lya[c], lza[c], hxa[c], hya[c], hza[c], L.best);
+                dist[4 * h + c] = boxEntryFma(L.oi, L.inv, lxa[c], lya[c], lza[c], hxa[c], hya[c], hza[c], L.best);
         } else {
 #pragma unroll
             for (int c = 0; c < 4; c++)
@@ -125,6 +126,30 @@ __device__ __forceinline__ void childEntries<WideNode<8>>(const WideNode<8>* nod
         cr[4 * h] = ch.x; cr[4 * h + 1] = ch.y; cr[4 * h + 2] = ch.z; cr[4 * h + 3] = ch.w;
         __builtin_amdgcn_sched_barrier(0);
     }
+}
+
+// Pushes the entered children that are not descended into; returns whether the lane
+// descends (into *next).  Closest-hit queries go to the nearest child and push the others
+// far to near.  Any-hit queries take the first entered child in slot order and skip the
+// sort: whether the ray is occluded does not depend on the order.
+template <int W, bool ANY, int S>
+__device__ __forceinline__ bool descend(LaneRay& L, LdsStack<S>& st, float* dist, int* cr, bool& overflow, int* next) {
+    if (ANY) {
+        bool taken = false;
+#pragma unroll
+        for (int c = 0; c < W; c++) {
+            const bool in = dist[c] != INFINITY;
+            if (in && taken) st.push(L.sp, cr[c], dist[c], overflow);
+            if (in && !taken) { *next = cr[c]; taken = true; }
+        }
+        return taken;
+    }
+    sortChildren<W>(dist, cr);
+#pragma unroll
+    for (int c = W - 1; c >= 1; c--)
+        if (dist[c] != INFINITY) st.push(L.sp, cr[c], dist[c], overflow);
+    *next = cr[0];
+    return dist[0] != INFINITY;
 }
 
 // One traversal step: visit an inner node (push far children, descend to the nearest) or
@@ -138,11 +163,8 @@ __device__ __forceinline__ bool traceStep(LaneRay& L, const N* nodes, const floa
         if (STATS) nv++;
         float dist[W]; int cr[W];
         childEntries<N>(nodes, L, dist, cr);
-        sortChildren<W>(dist, cr);
-#pragma unroll
-        for (int c = W - 1; c >= 1; c--)
-            if (dist[c] != INFINITY) st.push(L.sp, cr[c], dist[c], overflow);
-        if (dist[0] != INFINITY) { L.ref = cr[0]; return false; }
+        int next;
+        if (descend<W, ANY, S>(L, st, dist, cr, overflow, &next)) { L.ref = next; return false; }
     } else {
         // Records k.. of the leaf, kLeafBatch at a time; the reference count bounds the first
         // batches exactly, larger leaves continue until the last-in-leaf flag.
@@ -202,18 +224,15 @@ __device__ __forceinline__ int popNext(LaneRay& L, LdsStack<S>& st) {
     return kDone;
 }
 
-template <class N, bool STATS, int S>
+template <class N, bool ANY, bool STATS, int S>
 __device__ __forceinline__ void visitNode(LaneRay& L, const N* nodes, LdsStack<S>& st, bool& overflow,
                                           unsigned long long& nv) {
     constexpr int W = NodeTraits<N>::kWidth;
     if (STATS) nv++;
     float dist[W]; int cr[W];
     childEntries<N>(nodes, L, dist, cr);
-    sortChildren<W>(dist, cr);
-#pragma unroll
-    for (int c = W - 1; c >= 1; c--)
-        if (dist[c] != INFINITY) st.push(L.sp, cr[c], dist[c], overflow);
-    L.ref = dist[0] != INFINITY ? cr[0] : popNext<S>(L, st);
+    int next;
+    L.ref = descend<W, ANY, S>(L, st, dist, cr, overflow, &next) ? next : popNext<S>(L, st);
 }
 
 // Tests the triangles of leaf reference ref; true when an any-hit query is answered.
@@ -296,6 +315,7 @@ __device__ __forceinline__ void startRay(const TraceArgs& a, int i, LaneRay& L) 
     L.d = F3(r1.x, r1.y, r1.z);
     L.inv = safeInverse(L.d);
     L.nz = L.inv.x != 0 && L.inv.y != 0 && L.inv.z != 0;
+    L.oi = F3(L.o.x * L.inv.x, L.o.y * L.inv.y, L.o.z * L.inv.z);
     L.best = r0.w; L.bu = 0; L.bv = 0;
     L.bestIdx = -1; L.sp = 0;
     L.index = i;
@@ -304,12 +324,11 @@ __device__ __forceinline__ void startRay(const TraceArgs& a, int i, LaneRay& L) 
     L.leaf = 0;
 }
 
-// Persistent traversal.  Each wave takes rays from the shared cursor; with REFILL, lanes
-// whose ray finished take new rays as soon as a quarter of the wave is idle, instead of
-// waiting for the wave's slowest ray.
-#ifndef RT_FETCH_THRESHOLD
-#define RT_FETCH_THRESHOLD 40
-#endif
+// Persistent traversal.  Each wave's first 64 rays are fixed by its position in the grid;
+// after that it takes rays from the shared cursor (which counts the rays past the grid's
+// first batch), so the launch does not start with one atomic per wave on one address.
+// With REFILL, lanes whose ray finished take new rays as soon as fewer than refillBelow
+// lanes are busy, instead of waiting for the wave's slowest ray.
 #ifndef RT_TRACE_MIN_WAVES
 #define RT_TRACE_MIN_WAVES 1
 #endif
@@ -327,13 +346,18 @@ __global__ __launch_bounds__(kBlock, RT_TRACE_MIN_WAVES) void traceKernel(TraceA
     bool overflow = false;
     LaneRay L;
     L.index = -1;
-    bool exhausted = false;   // wave-uniform: the cursor has passed the end
+    const int firstRays = gridDim.x * kBlock;   // one ray per launched lane
+    {
+        const int i = blockIdx.x * kBlock + tid;
+        if (i < total) startRay(a, i, L);
+    }
+    bool exhausted = firstRays >= total;   // wave-uniform: the cursor has passed the end
     for (;;) {
         const unsigned long long idle = __ballot(L.index < 0);
         if (idle && !exhausted) {
             const int leader = __ffsll((long long)idle) - 1;
             int base = 0;
-            if (lane == leader) base = atomicAdd(a.work, __popcll(idle));
+            if (lane == leader) base = firstRays + atomicAdd(a.work, __popcll(idle));
             base = __shfl(base, leader, 64);
             if (L.index < 0) {
                 const int rank = (int)__builtin_amdgcn_mbcnt_hi((unsigned)(idle >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)idle, 0u));
@@ -347,7 +371,7 @@ __global__ __launch_bounds__(kBlock, RT_TRACE_MIN_WAVES) void traceKernel(TraceA
             for (;;) {
                 bool searching = true;
                 while ((unsigned)L.ref < (unsigned)kDone) {
-                    visitNode<N, STATS, S>(L, nodes, st, overflow, nv);
+                    visitNode<N, ANY, STATS, S>(L, nodes, st, overflow, nv);
                     if (L.ref < 0 && L.leaf >= 0) { searching = false; L.leaf = L.ref; L.ref = popNext<S>(L, st); }
                     if (!__any(searching)) break;
                 }
@@ -363,7 +387,7 @@ __global__ __launch_bounds__(kBlock, RT_TRACE_MIN_WAVES) void traceKernel(TraceA
                     L.index = -1;
                     break;
                 }
-                if (REFILL && !exhausted && __popcll(__ballot(true)) < RT_FETCH_THRESHOLD) break;
+                if (REFILL && !exhausted && __popcll(__ballot(true)) < a.refillBelow) break;
             }
         }
         if (!WW && L.index >= 0) {
@@ -373,7 +397,7 @@ __global__ __launch_bounds__(kBlock, RT_TRACE_MIN_WAVES) void traceKernel(TraceA
                     L.index = -1;
                     break;
                 }
-                if (REFILL && !exhausted && __popcll(__ballot(true)) < RT_FETCH_THRESHOLD) break;
+                if (REFILL && !exhausted && __popcll(__ballot(true)) < a.refillBelow) break;
             }
         }
     }
@@ -412,7 +436,11 @@ static hipError_t launchTraceN(bool anyHit, bool stats, int blocks, const TraceL
     }
     return hipGetLastError();
 }
-hipError_t launchTrace(const TraceLaunch& l, bool anyHit, bool stats, int blocks, const TraceArgs& a, hipStream_t s) {
+hipError_t launchTrace(const TraceLaunch& l, bool anyHit, bool stats, int blocks, const TraceArgs& args,
+                       hipStream_t s) {
+    if (l.refillBelow < 1 || l.refillBelow > 64) return hipErrorInvalidValue;
+    TraceArgs a = args;
+    a.refillBelow = l.refillBelow;
     if (l.quantized) {
         switch (l.width) {
             case 2: return launchTraceN<QNode<2>>(anyHit, stats, blocks, l, a, s);
@@ -492,19 +520,45 @@ hipError_t launchRaygen(const FrameArgs& f, hipStream_t s) {
     return hipGetLastError();
 }
 
-// Appends one 32-byte ray per predicated lane with one atomic per wave.
-__device__ __forceinline__ void appendRay(bool pred, int* counter, float4* queue, float4 a, float4 b) {
-    const unsigned long long mask = __ballot(pred);
-    if (mask == 0) return;
-    const int lane = threadIdx.x & 63;
-    const int leader = __ffsll((long long)mask) - 1;
-    int base = 0;
-    if (lane == leader) base = atomicAdd(counter, __popcll(mask));
-    base = __shfl(base, leader, 64);
-    if (pred) {
-        const int off = base + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
-        queue[2 * off] = a;
-        queue[2 * off + 1] = b;
+// Appends one 32-byte ray per predicated lane to each of two queues, with one atomic per
+// workgroup and queue.  Every thread of the workgroup calls it once per round of the
+// caller's grid-stride loop.  The per-wave counts alternate between two LDS buffers by
+// round: a wave can only write round k+2's counts after every wave has passed round k+1's
+// first barrier, which each wave reaches after it has read round k's counts.
+struct AppendLds {
+    int count[2][2][kBlock / 64];   // [round & 1][queue][wave]
+    int base[2];
+};
+__device__ __forceinline__ void appendRays2(AppendLds& sh, int round, bool p0, int* c0, float4* q0, float4 a0,
+                                            float4 b0, bool p1, int* c1, float4* q1, float4 a1, float4 b1) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const unsigned long long m0 = __ballot(p0), m1 = __ballot(p1);
+    int(*cnt)[kBlock / 64] = sh.count[round & 1];
+    if (lane == 0) {
+        cnt[0][wave] = __popcll(m0);
+        cnt[1][wave] = __popcll(m1);
+    }
+    __syncthreads();
+    if (tid < 2) {
+        int t = 0;
+        for (int w = 0; w < kBlock / 64; w++) t += cnt[tid][w];
+        sh.base[tid] = t ? atomicAdd(tid ? c1 : c0, t) : 0;
+    }
+    __syncthreads();
+    int o0 = sh.base[0], o1 = sh.base[1];
+    for (int w = 0; w < wave; w++) {
+        o0 += cnt[0][w];
+        o1 += cnt[1][w];
+    }
+    if (p0) {
+        const int off = o0 + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m0 >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m0, 0u));
+        q0[2 * off] = a0;
+        q0[2 * off + 1] = b0;
+    }
+    if (p1) {
+        const int off = o1 + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m1 >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m1, 0u));
+        q1[2 * off] = a1;
+        q1[2 * off + 1] = b1;
     }
 }
 
@@ -514,9 +568,11 @@ __device__ __forceinline__ void appendRay(bool pred, int* counter, float4* queue
 constexpr int kShadeBlocksPerCU = 5;
 __global__ __launch_bounds__(kBlock, kShadeBlocksPerCU) void shadeKernel(FrameArgs f) {
     __shared__ unsigned sShift[8], sTerm[kMaxRaysPerPixel * 8];
+    __shared__ AppendLds sApp;
     loadSobolTerms<8>(sShift, sTerm, f.rs.seed, 4 + 8 * f.bounce, f.sample0, f.R);
     const int count = f.counters[kCntRays + f.bounce];
-    for (int base = blockIdx.x * kBlock; base < count; base += gridDim.x * kBlock) {
+    int round = 0;
+    for (int base = blockIdx.x * kBlock; base < count; base += gridDim.x * kBlock, round++) {
         const int i = base + threadIdx.x;
         bool wantShadow = false, wantNext = false;
         float4 sa = make_float4(0, 0, 0, 0), sb = sa, na = sa, nb = sa;
@@ -553,8 +609,8 @@ __global__ __launch_bounds__(kBlock, kShadeBlocksPerCU) void shadeKernel(FrameAr
                 nb = make_float4(res.nextD.x, res.nextD.y, res.nextD.z, __int_as_float(slot));
             }
         }
-        appendRay(wantShadow, &f.counters[kCntShadow + f.bounce], f.shadow, sa, sb);
-        appendRay(wantNext, &f.counters[kCntRays + f.bounce + 1], f.rayOut, na, nb);
+        appendRays2(sApp, round, wantShadow, &f.counters[kCntShadow + f.bounce], f.shadow, sa, sb, wantNext,
+                    &f.counters[kCntRays + f.bounce + 1], f.rayOut, na, nb);
     }
 }
 int shadeBlocksPerCU() { return kShadeBlocksPerCU; }
@@ -622,8 +678,10 @@ hipError_t launchBatchRaygen(const BatchArgs& b, hipStream_t s) {
 }
 
 __global__ __launch_bounds__(kBlock, kShadeBlocksPerCU) void batchShadeKernel(BatchArgs b) {
+    __shared__ AppendLds sApp;
     const int count = b.counters[kCntRays + b.bounce];
-    for (int base = blockIdx.x * kBlock; base < count; base += gridDim.x * kBlock) {
+    int round = 0;
+    for (int base = blockIdx.x * kBlock; base < count; base += gridDim.x * kBlock, round++) {
         const int i = base + threadIdx.x;
         bool wantShadow = false, wantNext = false;
         float4 sa = make_float4(0, 0, 0, 0), sb = sa, na = sa, nb = sa;
@@ -663,8 +721,8 @@ __global__ __launch_bounds__(kBlock, kShadeBlocksPerCU) void batchShadeKernel(Ba
                 nb = make_float4(res.nextD.x, res.nextD.y, res.nextD.z, __int_as_float(slot));
             }
         }
-        appendRay(wantShadow, &b.counters[kCntShadow + b.bounce], b.shadow, sa, sb);
-        appendRay(wantNext, &b.counters[kCntRays + b.bounce + 1], b.rayOut, na, nb);
+        appendRays2(sApp, round, wantShadow, &b.counters[kCntShadow + b.bounce], b.shadow, sa, sb, wantNext,
+                    &b.counters[kCntRays + b.bounce + 1], b.rayOut, na, nb);
     }
 }
 hipError_t launchBatchShade(const BatchArgs& b, int blocks, hipStream_t s) {

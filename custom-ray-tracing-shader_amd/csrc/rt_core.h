@@ -96,12 +96,16 @@ RT_HD float boxEntry(f3 o, f3 inv, float lx, float ly, float lz, float hx, float
     float farT = fminf(fminf(bx, by), fminf(bz, limit));
     return nearT <= farT ? nearT : INFINITY;
 }
-// boxEntry for a ray whose inverse direction has no zero component: the slab selects drop
-// out and the remaining operations are exactly boxEntry's.
-RT_HD float boxEntryNZ(f3 o, f3 inv, float lx, float ly, float lz, float hx, float hy, float hz, float limit) {
-    const float ax = (lx - o.x) * inv.x, bx = (hx - o.x) * inv.x;
-    const float ay = (ly - o.y) * inv.y, by = (hy - o.y) * inv.y;
-    const float az = (lz - o.z) * inv.z, bz = (hz - o.z) * inv.z;
+// Slab test for a ray whose inverse direction has no zero component (the slab selects of
+// boxEntry drop out), with each slab distance as one fused multiply-add, lo * inv - o * inv,
+// where oi = o * inv is computed once per ray.  Its rounding error, like that of
+// (lo - o) * inv, is a few units in the last place of max(|o|, |lo|) in position, inside the
+// padding every primitive box carries, so the boxes it enters still contain every hit that
+// can win.
+RT_HD float boxEntryFma(f3 oi, f3 inv, float lx, float ly, float lz, float hx, float hy, float hz, float limit) {
+    const float ax = fmaf(lx, inv.x, -oi.x), bx = fmaf(hx, inv.x, -oi.x);
+    const float ay = fmaf(ly, inv.y, -oi.y), by = fmaf(hy, inv.y, -oi.y);
+    const float az = fmaf(lz, inv.z, -oi.z), bz = fmaf(hz, inv.z, -oi.z);
     float nearT = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fmaxf(fminf(az, bz), 0.0f));
     float farT = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fminf(fmaxf(az, bz), limit));
     return nearT <= farT ? nearT : INFINITY;
@@ -130,8 +134,8 @@ template <int W> struct alignas(64) WideNode {
     RT_HD float entry(int c, f3 o, f3 inv, float limit) const {
         return boxEntry(o, inv, lox[c], loy[c], loz[c], hix[c], hiy[c], hiz[c], limit);
     }
-    RT_HD float entryNZ(int c, f3 o, f3 inv, float limit) const {
-        return boxEntryNZ(o, inv, lox[c], loy[c], loz[c], hix[c], hiy[c], hiz[c], limit);
+    RT_HD float entryFma(int c, f3 oi, f3 inv, float limit) const {
+        return boxEntryFma(oi, inv, lox[c], loy[c], loz[c], hix[c], hiy[c], hiz[c], limit);
     }
 };
 static_assert(sizeof(WideNode<2>) == 64 && sizeof(WideNode<4>) == 128 && sizeof(WideNode<8>) == 256, "node sizes");
@@ -153,11 +157,11 @@ template <int W> struct alignas(64) QNode {
         return boxEntry(o, inv, px + (float)qlox[c] * sx, py + (float)qloy[c] * sy, pz + (float)qloz[c] * sz,
                         px + (float)qhix[c] * sx, py + (float)qhiy[c] * sy, pz + (float)qhiz[c] * sz, limit);
     }
-    RT_HD float entryNZ(int c, f3 o, f3 inv, float limit) const {
+    RT_HD float entryFma(int c, f3 oi, f3 inv, float limit) const {
         if (child[c] == kEmptyChild) return INFINITY;
         const float sx = step(0), sy = step(1), sz = step(2);
-        return boxEntryNZ(o, inv, px + (float)qlox[c] * sx, py + (float)qloy[c] * sy, pz + (float)qloz[c] * sz,
-                          px + (float)qhix[c] * sx, py + (float)qhiy[c] * sy, pz + (float)qhiz[c] * sz, limit);
+        return boxEntryFma(oi, inv, px + (float)qlox[c] * sx, py + (float)qloy[c] * sy, pz + (float)qloz[c] * sz,
+                           px + (float)qhix[c] * sx, py + (float)qhiy[c] * sy, pz + (float)qhiz[c] * sz, limit);
     }
 };
 static_assert(sizeof(QNode<2>) == 64 && sizeof(QNode<4>) == 64 && sizeof(QNode<8>) == 128, "quantized node sizes");

@@ -3,7 +3,10 @@
 Dataset: for each seeded scene (displaced icosphere in a lit room, triangle soup under sky
 light, or a Cornell-box variant), a 4-spp render supplies the network input planes
 (RtReadDenoiserInputs: radiance, albedo, normal, depth) and an independently seeded
-high-sample render supplies the target radiance.  Training runs entirely on the GPU
+high-sample render supplies the target radiance.  With --fine, part of the scenes are fine
+views instead: full-tessellation icosphere rooms seen through the field of view that a
+crop of a 1080-line frame spans, so surface detail covers as many pixels as it does in a
+full-resolution frame.  Training runs entirely on the GPU
 (RtDenoiserTrain: random crops + flips, Adam, L1 in log space, optionally plus an L1 term on
 the display-transformed radiance).  The trained FP32 weights are
 written with a versioned header to denoiser_weights.bin next to the library, where RtDenoise
@@ -90,25 +93,75 @@ def random_scene(seed):
     return tris, cam, st
 
 
+def fine_pool(count, subdiv, log=None):
+    """Displaced-icosphere rooms at full tessellation (other seeds than the benchmark scene),
+    shared by the fine views."""
+    pool = []
+    for k in range(count):
+        rng = np.random.default_rng(5000 + k)
+        tris = scenes.displaced_icosphere(subdiv=subdiv, seed=5000 + k, amplitude=float(rng.uniform(0.03, 0.08)),
+                                          metal_probability=float(rng.uniform(0.1, 0.45)))
+        tris["emission"] = np.where(tris["emission"] > 0, rng.uniform(8, 20), 0).astype(np.float32)
+        pool.append(tris)
+        if log:
+            log(f"fine mesh {k + 1}/{count}: {len(tris)} triangles")
+    return pool
+
+
+def fine_view(seed, size, full_height=1080):
+    """A size x size window of a full_height-line frame: the camera of a displaced-icosphere
+    room aimed at a random point of the sphere, with the field of view a window of that
+    size spans in the full frame, so features cover as many pixels as at full resolution."""
+    rng = np.random.default_rng(seed)
+    ang, dist = rng.uniform(-0.8, 0.8), rng.uniform(2.8, 4.2)
+    eye = (math.sin(ang) * dist, rng.uniform(0.9, 2.4), -math.cos(ang) * dist)
+    d = rng.normal(size=3)
+    target = np.array([0, 1.3, 0]) + d / np.linalg.norm(d) * rng.uniform(0, 1.1)
+    full = math.radians(rng.uniform(35, 60))
+    fov = 2 * math.degrees(math.atan(math.tan(full / 2) * size / full_height))
+    return scenes.make_camera(eye, tuple(target), fov_deg=fov, aspect=1.0), scenes.make_settings(8, sky=0.0)
+
+
 def heldout_scene(seed):
     tris, cam, st = random_scene(seed)
     return cam(256, 256), st, tris
 
 
-def build_dataset(lib, count, size, input_spp, target_spp, first_seed, log):
+def build_dataset(lib, count, size, input_spp, target_spp, first_seed, log, fine=0.0, pool=()):
+    """input_spp: one sample count, or a sequence cycled over the scenes (mixed noise levels).
+    A fraction `fine` of the scenes (chosen by seed) are fine views of the meshes in pool."""
+    spps = tuple(input_spp) if isinstance(input_spp, (tuple, list)) else (input_spp,)
     inputs = np.zeros((count, size, size, 10), np.float32)
     targets = np.zeros((count, size, size, 3), np.float32)
+    sessions = {}
     t0 = time.perf_counter()
-    for i in range(count):
-        tris, camf, st = random_scene(first_seed + i)
-        cam = camf(size, size)
-        with Session(lib) as s:
-            s.initialize(tris, size, size)
-            targets[i] = render_mean(s, cam, st, target_spp, seed=1000003 + i)
-            render_mean(s, cam, st, input_spp, seed=42 + i)
-            inputs[i] = s.denoiser_inputs()
-        if log and (i + 1) % 16 == 0:
-            log(f"dataset {i + 1}/{count} ({time.perf_counter() - t0:.1f} s)")
+    try:
+        for i in range(count):
+            seed = first_seed + i
+            if pool and np.random.default_rng(seed + 777).uniform() < fine:
+                k = i % len(pool)
+                if k not in sessions:
+                    sessions[k] = Session(lib)
+                    sessions[k].initialize(pool[k], size, size)
+                s, own = sessions[k], False
+                cam, st = fine_view(seed, size)
+            else:
+                tris, camf, st = random_scene(seed)
+                cam = camf(size, size)
+                s, own = Session(lib), True
+                s.initialize(tris, size, size)
+            try:
+                targets[i] = render_mean(s, cam, st, target_spp, seed=1000003 + i)
+                render_mean(s, cam, st, spps[i % len(spps)], seed=42 + i)
+                inputs[i] = s.denoiser_inputs()
+            finally:
+                if own:
+                    s.close()
+            if log and (i + 1) % 16 == 0:
+                log(f"dataset {i + 1}/{count} ({time.perf_counter() - t0:.1f} s)")
+    finally:
+        for s in sessions.values():
+            s.close()
     return inputs, targets
 
 
@@ -126,6 +179,9 @@ def main():
     ap.add_argument("--heldout", type=int, default=8)
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--input-spp", type=int, default=4)
+    ap.add_argument("--input-spp-mix", default=None,
+                    help="comma-separated input sample counts cycled over the training scenes, e.g. 4,4,8,16,32 "
+                         "(held-out scenes stay at --input-spp)")
     ap.add_argument("--target-spp", type=int, default=1024)
     ap.add_argument("--steps", type=int, default=3000)
     ap.add_argument("--batch", type=int, default=8)
@@ -133,6 +189,11 @@ def main():
     ap.add_argument("--lr", type=float, default=3e-4)
     ap.add_argument("--display-weight", type=float, default=0.0,
                     help="weight of the display-space L1 term added to the log-space L1 loss")
+    ap.add_argument("--fine", type=float, default=0.0,
+                    help="fraction of scenes that are fine views: full-tessellation icosphere rooms seen "
+                         "through the field of view of a crop of a 1080-line frame")
+    ap.add_argument("--fine-meshes", type=int, default=4)
+    ap.add_argument("--fine-subdiv", type=int, default=9)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--out", default=str(HERE / "denoiser_weights.bin"))
     ap.add_argument("--report", default=None, help="write a JSON summary here")
@@ -140,8 +201,12 @@ def main():
     log = lambda m: print(m, file=sys.stderr, flush=True)
     lib = load_library()
     t0 = time.perf_counter()
-    inputs, targets = build_dataset(lib, args.scenes, args.size, args.input_spp, args.target_spp, 100, log)
-    hin, htg = build_dataset(lib, args.heldout, args.size, args.input_spp, args.target_spp, 100000, log)
+    mix = tuple(int(x) for x in args.input_spp_mix.split(",")) if args.input_spp_mix else args.input_spp
+    pool = fine_pool(args.fine_meshes, args.fine_subdiv, log) if args.fine > 0 else ()
+    inputs, targets = build_dataset(lib, args.scenes, args.size, mix, args.target_spp, 100, log, args.fine, pool)
+    hin, htg = build_dataset(lib, args.heldout, args.size, args.input_spp, args.target_spp, 100000, log, args.fine,
+                             pool)
+    del pool
     data_s = time.perf_counter() - t0
     net = Denoiser(lib, seed=args.seed)
     net.set_display_loss(args.display_weight)
@@ -162,6 +227,8 @@ def main():
     net.save(args.out)
     flops = 3 * lib.RtDenoiserForwardFlops(args.batch, args.crop, args.crop)
     report = {"scenes": args.scenes, "heldout": args.heldout, "size": args.size, "input_spp": args.input_spp,
+              "input_spp_mix": args.input_spp_mix, "fine": args.fine, "fine_meshes": args.fine_meshes,
+              "fine_subdiv": args.fine_subdiv,
               "target_spp": args.target_spp, "steps": args.steps, "batch": args.batch, "crop": args.crop,
               "display_weight": args.display_weight,
               "dataset_seconds": round(data_s, 1), "train_seconds": round(train_s, 1),

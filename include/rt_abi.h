@@ -103,7 +103,8 @@ typedef struct RtHit { int index; float distance, u, v; } RtHit;
 /* Reference BvhNode layout (Native/bvh.cuh:7): inner nodes have count 0, first = left child. */
 typedef struct RtBvhNode { float low[3]; int first; float high[3]; int count; } RtBvhNode;
 
-/* Closest-hit (anyHit = 0) or any-hit queries against the session's GPU BVH on the GPU. */
+/* Closest-hit (anyHit = 0) or any-hit queries against the session's GPU BVH on the GPU;
+ * at most 2^30 rays per call. */
 int RtTraceRays(void* session, const RtRay* rays, int count, int anyHit, RtHit* hits);
 /* Scrambled Sobol values computed on the GPU: out[i*dimensions + d] = sample(pixels[i],
  * samples[i], seed, d) for d < dimensions <= 260 (Native/sampling.cuh:26-35). */
@@ -123,6 +124,8 @@ int RtRayCounts(void* session, unsigned long long* closest, unsigned long long* 
  * workgroups per compute unit, 1..32), "short_stack" (LDS stack entries per traversal lane,
  * 8 or 16), "while_while" (0/1: speculative while-while traversal loop, default 1),
  * "refill" (0/1: lanes take new rays while the wave traces others, default 1),
+ * "refill_below" (1..64: with refill, a wave fetches new rays once fewer lanes than this
+ * are busy, default 40),
  * "tiled_primary", "nontemporal", "stage_timing", "trace_stats" and "denoise_fallback" at the
  * next render or query.  Returns 1 if the key is known and the value valid.  RtGetOption
  * returns -1 for unknown keys; it also reports "node_bytes", the size of one wide node. */
@@ -196,7 +199,7 @@ int RtConv3x3Backward(const float* in, int n, int h, int w, int cin, const float
  * shared path queue, each frame replayed as a captured hipGraph ("use_graph", default 1).
  * Each view takes one progressive sample per frame (raysPerPixel is ignored) and equals a
  * session rendering its scene with raysPerPixel 1.  Options ("bvh_width", "bvh_quantized",
- * "blocks_per_cu", "short_stack", "while_while", "refill", "use_graph") must be set before
+ * "blocks_per_cu", "short_stack", "while_while", "refill", "refill_below", "use_graph") must be set before
  * RtBatchBuild.  RtBatchReadView
  * copies a view's RGBA8 image and/or its 10-float-per-pixel accumulation. */
 void* RtBatchCreate(void);
