@@ -38,9 +38,9 @@ HBM_PEAK_GBS = 8000.0
 # icosphere uses the autotuner's leaf size.
 TUNED = {
     "soup": {"bvh_width": 4, "build_max_leaf": 4, "blocks_per_cu": 4, "short_stack": 16, "while_while": 1, "refill": 1},
-    "icosphere": {"bvh_width": 4, "build_max_leaf": 2, "blocks_per_cu": 4, "short_stack": 16, "while_while": 0,
+    "icosphere": {"bvh_width": 4, "build_max_leaf": 2, "blocks_per_cu": 6, "short_stack": 8, "while_while": 0,
                   "refill": 0},
-    "cornell": {"bvh_width": 4, "build_max_leaf": 4, "blocks_per_cu": 6, "short_stack": 8, "while_while": 1, "refill": 1},
+    "cornell": {"bvh_width": 4, "build_max_leaf": 4, "blocks_per_cu": 6, "short_stack": 8, "while_while": 1, "refill": 0},
     "batch": {"bvh_width": 4, "blocks_per_cu": 6, "short_stack": 8, "while_while": 1, "refill": 1},
 }
 
@@ -69,7 +69,10 @@ def batch_views(count, seed=2024):
 def bench_batch(args):
     import native_api
     views = batch_views(args.batch_views)
-    cfg = TUNED["batch"]
+    cfg = dict(TUNED["batch"])
+    for k in ("bvh_width", "bvh_quantized", "blocks_per_cu", "short_stack", "refill", "refill_below", "while_while"):
+        if getattr(args, k) is not None:
+            cfg[k] = getattr(args, k)
     b = native_api.Batch(**cfg)
     for t, w, h, cam, st in views:
         b.add_view(t, w, h, cam, st)
@@ -196,7 +199,7 @@ def psnr_study(args):
     # Denoised frames at more samples: the smallest count whose denoised PSNR reaches plain
     # accumulation at 256 spp, and how much sooner it gets there.
     dn_ladder = []
-    for spp in (8, 16, 32, 64):
+    for spp in (8, 12, 16, 24, 32, 64):
         best = None
         for _ in range(2):
             t = time.perf_counter()
@@ -343,15 +346,22 @@ def main():
     width = s.lib.RtGetOption(s.handle, b"bvh_width")
     quantized = s.lib.RtGetOption(s.handle, b"bvh_quantized")
     node_b = s.lib.RtGetOption(s.handle, b"node_bytes")
-    launches = stage_ms["closest"][1] / 3
-    closest_ms = stage_ms["closest"][0] / 3
-    bytes_frame = 48 * closest_rays + stats["closest_nodes"] * node_b + stats["closest_tris"] * 48
+    # The traversal kernel: each launch traces one bounce's closest-hit rays together with the
+    # previous bounce's shadow rays (the "closest" stage), the last launch only shadow rays.
+    launches = (stage_ms["closest"][1] + stage_ms.get("shadow", [0, 0])[1]) / 3
+    trace_ms = (stage_ms["closest"][0] + stage_ms.get("shadow", [0, 0])[0]) / 3
+    rays_frame = closest_rays + shadow_rays
+    nodes_frame = stats["closest_nodes"] + stats["shadow_nodes"]
+    tris_frame = stats["closest_tris"] + stats["shadow_tris"]
+    bytes_frame = 48 * rays_frame + nodes_frame * node_b + tris_frame * 48
     bytes_launch = bytes_frame / launches
-    mean_launch_s = closest_ms / 1e3 / launches
+    mean_launch_s = trace_ms / 1e3 / launches
     achieved = bytes_launch / mean_launch_s / 1e9
     log(f"stages (ms/frame): " + ", ".join(f"{k} {v[0] / 3:.3f} ({v[1] // 3} launches)" for k, v in stage_ms.items()))
-    log(f"closest: {closest_rays} rays, {stats['closest_nodes'] / max(closest_rays, 1):.1f} nodes/ray, "
-        f"{stats['closest_tris'] / max(closest_rays, 1):.1f} tris/ray, {achieved:.0f} GB/s algorithmic")
+    for kind, n in (("closest", closest_rays), ("shadow", shadow_rays)):
+        log(f"{kind}: {n} rays, {stats[kind + '_nodes'] / max(n, 1):.1f} nodes/ray, "
+            f"{stats[kind + '_tris'] / max(n, 1):.1f} tris/ray")
+    log(f"traversal: {launches:.0f} launches/frame, {mean_launch_s * 1e3:.3f} ms each, {achieved:.0f} GB/s algorithmic")
 
     baseline = None
     if not args.no_cpu_baseline:
@@ -392,10 +402,11 @@ def main():
                    "device_ms_per_step": round(device_ms / args.steps, 4), "bvh_build_ms": round(build_ms, 1)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": f"traceKernel<{'QNode' if quantized else 'WideNode'}<{width}>,closest>", "node_bytes": node_b, "mean_launch_ms": round(mean_launch_s * 1e3, 4),
+                     "kernel": f"traceKernel<{'QNode' if quantized else 'WideNode'}<{width}>> (closest-hit + shadow queues)",
+                     "node_bytes": node_b, "mean_launch_ms": round(mean_launch_s * 1e3, 4),
                      "launches_per_step": launches, "algorithmic_bytes_per_launch": int(bytes_launch),
-                     "nodes_per_ray": round(stats["closest_nodes"] / max(closest_rays, 1), 2),
-                     "tris_per_ray": round(stats["closest_tris"] / max(closest_rays, 1), 2)},
+                     "nodes_per_ray": round(nodes_frame / max(rays_frame, 1), 2),
+                     "tris_per_ray": round(tris_frame / max(rays_frame, 1), 2)},
         "cpu_baseline": baseline,
         "time_to_equal_psnr": denoise,
     }

@@ -262,30 +262,32 @@ bool renderGpu(Session* s, const CameraData& cam, const RenderSettings& rs, floa
             StageScope sc(s, RT_STAGE_RAYGEN);
             if ((s->error = launchRaygen(f, st)) != hipSuccess) return false;
         }
-        for (int b = 0; b < rs.maxBounces; b++) {
+        // Bounce b's closest-hit rays and bounce b-1's shadow rays are independent, so one
+        // launch traces both queues; the shadow rays' radiance is still added before shade
+        // b adds bounce b's emission, the reference's order (Native/renderer.cu:232-267).
+        const bool nee = !s->lights.empty();
+        for (int b = 0; b <= rs.maxBounces; b++) {
             float4* in = (b & 1) ? s->dRayB : s->dRayA;
             float4* out = (b & 1) ? s->dRayA : s->dRayB;
+            const bool rays = b < rs.maxBounces, shadow = nee && b > 0;
+            if (!rays && !shadow) break;
             TraceArgs ta = baseTraceArgs(s);
-            ta.rays = in; ta.countPtr = ctr + kCntRays + b; ta.work = ctr + kWorkRays + b; ta.hits = s->dHits;
+            ta.work = ctr + kWorkRays + b;
             ta.stats = s->dStats;
+            if (rays) { ta.rays = in; ta.countPtr = ctr + kCntRays + b; ta.hits = s->dHits; }
+            if (shadow) {
+                ta.anyRays = s->dShadow; ta.anyCountPtr = ctr + kCntShadow + (b - 1);
+                ta.rad = s->dRad; ta.contrib = s->dNee;
+            }
             {
-                StageScope sc(s, RT_STAGE_CLOSEST);
-                if ((s->error = launchTrace(traceLaunch(s), false, s->traceStats, s->traceBlocks, ta, st)) != hipSuccess)
+                StageScope sc(s, rays ? RT_STAGE_CLOSEST : RT_STAGE_SHADOW);
+                if ((s->error = launchTrace(traceLaunch(s), s->traceStats, s->traceBlocks, ta, st)) != hipSuccess)
                     return false;
             }
+            if (!rays) break;
             f.bounce = b; f.rayIn = in; f.rayOut = out;
-            {
-                StageScope sc(s, RT_STAGE_SHADE);
-                if ((s->error = launchShade(f, s->cus * shadeBlocksPerCU(), st)) != hipSuccess) return false;
-            }
-            if (!s->lights.empty()) {
-                TraceArgs sa = baseTraceArgs(s);
-                sa.rays = s->dShadow; sa.countPtr = ctr + kCntShadow + b; sa.work = ctr + kWorkShadow + b;
-                sa.rad = s->dRad; sa.contrib = s->dNee; sa.stats = s->dStats + 2;
-                StageScope sc(s, RT_STAGE_SHADOW);
-                if ((s->error = launchTrace(traceLaunch(s), true, s->traceStats, s->traceBlocks, sa, st)) != hipSuccess)
-                    return false;
-            }
+            StageScope sc(s, RT_STAGE_SHADE);
+            if ((s->error = launchShade(f, s->cus * shadeBlocksPerCU(), st)) != hipSuccess) return false;
         }
         AccumArgs a{};
         a.thr = s->dThr; a.rad = s->dRad; a.alb = s->dAlb; a.nrm = s->dNrm;
@@ -725,8 +727,10 @@ int RtTraceRays(void* p, const RtRay* rays, int count, int anyHit, RtHit* hits) 
     if (ok) ok = (s->error = hipMemsetAsync(work, 0, sizeof(int), s->stream)) == hipSuccess;
     if (ok) {
         TraceArgs a = baseTraceArgs(s);
-        a.rays = dr; a.count = count; a.work = work; a.hits = dh;
-        ok = (s->error = launchTrace(traceLaunch(s), anyHit != 0, false, s->traceBlocks, a, s->stream)) == hipSuccess;
+        a.work = work;
+        if (anyHit) { a.anyRays = dr; a.anyCount = count; a.anyHits = dh; }
+        else { a.rays = dr; a.count = count; a.hits = dh; }
+        ok = (s->error = launchTrace(traceLaunch(s), false, s->traceBlocks, a, s->stream)) == hipSuccess;
     }
     if (ok) ok = (s->error = hipStreamSynchronize(s->stream)) == hipSuccess;
     if (ok) ok = (s->error = hipMemcpy(hits, dh, (size_t)count * sizeof(RtHit), hipMemcpyDeviceToHost)) == hipSuccess;

@@ -130,22 +130,26 @@ bool enqueueFrame(Batch* b) {
     B_TRY(launchResetCounters(b->dCounters, b->totalSlots, st));
     a.rayOut = b->dRayA;
     B_TRY(launchBatchRaygen(a, st));
-    for (int bounce = 0; bounce < b->maxBounces; bounce++) {
+    // As a session frame: one launch traces bounce k's closest-hit rays and bounce k-1's
+    // shadow rays; the last launch only the last bounce's shadow rays.
+    for (int bounce = 0; bounce <= b->maxBounces; bounce++) {
         float4* in = (bounce & 1) ? b->dRayB : b->dRayA;
         float4* out = (bounce & 1) ? b->dRayA : b->dRayB;
+        const bool rays = bounce < b->maxBounces;
         TraceArgs ta{};
         ta.nodes = b->dNodes; ta.recs = b->dRecs; ta.spill = b->dSpill; ta.spillEntries = b->spillEntries;
         ta.totalLanes = b->traceBlocks * kBlock; ta.overflow = b->dOverflow;
         ta.roots = b->dRoots; ta.viewBase = b->dViewBase; ta.views = b->views; ta.nontemporal = 1;
-        TraceArgs sa = ta;
-        ta.rays = in; ta.countPtr = b->dCounters + kCntRays + bounce; ta.work = b->dCounters + kWorkRays + bounce;
-        ta.hits = b->dHits;
-        B_TRY(launchTrace(launchOf(b), false, false, b->traceBlocks, ta, st));
+        ta.work = b->dCounters + kWorkRays + bounce;
+        if (rays) { ta.rays = in; ta.countPtr = b->dCounters + kCntRays + bounce; ta.hits = b->dHits; }
+        if (bounce > 0) {
+            ta.anyRays = b->dShadow; ta.anyCountPtr = b->dCounters + kCntShadow + (bounce - 1);
+            ta.rad = b->dRad; ta.contrib = b->dNee;
+        }
+        B_TRY(launchTrace(launchOf(b), false, b->traceBlocks, ta, st));
+        if (!rays) break;
         a.bounce = bounce; a.rayIn = in; a.rayOut = out;
         B_TRY(launchBatchShade(a, b->cus * shadeBlocksPerCU(), st));
-        sa.rays = b->dShadow; sa.countPtr = b->dCounters + kCntShadow + bounce; sa.work = b->dCounters + kWorkShadow + bounce;
-        sa.rad = b->dRad; sa.contrib = b->dNee;
-        B_TRY(launchTrace(launchOf(b), true, false, b->traceBlocks, sa, st));
     }
     B_TRY(launchBatchAccumulate(a, st));
     B_TRY(launchBatchAdvance(a, st));

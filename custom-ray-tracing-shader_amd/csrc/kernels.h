@@ -7,24 +7,30 @@ namespace rt {
 constexpr int kBlock = 256;       // threads per workgroup for every kernel here
 constexpr int kCounterStride = 4 * (kMaxBounces + 1);   // ints per chunk of frame counters
 // Frame counter layout within one chunk: [0,33) closest-ray queue sizes per bounce,
-// [33,66) shadow queue sizes, [66,99) closest work cursors, [99,132) shadow work cursors.
-constexpr int kCntRays = 0, kCntShadow = kMaxBounces + 1, kWorkRays = 2 * (kMaxBounces + 1),
-              kWorkShadow = 3 * (kMaxBounces + 1);
+// [33,66) shadow queue sizes, [66,99) work cursors of the traversal launches (launch b
+// traces bounce b's closest-hit rays and bounce b-1's shadow rays), [99,132) unused.
+constexpr int kCntRays = 0, kCntShadow = kMaxBounces + 1, kWorkRays = 2 * (kMaxBounces + 1);
 
+// One traversal launch covers two queues: closest-hit rays, then any-hit rays (ray indices
+// past the closest-hit count).  Either count may be zero.
 struct TraceArgs {
-    const float4* rays;      // 2 float4 per ray: (origin, limit), (direction, path slot bits)
+    const float4* rays;      // closest-hit queue, 2 float4 per ray: (origin, limit), (direction, path slot bits)
     const int* countPtr;     // device-resident ray count, or null to use count
     int count;
+    const float4* anyRays;   // any-hit queue, same layout
+    const int* anyCountPtr;  // device-resident any-hit count, or null to use anyCount
+    int anyCount;
     int* work;               // work cursor, starts at 0
     const void* nodes;       // WideNode<W>[] or QNode<W>[]
     const TriRecord* recs;
-    int4* hits;              // closest / query mode: RtHit per ray
-    float4* rad;             // shadow mode: radiance += contrib when unoccluded
+    int4* hits;              // RtHit per closest-hit ray
+    int4* anyHits;           // query mode: RtHit per any-hit ray; null in frame mode
+    float4* rad;             // frame mode: radiance += contrib of an unoccluded any-hit ray's slot
     const float4* contrib;
     int* spill;              // global spill ring: 2 ints per entry, entry-major
     int spillEntries;        // entries per lane beyond the LDS part
     int totalLanes;          // launched lanes (spill stride)
-    unsigned long long* stats;   // optional: [0] node visits, [1] triangle tests
+    unsigned long long* stats;   // optional: node visits, triangle tests of closest-hit rays, then of any-hit rays
     int* overflow;           // set to 1 if a lane's stack exceeded the spill ring
     int nontemporal;         // 1: stream rays in and hits out past the caches
     const int* roots;        // batched views: root node per view (null: root 0)
@@ -80,7 +86,7 @@ struct TraceLaunch {
     int refillBelow = 40;   // with refill: a wave fetches when fewer lanes than this are busy
     bool whileWhile = true; // speculative while-while loop (inner-node and leaf phases)
 };
-hipError_t launchTrace(const TraceLaunch& l, bool anyHit, bool stats, int blocks, const TraceArgs& a, hipStream_t s);
+hipError_t launchTrace(const TraceLaunch& l, bool stats, int blocks, const TraceArgs& a, hipStream_t s);
 hipError_t launchResetCounters(int* counters, int initialRays, hipStream_t s);
 hipError_t launchRaygen(const FrameArgs& f, hipStream_t s);
 hipError_t launchShade(const FrameArgs& f, int blocks, hipStream_t s);

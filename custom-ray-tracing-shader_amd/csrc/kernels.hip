@@ -45,8 +45,16 @@ struct LaneRay {
     float best, bu, bv;
     int bestIdx, sp, ref;
     int leaf;       // postponed leaf reference (< 0), or >= 0 for none
-    int index;      // queue position of the ray, -1 when the lane is idle
-    int slot;       // path slot (shadow mode)
+    int index;      // position of the ray in its queue, -1 when the lane is idle
+    int slot;       // path slot
+    bool any;       // any-hit query: answered by the first accepted hit
+};
+
+// Per-lane node visits and triangle tests by query kind (measurement launches only).
+struct TraceCounters {
+    unsigned long long nodesC = 0, trisC = 0, nodesA = 0, trisA = 0;
+    __device__ __forceinline__ void node(bool any) { if (any) nodesA++; else nodesC++; }
+    __device__ __forceinline__ void tri(bool any) { if (any) trisA++; else trisC++; }
 };
 
 template <int S>
@@ -131,10 +139,11 @@ __device__ __forceinline__ void childEntries<WideNode<8>>(const WideNode<8>* nod
 // Pushes the entered children that are not descended into; returns whether the lane
 // descends (into *next).  Closest-hit queries go to the nearest child and push the others
 // far to near.  Any-hit queries take the first entered child in slot order and skip the
-// sort: whether the ray is occluded does not depend on the order.
-template <int W, bool ANY, int S>
+// sort: whether the ray is occluded does not depend on the order.  MIXED: the launch has
+// any-hit rays (otherwise every query is closest-hit and the any-hit paths compile away).
+template <int W, int S, bool MIXED>
 __device__ __forceinline__ bool descend(LaneRay& L, LdsStack<S>& st, float* dist, int* cr, bool& overflow, int* next) {
-    if (ANY) {
+    if (MIXED && L.any) {
         bool taken = false;
 #pragma unroll
         for (int c = 0; c < W; c++) {
@@ -155,16 +164,16 @@ __device__ __forceinline__ bool descend(LaneRay& L, LdsStack<S>& st, float* dist
 // One traversal step: visit an inner node (push far children, descend to the nearest) or
 // test a leaf, then pop until a deferred entry survives the current best distance.
 // Returns true when the ray is finished.
-template <class N, bool ANY, bool STATS, int S>
+template <class N, bool MIXED, bool STATS, int S>
 __device__ __forceinline__ bool traceStep(LaneRay& L, const N* nodes, const float4* recs, LdsStack<S>& st,
-                                          bool& overflow, unsigned long long& nv, unsigned long long& tt) {
+                                          bool& overflow, TraceCounters& tc) {
     constexpr int W = NodeTraits<N>::kWidth;
     if (L.ref >= 0) {
-        if (STATS) nv++;
+        if (STATS) tc.node(L.any);
         float dist[W]; int cr[W];
         childEntries<N>(nodes, L, dist, cr);
         int next;
-        if (descend<W, ANY, S>(L, st, dist, cr, overflow, &next)) { L.ref = next; return false; }
+        if (descend<W, S, MIXED>(L, st, dist, cr, overflow, &next)) { L.ref = next; return false; }
     } else {
         // Records k.. of the leaf, kLeafBatch at a time; the reference count bounds the first
         // batches exactly, larger leaves continue until the last-in-leaf flag.
@@ -187,10 +196,10 @@ __device__ __forceinline__ bool traceStep(LaneRay& L, const N* nodes, const floa
                     float u = 0, v = 0;
                     const float t = intersectTri(L.o, L.d, F3(A[j].x, A[j].y, A[j].z), F3(B[j].x, B[j].y, B[j].z),
                                                  F3(C[j].x, C[j].y, C[j].z), (flags & kTriOneSided) != 0, u, v);
-                    if (STATS) tt++;
+                    if (STATS) tc.tri(L.any);
                     if (t > 0 && (t < L.best || (t == L.best && index < L.bestIdx))) {
                         L.best = t; L.bestIdx = index; L.bu = u; L.bv = v;
-                        if (ANY) return true;
+                        if (MIXED && L.any) return true;
                     }
                     last = (flags & kTriLastInLeaf) != 0;
                 }
@@ -224,20 +233,20 @@ __device__ __forceinline__ int popNext(LaneRay& L, LdsStack<S>& st) {
     return kDone;
 }
 
-template <class N, bool ANY, bool STATS, int S>
+template <class N, bool MIXED, bool STATS, int S>
 __device__ __forceinline__ void visitNode(LaneRay& L, const N* nodes, LdsStack<S>& st, bool& overflow,
-                                          unsigned long long& nv) {
+                                          TraceCounters& tc) {
     constexpr int W = NodeTraits<N>::kWidth;
-    if (STATS) nv++;
+    if (STATS) tc.node(L.any);
     float dist[W]; int cr[W];
     childEntries<N>(nodes, L, dist, cr);
     int next;
-    L.ref = descend<W, ANY, S>(L, st, dist, cr, overflow, &next) ? next : popNext<S>(L, st);
+    L.ref = descend<W, S, MIXED>(L, st, dist, cr, overflow, &next) ? next : popNext<S>(L, st);
 }
 
 // Tests the triangles of leaf reference ref; true when an any-hit query is answered.
-template <bool ANY, bool STATS>
-__device__ __forceinline__ bool testLeaf(LaneRay& L, int ref, const float4* recs, unsigned long long& tt) {
+template <bool MIXED, bool STATS>
+__device__ __forceinline__ bool testLeaf(LaneRay& L, int ref, const float4* recs, TraceCounters& tc) {
     int k = leafFirst(ref), known = leafBatch(ref);
     for (;;) {
         const int m = known < kLeafBatch ? known : kLeafBatch;
@@ -255,10 +264,10 @@ __device__ __forceinline__ bool testLeaf(LaneRay& L, int ref, const float4* recs
                 float u = 0, v = 0;
                 const float t = intersectTri(L.o, L.d, F3(A[j].x, A[j].y, A[j].z), F3(B[j].x, B[j].y, B[j].z),
                                              F3(C[j].x, C[j].y, C[j].z), (flags & kTriOneSided) != 0, u, v);
-                if (STATS) tt++;
+                if (STATS) tc.tri(L.any);
                 if (t > 0 && (t < L.best || (t == L.best && index < L.bestIdx))) {
                     L.best = t; L.bestIdx = index; L.bu = u; L.bv = v;
-                    if (ANY) return true;
+                    if (MIXED && L.any) return true;
                 }
                 last = (flags & kTriLastInLeaf) != 0;
             }
@@ -269,15 +278,15 @@ __device__ __forceinline__ bool testLeaf(LaneRay& L, int ref, const float4* recs
     }
 }
 
-template <bool ANY>
 __device__ __forceinline__ void finishRay(const TraceArgs& a, const LaneRay& L) {
-    if (a.hits) {
+    int4* hits = L.any ? a.anyHits : a.hits;
+    if (hits) {
         if (a.nontemporal) {
             typedef int v4i __attribute__((ext_vector_type(4)));
             const v4i hv = {L.bestIdx, __float_as_int(L.best), __float_as_int(L.bu), __float_as_int(L.bv)};
-            __builtin_nontemporal_store(hv, reinterpret_cast<v4i*>(a.hits + L.index));
+            __builtin_nontemporal_store(hv, reinterpret_cast<v4i*>(hits + L.index));
         } else {
-            a.hits[L.index] = make_int4(L.bestIdx, __float_as_int(L.best), __float_as_int(L.bu), __float_as_int(L.bv));
+            hits[L.index] = make_int4(L.bestIdx, __float_as_int(L.best), __float_as_int(L.bu), __float_as_int(L.bv));
         }
     } else if (L.bestIdx < 0) {
         // Shadow query found no occluder: add the stored next-event term.
@@ -298,18 +307,22 @@ __device__ __forceinline__ int findView(const int* viewBase, int views, int slot
     return lo;
 }
 
-__device__ __forceinline__ void startRay(const TraceArgs& a, int i, LaneRay& L) {
+// Ray i of the launch: closest-hit queue entry i, or any-hit entry i - closest when i >= closest.
+__device__ __forceinline__ void startRay(const TraceArgs& a, int i, int closest, LaneRay& L) {
+    L.any = i >= closest;
+    if (L.any) i -= closest;
+    const float4* rays = L.any ? a.anyRays : a.rays;
     // Rays are read once: keep them from displacing BVH nodes and records in the caches.
     float4 r0, r1;
     if (a.nontemporal) {
         typedef float v4f __attribute__((ext_vector_type(4)));
-        const v4f* rp = reinterpret_cast<const v4f*>(a.rays + 2 * i);
+        const v4f* rp = reinterpret_cast<const v4f*>(rays + 2 * i);
         const v4f q0 = __builtin_nontemporal_load(rp), q1 = __builtin_nontemporal_load(rp + 1);
         r0 = make_float4(q0.x, q0.y, q0.z, q0.w);
         r1 = make_float4(q1.x, q1.y, q1.z, q1.w);
     } else {
-        r0 = a.rays[2 * i];
-        r1 = a.rays[2 * i + 1];
+        r0 = rays[2 * i];
+        r1 = rays[2 * i + 1];
     }
     L.o = F3(r0.x, r0.y, r0.z);
     L.d = F3(r1.x, r1.y, r1.z);
@@ -324,7 +337,9 @@ __device__ __forceinline__ void startRay(const TraceArgs& a, int i, LaneRay& L) 
     L.leaf = 0;
 }
 
-// Persistent traversal.  Each wave's first 64 rays are fixed by its position in the grid;
+// Persistent traversal of a closest-hit queue followed by an any-hit queue, in one ray index
+// space, so the two overlap in one launch.  Each wave's first 64 rays are fixed by its
+// position in the grid;
 // after that it takes rays from the shared cursor (which counts the rays past the grid's
 // first batch), so the launch does not start with one atomic per wave on one address.
 // With REFILL, lanes whose ray finished take new rays as soon as fewer than refillBelow
@@ -332,24 +347,25 @@ __device__ __forceinline__ void startRay(const TraceArgs& a, int i, LaneRay& L) 
 #ifndef RT_TRACE_MIN_WAVES
 #define RT_TRACE_MIN_WAVES 1
 #endif
-template <class N, bool ANY, bool STATS, int S, bool REFILL, bool WW>
+template <class N, bool MIXED, bool STATS, int S, bool REFILL, bool WW>
 __global__ __launch_bounds__(kBlock, RT_TRACE_MIN_WAVES) void traceKernel(TraceArgs a) {
     __shared__ int sRef[S * kBlock];
     __shared__ float sDist[S * kBlock];
     const int tid = threadIdx.x, lane = tid & 63;
-    const int total = a.countPtr ? *a.countPtr : a.count;
+    const int closest = a.countPtr ? *a.countPtr : a.count;
+    const int total = closest + (a.anyCountPtr ? *a.anyCountPtr : a.anyCount);
     const N* nodes = static_cast<const N*>(a.nodes);
     const float4* recs = reinterpret_cast<const float4*>(a.recs);
     LdsStack<S> st{sRef, sDist, reinterpret_cast<int2*>(a.spill), a.spillEntries, a.totalLanes,
                    (int)(blockIdx.x * kBlock + tid), tid};
-    unsigned long long nv = 0, tt = 0;
+    TraceCounters tc;
     bool overflow = false;
     LaneRay L;
     L.index = -1;
     const int firstRays = gridDim.x * kBlock;   // one ray per launched lane
     {
         const int i = blockIdx.x * kBlock + tid;
-        if (i < total) startRay(a, i, L);
+        if (i < total) startRay(a, i, closest, L);
     }
     bool exhausted = firstRays >= total;   // wave-uniform: the cursor has passed the end
     for (;;) {
@@ -362,7 +378,7 @@ __global__ __launch_bounds__(kBlock, RT_TRACE_MIN_WAVES) void traceKernel(TraceA
             if (L.index < 0) {
                 const int rank = (int)__builtin_amdgcn_mbcnt_hi((unsigned)(idle >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)idle, 0u));
                 const int i = base + rank;
-                if (i < total) startRay(a, i, L);
+                if (i < total) startRay(a, i, closest, L);
             }
             exhausted = base + __popcll(idle) >= total;
         }
@@ -371,19 +387,19 @@ __global__ __launch_bounds__(kBlock, RT_TRACE_MIN_WAVES) void traceKernel(TraceA
             for (;;) {
                 bool searching = true;
                 while ((unsigned)L.ref < (unsigned)kDone) {
-                    visitNode<N, ANY, STATS, S>(L, nodes, st, overflow, nv);
+                    visitNode<N, MIXED, STATS, S>(L, nodes, st, overflow, tc);
                     if (L.ref < 0 && L.leaf >= 0) { searching = false; L.leaf = L.ref; L.ref = popNext<S>(L, st); }
                     if (!__any(searching)) break;
                 }
                 bool answered = false;
                 while (L.leaf < 0) {
-                    if (testLeaf<ANY, STATS>(L, L.leaf, recs, tt)) { answered = true; break; }
+                    if (testLeaf<MIXED, STATS>(L, L.leaf, recs, tc)) { answered = true; break; }
                     L.leaf = L.ref;
                     if (L.ref < 0) L.ref = popNext<S>(L, st);
                 }
                 if (answered) { L.ref = kDone; L.leaf = 0; }
                 if (L.ref == kDone && L.leaf >= 0) {
-                    finishRay<ANY>(a, L);
+                    finishRay(a, L);
                     L.index = -1;
                     break;
                 }
@@ -392,8 +408,8 @@ __global__ __launch_bounds__(kBlock, RT_TRACE_MIN_WAVES) void traceKernel(TraceA
         }
         if (!WW && L.index >= 0) {
             for (;;) {
-                if (traceStep<N, ANY, STATS, S>(L, nodes, recs, st, overflow, nv, tt)) {
-                    finishRay<ANY>(a, L);
+                if (traceStep<N, MIXED, STATS, S>(L, nodes, recs, st, overflow, tc)) {
+                    finishRay(a, L);
                     L.index = -1;
                     break;
                 }
@@ -402,16 +418,19 @@ __global__ __launch_bounds__(kBlock, RT_TRACE_MIN_WAVES) void traceKernel(TraceA
         }
     }
     if (STATS) {
-        atomicAdd(&a.stats[0], nv);
-        atomicAdd(&a.stats[1], tt);
+        atomicAdd(&a.stats[0], tc.nodesC);
+        atomicAdd(&a.stats[1], tc.trisC);
+        atomicAdd(&a.stats[2], tc.nodesA);
+        atomicAdd(&a.stats[3], tc.trisA);
     }
     if (overflow) atomicOr(a.overflow, 1);
 }
 
 template <class N, int S, bool R, bool WW>
-static void launchTraceNS(bool anyHit, bool stats, int blocks, const TraceArgs& a, hipStream_t s) {
+static void launchTraceNS(bool stats, int blocks, const TraceArgs& a, hipStream_t s) {
     dim3 grid(blocks), block(kBlock);
-    if (anyHit) {
+    const bool mixed = a.anyRays != nullptr;
+    if (mixed) {
         if (stats) hipLaunchKernelGGL((traceKernel<N, true, true, S, R, WW>), grid, block, 0, s, a);
         else hipLaunchKernelGGL((traceKernel<N, true, false, S, R, WW>), grid, block, 0, s, a);
     } else {
@@ -420,39 +439,39 @@ static void launchTraceNS(bool anyHit, bool stats, int blocks, const TraceArgs& 
     }
 }
 template <class N>
-static hipError_t launchTraceN(bool anyHit, bool stats, int blocks, const TraceLaunch& l, const TraceArgs& a,
-                               hipStream_t s) {
+static hipError_t launchTraceN(bool stats, int blocks, const TraceLaunch& l, const TraceArgs& a, hipStream_t s) {
     const int key = (l.stack == 16 ? 4 : l.stack == 8 ? 0 : -100) + (l.refill ? 2 : 0) + (l.whileWhile ? 1 : 0);
     switch (key) {
-        case 0: launchTraceNS<N, 8, false, false>(anyHit, stats, blocks, a, s); break;
-        case 1: launchTraceNS<N, 8, false, true>(anyHit, stats, blocks, a, s); break;
-        case 2: launchTraceNS<N, 8, true, false>(anyHit, stats, blocks, a, s); break;
-        case 3: launchTraceNS<N, 8, true, true>(anyHit, stats, blocks, a, s); break;
-        case 4: launchTraceNS<N, 16, false, false>(anyHit, stats, blocks, a, s); break;
-        case 5: launchTraceNS<N, 16, false, true>(anyHit, stats, blocks, a, s); break;
-        case 6: launchTraceNS<N, 16, true, false>(anyHit, stats, blocks, a, s); break;
-        case 7: launchTraceNS<N, 16, true, true>(anyHit, stats, blocks, a, s); break;
+        case 0: launchTraceNS<N, 8, false, false>(stats, blocks, a, s); break;
+        case 1: launchTraceNS<N, 8, false, true>(stats, blocks, a, s); break;
+        case 2: launchTraceNS<N, 8, true, false>(stats, blocks, a, s); break;
+        case 3: launchTraceNS<N, 8, true, true>(stats, blocks, a, s); break;
+        case 4: launchTraceNS<N, 16, false, false>(stats, blocks, a, s); break;
+        case 5: launchTraceNS<N, 16, false, true>(stats, blocks, a, s); break;
+        case 6: launchTraceNS<N, 16, true, false>(stats, blocks, a, s); break;
+        case 7: launchTraceNS<N, 16, true, true>(stats, blocks, a, s); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
-hipError_t launchTrace(const TraceLaunch& l, bool anyHit, bool stats, int blocks, const TraceArgs& args,
-                       hipStream_t s) {
+hipError_t launchTrace(const TraceLaunch& l, bool stats, int blocks, const TraceArgs& args, hipStream_t s) {
     if (l.refillBelow < 1 || l.refillBelow > 64) return hipErrorInvalidValue;
+    if ((!args.rays && (args.countPtr || args.count)) || (!args.anyRays && (args.anyCountPtr || args.anyCount)))
+        return hipErrorInvalidValue;
     TraceArgs a = args;
     a.refillBelow = l.refillBelow;
     if (l.quantized) {
         switch (l.width) {
-            case 2: return launchTraceN<QNode<2>>(anyHit, stats, blocks, l, a, s);
-            case 4: return launchTraceN<QNode<4>>(anyHit, stats, blocks, l, a, s);
-            case 8: return launchTraceN<QNode<8>>(anyHit, stats, blocks, l, a, s);
+            case 2: return launchTraceN<QNode<2>>(stats, blocks, l, a, s);
+            case 4: return launchTraceN<QNode<4>>(stats, blocks, l, a, s);
+            case 8: return launchTraceN<QNode<8>>(stats, blocks, l, a, s);
             default: return hipErrorInvalidValue;
         }
     }
     switch (l.width) {
-        case 2: return launchTraceN<WideNode<2>>(anyHit, stats, blocks, l, a, s);
-        case 4: return launchTraceN<WideNode<4>>(anyHit, stats, blocks, l, a, s);
-        case 8: return launchTraceN<WideNode<8>>(anyHit, stats, blocks, l, a, s);
+        case 2: return launchTraceN<WideNode<2>>(stats, blocks, l, a, s);
+        case 4: return launchTraceN<WideNode<4>>(stats, blocks, l, a, s);
+        case 8: return launchTraceN<WideNode<8>>(stats, blocks, l, a, s);
         default: return hipErrorInvalidValue;
     }
 }

@@ -140,7 +140,8 @@ int RtCopyOutput(void* session, unsigned char* output, int outputBytes);
 
 /* Measurement hooks.  With option "stage_timing" = 1 each GPU kernel launch of RtRender is
  * bracketed by HIP events on the session's stream; RtStageTimes returns, per stage
- * (0 reset, 1 raygen, 2 closest-hit traversal, 3 shade, 4 shadow traversal, 5 accumulate),
+ * (0 reset, 1 raygen, 2 traversal of a bounce's closest-hit rays together with the previous
+ * bounce's shadow rays, 3 shade, 4 traversal of the last bounce's shadow rays, 5 accumulate),
  * the summed milliseconds and launch counts of the last frame.  With option "trace_stats"
  * = 1 the traversal kernels also count node visits and triangle tests; RtTraceStats returns
  * {closest nodes, closest triangles, shadow nodes, shadow triangles} of the last frame. */
