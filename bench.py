@@ -173,11 +173,11 @@ def psnr_study(args):
     def plain(spp, seed=42):
         lib.RtReset(s.handle)
         st.seed = seed
-        done, t = 0, time.perf_counter()
-        while done < spp:
-            st.raysPerPixel = min(16, spp - done)
+        st.raysPerPixel = td.rays_per_call(spp)   # equal calls: changed settings restart accumulation
+        t = time.perf_counter()
+        for _ in range(spp // st.raysPerPixel):
             img, _ = s.render(cam, st)
-            done += st.raysPerPixel
+        assert lib.RtSampleCount(s.handle) == spp
         return img, time.perf_counter() - t
 
     ladder = []

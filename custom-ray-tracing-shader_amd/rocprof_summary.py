@@ -18,6 +18,8 @@ import csv
 import glob
 import json
 import os
+import re
+import sys
 from collections import defaultdict
 
 
@@ -51,7 +53,9 @@ def main():
     ap.add_argument("--copy-fetch", required=True)
     ap.add_argument("--copy-write", required=True)
     ap.add_argument("--copy-bytes", type=int, required=True)
-    ap.add_argument("--kernel", required=True, help="substring naming the dominant kernel")
+    ap.add_argument("--kernel", required=True,
+                    help="regular expression naming the dominant kernel; launches of every matching "
+                         "instantiation are pooled (means weighted by calls)")
     ap.add_argument("--title", required=True)
     ap.add_argument("--commands", required=True, help="file holding the commands that made the inputs")
     ap.add_argument("--bench-json", help="bench.py output line of the same workload (algorithmic bytes)")
@@ -74,7 +78,18 @@ def main():
              f"FETCH_SIZE x {fetch_scale:.3f}, WRITE_SIZE x {write_scale:.3f}.", "",
              "| kernel | calls | mean us | % time | HBM read MB/launch | HBM write MB/launch | L2 hit % |",
              "|---|---:|---:|---:|---:|---:|---:|"]
-    dominant = None
+    pat = re.compile(args.kernel)
+    pool = []   # (name, calls, mean ns, read B, write B, hit %) of the matching kernels
+    for r in stats:
+        name = r["Name"]
+        if pat.search(name):
+            f = fetch.get(name, {}).get("FETCH_SIZE")
+            w = write.get(name, {}).get("WRITE_SIZE")
+            t = tcc.get(name, {})
+            pool.append((name, int(r["Calls"]), float(r["AverageNs"]),
+                         f * fetch_scale * 1024 if f is not None else None,
+                         w * write_scale * 1024 if w is not None else None,
+                         (t["TCC_HIT_sum"], t["TCC_HIT_sum"] + t["TCC_MISS_sum"]) if t else None))
     for r in sorted(stats, key=lambda r: -float(r["Percentage"]))[:14]:
         name = r["Name"]
         f = fetch.get(name, {}).get("FETCH_SIZE")
@@ -86,21 +101,26 @@ def main():
         fmt = lambda v, p=1: "—" if v is None else f"{v:.{p}f}"
         lines.append(f"| `{short(name)}` | {r['Calls']} | {float(r['AverageNs']) / 1e3:.1f} | {float(r['Percentage']):.1f} "
                      f"| {fmt(rd)} | {fmt(wr)} | {fmt(hit)} |")
-        if dominant is None and args.kernel in name:
-            dominant = {"kernel": name, "mean_ms": float(r["AverageNs"]) / 1e6, "calls": int(r["Calls"]),
-                        "hbm_read_bytes_per_launch": rd * 1e6 if rd is not None else None,
-                        "hbm_write_bytes_per_launch": wr * 1e6 if wr is not None else None,
-                        "l2_hit_percent": hit}
-    if dominant is None:
+    if not pool:
         raise SystemExit(f"kernel {args.kernel!r} not in the trace")
+    calls = sum(p[1] for p in pool)
+    wmean = lambda i: (sum(p[1] * p[i] for p in pool) / calls) if all(p[i] is not None for p in pool) else None
+    hits = [p[5] for p in pool if p[5]]
+    dominant = {"kernel": " + ".join(p[0] for p in pool), "kernel_pattern": args.kernel,
+                "mean_ms": wmean(2) / 1e6, "calls": calls,
+                "hbm_read_bytes_per_launch": wmean(3), "hbm_write_bytes_per_launch": wmean(4),
+                "l2_hit_percent": 100 * sum(h[0] for h in hits) / max(sum(h[1] for h in hits), 1) if hits else None}
     traffic = (dominant["hbm_read_bytes_per_launch"] or 0) + (dominant["hbm_write_bytes_per_launch"] or 0)
     dominant["hbm_bytes_per_launch"] = traffic
     dominant["fetch_scale"], dominant["write_scale"] = fetch_scale, write_scale
-    lines += ["", f"Dominant kernel `{short(dominant['kernel'], 120)}`: mean {dominant['mean_ms'] * 1e3:.1f} us over "
-              f"{dominant['calls']} launches; HBM traffic {traffic / 1e6:.1f} MB per launch "
-              f"({traffic / dominant['mean_ms'] / 1e6:.0f} GB/s of HBM)."]
+    lines += ["", f"Dominant kernel (all launches matching `{args.kernel}`: "
+              + ", ".join(f"`{short(p[0], 90)}` x{p[1]}" for p in pool)
+              + f"): mean {dominant['mean_ms'] * 1e3:.1f} us over {calls} launches; HBM traffic "
+              f"{traffic / 1e6:.1f} MB per launch ({traffic / dominant['mean_ms'] / 1e6:.0f} GB/s of HBM)."]
     if args.sq:
-        sq = counter_means(args.sq).get(dominant["kernel"])
+        sqm = counter_means(args.sq)
+        sqs = [(p[1], sqm[p[0]]) for p in pool if p[0] in sqm]
+        sq = ({c: sum(n * d[c] for n, d in sqs) / sum(n for n, _ in sqs) for c in sqs[0][1]} if sqs else None)
         if sq:
             waves = max(sq["SQ_WAVES"], 1)
             util = sq["SQ_THREAD_CYCLES_VALU"] / max(sq["SQ_ACTIVE_INST_VALU"] * 64, 1)
@@ -120,6 +140,8 @@ def main():
                   f"triangles per ray), i.e. {alg / max(traffic, 1):.1f}x the HBM traffic: the rest is served "
                   f"by L2 / MALL."]
         dominant["algorithmic_bytes_per_launch"] = alg
+    lines += ["", "Summary made with:", "```", "python3 custom-ray-tracing-shader_amd/rocprof_summary.py " +
+              " ".join(f'"{a}"' if (" " in a or "|" in a) else a for a in sys.argv[1:]), "```"]
     with open(args.md, "w") as fh:
         fh.write("\n".join(lines) + "\n")
     with open(args.json, "w") as fh:

@@ -45,16 +45,22 @@ def psnr(a, b):
     return 99.0 if rmse == 0 else 20 * math.log10(255 / rmse)
 
 
+def rays_per_call(spp):
+    """Samples per RtRender call for an spp-sample image: the largest divisor of spp that is
+    at most 16.  Every call must use the same raysPerPixel, since a call whose settings
+    differ from the previous one's starts the accumulation over (Native/renderer.cu:388-389)."""
+    return max(d for d in range(1, 17) if spp % d == 0)
+
+
 def render_mean(session, camera, settings, spp, seed):
-    """Mean HDR radiance [h][w][3] of spp samples with the given seed (16 per call)."""
+    """Mean HDR radiance [h][w][3] of spp samples with the given seed."""
     lib = session.lib
     lib.RtReset(session.handle)
     settings.seed = seed
-    done = 0
-    while done < spp:
-        settings.raysPerPixel = min(16, spp - done)
+    settings.raysPerPixel = rays_per_call(spp)
+    for _ in range(spp // settings.raysPerPixel):
         session.render(camera, settings)
-        done += settings.raysPerPixel
+    assert lib.RtSampleCount(session.handle) == spp
     acc = session.accumulation()
     return acc[..., 0:3] / spp
 

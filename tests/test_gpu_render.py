@@ -289,3 +289,42 @@ def test_quantized_nodes_leave_images_unchanged(lib, width):
     assert np.array_equal(out[0][1].view(np.uint32), out[1][1].view(np.uint32))
     assert out[0][2] == out[1][2]
     assert np.array_equal(out[0][3], out[1][3])
+
+
+def test_frame_launches_pair_shadow_rays_with_the_next_bounce(lib):
+    """Each traversal launch of a frame traces one bounce's closest-hit rays together with
+    the previous bounce's shadow rays; only the last bounce's shadow rays get a launch of
+    their own, and a scene without emitters has none."""
+    from scenes import cornell, cornell_camera, make_settings, triangle_soup
+    for tris, st, shadow_launches in ((cornell(), make_settings(4, rpp=1), 1),
+                                      (triangle_soup(2000, seed=4), make_settings(3, rpp=1, sky=1.0), 0)):
+        with Session(lib) as s:
+            s.initialize(tris, 64, 48)
+            s.set_option("stage_timing", 1)
+            s.render_resident(cornell_camera(64, 48), st)
+            stages = s.stage_times()
+        assert stages["closest"][1] == st.maxBounces
+        assert stages["shade"][1] == st.maxBounces
+        assert stages["shadow"][1] == shadow_launches
+
+
+def test_frames_identical_across_traversal_loops(lib):
+    """Closest-hit and shadow rays share launches, and lanes refill across the two queues:
+    every loop variant must still make the same hits, so frames agree bit for bit."""
+    from scenes import displaced_icosphere, icosphere_camera, make_settings
+    tris = displaced_icosphere(subdiv=5, seed=11)
+    cam, st = icosphere_camera(96, 64), make_settings(8, rpp=2, seed=9)
+    out = []
+    for ww, refill, below in ((0, 0, 40), (1, 1, 40), (1, 0, 40), (0, 1, 8), (1, 1, 60)):
+        with Session(lib) as s:
+            s.set_option("while_while", ww)
+            s.set_option("refill", refill)
+            s.set_option("refill_below", below)
+            s.set_option("blocks_per_cu", 1)   # few lanes: every lane refills many times
+            s.initialize(tris, 96, 64)
+            for _ in range(2):
+                s.render_resident(cam, st)
+            out.append((s.accumulation(), s.ray_counts()))
+    for acc, counts in out[1:]:
+        assert np.array_equal(acc.view(np.uint32), out[0][0].view(np.uint32))
+        assert counts == out[0][1]
