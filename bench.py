@@ -37,10 +37,12 @@ HBM_PEAK_GBS = 8000.0
 # The soup keeps the reference builder's leaf <= 4 rule (its configuration names it); the
 # icosphere uses the autotuner's leaf size.
 TUNED = {
-    "soup": {"bvh_width": 4, "build_max_leaf": 4, "blocks_per_cu": 4, "short_stack": 16, "while_while": 1, "refill": 1},
+    "soup": {"bvh_width": 4, "build_max_leaf": 4, "blocks_per_cu": 4, "short_stack": 16, "while_while": 1, "refill": 1,
+             "pipelines": 2},
     "icosphere": {"bvh_width": 4, "build_max_leaf": 2, "blocks_per_cu": 6, "short_stack": 8, "while_while": 0,
-                  "refill": 0},
-    "cornell": {"bvh_width": 4, "build_max_leaf": 4, "blocks_per_cu": 6, "short_stack": 8, "while_while": 1, "refill": 0},
+                  "refill": 0, "pipelines": 4},
+    "cornell": {"bvh_width": 4, "build_max_leaf": 4, "blocks_per_cu": 6, "short_stack": 8, "while_while": 1, "refill": 0,
+                "pipelines": 4},
     "batch": {"bvh_width": 4, "blocks_per_cu": 6, "short_stack": 8, "while_while": 1, "refill": 1},
 }
 
@@ -256,6 +258,8 @@ def main():
     ap.add_argument("--short-stack", type=int)
     ap.add_argument("--refill", type=int)
     ap.add_argument("--refill-below", type=int)
+    ap.add_argument("--pipelines", type=int, choices=(1, 2, 3, 4),
+                    help="path pipelines per frame (the library's default is 2)")
     ap.add_argument("--build-max-leaf", type=int)
     ap.add_argument("--bvh-quantized", type=int, choices=(0, 1))
     ap.add_argument("--while-while", type=int, choices=(0, 1))
@@ -291,7 +295,7 @@ def main():
     log(f"workload: {desc}")
     cfg = dict(TUNED[args.workload])
     for k in ("bvh_width", "bvh_quantized", "build_max_leaf", "blocks_per_cu", "short_stack", "refill", "refill_below",
-              "while_while"):
+              "while_while", "pipelines"):
         if getattr(args, k) is not None:
             cfg[k] = getattr(args, k)
     if args.autotune:
@@ -303,7 +307,7 @@ def main():
         log(f"autotune: {len(trials)} configurations in {time.perf_counter() - t0:.1f} s, best {best}")
     s = native_api.Session()
     for k in ("bvh_width", "bvh_quantized", "build_max_leaf", "blocks_per_cu", "short_stack", "refill", "refill_below",
-              "while_while"):
+              "while_while", "pipelines"):
         if k in cfg:
             s.set_option(k, cfg[k])
     t0 = time.perf_counter()
@@ -346,6 +350,7 @@ def main():
     width = s.lib.RtGetOption(s.handle, b"bvh_width")
     quantized = s.lib.RtGetOption(s.handle, b"bvh_quantized")
     node_b = s.lib.RtGetOption(s.handle, b"node_bytes")
+    pipelines = s.lib.RtGetOption(s.handle, b"pipelines")   # timed frames; measurement frames run one
     # The traversal kernel: each launch traces one bounce's closest-hit rays together with the
     # previous bounce's shadow rays (the "closest" stage), the last launch only shadow rays.
     launches = (stage_ms["closest"][1] + stage_ms.get("shadow", [0, 0])[1]) / 3
@@ -398,6 +403,7 @@ def main():
         "data": "synthetic (seeded scene generator)",
         "config": {"workload": desc, "triangles": int(len(tris)), "width": w, "height": h,
                    "bvh_width": width, "launch": cfg, "autotuned_now": args.autotune,
+                   "pipelines": pipelines,
                    "rays_per_step": int(rays / args.steps),
                    "device_ms_per_step": round(device_ms / args.steps, 4), "bvh_build_ms": round(build_ms, 1)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -1,10 +1,11 @@
 """Autotuner for the traversal launch: BVH width and leaf size, persistent workgroups per
-CU, LDS short-stack depth, and the traversal loop (speculative while-while with per-lane ray
-refill, or one step per iteration).
+CU, LDS short-stack depth, the traversal loop (speculative while-while with per-lane ray
+refill, or one step per iteration) and the number of concurrent path pipelines.
 
 For a scene, camera and settings it initialises one session per (BVH width, leaf size),
 times a few device-resident frames for every (blocks_per_cu, short_stack, loop) combination,
-and keeps the configuration with the highest ray throughput.  Choices persist in a JSON cache keyed by
+then tries each pipeline count on the fastest of those, and keeps the configuration with the
+highest ray throughput.  Choices persist in a JSON cache keyed by
 scene size class and resolution ($RT_AUTOTUNE_CACHE, default ~/.cache/rt_amd/autotune.json),
 so later sessions of a similar scene reuse them without searching again.
 """
@@ -21,6 +22,7 @@ MAX_LEAF = (4,)
 BLOCKS = (4, 6, 8)
 STACKS = (8, 16)
 LOOPS = ((1, 1), (1, 0), (0, 0))   # (while_while, refill)
+PIPELINES = (1, 2, 3, 4)
 
 
 def cache_path():
@@ -49,7 +51,7 @@ def store_cache(cache):
 
 
 def apply(session, cfg):
-    for key in ("blocks_per_cu", "short_stack", "refill", "while_while"):
+    for key in ("blocks_per_cu", "short_stack", "refill", "while_while", "pipelines"):
         if key not in cfg:
             continue
         if not session.set_option(key, cfg[key]):
@@ -76,6 +78,17 @@ def autotune(triangles, camera, settings, width, height, frames=3, widths=WIDTHS
     results = []
     best = None
     configs = [(w, ml) for w in widths for ml in max_leaf]
+
+    def trial(s, cfg):
+        nonlocal best
+        apply(s, cfg)
+        cfg["mrays_per_s"] = round(measure(s, camera, settings, frames), 2)
+        results.append(cfg)
+        if log:
+            log(f"autotune {cfg}")
+        if best is None or cfg["mrays_per_s"] > best["mrays_per_s"]:
+            best = dict(cfg)
+
     for w, ml in configs:
         with Session(lib) as s:
             s.set_option("bvh_width", w)
@@ -84,15 +97,16 @@ def autotune(triangles, camera, settings, width, height, frames=3, widths=WIDTHS
             for b in BLOCKS:
                 for st in STACKS:
                     for ww, r in LOOPS:
-                        cfg = {"bvh_width": w, "build_max_leaf": ml, "blocks_per_cu": b, "short_stack": st,
-                               "while_while": ww, "refill": r}
-                        apply(s, cfg)
-                        cfg["mrays_per_s"] = round(measure(s, camera, settings, frames), 2)
-                        results.append(cfg)
-                        if log:
-                            log(f"autotune {cfg}")
-                        if best is None or cfg["mrays_per_s"] > best["mrays_per_s"]:
-                            best = dict(cfg)
+                        trial(s, {"bvh_width": w, "build_max_leaf": ml, "blocks_per_cu": b, "short_stack": st,
+                                  "while_while": ww, "refill": r, "pipelines": 2})
+    with Session(lib) as s:
+        s.set_option("bvh_width", best["bvh_width"])
+        s.set_option("build_max_leaf", best["build_max_leaf"])
+        s.initialize(triangles, width, height)
+        launch = {k: v for k, v in best.items() if k != "mrays_per_s"}
+        for p in PIPELINES:
+            if p != launch["pipelines"]:
+                trial(s, dict(launch, pipelines=p))
     if use_cache:
         cache[key] = best
         store_cache(cache)

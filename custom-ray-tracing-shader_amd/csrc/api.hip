@@ -46,6 +46,7 @@ struct Session {
     int refillBelow = 40;        // busy lanes below which a refilling wave fetches rays
     int whileWhile = 1;          // speculative while-while traversal loop
     int tiledPrimary = 1;        // primary rays queued in 8x8 pixel tiles
+    int pipelines = 2;           // independent path pipelines per chunk, each on its own stream
     int nontemporal = 1;         // rays / hits bypass cache retention in traversal
     std::vector<unsigned char> hostNodes;
     std::vector<TriRecord> hostRecs;
@@ -62,15 +63,17 @@ struct Session {
     float4 *dThr = nullptr, *dRad = nullptr, *dAlb = nullptr, *dNrm = nullptr, *dNee = nullptr;
     float4 *dRayA = nullptr, *dRayB = nullptr, *dShadow = nullptr;
     int4* dHits = nullptr;
-    int* dCounters = nullptr;     // kMaxRaysPerPixel chunks x kCounterStride
+    int* dCounters = nullptr;     // kMaxRaysPerPixel chunks x kPipelines x kCounterStride
     int* dSpill = nullptr;
     int* dOverflow = nullptr;
     unsigned long long* dStats = nullptr;
     long long slotCap = 0;
     int spillEntries = 0, traceBlocks = 0, traceLanes = 0, cus = 0;
-    size_t spillLanes = 0;           // lanes the spill ring is sized for
+    size_t spillLanes = 0;           // lanes each pipeline's spill ring is sized for
     hipStream_t stream = nullptr;
+    hipStream_t pipeStream[3] = {};  // streams of path pipelines 1..3 (default priority, as stream)
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t evFork = nullptr, evJoin[3] = {};
     // progressive state (Native/renderer.cu:65-68)
     int cpuThreads = 0, lastBackend = -1;
     unsigned samples = 0, version = 0;
@@ -97,20 +100,23 @@ struct Session {
     float lastInferMs = 0;
 };
 
+constexpr int kPipelines = 4;   // path pipelines per chunk at most
+
 // Brackets one launch with pooled events when stage timing is on.
 struct StageScope {
     Session* s;
     int stage, ev;
-    StageScope(Session* s_, int stage_) : s(s_), stage(stage_), ev(-1) {
+    hipStream_t st;
+    StageScope(Session* s_, int stage_, hipStream_t st_) : s(s_), stage(stage_), ev(-1), st(st_) {
         if (!s->stageTiming) return;
         const int used = (int)s->timedLaunches.size() * 2;
         if (used + 2 > (int)s->eventPool.size()) return;
         ev = used;
-        (void)hipEventRecord(s->eventPool[ev], s->stream);
+        (void)hipEventRecord(s->eventPool[ev], st);
     }
     ~StageScope() {
         if (ev < 0) return;
-        (void)hipEventRecord(s->eventPool[ev + 1], s->stream);
+        (void)hipEventRecord(s->eventPool[ev + 1], st);
         s->timedLaunches.emplace_back(stage, ev);
     }
 };
@@ -141,8 +147,14 @@ void freeDevice(Session* s) {
     s->eventPool.clear();
     if (s->ev0) (void)hipEventDestroy(s->ev0);
     if (s->ev1) (void)hipEventDestroy(s->ev1);
+    if (s->evFork) (void)hipEventDestroy(s->evFork);
     if (s->stream) (void)hipStreamDestroy(s->stream);
-    s->ev0 = s->ev1 = nullptr; s->stream = nullptr;
+    for (int k = 0; k < kPipelines - 1; k++) {
+        if (s->evJoin[k]) (void)hipEventDestroy(s->evJoin[k]);
+        if (s->pipeStream[k]) (void)hipStreamDestroy(s->pipeStream[k]);
+        s->evJoin[k] = nullptr; s->pipeStream[k] = nullptr;
+    }
+    s->ev0 = s->ev1 = s->evFork = nullptr; s->stream = nullptr;
 }
 
 SceneView deviceScene(const Session* s) {
@@ -212,16 +224,18 @@ bool prepareTraceGrid(Session* s) {
     if ((size_t)s->traceLanes > s->spillLanes) {
         devFree(s->dSpill);
         s->spillLanes = 0;
-        if (!devAlloc(s, s->dSpill, (size_t)s->spillEntries * s->traceLanes * 2)) return false;
+        if (!devAlloc(s, s->dSpill, (size_t)kPipelines * s->spillEntries * s->traceLanes * 2)) return false;
         s->spillLanes = (size_t)s->traceLanes;
     }
     return true;
 }
 
-TraceArgs baseTraceArgs(Session* s) {
+// Traversal arguments of path pipeline p (each concurrent pipeline has its own spill ring).
+TraceArgs baseTraceArgs(Session* s, int p = 0) {
     TraceArgs a{};
     a.nodes = s->dNodes; a.recs = s->dRecs;
-    a.spill = s->dSpill; a.spillEntries = s->spillEntries; a.totalLanes = s->traceLanes;
+    a.spill = s->dSpill + (size_t)p * s->spillEntries * s->spillLanes * 2;
+    a.spillEntries = s->spillEntries; a.totalLanes = s->traceLanes;
     a.overflow = s->dOverflow;
     a.nontemporal = s->nontemporal;
     return a;
@@ -244,50 +258,81 @@ bool renderGpu(Session* s, const CameraData& cam, const RenderSettings& rs, floa
     if (s->traceStats && (s->error = hipMemsetAsync(s->dStats, 0, 4 * sizeof(unsigned long long), st)) != hipSuccess)
         return false;
     if ((s->error = hipEventRecord(s->ev0, st)) != hipSuccess) return false;
+    // Path pipelines: a chunk's queue positions are split in P parts, each traced and shaded
+    // on its own stream with its own counters, queues and spill ring, so one part's kernels
+    // fill the GPU while another's launch drains its last rays.  Measurement frames (stage
+    // timing, trace statistics) run one pipeline, so launch times are not stretched by
+    // overlap.
+    const bool nee = !s->lights.empty();
+    const bool measure = s->stageTiming || s->traceStats;
+    std::vector<int> chunkPipelines;
     int chunk = 0;
     for (int r0 = 0; r0 < rpp; r0 += chunkR, chunk++) {
         const int R = std::min(chunkR, rpp - r0);
-        int* ctr = s->dCounters + (size_t)chunk * kCounterStride;
-        FrameArgs f{};
-        f.cam = cam; f.rs = rs; f.sc = deviceScene(s);
-        f.w = s->width; f.h = s->height; f.npix = npix; f.R = R; f.sample0 = s->samples + (unsigned)r0;
-        f.thr = s->dThr; f.rad = s->dRad; f.alb = s->dAlb; f.nrm = s->dNrm; f.nee = s->dNee;
-        f.shadow = s->dShadow; f.hits = s->dHits; f.counters = ctr; f.tiled = s->tiledPrimary;
-        {
-            StageScope sc(s, RT_STAGE_RESET);
-            if ((s->error = launchResetCounters(ctr, R * npix, st)) != hipSuccess) return false;
+        const int Q = R * npix;
+        const int P = measure ? 1 : std::max(1, std::min(s->pipelines, Q >> 16));   // >= 64K paths each
+        chunkPipelines.push_back(P);
+        hipStream_t pst[kPipelines] = {st, s->pipeStream[0], s->pipeStream[1], s->pipeStream[2]};
+        if (P > 1) {
+            if ((s->error = hipEventRecord(s->evFork, st)) != hipSuccess) return false;
+            for (int p = 1; p < P; p++)
+                if ((s->error = hipStreamWaitEvent(pst[p], s->evFork, 0)) != hipSuccess) return false;
         }
-        f.rayOut = s->dRayA;
-        {
-            StageScope sc(s, RT_STAGE_RAYGEN);
-            if ((s->error = launchRaygen(f, st)) != hipSuccess) return false;
+        int* ctr[kPipelines];
+        FrameArgs f[kPipelines];
+        const int part = (Q / P + 63) / 64 * 64;   // queue positions per pipeline, the last takes the rest
+        for (int p = 0; p < P; p++) {
+            const int qBase = p * part, qCount = p + 1 < P ? part : Q - qBase;
+            ctr[p] = s->dCounters + ((size_t)chunk * kPipelines + p) * kCounterStride;
+            FrameArgs& fp = f[p];
+            fp = FrameArgs{};
+            fp.cam = cam; fp.rs = rs; fp.sc = deviceScene(s);
+            fp.w = s->width; fp.h = s->height; fp.npix = npix; fp.R = R; fp.sample0 = s->samples + (unsigned)r0;
+            fp.thr = s->dThr; fp.rad = s->dRad; fp.alb = s->dAlb; fp.nrm = s->dNrm; fp.nee = s->dNee;
+            // Queue buffers: pipeline p owns positions [qBase, qBase + qCount) of each.
+            fp.shadow = s->dShadow + 2 * (size_t)qBase; fp.hits = s->dHits + qBase;
+            fp.counters = ctr[p]; fp.tiled = s->tiledPrimary;
+            fp.qBase = qBase; fp.qCount = qCount;
+            {
+                StageScope sc(s, RT_STAGE_RESET, pst[p]);
+                if ((s->error = launchResetCounters(ctr[p], qCount, pst[p])) != hipSuccess) return false;
+            }
+            fp.rayOut = s->dRayA + 2 * (size_t)qBase;
+            StageScope sc(s, RT_STAGE_RAYGEN, pst[p]);
+            if ((s->error = launchRaygen(fp, pst[p])) != hipSuccess) return false;
         }
         // Bounce b's closest-hit rays and bounce b-1's shadow rays are independent, so one
         // launch traces both queues; the shadow rays' radiance is still added before shade
         // b adds bounce b's emission, the reference's order (Native/renderer.cu:232-267).
-        const bool nee = !s->lights.empty();
         for (int b = 0; b <= rs.maxBounces; b++) {
-            float4* in = (b & 1) ? s->dRayB : s->dRayA;
-            float4* out = (b & 1) ? s->dRayA : s->dRayB;
             const bool rays = b < rs.maxBounces, shadow = nee && b > 0;
             if (!rays && !shadow) break;
-            TraceArgs ta = baseTraceArgs(s);
-            ta.work = ctr + kWorkRays + b;
-            ta.stats = s->dStats;
-            if (rays) { ta.rays = in; ta.countPtr = ctr + kCntRays + b; ta.hits = s->dHits; }
-            if (shadow) {
-                ta.anyRays = s->dShadow; ta.anyCountPtr = ctr + kCntShadow + (b - 1);
-                ta.rad = s->dRad; ta.contrib = s->dNee;
+            for (int p = 0; p < P; p++) {
+                const size_t off = 2 * (size_t)f[p].qBase;
+                float4* in = ((b & 1) ? s->dRayB : s->dRayA) + off;
+                float4* out = ((b & 1) ? s->dRayA : s->dRayB) + off;
+                TraceArgs ta = baseTraceArgs(s, p);
+                ta.work = ctr[p] + kWorkRays + b;
+                ta.stats = s->dStats;
+                if (rays) { ta.rays = in; ta.countPtr = ctr[p] + kCntRays + b; ta.hits = f[p].hits; }
+                if (shadow) {
+                    ta.anyRays = f[p].shadow; ta.anyCountPtr = ctr[p] + kCntShadow + (b - 1);
+                    ta.rad = s->dRad; ta.contrib = s->dNee;
+                }
+                {
+                    StageScope sc(s, rays ? RT_STAGE_CLOSEST : RT_STAGE_SHADOW, pst[p]);
+                    if ((s->error = launchTrace(traceLaunch(s), s->traceStats, s->traceBlocks, ta, pst[p])) != hipSuccess)
+                        return false;
+                }
+                if (!rays) continue;
+                f[p].bounce = b; f[p].rayIn = in; f[p].rayOut = out;
+                StageScope sc(s, RT_STAGE_SHADE, pst[p]);
+                if ((s->error = launchShade(f[p], s->cus * shadeBlocksPerCU(), pst[p])) != hipSuccess) return false;
             }
-            {
-                StageScope sc(s, rays ? RT_STAGE_CLOSEST : RT_STAGE_SHADOW);
-                if ((s->error = launchTrace(traceLaunch(s), s->traceStats, s->traceBlocks, ta, st)) != hipSuccess)
-                    return false;
-            }
-            if (!rays) break;
-            f.bounce = b; f.rayIn = in; f.rayOut = out;
-            StageScope sc(s, RT_STAGE_SHADE);
-            if ((s->error = launchShade(f, s->cus * shadeBlocksPerCU(), st)) != hipSuccess) return false;
+        }
+        for (int p = 1; p < P; p++) {
+            if ((s->error = hipEventRecord(s->evJoin[p - 1], pst[p])) != hipSuccess) return false;
+            if ((s->error = hipStreamWaitEvent(st, s->evJoin[p - 1], 0)) != hipSuccess) return false;
         }
         AccumArgs a{};
         a.thr = s->dThr; a.rad = s->dRad; a.alb = s->dAlb; a.nrm = s->dNrm;
@@ -297,23 +342,25 @@ bool renderGpu(Session* s, const CameraData& cam, const RenderSettings& rs, floa
         a.writeDisplay = (r0 + R >= rpp) ? 1 : 0;
         a.invSamples = 1.0f / (float)(s->samples + (unsigned)rpp);
         a.exposure = rs.exposure;
-        StageScope sc(s, RT_STAGE_ACCUMULATE);
+        StageScope sc(s, RT_STAGE_ACCUMULATE, st);
         if ((s->error = launchAccumulate(a, st)) != hipSuccess) return false;
     }
     if ((s->error = hipEventRecord(s->ev1, st)) != hipSuccess) return false;
     if ((s->error = hipEventSynchronize(s->ev1)) != hipSuccess) return false;
     if ((s->error = hipEventElapsedTime(ms, s->ev0, s->ev1)) != hipSuccess) return false;
-    std::vector<int> ctrs((size_t)chunk * kCounterStride);
+    std::vector<int> ctrs((size_t)chunk * kPipelines * kCounterStride);
     int overflow = 0;
     if ((s->error = hipMemcpy(ctrs.data(), s->dCounters, ctrs.size() * sizeof(int), hipMemcpyDeviceToHost)) != hipSuccess) return false;
     if ((s->error = hipMemcpy(&overflow, s->dOverflow, sizeof(int), hipMemcpyDeviceToHost)) != hipSuccess) return false;
     if (overflow) { s->hostError = "traversal stack exceeded its spill ring"; return false; }
     unsigned long long closest = 0, shadow = 0;
     for (int c = 0; c < chunk; c++)
-        for (int b = 0; b < rs.maxBounces; b++) {
-            closest += (unsigned)ctrs[(size_t)c * kCounterStride + kCntRays + b];
-            shadow += (unsigned)ctrs[(size_t)c * kCounterStride + kCntShadow + b];
-        }
+        for (int p = 0; p < chunkPipelines[c]; p++)
+            for (int b = 0; b < rs.maxBounces; b++) {
+                const size_t base = ((size_t)c * kPipelines + p) * kCounterStride;
+                closest += (unsigned)ctrs[base + kCntRays + b];
+                shadow += (unsigned)ctrs[base + kCntShadow + b];
+            }
     s->lastClosest = closest; s->lastShadow = shadow;
     for (int k = 0; k < RT_STAGE_COUNT; k++) { s->stageMs[k] = 0; s->stageLaunches[k] = 0; }
     for (const auto& tl : s->timedLaunches) {
@@ -484,8 +531,14 @@ int RtInitialize(void* p, const Triangle* triangles, int count, int w, int h) {
         }
     }
     if ((s->error = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking)) != hipSuccess) return 0;
+    for (int k = 0; k < kPipelines - 1; k++) {
+        if ((s->error = hipStreamCreateWithFlags(&s->pipeStream[k], hipStreamNonBlocking)) != hipSuccess) return 0;
+        if ((s->error = hipEventCreateWithFlags(&s->evJoin[k], hipEventDisableTiming)) != hipSuccess) return 0;
+    }
     if ((s->error = hipEventCreate(&s->ev0)) != hipSuccess) return 0;
     if ((s->error = hipEventCreate(&s->ev1)) != hipSuccess) return 0;
+    if ((s->error = hipEventCreateWithFlags(&s->evFork, hipEventDisableTiming)) != hipSuccess) return 0;
+
     if (!upload(s, s->dTris, s->tris.data(), s->tris.size())) return 0;
     if (!buildAccel(s)) return 0;
     if (!upload(s, s->dLights, s->lights.data(), s->lights.size())) return 0;
@@ -496,7 +549,8 @@ int RtInitialize(void* p, const Triangle* triangles, int count, int w, int h) {
         !devAlloc(s, s->dPixels, npix) || !devAlloc(s, s->dThr, slots) || !devAlloc(s, s->dRad, slots) ||
         !devAlloc(s, s->dAlb, slots) || !devAlloc(s, s->dNrm, slots) || !devAlloc(s, s->dNee, slots) ||
         !devAlloc(s, s->dRayA, 2 * slots) || !devAlloc(s, s->dRayB, 2 * slots) || !devAlloc(s, s->dShadow, 2 * slots) ||
-        !devAlloc(s, s->dHits, slots) || !devAlloc(s, s->dCounters, (size_t)kMaxRaysPerPixel * kCounterStride) ||
+        !devAlloc(s, s->dHits, slots) ||
+        !devAlloc(s, s->dCounters, (size_t)kMaxRaysPerPixel * kPipelines * kCounterStride) ||
         !devAlloc(s, s->dOverflow, 1) || !devAlloc(s, s->dStats, 4))
         return 0;
     s->cus = cus;
@@ -941,6 +995,10 @@ int RtSetOption(void* p, const char* key, int value) {
         if (value < 1 || value > 64) return 0;
         s->refillBelow = value; return 1;
     }
+    if (!std::strcmp(key, "pipelines")) {
+        if (value < 1 || value > kPipelines) return 0;
+        s->pipelines = value; return 1;
+    }
     if (!std::strcmp(key, "while_while")) {
         if (value != 0 && value != 1) return 0;
         s->whileWhile = value; return 1;
@@ -978,6 +1036,7 @@ int RtGetOption(void* p, const char* key) {
     if (!std::strcmp(key, "denoise_fallback")) return s->denoiseFallback;
     if (!std::strcmp(key, "refill")) return s->refill;
     if (!std::strcmp(key, "refill_below")) return s->refillBelow;
+    if (!std::strcmp(key, "pipelines")) return s->pipelines;
     if (!std::strcmp(key, "while_while")) return s->whileWhile;
     if (!std::strcmp(key, "tiled_primary")) return s->tiledPrimary;
     if (!std::strcmp(key, "nontemporal")) return s->nontemporal;

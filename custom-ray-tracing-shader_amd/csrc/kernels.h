@@ -48,7 +48,8 @@ struct FrameArgs {
     float4 *thr, *rad, *alb, *nrm, *nee;
     float4 *rayIn, *rayOut, *shadow;
     int4* hits;
-    int* counters;           // this chunk's counters
+    int* counters;           // this pipeline's counters
+    int qBase, qCount;       // raygen: queue positions [qBase, qBase + qCount) of the chunk
     int bounce;
     int tiled;               // 1: primary rays queued in 8x8 pixel tiles
 };

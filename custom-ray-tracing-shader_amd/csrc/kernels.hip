@@ -517,8 +517,9 @@ __device__ __forceinline__ int tiledPixel(int q, int w, int h) {
 __global__ __launch_bounds__(kBlock) void raygenKernel(FrameArgs f) {
     __shared__ unsigned sShift[4], sTerm[kMaxRaysPerPixel * 4];
     loadSobolTerms<4>(sShift, sTerm, f.rs.seed, 0, f.sample0, f.R);
-    const int q = blockIdx.x * kBlock + threadIdx.x;   // queue position
-    if (q >= f.R * f.npix) return;
+    const int l = blockIdx.x * kBlock + threadIdx.x;   // position in this pipeline's queue
+    if (l >= f.qCount) return;
+    const int q = f.qBase + l;                          // position in the chunk's order
     const int r = q / f.npix;
     const unsigned pixel = (unsigned)(f.tiled ? tiledPixel(q - r * f.npix, f.w, f.h) : q - r * f.npix);
     const int slot = r * f.npix + (int)pixel;
@@ -530,11 +531,12 @@ __global__ __launch_bounds__(kBlock) void raygenKernel(FrameArgs f) {
     f.rad[slot] = make_float4(0, 0, 0, 0);
     f.alb[slot] = make_float4(0, 0, 0, 0);
     f.nrm[slot] = make_float4(0, 0, 0, 0);
-    f.rayOut[2 * q] = make_float4(o.x, o.y, o.z, 1e30f);
-    f.rayOut[2 * q + 1] = make_float4(d.x, d.y, d.z, __int_as_float(slot));
+    f.rayOut[2 * l] = make_float4(o.x, o.y, o.z, 1e30f);
+    f.rayOut[2 * l + 1] = make_float4(d.x, d.y, d.z, __int_as_float(slot));
 }
 hipError_t launchRaygen(const FrameArgs& f, hipStream_t s) {
-    const int n = f.R * f.npix;
+    const int n = f.qCount;
+    if (n < 1 || f.qBase < 0 || f.qBase + n > f.R * f.npix) return hipErrorInvalidValue;
     hipLaunchKernelGGL(raygenKernel, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, f);
     return hipGetLastError();
 }

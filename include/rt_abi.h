@@ -125,7 +125,9 @@ int RtRayCounts(void* session, unsigned long long* closest, unsigned long long* 
  * 8 or 16), "while_while" (0/1: speculative while-while traversal loop, default 1),
  * "refill" (0/1: lanes take new rays while the wave traces others, default 1),
  * "refill_below" (1..64: with refill, a wave fetches new rays once fewer lanes than this
- * are busy, default 40),
+ * are busy, default 40), "pipelines" (1..4: a frame's paths split over that many streams
+ * whose launches overlap, at least 65536 paths each, default 2; frames with stage timing or
+ * trace statistics use one),
  * "tiled_primary", "nontemporal", "stage_timing", "trace_stats" and "denoise_fallback" at the
  * next render or query.  Returns 1 if the key is known and the value valid.  RtGetOption
  * returns -1 for unknown keys; it also reports "node_bytes", the size of one wide node. */

@@ -328,3 +328,26 @@ def test_frames_identical_across_traversal_loops(lib):
     for acc, counts in out[1:]:
         assert np.array_equal(acc.view(np.uint32), out[0][0].view(np.uint32))
         assert counts == out[0][1]
+
+
+@pytest.mark.parametrize("rpp", [1, 3])
+def test_path_pipelines_match_one(lib, rpp):
+    """Frames split over concurrent path pipelines (own streams, queues and counters; two at
+    1 spp here, up to four at 3 spp) equal single-pipeline frames bit for bit, with the same
+    ray counts."""
+    from scenes import displaced_icosphere, icosphere_camera, make_settings
+    tris = displaced_icosphere(subdiv=5, seed=2)
+    w, h = 512, 256   # 131072 paths per sample: the size from which a frame splits
+    cam, st = icosphere_camera(w, h), make_settings(8, rpp=rpp, seed=4)
+    out = {}
+    for pipes in (1, 2, 4):
+        with Session(lib) as s:
+            assert s.set_option("pipelines", pipes)
+            s.initialize(tris, w, h)
+            for _ in range(2):
+                img, _ = s.render(cam, st, 2)
+            out[pipes] = (img, s.accumulation(), s.ray_counts())
+    for pipes in (2, 4):
+        assert np.array_equal(out[1][0], out[pipes][0])
+        assert np.array_equal(out[1][1].view(np.uint32), out[pipes][1].view(np.uint32))
+        assert out[1][2] == out[pipes][2]
