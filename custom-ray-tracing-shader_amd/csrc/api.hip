@@ -1,6 +1,8 @@
 // api.hip - the plugin's C ABI: session state, validation, the GPU frame driver and the
 // CPU backends.  Entry points keep the reference plugin's names, argument meaning and
 // 1 = success / 0 = failure contract (Native/renderer.cu:309-458).
+#include <algorithm>
+#include <array>
 #include <chrono>
 #include <cstdlib>
 #include <cstring>
@@ -87,6 +89,7 @@ struct Session {
     std::vector<std::pair<int, int>> timedLaunches;   // (stage, first event index)
     float stageMs[RT_STAGE_COUNT] = {};
     int stageLaunches[RT_STAGE_COUNT] = {};
+    std::vector<std::array<float, 3>> launchSpans;   // last frame: (stage, start ms, end ms) from frame start
     unsigned long long traceCounts[4] = {};
     bool hasFrame = false;
     // denoiser
@@ -251,20 +254,22 @@ bool renderGpu(Session* s, const CameraData& cam, const RenderSettings& rs, floa
     if (!prepareTraceGrid(s)) return false;
     s->timedLaunches.clear();
     if (s->stageTiming && s->eventPool.empty()) {
+        // Timing-only events: no system-scope fence (cache writeback and invalidation) at
+        // each record, which would otherwise slow the frame being measured.  They are read
+        // after the frame's closing event has completed.
         s->eventPool.resize(4096);
         for (auto& e : s->eventPool)
-            if ((s->error = hipEventCreate(&e)) != hipSuccess) return false;
+            if ((s->error = hipEventCreateWithFlags(&e, hipEventDisableSystemFence)) != hipSuccess) return false;
     }
     if (s->traceStats && (s->error = hipMemsetAsync(s->dStats, 0, 4 * sizeof(unsigned long long), st)) != hipSuccess)
         return false;
     if ((s->error = hipEventRecord(s->ev0, st)) != hipSuccess) return false;
     // Path pipelines: a chunk's queue positions are split in P parts, each traced and shaded
     // on its own stream with its own counters, queues and spill ring, so one part's kernels
-    // fill the GPU while another's launch drains its last rays.  Measurement frames (stage
-    // timing, trace statistics) run one pipeline, so launch times are not stretched by
-    // overlap.
+    // fill the GPU while another's launch drains its last rays.  Frames with trace
+    // statistics (instrumented kernels) run one pipeline.
     const bool nee = !s->lights.empty();
-    const bool measure = s->stageTiming || s->traceStats;
+    const bool measure = s->traceStats;
     std::vector<int> chunkPipelines;
     int chunk = 0;
     for (int r0 = 0; r0 < rpp; r0 += chunkR, chunk++) {
@@ -363,12 +368,15 @@ bool renderGpu(Session* s, const CameraData& cam, const RenderSettings& rs, floa
             }
     s->lastClosest = closest; s->lastShadow = shadow;
     for (int k = 0; k < RT_STAGE_COUNT; k++) { s->stageMs[k] = 0; s->stageLaunches[k] = 0; }
+    s->launchSpans.clear();
     for (const auto& tl : s->timedLaunches) {
-        float ms2 = 0;
-        if ((s->error = hipEventElapsedTime(&ms2, s->eventPool[tl.second], s->eventPool[tl.second + 1])) != hipSuccess)
+        float t0 = 0, t1 = 0;
+        if ((s->error = hipEventElapsedTime(&t0, s->ev0, s->eventPool[tl.second])) != hipSuccess ||
+            (s->error = hipEventElapsedTime(&t1, s->ev0, s->eventPool[tl.second + 1])) != hipSuccess)
             return false;
-        s->stageMs[tl.first] += ms2;
+        s->stageMs[tl.first] += t1 - t0;
         s->stageLaunches[tl.first]++;
+        s->launchSpans.push_back({(float)tl.first, t0, t1});
     }
     if (s->traceStats &&
         (s->error = hipMemcpy(s->traceCounts, s->dStats, sizeof(s->traceCounts), hipMemcpyDeviceToHost)) != hipSuccess)
@@ -651,6 +659,22 @@ int RtStageTimes(void* p, float* milliseconds, int* launches, int count) {
         if (milliseconds) milliseconds[k] = s->stageMs[k];
         if (launches) launches[k] = s->stageLaunches[k];
     }
+    return 1;
+}
+
+int RtStageBusy(void* p, unsigned stageMask, float* milliseconds) {
+    Session* s = static_cast<Session*>(p);
+    if (!s || !milliseconds) return 0;
+    std::vector<std::pair<float, float>> spans;
+    for (const auto& l : s->launchSpans)
+        if (stageMask & (1u << (int)l[0])) spans.emplace_back(l[1], l[2]);
+    std::sort(spans.begin(), spans.end());
+    float busy = 0, end = -1e30f;
+    for (const auto& sp : spans) {
+        if (sp.first > end) { busy += sp.second - sp.first; end = sp.second; }
+        else if (sp.second > end) { busy += sp.second - end; end = sp.second; }
+    }
+    *milliseconds = busy;
     return 1;
 }
 

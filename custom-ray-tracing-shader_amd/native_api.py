@@ -105,6 +105,7 @@ def load_library(path=None):
     lib.RtRenderResident.argtypes = [vp, c.POINTER(Camera), c.POINTER(Settings), c.POINTER(c.c_float)]
     lib.RtCopyOutput.argtypes = [vp, c.POINTER(c.c_ubyte), c.c_int]
     lib.RtStageTimes.argtypes = [vp, c.POINTER(c.c_float), c.POINTER(c.c_int), c.c_int]
+    lib.RtStageBusy.argtypes = [vp, c.c_uint, c.POINTER(c.c_float)]
     lib.RtTraceStats.argtypes = [vp, c.POINTER(c.c_ulonglong)]
     lib.RtBandwidthProbe.argtypes = [c.c_longlong, c.c_int, c.POINTER(c.c_float)]
     fp = c.POINTER(c.c_float)
@@ -366,6 +367,12 @@ class Session:
         n = (c.c_int * len(STAGES))()
         self.check(self.lib.RtStageTimes(self.handle, ms, n, len(STAGES)))
         return {k: (ms[i], n[i]) for i, k in enumerate(STAGES)}
+
+    def stage_busy(self, *stages):
+        """Milliseconds of the last frame during which a launch of any of the named stages ran."""
+        ms = c.c_float()
+        self.check(self.lib.RtStageBusy(self.handle, sum(1 << STAGES.index(k) for k in stages), c.byref(ms)))
+        return ms.value
 
     def trace_stats(self):
         v = (c.c_ulonglong * 4)()

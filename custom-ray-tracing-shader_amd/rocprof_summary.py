@@ -140,6 +140,11 @@ def main():
                   f"triangles per ray), i.e. {alg / max(traffic, 1):.1f}x the HBM traffic: the rest is served "
                   f"by L2 / MALL."]
         dominant["algorithmic_bytes_per_launch"] = alg
+        if "busy_ms_per_step" in rf:
+            lines += [f"Launches of concurrent path pipelines overlap: per step the traversal ran for "
+                      f"{rf['busy_ms_per_step']:.3f} ms of union time over {rf['launches_per_step']:.0f} launches "
+                      f"({rf['effective_launch_ms'] * 1e3:.1f} us per launch), which bench.py prices the "
+                      f"algorithmic bytes on: {rf['achieved']:.0f} GB/s, {rf['frac']:.3f} of 8 TB/s."]
     lines += ["", "Summary made with:", "```", "python3 custom-ray-tracing-shader_amd/rocprof_summary.py " +
               " ".join(f'"{a}"' if (" " in a or "|" in a) else a for a in sys.argv[1:]), "```"]
     with open(args.md, "w") as fh:

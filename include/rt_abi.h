@@ -150,6 +150,11 @@ int RtCopyOutput(void* session, unsigned char* output, int outputBytes);
 enum { RT_STAGE_RESET = 0, RT_STAGE_RAYGEN, RT_STAGE_CLOSEST, RT_STAGE_SHADE, RT_STAGE_SHADOW, RT_STAGE_ACCUMULATE,
        RT_STAGE_COUNT };
 int RtStageTimes(void* session, float* milliseconds, int* launches, int count);
+/* With stage timing on: the milliseconds of the last frame during which at least one launch
+ * of the stages in stageMask (bit k = stage k) was running, the union of their launch
+ * intervals.  Launches of concurrent path pipelines overlap, so this is the figure to price
+ * a stage's throughput on; RtStageTimes sums launch durations. */
+int RtStageBusy(void* session, unsigned stageMask, float* milliseconds);
 int RtTraceStats(void* session, unsigned long long* counts4);
 /* Plain device-to-device copy of `bytes` (float4 grid-stride kernel), `iterations` times;
  * milliseconds = mean kernel time.  Calibrates HBM byte counters against a known size. */

@@ -303,9 +303,14 @@ def test_frame_launches_pair_shadow_rays_with_the_next_bounce(lib):
             s.set_option("stage_timing", 1)
             s.render_resident(cornell_camera(64, 48), st)
             stages = s.stage_times()
+            busy = s.stage_busy("closest", "shadow")
         assert stages["closest"][1] == st.maxBounces
         assert stages["shade"][1] == st.maxBounces
         assert stages["shadow"][1] == shadow_launches
+        # One pipeline (a small frame): the launches do not overlap, so their busy time is
+        # the sum of their durations.
+        total = stages["closest"][0] + stages["shadow"][0]
+        assert 0 < busy and abs(busy - total) <= 1e-3 + 1e-3 * total
 
 
 def test_frames_identical_across_traversal_loops(lib):
