@@ -201,7 +201,7 @@ def psnr_study(args):
     # Denoised frames at more samples: the smallest count whose denoised PSNR reaches plain
     # accumulation at 256 spp, and how much sooner it gets there.
     dn_ladder = []
-    for spp in (8, 12, 16, 24, 32, 64):
+    for spp in (8, 12, 16, 20, 24, 32, 64):
         best = None
         for _ in range(2):
             t = time.perf_counter()

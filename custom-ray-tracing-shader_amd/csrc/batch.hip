@@ -54,7 +54,7 @@ struct Batch {
     std::vector<PendingView> pending;
     std::string error;
     int width = 4, quantized = 0, blocksPerCU = 8, shortStack = 8, refill = 1, refillBelow = 40, whileWhile = 1,
-        useGraph = 1;
+        useGraph = 1, pipelines = 2;
     bool built = false;
     int cus = 0, views = 0, totalSlots = 0, maxBounces = 0;
     long long totalTris = 0;
@@ -73,15 +73,19 @@ struct Batch {
     float4 *dThr = nullptr, *dRad = nullptr, *dAlb = nullptr, *dNrm = nullptr, *dNee = nullptr;
     float4 *dRayA = nullptr, *dRayB = nullptr, *dShadow = nullptr;
     int4* dHits = nullptr;
-    int *dCounters = nullptr, *dSpill = nullptr, *dOverflow = nullptr;
+    int *dCounters = nullptr, *dSpill = nullptr, *dOverflow = nullptr;   // counters: kBatchPipelines x kCounterStride
     int spillEntries = 0, traceBlocks = 0;
     hipStream_t stream = nullptr;
+    hipStream_t pipeStream[3] = {};   // path pipelines 1..3 (default priority, as stream)
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t evFork = nullptr, evJoin[3] = {};
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     float buildMs = 0;
     unsigned long long lastClosest = 0, lastShadow = 0;
 };
+
+constexpr int kBatchPipelines = 4;   // path pipelines per frame at most
 
 template <class T> bool dalloc(Batch* b, T*& p, size_t count) {
     hipError_t e = hipMalloc(reinterpret_cast<void**>(&p), std::max<size_t>(count * sizeof(T), 16));
@@ -100,8 +104,14 @@ void release(Batch* b) {
     dfree(b->dShadow); dfree(b->dHits); dfree(b->dCounters); dfree(b->dSpill); dfree(b->dOverflow);
     if (b->ev0) (void)hipEventDestroy(b->ev0);
     if (b->ev1) (void)hipEventDestroy(b->ev1);
+    if (b->evFork) (void)hipEventDestroy(b->evFork);
     if (b->stream) (void)hipStreamDestroy(b->stream);
-    b->ev0 = b->ev1 = nullptr; b->stream = nullptr;
+    for (int k = 0; k < kBatchPipelines - 1; k++) {
+        if (b->evJoin[k]) (void)hipEventDestroy(b->evJoin[k]);
+        if (b->pipeStream[k]) (void)hipStreamDestroy(b->pipeStream[k]);
+        b->evJoin[k] = nullptr; b->pipeStream[k] = nullptr;
+    }
+    b->ev0 = b->ev1 = b->evFork = nullptr; b->stream = nullptr;
     b->built = false;
 }
 
@@ -124,35 +134,62 @@ TraceLaunch launchOf(const Batch* b) {
     return l;
 }
 
+// Path pipelines of a frame: its queue positions split in P parts (>= 64K paths each), each
+// on its own stream with its own counters, queue regions and spill ring, as in a session.
+int framePipelines(const Batch* b) { return std::max(1, std::min(b->pipelines, b->totalSlots >> 16)); }
+
 bool enqueueFrame(Batch* b) {
     hipStream_t st = b->stream;
-    BatchArgs a = batchArgs(b);
-    B_TRY(launchResetCounters(b->dCounters, b->totalSlots, st));
-    a.rayOut = b->dRayA;
-    B_TRY(launchBatchRaygen(a, st));
+    const int P = framePipelines(b);
+    hipStream_t pst[kBatchPipelines] = {st, b->pipeStream[0], b->pipeStream[1], b->pipeStream[2]};
+    if (P > 1) {
+        B_TRY(hipEventRecord(b->evFork, st));
+        for (int p = 1; p < P; p++) B_TRY(hipStreamWaitEvent(pst[p], b->evFork, 0));
+    }
+    const int part = (b->totalSlots / P + 63) / 64 * 64;
+    BatchArgs a[kBatchPipelines];
+    for (int p = 0; p < P; p++) {
+        const int qBase = p * part, qCount = p + 1 < P ? part : b->totalSlots - qBase;
+        a[p] = batchArgs(b);
+        a[p].counters = b->dCounters + (size_t)p * kCounterStride;
+        a[p].shadow = b->dShadow + 2 * (size_t)qBase; a[p].hits = b->dHits + qBase;
+        a[p].qBase = qBase; a[p].qCount = qCount;
+        B_TRY(launchResetCounters(a[p].counters, qCount, pst[p]));
+        a[p].rayOut = b->dRayA + 2 * (size_t)qBase;
+        B_TRY(launchBatchRaygen(a[p], pst[p]));
+    }
     // As a session frame: one launch traces bounce k's closest-hit rays and bounce k-1's
     // shadow rays; the last launch only the last bounce's shadow rays.
     for (int bounce = 0; bounce <= b->maxBounces; bounce++) {
-        float4* in = (bounce & 1) ? b->dRayB : b->dRayA;
-        float4* out = (bounce & 1) ? b->dRayA : b->dRayB;
         const bool rays = bounce < b->maxBounces;
-        TraceArgs ta{};
-        ta.nodes = b->dNodes; ta.recs = b->dRecs; ta.spill = b->dSpill; ta.spillEntries = b->spillEntries;
-        ta.totalLanes = b->traceBlocks * kBlock; ta.overflow = b->dOverflow;
-        ta.roots = b->dRoots; ta.viewBase = b->dViewBase; ta.views = b->views; ta.nontemporal = 1;
-        ta.work = b->dCounters + kWorkRays + bounce;
-        if (rays) { ta.rays = in; ta.countPtr = b->dCounters + kCntRays + bounce; ta.hits = b->dHits; }
-        if (bounce > 0) {
-            ta.anyRays = b->dShadow; ta.anyCountPtr = b->dCounters + kCntShadow + (bounce - 1);
-            ta.rad = b->dRad; ta.contrib = b->dNee;
+        for (int p = 0; p < P; p++) {
+            const size_t off = 2 * (size_t)a[p].qBase;
+            float4* in = ((bounce & 1) ? b->dRayB : b->dRayA) + off;
+            float4* out = ((bounce & 1) ? b->dRayA : b->dRayB) + off;
+            TraceArgs ta{};
+            ta.nodes = b->dNodes; ta.recs = b->dRecs;
+            ta.spill = b->dSpill + (size_t)p * b->spillEntries * b->traceBlocks * kBlock * 2;
+            ta.spillEntries = b->spillEntries;
+            ta.totalLanes = b->traceBlocks * kBlock; ta.overflow = b->dOverflow;
+            ta.roots = b->dRoots; ta.viewBase = b->dViewBase; ta.views = b->views; ta.nontemporal = 1;
+            ta.work = a[p].counters + kWorkRays + bounce;
+            if (rays) { ta.rays = in; ta.countPtr = a[p].counters + kCntRays + bounce; ta.hits = a[p].hits; }
+            if (bounce > 0) {
+                ta.anyRays = a[p].shadow; ta.anyCountPtr = a[p].counters + kCntShadow + (bounce - 1);
+                ta.rad = b->dRad; ta.contrib = b->dNee;
+            }
+            B_TRY(launchTrace(launchOf(b), false, b->traceBlocks, ta, pst[p]));
+            if (!rays) continue;
+            a[p].bounce = bounce; a[p].rayIn = in; a[p].rayOut = out;
+            B_TRY(launchBatchShade(a[p], b->cus * shadeBlocksPerCU(), pst[p]));
         }
-        B_TRY(launchTrace(launchOf(b), false, b->traceBlocks, ta, st));
-        if (!rays) break;
-        a.bounce = bounce; a.rayIn = in; a.rayOut = out;
-        B_TRY(launchBatchShade(a, b->cus * shadeBlocksPerCU(), st));
     }
-    B_TRY(launchBatchAccumulate(a, st));
-    B_TRY(launchBatchAdvance(a, st));
+    for (int p = 1; p < P; p++) {
+        B_TRY(hipEventRecord(b->evJoin[p - 1], pst[p]));
+        B_TRY(hipStreamWaitEvent(st, b->evJoin[p - 1], 0));
+    }
+    B_TRY(launchBatchAccumulate(a[0], st));
+    B_TRY(launchBatchAdvance(a[0], st));
     return true;
 }
 
@@ -165,6 +202,11 @@ bool build(Batch* b) {
     B_TRY(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
     B_TRY(hipEventCreate(&b->ev0));
     B_TRY(hipEventCreate(&b->ev1));
+    B_TRY(hipEventCreateWithFlags(&b->evFork, hipEventDisableTiming));
+    for (int k = 0; k < kBatchPipelines - 1; k++) {
+        B_TRY(hipStreamCreateWithFlags(&b->pipeStream[k], hipStreamNonBlocking));
+        B_TRY(hipEventCreateWithFlags(&b->evJoin[k], hipEventDisableTiming));
+    }
     b->views = (int)b->pending.size();
     // Host preprocessing per scene, as RtInitialize (Native/renderer.cu:345-353).
     std::vector<Triangle> tris;
@@ -253,7 +295,7 @@ bool build(Batch* b) {
     if (!dalloc(b, b->dSumRad, n) || !dalloc(b, b->dSumAlb, n) || !dalloc(b, b->dSumNrm, n) || !dalloc(b, b->dPixels, n) ||
         !dalloc(b, b->dThr, n) || !dalloc(b, b->dRad, n) || !dalloc(b, b->dAlb, n) || !dalloc(b, b->dNrm, n) ||
         !dalloc(b, b->dNee, n) || !dalloc(b, b->dRayA, 2 * n) || !dalloc(b, b->dRayB, 2 * n) ||
-        !dalloc(b, b->dShadow, 2 * n) || !dalloc(b, b->dHits, n) || !dalloc(b, b->dCounters, (size_t)kCounterStride) ||
+        !dalloc(b, b->dShadow, 2 * n) || !dalloc(b, b->dHits, n) || !dalloc(b, b->dCounters, (size_t)kBatchPipelines * kCounterStride) ||
         !dalloc(b, b->dOverflow, 1) || !dalloc(b, b->dViews, b->views) || !dalloc(b, b->dViewBase, b->views + 1) ||
         !dalloc(b, b->dRoots, b->views) || !dalloc(b, b->dSamples, b->views))
         return false;
@@ -275,7 +317,7 @@ bool build(Batch* b) {
     B_TRY(hipMemset(b->dOverflow, 0, sizeof(int)));
     b->traceBlocks = b->cus * b->blocksPerCU;
     b->spillEntries = 36 * (b->width - 1);
-    if (!dalloc(b, b->dSpill, (size_t)b->spillEntries * b->traceBlocks * kBlock * 2)) return false;
+    if (!dalloc(b, b->dSpill, (size_t)kBatchPipelines * b->spillEntries * b->traceBlocks * kBlock * 2)) return false;
     if (b->useGraph) {
         B_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
         const bool ok = enqueueFrame(b);
@@ -338,6 +380,7 @@ int RtBatchSetOption(void* p, const char* key, int value) {
     if (!std::strcmp(key, "refill_below") && value >= 1 && value <= 64) { b->refillBelow = value; return 1; }
     if (!std::strcmp(key, "while_while") && (value == 0 || value == 1)) { b->whileWhile = value; return 1; }
     if (!std::strcmp(key, "use_graph") && (value == 0 || value == 1)) { b->useGraph = value; return 1; }
+    if (!std::strcmp(key, "pipelines") && value >= 1 && value <= kBatchPipelines) { b->pipelines = value; return 1; }
     return 0;
 }
 
@@ -361,16 +404,17 @@ int RtBatchRender(void* p, int frames, float* deviceMs) {
     B_TRY(hipEventRecord(b->ev1, b->stream));
     B_TRY(hipEventSynchronize(b->ev1));
     B_TRY(hipEventElapsedTime(deviceMs, b->ev0, b->ev1));
-    int ctr[kCounterStride];
+    int ctr[kBatchPipelines * kCounterStride];
     int overflow = 0;
     B_TRY(hipMemcpy(ctr, b->dCounters, sizeof(ctr), hipMemcpyDeviceToHost));
     B_TRY(hipMemcpy(&overflow, b->dOverflow, sizeof(int), hipMemcpyDeviceToHost));
     if (overflow) { b->error = "traversal stack exceeded its spill ring"; return 0; }
     b->lastClosest = b->lastShadow = 0;
-    for (int k = 0; k < b->maxBounces; k++) {
-        b->lastClosest += (unsigned)ctr[kCntRays + k];
-        b->lastShadow += (unsigned)ctr[kCntShadow + k];
-    }
+    for (int p = 0; p < framePipelines(b); p++)
+        for (int k = 0; k < b->maxBounces; k++) {
+            b->lastClosest += (unsigned)ctr[p * kCounterStride + kCntRays + k];
+            b->lastShadow += (unsigned)ctr[p * kCounterStride + kCntShadow + k];
+        }
     return 1;
 }
 

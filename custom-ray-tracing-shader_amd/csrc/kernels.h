@@ -112,7 +112,8 @@ struct BatchArgs {
     float4 *thr, *rad, *alb, *nrm, *nee;
     float4 *rayIn, *rayOut, *shadow;
     int4* hits;
-    int* counters;
+    int* counters;             // this pipeline's counters
+    int qBase, qCount;         // raygen: queue positions [qBase, qBase + qCount) of the frame
     int bounce;
 };
 hipError_t launchBatchRaygen(const BatchArgs& b, hipStream_t s);

@@ -675,8 +675,9 @@ hipError_t launchAccumulate(const AccumArgs& a, hipStream_t s) {
 // The same stages as the single-view frame, with each path slot's view looked up from its
 // slot; Sobol values are formed per lane (views have their own seeds and sample counts).
 __global__ __launch_bounds__(kBlock) void batchRaygenKernel(BatchArgs b) {
-    const int q = blockIdx.x * kBlock + threadIdx.x;   // queue position
-    if (q >= b.totalSlots) return;
+    const int l = blockIdx.x * kBlock + threadIdx.x;   // position in this pipeline's queue
+    if (l >= b.qCount) return;
+    const int q = b.qBase + l;                          // position in the frame's order
     const int v = findView(b.viewBase, b.viewCount, q);
     const BatchView& V = b.views[v];
     const unsigned pixel = (unsigned)tiledPixel(q - V.slotBase, V.w, V.h);
@@ -690,11 +691,12 @@ __global__ __launch_bounds__(kBlock) void batchRaygenKernel(BatchArgs b) {
     b.rad[slot] = make_float4(0, 0, 0, 0);
     b.alb[slot] = make_float4(0, 0, 0, 0);
     b.nrm[slot] = make_float4(0, 0, 0, 0);
-    b.rayOut[2 * q] = make_float4(o.x, o.y, o.z, 1e30f);
-    b.rayOut[2 * q + 1] = make_float4(d.x, d.y, d.z, __int_as_float(slot));
+    b.rayOut[2 * l] = make_float4(o.x, o.y, o.z, 1e30f);
+    b.rayOut[2 * l + 1] = make_float4(d.x, d.y, d.z, __int_as_float(slot));
 }
 hipError_t launchBatchRaygen(const BatchArgs& b, hipStream_t s) {
-    hipLaunchKernelGGL(batchRaygenKernel, dim3((b.totalSlots + kBlock - 1) / kBlock), dim3(kBlock), 0, s, b);
+    if (b.qCount < 1 || b.qBase < 0 || b.qBase + b.qCount > b.totalSlots) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(batchRaygenKernel, dim3((b.qCount + kBlock - 1) / kBlock), dim3(kBlock), 0, s, b);
     return hipGetLastError();
 }
 

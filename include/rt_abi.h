@@ -207,7 +207,7 @@ int RtConv3x3Backward(const float* in, int n, int h, int w, int cin, const float
  * shared path queue, each frame replayed as a captured hipGraph ("use_graph", default 1).
  * Each view takes one progressive sample per frame (raysPerPixel is ignored) and equals a
  * session rendering its scene with raysPerPixel 1.  Options ("bvh_width", "bvh_quantized",
- * "blocks_per_cu", "short_stack", "while_while", "refill", "refill_below", "use_graph") must be set before
+ * "blocks_per_cu", "short_stack", "while_while", "refill", "refill_below", "pipelines", "use_graph") must be set before
  * RtBatchBuild.  RtBatchReadView
  * copies a view's RGBA8 image and/or its 10-float-per-pixel accumulation. */
 void* RtBatchCreate(void);

@@ -68,3 +68,25 @@ def test_batch_reset_and_counts(lib):
         b.render(1)
         assert np.array_equal(b.read(0)[0], first)
         assert not lib.RtBatchAddView(b.handle, None, 0, 1, 1, None, None)
+
+
+@pytest.mark.parametrize("use_graph", [1, 0])
+def test_batch_pipelines_match_one(lib, use_graph):
+    """A batch frame split over concurrent path pipelines (captured across streams in the
+    graph) equals the single-pipeline frame bit for bit."""
+    vs = views()
+    t = scenes.displaced_icosphere(subdiv=4, seed=8)
+    vs.append((t, 512, 512, scenes.icosphere_camera(512, 512), scenes.make_settings(6, seed=77)))  # 262K+ slots
+    out = {}
+    for pipes in (1, 2, 4):
+        with Batch(lib, use_graph=use_graph, pipelines=pipes) as b:
+            for tv, w, h, cam, st in vs:
+                b.add_view(tv, w, h, cam, st)
+            b.build()
+            b.render(2)
+            out[pipes] = ([b.read(v) for v in range(len(vs))], b.ray_counts())
+    for pipes in (2, 4):
+        assert out[pipes][1] == out[1][1]
+        for v in range(len(vs)):
+            assert np.array_equal(out[pipes][0][v][0], out[1][0][v][0]), (pipes, v)
+            assert np.array_equal(out[pipes][0][v][1].view(np.uint32), out[1][0][v][1].view(np.uint32)), (pipes, v)
