@@ -37,7 +37,7 @@ HBM_PEAK_GBS = 8000.0
 # The soup keeps the reference builder's leaf <= 4 rule (its configuration names it); the
 # icosphere uses the autotuner's leaf size.
 TUNED = {
-    "soup": {"bvh_width": 4, "build_max_leaf": 4, "blocks_per_cu": 4, "short_stack": 16, "while_while": 1, "refill": 1,
+    "soup": {"bvh_width": 4, "build_max_leaf": 2, "blocks_per_cu": 4, "short_stack": 16, "while_while": 1, "refill": 1,
              "pipelines": 2},
     "icosphere": {"bvh_width": 4, "build_max_leaf": 2, "blocks_per_cu": 6, "short_stack": 8, "while_while": 0,
                   "refill": 0, "pipelines": 4},
@@ -302,7 +302,7 @@ def main():
         import autotune
         t0 = time.perf_counter()
         best, trials = autotune.autotune(tris, cam, st, w, h, use_cache=False,
-                                         max_leaf=(4,) if args.workload == "soup" else (2, 3, 4))
+                                         max_leaf=(2, 4) if args.workload == "soup" else (2, 3, 4))
         cfg = {k: best[k] for k in cfg}
         log(f"autotune: {len(trials)} configurations in {time.perf_counter() - t0:.1f} s, best {best}")
     s = native_api.Session()
