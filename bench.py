@@ -40,7 +40,7 @@ TUNED = {
     "soup": {"bvh_width": 4, "build_max_leaf": 2, "blocks_per_cu": 4, "short_stack": 16, "while_while": 1, "refill": 1,
              "pipelines": 2},
     "icosphere": {"bvh_width": 4, "build_max_leaf": 2, "blocks_per_cu": 6, "short_stack": 8, "while_while": 0,
-                  "refill": 0, "pipelines": 4},
+                  "refill": 0, "pipelines": 4, "tail_bounce": 4},
     "cornell": {"bvh_width": 4, "build_max_leaf": 4, "blocks_per_cu": 6, "short_stack": 8, "while_while": 1, "refill": 0,
                 "pipelines": 4},
     "batch": {"bvh_width": 4, "blocks_per_cu": 6, "short_stack": 8, "while_while": 1, "refill": 1},
@@ -258,6 +258,8 @@ def main():
     ap.add_argument("--short-stack", type=int)
     ap.add_argument("--refill", type=int)
     ap.add_argument("--refill-below", type=int)
+    ap.add_argument("--tail-bounce", type=int,
+                    help="bounce from which paths complete in one launch (0: off)")
     ap.add_argument("--pipelines", type=int, choices=(1, 2, 3, 4),
                     help="path pipelines per frame (the library's default is 2)")
     ap.add_argument("--build-max-leaf", type=int)
@@ -295,7 +297,7 @@ def main():
     log(f"workload: {desc}")
     cfg = dict(TUNED[args.workload])
     for k in ("bvh_width", "bvh_quantized", "build_max_leaf", "blocks_per_cu", "short_stack", "refill", "refill_below",
-              "while_while", "pipelines"):
+              "while_while", "pipelines", "tail_bounce"):
         if getattr(args, k) is not None:
             cfg[k] = getattr(args, k)
     if args.autotune:
@@ -303,11 +305,11 @@ def main():
         t0 = time.perf_counter()
         best, trials = autotune.autotune(tris, cam, st, w, h, use_cache=False,
                                          max_leaf=(2, 4) if args.workload == "soup" else (2, 3, 4))
-        cfg = {k: best[k] for k in cfg}
+        cfg = {k: v for k, v in best.items() if k != "mrays_per_s"}
         log(f"autotune: {len(trials)} configurations in {time.perf_counter() - t0:.1f} s, best {best}")
     s = native_api.Session()
     for k in ("bvh_width", "bvh_quantized", "build_max_leaf", "blocks_per_cu", "short_stack", "refill", "refill_below",
-              "while_while", "pipelines"):
+              "while_while", "pipelines", "tail_bounce"):
         if k in cfg:
             s.set_option(k, cfg[k])
     t0 = time.perf_counter()
@@ -344,7 +346,7 @@ def main():
             a = stage_ms.setdefault(k, [0.0, 0])
             a[0] += ms
             a[1] += n
-        busy_ms += s.stage_busy("closest", "shadow")
+        busy_ms += s.stage_busy("closest", "shadow", "tail")
     s.set_option("stage_timing", 0)
 
     # Node and triangle counts: one instrumented frame after the timed region.
@@ -357,10 +359,12 @@ def main():
     quantized = s.lib.RtGetOption(s.handle, b"bvh_quantized")
     node_b = s.lib.RtGetOption(s.handle, b"node_bytes")
     pipelines = s.lib.RtGetOption(s.handle, b"pipelines")   # of the timed frames; the instrumented frame runs one
-    # The traversal kernel: each launch traces one bounce's closest-hit rays together with the
-    # previous bounce's shadow rays (the "closest" stage), the last launch only shadow rays.
-    launches = (stage_ms["closest"][1] + stage_ms["shadow"][1]) / args.steps
-    trace_ms = (stage_ms["closest"][0] + stage_ms["shadow"][0]) / args.steps   # summed launch durations
+    # The traversal kernels: each launch traces one bounce's closest-hit rays together with the
+    # previous bounce's shadow rays (the "closest" stage), shadow rays alone ("shadow"), or,
+    # with tail_bounce, every remaining bounce of the paths ("tail", which also shades them).
+    trace_stages = ("closest", "shadow", "tail")
+    launches = sum(stage_ms[k][1] for k in trace_stages) / args.steps
+    trace_ms = sum(stage_ms[k][0] for k in trace_stages) / args.steps   # summed launch durations
     busy_frame_ms = busy_ms / args.steps
     rays_frame = closest_rays + shadow_rays
     nodes_frame = stats["closest_nodes"] + stats["shadow_nodes"]

@@ -1,11 +1,12 @@
 """Autotuner for the traversal launch: BVH width and leaf size, persistent workgroups per
 CU, LDS short-stack depth, the traversal loop (speculative while-while with per-lane ray
-refill, or one step per iteration) and the number of concurrent path pipelines.
+refill, or one step per iteration), the number of concurrent path pipelines and the bounce
+from which paths complete in one launch.
 
 For a scene, camera and settings it initialises one session per (BVH width, leaf size),
 times a few device-resident frames for every (blocks_per_cu, short_stack, loop) combination,
-then tries each pipeline count on the fastest of those, and keeps the configuration with the
-highest ray throughput.  Choices persist in a JSON cache keyed by
+then tries each pipeline count and path-completion bounce on the fastest of those, and keeps
+the configuration with the highest ray throughput.  Choices persist in a JSON cache keyed by
 scene size class and resolution ($RT_AUTOTUNE_CACHE, default ~/.cache/rt_amd/autotune.json),
 so later sessions of a similar scene reuse them without searching again.
 """
@@ -23,6 +24,7 @@ BLOCKS = (4, 6, 8)
 STACKS = (8, 16)
 LOOPS = ((1, 1), (1, 0), (0, 0))   # (while_while, refill)
 PIPELINES = (1, 2, 3, 4)
+TAILS = (0, 3, 4, 5)   # path completion from this bounce (0: off), where below the bounce limit
 
 
 def cache_path():
@@ -51,7 +53,7 @@ def store_cache(cache):
 
 
 def apply(session, cfg):
-    for key in ("blocks_per_cu", "short_stack", "refill", "while_while", "pipelines"):
+    for key in ("blocks_per_cu", "short_stack", "refill", "while_while", "pipelines", "tail_bounce"):
         if key not in cfg:
             continue
         if not session.set_option(key, cfg[key]):
@@ -98,7 +100,7 @@ def autotune(triangles, camera, settings, width, height, frames=3, widths=WIDTHS
                 for st in STACKS:
                     for ww, r in LOOPS:
                         trial(s, {"bvh_width": w, "build_max_leaf": ml, "blocks_per_cu": b, "short_stack": st,
-                                  "while_while": ww, "refill": r, "pipelines": 2})
+                                  "while_while": ww, "refill": r, "pipelines": 2, "tail_bounce": 0})
     with Session(lib) as s:
         s.set_option("bvh_width", best["bvh_width"])
         s.set_option("build_max_leaf", best["build_max_leaf"])
@@ -107,6 +109,10 @@ def autotune(triangles, camera, settings, width, height, frames=3, widths=WIDTHS
         for p in PIPELINES:
             if p != launch["pipelines"]:
                 trial(s, dict(launch, pipelines=p))
+        launch = {k: v for k, v in best.items() if k != "mrays_per_s"}
+        for t in TAILS:
+            if t != launch.get("tail_bounce", 0) and t < settings.maxBounces:
+                trial(s, dict(launch, tail_bounce=t))
     if use_cache:
         cache[key] = best
         store_cache(cache)

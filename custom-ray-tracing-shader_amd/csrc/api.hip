@@ -49,6 +49,7 @@ struct Session {
     int whileWhile = 1;          // speculative while-while traversal loop
     int tiledPrimary = 1;        // primary rays queued in 8x8 pixel tiles
     int pipelines = 2;           // independent path pipelines per chunk, each on its own stream
+    int tailBounce = 0;          // >= 1: paths complete from this bounce on in one launch (0: off)
     int nontemporal = 1;         // rays / hits bypass cache retention in traversal
     std::vector<unsigned char> hostNodes;
     std::vector<TriRecord> hostRecs;
@@ -244,6 +245,9 @@ TraceArgs baseTraceArgs(Session* s, int p = 0) {
     return a;
 }
 
+// Shadow rays of bounce b-1 wait to be traced in launch b.
+static bool shadowAt(bool nee, int b) { return nee && b > 0; }
+
 // One GPU RtRender: chunks of R sample groups; see kernels.hip for the stage order.
 bool renderGpu(Session* s, const CameraData& cam, const RenderSettings& rs, float* ms) {
     const int npix = s->width * s->height;
@@ -309,8 +313,36 @@ bool renderGpu(Session* s, const CameraData& cam, const RenderSettings& rs, floa
         // Bounce b's closest-hit rays and bounce b-1's shadow rays are independent, so one
         // launch traces both queues; the shadow rays' radiance is still added before shade
         // b adds bounce b's emission, the reference's order (Native/renderer.cu:232-267).
+        // With tail_bounce = k, bounce k-1's shadow rays get a launch of their own and the
+        // path-completion kernel takes every path from bounce k to its end.
+        const int tail = (s->tailBounce >= 1 && s->tailBounce < rs.maxBounces) ? s->tailBounce : -1;
         for (int b = 0; b <= rs.maxBounces; b++) {
-            const bool rays = b < rs.maxBounces, shadow = nee && b > 0;
+            if (b == tail) {
+                for (int p = 0; p < P; p++) {
+                    const size_t off = 2 * (size_t)f[p].qBase;
+                    if (shadowAt(nee, b)) {
+                        TraceArgs sa = baseTraceArgs(s, p);
+                        sa.work = ctr[p] + kWorkRays + rs.maxBounces;   // unused by the per-bounce launches
+                        sa.stats = s->dStats;
+                        sa.anyRays = f[p].shadow; sa.anyCountPtr = ctr[p] + kCntShadow + (b - 1);
+                        sa.rad = s->dRad; sa.contrib = s->dNee;
+                        StageScope sc(s, RT_STAGE_SHADOW, pst[p]);
+                        if ((s->error = launchTrace(traceLaunch(s), s->traceStats, s->traceBlocks, sa, pst[p])) != hipSuccess)
+                            return false;
+                    }
+                    TraceArgs ta = baseTraceArgs(s, p);
+                    ta.rays = ((b & 1) ? s->dRayB : s->dRayA) + off;
+                    ta.countPtr = ctr[p] + kCntRays + b;
+                    ta.work = ctr[p] + kWorkRays + b;
+                    ta.stats = s->dStats;
+                    StageScope sc(s, RT_STAGE_TAIL, pst[p]);
+                    if ((s->error = launchPathTail(traceLaunch(s), s->traceStats, s->traceBlocks, s->cus, f[p], ta, b, pst[p])) !=
+                        hipSuccess)
+                        return false;
+                }
+                break;
+            }
+            const bool rays = b < rs.maxBounces, shadow = shadowAt(nee, b);
             if (!rays && !shadow) break;
             for (int p = 0; p < P; p++) {
                 const size_t off = 2 * (size_t)f[p].qBase;
@@ -1023,6 +1055,10 @@ int RtSetOption(void* p, const char* key, int value) {
         if (value < 1 || value > kPipelines) return 0;
         s->pipelines = value; return 1;
     }
+    if (!std::strcmp(key, "tail_bounce")) {
+        if (value < 0 || value > kMaxBounces) return 0;
+        s->tailBounce = value; return 1;
+    }
     if (!std::strcmp(key, "while_while")) {
         if (value != 0 && value != 1) return 0;
         s->whileWhile = value; return 1;
@@ -1061,6 +1097,7 @@ int RtGetOption(void* p, const char* key) {
     if (!std::strcmp(key, "refill")) return s->refill;
     if (!std::strcmp(key, "refill_below")) return s->refillBelow;
     if (!std::strcmp(key, "pipelines")) return s->pipelines;
+    if (!std::strcmp(key, "tail_bounce")) return s->tailBounce;
     if (!std::strcmp(key, "while_while")) return s->whileWhile;
     if (!std::strcmp(key, "tiled_primary")) return s->tiledPrimary;
     if (!std::strcmp(key, "nontemporal")) return s->nontemporal;

@@ -91,6 +91,10 @@ hipError_t launchTrace(const TraceLaunch& l, bool stats, int blocks, const Trace
 hipError_t launchResetCounters(int* counters, int initialRays, hipStream_t s);
 hipError_t launchRaygen(const FrameArgs& f, hipStream_t s);
 hipError_t launchShade(const FrameArgs& f, int blocks, hipStream_t s);
+// Completes every path of the closest-hit queue a.rays (bounce firstBounce, >= 1) through its
+// remaining bounces in one launch; a carries the traversal resources (no any-hit queue).
+hipError_t launchPathTail(const TraceLaunch& l, bool stats, int blocks, int cus, const FrameArgs& f, const TraceArgs& a,
+                          int firstBounce, hipStream_t s);
 int shadeBlocksPerCU();   // resident shade workgroups per compute unit
 hipError_t launchAccumulate(const AccumArgs& a, hipStream_t s);
 hipError_t launchSobolBatch(const unsigned* pixels, const unsigned* samples, int count, unsigned seed, int dims,

@@ -642,6 +642,174 @@ hipError_t launchShade(const FrameArgs& f, int blocks, hipStream_t s) {
     return hipGetLastError();
 }
 
+// ------------------------------------------------------------------ path completion
+// Late bounces hold few paths, and a launch per bounce ends only when its slowest ray does,
+// so a frame's tail is a chain of latency-bound launches.  From bounce firstBounce on, this
+// kernel carries each path through all its remaining bounces inside one launch: a lane
+// traces the closest hit, shades it (shadeBounce), traces the shadow ray and adds its term,
+// and continues with the next ray.  The lanes of a wave move through these phases together.
+// Each path sees the same operations in the same order as in the per-bounce kernels (its
+// emission, then its next-event term, bounce by bounce), so results are bit-identical.
+__device__ __forceinline__ void beginRay(LaneRay& L, f3 o, f3 d, float limit, bool any) {
+    L.o = o; L.d = d;
+    L.inv = safeInverse(d);
+    L.nz = L.inv.x != 0 && L.inv.y != 0 && L.inv.z != 0;
+    L.oi = F3(o.x * L.inv.x, o.y * L.inv.y, o.z * L.inv.z);
+    L.best = limit; L.bu = 0; L.bv = 0;
+    L.bestIdx = -1; L.sp = 0; L.ref = 0; L.leaf = 0;
+    L.any = any;
+}
+
+template <class N, bool STATS, int S>
+__global__ __launch_bounds__(kBlock) void pathTailKernel(FrameArgs f, TraceArgs a, int firstBounce) {
+    __shared__ int sRef[S * kBlock];
+    __shared__ float sDist[S * kBlock];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int total = *a.countPtr;
+    const N* nodes = static_cast<const N*>(a.nodes);
+    const float4* recs = reinterpret_cast<const float4*>(a.recs);
+    LdsStack<S> st{sRef, sDist, reinterpret_cast<int2*>(a.spill), a.spillEntries, a.totalLanes,
+                   (int)(blockIdx.x * kBlock + tid), tid};
+    TraceCounters tc;
+    bool overflow = false;
+    LaneRay L;
+    PathState ps;
+    unsigned moreClosest = 0, shadowRays = 0;   // queries beyond the input queue, for the frame counters
+    int bounce = 0, slot = 0, r = 0;
+    unsigned pixel = 0;
+    float radW = 0;
+    bool active = false;
+    const int firstRays = gridDim.x * kBlock;
+    auto startPath = [&](int i) {
+        const float4 r0 = a.rays[2 * i], r1 = a.rays[2 * i + 1];
+        slot = __float_as_int(r1.w);
+        r = slot / f.npix;
+        pixel = (unsigned)(slot - r * f.npix);
+        const float4 t4 = f.thr[slot], q4 = f.rad[slot];
+        ps.thr = F3(t4.x, t4.y, t4.z); ps.prevPdf = t4.w; ps.rad = F3(q4.x, q4.y, q4.z);
+        radW = q4.w;
+        bounce = firstBounce;
+        beginRay(L, F3(r0.x, r0.y, r0.z), F3(r1.x, r1.y, r1.z), r0.w, false);
+        active = true;
+    };
+    {
+        const int i = blockIdx.x * kBlock + tid;
+        if (i < total) startPath(i);
+    }
+    bool exhausted = firstRays >= total;
+    for (;;) {
+        const unsigned long long idle = __ballot(!active);
+        if (idle && !exhausted) {
+            const int leader = __ffsll((long long)idle) - 1;
+            int base = 0;
+            if (lane == leader) base = firstRays + atomicAdd(a.work, __popcll(idle));
+            base = __shfl(base, leader, 64);
+            if (!active) {
+                const int rank = (int)__builtin_amdgcn_mbcnt_hi((unsigned)(idle >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)idle, 0u));
+                if (base + rank < total) startPath(base + rank);
+            }
+            exhausted = base + __popcll(idle) >= total;
+        }
+        if (__ballot(active) == 0) break;
+        // closest hit of this bounce
+        if (active) while (!traceStep<N, true, STATS, S>(L, nodes, recs, st, overflow, tc)) {}
+        ShadeResult res;
+        res.cont = false; res.shadow = false;
+        if (active) {
+            RtHit hit;
+            hit.index = L.bestIdx; hit.distance = L.best; hit.u = L.bu; hit.v = L.bv;
+            const unsigned sample = f.sample0 + (unsigned)r, dimBase = 4 + 8 * (unsigned)bounce;
+            auto rng = [&](int k) { return sobolSample(gSobolDirs, pixel, sample, f.rs.seed, dimBase + (unsigned)k); };
+            FirstHit unused;   // first-hit outputs belong to bounce 0
+            shadeBounce(f.sc, f.rs, bounce, rng, L.o, L.d, hit, ps, &unused, res);
+        }
+        // next-event shadow ray
+        if (active && res.shadow) {
+            shadowRays++;
+            beginRay(L, res.nextO, res.sdir, res.slimit, true);
+            while (!traceStep<N, true, STATS, S>(L, nodes, recs, st, overflow, tc)) {}
+            if (L.bestIdx < 0) { ps.rad.x = ps.rad.x + res.contrib.x; ps.rad.y = ps.rad.y + res.contrib.y; ps.rad.z = ps.rad.z + res.contrib.z; }
+        }
+        if (active) {
+            if (res.cont) {
+                moreClosest++;
+                beginRay(L, res.nextO, res.nextD, 1e30f, false);
+                bounce++;
+            } else {
+                f.thr[slot] = make_float4(ps.thr.x, ps.thr.y, ps.thr.z, ps.prevPdf);
+                f.rad[slot] = make_float4(ps.rad.x, ps.rad.y, ps.rad.z, radW);
+                active = false;
+            }
+        }
+    }
+    // Query counts into the frame counters (their per-bounce split is not kept): one atomic
+    // per workgroup and counter.
+    __shared__ unsigned sCount[2];
+    if (tid < 2) sCount[tid] = 0;
+    __syncthreads();
+    for (int o = 32; o > 0; o >>= 1) {
+        moreClosest += __shfl_xor(moreClosest, o, 64);
+        shadowRays += __shfl_xor(shadowRays, o, 64);
+    }
+    if (lane == 0) {
+        atomicAdd(&sCount[0], moreClosest);
+        atomicAdd(&sCount[1], shadowRays);
+    }
+    __syncthreads();
+    if (tid == 0 && sCount[0]) atomicAdd(&f.counters[kCntRays + firstBounce + 1], (int)sCount[0]);
+    if (tid == 1 && sCount[1]) atomicAdd(&f.counters[kCntShadow + firstBounce], (int)sCount[1]);
+    if (STATS) {
+        atomicAdd(&a.stats[0], tc.nodesC);
+        atomicAdd(&a.stats[1], tc.trisC);
+        atomicAdd(&a.stats[2], tc.nodesA);
+        atomicAdd(&a.stats[3], tc.trisA);
+    }
+    if (overflow) atomicOr(a.overflow, 1);
+}
+
+// The grid is capped at the workgroups that fit on the GPU at once: a lane's first path is
+// fixed by its grid position, so a workgroup that had to wait for another to finish would
+// start its paths late.
+template <class N, int S>
+static hipError_t launchTailNS(bool stats, int blocks, int cus, const FrameArgs& f, const TraceArgs& a, int firstBounce,
+                               hipStream_t s) {
+    const void* k = stats ? reinterpret_cast<const void*>(&pathTailKernel<N, true, S>)
+                          : reinterpret_cast<const void*>(&pathTailKernel<N, false, S>);
+    int perCU = 0;
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, k, kBlock, 0);
+    if (e != hipSuccess) return e;
+    blocks = std::max(1, std::min(blocks, perCU * cus));
+    if (stats) hipLaunchKernelGGL((pathTailKernel<N, true, S>), dim3(blocks), dim3(kBlock), 0, s, f, a, firstBounce);
+    else hipLaunchKernelGGL((pathTailKernel<N, false, S>), dim3(blocks), dim3(kBlock), 0, s, f, a, firstBounce);
+    return hipGetLastError();
+}
+template <class N>
+static hipError_t launchTailN(bool stats, int blocks, int cus, const TraceLaunch& l, const FrameArgs& f,
+                              const TraceArgs& a, int firstBounce, hipStream_t s) {
+    if (l.stack == 16) return launchTailNS<N, 16>(stats, blocks, cus, f, a, firstBounce, s);
+    if (l.stack == 8) return launchTailNS<N, 8>(stats, blocks, cus, f, a, firstBounce, s);
+    return hipErrorInvalidValue;
+}
+hipError_t launchPathTail(const TraceLaunch& l, bool stats, int blocks, int cus, const FrameArgs& f, const TraceArgs& a,
+                          int firstBounce, hipStream_t s) {
+    if (!a.rays || !a.countPtr || a.anyRays || firstBounce < 1 || firstBounce >= f.rs.maxBounces || a.roots)
+        return hipErrorInvalidValue;
+    if (l.quantized) {
+        switch (l.width) {
+            case 2: return launchTailN<QNode<2>>(stats, blocks, cus, l, f, a, firstBounce, s);
+            case 4: return launchTailN<QNode<4>>(stats, blocks, cus, l, f, a, firstBounce, s);
+            case 8: return launchTailN<QNode<8>>(stats, blocks, cus, l, f, a, firstBounce, s);
+            default: return hipErrorInvalidValue;
+        }
+    }
+    switch (l.width) {
+        case 2: return launchTailN<WideNode<2>>(stats, blocks, cus, l, f, a, firstBounce, s);
+        case 4: return launchTailN<WideNode<4>>(stats, blocks, cus, l, f, a, firstBounce, s);
+        case 8: return launchTailN<WideNode<8>>(stats, blocks, cus, l, f, a, firstBounce, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
 // Native/renderer.cu:292-301: add the R paths of each pixel in sample order, then tone-map.
 __global__ __launch_bounds__(kBlock) void accumulateKernel(AccumArgs a) {
     const int i = blockIdx.x * kBlock + threadIdx.x;

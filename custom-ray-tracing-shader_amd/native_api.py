@@ -289,7 +289,7 @@ class Denoiser:
             raise ValueError(f"display loss weight {weight} rejected")
 
 
-STAGES = ("reset", "raygen", "closest", "shade", "shadow", "accumulate")
+STAGES = ("reset", "raygen", "closest", "shade", "shadow", "accumulate", "tail")
 
 
 def unpack(structure, values):

@@ -126,8 +126,9 @@ int RtRayCounts(void* session, unsigned long long* closest, unsigned long long* 
  * "refill" (0/1: lanes take new rays while the wave traces others, default 1),
  * "refill_below" (1..64: with refill, a wave fetches new rays once fewer lanes than this
  * are busy, default 40), "pipelines" (1..4: a frame's paths split over that many streams
- * whose launches overlap, at least 65536 paths each, default 2; frames with stage timing or
- * trace statistics use one),
+ * whose launches overlap, at least 65536 paths each, default 2; frames with trace
+ * statistics use one), "tail_bounce" (0 = off, default; k >= 1: from bounce k on, each path
+ * runs to its end inside one launch instead of one launch per bounce; images unchanged),
  * "tiled_primary", "nontemporal", "stage_timing", "trace_stats" and "denoise_fallback" at the
  * next render or query.  Returns 1 if the key is known and the value valid.  RtGetOption
  * returns -1 for unknown keys; it also reports "node_bytes", the size of one wide node. */
@@ -143,12 +144,14 @@ int RtCopyOutput(void* session, unsigned char* output, int outputBytes);
 /* Measurement hooks.  With option "stage_timing" = 1 each GPU kernel launch of RtRender is
  * bracketed by HIP events on the session's stream; RtStageTimes returns, per stage
  * (0 reset, 1 raygen, 2 traversal of a bounce's closest-hit rays together with the previous
- * bounce's shadow rays, 3 shade, 4 traversal of the last bounce's shadow rays, 5 accumulate),
+ * bounce's shadow rays, 3 shade, 4 traversal of shadow rays alone (the last bounce's, or those
+ * before the path-completion launch), 5 accumulate, 6 path completion: every remaining
+ * bounce of the paths from bounce "tail_bounce" on, traversal and shading in one launch),
  * the summed milliseconds and launch counts of the last frame.  With option "trace_stats"
  * = 1 the traversal kernels also count node visits and triangle tests; RtTraceStats returns
  * {closest nodes, closest triangles, shadow nodes, shadow triangles} of the last frame. */
 enum { RT_STAGE_RESET = 0, RT_STAGE_RAYGEN, RT_STAGE_CLOSEST, RT_STAGE_SHADE, RT_STAGE_SHADOW, RT_STAGE_ACCUMULATE,
-       RT_STAGE_COUNT };
+       RT_STAGE_TAIL, RT_STAGE_COUNT };
 int RtStageTimes(void* session, float* milliseconds, int* launches, int count);
 /* With stage timing on: the milliseconds of the last frame during which at least one launch
  * of the stages in stageMask (bit k = stage k) was running, the union of their launch

@@ -356,3 +356,32 @@ def test_path_pipelines_match_one(lib, rpp):
         assert np.array_equal(out[1][0], out[pipes][0])
         assert np.array_equal(out[1][1].view(np.uint32), out[pipes][1].view(np.uint32))
         assert out[1][2] == out[pipes][2]
+
+
+@pytest.mark.parametrize("scene", ["icosphere", "cornell_glossy", "soup"])
+def test_path_completion_matches_per_bounce_launches(lib, scene):
+    """Paths completed in one launch from bounce k on (tail_bounce = k) equal the per-bounce
+    launches bit for bit, with the same ray counts, for every k, one or two pipelines."""
+    from scenes import (cornell_camera, displaced_icosphere, icosphere_camera, make_settings, soup_camera,
+                        triangle_soup)
+    w, h = 512, 256
+    if scene == "icosphere":
+        tris, cam, st = displaced_icosphere(subdiv=5, seed=6), icosphere_camera(w, h), make_settings(8, rpp=1, seed=3)
+    elif scene == "cornell_glossy":
+        tris, cam, st = _glossy_cornell(0.8), cornell_camera(w, h), make_settings(6, rpp=2, seed=5)
+    else:
+        tris, cam, st = triangle_soup(30000, seed=2), soup_camera(w, h), make_settings(3, rpp=1, sky=1.0, seed=8)
+    out = {}
+    for tail in range(0, st.maxBounces + 1):
+        for pipes in ((1, 2) if tail in (0, 2) else (2,)):
+            with Session(lib) as s:
+                assert s.set_option("tail_bounce", tail) and s.set_option("pipelines", pipes)
+                s.initialize(tris, w, h)
+                for _ in range(2):
+                    img, _ = s.render(cam, st, 2)
+                out[tail, pipes] = (img, s.accumulation(), s.ray_counts())
+    ref = out[0, 1]
+    for key, (img, acc, counts) in out.items():
+        assert np.array_equal(img, ref[0]), key
+        assert np.array_equal(acc.view(np.uint32), ref[1].view(np.uint32)), key
+        assert counts == ref[2], key
