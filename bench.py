@@ -424,7 +424,8 @@ def main():
                    "device_ms_per_step": round(device_ms / args.steps, 4), "bvh_build_ms": round(build_ms, 1)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": f"traceKernel<{'QNode' if quantized else 'WideNode'}<{width}>> (closest-hit + shadow queues)",
+                     "kernel": f"traceKernel<{'QNode' if quantized else 'WideNode'}<{width}>> (closest-hit + shadow queues)"
+                               + (" + pathTailKernel" if stage_ms["tail"][1] else ""),
                      "node_bytes": node_b, "mean_launch_ms": round(mean_launch_s * 1e3, 4),
                      "busy_ms_per_step": round(busy_frame_ms, 4),
                      "effective_launch_ms": round(effective_launch_s * 1e3, 4),
