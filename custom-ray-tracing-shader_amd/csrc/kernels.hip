@@ -660,11 +660,11 @@ __device__ __forceinline__ void beginRay(LaneRay& L, f3 o, f3 d, float limit, bo
     L.any = any;
 }
 
-// Capped at 96 VGPRs (five workgroups per CU; a few spilled values) instead of 109 (four):
-// +2% on the icosphere.
-constexpr int kTailBlocksPerCU = 5;
+// With the 8-entry stack, capped at 96 VGPRs (five workgroups per CU; a few spilled values)
+// instead of 109 (four): +2% on the icosphere.  The 16-entry stack's LDS allows four.
+template <int S> constexpr int tailBlocksPerCU() { return S == 8 ? 5 : 4; }
 template <class N, bool STATS, int S>
-__global__ __launch_bounds__(kBlock, kTailBlocksPerCU) void pathTailKernel(FrameArgs f, TraceArgs a, int firstBounce) {
+__global__ __launch_bounds__(kBlock, tailBlocksPerCU<S>()) void pathTailKernel(FrameArgs f, TraceArgs a, int firstBounce) {
     __shared__ int sRef[S * kBlock];
     __shared__ float sDist[S * kBlock];
     const int tid = threadIdx.x, lane = tid & 63;
