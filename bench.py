@@ -165,6 +165,9 @@ def psnr_study(args):
     s = native_api.Session(lib)
     for k, v in TUNED["icosphere"].items():
         s.set_option(k, v)   # before initialize: width and leaf size shape the build
+    if args.denoise_ensemble:
+        s.set_option("denoise_ensemble", args.denoise_ensemble)
+    ensemble = s.lib.RtGetOption(s.handle, b"denoise_ensemble")
     s.initialize(tris, w, h)
     log(f"psnr study: {desc}; reference {args.reference_spp} spp")
     t0 = time.perf_counter()
@@ -226,6 +229,7 @@ def psnr_study(args):
         eq = {"spp": f">{ladder[-1]['spp']}", "seconds": ladder[-1]["seconds"]}
     p256 = next(x for x in ladder if x["spp"] == 256)
     result = {"workload": desc, "reference_spp": args.reference_spp, "reference_seconds": round(ref_s, 2),
+              "denoise_ensemble": ensemble,
               "plain": ladder, "denoised_4spp": {"seconds": round(sec, 4), "psnr_db": round(dn_psnr, 3),
                                                  "plain_4spp_psnr_db": round(td.psnr(plain_img, ref_img), 3),
                                                  "tiles": tiles, "tiles_fallback": rejected,
@@ -272,6 +276,8 @@ def main():
     ap.add_argument("--psnr-study", action="store_true",
                     help="time-to-equal-PSNR of the learned denoiser vs plain accumulation on the icosphere")
     ap.add_argument("--reference-spp", type=int, default=4096)
+    ap.add_argument("--denoise-ensemble", type=int, choices=(1, 2, 4),
+                    help="network passes per denoise (mirrored inputs averaged); default: the library's")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run the plain copy kernel over BYTES (HBM counter calibration) and exit")
     args = ap.parse_args()

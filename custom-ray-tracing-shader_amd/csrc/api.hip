@@ -99,6 +99,9 @@ struct Session {
     int denoiseFallback = 0;           // 0 per-tile check, 1 always plain accumulation, 2 never
     float* dDenoiseIn = nullptr;       // [hp][wp][10]
     float* dDenoiseOut = nullptr;      // [hp][wp][4]
+    float* dDenoiseAuxIn = nullptr;    // [hp][wp][10] mirrored input (self-ensemble)
+    float* dDenoiseAuxOut = nullptr;   // [hp][wp][4] its network output
+    int denoiseEnsemble = 1;           // network passes per RtDenoise: 1, 2 (x mirror) or 4 (x, y, xy)
     int* dTileStats = nullptr;         // [0] tiles, [1] rejected
     int lastTiles = 0, lastRejected = 0;
     float lastInferMs = 0;
@@ -146,6 +149,7 @@ void freeDevice(Session* s) {
     devFree(s->dRayA); devFree(s->dRayB); devFree(s->dShadow); devFree(s->dHits);
     devFree(s->dCounters); devFree(s->dSpill); devFree(s->dOverflow); devFree(s->dStats);
     devFree(s->dDenoiseIn); devFree(s->dDenoiseOut); devFree(s->dTileStats);
+    devFree(s->dDenoiseAuxIn); devFree(s->dDenoiseAuxOut);
     for (hipEvent_t e : s->eventPool)
         if (e) (void)hipEventDestroy(e);
     s->eventPool.clear();
@@ -784,6 +788,10 @@ static bool denoiseBuffers(Session* s, int& hp, int& wp) {
     if (!s->dDenoiseIn && !devAlloc(s, s->dDenoiseIn, (size_t)hp * wp * kDenoiseInputChannels)) return false;
     if (!s->dDenoiseOut && !devAlloc(s, s->dDenoiseOut, (size_t)hp * wp * 4)) return false;
     if (!s->dTileStats && !devAlloc(s, s->dTileStats, 2)) return false;
+    if (s->denoiseEnsemble > 1) {
+        if (!s->dDenoiseAuxIn && !devAlloc(s, s->dDenoiseAuxIn, (size_t)hp * wp * kDenoiseInputChannels)) return false;
+        if (!s->dDenoiseAuxOut && !devAlloc(s, s->dDenoiseAuxOut, (size_t)hp * wp * 4)) return false;
+    }
     return true;
 }
 
@@ -810,6 +818,18 @@ int RtDenoise(void* p, unsigned char* output, int outputBytes, float* millisecon
     if ((s->error = launchDenoisePrepare(a, st)) != hipSuccess) return 0;
     if ((s->error = denoiserInferDevice(s->denoiser, s->dDenoiseIn, 1, hp, wp, s->dDenoiseOut, st)) != hipSuccess) return 0;
     s->lastInferMs = denoiserLastForwardMs(s->denoiser);
+    // Self-ensemble: the network also sees the input mirrored (in x; or in x, y and both),
+    // its outputs mirrored back and averaged with the first (training flips crops the same way).
+    for (int e = 1; e < s->denoiseEnsemble; e++) {
+        const int flip = s->denoiseEnsemble == 2 ? 1 : e;
+        if ((s->error = launchMirrorPlanes(s->dDenoiseIn, s->dDenoiseAuxIn, hp, wp, kDenoiseInputChannels, flip, st)) !=
+                hipSuccess ||
+            (s->error = denoiserInferDevice(s->denoiser, s->dDenoiseAuxIn, 1, hp, wp, s->dDenoiseAuxOut, st)) != hipSuccess ||
+            (s->error = launchAddMirrored(s->dDenoiseOut, s->dDenoiseAuxOut, hp, wp, 4, flip,
+                                          e + 1 == s->denoiseEnsemble ? 1.0f / s->denoiseEnsemble : 1.0f, st)) != hipSuccess)
+            return 0;
+        s->lastInferMs += denoiserLastForwardMs(s->denoiser);
+    }
     if ((s->error = hipMemsetAsync(s->dTileStats, 0, 2 * sizeof(int), st)) != hipSuccess) return 0;
     if ((s->error = launchDenoisePresent(a, st)) != hipSuccess) return 0;
     if ((s->error = hipStreamSynchronize(st)) != hipSuccess) return 0;
@@ -1059,6 +1079,10 @@ int RtSetOption(void* p, const char* key, int value) {
         if (value < 0 || value > kMaxBounces) return 0;
         s->tailBounce = value; return 1;
     }
+    if (!std::strcmp(key, "denoise_ensemble")) {
+        if (value != 1 && value != 2 && value != 4) return 0;
+        s->denoiseEnsemble = value; return 1;
+    }
     if (!std::strcmp(key, "while_while")) {
         if (value != 0 && value != 1) return 0;
         s->whileWhile = value; return 1;
@@ -1098,6 +1122,7 @@ int RtGetOption(void* p, const char* key) {
     if (!std::strcmp(key, "refill_below")) return s->refillBelow;
     if (!std::strcmp(key, "pipelines")) return s->pipelines;
     if (!std::strcmp(key, "tail_bounce")) return s->tailBounce;
+    if (!std::strcmp(key, "denoise_ensemble")) return s->denoiseEnsemble;
     if (!std::strcmp(key, "while_while")) return s->whileWhile;
     if (!std::strcmp(key, "tiled_primary")) return s->tiledPrimary;
     if (!std::strcmp(key, "nontemporal")) return s->nontemporal;

@@ -77,6 +77,11 @@ struct DenoiseArgs {
 };
 hipError_t launchDenoisePrepare(const DenoiseArgs& a, hipStream_t s);
 hipError_t launchDenoisePresent(const DenoiseArgs& a, hipStream_t s);
+// Self-ensemble of the denoiser: out = the [hp][wp][C] image mirrored in x (flip & 1) and/or
+// y (flip & 2); acc = (acc + mirrored img) * scale.
+hipError_t launchMirrorPlanes(const float* in, float* out, int hp, int wp, int channels, int flip, hipStream_t s);
+hipError_t launchAddMirrored(float* acc, const float* img, int hp, int wp, int channels, int flip, float scale,
+                             hipStream_t s);
 
 // Variant of the persistent traversal kernel.
 struct TraceLaunch {

@@ -979,6 +979,38 @@ __global__ __launch_bounds__(kBlock) void denoisePrepareKernel(DenoiseArgs a) {
     o[6] = nu.x; o[7] = nu.y; o[8] = nu.z;
     o[9] = r.w > 0 ? n.w / r.w : 0.0f;
 }
+__global__ void mirrorPlanesKernel(const float* in, float* out, int hp, int wp, int channels, int flip) {
+    const long long i = blockIdx.x * (long long)kBlock + threadIdx.x;
+    if (i >= (long long)hp * wp * channels) return;
+    const int c = (int)(i % channels);
+    const long long p = i / channels;
+    const int x = (int)(p % wp), y = (int)(p / wp);
+    const int sx = (flip & 1) ? wp - 1 - x : x, sy = (flip & 2) ? hp - 1 - y : y;
+    out[i] = in[((size_t)sy * wp + sx) * channels + c];
+}
+hipError_t launchMirrorPlanes(const float* in, float* out, int hp, int wp, int channels, int flip, hipStream_t s) {
+    const long long n = (long long)hp * wp * channels;
+    hipLaunchKernelGGL(mirrorPlanesKernel, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, in, out, hp, wp,
+                       channels, flip);
+    return hipGetLastError();
+}
+__global__ void addMirroredKernel(float* acc, const float* img, int hp, int wp, int channels, int flip, float scale) {
+    const long long i = blockIdx.x * (long long)kBlock + threadIdx.x;
+    if (i >= (long long)hp * wp * channels) return;
+    const int c = (int)(i % channels);
+    const long long p = i / channels;
+    const int x = (int)(p % wp), y = (int)(p / wp);
+    const int sx = (flip & 1) ? wp - 1 - x : x, sy = (flip & 2) ? hp - 1 - y : y;
+    acc[i] = (acc[i] + img[((size_t)sy * wp + sx) * channels + c]) * scale;
+}
+hipError_t launchAddMirrored(float* acc, const float* img, int hp, int wp, int channels, int flip, float scale,
+                             hipStream_t s) {
+    const long long n = (long long)hp * wp * channels;
+    hipLaunchKernelGGL(addMirroredKernel, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, acc, img, hp, wp,
+                       channels, flip, scale);
+    return hipGetLastError();
+}
+
 hipError_t launchDenoisePrepare(const DenoiseArgs& a, hipStream_t s) {
     hipLaunchKernelGGL(denoisePrepareKernel, dim3((a.wp * a.hp + kBlock - 1) / kBlock), dim3(kBlock), 0, s, a);
     return hipGetLastError();

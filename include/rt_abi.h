@@ -129,6 +129,8 @@ int RtRayCounts(void* session, unsigned long long* closest, unsigned long long* 
  * whose launches overlap, at least 65536 paths each, default 2; frames with trace
  * statistics use one), "tail_bounce" (0 = off, default; k >= 1: from bounce k on, each path
  * runs to its end inside one launch instead of one launch per bounce; images unchanged),
+ * "denoise_ensemble" (1, 2 or 4 network passes per RtDenoise: inputs mirrored in x, or in x, y
+ * and both, outputs mirrored back and averaged; default 1),
  * "tiled_primary", "nontemporal", "stage_timing", "trace_stats" and "denoise_fallback" at the
  * next render or query.  Returns 1 if the key is known and the value valid.  RtGetOption
  * returns -1 for unknown keys; it also reports "node_bytes", the size of one wide node. */

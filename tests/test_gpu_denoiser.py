@@ -208,3 +208,29 @@ def test_trained_network_beats_plain_accumulation(lib):
     ref_img = td.display(ref, st.exposure)
     gain = td.psnr(den, ref_img) - td.psnr(noisy, ref_img)
     assert gain > 3.0, gain
+
+
+@pytest.mark.skipif(not WEIGHTS.exists(), reason="trained weights not built (train_denoiser.py)")
+def test_denoise_ensemble_passes(lib):
+    """denoise_ensemble 2 and 4 add network passes on mirrored inputs (their outputs mirrored
+    back and averaged): the frames change, the reported network time grows, and the quality
+    stays that of one pass (within 0.5 dB here)."""
+    import train_denoiser as td
+    cam, st, tris = td.heldout_scene(4321)
+    w = h = 128
+    out = {}
+    with Session(lib) as s:
+        s.initialize(tris, w, h)
+        ref_img = td.display(td.render_mean(s, cam, st, 512, seed=99), st.exposure)
+        lib.RtReset(s.handle)
+        st.raysPerPixel, st.seed = 8, 42
+        s.render(cam, st)
+        for e in (1, 2, 4):
+            assert s.set_option("denoise_ensemble", e)
+            den, _ = s.denoise()
+            out[e] = (den, td.psnr(den, ref_img), s.denoise_stats()[2])
+        assert not s.set_option("denoise_ensemble", 3)
+    assert not np.array_equal(out[1][0], out[2][0]) and not np.array_equal(out[2][0], out[4][0])
+    assert out[4][2] > out[2][2] > out[1][2] > 0
+    for e in (2, 4):
+        assert out[e][1] > out[1][1] - 0.5, (e, out[e][1], out[1][1])
