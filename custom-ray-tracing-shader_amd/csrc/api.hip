@@ -70,6 +70,10 @@ struct Session {
     int* dSpill = nullptr;
     int* dOverflow = nullptr;
     unsigned long long* dStats = nullptr;
+    float4* dQueryRays = nullptr;    // RtTraceRays buffers, grown on demand and kept
+    int4* dQueryHits = nullptr;
+    int* dQueryWork = nullptr;
+    size_t queryCap = 0;
     long long slotCap = 0;
     int spillEntries = 0, traceBlocks = 0, traceLanes = 0, cus = 0;
     size_t spillLanes = 0;           // lanes each pipeline's spill ring is sized for
@@ -150,6 +154,8 @@ void freeDevice(Session* s) {
     devFree(s->dCounters); devFree(s->dSpill); devFree(s->dOverflow); devFree(s->dStats);
     devFree(s->dDenoiseIn); devFree(s->dDenoiseOut); devFree(s->dTileStats);
     devFree(s->dDenoiseAuxIn); devFree(s->dDenoiseAuxOut);
+    devFree(s->dQueryRays); devFree(s->dQueryHits); devFree(s->dQueryWork);
+    s->queryCap = 0;
     for (hipEvent_t e : s->eventPool)
         if (e) (void)hipEventDestroy(e);
     s->eventPool.clear();
@@ -844,6 +850,8 @@ int RtDenoise(void* p, unsigned char* output, int outputBytes, float* millisecon
 }
 
 // ------------------------------------------------------------------ extensions
+static_assert(sizeof(RtRay) == 2 * sizeof(float4) && sizeof(RtHit) == sizeof(int4), "query records match the queues");
+
 int RtTraceRays(void* p, const RtRay* rays, int count, int anyHit, RtHit* hits) {
     Session* s = static_cast<Session*>(p);
     if (!s) return 0;
@@ -851,9 +859,16 @@ int RtTraceRays(void* p, const RtRay* rays, int count, int anyHit, RtHit* hits) 
     if (!s->ready || !rays || !hits || count < 0 || count > (1 << 30)) { s->error = hipErrorInvalidValue; return 0; }
     if (count == 0) return 1;
     if (!prepareTraceGrid(s)) return 0;
-    float4* dr = nullptr; int4* dh = nullptr; int* work = nullptr;
-    bool ok = upload(s, dr, reinterpret_cast<const float4*>(rays), (size_t)count * 2) && devAlloc(s, dh, (size_t)count) &&
-              devAlloc(s, work, 1);
+    if ((size_t)count > s->queryCap) {
+        devFree(s->dQueryRays); devFree(s->dQueryHits);
+        s->queryCap = 0;
+        if (!devAlloc(s, s->dQueryRays, 2 * (size_t)count) || !devAlloc(s, s->dQueryHits, (size_t)count)) return 0;
+        s->queryCap = (size_t)count;
+    }
+    if (!s->dQueryWork && !devAlloc(s, s->dQueryWork, 1)) return 0;
+    float4* dr = s->dQueryRays; int4* dh = s->dQueryHits; int* work = s->dQueryWork;
+    bool ok = (s->error = hipMemcpyAsync(dr, rays, (size_t)count * sizeof(RtRay), hipMemcpyHostToDevice, s->stream)) ==
+              hipSuccess;
     if (ok) ok = (s->error = hipMemsetAsync(work, 0, sizeof(int), s->stream)) == hipSuccess;
     if (ok) {
         TraceArgs a = baseTraceArgs(s);
@@ -867,7 +882,6 @@ int RtTraceRays(void* p, const RtRay* rays, int count, int anyHit, RtHit* hits) 
     int overflow = 0;
     if (ok) ok = (s->error = hipMemcpy(&overflow, s->dOverflow, sizeof(int), hipMemcpyDeviceToHost)) == hipSuccess;
     if (ok && overflow) { s->hostError = "traversal stack exceeded its spill ring"; ok = false; }
-    devFree(dr); devFree(dh); devFree(work);
     return ok ? 1 : 0;
 }
 
