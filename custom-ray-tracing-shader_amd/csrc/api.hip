@@ -69,6 +69,7 @@ struct Session {
     int* dCounters = nullptr;     // kMaxRaysPerPixel chunks x kPipelines x kCounterStride
     int* dSpill = nullptr;
     int* dOverflow = nullptr;
+    int* hCounters = nullptr;        // pinned host copy of the counters, then the overflow flag
     unsigned long long* dStats = nullptr;
     float4* dQueryRays = nullptr;    // RtTraceRays buffers, grown on demand and kept
     int4* dQueryHits = nullptr;
@@ -152,6 +153,8 @@ void freeDevice(Session* s) {
     devFree(s->dThr); devFree(s->dRad); devFree(s->dAlb); devFree(s->dNrm); devFree(s->dNee);
     devFree(s->dRayA); devFree(s->dRayB); devFree(s->dShadow); devFree(s->dHits);
     devFree(s->dCounters); devFree(s->dSpill); devFree(s->dOverflow); devFree(s->dStats);
+    if (s->hCounters) (void)hipHostFree(s->hCounters);
+    s->hCounters = nullptr;
     devFree(s->dDenoiseIn); devFree(s->dDenoiseOut); devFree(s->dTileStats);
     devFree(s->dDenoiseAuxIn); devFree(s->dDenoiseAuxOut);
     devFree(s->dQueryRays); devFree(s->dQueryHits); devFree(s->dQueryWork);
@@ -393,13 +396,17 @@ bool renderGpu(Session* s, const CameraData& cam, const RenderSettings& rs, floa
         if ((s->error = launchAccumulate(a, st)) != hipSuccess) return false;
     }
     if ((s->error = hipEventRecord(s->ev1, st)) != hipSuccess) return false;
-    if ((s->error = hipEventSynchronize(s->ev1)) != hipSuccess) return false;
+    // Counters and the overflow flag come back with the frame, in one wait.
+    const size_t nCtr = (size_t)chunk * kPipelines * kCounterStride;
+    const int* ctrs = s->hCounters;
+    if ((s->error = hipMemcpyAsync(s->hCounters, s->dCounters, nCtr * sizeof(int), hipMemcpyDeviceToHost, st)) !=
+            hipSuccess ||
+        (s->error = hipMemcpyAsync(s->hCounters + nCtr, s->dOverflow, sizeof(int), hipMemcpyDeviceToHost, st)) !=
+            hipSuccess ||
+        (s->error = hipStreamSynchronize(st)) != hipSuccess)
+        return false;
     if ((s->error = hipEventElapsedTime(ms, s->ev0, s->ev1)) != hipSuccess) return false;
-    std::vector<int> ctrs((size_t)chunk * kPipelines * kCounterStride);
-    int overflow = 0;
-    if ((s->error = hipMemcpy(ctrs.data(), s->dCounters, ctrs.size() * sizeof(int), hipMemcpyDeviceToHost)) != hipSuccess) return false;
-    if ((s->error = hipMemcpy(&overflow, s->dOverflow, sizeof(int), hipMemcpyDeviceToHost)) != hipSuccess) return false;
-    if (overflow) { s->hostError = "traversal stack exceeded its spill ring"; return false; }
+    if (s->hCounters[nCtr]) { s->hostError = "traversal stack exceeded its spill ring"; return false; }
     unsigned long long closest = 0, shadow = 0;
     for (int c = 0; c < chunk; c++)
         for (int p = 0; p < chunkPipelines[c]; p++)
@@ -603,6 +610,11 @@ int RtInitialize(void* p, const Triangle* triangles, int count, int w, int h) {
         !devAlloc(s, s->dCounters, (size_t)kMaxRaysPerPixel * kPipelines * kCounterStride) ||
         !devAlloc(s, s->dOverflow, 1) || !devAlloc(s, s->dStats, 4))
         return 0;
+    if ((s->error = hipHostMalloc(&s->hCounters, ((size_t)kMaxRaysPerPixel * kPipelines * kCounterStride + 1) * sizeof(int),
+                                  hipHostMallocDefault)) != hipSuccess) {
+        s->hCounters = nullptr;
+        return 0;
+    }
     s->cus = cus;
     s->spillLanes = 0;
     if (!prepareTraceGrid(s)) return 0;
