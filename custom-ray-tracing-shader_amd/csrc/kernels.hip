@@ -1,9 +1,12 @@
 // kernels.hip - gfx950 kernels of the wavefront path tracer.
 //
-// One RtRender call runs, per chunk of R sample groups (R*npix path slots):
-//   reset counters -> raygen -> for each bounce: closest-hit traversal (persistent lanes)
-//   -> shade (NEE shadow request + next ray, compacted with wave-aggregated atomics)
-//   -> any-hit traversal of the shadow queue (adds the NEE term when unoccluded)
+// One RtRender call runs, per chunk of R sample groups (R*npix path slots) and per path
+// pipeline (a part of the chunk's paths on its own stream):
+//   reset counters -> raygen -> for each bounce b: one traversal launch over bounce b's
+//   closest-hit queue and bounce b-1's shadow queue (persistent lanes; a shadow ray adds its
+//   NEE term when unoccluded) -> shade (NEE shadow request + next ray, compacted with
+//   workgroup-aggregated atomics) -> traversal of the last bounce's shadow queue
+//   [or, from a chosen bounce on, one path-completion launch for the remaining bounces]
 //   -> accumulate (adds the R groups to the FP32 sums in sample order, tone-maps).
 // Queue sizes live in device memory, so no host synchronisation happens inside a frame.
 #include "kernels.h"
