@@ -110,6 +110,10 @@ def autotune(triangles, camera, settings, width, height, frames=3, widths=WIDTHS
             if p != launch["pipelines"]:
                 trial(s, dict(launch, pipelines=p))
         launch = {k: v for k, v in best.items() if k != "mrays_per_s"}
+        for b in BLOCKS:   # the best workgroup count depends on how many pipelines share the GPU
+            if b != launch["blocks_per_cu"]:
+                trial(s, dict(launch, blocks_per_cu=b))
+        launch = {k: v for k, v in best.items() if k != "mrays_per_s"}
         for t in TAILS:
             if t != launch.get("tail_bounce", 0) and t < settings.maxBounces:
                 trial(s, dict(launch, tail_bounce=t))
