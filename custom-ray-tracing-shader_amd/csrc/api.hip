@@ -42,7 +42,7 @@ struct Session {
     int builtWidth = 4;
     int bvhQuantized = 0;        // option: quantized (QNode) wide nodes at the next RtInitialize
     int builtQuantized = 0;
-    int blocksPerCU = 8;         // persistent traversal workgroups per CU
+    int blocksPerCU = 3;         // persistent traversal workgroups per CU (per pipeline)
     int shortStack = 8;          // LDS stack entries per traversal lane (8 or 16)
     int refill = 1;              // per-lane ray refill in the traversal kernel
     int refillBelow = 40;        // busy lanes below which a refilling wave fetches rays
@@ -76,7 +76,7 @@ struct Session {
     int* dQueryWork = nullptr;
     size_t queryCap = 0;
     long long slotCap = 0;
-    int spillEntries = 0, traceBlocks = 0, traceLanes = 0, cus = 0;
+    int spillEntries = 0, traceBlocks = 0, traceLanes = 0, queryBlocks = 0, cus = 0;
     size_t spillLanes = 0;           // lanes each pipeline's spill ring is sized for
     hipStream_t stream = nullptr;
     hipStream_t pipeStream[3] = {};  // streams of path pipelines 1..3 (default priority, as stream)
@@ -234,15 +234,21 @@ TraceLaunch traceLaunch(const Session* s) {
 }
 
 // Sizes the persistent traversal grid from the current options; grows the spill ring.
+// RtTraceRays runs one launch with nothing beside it, so it takes at least this many
+// persistent workgroups per CU whatever blocks_per_cu (sized for concurrent pipelines) says.
+constexpr int kQueryBlocksPerCU = 6;
+
 bool prepareTraceGrid(Session* s) {
     s->traceBlocks = s->cus * s->blocksPerCU;
     s->traceLanes = s->traceBlocks * kBlock;
+    s->queryBlocks = s->cus * std::max(s->blocksPerCU, kQueryBlocksPerCU);
     s->spillEntries = 36 * (s->builtWidth - 1);   // >= (W-1) * (wide depth <= 33) entries
-    if ((size_t)s->traceLanes > s->spillLanes) {
+    const size_t lanes = (size_t)s->queryBlocks * kBlock;   // >= traceLanes
+    if (lanes > s->spillLanes) {
         devFree(s->dSpill);
         s->spillLanes = 0;
-        if (!devAlloc(s, s->dSpill, (size_t)kPipelines * s->spillEntries * s->traceLanes * 2)) return false;
-        s->spillLanes = (size_t)s->traceLanes;
+        if (!devAlloc(s, s->dSpill, (size_t)kPipelines * s->spillEntries * lanes * 2)) return false;
+        s->spillLanes = lanes;
     }
     return true;
 }
@@ -887,7 +893,8 @@ int RtTraceRays(void* p, const RtRay* rays, int count, int anyHit, RtHit* hits) 
         a.work = work;
         if (anyHit) { a.anyRays = dr; a.anyCount = count; a.anyHits = dh; }
         else { a.rays = dr; a.count = count; a.hits = dh; }
-        ok = (s->error = launchTrace(traceLaunch(s), false, s->traceBlocks, a, s->stream)) == hipSuccess;
+        a.totalLanes = s->queryBlocks * kBlock;
+        ok = (s->error = launchTrace(traceLaunch(s), false, s->queryBlocks, a, s->stream)) == hipSuccess;
     }
     if (ok) ok = (s->error = hipStreamSynchronize(s->stream)) == hipSuccess;
     if (ok) ok = (s->error = hipMemcpy(hits, dh, (size_t)count * sizeof(RtHit), hipMemcpyDeviceToHost)) == hipSuccess;

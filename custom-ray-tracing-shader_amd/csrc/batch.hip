@@ -53,7 +53,7 @@ struct PendingView {
 struct Batch {
     std::vector<PendingView> pending;
     std::string error;
-    int width = 4, quantized = 0, blocksPerCU = 8, shortStack = 8, refill = 1, refillBelow = 40, whileWhile = 1,
+    int width = 4, quantized = 0, blocksPerCU = 3, shortStack = 8, refill = 1, refillBelow = 40, whileWhile = 1,
         useGraph = 1, pipelines = 2;
     bool built = false;
     int cus = 0, views = 0, totalSlots = 0, maxBounces = 0;
