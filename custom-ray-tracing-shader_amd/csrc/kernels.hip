@@ -79,6 +79,56 @@ struct LdsStack {
         dist[slot * kBlock + tid] = dd;
         sp++;
     }
+    // Before n entries are written at sp..sp+n-1: the ring entries they displace (index
+    // p - S for each p >= S) move to the spill ring, as push() does one at a time.
+    __device__ __forceinline__ void spillBelow(int sp, int n, bool& overflow) {
+        if (sp + n <= S) return;
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const int p = sp + k;
+            if (k < n && p >= S) {
+                const int e = p - S, slot = p & (S - 1);
+                if (e < spillEntries)
+                    spill[(size_t)e * lanes + gl] = make_int2(ref[slot * kBlock + tid], __float_as_int(dist[slot * kBlock + tid]));
+                else
+                    overflow = true;
+            }
+        }
+    }
+    __device__ __forceinline__ void put(int p, int r, float dd) {
+        const int slot = p & (S - 1);
+        ref[slot * kBlock + tid] = r;
+        dist[slot * kBlock + tid] = dd;
+    }
+    // Pushes the entered children 3, 2, 1 of a distance-sorted 4-wide node, far to near, with
+    // the stack pointer and spill test computed once instead of per entry: entered children
+    // form a prefix of the sorted order (INFINITY sorts last), so child j <= n goes to
+    // sp + n - j.
+    __device__ __forceinline__ void pushSorted3(int& sp, const float* d, const int* r, bool& overflow) {
+        const bool v1 = d[1] != INFINITY, v2 = d[2] != INFINITY, v3 = d[3] != INFINITY;
+        const int n = (int)v1 + (int)v2 + (int)v3;
+        spillBelow(sp, n, overflow);
+        if (v1) put(sp + n - 1, r[1], d[1]);
+        if (v2) put(sp + n - 2, r[2], d[2]);
+        if (v3) put(sp + n - 3, r[3], d[3]);
+        sp += n;
+    }
+    // Any-hit visit of a 4-wide node: descends into the first entered child in slot order and
+    // pushes the other entered children in slot order (as push() calls would).  Child 0 is
+    // never pushed; child c > 0 is pushed when entered after an earlier entered child, at
+    // sp + (children pushed before it).
+    __device__ __forceinline__ bool descendAny4(int& sp, const float* d, const int* r, bool& overflow, int* next) {
+        const bool v0 = d[0] != INFINITY, v1 = d[1] != INFINITY, v2 = d[2] != INFINITY, v3 = d[3] != INFINITY;
+        *next = v0 ? r[0] : v1 ? r[1] : v2 ? r[2] : r[3];
+        const bool u1 = v0 && v1, u2 = (v0 || v1) && v2, u3 = (v0 || v1 || v2) && v3;
+        const int b2 = (int)u1, b3 = b2 + (int)u2, n = b3 + (int)u3;
+        spillBelow(sp, n, overflow);
+        if (u1) put(sp, r[1], d[1]);
+        if (u2) put(sp + b2, r[2], d[2]);
+        if (u3) put(sp + b3, r[3], d[3]);
+        sp += n;
+        return v0 || v1 || v2 || v3;
+    }
     __device__ __forceinline__ void pop(int& sp, int& r, float& dd) {
         sp--;
         const int slot = sp & (S - 1);
@@ -139,6 +189,25 @@ __device__ __forceinline__ void childEntries<WideNode<8>>(const WideNode<8>* nod
     }
 }
 
+// sortChildren<4>'s network (the same comparisons, so the same order, ties included) as
+// selects: every lane runs the five exchanges without branching.
+__device__ __forceinline__ void exchangeSelect(float& da, float& db, int& ra, int& rb) {
+    const bool s = db < da;
+    const float lo = s ? db : da, hi = s ? da : db;
+    const int rl = s ? rb : ra, rh = s ? ra : rb;
+    da = lo; db = hi; ra = rl; rb = rh;
+}
+__device__ __forceinline__ void sortChildrenSelect4(float* d, int* r) {
+    // Named values rather than array slots: selects between array elements would otherwise be
+    // rewritten into selects of indices.
+    float d0 = d[0], d1 = d[1], d2 = d[2], d3 = d[3];
+    int r0 = r[0], r1 = r[1], r2 = r[2], r3 = r[3];
+    exchangeSelect(d0, d1, r0, r1); exchangeSelect(d2, d3, r2, r3); exchangeSelect(d0, d2, r0, r2);
+    exchangeSelect(d1, d3, r1, r3); exchangeSelect(d1, d2, r1, r2);
+    d[0] = d0; d[1] = d1; d[2] = d2; d[3] = d3;
+    r[0] = r0; r[1] = r1; r[2] = r2; r[3] = r3;
+}
+
 // Pushes the entered children that are not descended into; returns whether the lane
 // descends (into *next).  Closest-hit queries go to the nearest child and push the others
 // far to near.  Any-hit queries take the first entered child in slot order and skip the
@@ -146,6 +215,9 @@ __device__ __forceinline__ void childEntries<WideNode<8>>(const WideNode<8>* nod
 // any-hit rays (otherwise every query is closest-hit and the any-hit paths compile away).
 template <int W, int S, bool MIXED>
 __device__ __forceinline__ bool descend(LaneRay& L, LdsStack<S>& st, float* dist, int* cr, bool& overflow, int* next) {
+    if constexpr (W == 4) {
+        if (MIXED && L.any) return st.descendAny4(L.sp, dist, cr, overflow, next);
+    }
     if (MIXED && L.any) {
         bool taken = false;
 #pragma unroll
@@ -155,6 +227,12 @@ __device__ __forceinline__ bool descend(LaneRay& L, LdsStack<S>& st, float* dist
             if (in && !taken) { *next = cr[c]; taken = true; }
         }
         return taken;
+    }
+    if constexpr (W == 4) {
+        sortChildrenSelect4(dist, cr);
+        st.pushSorted3(L.sp, dist, cr, overflow);
+        *next = cr[0];
+        return dist[0] != INFINITY;
     }
     sortChildren<W>(dist, cr);
 #pragma unroll
