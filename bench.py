@@ -72,7 +72,8 @@ def bench_batch(args):
     import native_api
     views = batch_views(args.batch_views)
     cfg = dict(TUNED["batch"])
-    for k in ("bvh_width", "bvh_quantized", "blocks_per_cu", "short_stack", "refill", "refill_below", "while_while"):
+    for k in ("bvh_width", "bvh_quantized", "blocks_per_cu", "short_stack", "refill", "refill_below", "while_while",
+              "pipelines", "xcd_segments"):
         if getattr(args, k) is not None:
             cfg[k] = getattr(args, k)
     b = native_api.Batch(**cfg)
@@ -269,6 +270,8 @@ def main():
     ap.add_argument("--build-max-leaf", type=int)
     ap.add_argument("--bvh-quantized", type=int, choices=(0, 1))
     ap.add_argument("--while-while", type=int, choices=(0, 1))
+    ap.add_argument("--xcd-segments", type=int, choices=(0, 1),
+                    help="per-XCD segment cursors in the traversal launches")
     ap.add_argument("--autotune", action="store_true",
                     help="search the launch configuration now instead of using the recorded tuned one")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
@@ -303,7 +306,7 @@ def main():
     log(f"workload: {desc}")
     cfg = dict(TUNED[args.workload])
     for k in ("bvh_width", "bvh_quantized", "build_max_leaf", "blocks_per_cu", "short_stack", "refill", "refill_below",
-              "while_while", "pipelines", "tail_bounce"):
+              "while_while", "pipelines", "tail_bounce", "xcd_segments"):
         if getattr(args, k) is not None:
             cfg[k] = getattr(args, k)
     if args.autotune:
@@ -315,7 +318,7 @@ def main():
         log(f"autotune: {len(trials)} configurations in {time.perf_counter() - t0:.1f} s, best {best}")
     s = native_api.Session()
     for k in ("bvh_width", "bvh_quantized", "build_max_leaf", "blocks_per_cu", "short_stack", "refill", "refill_below",
-              "while_while", "pipelines", "tail_bounce"):
+              "while_while", "pipelines", "tail_bounce", "xcd_segments"):
         if k in cfg:
             s.set_option(k, cfg[k])
     t0 = time.perf_counter()

@@ -51,6 +51,7 @@ struct Session {
     int pipelines = 2;           // independent path pipelines per chunk, each on its own stream
     int tailBounce = 0;          // >= 1: paths complete from this bounce on in one launch (0: off)
     int nontemporal = 1;         // rays / hits bypass cache retention in traversal
+    int xcdSegments = 1;         // per-XCD segment cursors in the frame's traversal launches
     std::vector<unsigned char> hostNodes;
     std::vector<TriRecord> hostRecs;
     std::vector<HostSum> hostSum;
@@ -67,6 +68,7 @@ struct Session {
     float4 *dRayA = nullptr, *dRayB = nullptr, *dShadow = nullptr;
     int4* dHits = nullptr;
     int* dCounters = nullptr;     // kMaxRaysPerPixel chunks x kPipelines x kCounterStride
+    int* dXcdWork = nullptr;      // kPipelines x kXcdSlotInts segment cursors
     int* dSpill = nullptr;
     int* dOverflow = nullptr;
     int* hCounters = nullptr;        // pinned host copy of the counters, then the overflow flag
@@ -152,7 +154,7 @@ void freeDevice(Session* s) {
     devFree(s->dSumRad); devFree(s->dSumAlb); devFree(s->dSumNrm); devFree(s->dPixels);
     devFree(s->dThr); devFree(s->dRad); devFree(s->dAlb); devFree(s->dNrm); devFree(s->dNee);
     devFree(s->dRayA); devFree(s->dRayB); devFree(s->dShadow); devFree(s->dHits);
-    devFree(s->dCounters); devFree(s->dSpill); devFree(s->dOverflow); devFree(s->dStats);
+    devFree(s->dCounters); devFree(s->dXcdWork); devFree(s->dSpill); devFree(s->dOverflow); devFree(s->dStats);
     if (s->hCounters) (void)hipHostFree(s->hCounters);
     s->hCounters = nullptr;
     devFree(s->dDenoiseIn); devFree(s->dDenoiseOut); devFree(s->dTileStats);
@@ -264,6 +266,12 @@ TraceArgs baseTraceArgs(Session* s, int p = 0) {
     return a;
 }
 
+// Segment cursors of pipeline p's work slot k (the launch that uses work cursor
+// kWorkRays + k), or null when the option is off.
+static int* xcdWork(Session* s, int p, int k) {
+    return s->xcdSegments ? s->dXcdWork + (size_t)p * kXcdSlotInts + (size_t)k * kXcdCursorInts : nullptr;
+}
+
 // Shadow rays of bounce b-1 wait to be traced in launch b.
 static bool shadowAt(bool nee, int b) { return nee && b > 0; }
 
@@ -323,7 +331,7 @@ bool renderGpu(Session* s, const CameraData& cam, const RenderSettings& rs, floa
             fp.qBase = qBase; fp.qCount = qCount;
             {
                 StageScope sc(s, RT_STAGE_RESET, pst[p]);
-                if ((s->error = launchResetCounters(ctr[p], qCount, pst[p])) != hipSuccess) return false;
+                if ((s->error = launchResetCounters(ctr[p], qCount, xcdWork(s, p, 0), pst[p])) != hipSuccess) return false;
             }
             fp.rayOut = s->dRayA + 2 * (size_t)qBase;
             StageScope sc(s, RT_STAGE_RAYGEN, pst[p]);
@@ -342,6 +350,7 @@ bool renderGpu(Session* s, const CameraData& cam, const RenderSettings& rs, floa
                     if (shadowAt(nee, b)) {
                         TraceArgs sa = baseTraceArgs(s, p);
                         sa.work = ctr[p] + kWorkRays + rs.maxBounces;   // unused by the per-bounce launches
+                        sa.xcdWork = xcdWork(s, p, rs.maxBounces);
                         sa.stats = s->dStats;
                         sa.anyRays = f[p].shadow; sa.anyCountPtr = ctr[p] + kCntShadow + (b - 1);
                         sa.rad = s->dRad; sa.contrib = s->dNee;
@@ -369,6 +378,7 @@ bool renderGpu(Session* s, const CameraData& cam, const RenderSettings& rs, floa
                 float4* out = ((b & 1) ? s->dRayA : s->dRayB) + off;
                 TraceArgs ta = baseTraceArgs(s, p);
                 ta.work = ctr[p] + kWorkRays + b;
+                ta.xcdWork = xcdWork(s, p, b);
                 ta.stats = s->dStats;
                 if (rays) { ta.rays = in; ta.countPtr = ctr[p] + kCntRays + b; ta.hits = f[p].hits; }
                 if (shadow) {
@@ -614,6 +624,7 @@ int RtInitialize(void* p, const Triangle* triangles, int count, int w, int h) {
         !devAlloc(s, s->dRayA, 2 * slots) || !devAlloc(s, s->dRayB, 2 * slots) || !devAlloc(s, s->dShadow, 2 * slots) ||
         !devAlloc(s, s->dHits, slots) ||
         !devAlloc(s, s->dCounters, (size_t)kMaxRaysPerPixel * kPipelines * kCounterStride) ||
+        !devAlloc(s, s->dXcdWork, (size_t)kPipelines * kXcdSlotInts) ||
         !devAlloc(s, s->dOverflow, 1) || !devAlloc(s, s->dStats, 4))
         return 0;
     if ((s->error = hipHostMalloc(&s->hCounters, ((size_t)kMaxRaysPerPixel * kPipelines * kCounterStride + 1) * sizeof(int),
@@ -1128,6 +1139,10 @@ int RtSetOption(void* p, const char* key, int value) {
         if (value != 0 && value != 1) return 0;
         s->nontemporal = value; return 1;
     }
+    if (!std::strcmp(key, "xcd_segments")) {
+        if (value != 0 && value != 1) return 0;
+        s->xcdSegments = value; return 1;
+    }
     if (!std::strcmp(key, "stage_timing")) {
         if (value != 0 && value != 1) return 0;
         s->stageTiming = value != 0; return 1;
@@ -1158,6 +1173,7 @@ int RtGetOption(void* p, const char* key) {
     if (!std::strcmp(key, "denoise_ensemble")) return s->denoiseEnsemble;
     if (!std::strcmp(key, "while_while")) return s->whileWhile;
     if (!std::strcmp(key, "tiled_primary")) return s->tiledPrimary;
+    if (!std::strcmp(key, "xcd_segments")) return s->xcdSegments;
     if (!std::strcmp(key, "nontemporal")) return s->nontemporal;
     if (!std::strcmp(key, "stage_timing")) return s->stageTiming ? 1 : 0;
     if (!std::strcmp(key, "trace_stats")) return s->traceStats ? 1 : 0;

@@ -54,7 +54,7 @@ struct Batch {
     std::vector<PendingView> pending;
     std::string error;
     int width = 4, quantized = 0, blocksPerCU = 3, shortStack = 8, refill = 1, refillBelow = 40, whileWhile = 1,
-        useGraph = 1, pipelines = 2;
+        useGraph = 1, pipelines = 2, xcdSegments = 1;
     bool built = false;
     int cus = 0, views = 0, totalSlots = 0, maxBounces = 0;
     long long totalTris = 0;
@@ -74,6 +74,7 @@ struct Batch {
     float4 *dRayA = nullptr, *dRayB = nullptr, *dShadow = nullptr;
     int4* dHits = nullptr;
     int *dCounters = nullptr, *dSpill = nullptr, *dOverflow = nullptr;   // counters: kBatchPipelines x kCounterStride
+    int* dXcdWork = nullptr;          // kBatchPipelines x kXcdSlotInts segment cursors
     int spillEntries = 0, traceBlocks = 0;
     hipStream_t stream = nullptr;
     hipStream_t pipeStream[3] = {};   // path pipelines 1..3 (default priority, as stream)
@@ -101,7 +102,7 @@ void release(Batch* b) {
     dfree(b->dTris); dfree(b->dLights); dfree(b->dNodes); dfree(b->dRecs); dfree(b->dViews); dfree(b->dViewBase);
     dfree(b->dRoots); dfree(b->dSamples); dfree(b->dSumRad); dfree(b->dSumAlb); dfree(b->dSumNrm); dfree(b->dPixels);
     dfree(b->dThr); dfree(b->dRad); dfree(b->dAlb); dfree(b->dNrm); dfree(b->dNee); dfree(b->dRayA); dfree(b->dRayB);
-    dfree(b->dShadow); dfree(b->dHits); dfree(b->dCounters); dfree(b->dSpill); dfree(b->dOverflow);
+    dfree(b->dShadow); dfree(b->dHits); dfree(b->dCounters); dfree(b->dXcdWork); dfree(b->dSpill); dfree(b->dOverflow);
     if (b->ev0) (void)hipEventDestroy(b->ev0);
     if (b->ev1) (void)hipEventDestroy(b->ev1);
     if (b->evFork) (void)hipEventDestroy(b->evFork);
@@ -138,6 +139,11 @@ TraceLaunch launchOf(const Batch* b) {
 // on its own stream with its own counters, queue regions and spill ring, as in a session.
 int framePipelines(const Batch* b) { return std::max(1, std::min(b->pipelines, b->totalSlots >> 16)); }
 
+// Segment cursors of pipeline p's launch k, or null when the option is off.
+int* xcdCursors(Batch* b, int p, int k) {
+    return b->xcdSegments ? b->dXcdWork + (size_t)p * kXcdSlotInts + (size_t)k * kXcdCursorInts : nullptr;
+}
+
 bool enqueueFrame(Batch* b) {
     hipStream_t st = b->stream;
     const int P = framePipelines(b);
@@ -154,7 +160,7 @@ bool enqueueFrame(Batch* b) {
         a[p].counters = b->dCounters + (size_t)p * kCounterStride;
         a[p].shadow = b->dShadow + 2 * (size_t)qBase; a[p].hits = b->dHits + qBase;
         a[p].qBase = qBase; a[p].qCount = qCount;
-        B_TRY(launchResetCounters(a[p].counters, qCount, pst[p]));
+        B_TRY(launchResetCounters(a[p].counters, qCount, xcdCursors(b, p, 0), pst[p]));
         a[p].rayOut = b->dRayA + 2 * (size_t)qBase;
         B_TRY(launchBatchRaygen(a[p], pst[p]));
     }
@@ -173,6 +179,7 @@ bool enqueueFrame(Batch* b) {
             ta.totalLanes = b->traceBlocks * kBlock; ta.overflow = b->dOverflow;
             ta.roots = b->dRoots; ta.viewBase = b->dViewBase; ta.views = b->views; ta.nontemporal = 1;
             ta.work = a[p].counters + kWorkRays + bounce;
+            ta.xcdWork = xcdCursors(b, p, bounce);
             if (rays) { ta.rays = in; ta.countPtr = a[p].counters + kCntRays + bounce; ta.hits = a[p].hits; }
             if (bounce > 0) {
                 ta.anyRays = a[p].shadow; ta.anyCountPtr = a[p].counters + kCntShadow + (bounce - 1);
@@ -296,6 +303,7 @@ bool build(Batch* b) {
         !dalloc(b, b->dThr, n) || !dalloc(b, b->dRad, n) || !dalloc(b, b->dAlb, n) || !dalloc(b, b->dNrm, n) ||
         !dalloc(b, b->dNee, n) || !dalloc(b, b->dRayA, 2 * n) || !dalloc(b, b->dRayB, 2 * n) ||
         !dalloc(b, b->dShadow, 2 * n) || !dalloc(b, b->dHits, n) || !dalloc(b, b->dCounters, (size_t)kBatchPipelines * kCounterStride) ||
+        !dalloc(b, b->dXcdWork, (size_t)kBatchPipelines * kXcdSlotInts) ||
         !dalloc(b, b->dOverflow, 1) || !dalloc(b, b->dViews, b->views) || !dalloc(b, b->dViewBase, b->views + 1) ||
         !dalloc(b, b->dRoots, b->views) || !dalloc(b, b->dSamples, b->views))
         return false;
@@ -381,6 +389,7 @@ int RtBatchSetOption(void* p, const char* key, int value) {
     if (!std::strcmp(key, "while_while") && (value == 0 || value == 1)) { b->whileWhile = value; return 1; }
     if (!std::strcmp(key, "use_graph") && (value == 0 || value == 1)) { b->useGraph = value; return 1; }
     if (!std::strcmp(key, "pipelines") && value >= 1 && value <= kBatchPipelines) { b->pipelines = value; return 1; }
+    if (!std::strcmp(key, "xcd_segments") && (value == 0 || value == 1)) { b->xcdSegments = value; return 1; }
     return 0;
 }
 

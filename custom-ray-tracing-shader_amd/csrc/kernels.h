@@ -10,6 +10,12 @@ constexpr int kCounterStride = 4 * (kMaxBounces + 1);   // ints per chunk of fra
 // [33,66) shadow queue sizes, [66,99) work cursors of the traversal launches (launch b
 // traces bounce b's closest-hit rays and bounce b-1's shadow rays), [99,132) unused.
 constexpr int kCntRays = 0, kCntShadow = kMaxBounces + 1, kWorkRays = 2 * (kMaxBounces + 1);
+// Segmented work cursors: a launch's ray index space is cut into kXcds contiguous segments,
+// one per group of workgroups that share an XCD (workgroups are dispatched to the eight XCDs
+// round-robin, so blockIdx.x % kXcds picks the group), each with its own cursor, kept
+// kXcdCursorStride ints apart.  A pipeline holds one set per work cursor slot above.
+constexpr int kXcds = 8, kXcdCursorStride = 32, kXcdCursorInts = kXcds * kXcdCursorStride;
+constexpr int kXcdSlotInts = (kMaxBounces + 1) * kXcdCursorInts;   // per pipeline
 
 // One traversal launch covers two queues: closest-hit rays, then any-hit rays (ray indices
 // past the closest-hit count).  Either count may be zero.
@@ -37,6 +43,8 @@ struct TraceArgs {
     const int* viewBase;     // batched views: first path slot per view, views + 1 entries
     int views;
     int refillBelow;         // set by launchTrace from TraceLaunch::refillBelow
+    int* xcdWork;            // optional: kXcds segment cursors (kXcdCursorStride apart), all 0 at
+                             // launch; null: one cursor (work) for the whole index space
 };
 
 struct FrameArgs {
@@ -93,7 +101,9 @@ struct TraceLaunch {
     bool whileWhile = true; // speculative while-while loop (inner-node and leaf phases)
 };
 hipError_t launchTrace(const TraceLaunch& l, bool stats, int blocks, const TraceArgs& a, hipStream_t s);
-hipError_t launchResetCounters(int* counters, int initialRays, hipStream_t s);
+// Zeroes one chunk's counters (closest-ray count of bounce 0 = initialRays) and, when
+// xcdWork is not null, kXcdSlotInts segment cursors.
+hipError_t launchResetCounters(int* counters, int initialRays, int* xcdWork, hipStream_t s);
 hipError_t launchRaygen(const FrameArgs& f, hipStream_t s);
 hipError_t launchShade(const FrameArgs& f, int blocks, hipStream_t s);
 // Completes every path of the closest-hit queue a.rays (bounce firstBounce, >= 1) through its

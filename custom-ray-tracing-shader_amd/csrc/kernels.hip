@@ -418,11 +418,14 @@ __device__ __forceinline__ void startRay(const TraceArgs& a, int i, int closest,
     L.leaf = 0;
 }
 
+// First index of segment k of n over [0, total).
+__device__ __forceinline__ int segmentBound(int total, int k, int n) { return (int)((long long)total * k / n); }
+
 // Persistent traversal of a closest-hit queue followed by an any-hit queue, in one ray index
 // space, so the two overlap in one launch.  Each wave's first 64 rays are fixed by its
-// position in the grid;
-// after that it takes rays from the shared cursor (which counts the rays past the grid's
-// first batch), so the launch does not start with one atomic per wave on one address.
+// position in the grid; after that it takes rays from a cursor (which counts the rays past
+// the first batch), so the launch does not start with one atomic per wave on one address.
+// The cursor is shared by the whole grid, or (a.xcdWork) one per XCD segment.
 // With REFILL, lanes whose ray finished take new rays as soon as fewer than refillBelow
 // lanes are busy, instead of waiting for the wave's slowest ray.
 #ifndef RT_TRACE_MIN_WAVES
@@ -443,27 +446,46 @@ __global__ __launch_bounds__(kBlock, RT_TRACE_MIN_WAVES) void traceKernel(TraceA
     bool overflow = false;
     LaneRay L;
     L.index = -1;
-    const int firstRays = gridDim.x * kBlock;   // one ray per launched lane
+    // With segment cursors, group g = blockIdx.x % X (the workgroups of one XCD) starts in
+    // segment g of the index space, so consecutive rays (neighbouring pixels, nearby paths)
+    // share that XCD's L2, and moves on to the other segments when its own is drained.  A
+    // launch with fewer than two rays per lane keeps one cursor: its waves would spend more on
+    // the atomics that find drained segments than they gain.
+    const int X = (a.xcdWork && gridDim.x % kXcds == 0 && total >= 2 * (int)gridDim.x * kBlock) ? kXcds : 1;
+    const int firstRays = gridDim.x / X * kBlock;   // per segment: one ray per lane of its group
+    int seg = (int)blockIdx.x % X, hops = 0;        // wave-uniform
+    int lo = segmentBound(total, seg, X), hi = segmentBound(total, seg + 1, X);
+    int* cursor = X > 1 ? a.xcdWork + seg * kXcdCursorStride : a.work;
+    bool exhausted = false;   // wave-uniform: every segment's cursor has passed its end
+    auto nextSegment = [&]() {
+        if (++hops >= X) { exhausted = true; return; }
+        seg = seg + 1 == X ? 0 : seg + 1;
+        lo = segmentBound(total, seg, X); hi = segmentBound(total, seg + 1, X);
+        cursor = a.xcdWork + seg * kXcdCursorStride;
+    };
     {
-        const int i = blockIdx.x * kBlock + tid;
-        if (i < total) startRay(a, i, closest, L);
+        const int i = lo + (int)blockIdx.x / X * kBlock + tid;
+        if (i < hi) startRay(a, i, closest, L);
     }
-    bool exhausted = firstRays >= total;   // wave-uniform: the cursor has passed the end
+    if (lo + firstRays >= hi) nextSegment();
     for (;;) {
         const unsigned long long idle = __ballot(L.index < 0);
         if (idle && !exhausted) {
             const int leader = __ffsll((long long)idle) - 1;
             int base = 0;
-            if (lane == leader) base = firstRays + atomicAdd(a.work, __popcll(idle));
+            if (lane == leader) base = lo + firstRays + atomicAdd(cursor, __popcll(idle));
             base = __shfl(base, leader, 64);
             if (L.index < 0) {
                 const int rank = (int)__builtin_amdgcn_mbcnt_hi((unsigned)(idle >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)idle, 0u));
                 const int i = base + rank;
-                if (i < total) startRay(a, i, closest, L);
+                if (i < hi) startRay(a, i, closest, L);
             }
-            exhausted = base + __popcll(idle) >= total;
+            if (base + __popcll(idle) >= hi) nextSegment();
         }
-        if (__ballot(L.index >= 0) == 0) break;
+        if (__ballot(L.index >= 0) == 0) {
+            if (exhausted) break;
+            continue;   // the segment ran dry before any lane got a ray: take the next one
+        }
         if (WW && L.index >= 0) {
             for (;;) {
                 bool searching = true;
@@ -558,13 +580,15 @@ hipError_t launchTrace(const TraceLaunch& l, bool stats, int blocks, const Trace
 }
 
 // ------------------------------------------------------------------ frame kernels
-__global__ void resetCountersKernel(int* c, int initialRays) {
+__global__ void resetCountersKernel(int* c, int initialRays, int* xcdWork) {
     for (int k = threadIdx.x; k < kCounterStride; k += blockDim.x) c[k] = 0;
+    if (xcdWork)
+        for (int k = threadIdx.x; k < kXcdSlotInts; k += blockDim.x) xcdWork[k] = 0;
     __syncthreads();
     if (threadIdx.x == 0) c[kCntRays] = initialRays;
 }
-hipError_t launchResetCounters(int* counters, int initialRays, hipStream_t s) {
-    hipLaunchKernelGGL(resetCountersKernel, dim3(1), dim3(kBlock), 0, s, counters, initialRays);
+hipError_t launchResetCounters(int* counters, int initialRays, int* xcdWork, hipStream_t s) {
+    hipLaunchKernelGGL(resetCountersKernel, dim3(1), dim3(kBlock), 0, s, counters, initialRays, xcdWork);
     return hipGetLastError();
 }
 

@@ -339,6 +339,9 @@ class Session:
     def set_option(self, key, value):
         return self.lib.RtSetOption(self.handle, key.encode(), int(value))
 
+    def get_option(self, key):
+        return self.lib.RtGetOption(self.handle, key.encode())
+
     def initialize(self, triangles, width, height):
         n = len(triangles)
         self.check(self.lib.RtInitialize(self.handle, triangle_pointer(triangles), n, width, height))

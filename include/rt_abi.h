@@ -130,7 +130,10 @@ int RtRayCounts(void* session, unsigned long long* closest, unsigned long long* 
  * statistics use one), "tail_bounce" (0 = off, default; k >= 1: from bounce k on, each path
  * runs to its end inside one launch instead of one launch per bounce; images unchanged),
  * "denoise_ensemble" (1, 2 or 4 network passes per RtDenoise: inputs mirrored in x, or in x, y
- * and both, outputs mirrored back and averaged; default 1),
+ * and both, outputs mirrored back and averaged; default 1), "xcd_segments" (0/1, default 1:
+ * a frame's traversal launches with at least two rays per lane hand out rays from eight
+ * contiguous segments, one per XCD group of workgroups, instead of one cursor; images
+ * unchanged),
  * "tiled_primary", "nontemporal", "stage_timing", "trace_stats" and "denoise_fallback" at the
  * next render or query.  Returns 1 if the key is known and the value valid.  RtGetOption
  * returns -1 for unknown keys; it also reports "node_bytes", the size of one wide node. */
@@ -212,7 +215,7 @@ int RtConv3x3Backward(const float* in, int n, int h, int w, int cin, const float
  * shared path queue, each frame replayed as a captured hipGraph ("use_graph", default 1).
  * Each view takes one progressive sample per frame (raysPerPixel is ignored) and equals a
  * session rendering its scene with raysPerPixel 1.  Options ("bvh_width", "bvh_quantized",
- * "blocks_per_cu", "short_stack", "while_while", "refill", "refill_below", "pipelines", "use_graph") must be set before
+ * "blocks_per_cu", "short_stack", "while_while", "refill", "refill_below", "pipelines", "xcd_segments", "use_graph") must be set before
  * RtBatchBuild.  RtBatchReadView
  * copies a view's RGBA8 image and/or its 10-float-per-pixel accumulation. */
 void* RtBatchCreate(void);

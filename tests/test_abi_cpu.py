@@ -64,6 +64,8 @@ def test_session_lifecycle_without_initialize(lib):
         assert lib.RtSetOption(s, b"bvh_width", 8) == 1 and lib.RtGetOption(s, b"bvh_width") == 8
         assert lib.RtSetOption(s, b"bvh_width", 3) == 0 and lib.RtSetOption(s, b"nonsense", 1) == 0
         assert lib.RtGetOption(s, b"nonsense") == -1
+        assert lib.RtGetOption(s, b"xcd_segments") == 1 and lib.RtSetOption(s, b"xcd_segments", 0) == 1
+        assert lib.RtGetOption(s, b"xcd_segments") == 0 and lib.RtSetOption(s, b"xcd_segments", 2) == 0
         # invalid arguments are rejected before any device work
         tris = np.zeros(1, dtype=native_api.TRIANGLE_DTYPE)
         assert lib.RtInitialize(s, native_api.triangle_pointer(tris), 0, 4, 4) == 0

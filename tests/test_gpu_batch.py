@@ -90,3 +90,24 @@ def test_batch_pipelines_match_one(lib, use_graph):
         for v in range(len(vs)):
             assert np.array_equal(out[pipes][0][v][0], out[1][0][v][0]), (pipes, v)
             assert np.array_equal(out[pipes][0][v][1].view(np.uint32), out[1][0][v][1].view(np.uint32)), (pipes, v)
+
+
+def test_batch_xcd_segments_match_one_cursor(lib):
+    """Batch launches that hand out rays from eight segment cursors (one workgroup per CU, so
+    the early bounces take the segmented path) equal the one-cursor frames bit for bit, with
+    the graph and two pipelines."""
+    vs = views()
+    t = scenes.displaced_icosphere(subdiv=4, seed=8)
+    vs.append((t, 512, 512, scenes.icosphere_camera(512, 512), scenes.make_settings(6, seed=77)))
+    out = {}
+    for seg in (0, 1):
+        with Batch(lib, pipelines=2, blocks_per_cu=1, xcd_segments=seg) as b:
+            for tv, w, h, cam, st in vs:
+                b.add_view(tv, w, h, cam, st)
+            b.build()
+            b.render(2)
+            out[seg] = ([b.read(v) for v in range(len(vs))], b.ray_counts())
+    assert out[0][1] == out[1][1]
+    for v in range(len(vs)):
+        assert np.array_equal(out[0][0][v][0], out[1][0][v][0]), v
+        assert np.array_equal(out[0][0][v][1].view(np.uint32), out[1][0][v][1].view(np.uint32)), v

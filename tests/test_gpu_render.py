@@ -358,6 +358,34 @@ def test_path_pipelines_match_one(lib, rpp):
         assert out[1][2] == out[pipes][2]
 
 
+@pytest.mark.parametrize("loop,pipes", [((1, 1), 1), ((0, 0), 2)])
+def test_xcd_segment_cursors_match_one_cursor(lib, loop, pipes):
+    """Traversal launches that hand out rays from eight segment cursors (one per XCD group of
+    workgroups, with stealing once a segment is drained) give the same frames bit for bit and
+    the same ray counts as one shared cursor, for both traversal loops, one and two pipelines
+    and the path-completion launch.  One workgroup per CU keeps the grid small enough that
+    the early bounces take the segmented path (two or more rays per lane) and the late ones
+    the shared cursor."""
+    from scenes import displaced_icosphere, icosphere_camera, make_settings
+    tris = displaced_icosphere(subdiv=5, seed=5)
+    w, h = 512, 256
+    cam, st = icosphere_camera(w, h), make_settings(8, rpp=4, seed=7)
+    out = {}
+    for seg in (0, 1):
+        with Session(lib) as s:
+            assert s.set_option("xcd_segments", seg) and s.get_option("xcd_segments") == seg
+            assert s.set_option("while_while", loop[0]) and s.set_option("refill", loop[1])
+            assert s.set_option("pipelines", pipes) and s.set_option("tail_bounce", 4)
+            assert s.set_option("blocks_per_cu", 1)
+            s.initialize(tris, w, h)
+            for _ in range(2):
+                img, _ = s.render(cam, st, 2)
+            out[seg] = (img, s.accumulation(), s.ray_counts())
+    assert np.array_equal(out[0][0], out[1][0])
+    assert np.array_equal(out[0][1].view(np.uint32), out[1][1].view(np.uint32))
+    assert out[0][2] == out[1][2]
+
+
 @pytest.mark.parametrize("scene", ["icosphere", "cornell_glossy", "soup"])
 def test_path_completion_matches_per_bounce_launches(lib, scene):
     """Paths completed in one launch from bounce k on (tail_bounce = k) equal the per-bounce
