@@ -418,8 +418,11 @@ __device__ __forceinline__ void startRay(const TraceArgs& a, int i, int closest,
     L.leaf = 0;
 }
 
-// First index of segment k of n over [0, total).
-__device__ __forceinline__ int segmentBound(int total, int k, int n) { return (int)((long long)total * k / n); }
+// First index of segment k over [0, total) cut into n = 1 or kXcds segments.
+__device__ __forceinline__ int segmentBound(int total, int k, int n) {
+    static_assert(kXcds == 8, "segment bounds shift by 3");
+    return n == 1 ? (k ? total : 0) : (int)(((long long)total * k) >> 3);
+}
 
 // Persistent traversal of a closest-hit queue followed by an any-hit queue, in one ray index
 // space, so the two overlap in one launch.  Each wave's first 64 rays are fixed by its
@@ -428,11 +431,16 @@ __device__ __forceinline__ int segmentBound(int total, int k, int n) { return (i
 // The cursor is shared by the whole grid, or (a.xcdWork) one per XCD segment.
 // With REFILL, lanes whose ray finished take new rays as soon as fewer than refillBelow
 // lanes are busy, instead of waiting for the wave's slowest ray.
-#ifndef RT_TRACE_MIN_WAVES
-#define RT_TRACE_MIN_WAVES 1
-#endif
+// Waves per SIMD the register allocation aims for.  The traversal is latency-bound, so its
+// throughput follows the resident waves: the 8-entry stack's 16 KB of LDS lets ten
+// workgroups share a CU, and the step loop fits 64 VGPRs (8 waves), the while-while loop 72
+// (7); the 16-entry stack's LDS allows five.  8-wide nodes and the instrumented variants keep
+// the compiler's own choice (a tighter budget would spill).
+template <class N, int S, bool WW, bool STATS> constexpr int traceWavesPerSimd() {
+    return (STATS || NodeTraits<N>::kWidth == 8) ? 1 : S == 16 ? 5 : WW ? 7 : 8;
+}
 template <class N, bool MIXED, bool STATS, int S, bool REFILL, bool WW>
-__global__ __launch_bounds__(kBlock, RT_TRACE_MIN_WAVES) void traceKernel(TraceArgs a) {
+__global__ __launch_bounds__(kBlock, (traceWavesPerSimd<N, S, WW, STATS>())) void traceKernel(TraceArgs a) {
     __shared__ int sRef[S * kBlock];
     __shared__ float sDist[S * kBlock];
     const int tid = threadIdx.x, lane = tid & 63;
@@ -474,7 +482,7 @@ __global__ __launch_bounds__(kBlock, RT_TRACE_MIN_WAVES) void traceKernel(TraceA
             const int leader = __ffsll((long long)idle) - 1;
             int base = 0;
             if (lane == leader) base = lo + firstRays + atomicAdd(cursor, __popcll(idle));
-            base = __shfl(base, leader, 64);
+            base = __builtin_amdgcn_readlane(base, leader);   // uniform: the segment state stays scalar
             if (L.index < 0) {
                 const int rank = (int)__builtin_amdgcn_mbcnt_hi((unsigned)(idle >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)idle, 0u));
                 const int i = base + rank;
