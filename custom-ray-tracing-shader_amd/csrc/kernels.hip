@@ -436,8 +436,14 @@ __device__ __forceinline__ int segmentBound(int total, int k, int n) {
 // workgroups share a CU, and the step loop fits 64 VGPRs (8 waves), the while-while loop 72
 // (7); the 16-entry stack's LDS allows five.  8-wide nodes and the instrumented variants keep
 // the compiler's own choice (a tighter budget would spill).
+#ifndef RT_STEP_WAVES
+#define RT_STEP_WAVES 8
+#endif
+#ifndef RT_WW_WAVES
+#define RT_WW_WAVES 7
+#endif
 template <class N, int S, bool WW, bool STATS> constexpr int traceWavesPerSimd() {
-    return (STATS || NodeTraits<N>::kWidth == 8) ? 1 : S == 16 ? 5 : WW ? 7 : 8;
+    return (STATS || NodeTraits<N>::kWidth == 8) ? 1 : S == 16 ? 5 : WW ? RT_WW_WAVES : RT_STEP_WAVES;
 }
 template <class N, bool MIXED, bool STATS, int S, bool REFILL, bool WW>
 __global__ __launch_bounds__(kBlock, (traceWavesPerSimd<N, S, WW, STATS>())) void traceKernel(TraceArgs a) {
@@ -699,7 +705,10 @@ __device__ __forceinline__ void appendRays2(AppendLds& sh, int round, bool p0, i
 // Grid-stride over the bounce's queue, so the grid is sized to the GPU, not to the largest
 // possible queue; every lane of a wave reaches the appends in each round.  Capped at 96
 // VGPRs so five workgroups fit on a compute unit without spilling.
-constexpr int kShadeBlocksPerCU = 5;
+#ifndef RT_SHADE_WAVES
+#define RT_SHADE_WAVES 5
+#endif
+constexpr int kShadeBlocksPerCU = RT_SHADE_WAVES;
 __global__ __launch_bounds__(kBlock, kShadeBlocksPerCU) void shadeKernel(FrameArgs f) {
     __shared__ unsigned sShift[8], sTerm[kMaxRaysPerPixel * 8];
     __shared__ AppendLds sApp;
@@ -775,7 +784,10 @@ __device__ __forceinline__ void beginRay(LaneRay& L, f3 o, f3 d, float limit, bo
 
 // With the 8-entry stack, capped at 96 VGPRs (five workgroups per CU; a few spilled values)
 // instead of 109 (four): +2% on the icosphere.  The 16-entry stack's LDS allows four.
-template <int S> constexpr int tailBlocksPerCU() { return S == 8 ? 5 : 4; }
+#ifndef RT_TAIL_WAVES
+#define RT_TAIL_WAVES 5
+#endif
+template <int S> constexpr int tailBlocksPerCU() { return S == 8 ? RT_TAIL_WAVES : 4; }
 template <class N, bool STATS, int S>
 __global__ __launch_bounds__(kBlock, tailBlocksPerCU<S>()) void pathTailKernel(FrameArgs f, TraceArgs a, int firstBounce) {
     __shared__ int sRef[S * kBlock];
