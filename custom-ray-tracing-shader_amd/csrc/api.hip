@@ -362,6 +362,7 @@ bool renderGpu(Session* s, const CameraData& cam, const RenderSettings& rs, floa
                     ta.rays = ((b & 1) ? s->dRayB : s->dRayA) + off;
                     ta.countPtr = ctr[p] + kCntRays + b;
                     ta.work = ctr[p] + kWorkRays + b;
+                    ta.xcdWork = xcdWork(s, p, b);
                     ta.stats = s->dStats;
                     StageScope sc(s, RT_STAGE_TAIL, pst[p]);
                     if ((s->error = launchPathTail(traceLaunch(s), s->traceStats, s->traceBlocks, s->cus, f[p], ta, b, pst[p])) !=

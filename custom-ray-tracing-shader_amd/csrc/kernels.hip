@@ -424,6 +424,56 @@ __device__ __forceinline__ int segmentBound(int total, int k, int n) {
     return n == 1 ? (k ? total : 0) : (int)(((long long)total * k) >> 3);
 }
 
+// Hands out a persistent launch's ray indices [0, total).  Each lane's first index is fixed
+// by its grid position; after that a wave takes indices from a cursor (counting past the
+// first batch), so the launch does not open with one atomic per wave on one address.
+// With segment cursors (xcdWork), group g = blockIdx.x % 8 (the workgroups of one XCD)
+// starts in segment g of the index space, so consecutive rays (neighbouring pixels, nearby
+// paths) share that XCD's L2, and moves on to the other segments once its own is drained.
+// A launch with fewer rays than lanes keeps one cursor: its waves would spend more on the
+// atomics that find drained segments than they gain.  All state is wave-uniform.
+#ifndef RT_SEGMENT_RAYS_PER_LANE
+#define RT_SEGMENT_RAYS_PER_LANE 1
+#endif
+struct RayDealer {
+    int X, firstRays, total, seg, hops, lo, hi;
+    int *cursor, *segCursors;
+    bool exhausted;   // every segment's cursor has passed its end
+    __device__ __forceinline__ void init(int* xcdWork, int* work, int n) {
+        total = n;
+        X = (xcdWork && gridDim.x % kXcds == 0 && total >= RT_SEGMENT_RAYS_PER_LANE * (int)gridDim.x * kBlock) ? kXcds : 1;
+        firstRays = (int)gridDim.x / X * kBlock;   // per segment: one ray per lane of its group
+        seg = (int)blockIdx.x % X; hops = 0;
+        segCursors = xcdWork;
+        lo = segmentBound(total, seg, X); hi = segmentBound(total, seg + 1, X);
+        cursor = X > 1 ? segCursors + seg * kXcdCursorStride : work;
+        exhausted = false;
+    }
+    // This lane's first index, or -1.  Call once, before take().
+    __device__ __forceinline__ int firstIndex(int tid) {
+        const int i = lo + (int)blockIdx.x / X * kBlock + tid, end = hi;
+        if (lo + firstRays >= hi) next();
+        return i < end ? i : -1;
+    }
+    __device__ __forceinline__ void next() {
+        if (++hops >= X) { exhausted = true; return; }
+        seg = seg + 1 == X ? 0 : seg + 1;
+        lo = segmentBound(total, seg, X); hi = segmentBound(total, seg + 1, X);
+        cursor = segCursors + seg * kXcdCursorStride;
+    }
+    // First index of the idle lanes' batch from the current segment (indices >= hi are none).
+    __device__ __forceinline__ int take(unsigned long long idle, int lane) {
+        const int leader = __ffsll((long long)idle) - 1;
+        int base = 0;
+        if (lane == leader) base = lo + firstRays + atomicAdd(cursor, __popcll(idle));
+        return __builtin_amdgcn_readlane(base, leader);
+    }
+    // After a take(): moves on when the batch reached the segment's end.
+    __device__ __forceinline__ void after(int base, unsigned long long idle) {
+        if (base + __popcll(idle) >= hi) next();
+    }
+};
+
 // Persistent traversal of a closest-hit queue followed by an any-hit queue, in one ray index
 // space, so the two overlap in one launch.  Each wave's first 64 rays are fixed by its
 // position in the grid; after that it takes rays from a cursor (which counts the rays past
@@ -460,44 +510,24 @@ __global__ __launch_bounds__(kBlock, (traceWavesPerSimd<N, S, WW, STATS>())) voi
     bool overflow = false;
     LaneRay L;
     L.index = -1;
-    // With segment cursors, group g = blockIdx.x % X (the workgroups of one XCD) starts in
-    // segment g of the index space, so consecutive rays (neighbouring pixels, nearby paths)
-    // share that XCD's L2, and moves on to the other segments when its own is drained.  A
-    // launch with fewer than two rays per lane keeps one cursor: its waves would spend more on
-    // the atomics that find drained segments than they gain.
-    const int X = (a.xcdWork && gridDim.x % kXcds == 0 && total >= 2 * (int)gridDim.x * kBlock) ? kXcds : 1;
-    const int firstRays = gridDim.x / X * kBlock;   // per segment: one ray per lane of its group
-    int seg = (int)blockIdx.x % X, hops = 0;        // wave-uniform
-    int lo = segmentBound(total, seg, X), hi = segmentBound(total, seg + 1, X);
-    int* cursor = X > 1 ? a.xcdWork + seg * kXcdCursorStride : a.work;
-    bool exhausted = false;   // wave-uniform: every segment's cursor has passed its end
-    auto nextSegment = [&]() {
-        if (++hops >= X) { exhausted = true; return; }
-        seg = seg + 1 == X ? 0 : seg + 1;
-        lo = segmentBound(total, seg, X); hi = segmentBound(total, seg + 1, X);
-        cursor = a.xcdWork + seg * kXcdCursorStride;
-    };
+    RayDealer deal;
+    deal.init(a.xcdWork, a.work, total);
     {
-        const int i = lo + (int)blockIdx.x / X * kBlock + tid;
-        if (i < hi) startRay(a, i, closest, L);
+        const int i = deal.firstIndex(tid);
+        if (i >= 0) startRay(a, i, closest, L);
     }
-    if (lo + firstRays >= hi) nextSegment();
     for (;;) {
         const unsigned long long idle = __ballot(L.index < 0);
-        if (idle && !exhausted) {
-            const int leader = __ffsll((long long)idle) - 1;
-            int base = 0;
-            if (lane == leader) base = lo + firstRays + atomicAdd(cursor, __popcll(idle));
-            base = __builtin_amdgcn_readlane(base, leader);   // uniform: the segment state stays scalar
+        if (idle && !deal.exhausted) {
+            const int base = deal.take(idle, lane);
             if (L.index < 0) {
-                const int rank = (int)__builtin_amdgcn_mbcnt_hi((unsigned)(idle >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)idle, 0u));
-                const int i = base + rank;
-                if (i < hi) startRay(a, i, closest, L);
+                const int i = base + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(idle >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)idle, 0u));
+                if (i < deal.hi) startRay(a, i, closest, L);
             }
-            if (base + __popcll(idle) >= hi) nextSegment();
+            deal.after(base, idle);
         }
         if (__ballot(L.index >= 0) == 0) {
-            if (exhausted) break;
+            if (deal.exhausted) break;
             continue;   // the segment ran dry before any lane got a ray: take the next one
         }
         if (WW && L.index >= 0) {
@@ -520,7 +550,7 @@ __global__ __launch_bounds__(kBlock, (traceWavesPerSimd<N, S, WW, STATS>())) voi
                     L.index = -1;
                     break;
                 }
-                if (REFILL && !exhausted && __popcll(__ballot(true)) < a.refillBelow) break;
+                if (REFILL && !deal.exhausted && __popcll(__ballot(true)) < a.refillBelow) break;
             }
         }
         if (!WW && L.index >= 0) {
@@ -530,7 +560,7 @@ __global__ __launch_bounds__(kBlock, (traceWavesPerSimd<N, S, WW, STATS>())) voi
                     L.index = -1;
                     break;
                 }
-                if (REFILL && !exhausted && __popcll(__ballot(true)) < a.refillBelow) break;
+                if (REFILL && !deal.exhausted && __popcll(__ballot(true)) < a.refillBelow) break;
             }
         }
     }
@@ -807,7 +837,6 @@ __global__ __launch_bounds__(kBlock, tailBlocksPerCU<S>()) void pathTailKernel(F
     unsigned pixel = 0;
     float radW = 0;
     bool active = false;
-    const int firstRays = gridDim.x * kBlock;
     auto startPath = [&](int i) {
         const float4 r0 = a.rays[2 * i], r1 = a.rays[2 * i + 1];
         slot = __float_as_int(r1.w);
@@ -820,25 +849,26 @@ __global__ __launch_bounds__(kBlock, tailBlocksPerCU<S>()) void pathTailKernel(F
         beginRay(L, F3(r0.x, r0.y, r0.z), F3(r1.x, r1.y, r1.z), r0.w, false);
         active = true;
     };
+    RayDealer deal;
+    deal.init(a.xcdWork, a.work, total);
     {
-        const int i = blockIdx.x * kBlock + tid;
-        if (i < total) startPath(i);
+        const int i = deal.firstIndex(tid);
+        if (i >= 0) startPath(i);
     }
-    bool exhausted = firstRays >= total;
     for (;;) {
         const unsigned long long idle = __ballot(!active);
-        if (idle && !exhausted) {
-            const int leader = __ffsll((long long)idle) - 1;
-            int base = 0;
-            if (lane == leader) base = firstRays + atomicAdd(a.work, __popcll(idle));
-            base = __shfl(base, leader, 64);
+        if (idle && !deal.exhausted) {
+            const int base = deal.take(idle, lane);
             if (!active) {
-                const int rank = (int)__builtin_amdgcn_mbcnt_hi((unsigned)(idle >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)idle, 0u));
-                if (base + rank < total) startPath(base + rank);
+                const int i = base + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(idle >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)idle, 0u));
+                if (i < deal.hi) startPath(i);
             }
-            exhausted = base + __popcll(idle) >= total;
+            deal.after(base, idle);
         }
-        if (__ballot(active) == 0) break;
+        if (__ballot(active) == 0) {
+            if (deal.exhausted) break;
+            continue;
+        }
         // closest hit of this bounce
         if (active) while (!traceStep<N, true, STATS, S>(L, nodes, recs, st, overflow, tc)) {}
         ShadeResult res;
