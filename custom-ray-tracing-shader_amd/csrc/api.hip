@@ -43,7 +43,7 @@ struct Session {
     int bvhQuantized = 0;        // option: quantized (QNode) wide nodes at the next RtInitialize
     int builtQuantized = 0;
     int blocksPerCU = 3;         // persistent traversal workgroups per CU (per pipeline)
-    int shortStack = 8;          // LDS stack entries per traversal lane (8 or 16)
+    int shortStack = 8;          // LDS stack entries per traversal lane (8, 12 or 16)
     int refill = 1;              // per-lane ray refill in the traversal kernel
     int refillBelow = 40;        // busy lanes below which a refilling wave fetches rays
     int whileWhile = 1;          // speculative while-while traversal loop
@@ -1105,7 +1105,7 @@ int RtSetOption(void* p, const char* key, int value) {
         s->denoiseFallback = value; return 1;
     }
     if (!std::strcmp(key, "short_stack")) {
-        if (value != 8 && value != 16) return 0;
+        if (value != 8 && value != 12 && value != 16) return 0;   // 12: 4-wide nodes only (else the render fails)
         s->shortStack = value; return 1;
     }
     if (!std::strcmp(key, "refill")) {

@@ -383,7 +383,7 @@ int RtBatchSetOption(void* p, const char* key, int value) {
     if (!std::strcmp(key, "bvh_width") && (value == 2 || value == 4 || value == 8)) { b->width = value; return 1; }
     if (!std::strcmp(key, "bvh_quantized") && (value == 0 || value == 1)) { b->quantized = value; return 1; }
     if (!std::strcmp(key, "blocks_per_cu") && value >= 1 && value <= 32) { b->blocksPerCU = value; return 1; }
-    if (!std::strcmp(key, "short_stack") && (value == 8 || value == 16)) { b->shortStack = value; return 1; }
+    if (!std::strcmp(key, "short_stack") && (value == 8 || value == 12 || value == 16)) { b->shortStack = value; return 1; }
     if (!std::strcmp(key, "refill") && (value == 0 || value == 1)) { b->refill = value; return 1; }
     if (!std::strcmp(key, "refill_below") && value >= 1 && value <= 64) { b->refillBelow = value; return 1; }
     if (!std::strcmp(key, "while_while") && (value == 0 || value == 1)) { b->whileWhile = value; return 1; }

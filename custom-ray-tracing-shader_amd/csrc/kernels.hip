@@ -11,6 +11,7 @@
 // Queue sizes live in device memory, so no host synchronisation happens inside a frame.
 #include "kernels.h"
 #include "traverse.h"
+#include <type_traits>
 
 namespace rt {
 
@@ -66,8 +67,12 @@ struct LdsStack {
     float* dist;
     int2* spill;
     int spillEntries, lanes, gl, tid;
+    // Ring slot of stack position p (S = 8, 12 or 16 entries).
+    static __device__ __forceinline__ int ringSlot(int p) {
+        return (S & (S - 1)) == 0 ? (p & (S - 1)) : (int)((unsigned)p % (unsigned)S);
+    }
     __device__ __forceinline__ void push(int& sp, int r, float dd, bool& overflow) {
-        const int slot = sp & (S - 1);
+        const int slot = ringSlot(sp);
         if (sp >= S) {
             const int e = sp - S;
             if (e < spillEntries)
@@ -87,7 +92,7 @@ struct LdsStack {
         for (int k = 0; k < 3; k++) {
             const int p = sp + k;
             if (k < n && p >= S) {
-                const int e = p - S, slot = p & (S - 1);
+                const int e = p - S, slot = ringSlot(p);
                 if (e < spillEntries)
                     spill[(size_t)e * lanes + gl] = make_int2(ref[slot * kBlock + tid], __float_as_int(dist[slot * kBlock + tid]));
                 else
@@ -96,7 +101,7 @@ struct LdsStack {
         }
     }
     __device__ __forceinline__ void put(int p, int r, float dd) {
-        const int slot = p & (S - 1);
+        const int slot = ringSlot(p);
         ref[slot * kBlock + tid] = r;
         dist[slot * kBlock + tid] = dd;
     }
@@ -131,7 +136,7 @@ struct LdsStack {
     }
     __device__ __forceinline__ void pop(int& sp, int& r, float& dd) {
         sp--;
-        const int slot = sp & (S - 1);
+        const int slot = ringSlot(sp);
         r = ref[slot * kBlock + tid];
         dd = dist[slot * kBlock + tid];
         if (sp >= S) {
@@ -493,7 +498,7 @@ struct RayDealer {
 #define RT_WW_WAVES 7
 #endif
 template <class N, int S, bool WW, bool STATS> constexpr int traceWavesPerSimd() {
-    return (STATS || NodeTraits<N>::kWidth == 8) ? 1 : S == 16 ? 5 : WW ? RT_WW_WAVES : RT_STEP_WAVES;
+    return (STATS || NodeTraits<N>::kWidth == 8) ? 1 : S == 16 ? 5 : S == 12 ? 6 : WW ? RT_WW_WAVES : RT_STEP_WAVES;
 }
 template <class N, bool MIXED, bool STATS, int S, bool REFILL, bool WW>
 __global__ __launch_bounds__(kBlock, (traceWavesPerSimd<N, S, WW, STATS>())) void traceKernel(TraceArgs a) {
@@ -587,7 +592,9 @@ static void launchTraceNS(bool stats, int blocks, const TraceArgs& a, hipStream_
 }
 template <class N>
 static hipError_t launchTraceN(bool stats, int blocks, const TraceLaunch& l, const TraceArgs& a, hipStream_t s) {
-    const int key = (l.stack == 16 ? 4 : l.stack == 8 ? 0 : -100) + (l.refill ? 2 : 0) + (l.whileWhile ? 1 : 0);
+    const int key = (l.stack == 16 ? 4 : l.stack == 8 ? 0 : l.stack == 12 ? 8 : -100) + (l.refill ? 2 : 0) +
+                    (l.whileWhile ? 1 : 0);
+    if (key >= 8 && !std::is_same<N, WideNode<4>>::value) return hipErrorInvalidValue;   // 12 entries: 4-wide nodes only
     switch (key) {
         case 0: launchTraceNS<N, 8, false, false>(stats, blocks, a, s); break;
         case 1: launchTraceNS<N, 8, false, true>(stats, blocks, a, s); break;
@@ -597,6 +604,10 @@ static hipError_t launchTraceN(bool stats, int blocks, const TraceLaunch& l, con
         case 5: launchTraceNS<N, 16, false, true>(stats, blocks, a, s); break;
         case 6: launchTraceNS<N, 16, true, false>(stats, blocks, a, s); break;
         case 7: launchTraceNS<N, 16, true, true>(stats, blocks, a, s); break;
+        case 8: launchTraceNS<WideNode<4>, 12, false, false>(stats, blocks, a, s); break;
+        case 9: launchTraceNS<WideNode<4>, 12, false, true>(stats, blocks, a, s); break;
+        case 10: launchTraceNS<WideNode<4>, 12, true, false>(stats, blocks, a, s); break;
+        case 11: launchTraceNS<WideNode<4>, 12, true, true>(stats, blocks, a, s); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -946,6 +957,7 @@ static hipError_t launchTailN(bool stats, int blocks, int cus, const TraceLaunch
                               const TraceArgs& a, int firstBounce, hipStream_t s) {
     if (l.stack == 16) return launchTailNS<N, 16>(stats, blocks, cus, f, a, firstBounce, s);
     if (l.stack == 8) return launchTailNS<N, 8>(stats, blocks, cus, f, a, firstBounce, s);
+    if (l.stack == 12) return launchTailNS<N, 16>(stats, blocks, cus, f, a, firstBounce, s);   // no 12-entry variant
     return hipErrorInvalidValue;
 }
 hipError_t launchPathTail(const TraceLaunch& l, bool stats, int blocks, int cus, const FrameArgs& f, const TraceArgs& a,

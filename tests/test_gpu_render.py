@@ -320,11 +320,13 @@ def test_frames_identical_across_traversal_loops(lib):
     tris = displaced_icosphere(subdiv=5, seed=11)
     cam, st = icosphere_camera(96, 64), make_settings(8, rpp=2, seed=9)
     out = []
-    for ww, refill, below in ((0, 0, 40), (1, 1, 40), (1, 0, 40), (0, 1, 8), (1, 1, 60)):
+    for ww, refill, below, stack in ((0, 0, 40, 8), (1, 1, 40, 8), (1, 0, 40, 8), (0, 1, 8, 8), (1, 1, 60, 8),
+                                     (1, 1, 40, 12), (0, 0, 40, 16)):
         with Session(lib) as s:
             s.set_option("while_while", ww)
             s.set_option("refill", refill)
             s.set_option("refill_below", below)
+            assert s.set_option("short_stack", stack)
             s.set_option("blocks_per_cu", 1)   # few lanes: every lane refills many times
             s.initialize(tris, 96, 64)
             for _ in range(2):

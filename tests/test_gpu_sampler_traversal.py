@@ -93,11 +93,11 @@ def test_soup_queries_vs_oracle_and_fp64(lib, width, quantized):
         assert diff.mean() < 1e-3
 
 
-@pytest.mark.parametrize("stack", [8, 16])
+@pytest.mark.parametrize("stack", [8, 12, 16])
 @pytest.mark.parametrize("while_while,refill", [(0, 0), (0, 1), (1, 0), (1, 1)])
 def test_traversal_loop_variants_match_brute_force(lib, vbvh, while_while, refill, stack):
     """Every persistent-loop variant (step loop or speculative while-while, with or without
-    per-lane refill, 8- or 16-entry LDS stack) returns the brute-force hit records."""
+    per-lane refill, 8-, 12- or 16-entry LDS stack ring) returns the brute-force hit records."""
     from scenes import triangle_soup
     rays, scenes = vbvh
     cases = [(scenes["overlap"], rays), (scenes["unbalanced"], rays), (triangle_soup(5000, seed=3), soup_rays(6000, 2))]
