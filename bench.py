@@ -39,11 +39,11 @@ HBM_PEAK_GBS = 8000.0
 TUNED = {
     "soup": {"bvh_width": 4, "build_max_leaf": 2, "blocks_per_cu": 2, "short_stack": 16, "while_while": 1, "refill": 1,
              "pipelines": 3},
-    "icosphere": {"bvh_width": 4, "build_max_leaf": 2, "blocks_per_cu": 3, "short_stack": 8, "while_while": 1,
+    "icosphere": {"bvh_width": 4, "build_max_leaf": 2, "blocks_per_cu": 3, "short_stack": 12, "while_while": 1,
                   "refill": 1, "pipelines": 4, "tail_bounce": 4},
-    "cornell": {"bvh_width": 4, "build_max_leaf": 4, "blocks_per_cu": 8, "short_stack": 8, "while_while": 1, "refill": 0,
+    "cornell": {"bvh_width": 4, "build_max_leaf": 4, "blocks_per_cu": 8, "short_stack": 12, "while_while": 1, "refill": 0,
                 "pipelines": 4},
-    "batch": {"bvh_width": 4, "blocks_per_cu": 3, "short_stack": 8, "while_while": 1, "refill": 1, "pipelines": 2},
+    "batch": {"bvh_width": 4, "blocks_per_cu": 3, "short_stack": 12, "while_while": 1, "refill": 1, "pipelines": 2},
 }
 
 

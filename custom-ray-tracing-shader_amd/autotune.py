@@ -21,7 +21,7 @@ from native_api import Session
 WIDTHS = (4, 8)
 MAX_LEAF = (4,)
 BLOCKS = (2, 3, 4, 6, 8)
-STACKS = (8, 16)
+STACKS = (8, 12, 16)
 LOOPS = ((1, 1), (1, 0), (0, 0))   # (while_while, refill)
 PIPELINES = (1, 2, 3, 4)
 TAILS = (0, 3, 4, 5)   # path completion from this bounce (0: off), where below the bounce limit

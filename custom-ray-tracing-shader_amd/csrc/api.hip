@@ -43,7 +43,7 @@ struct Session {
     int bvhQuantized = 0;        // option: quantized (QNode) wide nodes at the next RtInitialize
     int builtQuantized = 0;
     int blocksPerCU = 3;         // persistent traversal workgroups per CU (per pipeline)
-    int shortStack = 8;          // LDS stack entries per traversal lane (8, 12 or 16)
+    int shortStack = 12;         // LDS stack entries per traversal lane (8, 12 or 16)
     int refill = 1;              // per-lane ray refill in the traversal kernel
     int refillBelow = 40;        // busy lanes below which a refilling wave fetches rays
     int whileWhile = 1;          // speculative while-while traversal loop
@@ -193,6 +193,10 @@ bool buildAccel(Session* s) {
     }
     s->dBinNodes = r.binNodes; s->dOrder = r.order; s->dRecs = r.recs; s->dNodes = r.wideNodes;
     s->binNodeCount = r.binNodeCount; s->wideNodeCount = r.wideCount; s->bvhDepth = r.depth;
+    if (s->bvhWidth == 4 && !s->bvhQuantized && r.wideCount >= kMaxWide4Nodes) {
+        s->hostError = "scene needs more than 2^25 4-wide nodes (4 GB); use bvh_width 8 or bvh_quantized";
+        return false;
+    }
     s->builtWidth = s->bvhWidth;
     s->builtQuantized = s->bvhQuantized;
     s->buildMs = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -1105,7 +1109,7 @@ int RtSetOption(void* p, const char* key, int value) {
         s->denoiseFallback = value; return 1;
     }
     if (!std::strcmp(key, "short_stack")) {
-        if (value != 8 && value != 12 && value != 16) return 0;   // 12: 4-wide nodes only (else the render fails)
+        if (value != 8 && value != 12 && value != 16) return 0;
         s->shortStack = value; return 1;
     }
     if (!std::strcmp(key, "refill")) {

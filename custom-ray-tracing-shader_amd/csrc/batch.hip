@@ -53,7 +53,7 @@ struct PendingView {
 struct Batch {
     std::vector<PendingView> pending;
     std::string error;
-    int width = 4, quantized = 0, blocksPerCU = 3, shortStack = 8, refill = 1, refillBelow = 40, whileWhile = 1,
+    int width = 4, quantized = 0, blocksPerCU = 3, shortStack = 12, refill = 1, refillBelow = 40, whileWhile = 1,
         useGraph = 1, pipelines = 2, xcdSegments = 1;
     bool built = false;
     int cus = 0, views = 0, totalSlots = 0, maxBounces = 0;
@@ -255,6 +255,10 @@ bool build(Batch* b) {
     for (int v = 0; v < b->views; v++) {
         B_TRY(buildBvhGpu(b->dTris + triOff[v], (int)b->pending[v].tris.size(), prm, res[v], b->stream));
         nodeTotal += res[v].wideCount;
+    }
+    if (b->width == 4 && !b->quantized && nodeTotal >= kMaxWide4Nodes) {
+        b->error = "batch needs more than 2^25 4-wide nodes (4 GB); use bvh_width 8 or bvh_quantized";
+        return false;
     }
     const size_t nodeBytes = wideNodeBytes(b->width, b->quantized);
     unsigned char* nodes = nullptr;

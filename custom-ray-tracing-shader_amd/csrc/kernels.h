@@ -14,6 +14,8 @@ constexpr int kCntRays = 0, kCntShadow = kMaxBounces + 1, kWorkRays = 2 * (kMaxB
 // one per group of workgroups that share an XCD (workgroups are dispatched to the eight XCDs
 // round-robin, so blockIdx.x % kXcds picks the group), each with its own cursor, kept
 // kXcdCursorStride ints apart.  A pipeline holds one set per work cursor slot above.
+// 4-wide full-precision nodes are addressed by 32-bit byte offsets in the traversal.
+constexpr long long kMaxWide4Nodes = 1LL << 25;
 constexpr int kXcds = 8, kXcdCursorStride = 32, kXcdCursorInts = kXcds * kXcdCursorStride;
 constexpr int kXcdSlotInts = (kMaxBounces + 1) * kXcdCursorInts;   // per pipeline
 

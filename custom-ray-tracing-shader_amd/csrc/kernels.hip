@@ -166,6 +166,52 @@ __device__ __forceinline__ void childEntries(const N* nodes, const LaneRay& L, f
 #pragma unroll
     for (int c = 0; c < W; c++) cr[c] = n.child[c];
 }
+// 4-wide nodes, rays with no zero inverse component: the entry and exit planes of each axis
+// are loaded directly (offsets chosen by the direction's signs, the far plane 16 bytes away
+// from the near one), so each slab distance needs no min/max pair.  fma(x, inv, -oi) is
+// monotonic in x, so the plane chosen by the sign gives exactly min / max of the two slab
+// distances that boxEntryFma takes: the same entries, bit for bit.
+template <>
+__device__ __forceinline__ void childEntries<WideNode<4>>(const WideNode<4>* nodes, const LaneRay& L, float* dist,
+                                                         int* cr) {
+    // A 4-wide node is six float4 planes, lox hix loy hiy loz hiz (16 B each), then the child
+    // references.  A ray enters along an axis through the low plane when its inverse
+    // direction's sign bit is clear, else through the high plane: byte offset 16 * sign.
+    const unsigned sx = (__float_as_uint(L.inv.x) >> 27) & 16u, sy = (__float_as_uint(L.inv.y) >> 27) & 16u,
+                   sz = (__float_as_uint(L.inv.z) >> 27) & 16u;
+    // 32-bit byte offsets from the node array (builds refuse more than kMaxWide4Nodes nodes).
+    const char* arr = reinterpret_cast<const char*>(nodes);
+    const unsigned node = (unsigned)L.ref << 7;
+    const float4 nx = *reinterpret_cast<const float4*>(arr + (node | sx));
+    const float4 fx = *reinterpret_cast<const float4*>(arr + (node | (sx ^ 16u)));
+    const float4 ny = *reinterpret_cast<const float4*>(arr + (node | (32u + sy)));
+    const float4 fy = *reinterpret_cast<const float4*>(arr + (node | (32u + (sy ^ 16u))));
+    const float4 nz = *reinterpret_cast<const float4*>(arr + (node | (64u + sz)));
+    const float4 fz = *reinterpret_cast<const float4*>(arr + (node | (64u + (sz ^ 16u))));
+    const int4 ch = *reinterpret_cast<const int4*>(arr + (node | 96u));
+    const float nxa[4] = {nx.x, nx.y, nx.z, nx.w}, fxa[4] = {fx.x, fx.y, fx.z, fx.w};
+    const float nya[4] = {ny.x, ny.y, ny.z, ny.w}, fya[4] = {fy.x, fy.y, fy.z, fy.w};
+    const float nza[4] = {nz.x, nz.y, nz.z, nz.w}, fza[4] = {fz.x, fz.y, fz.z, fz.w};
+    if (L.nz) {
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            const float tnx = fmaf(nxa[c], L.inv.x, -L.oi.x), tfx = fmaf(fxa[c], L.inv.x, -L.oi.x);
+            const float tny = fmaf(nya[c], L.inv.y, -L.oi.y), tfy = fmaf(fya[c], L.inv.y, -L.oi.y);
+            const float tnz = fmaf(nza[c], L.inv.z, -L.oi.z), tfz = fmaf(fza[c], L.inv.z, -L.oi.z);
+            const float nearT = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, 0.0f));
+            const float farT = fminf(fminf(tfx, tfy), fminf(tfz, L.best));
+            dist[c] = nearT <= farT ? nearT : INFINITY;
+        }
+    } else {
+        // A zero inverse component: the full slab test on the planes put back in order.
+#pragma unroll
+        for (int c = 0; c < 4; c++)
+            dist[c] = boxEntry(L.o, L.inv, sx ? fxa[c] : nxa[c], sy ? fya[c] : nya[c], sz ? fza[c] : nza[c],
+                               sx ? nxa[c] : fxa[c], sy ? nya[c] : fya[c], sz ? nza[c] : fza[c], L.best);
+    }
+    cr[0] = ch.x; cr[1] = ch.y; cr[2] = ch.z; cr[3] = ch.w;
+}
+
 // The 256-byte node in two halves of four children: the second half's loads are issued
 // after the first half's slab tests (they hit in L1: both halves span the same two cache
 // lines), so only half the node's registers are live at once.
@@ -488,11 +534,12 @@ struct RayDealer {
 // lanes are busy, instead of waiting for the wave's slowest ray.
 // Waves per SIMD the register allocation aims for.  The traversal is latency-bound, so its
 // throughput follows the resident waves: the 8-entry stack's 16 KB of LDS lets ten
-// workgroups share a CU, and the step loop fits 64 VGPRs (8 waves), the while-while loop 72
-// (7); the 16-entry stack's LDS allows five.  8-wide nodes and the instrumented variants keep
-// the compiler's own choice (a tighter budget would spill).
+// workgroups share a CU, and both loops fit 72 VGPRs (7 waves) without spilling (64 VGPRs, 8
+// waves, spills ~25 values of the step loop and measured 8% slower); the 12- and 16-entry
+// stacks' LDS allows six and five.  8-wide nodes and the instrumented variants keep the
+// compiler's own choice (a tighter budget would spill).
 #ifndef RT_STEP_WAVES
-#define RT_STEP_WAVES 8
+#define RT_STEP_WAVES 7
 #endif
 #ifndef RT_WW_WAVES
 #define RT_WW_WAVES 7
@@ -592,9 +639,9 @@ static void launchTraceNS(bool stats, int blocks, const TraceArgs& a, hipStream_
 }
 template <class N>
 static hipError_t launchTraceN(bool stats, int blocks, const TraceLaunch& l, const TraceArgs& a, hipStream_t s) {
-    const int key = (l.stack == 16 ? 4 : l.stack == 8 ? 0 : l.stack == 12 ? 8 : -100) + (l.refill ? 2 : 0) +
-                    (l.whileWhile ? 1 : 0);
-    if (key >= 8 && !std::is_same<N, WideNode<4>>::value) return hipErrorInvalidValue;   // 12 entries: 4-wide nodes only
+    // The 12-entry ring exists for 4-wide nodes; other layouts take 16 entries for it.
+    const int stack = (l.stack == 12 && !std::is_same<N, WideNode<4>>::value) ? 16 : l.stack;
+    const int key = (stack == 16 ? 4 : stack == 8 ? 0 : stack == 12 ? 8 : -100) + (l.refill ? 2 : 0) + (l.whileWhile ? 1 : 0);
     switch (key) {
         case 0: launchTraceNS<N, 8, false, false>(stats, blocks, a, s); break;
         case 1: launchTraceNS<N, 8, false, true>(stats, blocks, a, s); break;
@@ -823,12 +870,12 @@ __device__ __forceinline__ void beginRay(LaneRay& L, f3 o, f3 d, float limit, bo
     L.any = any;
 }
 
-// With the 8-entry stack, capped at 96 VGPRs (five workgroups per CU; a few spilled values)
-// instead of 109 (four): +2% on the icosphere.  The 16-entry stack's LDS allows four.
+// With the 8- and 12-entry stacks, capped at 96 VGPRs (five workgroups per CU; a few spilled
+// values) instead of 109 (four): +2% on the icosphere.  The 16-entry stack's LDS allows four.
 #ifndef RT_TAIL_WAVES
 #define RT_TAIL_WAVES 5
 #endif
-template <int S> constexpr int tailBlocksPerCU() { return S == 8 ? RT_TAIL_WAVES : 4; }
+template <int S> constexpr int tailBlocksPerCU() { return S == 16 ? 4 : RT_TAIL_WAVES; }
 template <class N, bool STATS, int S>
 __global__ __launch_bounds__(kBlock, tailBlocksPerCU<S>()) void pathTailKernel(FrameArgs f, TraceArgs a, int firstBounce) {
     __shared__ int sRef[S * kBlock];
@@ -957,7 +1004,10 @@ static hipError_t launchTailN(bool stats, int blocks, int cus, const TraceLaunch
                               const TraceArgs& a, int firstBounce, hipStream_t s) {
     if (l.stack == 16) return launchTailNS<N, 16>(stats, blocks, cus, f, a, firstBounce, s);
     if (l.stack == 8) return launchTailNS<N, 8>(stats, blocks, cus, f, a, firstBounce, s);
-    if (l.stack == 12) return launchTailNS<N, 16>(stats, blocks, cus, f, a, firstBounce, s);   // no 12-entry variant
+    if (l.stack == 12) {
+        if constexpr (std::is_same<N, WideNode<4>>::value) return launchTailNS<N, 12>(stats, blocks, cus, f, a, firstBounce, s);
+        return launchTailNS<N, 16>(stats, blocks, cus, f, a, firstBounce, s);   // the 12-entry ring is 4-wide only
+    }
     return hipErrorInvalidValue;
 }
 hipError_t launchPathTail(const TraceLaunch& l, bool stats, int blocks, int cus, const FrameArgs& f, const TraceArgs& a,

@@ -122,7 +122,7 @@ int RtRayCounts(void* session, unsigned long long* closest, unsigned long long* 
  * default 32) and "build_max_leaf" (leaf size limit, 1..16, default 4 as the reference
  * builder) take effect at the next RtInitialize; "blocks_per_cu" (persistent traversal
  * workgroups per compute unit and pipeline, 1..32, default 3), "short_stack" (LDS stack entries per traversal lane,
- * 8, 12 or 16; 12 with 4-wide nodes only: other widths fail the render), "while_while" (0/1: speculative while-while traversal loop, default 1),
+ * 8, 12 or 16, default 12; layouts other than 4-wide full-precision nodes use 16 for 12), "while_while" (0/1: speculative while-while traversal loop, default 1),
  * "refill" (0/1: lanes take new rays while the wave traces others, default 1),
  * "refill_below" (1..64: with refill, a wave fetches new rays once fewer lanes than this
  * are busy, default 40), "pipelines" (1..4: a frame's paths split over that many streams
