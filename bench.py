@@ -37,8 +37,8 @@ HBM_PEAK_GBS = 8000.0
 # The soup keeps the reference builder's leaf <= 4 rule (its configuration names it); the
 # icosphere uses the autotuner's leaf size.
 TUNED = {
-    "soup": {"bvh_width": 4, "build_max_leaf": 2, "blocks_per_cu": 2, "short_stack": 16, "while_while": 1, "refill": 1,
-             "pipelines": 3},
+    "soup": {"bvh_width": 4, "build_max_leaf": 2, "blocks_per_cu": 2, "short_stack": 12, "while_while": 1, "refill": 1,
+             "pipelines": 4},
     "icosphere": {"bvh_width": 4, "build_max_leaf": 2, "blocks_per_cu": 3, "short_stack": 12, "while_while": 1,
                   "refill": 1, "pipelines": 4, "tail_bounce": 4},
     "cornell": {"bvh_width": 4, "build_max_leaf": 4, "blocks_per_cu": 8, "short_stack": 12, "while_while": 1, "refill": 0,
