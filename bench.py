@@ -153,6 +153,30 @@ def cpu_baseline(tris, cam, st, w, h, budget_s):
                       f"{rays} ray queries in {elapsed:.1f} s"}
 
 
+def denoise_timing(args):
+    """RtDenoise on a 4-spp 1080p frame of the icosphere workload, repeated: the workload for
+    profiling the denoiser's kernels (rocprofv3 --kernel-trace) apart from the study."""
+    import native_api
+    desc, tris, cam, st, w, h = workload("icosphere", 0)
+    s = native_api.Session()
+    for k, v in TUNED["icosphere"].items():
+        s.set_option(k, v)
+    s.initialize(tris, w, h)
+    st.raysPerPixel = 4
+    s.render(cam, st)
+    times = []
+    for _ in range(args.denoise_timing):
+        _, ms = s.denoise()
+        times.append(ms)
+    tiles, rejected, infer_ms = s.denoise_stats()
+    flops = s.lib.RtDenoiserForwardFlops(1, (h + 15) // 16 * 16, (w + 15) // 16 * 16)
+    s.close()
+    print(json.dumps({"workload": desc + ", 4 spp", "denoise_calls": len(times),
+                      "rtdenoise_ms_min": round(min(times), 3), "network_ms_last": round(infer_ms, 3),
+                      "network_flop": flops, "tiles": tiles, "tiles_fallback": rejected}), flush=True)
+    return 0
+
+
 def psnr_study(args):
     """Time to equal PSNR: learned denoise at 4 spp vs plain accumulation, against a
     high-sample render (seed 20261023), 8-bit display PSNR as Tools/verify_quality.py:53-54.
@@ -279,6 +303,8 @@ def main():
     ap.add_argument("--psnr-study", action="store_true",
                     help="time-to-equal-PSNR of the learned denoiser vs plain accumulation on the icosphere")
     ap.add_argument("--reference-spp", type=int, default=4096)
+    ap.add_argument("--denoise-timing", type=int, default=0, metavar="CALLS",
+                    help="only time RtDenoise CALLS times on a 4-spp icosphere frame (for profiling) and exit")
     ap.add_argument("--denoise-ensemble", type=int, choices=(1, 2, 4),
                     help="network passes per denoise (mirrored inputs averaged); default: the library's")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
@@ -299,6 +325,8 @@ def main():
         return 0
     if args.psnr_study:
         return psnr_study(args)
+    if args.denoise_timing:
+        return denoise_timing(args)
     if args.workload == "batch":
         return bench_batch(args)
     import native_api

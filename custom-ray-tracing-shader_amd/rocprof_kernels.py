@@ -1,0 +1,47 @@
+"""Per-kernel summary of a rocprofv3 --kernel-trace --stats run, for kernels matching a
+pattern, per call of the profiled workload (e.g. per RtDenoise), with the MFMA utilisation
+of a given FLOP count over the matched kernels' time (dense BF16 peak 2.5 PFLOP/s)."""
+import argparse
+import csv
+import glob
+import os
+import re
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--trace", required=True, help="rocprofv3 output directory (csv)")
+    ap.add_argument("--match", required=True, help="regex over kernel names")
+    ap.add_argument("--calls", type=int, required=True, help="workload calls in the profiled run")
+    ap.add_argument("--flop", type=float, default=0.0, help="FLOP of one workload call on MFMA")
+    ap.add_argument("--mfma-match", default="conv", help="regex of the kernels the FLOP run on")
+    ap.add_argument("--title", required=True)
+    ap.add_argument("--command", required=True, help="the command that made the trace")
+    ap.add_argument("--md", required=True)
+    args = ap.parse_args()
+    stats = glob.glob(os.path.join(args.trace, "**", "*kernel_stats.csv"), recursive=True)
+    if not stats:
+        raise SystemExit(f"no kernel_stats.csv under {args.trace}")
+    pat, mpat = re.compile(args.match), re.compile(args.mfma_match)
+    rows = [r for r in csv.DictReader(open(stats[0])) if pat.search(r["Name"])]
+    rows.sort(key=lambda r: -float(r["TotalDurationNs"]))
+    total = sum(float(r["TotalDurationNs"]) for r in rows) / args.calls / 1e3
+    mfma_us = sum(float(r["TotalDurationNs"]) for r in rows if mpat.search(r["Name"])) / args.calls / 1e3
+    out = [f"# {args.title}", "", "Command (from the repository root, on one MI355X):", "", "```", args.command, "```", "",
+           "| kernel | launches / call | mean us | us / call |", "|---|---:|---:|---:|"]
+    for r in rows:
+        name = re.sub(r"rt::|\(anonymous namespace\)::", "", r["Name"])
+        name = name if len(name) <= 72 else name[:69] + "..."
+        per = float(r["TotalDurationNs"]) / args.calls / 1e3
+        out.append(f"| `{name}` | {int(r['Calls']) / args.calls:.1f} | {float(r['AverageNs']) / 1e3:.1f} | {per:.1f} |")
+    out += ["", f"Matched kernels: {total:.1f} us per call."]
+    if args.flop and mfma_us:
+        rate = args.flop / (mfma_us * 1e-6)
+        out.append(f"Kernels on MFMA ({args.mfma_match}): {mfma_us:.1f} us per call for {args.flop / 1e9:.1f} GFLOP: "
+                   f"{rate / 1e12:.0f} TFLOP/s, {rate / 2.5e15:.3f} of the 2.5 PFLOP/s dense BF16 peak.")
+    open(args.md, "w").write("\n".join(out) + "\n")
+    print("\n".join(out))
+
+
+if __name__ == "__main__":
+    main()
