@@ -43,7 +43,7 @@ TUNED = {
                   "refill": 1, "pipelines": 4, "tail_bounce": 4},
     "cornell": {"bvh_width": 4, "build_max_leaf": 4, "blocks_per_cu": 8, "short_stack": 12, "while_while": 1, "refill": 0,
                 "pipelines": 4},
-    "batch": {"bvh_width": 4, "blocks_per_cu": 3, "short_stack": 12, "while_while": 1, "refill": 1, "pipelines": 2},
+    "batch": {"bvh_width": 4, "blocks_per_cu": 3, "short_stack": 12, "while_while": 1, "refill": 1, "pipelines": 3},
 }
 
 
