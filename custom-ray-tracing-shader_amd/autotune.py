@@ -5,8 +5,9 @@ from which paths complete in one launch.
 
 For a scene, camera and settings it initialises one session per (BVH width, leaf size),
 times a few device-resident frames for every (blocks_per_cu, short_stack, loop) combination,
-then tries each pipeline count and path-completion bounce on the fastest of those, and keeps
-the configuration with the highest ray throughput.  Choices persist in a JSON cache keyed by
+then, on the fastest of those, tries each pipeline count, each workgroup count, each
+path-completion bounce and again each (short_stack, loop), and keeps the configuration with
+the highest ray throughput.  Choices persist in a JSON cache keyed by
 scene size class and resolution ($RT_AUTOTUNE_CACHE, default ~/.cache/rt_amd/autotune.json),
 so later sessions of a similar scene reuse them without searching again.
 """
@@ -60,17 +61,23 @@ def apply(session, cfg):
             raise ValueError(f"option {key}={cfg[key]} rejected")
 
 
-def measure(session, camera, settings, frames):
+def measure(session, camera, settings, frames, repeats=2):
+    """Best of `repeats` timed runs of `frames` frames each, after one warm frame (option
+    changes re-size buffers on the next frame): single short runs are too noisy to rank
+    configurations a few percent apart."""
     session.render_resident(camera, settings)          # warm (and reset accumulation)
-    rays, t0 = 0, time.perf_counter()
-    for _ in range(frames):
-        session.render_resident(camera, settings)
-        c, s = session.ray_counts()
-        rays += c + s
-    return rays / (time.perf_counter() - t0) / 1e6
+    best = 0.0
+    for _ in range(repeats):
+        rays, t0 = 0, time.perf_counter()
+        for _ in range(frames):
+            session.render_resident(camera, settings)
+            c, s = session.ray_counts()
+            rays += c + s
+        best = max(best, rays / (time.perf_counter() - t0) / 1e6)
+    return best
 
 
-def autotune(triangles, camera, settings, width, height, frames=3, widths=WIDTHS, use_cache=True, lib=None, log=None,
+def autotune(triangles, camera, settings, width, height, frames=6, widths=WIDTHS, use_cache=True, lib=None, log=None,
              max_leaf=MAX_LEAF):
     """Returns (best config dict, list of all measurements)."""
     key = scene_key(len(triangles), width, height, settings.maxBounces)
@@ -117,6 +124,13 @@ def autotune(triangles, camera, settings, width, height, frames=3, widths=WIDTHS
         for t in TAILS:
             if t != launch.get("tail_bounce", 0) and t < settings.maxBounces:
                 trial(s, dict(launch, tail_bounce=t))
+        # The stack and loop ranking shifts with the pipelines and workgroups that share the
+        # GPU: sweep them again at the chosen ones.
+        launch = {k: v for k, v in best.items() if k != "mrays_per_s"}
+        for st in STACKS:
+            for ww, r in LOOPS:
+                if (st, ww, r) != (launch["short_stack"], launch["while_while"], launch["refill"]):
+                    trial(s, dict(launch, short_stack=st, while_while=ww, refill=r))
     if use_cache:
         cache[key] = best
         store_cache(cache)
