@@ -76,6 +76,12 @@ def bench_batch(args):
               "pipelines", "xcd_segments"):
         if getattr(args, k) is not None:
             cfg[k] = getattr(args, k)
+    if args.autotune:
+        import autotune
+        t0 = time.perf_counter()
+        best, trials = autotune.autotune_batch(views)
+        cfg = {k: v for k, v in best.items() if k != "mrays_per_s"}
+        log(f"autotune: {len(trials)} batch configurations in {time.perf_counter() - t0:.1f} s, best {best}")
     b = native_api.Batch(**cfg)
     for t, w, h, cam, st in views:
         b.add_view(t, w, h, cam, st)
@@ -98,7 +104,8 @@ def bench_batch(args):
             "data": "synthetic (seeded scene generators)",
             "config": {"workload": f"{len(views)} batched views (soup / displaced sphere), one persistent launch per "
                                    f"stage, hipGraph per frame", "views": len(views), "triangles": tris,
-                       "pixels": pixels, "launch": cfg, "build_ms": round(build_ms, 1)}}
+                       "pixels": pixels, "launch": cfg, "autotuned_now": args.autotune,
+                       "build_ms": round(build_ms, 1)}}
     print(json.dumps(line), flush=True)
     return 0
 

@@ -17,7 +17,7 @@ import os
 import time
 from pathlib import Path
 
-from native_api import Session
+from native_api import Batch, Session
 
 WIDTHS = (4, 8)
 MAX_LEAF = (4,)
@@ -135,3 +135,61 @@ def autotune(triangles, camera, settings, width, height, frames=6, widths=WIDTHS
         cache[key] = best
         store_cache(cache)
     return best, results
+
+
+def measure_batch(views, cfg, frames=4, repeats=2, lib=None):
+    """Rays per second of a batch built with launch options cfg: its frames are captured as
+    one graph per frame at build time, so each configuration is a fresh build."""
+    with Batch(lib, **cfg) as b:
+        for t, w, h, cam, st in views:
+            b.add_view(t, w, h, cam, st)
+        b.build()
+        b.render(1)   # warm
+        best = 0.0
+        for _ in range(repeats):
+            rays, t0 = 0, time.perf_counter()
+            for _ in range(frames):
+                b.render(1)
+                c, s = b.ray_counts()
+                rays += c + s
+            best = max(best, rays / (time.perf_counter() - t0) / 1e6)
+    return best
+
+
+def autotune_batch(views, widths=WIDTHS, lib=None, log=None):
+    """Launch options of a batch of views (BVH width, LDS stack, loop, path pipelines,
+    persistent workgroups per CU), searched as autotune() does: the width x stack x loop grid at
+    2 pipelines and 3 workgroups per CU, then pipelines, workgroups, and stack x loop again.
+    Returns (best config dict, list of all measurements)."""
+    results, best = [], None
+
+    def trial(cfg):
+        nonlocal best
+        cfg = dict(cfg)
+        cfg["mrays_per_s"] = round(measure_batch(views, cfg, lib=lib), 2)
+        results.append(cfg)
+        if log:
+            log(f"autotune batch {cfg}")
+        if best is None or cfg["mrays_per_s"] > best["mrays_per_s"]:
+            best = dict(cfg)
+
+    for w in widths:
+        for st in STACKS:
+            for ww, r in LOOPS:
+                trial({"bvh_width": w, "blocks_per_cu": 3, "short_stack": st, "while_while": ww, "refill": r,
+                       "pipelines": 2})
+    launch = {k: v for k, v in best.items() if k != "mrays_per_s"}
+    for p in PIPELINES:
+        if p != launch["pipelines"]:
+            trial(dict(launch, pipelines=p))
+    launch = {k: v for k, v in best.items() if k != "mrays_per_s"}
+    for b in BLOCKS:
+        if b != launch["blocks_per_cu"]:
+            trial(dict(launch, blocks_per_cu=b))
+    launch = {k: v for k, v in best.items() if k != "mrays_per_s"}
+    for st in STACKS:
+        for ww, r in LOOPS:
+            if (st, ww, r) != (launch["short_stack"], launch["while_while"], launch["refill"]):
+                trial(dict(launch, short_stack=st, while_while=ww, refill=r))
+    return best, results
+
