@@ -34,7 +34,8 @@ def main():
         name = name if len(name) <= 72 else name[:69] + "..."
         per = float(r["TotalDurationNs"]) / args.calls / 1e3
         out.append(f"| `{name}` | {int(r['Calls']) / args.calls:.1f} | {float(r['AverageNs']) / 1e3:.1f} | {per:.1f} |")
-    out += ["", f"Matched kernels: {total:.1f} us per call."]
+    out += ["", f"Matched kernels: {total:.1f} us of summed launch durations per call (launches on concurrent "
+                "streams overlap, so this can exceed the call's wall time)."]
     if args.flop and mfma_us:
         rate = args.flop / (mfma_us * 1e-6)
         out.append(f"Kernels on MFMA ({args.mfma_match}): {mfma_us:.1f} us per call for {args.flop / 1e9:.1f} GFLOP: "
