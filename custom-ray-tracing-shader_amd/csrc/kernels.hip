@@ -844,7 +844,10 @@ __global__ __launch_bounds__(kBlock, kShadeBlocksPerCU) void shadeKernel(FrameAr
                     &f.counters[kCntRays + f.bounce + 1], f.rayOut, na, nb);
     }
 }
-int shadeBlocksPerCU() { return kShadeBlocksPerCU; }
+#ifndef RT_SHADE_GRID_PER_CU
+#define RT_SHADE_GRID_PER_CU kShadeBlocksPerCU
+#endif
+int shadeBlocksPerCU() { return RT_SHADE_GRID_PER_CU; }
 hipError_t launchShade(const FrameArgs& f, int blocks, hipStream_t s) {
     const int n = f.R * f.npix;
     const int need = (n + kBlock - 1) / kBlock;
