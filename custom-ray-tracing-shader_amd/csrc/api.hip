@@ -48,7 +48,7 @@ struct Session {
     int refillBelow = 40;        // busy lanes below which a refilling wave fetches rays
     int whileWhile = 1;          // speculative while-while traversal loop
     int tiledPrimary = 1;        // primary rays queued in 8x8 pixel tiles
-    int pipelines = 2;           // independent path pipelines per chunk, each on its own stream
+    int pipelines = 4;           // independent path pipelines per chunk, each on its own stream
     int tailBounce = 0;          // >= 1: paths complete from this bounce on in one launch (0: off)
     int nontemporal = 1;         // rays / hits bypass cache retention in traversal
     int xcdSegments = 1;         // per-XCD segment cursors in the frame's traversal launches

@@ -54,7 +54,7 @@ struct Batch {
     std::vector<PendingView> pending;
     std::string error;
     int width = 4, quantized = 0, blocksPerCU = 3, shortStack = 12, refill = 1, refillBelow = 40, whileWhile = 1,
-        useGraph = 1, pipelines = 2, xcdSegments = 1;
+        useGraph = 1, pipelines = 3, xcdSegments = 1;
     bool built = false;
     int cus = 0, views = 0, totalSlots = 0, maxBounces = 0;
     long long totalTris = 0;

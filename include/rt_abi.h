@@ -126,7 +126,7 @@ int RtRayCounts(void* session, unsigned long long* closest, unsigned long long* 
  * "refill" (0/1: lanes take new rays while the wave traces others, default 1),
  * "refill_below" (1..64: with refill, a wave fetches new rays once fewer lanes than this
  * are busy, default 40), "pipelines" (1..4: a frame's paths split over that many streams
- * whose launches overlap, at least 65536 paths each, default 2; frames with trace
+ * whose launches overlap, at least 65536 paths each, default 4; frames with trace
  * statistics use one), "tail_bounce" (0 = off, default; k >= 1: from bounce k on, each path
  * runs to its end inside one launch instead of one launch per bounce; images unchanged),
  * "denoise_ensemble" (1, 2 or 4 network passes per RtDenoise: inputs mirrored in x, or in x, y
