@@ -298,6 +298,8 @@ bool renderGpu(Session* s, const CameraData& cam, const RenderSettings& rs, floa
     }
     if (s->traceStats && (s->error = hipMemsetAsync(s->dStats, 0, 4 * sizeof(unsigned long long), st)) != hipSuccess)
         return false;
+    // Each frame starts with a clear overflow flag, so one failed frame does not fail the next.
+    if ((s->error = launchZeroInts(s->dOverflow, 1, st)) != hipSuccess) return false;
     if ((s->error = hipEventRecord(s->ev0, st)) != hipSuccess) return false;
     // Path pipelines: a chunk's queue positions are split in P parts, each traced and shaded
     // on its own stream with its own counters, queues and spill ring, so one part's kernels
@@ -427,7 +429,13 @@ bool renderGpu(Session* s, const CameraData& cam, const RenderSettings& rs, floa
         (s->error = hipStreamSynchronize(st)) != hipSuccess)
         return false;
     if ((s->error = hipEventElapsedTime(ms, s->ev0, s->ev1)) != hipSuccess) return false;
-    if (s->hCounters[nCtr]) { s->hostError = "traversal stack exceeded its spill ring"; return false; }
+    if (s->hCounters[nCtr]) {
+        // The frame's accumulate kernel has already added its samples: restart the
+        // accumulation so the next frame does not build on an incomplete one.
+        RtReset(s);
+        s->hostError = "traversal stack exceeded its spill ring";
+        return false;
+    }
     unsigned long long closest = 0, shadow = 0;
     for (int c = 0; c < chunk; c++)
         for (int p = 0; p < chunkPipelines[c]; p++)
@@ -904,6 +912,7 @@ int RtTraceRays(void* p, const RtRay* rays, int count, int anyHit, RtHit* hits) 
     bool ok = (s->error = hipMemcpyAsync(dr, rays, (size_t)count * sizeof(RtRay), hipMemcpyHostToDevice, s->stream)) ==
               hipSuccess;
     if (ok) ok = (s->error = hipMemsetAsync(work, 0, sizeof(int), s->stream)) == hipSuccess;
+    if (ok) ok = (s->error = launchZeroInts(s->dOverflow, 1, s->stream)) == hipSuccess;
     if (ok) {
         TraceArgs a = baseTraceArgs(s);
         a.work = work;

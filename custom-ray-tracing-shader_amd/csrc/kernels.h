@@ -143,6 +143,8 @@ hipError_t launchBatchAccumulate(const BatchArgs& b, hipStream_t s);
 hipError_t launchBatchAdvance(const BatchArgs& b, hipStream_t s);
 
 hipError_t launchCopy(const float4* in, float4* out, long long n, int blocks, hipStream_t s);
+// Sets n ints to zero with a kernel (no memset node, so the launch may sit in a captured graph).
+hipError_t launchZeroInts(int* p, int n, hipStream_t s);
 const unsigned* hostSobolDirections();
 
 }  // namespace rt

@@ -1305,6 +1305,15 @@ __global__ __launch_bounds__(kBlock) void copyKernel(const float4* __restrict__ 
     for (long long i = blockIdx.x * (long long)kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock)
         out[i] = in[i];
 }
+__global__ void zeroIntsKernel(int* p, int n) {
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) p[k] = 0;
+}
+hipError_t launchZeroInts(int* p, int n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(zeroIntsKernel, dim3(std::min((n + kBlock - 1) / kBlock, 1024)), dim3(kBlock), 0, s, p, n);
+    return hipGetLastError();
+}
+
 hipError_t launchCopy(const float4* in, float4* out, long long n, int blocks, hipStream_t s) {
     hipLaunchKernelGGL(copyKernel, dim3(blocks), dim3(kBlock), 0, s, in, out, n);
     return hipGetLastError();
