@@ -11,7 +11,9 @@ so a wanted *linear* albedo a is passed as srgb_encode(a).
   displaced_icosphere(...)  icosphere (20*4^subdiv triangles) displaced along the normal by
                             6-octave seeded value-noise fBm (amplitude .05, lacunarity 2,
                             gain .5), Lambertian or metal (p = .3, roughness U(0,1)), inside a
-                            20-triangle room with an emissive ceiling panel
+                            20-triangle room with two emissive ceiling panels
+  displaced_geodesic(...)   the same surface, materials and room on a Class I geodesic sphere
+                            of frequency f (20*f^2 triangles; f = 592 gives 7,009,280)
 """
 import math
 
@@ -158,8 +160,7 @@ def soup_camera(width=1920, height=1080):
 
 
 # ----------------------------------------------------------------------------- icosphere
-def icosphere(subdiv):
-    """Unit icosphere with shared vertices: (vertices (V,3) float64, faces (F,3) int64)."""
+def _icosahedron():
     t = (1 + 5 ** 0.5) / 2
     v = np.array([(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t), (0, 1, -t),
                   (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)], dtype=np.float64)
@@ -167,6 +168,12 @@ def icosphere(subdiv):
     f = np.array([(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2),
                   (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5),
                   (2, 4, 11), (6, 2, 10), (8, 6, 7), (9, 8, 1)], dtype=np.int64)
+    return v, f
+
+
+def icosphere(subdiv):
+    """Unit icosphere with shared vertices: (vertices (V,3) float64, faces (F,3) int64)."""
+    v, f = _icosahedron()
     for _ in range(subdiv):
         e = np.concatenate([f[:, [0, 1]], f[:, [1, 2]], f[:, [2, 0]]])
         e.sort(axis=1)
@@ -243,9 +250,71 @@ def room(size=4.0, height=4.0, light=1.0, light_emission=12.0):
     return t
 
 
+def geodesic_sphere(frequency):
+    """Class I geodesic sphere: every icosahedron face cut into frequency^2 triangles on its
+    barycentric grid, grid points projected to the unit sphere.  Vertices on the icosahedron's
+    edges and corners are shared by the faces that meet there, so the mesh is closed.
+    Returns (vertices (10 f^2 + 2, 3) float64, faces (20 f^2, 3) int64), faces wound as the
+    icosahedron's (outward)."""
+    nu = int(frequency)
+    if nu < 1:
+        raise ValueError("frequency must be >= 1")
+    cv, cf = _icosahedron()
+    # Edges, each once with its lower corner first; interior edge points k = 1..nu-1 from that corner.
+    pairs = np.sort(np.concatenate([cf[:, [0, 1]], cf[:, [1, 2]], cf[:, [2, 0]]]), axis=1)
+    edges = np.unique(pairs, axis=0)
+    edge_id = {(int(a), int(b)): e for e, (a, b) in enumerate(edges)}
+    k = np.arange(1, nu, dtype=np.float64) / nu
+    ep = cv[edges[:, 0], None, :] + (cv[edges[:, 1]] - cv[edges[:, 0]])[:, None, :] * k[None, :, None]
+    n_edge = nu - 1
+    edge_base = len(cv)
+    # Interior grid points of each face: i >= 1, j >= 1, i + j <= nu - 1.
+    ii, jj = np.meshgrid(np.arange(nu + 1), np.arange(nu + 1), indexing="ij")
+    inner = (ii >= 1) & (jj >= 1) & (ii + jj <= nu - 1)
+    gi, gj = ii[inner], jj[inner]
+    n_inner = len(gi)
+    face_base = edge_base + len(edges) * n_edge
+    fp = (cv[cf[:, 0], None, :] + (cv[cf[:, 1]] - cv[cf[:, 0]])[:, None, :] * (gi / nu)[None, :, None]
+          + (cv[cf[:, 2]] - cv[cf[:, 0]])[:, None, :] * (gj / nu)[None, :, None])
+    verts = np.concatenate([cv, ep.reshape(-1, 3), fp.reshape(-1, 3)])
+    verts /= np.linalg.norm(verts, axis=1, keepdims=True)
+    inner_index = np.full((nu + 1, nu + 1), -1, np.int64)
+    inner_index[gi, gj] = np.arange(n_inner)
+
+    def edge_points(a, b, t):
+        """Vertex ids of the points at steps t (1..nu-1) from corner a toward corner b."""
+        e = edge_id[(min(a, b), max(a, b))]
+        step = t if a < b else nu - t
+        return edge_base + e * n_edge + step - 1
+
+    faces = []
+    for fi, (c0, c1, c2) in enumerate(cf):
+        c0, c1, c2 = int(c0), int(c1), int(c2)
+        ids = np.full((nu + 1, nu + 1), -1, np.int64)
+        ids[0, 0], ids[nu, 0], ids[0, nu] = c0, c1, c2
+        t = np.arange(1, nu)
+        ids[t, 0] = edge_points(c0, c1, t)
+        ids[0, t] = edge_points(c0, c2, t)
+        ids[nu - t, t] = edge_points(c1, c2, t)
+        ids[gi, gj] = face_base + fi * n_inner + inner_index[gi, gj]
+        ui, uj = np.nonzero(np.add.outer(np.arange(nu + 1), np.arange(nu + 1)) <= nu - 1)
+        up = np.stack([ids[ui, uj], ids[ui + 1, uj], ids[ui, uj + 1]], 1)
+        di, dj = np.nonzero(np.add.outer(np.arange(nu), np.arange(nu)) <= nu - 2)
+        down = np.stack([ids[di + 1, dj], ids[di + 1, dj + 1], ids[di, dj + 1]], 1)
+        faces.append(np.concatenate([up, down]))
+    return verts, np.concatenate(faces)
+
+
 def displaced_icosphere(subdiv=9, seed=7, amplitude=0.05, metal_probability=0.3, center=(0, 1.3, 0), radius=1.0):
+    return _displaced_mesh(*icosphere(subdiv), seed, amplitude, metal_probability, center, radius)
+
+
+def displaced_geodesic(frequency=592, seed=7, amplitude=0.05, metal_probability=0.3, center=(0, 1.3, 0), radius=1.0):
+    return _displaced_mesh(*geodesic_sphere(frequency), seed, amplitude, metal_probability, center, radius)
+
+
+def _displaced_mesh(v, f, seed, amplitude, metal_probability, center, radius):
     rng = np.random.default_rng(seed)
-    v, f = icosphere(subdiv)
     disp = fbm(v, seed) * amplitude
     pos = v * (radius + disp)[:, None] + np.asarray(center)
     # Area-weighted vertex normals of the displaced surface, for smooth shading.
@@ -268,3 +337,24 @@ def displaced_icosphere(subdiv=9, seed=7, amplitude=0.05, metal_probability=0.3,
 
 def icosphere_camera(width=1920, height=1080):
     return make_camera((0, 1.6, -3.6), (0, 1.25, 0), fov_deg=50, aspect=width / height)
+
+
+# ----------------------------------------------------------------------------- batched views
+def batch_views(count, seed=2024):
+    """Batched views: scenes from the soup / displaced-sphere generators with triangle counts
+    log-uniform in [1e3, 2e6], image sides log-uniform in [256, 1024], 2..8 bounces."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for v in range(count):
+        n = int(round(10 ** rng.uniform(3, np.log10(2e6))))
+        w, h = (int(round(2 ** rng.uniform(8, 10))) for _ in range(2))
+        bounces = int(rng.integers(2, 9))
+        if v % 2 == 0:
+            tris = triangle_soup(n, seed=seed + v)
+            cam, st = soup_camera(w, h), make_settings(bounces, sky=1.0, seed=seed + v)
+        else:
+            sub = int(np.clip(round(np.log(max(n - 20, 20) / 20) / np.log(4)), 1, 8))
+            tris = displaced_icosphere(subdiv=sub, seed=seed + v)
+            cam, st = icosphere_camera(w, h), make_settings(bounces, seed=seed + v)
+        out.append((tris, w, h, cam, st))
+    return out

@@ -79,6 +79,7 @@ def lib():
         L.or_scene_create.argtypes = [P, c.c_int, P]
         L.or_scene_create.restype = P
         L.or_scene_free.argtypes = [P]
+        L.or_scene_trace_rays.argtypes = [P, P, c.c_int, c.c_int, P]
         L.or_scene_bvh_nodes.argtypes = [P]
         L.or_scene_bvh_depth.argtypes = [P]
         L.or_scene_light_area.argtypes = [P]
@@ -231,6 +232,13 @@ class OracleScene:
     @property
     def light_area(self):
         return float(lib().or_scene_light_area(self.h))
+
+    def trace(self, rays, any_hit=False):
+        """Hit records of ray queries against the scene's own BVH (the one render uses)."""
+        rays = np.ascontiguousarray(rays, dtype=RAY)
+        hits = np.zeros(len(rays), dtype=HIT)
+        lib().or_scene_trace_rays(self.h, _p(rays), len(rays), int(any_hit), _p(hits))
+        return hits
 
     def render(self, camera, settings, width, height, sample=0, total=None, threads=0, rows=None):
         """One RtRender call: raysPerPixel samples starting at `sample` added into `total`

@@ -524,6 +524,11 @@ void* or_scene_create(const Triangle* tris, int n, const unsigned* dirs) {
     return s;
 }
 void or_scene_free(void* h) { delete static_cast<Scene*>(h); }
+// Ray queries against the scene's own 12-bin BVH (the one or_render traverses).
+void or_scene_trace_rays(void* h, const RtRay* rays, int count, int any, RtHit* hits) {
+    const Scene& sc = *static_cast<Scene*>(h);
+    or_trace_rays((void*)&sc.bvh, sc.tris.data(), rays, count, any, hits, nullptr, nullptr);
+}
 int or_scene_bvh_nodes(void* h) { return (int)static_cast<Scene*>(h)->bvh.nodes.size(); }
 int or_scene_bvh_depth(void* h) { return static_cast<Scene*>(h)->bvh.depth; }
 float or_scene_light_area(void* h) { return static_cast<Scene*>(h)->lightArea; }

@@ -1,0 +1,179 @@
+"""The benchmark configurations against the oracle, at their own workloads, with the launch
+options the benchmark runs (Native/renderer.cu:197-302 per view and per path):
+
+  C1  Cornell box, 36 triangles, 256x256, 16 spp, 4 bounces
+  C2  1M-triangle random soup, 1920x1080, 3 bounces, progressive 1-spp frames, sky 1
+  C3  displaced sphere + 20-triangle room, 8 bounces + Russian roulette, glossy metal (p = .3),
+      two emitters: reduced tessellation at reduced resolution here (full image test), the
+      full 5.24M / 7.0M scenes in test_gpu_full_scenes.py
+  C4  batched views: every view of a mixed batch, and a 64-view batch
+
+Images: HDR mean relative error <= 1e-3 at identical sample indices, the reference's display
+gate (MAE <= 0.5, <= 1% of channels off by more than 1/255, Tools/verify_demo_scenes.py:54-56),
+covered counts exact, ray counts within 1e-3 (paths differ only where libm transcendentals
+differ by ulps between the GPU and the host)."""
+import numpy as np
+import pytest
+
+import oracle
+import scenes
+from bench import TUNED
+from helpers import cam_record, image_agreement, settings_record
+from native_api import Batch, Session
+
+pytestmark = pytest.mark.gpu
+
+
+def oracle_frames(tris, cam, st, w, h, calls, scene=None):
+    sc = scene or oracle.OracleScene(tris)
+    total, sample, closest, shadow = None, 0, 0, 0
+    for _ in range(calls):
+        rgba, total, (c, s) = sc.render(cam_record(cam), settings_record(st), w, h, sample, total)
+        sample += st.raysPerPixel
+        closest, shadow = closest + c, shadow + s
+    return rgba, total, (closest, shadow)
+
+
+def hdr_mean_relative_error(gpu, ref):
+    g, r = gpu[..., :3].astype(np.float64), ref[..., :3].astype(np.float64)
+    return float(np.mean(np.abs(g - r) / np.maximum(np.abs(r), 1e-2)))
+
+
+def gpu_frames(lib, tris, cam, st, w, h, calls, options):
+    """`calls` RtRender calls on the GPU; returns (image, accumulation, summed ray counts)."""
+    closest = shadow = 0
+    with Session(lib) as s:
+        for k, v in options.items():
+            assert s.set_option(k, v), k
+        s.initialize(tris, w, h)
+        for _ in range(calls):
+            img, _ = s.render(cam, st, 2)
+            c, sh = s.ray_counts()
+            closest, shadow = closest + c, shadow + sh
+        return img, s.accumulation(), (closest, shadow)
+
+
+def assert_frame_matches(img, acc, counts, ref_img, ref_acc, ref_counts, label):
+    rel = hdr_mean_relative_error(acc, ref_acc)
+    mae, over1, worst = image_agreement(img, ref_img)
+    print(f"{label}: HDR mean rel err {rel:.2e}, display MAE {mae:.4f}, >1/255 {over1:.2e}, worst {worst}, "
+          f"rays {counts} vs oracle {ref_counts}", flush=True)
+    assert rel <= 1e-3, rel
+    assert mae <= 0.5 and over1 <= 0.01, (mae, over1)
+    assert np.array_equal(acc[..., 9], ref_acc[..., 9])
+    for got, want in zip(counts, ref_counts):
+        assert abs(got - want) <= 1e-3 * max(want, 1), (counts, ref_counts)
+
+
+def test_c1_cornell_config_matches_oracle(lib):
+    w = h = 256
+    tris, cam, st = scenes.cornell(), scenes.cornell_camera(w, h), scenes.make_settings(4, rpp=16)
+    got = gpu_frames(lib, tris, cam, st, w, h, 1, TUNED["cornell"])
+    assert_frame_matches(*got, *oracle_frames(tris, cam, st, w, h, 1), "C1 cornell 256^2 16 spp")
+
+
+@pytest.mark.parametrize("mesh", ["icosphere6", "geodesic64"])
+def test_c3_workload_reduced_matches_oracle(lib, mesh):
+    """The C3 material mix and integrator at the benchmark's launch options (leaf 2, 12-entry
+    stack, while-while + refill, pipelines, path completion from bounce 4): 81,920-triangle
+    tessellations of the displaced sphere, 256x144, 8 bounces + RR, 4 spp per call, two calls."""
+    tris = (scenes.displaced_icosphere(subdiv=6, seed=7) if mesh == "icosphere6"
+            else scenes.displaced_geodesic(frequency=64, seed=7))
+    assert (tris["specularProbability"] > 0).mean() > 0.25 and (tris["emission"] > 0).sum() == 4
+    w, h = 256, 144
+    cam, st = scenes.icosphere_camera(w, h), scenes.make_settings(8, rpp=4)
+    got = gpu_frames(lib, tris, cam, st, w, h, 2, TUNED["icosphere"])
+    assert_frame_matches(*got, *oracle_frames(tris, cam, st, w, h, 2), f"C3 {mesh} 256x144 2x4 spp")
+
+
+def test_c2_soup_full_frames_match_oracle(lib):
+    """C2 at its size: the 1M soup, 1920x1080, 3 bounces, two progressive 1-spp frames, at the
+    configuration's leaf <= 4 and at the benchmark's leaf <= 2 (bit-identical frames)."""
+    w, h = 1920, 1080
+    tris = scenes.triangle_soup(1_000_000, seed=1)
+    cam, st = scenes.soup_camera(w, h), scenes.make_settings(3, rpp=1, sky=1.0)
+    leaf4 = gpu_frames(lib, tris, cam, st, w, h, 2, dict(TUNED["soup"], build_max_leaf=4))
+    leaf2 = gpu_frames(lib, tris, cam, st, w, h, 2, TUNED["soup"])
+    assert np.array_equal(leaf4[1].view(np.uint32), leaf2[1].view(np.uint32)) and leaf4[2] == leaf2[2]
+    assert_frame_matches(*leaf4, *oracle_frames(tris, cam, st, w, h, 2), "C2 soup 1M 1080p 2x1 spp")
+
+
+def _batch_and_singles(lib, views, frames, options):
+    with Batch(lib, **options) as b:
+        for t, w, h, cam, st in views:
+            b.add_view(t, w, h, cam, st)
+        b.build()
+        counts = (0, 0)
+        for _ in range(frames):   # RtBatchRayCounts reports the last frame
+            b.render(1)
+            c1, s1 = b.ray_counts()
+            counts = (counts[0] + c1, counts[1] + s1)
+        got = [b.read(v) for v in range(len(views))]
+        import ctypes as c
+        samples = []
+        for v in range(len(views)):
+            n = c.c_uint()
+            assert lib.RtBatchViewInfo(b.handle, v, None, None, c.byref(n))
+            samples.append(n.value)
+    return got, counts, samples
+
+
+def test_c4_batch_views_match_oracle(lib):
+    """Every view of a mixed batch (Cornell, soup, displaced sphere, thin-lens Cornell) against
+    the oracle rendering that view alone with one sample per call."""
+    vs = [(scenes.cornell(), 64, 48, scenes.cornell_camera(64, 48), scenes.make_settings(4, seed=7)),
+          (scenes.triangle_soup(20000, seed=5), 80, 60, scenes.soup_camera(80, 60),
+           scenes.make_settings(3, sky=1.0, seed=42)),
+          (scenes.displaced_icosphere(subdiv=4, seed=3), 96, 64, scenes.icosphere_camera(96, 64),
+           scenes.make_settings(8, seed=1234)),
+          (scenes.cornell(), 33, 17, scenes.cornell_camera(33, 17), scenes.make_settings(2, seed=9, aperture=0.05,
+                                                                                         focus=3.0))]
+    frames = 4
+    got, counts, samples = _batch_and_singles(lib, vs, frames, TUNED["batch"])
+    assert samples == [frames] * len(vs)
+    total = [0, 0]
+    for v, (t, w, h, cam, st) in enumerate(vs):
+        st.raysPerPixel = 1
+        ref_img, ref_acc, ref_counts = oracle_frames(t, cam, st, w, h, frames)
+        total = [total[0] + ref_counts[0], total[1] + ref_counts[1]]
+        rel = hdr_mean_relative_error(got[v][1], ref_acc)
+        mae, over1, _ = image_agreement(got[v][0], ref_img)
+        print(f"C4 view {v}: HDR mean rel err {rel:.2e}, display MAE {mae:.4f}", flush=True)
+        assert rel <= 1e-3 and mae <= 0.5 and over1 <= 0.01, (v, rel, mae, over1)
+        assert np.array_equal(got[v][1][..., 9], ref_acc[..., 9]), v
+    for g, r in zip(counts, total):
+        assert abs(g - r) <= 1e-3 * max(r, 1), (counts, total)
+
+
+@pytest.mark.timeout(900)
+def test_c4_64_view_batch(lib):
+    """The benchmark's 64-view batch (17.3M triangles, 16.6M pixels, 2..8 bounces) traced as one
+    batch for two frames: every view has two samples and the accumulation of a session
+    rendering it alone, bit for bit; the batch's ray counts are the sessions' sums; the eight
+    cheapest views also match the oracle."""
+    vs = scenes.batch_views(64)
+    frames = 2
+    opts = TUNED["batch"]
+    got, counts, samples = _batch_and_singles(lib, vs, frames, opts)
+    assert samples == [frames] * len(vs)
+    closest = shadow = 0
+    for v, (t, w, h, cam, st) in enumerate(vs):
+        st.raysPerPixel = 1
+        img, acc, (c, sh) = gpu_frames(lib, t, cam, st, w, h, frames,
+                                       {k: opts[k] for k in ("bvh_width", "short_stack", "while_while", "refill")})
+        closest, shadow = closest + c, shadow + sh
+        cov = got[v][1][..., 9]
+        assert ((cov >= 0) & (cov <= frames) & (cov == np.round(cov))).all(), v
+        assert np.array_equal(got[v][0], img), v
+        assert np.array_equal(got[v][1].view(np.uint32), acc.view(np.uint32)), v
+    assert counts == (closest, shadow)
+    cost = sorted(range(len(vs)), key=lambda i: vs[i][1] * vs[i][2] * np.log2(len(vs[i][0])))
+    for v in cost[:8]:
+        t, w, h, cam, st = vs[v]
+        ref_img, ref_acc, _ = oracle_frames(t, cam, st, w, h, frames)
+        rel = hdr_mean_relative_error(got[v][1], ref_acc)
+        mae, over1, _ = image_agreement(got[v][0], ref_img)
+        print(f"64-view batch, view {v} ({len(t)} tris, {w}x{h}): HDR mean rel err {rel:.2e}, MAE {mae:.4f}",
+              flush=True)
+        assert rel <= 1e-3 and mae <= 0.5 and over1 <= 0.01, (v, rel, mae, over1)
+        assert np.array_equal(got[v][1][..., 9], ref_acc[..., 9]), v
