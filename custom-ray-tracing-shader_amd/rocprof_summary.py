@@ -2,7 +2,7 @@
 with the traversal kernel's HBM bytes per launch (read by bench.py as roofline.traffic).
 
 Inputs (directories written by rocprofv3 -d ... --output-format csv):
-  --trace DIR        --kernel-trace --stats run            (*_kernel_stats.csv)
+  --trace DIR        --kernel-trace --stats run            (*_kernel_stats.csv, *_kernel_trace.csv)
   --fetch DIR        --pmc FETCH_SIZE run of the workload   (*_counter_collection.csv)
   --write DIR        --pmc WRITE_SIZE run of the workload
   --tcc DIR          --pmc TCC_HIT_sum TCC_MISS_sum run     (optional)
@@ -12,6 +12,11 @@ Inputs (directories written by rocprofv3 -d ... --output-format csv):
   --copy-bytes BYTES                    size the probe copied (reads BYTES, writes BYTES)
 FETCH_SIZE and WRITE_SIZE are in KiB per dispatch; each is multiplied by
 (expected KiB of the copy) / (counter KiB of the copy) before use.
+
+Launches of concurrent path pipelines overlap, so the dominant kernel's time per launch is
+also priced as the union of its launch intervals (Start_Timestamp..End_Timestamp of every
+matching dispatch in the kernel trace) divided by the launches: with --bench-json, the
+algorithmic bytes per launch over that time give the roofline fraction from this trace alone.
 """
 import argparse
 import csv
@@ -28,6 +33,22 @@ def one_csv(directory, suffix):
     if not files:
         raise SystemExit(f"no *{suffix} under {directory}")
     return list(csv.DictReader(open(files[0])))
+
+
+def busy_union_ns(rows, pattern):
+    """(launches, union of their [start, end) intervals in ns, summed durations in ns)."""
+    spans = sorted((int(r["Start_Timestamp"]), int(r["End_Timestamp"])) for r in rows
+                   if pattern.search(r["Kernel_Name"]))
+    busy, end, total = 0, None, 0
+    for a, b in spans:
+        total += b - a
+        if end is None or a > end:
+            busy += b - a
+            end = b
+        elif b > end:
+            busy += b - end
+            end = b
+    return len(spans), busy, total
 
 
 def counter_means(directory):
@@ -106,13 +127,19 @@ def main():
     calls = sum(p[1] for p in pool)
     wmean = lambda i: (sum(p[1] * p[i] for p in pool) / calls) if all(p[i] is not None for p in pool) else None
     hits = [p[5] for p in pool if p[5]]
+    launches, busy_ns, summed_ns = busy_union_ns(one_csv(args.trace, "kernel_trace.csv"), pat)
     dominant = {"kernel": " + ".join(p[0] for p in pool), "kernel_pattern": args.kernel,
+                "trace_launches": launches, "trace_busy_ms": busy_ns / 1e6,
+                "trace_union_ms_per_launch": busy_ns / 1e6 / max(launches, 1),
                 "mean_ms": wmean(2) / 1e6, "calls": calls,
                 "hbm_read_bytes_per_launch": wmean(3), "hbm_write_bytes_per_launch": wmean(4),
                 "l2_hit_percent": 100 * sum(h[0] for h in hits) / max(sum(h[1] for h in hits), 1) if hits else None}
     traffic = (dominant["hbm_read_bytes_per_launch"] or 0) + (dominant["hbm_write_bytes_per_launch"] or 0)
     dominant["hbm_bytes_per_launch"] = traffic
     dominant["fetch_scale"], dominant["write_scale"] = fetch_scale, write_scale
+    lines += ["", f"Kernel trace: {launches} matching launches, summed durations {summed_ns / 1e6:.3f} ms, union of "
+              f"their intervals {busy_ns / 1e6:.3f} ms ({busy_ns / 1e3 / max(launches, 1):.1f} us per launch; "
+              f"{summed_ns / max(busy_ns, 1):.2f} launches running on average while any runs)."]
     lines += ["", f"Dominant kernel (all launches matching `{args.kernel}`: "
               + ", ".join(f"`{short(p[0], 90)}` x{p[1]}" for p in pool)
               + f"): mean {dominant['mean_ms'] * 1e3:.1f} us over {calls} launches; HBM traffic "
@@ -140,11 +167,20 @@ def main():
                   f"triangles per ray), i.e. {alg / max(traffic, 1):.1f}x the HBM traffic: the rest is served "
                   f"by L2 / MALL."]
         dominant["algorithmic_bytes_per_launch"] = alg
+        per_launch_s = busy_ns / 1e9 / max(launches, 1)
+        achieved = alg / per_launch_s / 1e9
+        dominant["trace_achieved_GBs"] = achieved
+        dominant["trace_frac"] = achieved / 8000.0
+        dominant["trace_hbm_traffic_frac"] = traffic / per_launch_s / 1e9 / 8000.0
+        lines += [f"Roofline from this trace: algorithmic bytes per launch / union time per launch = "
+                  f"{alg / 1e6:.1f} MB / {per_launch_s * 1e6:.1f} us = {achieved:.0f} GB/s, "
+                  f"**{achieved / 8000:.3f} of 8 TB/s** (a rate of algorithmic bytes); measured HBM traffic over "
+                  f"the same time {traffic / per_launch_s / 1e9:.0f} GB/s, {traffic / per_launch_s / 1e9 / 8000:.3f} "
+                  f"of 8 TB/s."]
         if "busy_ms_per_step" in rf:
-            lines += [f"Launches of concurrent path pipelines overlap: per step the traversal ran for "
-                      f"{rf['busy_ms_per_step']:.3f} ms of union time over {rf['launches_per_step']:.0f} launches "
-                      f"({rf['effective_launch_ms'] * 1e3:.1f} us per launch), which bench.py prices the "
-                      f"algorithmic bytes on: {rf['achieved']:.0f} GB/s, {rf['frac']:.3f} of 8 TB/s."]
+            lines += [f"bench.py's own figure (HIP events, union per step {rf['busy_ms_per_step']:.3f} ms over "
+                      f"{rf['launches_per_step']:.0f} launches, {rf['effective_launch_ms'] * 1e3:.1f} us per launch): "
+                      f"{rf['achieved']:.0f} GB/s, {rf['frac']:.3f}."]
     lines += ["", "Summary made with:", "```", "python3 custom-ray-tracing-shader_amd/rocprof_summary.py " +
               " ".join(f'"{a}"' if (" " in a or "|" in a) else a for a in sys.argv[1:]), "```"]
     with open(args.md, "w") as fh:
