@@ -8,6 +8,7 @@
 #include <cstring>
 #include <string>
 #include <vector>
+#include <mutex>
 #include <dlfcn.h>
 #include <omp.h>
 #include "kernels.h"
@@ -18,7 +19,34 @@
 
 using namespace rt;
 
+// Device scratch of RtTraceRaysNoAlloc, one copy per GPU for the life of the process: the
+// traversal's work cursor (int 0) and overflow flag (int 1) in the first 256 bytes, then the
+// stack spill ring.  Host code never uses the array's name as an address: the per-device
+// address comes from hipGetSymbolAddress (scratchAddress below).
+constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
+constexpr size_t kNoAllocHeaderBytes = 256;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
 namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex gScratchMutex;
+void* gScratch[kMaxDevices] = {};
+
+// The scratch array's address on `device`, which must be the current device; resolved once per
+// device.
+hipError_t scratchAddress(int device, void** out) {
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(gScratchMutex);
+    if (!gScratch[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        gScratch[device] = p;
+    }
+    *out = gScratch[device];
+    return hipSuccess;
+}
 
 struct HostSum { f3 rad, alb, nrm; float covered, depth, lum2; };
 
@@ -34,6 +62,7 @@ struct Session {
     std::vector<int> binOrder;
     int binNodeCount = 0, wideNodeCount = 0;
     int bvhDepth = 0;
+    int wideDepth = 0;           // deepest wide node (root 0): bounds a traversal stack by (W-1)(wideDepth+1)
     float buildMs = 0;
     int width = 0, height = 0;
     int bvhWidth = 4;            // option, used at the next RtInitialize
@@ -85,6 +114,7 @@ struct Session {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     hipEvent_t evFork = nullptr, evJoin[3] = {};
     // progressive state (Native/renderer.cu:65-68)
+    int device = 0;              // GPU current at RtInitialize
     int cpuThreads = 0, lastBackend = -1;
     unsigned samples = 0, version = 0;
     CameraData lastCamera{};
@@ -192,7 +222,7 @@ bool buildAccel(Session* s) {
         return false;
     }
     s->dBinNodes = r.binNodes; s->dOrder = r.order; s->dRecs = r.recs; s->dNodes = r.wideNodes;
-    s->binNodeCount = r.binNodeCount; s->wideNodeCount = r.wideCount; s->bvhDepth = r.depth;
+    s->binNodeCount = r.binNodeCount; s->wideNodeCount = r.wideCount; s->bvhDepth = r.depth; s->wideDepth = r.wideDepth;
     if (s->bvhWidth == 4 && !s->bvhQuantized && r.wideCount >= kMaxWide4Nodes) {
         s->hostError = "scene needs more than 2^25 4-wide nodes (4 GB); use bvh_width 8 or bvh_quantized";
         return false;
@@ -602,6 +632,12 @@ int RtInitialize(void* p, const Triangle* triangles, int count, int w, int h) {
     int device = 0, cus = 0;
     if ((s->error = hipGetDevice(&device)) != hipSuccess) return 0;
     if ((s->error = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device)) != hipSuccess) return 0;
+    void* scratchPtr = nullptr;   // set-up of RtTraceRaysNoAlloc's buffer on this device
+    if ((s->error = scratchAddress(device, &scratchPtr)) != hipSuccess) {
+        s->hostError = std::string("no-allocation scratch buffer: ") + hipGetErrorString(s->error);
+        return 0;
+    }
+    s->device = device;
     s->tris.assign(triangles, triangles + count);
     s->width = w; s->height = h;
     s->hostSum.assign((size_t)w * h, HostSum{});
@@ -925,6 +961,54 @@ int RtTraceRays(void* p, const RtRay* rays, int count, int anyHit, RtHit* hits) 
     if (ok) ok = (s->error = hipMemcpy(hits, dh, (size_t)count * sizeof(RtHit), hipMemcpyDeviceToHost)) == hipSuccess;
     int overflow = 0;
     if (ok) ok = (s->error = hipMemcpy(&overflow, s->dOverflow, sizeof(int), hipMemcpyDeviceToHost)) == hipSuccess;
+    if (ok && overflow) { s->hostError = "traversal stack exceeded its spill ring"; ok = false; }
+    return ok ? 1 : 0;
+}
+
+int RtTraceRaysNoAlloc(void* p, const RtRay* deviceRays, int count, int anyHit, RtHit* deviceHits) {
+    Session* s = static_cast<Session*>(p);
+    if (!s) return 0;
+    s->hostError.clear();
+    if (!s->ready || !deviceRays || !deviceHits || count < 0 || count > (1 << 30)) {
+        s->error = hipErrorInvalidValue;
+        return 0;
+    }
+    if (count == 0) return 1;
+    void* base = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(gScratchMutex);
+        base = (s->device >= 0 && s->device < kMaxDevices) ? gScratch[s->device] : nullptr;
+    }
+    if (!base) { s->hostError = "no-allocation scratch buffer not set up on this device"; s->error = hipErrorInvalidValue; return 0; }
+    // Stack entries a lane can hold beyond its LDS ring: (W-1) per inner node on the deepest path.
+    const int ldsEntries = (s->shortStack == 12 && !(s->builtWidth == 4 && !s->builtQuantized)) ? 16 : s->shortStack;
+    const int spillEntries = std::max(1, (s->builtWidth - 1) * (s->wideDepth + 1) - ldsEntries);
+    const size_t laneBytes = (size_t)spillEntries * 8;
+    const long long fitBlocks = (long long)((kNoAllocScratchBytes - kNoAllocHeaderBytes) / laneBytes / kBlock);
+    const int blocks = (int)std::min<long long>((long long)s->cus * std::max(s->blocksPerCU, kQueryBlocksPerCU), fitBlocks);
+    if (blocks < s->cus) {
+        s->hostError = "BVH too deep for the no-allocation scratch buffer (" + std::to_string(kNoAllocScratchBytes >> 20) +
+                       " MiB): needs " + std::to_string(spillEntries) + " spill entries per lane";
+        s->error = hipErrorInvalidValue;
+        return 0;
+    }
+    int* header = static_cast<int*>(base);
+    TraceArgs a = baseTraceArgs(s);
+    a.spill = reinterpret_cast<int*>(static_cast<unsigned char*>(base) + kNoAllocHeaderBytes);
+    a.spillEntries = spillEntries;
+    a.totalLanes = blocks * kBlock;
+    a.work = header;
+    a.overflow = header + 1;
+    const float4* dr = reinterpret_cast<const float4*>(deviceRays);
+    int4* dh = reinterpret_cast<int4*>(deviceHits);
+    if (anyHit) { a.anyRays = dr; a.anyCount = count; a.anyHits = dh; }
+    else { a.rays = dr; a.count = count; a.hits = dh; }
+    int overflow = 0;
+    bool ok = (s->error = launchZeroInts(header, 2, s->stream)) == hipSuccess;
+    if (ok) ok = (s->error = launchTrace(traceLaunch(s), false, blocks, a, s->stream)) == hipSuccess;
+    if (ok) ok = (s->error = hipMemcpyAsync(&overflow, header + 1, sizeof(int), hipMemcpyDeviceToHost, s->stream)) ==
+                 hipSuccess;
+    if (ok) ok = (s->error = hipStreamSynchronize(s->stream)) == hipSuccess;
     if (ok && overflow) { s->hostError = "traversal stack exceeded its spill ring"; ok = false; }
     return ok ? 1 : 0;
 }

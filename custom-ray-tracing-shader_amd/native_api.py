@@ -95,6 +95,7 @@ def load_library(path=None):
                              c.POINTER(c.c_float)]
     lib.RtDenoise.argtypes = [vp, c.POINTER(c.c_ubyte), c.c_int, c.POINTER(c.c_float)]
     lib.RtTraceRays.argtypes = [vp, c.POINTER(Ray), c.c_int, c.c_int, c.POINTER(Hit)]
+    lib.RtTraceRaysNoAlloc.argtypes = [vp, vp, c.c_int, c.c_int, vp]   # device rays, device hits
     lib.RtSobolBatch.argtypes = [c.POINTER(c.c_uint), c.POINTER(c.c_uint), c.c_int, c.c_uint, c.c_int,
                                  c.POINTER(c.c_float)]
     lib.RtBvhExport.argtypes = [vp, c.POINTER(BvhNode), c.c_int, c.POINTER(c.c_int), c.c_int]

@@ -106,6 +106,17 @@ typedef struct RtBvhNode { float low[3]; int first; float high[3]; int count; } 
 /* Closest-hit (anyHit = 0) or any-hit queries against the session's GPU BVH on the GPU;
  * at most 2^30 rays per call. */
 int RtTraceRays(void* session, const RtRay* rays, int count, int anyHit, RtHit* hits);
+/* No counterpart in the reference.  For callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): the
+ * RtTraceRays query over rays and hits that already live in device memory the caller owns.
+ * The traversal's work cursor, overflow flag and stack spill ring live in a library-static
+ * 64 MiB device buffer, one per GPU, resolved by RtInitialize and held until the process
+ * ends; the call makes no hipMalloc, hipFree or other allocation.  It launches as many
+ * persistent workgroups as that buffer's spill ring serves at the session's BVH depth and
+ * fails (RtLastError says why) when that is fewer than one per compute unit.  Every call on a
+ * GPU shares that GPU's buffer, so calls that overlap on one GPU race.  Synchronous on the
+ * session's stream (the caller's writes of the rays must be complete); at most 2^30 rays. */
+int RtTraceRaysNoAlloc(void* session, const RtRay* deviceRays, int count, int anyHit, RtHit* deviceHits);
 /* Scrambled Sobol values computed on the GPU: out[i*dimensions + d] = sample(pixels[i],
  * samples[i], seed, d) for d < dimensions <= 260 (Native/sampling.cuh:26-35). */
 int RtSobolBatch(const unsigned* pixels, const unsigned* samples, int count, unsigned seed,
