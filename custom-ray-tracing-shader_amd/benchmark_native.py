@@ -60,7 +60,7 @@ def run(snapshot_path, output, library=LIBRARY, backend=None, sizes="320x180,128
         log=print):
     output = Path(output)
     if (output / "results.json").exists():
-        raise FileExistsError("Use a fresh output directory")
+        raise FileExistsError(f"{output / 'results.json'} already exists; give an empty output directory")
     output.mkdir(parents=True, exist_ok=True)
     scene_bytes = Path(snapshot_path).read_bytes()
     (output / "scene.json").write_bytes(scene_bytes)
@@ -111,7 +111,7 @@ def run(snapshot_path, output, library=LIBRARY, backend=None, sizes="320x180,128
                                             c.byref(ms)):
                             error = lib.RtLastError(session).decode()
                             case["backends"].append({"backend": name, "unavailable": error})
-                            log(f"{size}, {b} hits, {name}: unavailable ({error})")
+                            log(f"{name} at {size}, {b} bounces: not available here ({error})")
                             continue
                     lib.RtReset(session)
                     compute, calls = [], []
@@ -142,7 +142,8 @@ def run(snapshot_path, output, library=LIBRARY, backend=None, sizes="320x180,128
                            "channels_over_1_fraction": float((errors > 1).mean()) if errors is not None else None,
                            "embree_build_ms": lib.RtEmbreeBuildMilliseconds(session) if mode == 3 else None}
                     case["backends"].append(row)
-                    log(f"{size}, {b} hits, {name}: {row['compute_median_ms']:.3f} ms, max error {maximum}")
+                    log(f"{name} at {size}, {b} bounces: compute median {row['compute_median_ms']:.3f} ms, "
+                        f"largest channel difference from backend 0: {maximum}")
                 results.append(case)
                 write_results()
     finally:

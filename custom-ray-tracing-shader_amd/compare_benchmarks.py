@@ -32,25 +32,25 @@ def case_key(case):
 
 def compare(before, after, log=None):
     """before/after: parsed results.json documents.  Returns the comparison document."""
-    _require(before["snapshot_sha256"] == after["snapshot_sha256"], "Scene snapshots differ")
-    _require(before["measured_passes"] == after["measured_passes"], "Measured pass counts differ")
-    _require(before["warmup_passes"] == after["warmup_passes"], "Warm-up pass counts differ")
-    _require(before["omp_threads"] == after["omp_threads"], "CPU worker configuration differs")
+    _require(before["snapshot_sha256"] == after["snapshot_sha256"], "the two runs rendered different scene snapshots (sha256 mismatch)")
+    _require(before["measured_passes"] == after["measured_passes"], "the two runs timed a different number of passes")
+    _require(before["warmup_passes"] == after["warmup_passes"], "the two runs used a different number of warm-up passes")
+    _require(before["omp_threads"] == after["omp_threads"], "the two runs used different OpenMP thread settings")
     old = {case_key(case): case for case in before["results"]}
-    _require(set(old) == {case_key(case) for case in after["results"]}, "Benchmark cases differ")
+    _require(set(old) == {case_key(case) for case in after["results"]}, "the two runs cover different (width, height, bounces, rays per pixel) cases")
     rows = []
     for case in after["results"]:
         previous = old[case_key(case)]
-        _require(len(previous["backends"]) == len(case["backends"]), "Backend lists differ")
+        _require(len(previous["backends"]) == len(case["backends"]), "a case lists a different number of backends in the two runs")
         for a, b in zip(previous["backends"], case["backends"]):
-            _require(a["backend"] == b["backend"], "Workload differs")
+            _require(a["backend"] == b["backend"], f"backend order changed: {a['backend']} vs {b['backend']}")
             if "unavailable" in a or "unavailable" in b:
-                _require("unavailable" in a and "unavailable" in b, "Backend availability differs: " + b["backend"])
+                _require("unavailable" in a and "unavailable" in b, f"backend {b['backend']} is available in one run only")
                 continue
             _require(a["samples_per_pixel"] == b["samples_per_pixel"] and a["cpu_threads"] == b["cpu_threads"],
-                     "Workload differs")
+                     f"backend {b['backend']} rendered a different workload (samples per pixel or CPU threads)")
             _require(a["raw_image_sha256"] == b["raw_image_sha256"],
-                     "Image changed: " + b["backend"] + " " + str(case_key(case)))
+                     f"output hash of backend {b['backend']} changed for case {case_key(case)}")
             rows.append({"width": case["width"], "height": case["height"], "bounces": case["bounces"],
                          "rays_per_pixel": case["rays_per_pixel"], "samples_per_pixel": b["samples_per_pixel"],
                          "backend": b["backend"], "before_compute_ms": a["compute_median_ms"],
@@ -59,9 +59,9 @@ def compare(before, after, log=None):
                          "before_call_ms": a["call_median_ms"], "after_call_ms": b["call_median_ms"],
                          "call_speedup": a["call_median_ms"] / b["call_median_ms"], "image_identical": True})
             if log:
-                log(f'{case["width"]}x{case["height"]}, {case["bounces"]} hits, {b["backend"]}: '
-                    f'{a["compute_median_ms"]:.3f} -> {b["compute_median_ms"]:.3f} ms '
-                    f'({a["compute_median_ms"] / b["compute_median_ms"]:.2f}x), image identical')
+                log(f'{b["backend"]} at {case["width"]}x{case["height"]}, {case["bounces"]} bounces: compute median '
+                    f'{a["compute_median_ms"]:.3f} ms before, {b["compute_median_ms"]:.3f} ms after '
+                    f'(x{a["compute_median_ms"] / b["compute_median_ms"]:.2f}); same output hash')
     return {"snapshot_sha256": before["snapshot_sha256"], "triangles": before["triangles"],
             "measured_passes": before["measured_passes"], "method": METHOD, "results": rows}
 

@@ -141,7 +141,7 @@ bool EmbreeScene::build(const std::vector<Triangle>& tris, std::string& error) {
     release();
     const auto t0 = std::chrono::steady_clock::now();
     device = api.newDevice(nullptr);
-    if (!device) { error = "Could not create an Embree CPU device"; return false; }
+    if (!device) { error = "rtcNewDevice returned no device"; return false; }
     scene = api.newScene(device);
     api.setSceneFlags(scene, kSceneFlagRobust);
     api.setSceneBuildQuality(scene, kBuildQualityHigh);
@@ -150,7 +150,7 @@ bool EmbreeScene::build(const std::vector<Triangle>& tris, std::string& error) {
     unsigned* idx = static_cast<unsigned*>(api.setNewGeometryBuffer(geom, kBufferIndex, 0, kFormatUInt3, 12, tris.size()));
     if (!v || !idx) {
         api.releaseGeometry(geom);
-        error = "Could not allocate Embree geometry";
+        error = "rtcNewGeometry returned no triangle geometry";
         return false;
     }
     // Vertices origin, origin + edge1, origin + edge2 (Native/embree.cuh:76-79).
@@ -170,7 +170,7 @@ bool EmbreeScene::build(const std::vector<Triangle>& tris, std::string& error) {
     api.releaseGeometry(geom);
     api.commitScene(scene);
     if (api.getDeviceError(device) != kErrorNone) {
-        error = "Embree scene build failed";
+        error = "rtcCommitScene reported an error while building the scene";
         return false;
     }
     buildMs = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();

@@ -88,14 +88,14 @@ def test_compare_reproduces_reference_comparison():
 
 
 @pytest.mark.parametrize("mutate,message", [
-    (lambda b, a: a["results"][1]["backends"][2].__setitem__("raw_image_sha256", "0" * 64), "Image changed: CUDA GPU"),
-    (lambda b, a: a.__setitem__("snapshot_sha256", "f" * 64), "Scene snapshots differ"),
-    (lambda b, a: a.__setitem__("measured_passes", 6), "Measured pass counts differ"),
-    (lambda b, a: a.__setitem__("warmup_passes", 2), "Warm-up pass counts differ"),
-    (lambda b, a: a.__setitem__("omp_threads", "8"), "CPU worker configuration differs"),
-    (lambda b, a: a["results"][0].__setitem__("bounces", 4), "Benchmark cases differ"),
-    (lambda b, a: a["results"][0]["backends"][0].__setitem__("samples_per_pixel", 7), "Workload differs"),
-    (lambda b, a: a["results"][0]["backends"][1].__setitem__("cpu_threads", 4), "Workload differs"),
+    (lambda b, a: a["results"][1]["backends"][2].__setitem__("raw_image_sha256", "0" * 64), "output hash of backend CUDA GPU changed"),
+    (lambda b, a: a.__setitem__("snapshot_sha256", "f" * 64), "different scene snapshots"),
+    (lambda b, a: a.__setitem__("measured_passes", 6), "different number of passes"),
+    (lambda b, a: a.__setitem__("warmup_passes", 2), "different number of warm-up passes"),
+    (lambda b, a: a.__setitem__("omp_threads", "8"), "different OpenMP thread settings"),
+    (lambda b, a: a["results"][0].__setitem__("bounces", 4), "cover different"),
+    (lambda b, a: a["results"][0]["backends"][0].__setitem__("samples_per_pixel", 7), "rendered a different workload"),
+    (lambda b, a: a["results"][0]["backends"][1].__setitem__("cpu_threads", 4), "rendered a different workload"),
 ])
 def test_compare_rejects_every_mismatch(mutate, message):
     before, after, _ = _runs()
@@ -112,7 +112,7 @@ def test_compare_carries_unavailable_backends():
             case["backends"].append({"backend": "Embree CPU multi", "unavailable": "not installed"})
     assert compare(before, after) == expected
     after["results"][0]["backends"][-1] = dict(before["results"][0]["backends"][0], backend="Embree CPU multi")
-    with pytest.raises(ComparisonError, match="availability"):
+    with pytest.raises(ComparisonError, match="available in one run only"):
         compare(before, after)
 
 
