@@ -704,6 +704,22 @@ __device__ __forceinline__ void loadSobolTerms(unsigned* sShift, unsigned* sTerm
     if (tid < R * K) sTerm[tid] = grayTerm(gSobolDirs, dimBase + (tid % K), sample0 + tid / K);
     __syncthreads();
 }
+// The same for a number of dimensions known at launch: shifts of dimensions [dimBase,
+// dimBase + dims), terms of samples sample0 .. sample0 + R - 1 at sTerm[r * dims + k].
+__device__ __forceinline__ void loadSobolTermsN(unsigned* sShift, unsigned* sTerm, unsigned seed, unsigned dimBase,
+                                                int dims, unsigned sample0, int R) {
+    for (int i = threadIdx.x; i < dims; i += blockDim.x) sShift[i] = dimShift(seed, dimBase + (unsigned)i);
+    for (int i = threadIdx.x; i < R * dims; i += blockDim.x)
+        sTerm[i] = grayTerm(gSobolDirs, dimBase + (unsigned)(i % dims), sample0 + (unsigned)(i / dims));
+    __syncthreads();
+}
+// Direction-number rows [dimBase, dimBase + rows) of the table in LDS, for launches whose lanes
+// sample different (seed, sample) pairs (batched views): a lane forms its own shift and term
+// from them.
+__device__ __forceinline__ void loadSobolRows(unsigned* sDirs, unsigned dimBase, int rows) {
+    for (int i = threadIdx.x; i < rows * 32; i += blockDim.x) sDirs[i] = gSobolDirs[dimBase * 32 + (unsigned)i];
+    __syncthreads();
+}
 
 // Bijection on [0, w*h): queue position -> pixel, visiting 8x8 tiles first (each wave of
 // primary rays covers one compact tile), then the right and bottom remainder strips.
@@ -889,6 +905,11 @@ __global__ __launch_bounds__(kBlock, tailBlocksPerCU<S>()) void pathTailKernel(F
     const float4* recs = reinterpret_cast<const float4*>(a.recs);
     LdsStack<S> st{sRef, sDist, reinterpret_cast<int2*>(a.spill), a.spillEntries, a.totalLanes,
                    (int)(blockIdx.x * kBlock + tid), tid};
+    // Sobol shifts of the launch's dimensions (bounces firstBounce .. maxBounces - 1, 8 each),
+    // then their terms for the chunk's R samples; sized at launch (tailSobolBytes).
+    extern __shared__ unsigned sSobol[];
+    const int tailDims = 8 * (f.rs.maxBounces - firstBounce);
+    loadSobolTermsN(sSobol, sSobol + tailDims, f.rs.seed, 4 + 8 * (unsigned)firstBounce, tailDims, f.sample0, f.R);
     TraceCounters tc;
     bool overflow = false;
     LaneRay L;
@@ -937,8 +958,9 @@ __global__ __launch_bounds__(kBlock, tailBlocksPerCU<S>()) void pathTailKernel(F
         if (active) {
             RtHit hit;
             hit.index = L.bestIdx; hit.distance = L.best; hit.u = L.bu; hit.v = L.bv;
-            const unsigned sample = f.sample0 + (unsigned)r, dimBase = 4 + 8 * (unsigned)bounce;
-            auto rng = [&](int k) { return sobolSample(gSobolDirs, pixel, sample, f.rs.seed, dimBase + (unsigned)k); };
+            const int kb = 8 * (bounce - firstBounce);
+            const unsigned* terms = sSobol + tailDims + r * tailDims + kb;
+            auto rng = [&](int k) { return sobolFromParts(pixel, sSobol[kb + k], terms[k]); };
             FirstHit unused;   // first-hit outputs belong to bounce 0
             shadeBounce(f.sc, f.rs, bounce, rng, L.o, L.d, hit, ps, &unused, res);
         }
@@ -994,12 +1016,14 @@ static hipError_t launchTailNS(bool stats, int blocks, int cus, const FrameArgs&
                                hipStream_t s) {
     const void* k = stats ? reinterpret_cast<const void*>(&pathTailKernel<N, true, S>)
                           : reinterpret_cast<const void*>(&pathTailKernel<N, false, S>);
+    if (firstBounce < 1 || firstBounce >= f.rs.maxBounces || f.R < 1 || f.R > kMaxRaysPerPixel) return hipErrorInvalidValue;
+    const size_t sobolBytes = (size_t)8 * (f.rs.maxBounces - firstBounce) * (1 + f.R) * sizeof(unsigned);
     int perCU = 0;
-    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, k, kBlock, 0);
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, k, kBlock, sobolBytes);
     if (e != hipSuccess) return e;
     blocks = std::max(1, std::min(blocks, perCU * cus));
-    if (stats) hipLaunchKernelGGL((pathTailKernel<N, true, S>), dim3(blocks), dim3(kBlock), 0, s, f, a, firstBounce);
-    else hipLaunchKernelGGL((pathTailKernel<N, false, S>), dim3(blocks), dim3(kBlock), 0, s, f, a, firstBounce);
+    if (stats) hipLaunchKernelGGL((pathTailKernel<N, true, S>), dim3(blocks), dim3(kBlock), sobolBytes, s, f, a, firstBounce);
+    else hipLaunchKernelGGL((pathTailKernel<N, false, S>), dim3(blocks), dim3(kBlock), sobolBytes, s, f, a, firstBounce);
     return hipGetLastError();
 }
 template <class N>
@@ -1066,6 +1090,8 @@ hipError_t launchAccumulate(const AccumArgs& a, hipStream_t s) {
 // The same stages as the single-view frame, with each path slot's view looked up from its
 // slot; Sobol values are formed per lane (views have their own seeds and sample counts).
 __global__ __launch_bounds__(kBlock) void batchRaygenKernel(BatchArgs b) {
+    __shared__ unsigned sDirs[4 * 32];   // direction numbers of the camera dimensions 0..3
+    loadSobolRows(sDirs, 0, 4);
     const int l = blockIdx.x * kBlock + threadIdx.x;   // position in this pipeline's queue
     if (l >= b.qCount) return;
     const int q = b.qBase + l;                          // position in the frame's order
@@ -1075,7 +1101,7 @@ __global__ __launch_bounds__(kBlock) void batchRaygenKernel(BatchArgs b) {
     const int slot = V.slotBase + (int)pixel;
     const int x = (int)(pixel % (unsigned)V.w), y = (int)(pixel / (unsigned)V.w);
     const unsigned sample = b.samples[v];
-    auto rng = [&](int k) { return sobolSample(gSobolDirs, pixel, sample, V.rs.seed, (unsigned)k); };
+    auto rng = [&](int k) { return sobolFromParts(pixel, dimShift(V.rs.seed, (unsigned)k), grayTerm(sDirs, (unsigned)k, sample)); };
     f3 o, d;
     cameraRay(V.cam, V.rs, x, y, V.w, V.h, rng, o, d);
     b.thr[slot] = make_float4(1, 1, 1, 0);
@@ -1093,6 +1119,9 @@ hipError_t launchBatchRaygen(const BatchArgs& b, hipStream_t s) {
 
 __global__ __launch_bounds__(kBlock, kShadeBlocksPerCU) void batchShadeKernel(BatchArgs b) {
     __shared__ AppendLds sApp;
+    __shared__ unsigned sDirs[8 * 32];   // direction numbers of this bounce's dimensions
+    const unsigned dimBase = 4 + 8 * (unsigned)b.bounce;
+    loadSobolRows(sDirs, dimBase, 8);
     const int count = b.counters[kCntRays + b.bounce];
     int round = 0;
     for (int base = blockIdx.x * kBlock; base < count; base += gridDim.x * kBlock, round++) {
@@ -1106,7 +1135,6 @@ __global__ __launch_bounds__(kBlock, kShadeBlocksPerCU) void batchShadeKernel(Ba
             const BatchView& V = b.views[v];
             const unsigned pixel = (unsigned)(slot - V.slotBase);
             const unsigned sample = b.samples[v];
-            const unsigned dimBase = 4 + 8 * (unsigned)b.bounce;
             const int4 hv = b.hits[i];
             RtHit hit;
             hit.index = hv.x; hit.distance = __int_as_float(hv.y); hit.u = __int_as_float(hv.z); hit.v = __int_as_float(hv.w);
@@ -1114,7 +1142,9 @@ __global__ __launch_bounds__(kBlock, kShadeBlocksPerCU) void batchShadeKernel(Ba
             PathState ps;
             ps.thr = F3(t4.x, t4.y, t4.z); ps.prevPdf = t4.w; ps.rad = F3(q4.x, q4.y, q4.z);
             FirstHit fh; fh.covered = false;
-            auto rng = [&](int k) { return sobolSample(gSobolDirs, pixel, sample, V.rs.seed, dimBase + (unsigned)k); };
+            auto rng = [&](int k) {
+                return sobolFromParts(pixel, dimShift(V.rs.seed, dimBase + (unsigned)k), grayTerm(sDirs, (unsigned)k, sample));
+            };
             ShadeResult res;
             shadeBounce(V.sc, V.rs, b.bounce, rng, F3(r0.x, r0.y, r0.z), F3(r1.x, r1.y, r1.z), hit, ps, &fh, res);
             b.thr[slot] = make_float4(ps.thr.x, ps.thr.y, ps.thr.z, ps.prevPdf);
@@ -1322,10 +1352,12 @@ hipError_t launchCopy(const float4* in, float4* out, long long n, int blocks, hi
 // ------------------------------------------------------------------ Sobol batch
 __global__ __launch_bounds__(kBlock) void sobolBatchKernel(const unsigned* pixels, const unsigned* samples, int count,
                                                            unsigned seed, int dims, float* out) {
+    __shared__ unsigned sDirs[kSobolDims * 32];   // the requested dimensions' direction numbers
+    loadSobolRows(sDirs, 0, dims);
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= count) return;
     const unsigned p = pixels[i], s = samples[i];
-    for (int d = 0; d < dims; d++) out[(size_t)i * dims + d] = sobolSample(gSobolDirs, p, s, seed, (unsigned)d);
+    for (int d = 0; d < dims; d++) out[(size_t)i * dims + d] = sobolSample(sDirs, p, s, seed, (unsigned)d);
 }
 hipError_t launchSobolBatch(const unsigned* pixels, const unsigned* samples, int count, unsigned seed, int dims,
                             float* out, hipStream_t s) {

@@ -234,3 +234,31 @@ def test_denoise_ensemble_passes(lib):
     assert out[4][2] > out[2][2] > out[1][2] > 0
     for e in (2, 4):
         assert out[e][1] > out[1][1] - 0.5, (e, out[e][1], out[1][1])
+
+
+@pytest.mark.skipif(not WEIGHTS.exists(), reason="trained weights not built (train_denoiser.py)")
+def test_variance_check_rejects_corrupted_network(lib):
+    """With the per-tile check on (denoise_fallback 0), the trained network passes most tiles of
+    a Cornell frame; the same network made about ten times too bright (ln 10 added to its three
+    output biases: the output is a residual in log space) fails every tile, and the frame is
+    exactly the plain accumulation, as the reference shows the raw image when its denoiser
+    cannot run (UnityRayTracer/Assets/RayTracing/RayTraceSession.cs:213-217)."""
+    tris, camf, st = _scene()
+    w, h = 96, 80
+    cam = camf(w, h)
+    with Denoiser(lib, weights=WEIGHTS) as net, Session(lib) as s:
+        s.initialize(tris, w, h)
+        lib.RtSetDenoiser(s.handle, net.handle)
+        assert s.get_option("denoise_fallback") == 0
+        plain, _ = s.render(cam, st)
+        good, _ = s.denoise()
+        tiles, rejected, _ = s.denoise_stats()
+        assert tiles == 6 * 5 and rejected < tiles // 4, rejected
+        assert not np.array_equal(good, plain)
+        params = net.params
+        params[-32:-29] += np.log(10.0)   # the output layer's biases (32 padded channels, 3 used)
+        net.params = params
+        bad, _ = s.denoise()
+        tiles, rejected, _ = s.denoise_stats()
+        assert rejected == tiles, (rejected, tiles)
+        assert np.array_equal(bad, plain)

@@ -86,8 +86,20 @@ def test_soup_queries_vs_oracle_and_fp64(lib, width, quantized):
         both = (gpu["index"] >= 0) & (idx64 >= 0) & (gpu["index"] == idx64)
         assert both.sum() > 100
         rel = np.abs(gpu["distance"][both] - t64[both]) / np.maximum(1, np.abs(t64[both]))
-        # FP32 Moller-Trumbore vs FP64: within 1e-5 relative except at grazing angles.
-        assert np.quantile(rel, 0.99) <= 1e-5
+        # FP32 Moller-Trumbore (the reference's arithmetic, kept bit for bit) vs FP64: within 1e-5
+        # relative for every hit more than 11.5 degrees off the triangle's plane; grazing hits lose
+        # precision as det -> 0, and their share and worst error are reported.
+        t = tris[gpu["index"][both]]
+        n = np.cross(np.stack([t["edge1"][a] for a in "xyz"], 1).astype(np.float64),
+                     np.stack([t["edge2"][a] for a in "xyz"], 1).astype(np.float64))
+        n /= np.linalg.norm(n, axis=1, keepdims=True)
+        cosine = np.abs((n * rays[:3000]["direction"][both].astype(np.float64)).sum(1))
+        steep = cosine > np.sin(np.radians(11.5))
+        assert (rel[steep] <= 1e-5).all(), rel[steep].max()
+        over = rel > 1e-5
+        print(f"W{width} q{quantized}: {both.sum()} hits, {over.sum()} beyond 1e-5 relative of FP64 "
+              f"({over.mean():.2%}, all within {np.degrees(np.arcsin(cosine[over].max())) if over.any() else 0:.1f} "
+              f"deg of grazing), worst {rel.max():.2e}")
         # Index disagreements with FP64 only where FP64 distances are within FP32 resolution.
         diff = (gpu["index"] != idx64) & (gpu["index"] >= 0) & (idx64 >= 0)
         assert diff.mean() < 1e-3
@@ -180,3 +192,88 @@ def test_gpu_build_soup_matches_reference_12_bins(lib):
         nodes, order = s.bvh(), s.bvh_order(len(tris))
     assert len(nodes) == len(ref_nodes)
     assert _leaf_sets(nodes, order) == _leaf_sets(ref_nodes, ref_order)
+
+
+def _mirror_room():
+    """A room whose paths avoid every transcendental: the camera sees only a mirror floor
+    (reflection is exact arithmetic; its shading normals vary across each triangle, so the
+    hit's u, v shape the path), the reflected ray ends on a Lambertian wall or ceiling where
+    the last bounce's next-event estimate uses only +, *, / and sqrt, or on an emitter."""
+    from scenes import quad
+    s, top = 3.0, 3.0
+    floor = quad((-s, 0, -s), (-s, 0, s), (s, 0, s), (s, 0, -s), color=(.5, .5, .5), specular=(.9, .85, .8),
+                 smoothness=1.0, specular_probability=1.0)
+    tilts = np.array([[.15, 1, 0], [0, 1, .2], [-.1, 1, -.1]], np.float64)
+    tilts /= np.linalg.norm(tilts, axis=1, keepdims=True)
+    for k, name in enumerate(("normal0", "normal1", "normal2")):
+        for a, axis in enumerate("xyz"):
+            floor[name][axis] = tilts[k, a]
+    parts = [floor,
+             quad((-s, top, -s), (s, top, -s), (s, top, s), (-s, top, s), color=(.7, .7, .7)),
+             quad((-s, 0, s), (-s, top, s), (s, top, s), (s, 0, s), color=(.6, .6, .75)),
+             quad((s, 0, -s), (s, top, -s), (-s, top, -s), (-s, 0, -s), color=(.7, .6, .5)),
+             quad((-s, 0, -s), (-s, top, -s), (-s, top, s), (-s, 0, s), color=(.65, .2, .15)),
+             quad((s, 0, s), (s, top, s), (s, top, -s), (s, 0, -s), color=(.15, .45, .2)),
+             quad((-.6, top - .01, -.6), (.6, top - .01, -.6), (.6, top - .01, .6), (-.6, top - .01, .6),
+                  color=(1, 1, 1), emission=8.0, one_sided=1),
+             quad((1.5, top - .01, 1.0), (2.5, top - .01, 1.0), (2.5, top - .01, 2.0), (1.5, top - .01, 2.0),
+                  color=(1, .8, .6), emission=5.0, one_sided=1)]
+    return np.concatenate(parts)
+
+
+@pytest.mark.parametrize("tail,pipes", [(0, 1), (1, 2), (0, 4)])
+def test_sobol_production_paths_bit_exact_through_images(lib, tail, pipes):
+    """The samplers the frame kernels use (per-launch shifts and Gray-code terms in LDS for the
+    camera, each bounce's shading and the path-completion kernel) give the oracle's Sobol values
+    bit for bit: on a scene whose path arithmetic is exact, 3 calls of 16 samples accumulate
+    to the oracle's sums bit for bit, with the same ray counts."""
+    from helpers import cam_record, settings_record
+    from scenes import make_camera, make_settings
+    tris = _mirror_room()
+    w = h = 128
+    cam = make_camera((0, 2.2, -.4), (0, 0, .5), fov_deg=45, aspect=1.0)
+    st = make_settings(2, rpp=16, seed=77)
+    sc = oracle.OracleScene(tris)
+    total, sample, ref_counts = None, 0, [0, 0]
+    with Session(lib) as s:
+        assert s.set_option("tail_bounce", tail) and s.set_option("pipelines", pipes)
+        s.initialize(tris, w, h)
+        counts = [0, 0]
+        for _ in range(3):
+            s.render(cam, st, 2)
+            c, sh = s.ray_counts()
+            counts = [counts[0] + c, counts[1] + sh]
+            _, total, (rc, rs) = sc.render(cam_record(cam), settings_record(st), w, h, sample, total)
+            sample += st.raysPerPixel
+            ref_counts = [ref_counts[0] + rc, ref_counts[1] + rs]
+        acc = s.accumulation()
+    assert (total[..., 9] == 48).all()                       # every path starts on the mirror floor
+    assert np.allclose(total[..., 3:6] / 48, (.9, .85, .8), rtol=1e-5)
+    assert (total[..., 0:3] > 0).mean() > 0.2                 # walls lit by next-event estimation
+    assert np.array_equal(acc.view(np.uint32), total.view(np.uint32))
+    assert counts == ref_counts
+
+
+def test_batch_sobol_bit_exact_through_images(lib):
+    """The batch kernels' samplers (direction rows in LDS, per-view seed and sample) on the same
+    exact-arithmetic scene: each view's accumulation over 8 frames equals the oracle's sums."""
+    from helpers import cam_record, settings_record
+    from native_api import Batch
+    from scenes import make_camera, make_settings
+    tris = _mirror_room()
+    views = [(96, 80, make_camera((0, 2.2, -.4), (0, 0, .5), fov_deg=45, aspect=96 / 80), make_settings(2, seed=5)),
+             (64, 64, make_camera((.3, 2.0, .2), (0, 0, .4), fov_deg=40), make_settings(2, seed=123456))]
+    frames = 8
+    with Batch(lib) as b:
+        for w, h, cam, st in views:
+            b.add_view(tris, w, h, cam, st)
+        b.build()
+        b.render(frames)
+        got = [b.read(v)[1] for v in range(len(views))]
+    sc = oracle.OracleScene(tris)
+    for v, (w, h, cam, st) in enumerate(views):
+        total = None
+        for sample in range(frames):
+            _, total, _ = sc.render(cam_record(cam), settings_record(st), w, h, sample, total)
+        assert (total[..., 9] == frames).all()
+        assert np.array_equal(got[v].view(np.uint32), total.view(np.uint32)), v
