@@ -42,8 +42,9 @@ for p in (str(ROOT), str(ROOT / "custom-ray-tracing-shader_amd")):
         sys.path.insert(0, p)
 
 HBM_PEAK_GBS = 8000.0
-# Launch configurations chosen by autotune.py on an MI355X for each workload (bench.py
-# --autotune repeats the search); the BVH width also fixes the node size below.  Leaf size
+# Launch configurations chosen by the library's autotuner (RtAutotune / RtBatchAutotune) on an
+# MI355X for each workload (bench.py --autotune repeats the search); the BVH width also fixes
+# the node size below.  Leaf size
 # changes the work per ray, never the hits or images: the soup runs leaf <= 2 (its
 # configuration's leaf <= 4 renders identical frames, tests/test_gpu_configs.py).
 TUNED = {
@@ -73,7 +74,8 @@ def bench_batch(args):
         t0 = time.perf_counter()
         best, trials = autotune.autotune_batch(views)
         cfg = {k: v for k, v in best.items() if k != "mrays_per_s"}
-        log(f"autotune: {len(trials)} batch configurations in {time.perf_counter() - t0:.1f} s, best {best}")
+        log(f"autotune (library search): {trials} batch configurations in {time.perf_counter() - t0:.1f} s, "
+            f"best {best}")
     b = native_api.Batch(**cfg)
     for t, w, h, cam, st in views:
         b.add_view(t, w, h, cam, st)
@@ -393,10 +395,9 @@ def main():
     if args.autotune:
         import autotune
         t0 = time.perf_counter()
-        best, trials = autotune.autotune(tris, cam, st, w, h, use_cache=False,
-                                         max_leaf=(2, 4) if args.workload == "soup" else (2, 3, 4))
+        best, trials = autotune.autotune(tris, cam, st, w, h)
         cfg = {k: v for k, v in best.items() if k != "mrays_per_s"}
-        log(f"autotune: {len(trials)} configurations in {time.perf_counter() - t0:.1f} s, best {best}")
+        log(f"autotune (library search): {trials} configurations in {time.perf_counter() - t0:.1f} s, best {best}")
     s = native_api.Session()
     for k in ("bvh_width", "bvh_quantized", "build_max_leaf", "blocks_per_cu", "short_stack", "refill", "refill_below",
               "while_while", "pipelines", "tail_bounce", "xcd_segments"):

@@ -16,6 +16,7 @@
 #include "bvh_gpu.h"
 #include "denoiser.h"
 #include "embree_cpu.h"
+#include "tuning.h"
 
 using namespace rt;
 
@@ -81,6 +82,9 @@ struct Session {
     int tailBounce = 0;          // >= 1: paths complete from this bounce on in one launch (0: off)
     int nontemporal = 1;         // rays / hits bypass cache retention in traversal
     int xcdSegments = 1;         // per-XCD segment cursors in the frame's traversal launches
+    unsigned userSet = 0;        // tuned options the caller set (bit k: kTunedFields[k])
+    int autotuneCache = 1;       // 1: apply the autotuner's cached launch options the caller did not set
+    int tunedBounces = -1;       // bounce count the cached launch options were last looked up for
     std::vector<unsigned char> hostNodes;
     std::vector<TriRecord> hostRecs;
     std::vector<HostSum> hostSum;
@@ -493,6 +497,36 @@ bool renderGpu(Session* s, const CameraData& cam, const RenderSettings& rs, floa
     return true;
 }
 
+int setOption(Session* s, const char* key, int value);
+
+// Options of o that the caller did not set (with build = false, only the launch options;
+// the BVH layout is fixed at RtInitialize).
+void applyTuned(Session* s, const LaunchOptions& o, bool build) {
+    for (int k = build ? 0 : 2; k < 8; k++) {
+        const int v = o.*(kTunedFields[k].field);
+        if (v >= 0 && !(s->userSet & (1u << k))) (void)setOption(s, kTunedFields[k].name, v);
+    }
+}
+
+// Before a frame: the cached launch options of this scene class and bounce count, once per
+// bounce count.
+void applyTunedLaunch(Session* s, int maxBounces) {
+    if (!s->autotuneCache || s->tunedBounces == maxBounces) return;
+    s->tunedBounces = maxBounces;
+    LaunchOptions o;
+    if (tuneLookup(sessionTuneKey((long long)s->tris.size(), s->width, s->height, maxBounces), false, o))
+        applyTuned(s, o, false);
+}
+
+// Rebuilds the session's BVH with another width or leaf size (the autotuner's layout trials).
+bool rebuildAccel(Session* s, int width, int maxLeaf) {
+    devFree(s->dNodes); devFree(s->dRecs); devFree(s->dBinNodes); devFree(s->dOrder); devFree(s->dSpill);
+    s->spillLanes = 0;
+    s->hostNodes.clear(); s->hostRecs.clear(); s->binNodes.clear(); s->binOrder.clear();
+    s->bvhWidth = width; s->buildMaxLeaf = maxLeaf;
+    return buildAccel(s) && prepareTraceGrid(s);
+}
+
 bool validRender(Session* s, const CameraData* camera, const RenderSettings* settings) {
     return camera && settings && s->ready && settings->raysPerPixel >= 1 && settings->raysPerPixel <= kMaxRaysPerPixel &&
            settings->maxBounces >= 1 && settings->maxBounces <= kMaxBounces && settings->exposure > 0 &&
@@ -640,6 +674,11 @@ int RtInitialize(void* p, const Triangle* triangles, int count, int w, int h) {
     s->device = device;
     s->tris.assign(triangles, triangles + count);
     s->width = w; s->height = h;
+    s->tunedBounces = -1;
+    if (s->autotuneCache) {   // the autotuner's choice for this scene class (any bounce count)
+        LaunchOptions o;
+        if (tuneLookup(sessionTunePrefix(count, w, h), true, o)) applyTuned(s, o, true);
+    }
     s->hostSum.assign((size_t)w * h, HostSum{});
     // Native/renderer.cu:345-353: linear colours, emitter CDF by FP32 running area sum.
     for (int i = 0; i < count; i++) {
@@ -713,6 +752,7 @@ int RtRender(void* p, const CameraData* camera, const RenderSettings* settings, 
         RtReset(s);
     s->lastBackend = backend; s->lastCamera = *camera; s->lastSettings = *settings; s->cpuThreads = 0;
     if (backend == RT_BACKEND_GPU) {
+        applyTunedLaunch(s, settings->maxBounces);
         if (!renderGpu(s, *camera, *settings, computeMs)) return 0;
         s->error = hipMemcpy(output, s->dPixels, (size_t)s->width * s->height * 4, hipMemcpyDeviceToHost);
         if (s->error != hipSuccess) return 0;
@@ -755,9 +795,72 @@ int RtRenderResident(void* p, const CameraData* camera, const RenderSettings* se
         std::memcmp(settings, &s->lastSettings, sizeof(RenderSettings)))
         RtReset(s);
     s->lastBackend = RT_BACKEND_GPU; s->lastCamera = *camera; s->lastSettings = *settings; s->cpuThreads = 0;
+    applyTunedLaunch(s, settings->maxBounces);
     if (!renderGpu(s, *camera, *settings, computeMs)) return 0;
     s->samples += (unsigned)settings->raysPerPixel;
     s->error = hipSuccess;
+    return 1;
+}
+
+int RtAutotune(void* p, const CameraData* camera, const RenderSettings* settings, int frames, float* mraysPerSecond,
+               int* trialCount) {
+    Session* s = static_cast<Session*>(p);
+    if (!s) return 0;
+    if (frames < 1 || !validRender(s, camera, settings)) { s->error = hipErrorInvalidValue; return 0; }
+    s->hostError.clear();
+    TuneSpace space;
+    // Options the caller set stay fixed: the search covers the others.
+    auto fixed = [&](int k, int v) { return (s->userSet & (1u << k)) ? std::vector<int>{v} : std::vector<int>(); };
+    if (!fixed(0, s->builtWidth).empty()) space.widths = fixed(0, s->builtWidth);
+    if (!fixed(1, s->buildMaxLeaf).empty()) space.leaves = fixed(1, s->buildMaxLeaf);
+    if (!fixed(2, s->blocksPerCU).empty()) space.blocks = fixed(2, s->blocksPerCU);
+    if (!fixed(3, s->shortStack).empty()) space.stacks = fixed(3, s->shortStack);
+    if (s->userSet & (3u << 4)) space.loops = {{s->whileWhile, s->refill}};
+    if (!fixed(6, s->pipelines).empty()) space.pipelines = fixed(6, s->pipelines);
+    if (!fixed(7, s->tailBounce).empty()) space.tails = fixed(7, s->tailBounce);
+    bool failed = false;
+    auto apply = [&](const LaunchOptions& c) {
+        if (c.bvhWidth != s->builtWidth || c.buildMaxLeaf != s->buildMaxLeaf || !s->dNodes)
+            if (!rebuildAccel(s, c.bvhWidth, c.buildMaxLeaf)) return false;
+        (void)setOption(s, "blocks_per_cu", c.blocksPerCU);
+        (void)setOption(s, "short_stack", c.shortStack);
+        (void)setOption(s, "while_while", c.whileWhile);
+        (void)setOption(s, "refill", c.refill);
+        (void)setOption(s, "pipelines", c.pipelines);
+        (void)setOption(s, "tail_bounce", c.tailBounce);
+        return true;
+    };
+    auto measure = [&](const LaunchOptions& c) -> double {
+        if (failed) return -1;
+        if (!apply(c)) { failed = true; return -1; }
+        float ms = 0;
+        RtReset(s);
+        if (!renderGpu(s, *camera, *settings, &ms)) { failed = true; return -1; }   // warm: buffers, caches
+        double best = 0;
+        for (int rep = 0; rep < 2; rep++) {
+            unsigned long long rays = 0;
+            const auto t0 = std::chrono::steady_clock::now();
+            for (int f = 0; f < frames; f++) {
+                if (!renderGpu(s, *camera, *settings, &ms)) { failed = true; return -1; }
+                rays += s->lastClosest + s->lastShadow;
+            }
+            best = std::max(best, rays / std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() / 1e6);
+        }
+        return best;
+    };
+    int trials = 0;
+    const LaunchOptions best = tuneSearch(space, settings->maxBounces, measure, &trials);
+    RtReset(s);
+    s->hasFrame = false;
+    if (failed || best.mraysPerSecond <= 0 || !apply(best)) {
+        if (s->hostError.empty()) s->hostError = "autotune trial failed";
+        return 0;
+    }
+    s->tunedBounces = settings->maxBounces;
+    if (!tuneStore(sessionTuneKey((long long)s->tris.size(), s->width, s->height, settings->maxBounces), best))
+        s->hostError = "autotune: could not write " + tuneCachePath();
+    if (mraysPerSecond) *mraysPerSecond = (float)best.mraysPerSecond;
+    if (trialCount) *trialCount = trials;
     return 1;
 }
 
@@ -1174,9 +1277,14 @@ int RtConv3x3Backward(const float* in, int n, int h, int w, int cin, const float
     return convBackwardHost(in, n, h, w, cin, weight, gradOut, cout, gradIn, gradWeight, gradBias) == hipSuccess;
 }
 
-int RtSetOption(void* p, const char* key, int value) {
-    Session* s = static_cast<Session*>(p);
-    if (!s || !key) return 0;
+}  // extern "C"
+
+namespace {
+int setOption(Session* s, const char* key, int value) {
+    if (!std::strcmp(key, "autotune_cache")) {
+        if (value != 0 && value != 1) return 0;
+        s->autotuneCache = value; return 1;
+    }
     if (!std::strcmp(key, "bvh_width")) {
         if (value != 2 && value != 4 && value != 8) return 0;
         s->bvhWidth = value; return 1;
@@ -1251,6 +1359,18 @@ int RtSetOption(void* p, const char* key, int value) {
     }
     return 0;
 }
+}  // namespace
+
+extern "C" {
+
+int RtSetOption(void* p, const char* key, int value) {
+    Session* s = static_cast<Session*>(p);
+    if (!s || !key) return 0;
+    if (!setOption(s, key, value)) return 0;
+    for (int k = 0; k < 8; k++)
+        if (!std::strcmp(key, kTunedFields[k].name)) s->userSet |= 1u << k;
+    return 1;
+}
 
 int RtGetOption(void* p, const char* key) {
     Session* s = static_cast<Session*>(p);
@@ -1275,6 +1395,7 @@ int RtGetOption(void* p, const char* key) {
     if (!std::strcmp(key, "nontemporal")) return s->nontemporal;
     if (!std::strcmp(key, "stage_timing")) return s->stageTiming ? 1 : 0;
     if (!std::strcmp(key, "trace_stats")) return s->traceStats ? 1 : 0;
+    if (!std::strcmp(key, "autotune_cache")) return s->autotuneCache;
     return -1;
 }
 

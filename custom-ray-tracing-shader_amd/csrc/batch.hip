@@ -11,6 +11,7 @@
 #include <string>
 #include <vector>
 #include "bvh_gpu.h"
+#include "tuning.h"
 #include "kernels.h"
 
 using namespace rt;
@@ -55,6 +56,8 @@ struct Batch {
     std::string error;
     int width = 4, quantized = 0, blocksPerCU = 3, shortStack = 12, refill = 1, refillBelow = 40, whileWhile = 1,
         useGraph = 1, pipelines = 3, xcdSegments = 1;
+    unsigned userSet = 0;             // tuned options the caller set (bit k: kTunedFields[k])
+    int autotuneCache = 1;            // 1: RtBatchBuild applies cached launch options the caller did not set
     bool built = false;
     int cus = 0, views = 0, totalSlots = 0, maxBounces = 0;
     long long totalTris = 0;
@@ -147,6 +150,8 @@ int* xcdCursors(Batch* b, int p, int k) {
 bool enqueueFrame(Batch* b) {
     hipStream_t st = b->stream;
     const int P = framePipelines(b);
+    // A frame starts with a clear overflow flag (a fill kernel: the frame may be a captured graph).
+    B_TRY(launchZeroInts(b->dOverflow, 1, st));
     hipStream_t pst[kBatchPipelines] = {st, b->pipeStream[0], b->pipeStream[1], b->pipeStream[2]};
     if (P > 1) {
         B_TRY(hipEventRecord(b->evFork, st));
@@ -198,6 +203,64 @@ bool enqueueFrame(Batch* b) {
     B_TRY(launchBatchAccumulate(a[0], st));
     B_TRY(launchBatchAdvance(a[0], st));
     return true;
+}
+
+// The traversal grid and spill rings for the current launch options, and the frame's graph.
+bool prepareLaunch(Batch* b) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr; b->graph = nullptr;
+    dfree(b->dSpill);
+    b->traceBlocks = b->cus * b->blocksPerCU;
+    b->spillEntries = 36 * (b->width - 1);
+    if (!dalloc(b, b->dSpill, (size_t)kBatchPipelines * b->spillEntries * b->traceBlocks * kBlock * 2)) return false;
+    if (b->useGraph) {
+        B_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        const bool ok = enqueueFrame(b);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(b->stream, &g);
+        if (!ok) { if (g) (void)hipGraphDestroy(g); return false; }
+        B_TRY(e);
+        b->graph = g;
+        B_TRY(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    }
+    return true;
+}
+
+// Scene class of the pending views (the autotuner's cache key).
+std::string batchKey(const Batch* b) {
+    long long tris = 0, px = 0;
+    int bounces = 0;
+    for (const PendingView& pv : b->pending) {
+        tris += (long long)pv.tris.size();
+        px += (long long)pv.w * pv.h;
+        bounces = std::max(bounces, pv.rs.maxBounces);
+    }
+    return batchTuneKey((int)b->pending.size(), tris, px, bounces);
+}
+
+int setBatchOption(Batch* b, const char* key, int value) {
+    if (!std::strcmp(key, "bvh_width") && (value == 2 || value == 4 || value == 8)) { b->width = value; return 1; }
+    if (!std::strcmp(key, "bvh_quantized") && (value == 0 || value == 1)) { b->quantized = value; return 1; }
+    if (!std::strcmp(key, "blocks_per_cu") && value >= 1 && value <= 32) { b->blocksPerCU = value; return 1; }
+    if (!std::strcmp(key, "short_stack") && (value == 8 || value == 12 || value == 16)) { b->shortStack = value; return 1; }
+    if (!std::strcmp(key, "refill") && (value == 0 || value == 1)) { b->refill = value; return 1; }
+    if (!std::strcmp(key, "refill_below") && value >= 1 && value <= 64) { b->refillBelow = value; return 1; }
+    if (!std::strcmp(key, "while_while") && (value == 0 || value == 1)) { b->whileWhile = value; return 1; }
+    if (!std::strcmp(key, "use_graph") && (value == 0 || value == 1)) { b->useGraph = value; return 1; }
+    if (!std::strcmp(key, "pipelines") && value >= 1 && value <= kBatchPipelines) { b->pipelines = value; return 1; }
+    if (!std::strcmp(key, "xcd_segments") && (value == 0 || value == 1)) { b->xcdSegments = value; return 1; }
+    if (!std::strcmp(key, "autotune_cache") && (value == 0 || value == 1)) { b->autotuneCache = value; return 1; }
+    return 0;
+}
+
+// Options of o (path completion and leaf size have no batch counterpart) that the caller
+// did not set.
+void applyTuned(Batch* b, const LaunchOptions& o) {
+    for (int k = 0; k < 8; k++) {
+        const int v = o.*(kTunedFields[k].field);
+        if (v >= 0 && !(b->userSet & (1u << k))) (void)setBatchOption(b, kTunedFields[k].name, v);
+    }
 }
 
 bool build(Batch* b) {
@@ -327,19 +390,7 @@ bool build(Batch* b) {
     B_TRY(hipMemcpy(b->dRoots, roots.data(), roots.size() * sizeof(int), hipMemcpyHostToDevice));
     B_TRY(hipMemset(b->dSamples, 0, b->views * sizeof(unsigned)));
     B_TRY(hipMemset(b->dOverflow, 0, sizeof(int)));
-    b->traceBlocks = b->cus * b->blocksPerCU;
-    b->spillEntries = 36 * (b->width - 1);
-    if (!dalloc(b, b->dSpill, (size_t)kBatchPipelines * b->spillEntries * b->traceBlocks * kBlock * 2)) return false;
-    if (b->useGraph) {
-        B_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-        const bool ok = enqueueFrame(b);
-        hipGraph_t g = nullptr;
-        const hipError_t e = hipStreamEndCapture(b->stream, &g);
-        if (!ok) { if (g) (void)hipGraphDestroy(g); return false; }
-        B_TRY(e);
-        b->graph = g;
-        B_TRY(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
-    }
+    if (!prepareLaunch(b)) return false;
     b->buildMs = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
     b->built = true;
     return true;
@@ -384,25 +435,99 @@ int RtBatchAddView(void* p, const Triangle* triangles, int count, int width, int
 int RtBatchSetOption(void* p, const char* key, int value) {
     Batch* b = static_cast<Batch*>(p);
     if (!b || !key || b->built) return 0;
-    if (!std::strcmp(key, "bvh_width") && (value == 2 || value == 4 || value == 8)) { b->width = value; return 1; }
-    if (!std::strcmp(key, "bvh_quantized") && (value == 0 || value == 1)) { b->quantized = value; return 1; }
-    if (!std::strcmp(key, "blocks_per_cu") && value >= 1 && value <= 32) { b->blocksPerCU = value; return 1; }
-    if (!std::strcmp(key, "short_stack") && (value == 8 || value == 12 || value == 16)) { b->shortStack = value; return 1; }
-    if (!std::strcmp(key, "refill") && (value == 0 || value == 1)) { b->refill = value; return 1; }
-    if (!std::strcmp(key, "refill_below") && value >= 1 && value <= 64) { b->refillBelow = value; return 1; }
-    if (!std::strcmp(key, "while_while") && (value == 0 || value == 1)) { b->whileWhile = value; return 1; }
-    if (!std::strcmp(key, "use_graph") && (value == 0 || value == 1)) { b->useGraph = value; return 1; }
-    if (!std::strcmp(key, "pipelines") && value >= 1 && value <= kBatchPipelines) { b->pipelines = value; return 1; }
-    if (!std::strcmp(key, "xcd_segments") && (value == 0 || value == 1)) { b->xcdSegments = value; return 1; }
-    return 0;
+    if (!setBatchOption(b, key, value)) return 0;
+    for (int k = 0; k < 8; k++)
+        if (!std::strcmp(key, kTunedFields[k].name)) b->userSet |= 1u << k;
+    return 1;
+}
+
+int RtBatchGetOption(void* p, const char* key) {
+    Batch* b = static_cast<Batch*>(p);
+    if (!b || !key) return -1;
+    if (!std::strcmp(key, "bvh_width")) return b->width;
+    if (!std::strcmp(key, "bvh_quantized")) return b->quantized;
+    if (!std::strcmp(key, "blocks_per_cu")) return b->blocksPerCU;
+    if (!std::strcmp(key, "short_stack")) return b->shortStack;
+    if (!std::strcmp(key, "refill")) return b->refill;
+    if (!std::strcmp(key, "refill_below")) return b->refillBelow;
+    if (!std::strcmp(key, "while_while")) return b->whileWhile;
+    if (!std::strcmp(key, "use_graph")) return b->useGraph;
+    if (!std::strcmp(key, "pipelines")) return b->pipelines;
+    if (!std::strcmp(key, "xcd_segments")) return b->xcdSegments;
+    if (!std::strcmp(key, "autotune_cache")) return b->autotuneCache;
+    return -1;
 }
 
 int RtBatchBuild(void* p, float* buildMs) {
     Batch* b = static_cast<Batch*>(p);
     if (!b) return 0;
     if (b->built) return 1;
+    LaunchOptions tuned;
+    if (b->autotuneCache && !b->pending.empty() && tuneLookup(batchKey(b), false, tuned)) applyTuned(b, tuned);
     if (!build(b)) { release(b); return 0; }
     if (buildMs) *buildMs = b->buildMs;
+    return 1;
+}
+
+int RtBatchAutotune(void* p, int frames, float* mraysPerSecond, int* trialCount) {
+    Batch* b = static_cast<Batch*>(p);
+    if (!b || frames < 1 || b->pending.empty()) { if (b) b->error = "invalid argument"; return 0; }
+    const int graph = b->useGraph;
+    TuneSpace space;
+    space.widths = (b->userSet & 1u) ? std::vector<int>{b->width} : std::vector<int>{4, 8};
+    space.leaves = {-1};     // the batch build keeps the reference's leaf rule
+    space.tails.clear();     // no path completion in batches
+    int maxBounces = 0;
+    for (const PendingView& pv : b->pending) maxBounces = std::max(maxBounces, pv.rs.maxBounces);
+    bool failed = false;
+    auto measure = [&](const LaunchOptions& c) -> double {
+        if (failed) return -1;
+        const bool rebuild = !b->built || c.bvhWidth != b->width;
+        (void)setBatchOption(b, "bvh_width", c.bvhWidth);
+        (void)setBatchOption(b, "blocks_per_cu", c.blocksPerCU);
+        (void)setBatchOption(b, "short_stack", c.shortStack);
+        (void)setBatchOption(b, "while_while", c.whileWhile);
+        (void)setBatchOption(b, "refill", c.refill);
+        (void)setBatchOption(b, "pipelines", c.pipelines);
+        if (rebuild) {
+            release(b);
+            if (!build(b)) { release(b); failed = true; return -1; }
+        } else if (!prepareLaunch(b)) {
+            failed = true;
+            return -1;
+        }
+        float ms = 0;
+        if (!RtBatchRender(b, 1, &ms)) { failed = true; return -1; }   // warm
+        double best = 0;
+        for (int rep = 0; rep < 2; rep++) {
+            unsigned long long rays = 0;
+            const auto t0 = std::chrono::steady_clock::now();
+            for (int f = 0; f < frames; f++) {
+                if (!RtBatchRender(b, 1, &ms)) { failed = true; return -1; }
+                rays += b->lastClosest + b->lastShadow;
+            }
+            const double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            best = std::max(best, rays / s / 1e6);
+        }
+        return best;
+    };
+    int trials = 0;
+    const LaunchOptions best = tuneSearch(space, maxBounces, measure, &trials);
+    b->useGraph = graph;
+    if (failed || best.mraysPerSecond <= 0) { if (b->error.empty()) b->error = "autotune trial failed"; return 0; }
+    const bool rebuild = best.bvhWidth != b->width;
+    (void)setBatchOption(b, "bvh_width", best.bvhWidth);
+    (void)setBatchOption(b, "blocks_per_cu", best.blocksPerCU);
+    (void)setBatchOption(b, "short_stack", best.shortStack);
+    (void)setBatchOption(b, "while_while", best.whileWhile);
+    (void)setBatchOption(b, "refill", best.refill);
+    (void)setBatchOption(b, "pipelines", best.pipelines);
+    if (rebuild) { release(b); if (!build(b)) { release(b); return 0; } }
+    else if (!prepareLaunch(b)) return 0;
+    if (!RtBatchReset(b)) return 0;
+    (void)tuneStore(batchKey(b), best);
+    if (mraysPerSecond) *mraysPerSecond = (float)best.mraysPerSecond;
+    if (trialCount) *trialCount = trials;
     return 1;
 }
 

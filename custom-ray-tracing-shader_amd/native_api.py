@@ -104,6 +104,8 @@ def load_library(path=None):
     lib.RtSetOption.argtypes = [vp, c.c_char_p, c.c_int]
     lib.RtGetOption.argtypes = [vp, c.c_char_p]
     lib.RtRenderResident.argtypes = [vp, c.POINTER(Camera), c.POINTER(Settings), c.POINTER(c.c_float)]
+    lib.RtAutotune.argtypes = [vp, c.POINTER(Camera), c.POINTER(Settings), c.c_int, c.POINTER(c.c_float),
+                               c.POINTER(c.c_int)]
     lib.RtCopyOutput.argtypes = [vp, c.POINTER(c.c_ubyte), c.c_int]
     lib.RtStageTimes.argtypes = [vp, c.POINTER(c.c_float), c.POINTER(c.c_int), c.c_int]
     lib.RtStageBusy.argtypes = [vp, c.c_uint, c.POINTER(c.c_float)]
@@ -144,6 +146,8 @@ def load_library(path=None):
                                    c.POINTER(Settings)]
     lib.RtBatchSetOption.argtypes = [vp, c.c_char_p, c.c_int]
     lib.RtBatchBuild.argtypes = [vp, fp]
+    lib.RtBatchGetOption.argtypes = [vp, c.c_char_p]
+    lib.RtBatchAutotune.argtypes = [vp, c.c_int, fp, c.POINTER(c.c_int)]
     lib.RtBatchRender.argtypes = [vp, c.c_int, fp]
     lib.RtBatchReset.argtypes = [vp]
     lib.RtBatchRayCounts.argtypes = [vp, c.POINTER(c.c_ulonglong), c.POINTER(c.c_ulonglong)]

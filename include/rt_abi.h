@@ -151,6 +151,20 @@ int RtRayCounts(void* session, unsigned long long* closest, unsigned long long* 
 int RtSetOption(void* session, const char* key, int value);
 int RtGetOption(void* session, const char* key);
 
+/* No counterpart in the reference, which launches a fixed 16x16 grid (Native/renderer.cu:397-398).
+ * The launch autotuner: times `frames` device-resident frames of camera/settings for each
+ * launch configuration of a staged search (BVH width {4, 8} x leaf size {2, 4} x persistent
+ * workgroups per CU x LDS stack x traversal loop at two pipelines, then pipelines, workgroups,
+ * the path-completion bounce and stack x loop again; options the caller set with RtSetOption
+ * stay fixed), leaves the session on the fastest (rebuilding its BVH if the layout changed;
+ * samples reset) and stores it in the autotune cache ($RT_AUTOTUNE_CACHE, else
+ * ~/.cache/rt_amd/autotune.json) under the scene class (triangle count to the nearest power of
+ * two, image size, bounces).  RtInitialize applies a cached entry of the scene class, and each
+ * render the entry of its bounce count, to every tuned option the caller has not set, unless
+ * option "autotune_cache" is 0.  Reports the best Mrays/s and the number of trials. */
+int RtAutotune(void* session, const CameraData* camera, const RenderSettings* settings, int frames,
+               float* mraysPerSecond, int* trialCount);
+
 /* RtRender on the GPU (backend 2) without the RGBA readback: the frame stays in device
  * memory for GPU-resident consumers; RtCopyOutput copies the last frame to the host. */
 int RtRenderResident(void* session, const CameraData* camera, const RenderSettings* settings,
@@ -236,6 +250,14 @@ int RtBatchAddView(void* batch, const Triangle* triangles, int count, int width,
                    const CameraData* camera, const RenderSettings* settings);
 int RtBatchSetOption(void* batch, const char* key, int value);
 int RtBatchBuild(void* batch, float* buildMs);
+int RtBatchGetOption(void* batch, const char* key);
+/* The launch autotuner for a batch (no counterpart in the reference): BVH width x workgroups x
+ * stack x loop, then pipelines, workgroups and stack x loop again, each trial re-capturing the
+ * frame graph (a rebuild when the width changes); the batch is left built with the fastest,
+ * samples reset, and the choice stored in the autotune cache under the batch's class (views,
+ * triangles and pixels to the nearest power of two, largest bounce count), which RtBatchBuild
+ * applies to options the caller did not set unless "autotune_cache" is 0. */
+int RtBatchAutotune(void* batch, int frames, float* mraysPerSecond, int* trialCount);
 int RtBatchRender(void* batch, int frames, float* deviceMs);
 int RtBatchReset(void* batch);
 int RtBatchRayCounts(void* batch, unsigned long long* closest, unsigned long long* shadow);
