@@ -128,9 +128,11 @@ void initContext(EmContext& c) {
 const std::string& embreeLoadError() { return EmbreeApi::get().error; }
 
 void EmbreeScene::release() {
-    EmbreeApi& api = EmbreeApi::get();
-    if (scene) api.releaseScene(scene);
-    if (device) api.releaseDevice(device);
+    if (scene || device) {   // only a built scene needs the library (and loads it)
+        EmbreeApi& api = EmbreeApi::get();
+        if (scene) api.releaseScene(scene);
+        if (device) api.releaseDevice(device);
+    }
     scene = device = nullptr;
     buildMs = 0;
 }

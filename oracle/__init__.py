@@ -27,7 +27,8 @@ from pathlib import Path
 import numpy as np
 
 HERE = Path(__file__).resolve().parent
-LIB_PATH = HERE / "build" / "liboracle.so"
+# RT_ORACLE_LIB selects another build (the sanitizer build, make SAN=1: build_san/liboracle.so).
+LIB_PATH = Path(os.environ.get("RT_ORACLE_LIB", HERE / "build" / "liboracle.so"))
 
 VEC = np.dtype([("x", "<f4"), ("y", "<f4"), ("z", "<f4")])
 TRIANGLE = np.dtype([("origin", VEC), ("edge1", VEC), ("edge2", VEC), ("color", VEC), ("emission", "<f4"),
