@@ -82,6 +82,7 @@ struct Session {
     int tailBounce = 0;          // >= 1: paths complete from this bounce on in one launch (0: off)
     int nontemporal = 1;         // rays / hits bypass cache retention in traversal
     int xcdSegments = 1;         // per-XCD segment cursors in the frame's traversal launches
+    int watertight = 0;          // 1: watertight ray/triangle test (Woop et al. 2013) instead of the reference's
     unsigned userSet = 0;        // tuned options the caller set (bit k: kTunedFields[k])
     int autotuneCache = 1;       // 1: apply the autotuner's cached launch options the caller did not set
     int tunedBounces = -1;       // bounce count the cached launch options were last looked up for
@@ -301,6 +302,7 @@ TraceArgs baseTraceArgs(Session* s, int p = 0) {
     a.spillEntries = s->spillEntries; a.totalLanes = s->traceLanes;
     a.overflow = s->dOverflow;
     a.nontemporal = s->nontemporal;
+    a.watertight = s->watertight;
     return a;
 }
 
@@ -382,7 +384,9 @@ bool renderGpu(Session* s, const CameraData& cam, const RenderSettings& rs, floa
         // b adds bounce b's emission, the reference's order (Native/renderer.cu:232-267).
         // With tail_bounce = k, bounce k-1's shadow rays get a launch of their own and the
         // path-completion kernel takes every path from bounce k to its end.
-        const int tail = (s->tailBounce >= 1 && s->tailBounce < rs.maxBounces) ? s->tailBounce : -1;
+        // Path completion is built for the reference's intersection test: with the watertight
+        // test every bounce gets its own launches (images do not depend on the split).
+        const int tail = (s->tailBounce >= 1 && s->tailBounce < rs.maxBounces && !s->watertight) ? s->tailBounce : -1;
         for (int b = 0; b <= rs.maxBounces; b++) {
             if (b == tail) {
                 for (int p = 0; p < P; p++) {
@@ -601,8 +605,9 @@ void renderBvhRows(Session* s, const CameraData& cam, const RenderSettings& rs, 
                    unsigned long long& closest, unsigned long long& shadow) {
     const N* nodes = reinterpret_cast<const N*>(s->hostNodes.data());
     const TriRecord* recs = s->hostRecs.data();
-    auto trace = [nodes, recs](f3 o, f3 d, float limit, bool any) {
-        return any ? traverseHost<N, true>(nodes, recs, o, d, limit) : traverseHost<N, false>(nodes, recs, o, d, limit);
+    const bool wt = s->watertight != 0;
+    auto trace = [nodes, recs, wt](f3 o, f3 d, float limit, bool any) {
+        return any ? traverseHost<N, true>(nodes, recs, o, d, limit, wt) : traverseHost<N, false>(nodes, recs, o, d, limit, wt);
     };
     renderCpuRows(s, cam, rs, out, multi, trace, closest, shadow);
 }
@@ -1349,6 +1354,10 @@ int setOption(Session* s, const char* key, int value) {
         if (value != 0 && value != 1) return 0;
         s->xcdSegments = value; return 1;
     }
+    if (!std::strcmp(key, "intersect_watertight")) {
+        if (value != 0 && value != 1) return 0;
+        s->watertight = value; return 1;
+    }
     if (!std::strcmp(key, "stage_timing")) {
         if (value != 0 && value != 1) return 0;
         s->stageTiming = value != 0; return 1;
@@ -1396,6 +1405,7 @@ int RtGetOption(void* p, const char* key) {
     if (!std::strcmp(key, "stage_timing")) return s->stageTiming ? 1 : 0;
     if (!std::strcmp(key, "trace_stats")) return s->traceStats ? 1 : 0;
     if (!std::strcmp(key, "autotune_cache")) return s->autotuneCache;
+    if (!std::strcmp(key, "intersect_watertight")) return s->watertight;
     return -1;
 }
 

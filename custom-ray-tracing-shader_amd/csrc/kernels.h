@@ -47,6 +47,7 @@ struct TraceArgs {
     int refillBelow;         // set by launchTrace from TraceLaunch::refillBelow
     int* xcdWork;            // optional: kXcds segment cursors (kXcdCursorStride apart), all 0 at
                              // launch; null: one cursor (work) for the whole index space
+    int watertight;          // 1: watertight ray/triangle test instead of the reference's
 };
 
 struct FrameArgs {

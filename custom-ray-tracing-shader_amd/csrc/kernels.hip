@@ -296,7 +296,7 @@ __device__ __forceinline__ bool descend(LaneRay& L, LdsStack<S>& st, float* dist
 // One traversal step: visit an inner node (push far children, descend to the nearest) or
 // test a leaf, then pop until a deferred entry survives the current best distance.
 // Returns true when the ray is finished.
-template <class N, bool MIXED, bool STATS, int S>
+template <class N, bool MIXED, bool STATS, int S, bool WT = false>
 __device__ __forceinline__ bool traceStep(LaneRay& L, const N* nodes, const float4* recs, LdsStack<S>& st,
                                           bool& overflow, TraceCounters& tc) {
     constexpr int W = NodeTraits<N>::kWidth;
@@ -326,8 +326,8 @@ __device__ __forceinline__ bool traceStep(LaneRay& L, const N* nodes, const floa
                 if (j < m && !last) {
                     const int flags = __float_as_int(B[j].w), index = __float_as_int(A[j].w);
                     float u = 0, v = 0;
-                    const float t = intersectTri(L.o, L.d, F3(A[j].x, A[j].y, A[j].z), F3(B[j].x, B[j].y, B[j].z),
-                                                 F3(C[j].x, C[j].y, C[j].z), (flags & kTriOneSided) != 0, u, v);
+                    const float t = intersectTriMode(WT, L.o, L.d, F3(A[j].x, A[j].y, A[j].z), F3(B[j].x, B[j].y, B[j].z),
+                                                     F3(C[j].x, C[j].y, C[j].z), (flags & kTriOneSided) != 0, u, v);
                     if (STATS) tc.tri(L.any);
                     if (t > 0 && (t < L.best || (t == L.best && index < L.bestIdx))) {
                         L.best = t; L.bestIdx = index; L.bu = u; L.bv = v;
@@ -547,8 +547,10 @@ struct RayDealer {
 template <class N, int S, bool WW, bool STATS> constexpr int traceWavesPerSimd() {
     return (STATS || NodeTraits<N>::kWidth == 8) ? 1 : S == 16 ? 5 : S == 12 ? 6 : WW ? RT_WW_WAVES : RT_STEP_WAVES;
 }
-template <class N, bool MIXED, bool STATS, int S, bool REFILL, bool WW>
+// WT: the watertight ray/triangle test (instantiated with the step loop and 16 entries only).
+template <class N, bool MIXED, bool STATS, int S, bool REFILL, bool WW, bool WT = false>
 __global__ __launch_bounds__(kBlock, (traceWavesPerSimd<N, S, WW, STATS>())) void traceKernel(TraceArgs a) {
+    static_assert(!(WT && WW), "the watertight test is built into the step loop");
     __shared__ int sRef[S * kBlock];
     __shared__ float sDist[S * kBlock];
     const int tid = threadIdx.x, lane = tid & 63;
@@ -607,7 +609,7 @@ __global__ __launch_bounds__(kBlock, (traceWavesPerSimd<N, S, WW, STATS>())) voi
         }
         if (!WW && L.index >= 0) {
             for (;;) {
-                if (traceStep<N, MIXED, STATS, S>(L, nodes, recs, st, overflow, tc)) {
+                if (traceStep<N, MIXED, STATS, S, WT>(L, nodes, recs, st, overflow, tc)) {
                     finishRay(a, L);
                     L.index = -1;
                     break;
@@ -625,21 +627,25 @@ __global__ __launch_bounds__(kBlock, (traceWavesPerSimd<N, S, WW, STATS>())) voi
     if (overflow) atomicOr(a.overflow, 1);
 }
 
-template <class N, int S, bool R, bool WW>
+template <class N, int S, bool R, bool WW, bool WT = false>
 static void launchTraceNS(bool stats, int blocks, const TraceArgs& a, hipStream_t s) {
     dim3 grid(blocks), block(kBlock);
     const bool mixed = a.anyRays != nullptr;
     if (mixed) {
-        if (stats) hipLaunchKernelGGL((traceKernel<N, true, true, S, R, WW>), grid, block, 0, s, a);
-        else hipLaunchKernelGGL((traceKernel<N, true, false, S, R, WW>), grid, block, 0, s, a);
+        if (stats) hipLaunchKernelGGL((traceKernel<N, true, true, S, R, WW, WT>), grid, block, 0, s, a);
+        else hipLaunchKernelGGL((traceKernel<N, true, false, S, R, WW, WT>), grid, block, 0, s, a);
     } else {
-        if (stats) hipLaunchKernelGGL((traceKernel<N, false, true, S, R, WW>), grid, block, 0, s, a);
-        else hipLaunchKernelGGL((traceKernel<N, false, false, S, R, WW>), grid, block, 0, s, a);
+        if (stats) hipLaunchKernelGGL((traceKernel<N, false, true, S, R, WW, WT>), grid, block, 0, s, a);
+        else hipLaunchKernelGGL((traceKernel<N, false, false, S, R, WW, WT>), grid, block, 0, s, a);
     }
 }
 template <class N>
 static hipError_t launchTraceN(bool stats, int blocks, const TraceLaunch& l, const TraceArgs& a, hipStream_t s) {
     // The 12-entry ring exists for 4-wide nodes; other layouts take 16 entries for it.
+    if (a.watertight) {   // one loop shape: the loop and stack change the work, never the hits
+        launchTraceNS<N, 16, false, false, true>(stats, blocks, a, s);
+        return hipGetLastError();
+    }
     const int stack = (l.stack == 12 && !std::is_same<N, WideNode<4>>::value) ? 16 : l.stack;
     const int key = (stack == 16 ? 4 : stack == 8 ? 0 : stack == 12 ? 8 : -100) + (l.refill ? 2 : 0) + (l.whileWhile ? 1 : 0);
     switch (key) {

@@ -75,6 +75,52 @@ RT_HD float intersectTri(f3 o, f3 d, f3 v0, f3 e1, f3 e2, bool oneSided, float& 
     float t = dotv(e2, q) * inv;
     return t > 1e-4f ? t : -1;
 }
+// a*b - c*d with the rounding error of c*d compensated by fused multiply-adds (Kahan's
+// difference of products, as pbrt-v4's DifferenceOfProducts): within 1.5 ulp of the exact
+// value, so its sign is the exact value's sign whenever that is not zero.
+RT_HD float diffProducts(float a, float b, float c, float d) {
+    const float cd = c * d;
+    const float err = fmaf(-c, d, cd);
+    return fmaf(a, b, -cd) + err;
+}
+// Watertight ray/triangle test (Woop, Benthin and Wald, "Watertight Ray/Triangle
+// Intersection", JCGT 2(1), 2013): vertices translated to the ray origin, sheared and scaled so
+// the ray runs along +z, then the three 2D edge functions of the projected triangle.  An edge
+// shared by two triangles gets the same edge function up to sign in both (the ray's shear is
+// the same), so a ray through the edge cannot miss both: no gaps between triangles that share
+// vertices.  The paper's double-precision fallback for zero edge functions is replaced by
+// diffProducts, whose sign is exact.  Acceptance keeps the reference's other rules
+// (Native/renderer.cu:99-112): one-sided triangles reject det < 0 (det has the sign of the
+// Moller-Trumbore determinant), t > 1e-4; only an exactly zero det counts as parallel.
+// Vertices are v0, v0 + e1, v0 + e2 in FP32, as the ABI's triangles define them.  u, v are the
+// barycentric weights of v1 and v2.
+RT_HD float intersectTriWatertight(f3 o, f3 d, f3 v0, f3 e1, f3 e2, bool oneSided, float& u, float& v) {
+    const float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+    const int kz = (ax >= ay && ax >= az) ? 0 : (ay >= az ? 1 : 2);
+    int kx = kz == 2 ? 0 : kz + 1, ky = kx == 2 ? 0 : kx + 1;
+    auto comp = [](f3 q, int k) { return k == 0 ? q.x : (k == 1 ? q.y : q.z); };
+    const float dz = comp(d, kz);
+    if (dz < 0) { const int t = kx; kx = ky; ky = t; }
+    const float sx = comp(d, kx) / dz, sy = comp(d, ky) / dz, sz = 1.0f / dz;
+    const f3 A = v0 - o, B = (v0 + e1) - o, C = (v0 + e2) - o;
+    const float Ax = comp(A, kx) - sx * comp(A, kz), Ay = comp(A, ky) - sy * comp(A, kz);
+    const float Bx = comp(B, kx) - sx * comp(B, kz), By = comp(B, ky) - sy * comp(B, kz);
+    const float Cx = comp(C, kx) - sx * comp(C, kz), Cy = comp(C, ky) - sy * comp(C, kz);
+    const float U = diffProducts(Cx, By, Cy, Bx), V = diffProducts(Ax, Cy, Ay, Cx), W = diffProducts(Bx, Ay, By, Ax);
+    if ((U < 0 || V < 0 || W < 0) && (U > 0 || V > 0 || W > 0)) return -1;
+    const float det = U + V + W;
+    if (det == 0 || (oneSided && det < 0)) return -1;
+    const float T = U * (sz * comp(A, kz)) + V * (sz * comp(B, kz)) + W * (sz * comp(C, kz));
+    const float t = T / det;
+    if (!(t > 1e-4f)) return -1;
+    u = V / det;
+    v = W / det;
+    return t;
+}
+// The reference's test or the watertight one.
+RT_HD float intersectTriMode(bool watertight, f3 o, f3 d, f3 v0, f3 e1, f3 e2, bool oneSided, float& u, float& v) {
+    return watertight ? intersectTriWatertight(o, d, v0, e1, e2, oneSided, u, v) : intersectTri(o, d, v0, e1, e2, oneSided, u, v);
+}
 // Native/renderer.cu:142-143: components below FLT_MIN become a containment test.
 RT_HD f3 safeInverse(f3 d) {
     return F3(fabsf(d.x) < FLT_MIN ? 0 : 1 / d.x, fabsf(d.y) < FLT_MIN ? 0 : 1 / d.y, fabsf(d.z) < FLT_MIN ? 0 : 1 / d.z);

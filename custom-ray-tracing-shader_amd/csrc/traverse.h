@@ -31,12 +31,12 @@ template <> RT_HD void sortChildren<8>(float* d, int* r) {
 // Tests the triangles of one leaf (records first.. up to the last-in-leaf flag).
 template <bool ANY>
 RT_HD bool leafTest(const TriRecord* recs, int first, f3 o, f3 d, float& best, int& bestIdx, float& bu, float& bv,
-                    unsigned long long& triTests) {
+                    unsigned long long& triTests, bool watertight) {
     for (int k = first;; k++) {
         const TriRecord& r = recs[k];
         float u = 0, v = 0;
-        float t = intersectTri(o, d, F3(r.v0x, r.v0y, r.v0z), F3(r.e1x, r.e1y, r.e1z), F3(r.e2x, r.e2y, r.e2z),
-                               (r.flags & kTriOneSided) != 0, u, v);
+        float t = intersectTriMode(watertight, o, d, F3(r.v0x, r.v0y, r.v0z), F3(r.e1x, r.e1y, r.e1z),
+                                   F3(r.e2x, r.e2y, r.e2z), (r.flags & kTriOneSided) != 0, u, v);
         triTests++;
         if (t > 0 && (t < best || (t == best && r.index < bestIdx))) {
             best = t; bestIdx = r.index; bu = u; bv = v;
@@ -49,7 +49,7 @@ RT_HD bool leafTest(const TriRecord* recs, int first, f3 o, f3 d, float& best, i
 // Host traversal with a plain array stack; the same visiting rules as the device kernel.
 // N is WideNode<W> or QNode<W>.
 template <class N, bool ANY>
-inline RtHit traverseHost(const N* nodes, const TriRecord* recs, f3 o, f3 d, float limit) {
+inline RtHit traverseHost(const N* nodes, const TriRecord* recs, f3 o, f3 d, float limit, bool watertight = false) {
     constexpr int W = NodeTraits<N>::kWidth;
     f3 inv = safeInverse(d);
     float best = limit, bu = 0, bv = 0;
@@ -71,7 +71,7 @@ inline RtHit traverseHost(const N* nodes, const TriRecord* recs, f3 o, f3 d, flo
                 if (dist[c] != INFINITY) { sref[sp] = cr[c]; sdist[sp] = dist[c]; sp++; }
             if (dist[0] != INFINITY) { ref = cr[0]; continue; }
         } else {
-            if (leafTest<ANY>(recs, leafFirst(ref), o, d, best, bestIdx, bu, bv, tt)) break;
+            if (leafTest<ANY>(recs, leafFirst(ref), o, d, best, bestIdx, bu, bv, tt, watertight)) break;
         }
         bool found = false;
         while (sp > 0) {

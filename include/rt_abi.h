@@ -145,6 +145,10 @@ int RtRayCounts(void* session, unsigned long long* closest, unsigned long long* 
  * a frame's traversal launches with at least two rays per lane hand out rays from eight
  * contiguous segments, one per XCD group of workgroups, instead of one cursor; images
  * unchanged),
+ * "intersect_watertight" (0/1, default 0: the reference's Moller-Trumbore test and its
+ * acceptance rules, Native/renderer.cu:99-112; 1: the watertight test of Woop, Benthin and Wald
+ * 2013 with the same one-sided and t > 1e-4 rules, so a ray through an edge or vertex shared
+ * by two triangles cannot pass between them),
  * "tiled_primary", "nontemporal", "stage_timing", "trace_stats" and "denoise_fallback" at the
  * next render or query.  Returns 1 if the key is known and the value valid.  RtGetOption
  * returns -1 for unknown keys; it also reports "node_bytes", the size of one wide node. */

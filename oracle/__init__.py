@@ -13,6 +13,8 @@ Function -> reference -> what pins it (besides itself):
                                                  (Tools/verify_sampling.cu:16-25); golden PNGs
   intersect32/64     Native/renderer.cu:99-112   closed-form hits, parallel/one-sided/degenerate
                                                  cases, FP32 vs FP64 agreement
+  intersect_wt32     Woop et al. 2013 (watertight) closed-form hits, FP64 agreement, no gaps at
+                                                 shared edges/vertices of triangle fans
   build_bvh / trace  Native/bvh.cuh:35-105,      structural invariants + brute-force
                      Native/renderer.cu:136-178  differential (Tools/verify_bvh.cu:9-66)
   sah_cost           Native/bvh.cuh:66,89        hand-built trees with known areas
@@ -66,6 +68,9 @@ def lib():
         L.or_intersect64.restype = c.c_double
         L.or_brute32.argtypes = [P, c.c_int, P, c.c_int, c.c_int, P]
         L.or_brute64.argtypes = [P, c.c_int, P, c.c_int, P, P]
+        L.or_brute_wt32.argtypes = [P, c.c_int, P, c.c_int, c.c_int, P]
+        L.or_intersect_wt32.argtypes = [P, P, P, P, P]
+        L.or_intersect_wt32.restype = c.c_float
         L.or_bvh_build.argtypes = [P, c.c_int, c.c_int, c.c_int]
         L.or_bvh_build.restype = P
         L.or_bvh_import.argtypes = [P, c.c_int, P, c.c_int]
@@ -167,6 +172,22 @@ def brute32(tris, rays, any_hit=False):
     tris = np.ascontiguousarray(tris, dtype=TRIANGLE); rays = np.ascontiguousarray(rays, dtype=RAY)
     hits = np.zeros(len(rays), dtype=HIT)
     lib().or_brute32(_p(tris), len(tris), _p(rays), len(rays), int(any_hit), _p(hits))
+    return hits
+
+
+def intersect_wt32(tri, o, d):
+    """Watertight test (Woop, Benthin and Wald 2013) with the reference's acceptance rules."""
+    tri = np.ascontiguousarray(tri, dtype=TRIANGLE).reshape(1)
+    o = np.asarray(o, dtype=np.float32); d = np.asarray(d, dtype=np.float32)
+    u, v = np.zeros(1, np.float32), np.zeros(1, np.float32)
+    t = lib().or_intersect_wt32(_p(tri), _p(o), _p(d), _p(u), _p(v))
+    return float(t), float(u[0]), float(v[0])
+
+
+def brute_wt32(tris, rays, any_hit=False):
+    tris = np.ascontiguousarray(tris, dtype=TRIANGLE); rays = np.ascontiguousarray(rays, dtype=RAY)
+    hits = np.zeros(len(rays), dtype=HIT)
+    lib().or_brute_wt32(_p(tris), len(tris), _p(rays), len(rays), int(any_hit), _p(hits))
     return hits
 
 

@@ -77,6 +77,42 @@ inline float mt32(V o, V d, const Triangle& tri, float& u, float& v) {
     float t = dot3(e2, q) * inv;
     return t > 1e-4f ? t : -1;
 }
+// Watertight ray/triangle test, restated from Woop, Benthin and Wald, "Watertight
+// Ray/Triangle Intersection" (JCGT 2(1), 2013, Fig. 3-5): translate the vertices to the ray
+// origin, shear and scale so the ray runs along +z (axis kz = largest |d|, kx/ky swapped when
+// d[kz] < 0), 2D edge functions U, V, W of the projected triangle, reject mixed signs, t =
+// (U Az + V Bz + W Cz) / det.  The edge functions use Kahan's FMA difference of products in
+// FP32 (exact sign) where the paper falls back to FP64; the acceptance rules are the
+// reference's: one-sided rejects det < 0, t > 1e-4 (Native/renderer.cu:99-112), only det == 0
+// is parallel.  Vertices are origin, origin + edge1, origin + edge2 in FP32.
+inline float diffProd(float a, float b, float c, float d) {
+    float cd = c * d;
+    float err = fmaf(-c, d, cd);
+    return fmaf(a, b, -cd) + err;
+}
+inline float wt32(V o, V d, const Triangle& tri, float& u, float& v) {
+    int kz = 0;
+    if (fabsf(d.y) > fabsf(comp(d, kz))) kz = 1;
+    if (fabsf(d.z) > fabsf(comp(d, kz))) kz = 2;
+    int kx = (kz + 1) % 3, ky = (kx + 1) % 3;
+    if (comp(d, kz) < 0) std::swap(kx, ky);
+    float sx = comp(d, kx) / comp(d, kz), sy = comp(d, ky) / comp(d, kz), sz = 1.0f / comp(d, kz);
+    V p0 = fromAbi(tri.origin);
+    V a = p0 - o, b = (p0 + fromAbi(tri.edge1)) - o, c = (p0 + fromAbi(tri.edge2)) - o;
+    float ax = comp(a, kx) - sx * comp(a, kz), ay = comp(a, ky) - sy * comp(a, kz);
+    float bx = comp(b, kx) - sx * comp(b, kz), by = comp(b, ky) - sy * comp(b, kz);
+    float cx = comp(c, kx) - sx * comp(c, kz), cy = comp(c, ky) - sy * comp(c, kz);
+    float U = diffProd(cx, by, cy, bx), Vv = diffProd(ax, cy, ay, cx), W = diffProd(bx, ay, by, ax);
+    if ((U < 0 || Vv < 0 || W < 0) && (U > 0 || Vv > 0 || W > 0)) return -1;
+    float det = U + Vv + W;
+    if (det == 0 || (tri.oneSided && det < 0)) return -1;
+    float T = U * (sz * comp(a, kz)) + Vv * (sz * comp(b, kz)) + W * (sz * comp(c, kz));
+    float t = T / det;
+    if (!(t > 1e-4f)) return -1;
+    u = Vv / det;
+    v = W / det;
+    return t;
+}
 // The same test in double precision (inputs widened exactly), for hit-distance parity.
 inline double mt64(const double o[3], const double d[3], const Triangle& tri, double& u, double& v) {
     double e1[3] = {tri.edge1.x, tri.edge1.y, tri.edge1.z}, e2[3] = {tri.edge2.x, tri.edge2.y, tri.edge2.z};
@@ -427,6 +463,23 @@ double or_intersect64(const Triangle* t, const double* o, const double* d, doubl
 }
 
 // Brute-force loops (Tools/verify_bvh.cu:9-19): lowest index wins exact ties.
+// Brute force with the watertight test; ties go to the lower index as Native/renderer.cu:152-153.
+void or_brute_wt32(const Triangle* tris, int n, const RtRay* rays, int count, int any, RtHit* hits) {
+    #pragma omp parallel for schedule(dynamic, 64)
+    for (int r = 0; r < count; r++) {
+        V o = mk(rays[r].origin[0], rays[r].origin[1], rays[r].origin[2]);
+        V d = mk(rays[r].direction[0], rays[r].direction[1], rays[r].direction[2]);
+        RtHit h{-1, rays[r].limit, 0, 0};
+        for (int i = 0; i < n; i++) {
+            float u, v, t = wt32(o, d, tris[i], u, v);
+            if (t > 0 && t < h.distance) { h.index = i; h.distance = t; h.u = u; h.v = v; if (any) break; }
+        }
+        hits[r] = h;
+    }
+}
+float or_intersect_wt32(const Triangle* tri, const float* o, const float* d, float* u, float* v) {
+    return wt32(mk(o[0], o[1], o[2]), mk(d[0], d[1], d[2]), *tri, *u, *v);
+}
 void or_brute32(const Triangle* tris, int n, const RtRay* rays, int count, int any, RtHit* hits) {
     #pragma omp parallel for schedule(dynamic, 64)
     for (int r = 0; r < count; r++) {

@@ -86,6 +86,27 @@ def test_c3_workload_reduced_matches_oracle(lib, mesh):
     assert_frame_matches(*got, *oracle_frames(tris, cam, st, w, h, 2), f"C3 {mesh} 256x144 2x4 spp")
 
 
+def test_c3_watertight_frames(lib):
+    """intersect_watertight = 1 on the C3 workload (reduced): the GPU frame equals the library's
+    CPU backend with the same test (same traversal and intersection on the host) within the
+    image gate, and differs from the oracle's reference-test frame only where the two tests
+    disagree (well inside the 1e-3 HDR gate)."""
+    tris = scenes.displaced_icosphere(subdiv=6, seed=7)
+    w, h = 192, 108
+    cam, st = scenes.icosphere_camera(w, h), scenes.make_settings(8, rpp=4)
+    opts = dict(TUNED["icosphere"], intersect_watertight=1)
+    img, acc, counts = gpu_frames(lib, tris, cam, st, w, h, 1, opts)
+    with Session(lib) as s:
+        for k, v in opts.items():
+            assert s.set_option(k, v)
+        s.initialize(tris, w, h)
+        cpu_img, _ = s.render(cam, st, 1)
+        cpu_acc = s.accumulation()
+        cpu_counts = s.ray_counts()
+    assert_frame_matches(img, acc, counts, cpu_img, cpu_acc, cpu_counts, "C3 watertight GPU vs CPU backend")
+    assert_frame_matches(img, acc, counts, *oracle_frames(tris, cam, st, w, h, 1), "C3 watertight vs oracle")
+
+
 def test_c2_soup_full_frames_match_oracle(lib):
     """C2 at its size: the 1M soup, 1920x1080, 3 bounces, two progressive 1-spp frames, at the
     configuration's leaf <= 4 and at the benchmark's leaf <= 2 (bit-identical frames)."""

@@ -101,6 +101,12 @@ def test_full_scene_hit_records_match_oracle(lib, full_scene):
         s.initialize(tris, 1, 1)
         got_c = s.trace(closest)
         got_s = s.trace(shadow, any_hit=True)
+        assert s.set_option("intersect_watertight", 1)
+        wt_c = s.trace(closest)
+    changed = wt_c["index"] != got_c["index"]
+    print(f"{name}: the watertight test changes {changed.sum()} of {len(closest)} closest hits "
+          f"({(got_c['index'][changed] < 0).sum()} were misses of the reference test)", flush=True)
+    assert changed.mean() < 1e-3
     ref_c = osc.trace(closest)
     ref_s = osc.trace(shadow, any_hit=True)
     print(f"{name}: {len(closest)} closest-hit rays ({(ref_c['index'] >= 0).mean():.3f} hit), {len(shadow)} "
@@ -110,6 +116,28 @@ def test_full_scene_hit_records_match_oracle(lib, full_scene):
     for k in ("distance", "u", "v"):
         assert np.array_equal(got_c[k][hit].view(np.uint32), ref_c[k][hit].view(np.uint32)), k
     assert np.array_equal(got_s["index"] >= 0, ref_s["index"] >= 0)
+    # Hit distance against FP64 Moller-Trumbore on the hit triangle: within 1e-5 relative for
+    # every hit more than 11.5 degrees off the triangle's plane; the grazing rest is reported.
+    t64, cosine = mt64(tris, got_c["index"][hit], closest[hit])
+    rel = np.abs(got_c["distance"][hit] - t64) / np.maximum(1, np.abs(t64))
+    steep = cosine > np.sin(np.radians(11.5))
+    over = rel > 1e-5
+    print(f"{name}: {hit.sum()} hits vs FP64: {over.sum()} beyond 1e-5 relative ({over.mean():.4%}), worst "
+          f"{rel.max():.2e}; steeper than 11.5 deg: worst {rel[steep].max():.2e}", flush=True)
+    assert (rel[steep] <= 1e-5).all()
+
+
+def mt64(tris, index, rays):
+    """FP64 Moller-Trumbore distance of each ray to triangle index[i], and |cos| of incidence."""
+    t = tris[index]
+    v0, e1, e2 = (np.stack([t[f][a] for a in "xyz"], 1).astype(np.float64) for f in ("origin", "edge1", "edge2"))
+    o, d = rays["origin"].astype(np.float64), rays["direction"].astype(np.float64)
+    p = np.cross(d, e2)
+    inv = 1 / (e1 * p).sum(1)
+    q = np.cross(o - v0, e1)
+    n = np.cross(e1, e2)
+    cosine = np.abs((n * d).sum(1)) / (np.linalg.norm(n, axis=1) * np.linalg.norm(d, axis=1))
+    return (e2 * q).sum(1) * inv, cosine
 
 
 def test_full_scene_frame_matches_oracle(lib, full_scene):

@@ -277,3 +277,39 @@ def test_batch_sobol_bit_exact_through_images(lib):
             _, total, _ = sc.render(cam_record(cam), settings_record(st), w, h, sample, total)
         assert (total[..., 9] == frames).all()
         assert np.array_equal(got[v].view(np.uint32), total.view(np.uint32)), v
+
+
+@pytest.mark.parametrize("quantized", [0, 1])
+@pytest.mark.parametrize("width", [2, 4, 8])
+def test_watertight_hits_match_oracle(lib, vbvh, width, quantized):
+    """intersect_watertight = 1: closest-hit index, t, u, v bit-identical to the oracle's
+    watertight brute force (Woop, Benthin and Wald 2013), any-hit occlusion equal, on the
+    verify_bvh scenes (Tools/verify_bvh.cu:104-139) and at the seams of a triangle fan."""
+    from test_oracle_geometry import _fan, _rays_at_fan_seams
+    rays, sc = vbvh
+    cases = [(sc[n], rays) for n in ("edges", "scattered", "overlap", "unbalanced")]
+    cases.append((_fan(), _rays_at_fan_seams(20000, 9)))
+    for tris, r in cases:
+        with Session(lib) as s:
+            assert s.set_option("bvh_width", width) and s.set_option("bvh_quantized", quantized)
+            assert s.set_option("intersect_watertight", 1) and s.get_option("intersect_watertight") == 1
+            s.initialize(tris, 1, 1)
+            for any_hit in (False, True):
+                compare_hits(s.trace(r, any_hit), oracle.brute_wt32(tris, r, any_hit), any_hit)
+
+
+def test_watertight_fan_has_no_gaps_on_the_gpu(lib):
+    """Rays through the shared vertex and spokes of a triangle fan always hit with the
+    watertight test; the reference's test (the default) misses exactly where the oracle's does."""
+    from test_oracle_geometry import _fan, _rays_at_fan_seams
+    tris, rays = _fan(), _rays_at_fan_seams(100000, 12)
+    out = {}
+    for wt in (0, 1):
+        with Session(lib) as s:
+            assert s.set_option("intersect_watertight", wt)
+            s.initialize(tris, 1, 1)
+            out[wt] = s.trace(rays)
+    assert (out[1]["index"] >= 0).all()
+    misses = out[0]["index"] < 0
+    print(f"fan seams: reference test misses {misses.sum()} of {len(rays)} rays, watertight test 0")
+    assert misses.sum() > 100 and np.array_equal(misses, oracle.brute32(tris, rays)["index"] < 0)

@@ -138,3 +138,91 @@ def test_12_bin_build_matches_recorded_dragon_shape():
     bvh = oracle.Bvh(tris, bins=12)
     nodes, _ = bvh.export()
     assert len(nodes) == 659 and bvh.depth == 17
+
+
+# ---------------------------------------------------------------- watertight test (Woop et al. 2013)
+def _fan(n=7, r=1.0, center=(0.1, -0.2), z=0.0):
+    """n triangles around a shared centre vertex; every triangle's origin is the centre, so the
+    shared vertices reconstruct to the same floats in both triangles of each edge."""
+    from helpers import vbvh_triangle
+    c = np.array([center[0], center[1], z], np.float32)
+    ang = np.linspace(0, 2 * np.pi, n, endpoint=False) + 0.3
+    pts = [np.array([center[0] + r * np.cos(a), center[1] + r * np.sin(a), z], np.float32) for a in ang]
+    return np.concatenate([vbvh_triangle(c, pts[i], pts[(i + 1) % n]) for i in range(n)])
+
+
+def _rays_at_fan_seams(count, seed, n=7, center=(0.1, -0.2)):
+    """Rays from both sides aimed at the centre vertex and at points on the shared spokes."""
+    rng = np.random.default_rng(seed)
+    o = np.column_stack([rng.uniform(-2, 2, count), rng.uniform(-2, 2, count),
+                         rng.uniform(0.5, 3, count) * np.where(rng.uniform(size=count) < .5, 1, -1)])
+    ang = 0.3 + 2 * np.pi * rng.integers(0, n, count) / n
+    s = np.where(rng.uniform(size=count) < 0.2, 0.0, rng.uniform(0, 0.9, count))
+    tgt = np.column_stack([center[0] + s * np.cos(ang), center[1] + s * np.sin(ang), np.zeros(count)])
+    d = tgt - o
+    rays = np.zeros(count, oracle.RAY)
+    rays["origin"] = o.astype(np.float32)
+    rays["direction"] = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+    rays["limit"] = 1e30
+    return rays
+
+
+def test_watertight_closed_form_cases():
+    from helpers import vbvh_triangle
+    tri = vbvh_triangle((0, 0, 0), (2, 0, 0), (0, 2, 0))
+    t, u, v = oracle.intersect_wt32(tri, (0.5, 0.25, 3.0), (0, 0, -1))
+    assert (t, u, v) == (3.0, 0.25, 0.125)                       # exact: powers of two throughout
+    assert oracle.intersect_wt32(tri, (0.5, 0.25, 3.0), (0, 0, 1))[0] == -1          # behind the origin
+    assert oracle.intersect_wt32(tri, (0.5, 0.25, 1e-5), (0, 0, -1))[0] == -1        # t <= 1e-4
+    assert oracle.intersect_wt32(tri, (1.5, 1.5, 3.0), (0, 0, -1))[0] == -1          # outside
+    assert oracle.intersect_wt32(tri, (0.5, 0.5, 0.0), (1, 0, 0))[0] == -1           # parallel, in plane
+    one = vbvh_triangle((0, 0, 0), (2, 0, 0), (0, 2, 0), sided=1)                   # normal +z
+    assert oracle.intersect_wt32(one, (0.5, 0.25, 3.0), (0, 0, -1))[0] == 3.0        # front
+    assert oracle.intersect_wt32(one, (0.5, 0.25, -3.0), (0, 0, 1))[0] == -1         # back face culled
+    zero = vbvh_triangle((1, 1, 0), (1, 1, 0), (1, 1, 0))
+    assert oracle.intersect_wt32(zero, (1, 1, 3.0), (0, 0, -1))[0] == -1             # zero area: det 0
+
+
+def test_watertight_agrees_with_fp64_moller_trumbore():
+    """Away from the triangle edges the watertight test accepts exactly the rays FP64
+    Moller-Trumbore accepts, with t, u, v within 1e-5 of it."""
+    from scenes import triangles_from_vertices
+    rng = np.random.default_rng(8)
+    tris = triangles_from_vertices(*(rng.uniform(-1, 1, (400, 3)) for _ in range(3)))
+    checked = 0
+    for k in range(3000):
+        i = int(rng.integers(len(tris)))
+        v0 = np.array([tris[i]["origin"][a] for a in "xyz"], np.float64)
+        e1 = np.array([tris[i]["edge1"][a] for a in "xyz"], np.float64)
+        e2 = np.array([tris[i]["edge2"][a] for a in "xyz"], np.float64)
+        b = rng.uniform(-0.3, 1.0, 2)
+        target = v0 + b[0] * e1 + b[1] * e2
+        o = (target + rng.normal(size=3)).astype(np.float32)
+        d = ((target - o) / np.linalg.norm(target - o)).astype(np.float32)
+        t64, u64, v64 = oracle.intersect64(tris[i], o.astype(np.float64), d.astype(np.float64))
+        n = np.cross(e1, e2)
+        if abs(np.dot(n, d)) / np.linalg.norm(n) < 0.2:
+            continue                                             # grazing: both lose digits
+        inside = b[0] > 1e-3 and b[1] > 1e-3 and b[0] + b[1] < 1 - 1e-3
+        outside = b[0] < -1e-3 or b[1] < -1e-3 or b[0] + b[1] > 1 + 1e-3
+        t, u, v = oracle.intersect_wt32(tris[i], o, d)
+        if inside:
+            assert t > 0 and t64 > 0
+            assert abs(t - t64) <= 1e-5 * max(1, t64) and abs(u - u64) <= 1e-5 and abs(v - v64) <= 1e-5
+            checked += 1
+        elif outside:
+            assert t == -1
+    assert checked > 500
+
+
+def test_watertight_fan_has_no_gaps():
+    """Rays through the shared vertex and spokes of a triangle fan: the watertight test hits a
+    triangle every time; the reference's test lets some through (gaps at the seams)."""
+    tris = _fan()
+    rays = _rays_at_fan_seams(50000, 4)
+    wt = oracle.brute_wt32(tris, rays)
+    mt = oracle.brute32(tris, rays)
+    assert (wt["index"] >= 0).all()
+    assert (mt["index"] < 0).sum() > 100
+    both = (mt["index"] >= 0) & (wt["index"] == mt["index"])
+    assert np.abs(wt["distance"][both] - mt["distance"][both]).max() <= 1e-5 * 3
