@@ -48,7 +48,7 @@ HBM_PEAK_GBS = 8000.0
 # changes the work per ray, never the hits or images: the soup runs leaf <= 2 (its
 # configuration's leaf <= 4 renders identical frames, tests/test_gpu_configs.py).
 TUNED = {
-    "soup": {"bvh_width": 4, "build_max_leaf": 2, "blocks_per_cu": 2, "short_stack": 12, "while_while": 1, "refill": 1,
+    "soup": {"bvh_width": 4, "build_max_leaf": 2, "build_bins": 32, "blocks_per_cu": 2, "short_stack": 12, "while_while": 1, "refill": 1,
              "pipelines": 4},
     "icosphere": {"bvh_width": 4, "build_max_leaf": 2, "blocks_per_cu": 3, "short_stack": 12, "while_while": 1,
                   "refill": 1, "pipelines": 4, "tail_bounce": 4},
@@ -114,7 +114,7 @@ def workload(name, tris_count):
         w, h = 1920, 1080
         tris = scenes.triangle_soup(tris_count, seed=1)
         return (f"random triangle soup {len(tris)} tris, 1920x1080, 3 bounces, 1 spp/step, sky 1, "
-                f"GPU binned-SAH build 32 bins leaf <= {TUNED['soup']['build_max_leaf']}",
+                f"GPU binned-SAH build {TUNED['soup']['build_bins']} bins leaf <= {TUNED['soup']['build_max_leaf']}",
                 tris, scenes.soup_camera(w, h), scenes.make_settings(3, rpp=1, sky=1.0), w, h)
     if name == "geodesic":
         w, h = 1920, 1080

@@ -67,7 +67,7 @@ struct Session {
     float buildMs = 0;
     int width = 0, height = 0;
     int bvhWidth = 4;            // option, used at the next RtInitialize
-    int buildBins = 32;          // option: SAH bins per axis of the GPU build
+    int buildBins = 16;          // option: SAH bins per axis of the GPU build
     int buildMaxLeaf = 4;        // option: leaf size limit of the GPU build
     int builtWidth = 4;
     int bvhQuantized = 0;        // option: quantized (QNode) wide nodes at the next RtInitialize

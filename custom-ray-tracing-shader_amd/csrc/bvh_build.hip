@@ -36,7 +36,15 @@ __device__ __forceinline__ int ordKey(float f) {
 }
 __device__ __forceinline__ float ordVal(int k) { return __int_as_float(k >= 0 ? k : k ^ 0x7FFFFFFF); }
 
+// A task is a node still to be decided, owning positions [first, first + count) of the
+// primitive order.  binSlot >= 0: a large task binned by the position-parallel passes (its
+// slot in the global bins); kSmallTask: binned by one wavefront in LDS; kLeafTask: a leaf by
+// the leaf rule (count <= maxLeaf or depth limit).
 struct Task { int node, first, count, depth, binSlot; };
+constexpr int kLeafTask = -1, kSmallTask = -2;
+// Tasks up to this many primitives are binned by one wavefront; larger ones by the
+// position-parallel passes (at most n / kSmallMax of them per level).
+constexpr int kSmallMax = 2048;
 
 struct SplitInfo { int axis, bin, leftCount, childTask; };
 
@@ -45,9 +53,11 @@ __device__ __forceinline__ float halfAreaOf(float lx, float ly, float lz, float 
     return x * y + x * z + y * z;
 }
 
-// Primitive boxes with the reference's padding (Native/bvh.cuh:38-48).
-__global__ void primPrepKernel(const Triangle* tris, int n, float4* plo, float4* phi, float4* cen, int* order,
-                               int* posTask) {
+// Primitive records in position order (the order array's current permutation), so that every
+// pass reads them contiguously and the partition moves them with their positions:
+//   pa = (box lo xyz, centroid x), pb = (box hi xyz, centroid y), pc = (centroid z, index bits).
+// Primitive boxes carry the reference's padding (Native/bvh.cuh:38-48).
+__global__ void primPrepKernel(const Triangle* tris, int n, float4* pa, float4* pb, float2* pc, int* posTask) {
     const int i = blockIdx.x * kB + threadIdx.x;
     if (i >= n) return;
     const Triangle t = tris[i];
@@ -62,63 +72,90 @@ __global__ void primPrepKernel(const Triangle* tris, int n, float4* plo, float4*
         const float pad = 1e-5f + 1e-6f * fmaxf(fabsf(lo[k]), fabsf(hi[k]));
         lo[k] -= pad; hi[k] += pad;
     }
-    plo[i] = make_float4(lo[0], lo[1], lo[2], 0);
-    phi[i] = make_float4(hi[0], hi[1], hi[2], 0);
-    cen[i] = make_float4(c[0], c[1], c[2], 0);
-    order[i] = i;
+    pa[i] = make_float4(lo[0], lo[1], lo[2], c[0]);
+    pb[i] = make_float4(hi[0], hi[1], hi[2], c[1]);
+    pc[i] = make_float2(c[2], __int_as_float(i));
     posTask[i] = 0;
+}
+
+__device__ __forceinline__ int boxInit(int f) {
+    return (f < 3 || (f >= 6 && f < 9)) ? 0x7F800000 : (int)(0xFF800000u ^ 0x7FFFFFFFu);
 }
 
 __global__ void initBoxKernel(int* box, int count12) {
     const int i = blockIdx.x * kB + threadIdx.x;
     if (i >= count12) return;
-    const int f = i % 12;   // 0..2 node lo, 3..5 node hi, 6..8 centroid lo, 9..11 centroid hi
-    box[i] = (f < 3 || (f >= 6 && f < 9)) ? 0x7F800000 : (int)(0xFF800000u ^ 0x7FFFFFFFu);
+    box[i] = boxInit(i % 12);   // 0..2 node lo, 3..5 node hi, 6..8 centroid lo, 9..11 centroid hi
 }
 
-// Per-task node box and centroid box.
-__global__ __launch_bounds__(kB) void taskBoundsKernel(const int* order, const int* posTask, const float4* plo,
-                                                       const float4* phi, const float4* cen, int* box, int n) {
-    __shared__ int sBox[12];
-    __shared__ int sUniform;
-    const int base = blockIdx.x * kB * kPer;
-    const int last = min(base + kB * kPer, n) - 1;
-    if (threadIdx.x == 0) {
-        const int t0 = posTask[base], t1 = posTask[last];
-        sUniform = (t0 >= 0 && t0 == t1) ? t0 : -1;
-    }
-    if (threadIdx.x < 12) sBox[threadIdx.x] = (threadIdx.x < 3 || (threadIdx.x >= 6 && threadIdx.x < 9)) ? 0x7F800000 : (int)(0xFF800000u ^ 0x7FFFFFFFu);
-    __syncthreads();
-    const int uni = sUniform;
-    for (int k = 0; k < kPer; k++) {
-        const int j = base + k * kB + threadIdx.x;
-        if (j > last) break;
-        const int t = uni >= 0 ? uni : posTask[j];
-        if (t < 0) continue;
-        const int p = order[j];
-        const float4 l = plo[p], h = phi[p], c = cen[p];
-        int* dst = uni >= 0 ? sBox : box + (size_t)t * 12;
-        atomicMin(&dst[0], ordKey(l.x)); atomicMin(&dst[1], ordKey(l.y)); atomicMin(&dst[2], ordKey(l.z));
-        atomicMax(&dst[3], ordKey(h.x)); atomicMax(&dst[4], ordKey(h.y)); atomicMax(&dst[5], ordKey(h.z));
-        atomicMin(&dst[6], ordKey(c.x)); atomicMin(&dst[7], ordKey(c.y)); atomicMin(&dst[8], ordKey(c.z));
-        atomicMax(&dst[9], ordKey(c.x)); atomicMax(&dst[10], ordKey(c.y)); atomicMax(&dst[11], ordKey(c.z));
-    }
-    __syncthreads();
-    if (uni >= 0 && threadIdx.x < 12) {
-        int* g = box + (size_t)uni * 12 + threadIdx.x;
+// Node and centroid boxes of the large tasks.  A workgroup walks kChunks consecutive chunks of
+// 1024 positions; chunks that belong to one task accumulate in LDS, flushed to the task's
+// global box (12 atomics) when the task changes or the walk ends, so a task's box takes a few
+// atomics per workgroup instead of per chunk.  Chunks holding a task boundary go per position.
+constexpr int kChunks = 16;
+__device__ __forceinline__ int chunkTask(const int* posTask, const Task* tasks, int base, int last) {
+    const int t0 = posTask[base], t1 = posTask[last];
+    return (t0 >= 0 && t0 == t1) ? (tasks[t0].binSlot >= 0 ? t0 : -2) : -1;   // -2: one small/leaf task
+}
+__device__ __forceinline__ void flushBox(int* sBox, int* box, int task) {
+    if (task >= 0 && threadIdx.x < 12) {
+        int* g = box + (size_t)task * 12 + threadIdx.x;
         const int f = threadIdx.x;
         if (f < 3 || (f >= 6 && f < 9)) atomicMin(g, sBox[f]);
         else atomicMax(g, sBox[f]);
     }
 }
+__global__ __launch_bounds__(kB) void bigBoundsKernel(const int* posTask, const Task* tasks, const float4* pa,
+                                                      const float4* pb, const float2* pc, int* box, int n) {
+    __shared__ int sBox[12];
+    __shared__ int sUniform, sRun;
+    if (threadIdx.x < 12) sBox[threadIdx.x] = boxInit(threadIdx.x);
+    if (threadIdx.x == 0) sRun = -1;
+    __syncthreads();
+    for (int ch = 0; ch < kChunks; ch++) {
+        const int base = (blockIdx.x * kChunks + ch) * kB * kPer;
+        if (base >= n) break;
+        const int last = min(base + kB * kPer, n) - 1;
+        if (threadIdx.x == 0) sUniform = chunkTask(posTask, tasks, base, last);
+        __syncthreads();
+        const int uni = sUniform, run = sRun;
+        if (uni != run && run >= 0) {   // the accumulated task ends here
+            flushBox(sBox, box, run);
+            __syncthreads();
+            if (threadIdx.x < 12) sBox[threadIdx.x] = boxInit(threadIdx.x);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) sRun = uni >= 0 ? uni : -1;
+        if (uni != -2) {
+            for (int k = 0; k < kPer; k++) {
+                const int j = base + k * kB + threadIdx.x;
+                if (j > last) break;
+                const int t = uni >= 0 ? uni : posTask[j];
+                if (t < 0 || (uni < 0 && tasks[t].binSlot < 0)) continue;
+                const float4 l = pa[j], h = pb[j];
+                const float2 c = pc[j];
+                int* dst = uni >= 0 ? sBox : box + (size_t)t * 12;
+                atomicMin(&dst[0], ordKey(l.x)); atomicMin(&dst[1], ordKey(l.y)); atomicMin(&dst[2], ordKey(l.z));
+                atomicMax(&dst[3], ordKey(h.x)); atomicMax(&dst[4], ordKey(h.y)); atomicMax(&dst[5], ordKey(h.z));
+                atomicMin(&dst[6], ordKey(l.w)); atomicMin(&dst[7], ordKey(h.w)); atomicMin(&dst[8], ordKey(c.x));
+                atomicMax(&dst[9], ordKey(l.w)); atomicMax(&dst[10], ordKey(h.w)); atomicMax(&dst[11], ordKey(c.x));
+            }
+        }
+        __syncthreads();
+    }
+    flushBox(sBox, box, sRun);
+}
+
+__device__ __forceinline__ void binInit(int* b) {
+    b[0] = 0;
+    b[1] = b[2] = b[3] = 0x7F800000;
+    b[4] = b[5] = b[6] = (int)(0xFF800000u ^ 0x7FFFFFFFu);
+}
 
 __global__ void initBinsKernel(int* bins, long long slotsTimesBins) {
     const long long i = blockIdx.x * (long long)kB + threadIdx.x;
     if (i >= slotsTimesBins) return;
-    int* b = bins + i * kBinInts;
-    b[0] = 0;
-    b[1] = b[2] = b[3] = 0x7F800000;
-    b[4] = b[5] = b[6] = (int)(0xFF800000u ^ 0x7FFFFFFFu);
+    binInit(bins + i * kBinInts);
 }
 
 __device__ __forceinline__ int binIndex(float c, float lo, float scale, int bins) {
@@ -126,210 +163,594 @@ __device__ __forceinline__ int binIndex(float c, float lo, float scale, int bins
     return b < 0 ? 0 : (b > bins - 1 ? bins - 1 : b);
 }
 
-// Bins of every task that may split (binSlot >= 0); 3 axes x bins records of kBinInts.
-__global__ __launch_bounds__(kB) void binKernel(const int* order, const int* posTask, const Task* tasks,
-                                                const float4* plo, const float4* phi, const float4* cen,
-                                                const int* box, int* bins, int nbins, int n) {
-    __shared__ int sBins[3 * kMaxBins * kBinInts];
-    __shared__ int sUniform;
-    __shared__ float sLo[3], sScale[3];
-    const int base = blockIdx.x * kB * kPer;
-    const int last = min(base + kB * kPer, n) - 1;
-    if (threadIdx.x == 0) {
-        const int t0 = posTask[base], t1 = posTask[last];
-        sUniform = (t0 >= 0 && t0 == t1 && tasks[t0].binSlot >= 0) ? t0 : -1;
-        if (sUniform >= 0) {
-            const int* bx = box + (size_t)t0 * 12;
-            for (int a = 0; a < 3; a++) {
-                const float lo = ordVal(bx[6 + a]), ext = ordVal(bx[9 + a]) - lo;
-                sLo[a] = lo;
-                sScale[a] = ext > 1e-12f ? nbins / ext : 0.0f;
-            }
-        }
+// Centroid bin origin and scale per axis of a task's box (scale 0: no extent, axis skipped).
+__device__ __forceinline__ void binFrame(const int* bx, int nbins, float* lo3, float* sc3) {
+    for (int a = 0; a < 3; a++) {
+        const float lo = ordVal(bx[6 + a]), ext = ordVal(bx[9 + a]) - lo;
+        lo3[a] = lo;
+        sc3[a] = ext > 1e-12f ? nbins / ext : 0.0f;
     }
+}
+
+// Adds one primitive to a task's 3 x nbins bins (counts and boxes as ordered ints).
+__device__ __forceinline__ void binAdd(int* binsOfTask, int nbins, const float* lo3, const float* sc3, float4 l, float4 h,
+                                       float cz) {
+    const float cc[3] = {l.w, h.w, cz};
+    for (int a = 0; a < 3; a++) {
+        if (sc3[a] == 0.0f) continue;
+        int* b = binsOfTask + (a * nbins + binIndex(cc[a], lo3[a], sc3[a], nbins)) * kBinInts;
+        atomicAdd(&b[0], 1);
+        atomicMin(&b[1], ordKey(l.x)); atomicMin(&b[2], ordKey(l.y)); atomicMin(&b[3], ordKey(l.z));
+        atomicMax(&b[4], ordKey(h.x)); atomicMax(&b[5], ordKey(h.y)); atomicMax(&b[6], ordKey(h.z));
+    }
+}
+
+// Bins of the large tasks, accumulated in LDS over runs of chunks of one task as the bounds
+// pass, flushed to the task's global bins when the task changes.
+__device__ __forceinline__ void flushBins(const int* sBins, int* bins, const Task* tasks, int task, int nbins) {
+    if (task < 0) return;
+    int* g = bins + (size_t)tasks[task].binSlot * 3 * nbins * kBinInts;
     for (int k = threadIdx.x; k < 3 * nbins; k += kB) {
-        int* b = sBins + k * kBinInts;
-        b[0] = 0; b[1] = b[2] = b[3] = 0x7F800000; b[4] = b[5] = b[6] = (int)(0xFF800000u ^ 0x7FFFFFFFu);
+        const int* sb = sBins + k * kBinInts;
+        if (sb[0] == 0) continue;
+        int* d = g + k * kBinInts;
+        atomicAdd(&d[0], sb[0]);
+        atomicMin(&d[1], sb[1]); atomicMin(&d[2], sb[2]); atomicMin(&d[3], sb[3]);
+        atomicMax(&d[4], sb[4]); atomicMax(&d[5], sb[5]); atomicMax(&d[6], sb[6]);
     }
+}
+__global__ __launch_bounds__(kB) void bigBinKernel(const int* posTask, const Task* tasks, const float4* pa,
+                                                   const float4* pb, const float2* pc, const int* box, int* bins,
+                                                   int nbins, int n) {
+    __shared__ int sBins[3 * kMaxBins * kBinInts];
+    __shared__ int sUniform, sRun;
+    __shared__ float sLo[3], sScale[3];
+    for (int k = threadIdx.x; k < 3 * nbins; k += kB) binInit(sBins + k * kBinInts);
+    if (threadIdx.x == 0) sRun = -1;
     __syncthreads();
-    const int uni = sUniform;
-    for (int k = 0; k < kPer; k++) {
-        const int j = base + k * kB + threadIdx.x;
-        if (j > last) break;
-        const int t = uni >= 0 ? uni : posTask[j];
-        if (t < 0) continue;
-        int* dstBase;
-        float lo3[3], sc3[3];
-        if (uni >= 0) {
-            dstBase = sBins;
-            for (int a = 0; a < 3; a++) { lo3[a] = sLo[a]; sc3[a] = sScale[a]; }
-        } else {
-            const int slot = tasks[t].binSlot;
-            if (slot < 0) continue;
-            dstBase = bins + (size_t)slot * 3 * nbins * kBinInts;
-            const int* bx = box + (size_t)t * 12;
-            for (int a = 0; a < 3; a++) {
-                const float lo = ordVal(bx[6 + a]), ext = ordVal(bx[9 + a]) - lo;
-                lo3[a] = lo;
-                sc3[a] = ext > 1e-12f ? nbins / ext : 0.0f;
+    for (int ch = 0; ch < kChunks; ch++) {
+        const int base = (blockIdx.x * kChunks + ch) * kB * kPer;
+        if (base >= n) break;
+        const int last = min(base + kB * kPer, n) - 1;
+        if (threadIdx.x == 0) {
+            sUniform = chunkTask(posTask, tasks, base, last);
+            if (sUniform >= 0 && sUniform != sRun) binFrame(box + (size_t)sUniform * 12, nbins, sLo, sScale);
+        }
+        __syncthreads();
+        const int uni = sUniform, run = sRun;
+        if (uni != run && run >= 0) {
+            flushBins(sBins, bins, tasks, run, nbins);
+            __syncthreads();
+            for (int k = threadIdx.x; k < 3 * nbins; k += kB) binInit(sBins + k * kBinInts);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) sRun = uni >= 0 ? uni : -1;
+        if (uni != -2) {
+            for (int k = 0; k < kPer; k++) {
+                const int j = base + k * kB + threadIdx.x;
+                if (j > last) break;
+                const int t = uni >= 0 ? uni : posTask[j];
+                if (t < 0) continue;
+                int* dst;
+                float lo3[3], sc3[3];
+                if (uni >= 0) {
+                    dst = sBins;
+                    for (int a = 0; a < 3; a++) { lo3[a] = sLo[a]; sc3[a] = sScale[a]; }
+                } else {
+                    const int slot = tasks[t].binSlot;
+                    if (slot < 0) continue;
+                    dst = bins + (size_t)slot * 3 * nbins * kBinInts;
+                    binFrame(box + (size_t)t * 12, nbins, lo3, sc3);
+                }
+                binAdd(dst, nbins, lo3, sc3, pa[j], pb[j], pc[j].x);
             }
         }
-        const int p = order[j];
-        const float4 l = plo[p], h = phi[p], c = cen[p];
-        const float cc[3] = {c.x, c.y, c.z};
-        for (int a = 0; a < 3; a++) {
-            if (sc3[a] == 0.0f) continue;
-            int* b = dstBase + (a * nbins + binIndex(cc[a], lo3[a], sc3[a], nbins)) * kBinInts;
-            atomicAdd(&b[0], 1);
-            atomicMin(&b[1], ordKey(l.x)); atomicMin(&b[2], ordKey(l.y)); atomicMin(&b[3], ordKey(l.z));
-            atomicMax(&b[4], ordKey(h.x)); atomicMax(&b[5], ordKey(h.y)); atomicMax(&b[6], ordKey(h.z));
-        }
+        __syncthreads();
     }
-    __syncthreads();
-    if (uni >= 0) {
-        int* g = bins + (size_t)tasks[uni].binSlot * 3 * nbins * kBinInts;
-        for (int k = threadIdx.x; k < 3 * nbins; k += kB) {
-            const int* s = sBins + k * kBinInts;
-            if (s[0] == 0) continue;
-            int* d = g + k * kBinInts;
-            atomicAdd(&d[0], s[0]);
-            atomicMin(&d[1], s[1]); atomicMin(&d[2], s[2]); atomicMin(&d[3], s[3]);
-            atomicMax(&d[4], s[4]); atomicMax(&d[5], s[5]); atomicMax(&d[6], s[6]);
-        }
-    }
+    flushBins(sBins, bins, tasks, sRun, nbins);
 }
 
 struct LevelArgs {
     const Task* tasks; int taskCount;
-    const int* box; const int* bins; int nbins; int maxLeaf;
-    RtBvhNode* nodes; int* counters;   // [0] node count, [1] next task count, [2] next bin slots, [3] max depth
+    int* box; const int* bins; int nbins; int maxLeaf;
+    const float4 *pa, *pb;
+    RtBvhNode* nodes;
+    int* counters;         // [2] next bin slots, [4] error, [5] next small tasks
     Task* nextTasks; SplitInfo* split;
+    int* splitFlag;        // 1 per task that splits (scanned into its child slots)
+    const int* childBase;  // exclusive scan of splitFlag: split k of the level -> slots 2k, 2k + 1
+    const int* blockPrefix;   // scan of the per-1024-task block sums (added to childBase)
+    int nodeBase;          // nodes allocated before this level's children
+    const int* smallList; int smallCount;   // this level's small tasks
+    int* nextSmallList;
 };
 
-// One lane per task: the reference's leaf test and SAH sweep (Native/bvh.cuh:56-100).
-__global__ void sahKernel(LevelArgs a) {
+// The reference's SAH sweep (Native/bvh.cuh:56-100) over one task's bins: the first strictly
+// better (axis, bin) in x, y, z and ascending-bin order against the leaf cost area * count.
+__device__ void sahSweep(const int* tb, const int* bx, int count, int nbins, SplitInfo& si) {
+    const float blx = ordVal(bx[0]), bly = ordVal(bx[1]), blz = ordVal(bx[2]);
+    const float bhx = ordVal(bx[3]), bhy = ordVal(bx[4]), bhz = ordVal(bx[5]);
+    const float area = halfAreaOf(blx, bly, blz, bhx, bhy, bhz);
+    float best = area * count;
+    for (int axis = 0; axis < 3; axis++) {
+        const float ext = ordVal(bx[9 + axis]) - ordVal(bx[6 + axis]);
+        if (ext <= 1e-12f) continue;
+        const int* ab = tb + axis * nbins * kBinInts;
+        // suffix sums from the right, as in the reference (bins 1..nbins-1 are needed)
+        float rlx = FLT_MAX, rly = FLT_MAX, rlz = FLT_MAX, rhx = -FLT_MAX, rhy = -FLT_MAX, rhz = -FLT_MAX;
+        float sufArea[kMaxBins];
+        int sufCount[kMaxBins];
+        int rc = 0;
+        for (int b = nbins - 1; b >= 0; b--) {
+            const int* e = ab + b * kBinInts;
+            if (e[0]) {
+                rlx = fminf(rlx, ordVal(e[1])); rly = fminf(rly, ordVal(e[2])); rlz = fminf(rlz, ordVal(e[3]));
+                rhx = fmaxf(rhx, ordVal(e[4])); rhy = fmaxf(rhy, ordVal(e[5])); rhz = fmaxf(rhz, ordVal(e[6]));
+            }
+            rc += e[0];
+            sufArea[b] = halfAreaOf(rlx, rly, rlz, rhx, rhy, rhz);
+            sufCount[b] = rc;
+        }
+        float llx = FLT_MAX, lly = FLT_MAX, llz = FLT_MAX, lhx = -FLT_MAX, lhy = -FLT_MAX, lhz = -FLT_MAX;
+        int lc = 0;
+        for (int b = 0; b < nbins - 1; b++) {
+            const int* e = ab + b * kBinInts;
+            if (e[0]) {
+                llx = fminf(llx, ordVal(e[1])); lly = fminf(lly, ordVal(e[2])); llz = fminf(llz, ordVal(e[3]));
+                lhx = fmaxf(lhx, ordVal(e[4])); lhy = fmaxf(lhy, ordVal(e[5])); lhz = fmaxf(lhz, ordVal(e[6]));
+            }
+            lc += e[0];
+            if (!lc || !sufCount[b + 1]) continue;
+            const float cost = area + halfAreaOf(llx, lly, llz, lhx, lhy, lhz) * lc + sufArea[b + 1] * sufCount[b + 1];
+            if (cost < best) { best = cost; si.axis = axis; si.bin = b; si.leftCount = lc; }
+        }
+    }
+}
+
+__device__ __forceinline__ bool splits(const SplitInfo& si, int count) {
+    return si.axis >= 0 && si.leftCount > 0 && si.leftCount < count;
+}
+
+// One lane per task: the box of each leaf task (from its few primitives) and the SAH choice of
+// each large task (from the global bins); small tasks are decided by smallTaskKernel.
+__global__ void decideKernel(LevelArgs a) {
     const int i = blockIdx.x * kB + threadIdx.x;
     if (i >= a.taskCount) return;
     const Task t = a.tasks[i];
-    atomicMax(&a.counters[3], t.depth);
-    const int* bx = a.box + (size_t)i * 12;
-    const float blx = ordVal(bx[0]), bly = ordVal(bx[1]), blz = ordVal(bx[2]);
-    const float bhx = ordVal(bx[3]), bhy = ordVal(bx[4]), bhz = ordVal(bx[5]);
-    RtBvhNode nd;
-    nd.low[0] = blx; nd.low[1] = bly; nd.low[2] = blz;
-    nd.high[0] = bhx; nd.high[1] = bhy; nd.high[2] = bhz;
-    nd.first = t.first; nd.count = t.count;
+    if (t.binSlot == kSmallTask) return;
     SplitInfo si; si.axis = -1; si.bin = -1; si.leftCount = 0; si.childTask = -1;
-    if (t.binSlot >= 0) {
-        const float area = halfAreaOf(blx, bly, blz, bhx, bhy, bhz);
-        float best = area * t.count;
-        const int* tb = a.bins + (size_t)t.binSlot * 3 * a.nbins * kBinInts;
-        for (int axis = 0; axis < 3; axis++) {
-            const float ext = ordVal(bx[9 + axis]) - ordVal(bx[6 + axis]);
-            if (ext <= 1e-12f) continue;
-            const int* ab = tb + axis * a.nbins * kBinInts;
-            // suffix sums from the right, as in the reference (bins 1..nbins-1 are needed)
-            float rlx = FLT_MAX, rly = FLT_MAX, rlz = FLT_MAX, rhx = -FLT_MAX, rhy = -FLT_MAX, rhz = -FLT_MAX;
-            float sufArea[kMaxBins];
-            int sufCount[kMaxBins];
-            int rc = 0;
-            for (int b = a.nbins - 1; b >= 0; b--) {
-                const int* e = ab + b * kBinInts;
-                if (e[0]) {
-                    rlx = fminf(rlx, ordVal(e[1])); rly = fminf(rly, ordVal(e[2])); rlz = fminf(rlz, ordVal(e[3]));
-                    rhx = fmaxf(rhx, ordVal(e[4])); rhy = fmaxf(rhy, ordVal(e[5])); rhz = fmaxf(rhz, ordVal(e[6]));
-                }
-                rc += e[0];
-                sufArea[b] = halfAreaOf(rlx, rly, rlz, rhx, rhy, rhz);
-                sufCount[b] = rc;
-            }
-            float llx = FLT_MAX, lly = FLT_MAX, llz = FLT_MAX, lhx = -FLT_MAX, lhy = -FLT_MAX, lhz = -FLT_MAX;
-            int lc = 0;
-            for (int b = 0; b < a.nbins - 1; b++) {
-                const int* e = ab + b * kBinInts;
-                if (e[0]) {
-                    llx = fminf(llx, ordVal(e[1])); lly = fminf(lly, ordVal(e[2])); llz = fminf(llz, ordVal(e[3]));
-                    lhx = fmaxf(lhx, ordVal(e[4])); lhy = fmaxf(lhy, ordVal(e[5])); lhz = fmaxf(lhz, ordVal(e[6]));
-                }
-                lc += e[0];
-                if (!lc || !sufCount[b + 1]) continue;
-                const float cost = area + halfAreaOf(llx, lly, llz, lhx, lhy, lhz) * lc + sufArea[b + 1] * sufCount[b + 1];
-                if (cost < best) { best = cost; si.axis = axis; si.bin = b; si.leftCount = lc; }
-            }
+    if (t.binSlot == kLeafTask) {
+        int b[6];
+        for (int f = 0; f < 6; f++) b[f] = boxInit(f);
+        for (int j = t.first; j < t.first + t.count; j++) {
+            const float4 l = a.pa[j], h = a.pb[j];
+            b[0] = min(b[0], ordKey(l.x)); b[1] = min(b[1], ordKey(l.y)); b[2] = min(b[2], ordKey(l.z));
+            b[3] = max(b[3], ordKey(h.x)); b[4] = max(b[4], ordKey(h.y)); b[5] = max(b[5], ordKey(h.z));
         }
+        int* bx = a.box + (size_t)i * 12;
+        for (int f = 0; f < 6; f++) bx[f] = b[f];
+    } else {
+        sahSweep(a.bins + (size_t)t.binSlot * 3 * a.nbins * kBinInts, a.box + (size_t)i * 12, t.count, a.nbins, si);
+        if (!splits(si, t.count)) si.axis = -1;
     }
-    if (si.axis >= 0 && si.leftCount > 0 && si.leftCount < t.count) {
-        const int child = atomicAdd(&a.counters[0], 2);
+    a.split[i] = si;
+    a.splitFlag[i] = si.axis >= 0 ? 1 : 0;
+}
+
+// Exclusive scan of the split flags: per 1024 tasks a local scan (written to childBase) and the
+// block's sum; then one workgroup scans the block sums (blockPrefix).
+constexpr int kScanBlock = 1024;
+__global__ __launch_bounds__(kScanBlock) void scanBlocksKernel(const int* flags, int* local, int* blockSums, int n) {
+    __shared__ int sWave[kScanBlock / 64];
+    const int i = blockIdx.x * kScanBlock + threadIdx.x, lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int f = i < n ? flags[i] : 0;
+    int v = f;   // inclusive scan within the wave
+    for (int o = 1; o < 64; o <<= 1) { const int u = __shfl_up(v, o, 64); if (lane >= o) v += u; }
+    if (lane == 63) sWave[wave] = v;
+    __syncthreads();
+    if (threadIdx.x < kScanBlock / 64) {
+        int w = sWave[threadIdx.x];
+        for (int o = 1; o < kScanBlock / 64; o <<= 1) {
+            const int u = __shfl_up(w, o, 64);
+            if ((int)threadIdx.x >= o) w += u;
+        }
+        sWave[threadIdx.x] = w;
+    }
+    __syncthreads();
+    const int before = (wave ? sWave[wave - 1] : 0) + v - f;
+    if (i < n) local[i] = before;
+    if (threadIdx.x == kScanBlock - 1) blockSums[blockIdx.x] = sWave[kScanBlock / 64 - 1];
+}
+__global__ __launch_bounds__(kScanBlock) void scanTopKernel(const int* blockSums, int* blockPrefix, int blocks, int* total) {
+    __shared__ int sCarry;
+    if (threadIdx.x == 0) sCarry = 0;
+    __syncthreads();
+    for (int base = 0; base < blocks; base += kScanBlock) {
+        const int i = base + threadIdx.x, lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+        __shared__ int sWave[kScanBlock / 64];
+        const int f = i < blocks ? blockSums[i] : 0;
+        int v = f;
+        for (int o = 1; o < 64; o <<= 1) { const int u = __shfl_up(v, o, 64); if (lane >= o) v += u; }
+        if (lane == 63) sWave[wave] = v;
+        __syncthreads();
+        if (threadIdx.x < kScanBlock / 64) {
+            int w = sWave[threadIdx.x];
+            for (int o = 1; o < kScanBlock / 64; o <<= 1) {
+                const int u = __shfl_up(w, o, 64);
+                if ((int)threadIdx.x >= o) w += u;
+            }
+            sWave[threadIdx.x] = w;
+        }
+        __syncthreads();
+        const int carry = sCarry;
+        if (i < blocks) blockPrefix[i] = carry + (wave ? sWave[wave - 1] : 0) + v - f;
+        __syncthreads();
+        if (threadIdx.x == 0) sCarry = carry + sWave[kScanBlock / 64 - 1];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *total = sCarry;
+}
+
+// One lane per task: the task's node and, for a split, its two children in the slots the scan
+// gave it (nodes nodeBase + 2k, 2k + 1), classified for the next level.  The atomics (next
+// small list, large-task bin slots) are wave-aggregated by the compiler.
+__global__ void emitKernel(LevelArgs a) {
+    const int i = blockIdx.x * kB + threadIdx.x;
+    if (i >= a.taskCount) return;
+    const Task t = a.tasks[i];
+    SplitInfo si = a.split[i];
+    const int* bx = a.box + (size_t)i * 12;
+    RtBvhNode nd;
+    nd.low[0] = ordVal(bx[0]); nd.low[1] = ordVal(bx[1]); nd.low[2] = ordVal(bx[2]);
+    nd.high[0] = ordVal(bx[3]); nd.high[1] = ordVal(bx[4]); nd.high[2] = ordVal(bx[5]);
+    nd.first = t.first; nd.count = t.count;
+    if (si.axis >= 0) {
+        const int k = a.childBase[i] + a.blockPrefix[i / kScanBlock];
+        const int child = a.nodeBase + 2 * k;
         nd.first = child; nd.count = 0;
-        const int slot = atomicAdd(&a.counters[1], 2);
-        si.childTask = slot;
+        si.childTask = 2 * k;
         const int counts[2] = {si.leftCount, t.count - si.leftCount};
         const int firsts[2] = {t.first, t.first + si.leftCount};
         for (int c = 0; c < 2; c++) {
             Task ct;
             ct.node = child + c; ct.first = firsts[c]; ct.count = counts[c]; ct.depth = t.depth + 1;
-            const bool mayBeLeaf = ct.count <= a.maxLeaf || ct.depth == kBvhDepthLimit;
-            ct.binSlot = mayBeLeaf ? -1 : atomicAdd(&a.counters[2], 1);
-            a.nextTasks[slot + c] = ct;
+            if (ct.count <= a.maxLeaf || ct.depth == kBvhDepthLimit) ct.binSlot = kLeafTask;
+            else if (ct.count <= kSmallMax) ct.binSlot = kSmallTask;
+            else ct.binSlot = atomicAdd(&a.counters[2], 1);
+            a.nextTasks[2 * k + c] = ct;
+            if (ct.binSlot == kSmallTask) a.nextSmallList[atomicAdd(&a.counters[5], 1)] = 2 * k + c;
         }
-    } else {
-        si.axis = -1;
+        a.split[i] = si;
     }
     a.nodes[t.node] = nd;
-    a.split[i] = si;
 }
 
-// Scatter each split task's positions to its left / right child ranges.  When a whole wave
-// works on one task (the common case near the root), each side takes one atomic per wave.
-__global__ __launch_bounds__(kB) void partitionKernel(const int* order, int* orderOut, const int* posTask,
-                                                      int* posTaskOut, const float4* cen, const Task* tasks,
-                                                      const SplitInfo* split, const int* box, int* cursors,
-                                                      int nbins, int n, int* error) {
-    const int j = blockIdx.x * kB + threadIdx.x;
-    const int lane = threadIdx.x & 63;
-    const int t = j < n ? posTask[j] : -2;
-    const SplitInfo si = t >= 0 ? split[t] : SplitInfo{-1, -1, 0, -1};
-    const bool moving = t >= 0 && si.axis >= 0;
-    if (j < n && !moving) {
-        orderOut[j] = order[j];
-        posTaskOut[j] = -1;
+// Inclusive scans across the first `lanes` lanes (lanes above take no part in the result).
+__device__ __forceinline__ int scanMinUp(int v, int lane) {
+    for (int o = 1; o < 32; o <<= 1) { const int u = __shfl_up(v, o, 64); if (lane >= o) v = min(v, u); }
+    return v;
+}
+__device__ __forceinline__ int scanMaxUp(int v, int lane) {
+    for (int o = 1; o < 32; o <<= 1) { const int u = __shfl_up(v, o, 64); if (lane >= o) v = max(v, u); }
+    return v;
+}
+__device__ __forceinline__ int scanAddUp(int v, int lane) {
+    for (int o = 1; o < 32; o <<= 1) { const int u = __shfl_up(v, o, 64); if (lane >= o) v += u; }
+    return v;
+}
+__device__ __forceinline__ int scanMinDown(int v, int lane, int n) {
+    for (int o = 1; o < 32; o <<= 1) { const int u = __shfl_down(v, o, 64); if (lane + o < n) v = min(v, u); }
+    return v;
+}
+__device__ __forceinline__ int scanMaxDown(int v, int lane, int n) {
+    for (int o = 1; o < 32; o <<= 1) { const int u = __shfl_down(v, o, 64); if (lane + o < n) v = max(v, u); }
+    return v;
+}
+__device__ __forceinline__ int scanAddDown(int v, int lane, int n) {
+    for (int o = 1; o < 32; o <<= 1) { const int u = __shfl_down(v, o, 64); if (lane + o < n) v += u; }
+    return v;
+}
+
+// sahSweep across a wavefront, lane b = bin b: the left boxes and counts of bins 0..b and the
+// right ones of b+1..nbins-1 by prefix / suffix scans (min, max and sums are exact, so every
+// cost is the sequential sweep's float), then per axis the smallest cost at the lowest bin,
+// and across axes the first strictly better than the best so far: the sequential sweep's
+// choice.  The result is the same in every lane.
+__device__ void sahSweepWave(const int* tb, const int* bx, int count, int nbins, int lane, SplitInfo& si) {
+    const float area = halfAreaOf(ordVal(bx[0]), ordVal(bx[1]), ordVal(bx[2]), ordVal(bx[3]), ordVal(bx[4]), ordVal(bx[5]));
+    float best = area * count;
+    for (int axis = 0; axis < 3; axis++) {
+        const float ext = ordVal(bx[9 + axis]) - ordVal(bx[6 + axis]);
+        if (ext <= 1e-12f) continue;
+        const int* e = tb + (axis * nbins + (lane < nbins ? lane : 0)) * kBinInts;
+        const bool in = lane < nbins;
+        const int c = in ? e[0] : 0;
+        int lo[3], hi[3];
+        for (int q = 0; q < 3; q++) {
+            lo[q] = in ? e[1 + q] : 0x7F800000;
+            hi[q] = in ? e[4 + q] : (int)(0xFF800000u ^ 0x7FFFFFFFu);
+        }
+        const int lc = scanAddUp(c, lane), rc = scanAddDown(c, lane, nbins);
+        int llo[3], lhi[3], rlo[3], rhi[3];
+        for (int q = 0; q < 3; q++) {
+            llo[q] = scanMinUp(lo[q], lane); lhi[q] = scanMaxUp(hi[q], lane);
+            rlo[q] = scanMinDown(lo[q], lane, nbins); rhi[q] = scanMaxDown(hi[q], lane, nbins);
+        }
+        // the right side of split b is the suffix from bin b + 1
+        const int rcNext = __shfl_down(rc, 1, 64);
+        int rloN[3], rhiN[3];
+        for (int q = 0; q < 3; q++) { rloN[q] = __shfl_down(rlo[q], 1, 64); rhiN[q] = __shfl_down(rhi[q], 1, 64); }
+        const bool valid = lane < nbins - 1 && lc > 0 && rcNext > 0;
+        float cost = INFINITY;
+        if (valid) {
+            const float al = halfAreaOf(ordVal(llo[0]), ordVal(llo[1]), ordVal(llo[2]), ordVal(lhi[0]), ordVal(lhi[1]),
+                                        ordVal(lhi[2]));
+            const float ar = halfAreaOf(ordVal(rloN[0]), ordVal(rloN[1]), ordVal(rloN[2]), ordVal(rhiN[0]),
+                                        ordVal(rhiN[1]), ordVal(rhiN[2]));
+            cost = area + al * lc + ar * rcNext;
+        }
+        // smallest cost, lowest bin on ties
+        float mc = cost;
+        int mb = valid ? lane : 64;
+        for (int o = 32; o > 0; o >>= 1) {
+            const float oc = __shfl_xor(mc, o, 64);
+            const int ob = __shfl_xor(mb, o, 64);
+            if (oc < mc || (oc == mc && ob < mb)) { mc = oc; mb = ob; }
+        }
+        if (mb < 64 && mc < best) {
+            best = mc;
+            si.axis = axis; si.bin = mb; si.leftCount = __shfl(lc, mb, 64);
+        }
     }
-    int right = 0, p = 0;
-    if (moving) {
-        p = order[j];
-        const float4 c = cen[p];
-        const float cc = si.axis == 0 ? c.x : (si.axis == 1 ? c.y : c.z);
-        const int* bx = box + (size_t)t * 12;
+}
+
+// One wavefront per small task: bounds (wave reductions), bins (LDS atomics) and the SAH
+// sweep (across the wave), with no global atomics; the task's records move in smallMoveKernel.
+constexpr int kWavesPerBlock = kB / 64;
+__global__ __launch_bounds__(kB) void smallTaskKernel(LevelArgs a, const float2* pc) {
+    __shared__ int sBins[kWavesPerBlock][3 * kMaxBins * kBinInts];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int w = blockIdx.x * kWavesPerBlock + wave;
+    if (w >= a.smallCount) return;   // the whole wave leaves; no workgroup barrier below
+    const int i = a.smallList[w];
+    const Task t = a.tasks[i];
+    int b[12];
+    for (int f = 0; f < 12; f++) b[f] = boxInit(f);
+    const int end = t.first + t.count;
+    for (int j = t.first + lane; j < end; j += 64) {
+        const float4 l = a.pa[j], h = a.pb[j];
+        const float cz = pc[j].x;
+        b[0] = min(b[0], ordKey(l.x)); b[1] = min(b[1], ordKey(l.y)); b[2] = min(b[2], ordKey(l.z));
+        b[3] = max(b[3], ordKey(h.x)); b[4] = max(b[4], ordKey(h.y)); b[5] = max(b[5], ordKey(h.z));
+        b[6] = min(b[6], ordKey(l.w)); b[7] = min(b[7], ordKey(h.w)); b[8] = min(b[8], ordKey(cz));
+        b[9] = max(b[9], ordKey(l.w)); b[10] = max(b[10], ordKey(h.w)); b[11] = max(b[11], ordKey(cz));
+    }
+    for (int f = 0; f < 12; f++) {
+        const bool isMin = f < 3 || (f >= 6 && f < 9);
+        for (int o = 32; o > 0; o >>= 1) {
+            const int v = __shfl_xor(b[f], o, 64);
+            b[f] = isMin ? min(b[f], v) : max(b[f], v);
+        }
+    }
+    int* bx = a.box + (size_t)i * 12;
+    if (lane < 12) {
+        int v = b[0];
+        for (int f = 1; f < 12; f++) if (lane == f) v = b[f];
+        bx[lane] = v;
+    }
+    int* bins = sBins[wave];
+    for (int k = lane; k < 3 * a.nbins; k += 64) binInit(bins + k * kBinInts);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    float lo3[3], sc3[3];
+    for (int q = 0; q < 3; q++) {
+        const float lo = ordVal(b[6 + q]), ext = ordVal(b[9 + q]) - lo;
+        lo3[q] = lo;
+        sc3[q] = ext > 1e-12f ? a.nbins / ext : 0.0f;
+    }
+    for (int j = t.first + lane; j < end; j += 64) binAdd(bins, a.nbins, lo3, sc3, a.pa[j], a.pb[j], pc[j].x);
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    SplitInfo si; si.axis = -1; si.bin = -1; si.leftCount = 0; si.childTask = -1;
+    sahSweepWave(bins, b, t.count, a.nbins, lane, si);
+    if (!splits(si, t.count)) si.axis = -1;
+    if (lane == 0) {
+        a.split[i] = si;
+        a.splitFlag[i] = si.axis >= 0 ? 1 : 0;
+    }
+}
+
+// One wavefront per small task, after emitKernel: its records to the left / right child ranges
+// of the other buffer (ballot compaction in rounds of 64, order kept), or, for a leaf, copied
+// once and marked finished in both buffers.
+struct MoveArgs {
+    const float4 *pa, *pb;
+    const float2* pc;
+    float4 *paOut, *pbOut;
+    float2* pcOut;
+    int *posTask, *posTaskOut;
+    const Task* tasks; const SplitInfo* split; const int* box;
+    const int* smallList; int smallCount; int nbins;
+    int* error;
+};
+__global__ __launch_bounds__(kB) void smallMoveKernel(MoveArgs m) {
+    const int lane = threadIdx.x & 63;
+    const int w = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
+    if (w >= m.smallCount) return;
+    const int i = m.smallList[w];
+    const Task t = m.tasks[i];
+    const SplitInfo si = m.split[i];
+    const bool split = si.axis >= 0;
+    const int end = t.first + t.count;
+    float lo = 0, scale = 0;
+    if (split) {
+        const int* bx = m.box + (size_t)i * 12;
+        lo = ordVal(bx[6 + si.axis]);
+        scale = m.nbins / (ordVal(bx[9 + si.axis]) - lo);
+    }
+    int nl = 0, nr = 0;
+    for (int j0 = t.first; j0 < end; j0 += 64) {
+        const int j = j0 + lane;
+        const bool live = j < end;
+        float4 l = make_float4(0, 0, 0, 0), h = l;
+        float2 c = make_float2(0, 0);
+        if (live) { l = m.pa[j]; h = m.pb[j]; c = m.pc[j]; }
+        int right = 0;
+        if (live && split) {
+            const float cc = si.axis == 0 ? l.w : (si.axis == 1 ? h.w : c.x);
+            right = binIndex(cc, lo, scale, m.nbins) <= si.bin ? 0 : 1;
+        }
+        const unsigned long long liveMask = __ballot(live);
+        const unsigned long long mr = __ballot(live && right), ml = liveMask & ~mr;
+        const unsigned long long mine = right ? mr : ml;
+        const int rank = (int)__builtin_amdgcn_mbcnt_hi((unsigned)(mine >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mine, 0u));
+        if (live) {
+            int dst = j, task = -1;
+            if (split) {
+                dst = right ? t.first + si.leftCount + nr + rank : t.first + nl + rank;
+                task = si.childTask + right;
+                if (dst >= (right ? end : t.first + si.leftCount)) {   // bins and partition disagree: refuse
+                    atomicOr(m.error, 1);
+                    dst = -1;
+                }
+            } else {
+                m.posTask[j] = -1;   // a leaf: finished in both buffers
+            }
+            if (dst >= 0) {
+                m.paOut[dst] = l; m.pbOut[dst] = h; m.pcOut[dst] = c;
+                m.posTaskOut[dst] = task;
+            }
+        }
+        nl += __popcll(ml);
+        nr += __popcll(mr);
+    }
+}
+
+// Moves the records of large and leaf tasks (small tasks move in smallMoveKernel): a split
+// task's records go to its left / right child range in the other buffer; a record whose task
+// became a leaf this level is copied once to the other buffer and marked finished in both (its
+// position never changes again); finished records are skipped.  A workgroup takes 1024
+// positions: when they all belong to one splitting task it places them with one cursor atomic
+// per side (ranks from a workgroup prefix sum), else per position (one atomic per wave and side
+// when the wave is uniform).
+__global__ __launch_bounds__(kB) void partitionKernel(const float4* pa, const float4* pb, const float2* pc, int* posTask,
+                                                      float4* paOut, float4* pbOut, float2* pcOut, int* posTaskOut,
+                                                      const Task* tasks, const SplitInfo* split, const int* box,
+                                                      int* cursors, int nbins, int n, int* error) {
+    __shared__ int sUniform, sWaveL[kB / 64], sWaveR[kB / 64], sBaseL, sBaseR;
+    const int base = blockIdx.x * kB * kPer;
+    const int last = min(base + kB * kPer, n) - 1;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (threadIdx.x == 0) {
+        const int t0 = posTask[base], t1 = posTask[last];
+        sUniform = (t0 >= 0 && t0 == t1 && tasks[t0].binSlot >= 0 && split[t0].axis >= 0) ? t0 : -1;
+    }
+    __syncthreads();
+    const int uni = sUniform;
+    if (uni >= 0) {
+        const SplitInfo si = split[uni];
+        const int* bx = box + (size_t)uni * 12;
         const float lo = ordVal(bx[6 + si.axis]);
         const float scale = nbins / (ordVal(bx[9 + si.axis]) - lo);
-        right = binIndex(cc, lo, scale, nbins) <= si.bin ? 0 : 1;
-    }
-    // Wave-uniform task?  (all lanes of the wave reach this point)
-    const int t0 = __shfl(t, 0, 64);
-    const unsigned long long all = __ballot(true);
-    const bool uniform = __ballot(moving && t == t0) == all;
-    int k;
-    if (uniform) {
-        const unsigned long long mr = __ballot(right == 1), ml = all & ~mr;
-        int baseL = 0, baseR = 0;
-        if (lane == 0) {
-            if (ml) baseL = atomicAdd(&cursors[2 * t0], __popcll(ml));
-            if (mr) baseR = atomicAdd(&cursors[2 * t0 + 1], __popcll(mr));
+        float4 l[kPer], h[kPer];
+        float2 c[kPer];
+        int right[kPer], nl = 0, nr = 0;
+        for (int k = 0; k < kPer; k++) {
+            const int j = base + k * kB + threadIdx.x;
+            right[k] = -1;
+            if (j > last) continue;
+            l[k] = pa[j]; h[k] = pb[j]; c[k] = pc[j];
+            const float cc = si.axis == 0 ? l[k].w : (si.axis == 1 ? h[k].w : c[k].x);
+            right[k] = binIndex(cc, lo, scale, nbins) <= si.bin ? 0 : 1;
+            nl += right[k] == 0; nr += right[k] == 1;
         }
-        baseL = __shfl(baseL, 0, 64);
-        baseR = __shfl(baseR, 0, 64);
-        const unsigned long long mine = right ? mr : ml;
-        k = (right ? baseR : baseL) +
-            (int)__builtin_amdgcn_mbcnt_hi((unsigned)(mine >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mine, 0u));
-    } else {
-        k = moving ? atomicAdd(&cursors[2 * t + right], 1) : 0;
+        // exclusive prefix of (nl, nr) over the workgroup's threads
+        int il = nl, ir = nr;
+        for (int o = 1; o < 64; o <<= 1) {
+            const int ul = __shfl_up(il, o, 64), ur = __shfl_up(ir, o, 64);
+            if (lane >= o) { il += ul; ir += ur; }
+        }
+        if (lane == 63) { sWaveL[wave] = il; sWaveR[wave] = ir; }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int tl = 0, tr = 0;
+            for (int w = 0; w < kB / 64; w++) {
+                const int a = sWaveL[w], b = sWaveR[w];
+                sWaveL[w] = tl; sWaveR[w] = tr;
+                tl += a; tr += b;
+            }
+            sBaseL = tl ? atomicAdd(&cursors[2 * uni], tl) : 0;
+            sBaseR = tr ? atomicAdd(&cursors[2 * uni + 1], tr) : 0;
+            if (sBaseL + tl > si.leftCount || sBaseR + tr > tasks[uni].count - si.leftCount) atomicOr(error, 1);
+        }
+        __syncthreads();
+        int offL = sBaseL + sWaveL[wave] + il - nl, offR = sBaseR + sWaveR[wave] + ir - nr;
+        const int first = tasks[uni].first;
+        for (int k = 0; k < kPer; k++) {
+            if (right[k] < 0) continue;
+            const int kk = right[k] ? offR++ : offL++;
+            const int limit = right[k] ? tasks[uni].count - si.leftCount : si.leftCount;
+            if (kk >= limit) continue;   // reported above: bins and partition disagree
+            const int dst = first + (right[k] ? si.leftCount : 0) + kk;
+            paOut[dst] = l[k]; pbOut[dst] = h[k]; pcOut[dst] = c[k];
+            posTaskOut[dst] = si.childTask + right[k];
+        }
+        return;
     }
-    if (!moving) return;
-    const int limit = right ? tasks[t].count - si.leftCount : si.leftCount;
-    if (k >= limit) { atomicOr(error, 1); return; }   // bins and partition disagree: refuse
-    const int dst = tasks[t].first + (right ? si.leftCount : 0) + k;
-    orderOut[dst] = p;
-    posTaskOut[dst] = si.childTask + right;
+    for (int kq = 0; kq < kPer; kq++) {
+        const int j = base + kq * kB + threadIdx.x;
+        int t = j <= last ? posTask[j] : -2;
+        if (t >= 0 && tasks[t].binSlot == kSmallTask) t = -2;   // moved by its task's wavefront
+        const SplitInfo si = t >= 0 ? split[t] : SplitInfo{-1, -1, 0, -1};
+        const bool moving = t >= 0 && si.axis >= 0;
+        float4 l = make_float4(0, 0, 0, 0), h = l;
+        float2 c = make_float2(0, 0);
+        if (t >= 0) { l = pa[j]; h = pb[j]; c = pc[j]; }
+        if (t >= 0 && !moving) {   // finishes here: into the other buffer once, finished in both
+            paOut[j] = l; pbOut[j] = h; pcOut[j] = c;
+            posTaskOut[j] = -1;
+            posTask[j] = -1;
+        }
+        int right = 0;
+        if (moving) {
+            const float cc = si.axis == 0 ? l.w : (si.axis == 1 ? h.w : c.x);
+            const int* bx = box + (size_t)t * 12;
+            const float lo = ordVal(bx[6 + si.axis]);
+            const float scale = nbins / (ordVal(bx[9 + si.axis]) - lo);
+            right = binIndex(cc, lo, scale, nbins) <= si.bin ? 0 : 1;
+        }
+        // Wave-uniform task?  (all lanes of the wave reach this point)
+        const int t0 = __shfl(t, 0, 64);
+        const unsigned long long all = __ballot(true);
+        const bool uniform = __ballot(moving && t == t0) == all;
+        int k;
+        if (uniform) {
+            const unsigned long long mr = __ballot(right == 1), ml = all & ~mr;
+            int baseL = 0, baseR = 0;
+            if (lane == 0) {
+                if (ml) baseL = atomicAdd(&cursors[2 * t0], __popcll(ml));
+                if (mr) baseR = atomicAdd(&cursors[2 * t0 + 1], __popcll(mr));
+            }
+            baseL = __shfl(baseL, 0, 64);
+            baseR = __shfl(baseR, 0, 64);
+            const unsigned long long mine = right ? mr : ml;
+            k = (right ? baseR : baseL) +
+                (int)__builtin_amdgcn_mbcnt_hi((unsigned)(mine >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mine, 0u));
+        } else {
+            k = moving ? atomicAdd(&cursors[2 * t + right], 1) : 0;
+        }
+        if (!moving) continue;
+        const int limit = right ? tasks[t].count - si.leftCount : si.leftCount;
+        if (k >= limit) { atomicOr(error, 1); continue; }   // bins and partition disagree: refuse
+        const int dst = tasks[t].first + (right ? si.leftCount : 0) + k;
+        paOut[dst] = l; pbOut[dst] = h; pcOut[dst] = c;
+        posTaskOut[dst] = si.childTask + right;
+    }
+}
+
+__global__ void orderKernel(const float2* pc, int* order, int n) {
+    const int j = blockIdx.x * kB + threadIdx.x;
+    if (j < n) order[j] = __float_as_int(pc[j].y);
 }
 
 __global__ void leafMarkKernel(const RtBvhNode* nodes, int count, int* lastFlag) {
@@ -446,7 +867,7 @@ static hipError_t collapseW(const RtBvhNode* dBin, int binCount, void** wideOut,
     N* out = nullptr;
     WideTask *ta = nullptr, *tb = nullptr;
     int* ctr = nullptr;
-    int h[3] = {1, 1, 0};   // wide node count, next task count, max depth
+    int* h = nullptr;   // pinned: wide node count, next task count, max depth
     int taskCount = 1;
     const WideTask root{0, 0, 0};
     // At most one wide node per binary inner node, plus the root.
@@ -454,15 +875,17 @@ static hipError_t collapseW(const RtBvhNode* dBin, int binCount, void** wideOut,
     RT_TRY(hipMalloc(&ta, (size_t)(binCount + 1) * sizeof(WideTask)));
     RT_TRY(hipMalloc(&tb, (size_t)(binCount + 1) * sizeof(WideTask)));
     RT_TRY(hipMalloc(&ctr, 3 * sizeof(int)));
+    RT_TRY(hipHostMalloc(&h, 3 * sizeof(int), hipHostMallocDefault));
+    h[0] = 1; h[1] = 1; h[2] = 0;
     RT_TRY(hipStreamSynchronize(s));
     RT_TRY(hipMemcpy(ta, &root, sizeof(root), hipMemcpyHostToDevice));
-    RT_TRY(hipMemcpy(ctr, h, sizeof(h), hipMemcpyHostToDevice));
+    RT_TRY(hipMemcpy(ctr, h, 3 * sizeof(int), hipMemcpyHostToDevice));
     while (taskCount > 0) {
         RT_TRY(hipMemsetAsync(ctr + 1, 0, sizeof(int), s));
         hipLaunchKernelGGL((collapseKernel<N>), dim3(blocksFor(taskCount)), dim3(kB), 0, s, ta, taskCount, dBin, out,
                            ctr, tb);
         RT_TRY(hipGetLastError());
-        RT_TRY(hipMemcpyAsync(h, ctr, sizeof(h), hipMemcpyDeviceToHost, s));
+        RT_TRY(hipMemcpyAsync(h, ctr, 3 * sizeof(int), hipMemcpyDeviceToHost, s));
         RT_TRY(hipStreamSynchronize(s));
         taskCount = h[1];
         std::swap(ta, tb);
@@ -475,81 +898,124 @@ done:
     if (ta) (void)hipFree(ta);
     if (tb) (void)hipFree(tb);
     if (ctr) (void)hipFree(ctr);
+    if (h) (void)hipHostFree(h);
     return err;
 }
 
 hipError_t buildBvhGpu(const Triangle* dTris, int n, const GpuBuildParams& prm, GpuBuildResult& res, hipStream_t s) {
     hipError_t err = hipSuccess;
     const int nbins = prm.bins;
-    float4 *plo = nullptr, *phi = nullptr, *cen = nullptr;
-    int *order = nullptr, *order2 = nullptr, *pos = nullptr, *pos2 = nullptr, *lastFlag = nullptr;
+    float4 *pa = nullptr, *pb = nullptr, *pa2 = nullptr, *pb2 = nullptr;
+    float2 *pc = nullptr, *pc2 = nullptr;
+    int *order = nullptr, *pos = nullptr, *pos2 = nullptr, *lastFlag = nullptr;
     Task *tasks = nullptr, *next = nullptr;
     SplitInfo* split = nullptr;
-    int *box = nullptr, *bins = nullptr, *cursors = nullptr, *ctr = nullptr;
+    int *box = nullptr, *bins = nullptr, *cursors = nullptr, *ctr = nullptr, *small = nullptr, *small2 = nullptr;
+    int *flags = nullptr, *childBase = nullptr, *blockSums = nullptr, *blockPrefix = nullptr;
     RtBvhNode* nodes = nullptr;
-    size_t taskCap = 0, nextCap = 0, splitCap = 0, boxCap = 0, binCap = 0, curCap = 0;
-    int taskCount = 1, binSlots = 0;
-    int h[5] = {1, 0, 0, 0, 0};   // nodes, next tasks, next bin slots, max depth, partition error
+    size_t taskCap = 0, nextCap = 0, splitCap = 0, boxCap = 0, binCap = 0, curCap = 0, smallCap = 0, small2Cap = 0,
+           flagCap = 0, baseCap = 0, sumCap = 0, prefixCap = 0;
+    int taskCount = 1, binSlots = 0, smallCount = 0, nodeCount = 1, level = 0, lastLevel = 0;
+    int* h = nullptr;   // pinned: [2] next bin slots, [4] partition error, [5] next small tasks, [6] splits
     if (nbins < 2 || nbins > kMaxBins || n < 1 || n >= kLeafFirstLimit) return hipErrorInvalidValue;
-    RT_TRY(hipMalloc(&plo, (size_t)n * 16));
-    RT_TRY(hipMalloc(&phi, (size_t)n * 16));
-    RT_TRY(hipMalloc(&cen, (size_t)n * 16));
-    RT_TRY(hipMalloc(&order, (size_t)n * 4));
-    RT_TRY(hipMalloc(&order2, (size_t)n * 4));
+    RT_TRY(hipHostMalloc(&h, 7 * sizeof(int), hipHostMallocDefault));
+    for (int k = 0; k < 7; k++) h[k] = 0;
+    RT_TRY(hipMalloc(&pa, (size_t)n * 16));
+    RT_TRY(hipMalloc(&pb, (size_t)n * 16));
+    RT_TRY(hipMalloc(&pc, (size_t)n * 8));
+    RT_TRY(hipMalloc(&pa2, (size_t)n * 16));
+    RT_TRY(hipMalloc(&pb2, (size_t)n * 16));
+    RT_TRY(hipMalloc(&pc2, (size_t)n * 8));
     RT_TRY(hipMalloc(&pos, (size_t)n * 4));
     RT_TRY(hipMalloc(&pos2, (size_t)n * 4));
     RT_TRY(hipMalloc(&nodes, (size_t)(2 * n) * sizeof(RtBvhNode)));
-    RT_TRY(hipMalloc(&ctr, 5 * sizeof(int)));
-    hipLaunchKernelGGL(primPrepKernel, dim3(blocksFor(n)), dim3(kB), 0, s, dTris, n, plo, phi, cen, order, pos);
+    RT_TRY(hipMalloc(&ctr, 7 * sizeof(int)));
+    hipLaunchKernelGGL(primPrepKernel, dim3(blocksFor(n)), dim3(kB), 0, s, dTris, n, pa, pb, pc, pos);
     RT_TRY(hipGetLastError());
     {
         RT_TRY(grow(tasks, taskCap, 1));
-        Task root{0, 0, n, 0, (n <= prm.maxLeaf) ? -1 : 0};
+        Task root{0, 0, n, 0, n <= prm.maxLeaf ? kLeafTask : (n <= kSmallMax ? kSmallTask : 0)};
         binSlots = root.binSlot >= 0 ? 1 : 0;
+        smallCount = root.binSlot == kSmallTask ? 1 : 0;
+        RT_TRY(grow(small, smallCap, 1));
+        const int zero = 0;
         RT_TRY(hipStreamSynchronize(s));
         RT_TRY(hipMemcpy(tasks, &root, sizeof(root), hipMemcpyHostToDevice));
-        RT_TRY(hipMemcpy(ctr, h, sizeof(h), hipMemcpyHostToDevice));
+        RT_TRY(hipMemcpy(small, &zero, sizeof(int), hipMemcpyHostToDevice));
+        RT_TRY(hipMemcpy(ctr, h, 7 * sizeof(int), hipMemcpyHostToDevice));
     }
+    // One level of tasks per round: bounds and bins of the large tasks (position-parallel),
+    // every task's decision (leaf box, SAH split), an exclusive scan of the splits that places
+    // the children (nodes nodeCount + 2k, tasks 2k, 2k + 1), then the records' move.
     while (taskCount > 0) {
+        const int scanBlocks = (taskCount + kScanBlock - 1) / kScanBlock;
         RT_TRY(grow(box, boxCap, (size_t)taskCount * 12));
         RT_TRY(grow(split, splitCap, (size_t)taskCount));
         RT_TRY(grow(next, nextCap, (size_t)taskCount * 2));
+        RT_TRY(grow(small2, small2Cap, (size_t)taskCount * 2));
         RT_TRY(grow(cursors, curCap, (size_t)taskCount * 2));
+        RT_TRY(grow(flags, flagCap, (size_t)taskCount));
+        RT_TRY(grow(childBase, baseCap, (size_t)taskCount));
+        RT_TRY(grow(blockSums, sumCap, (size_t)scanBlocks));
+        RT_TRY(grow(blockPrefix, prefixCap, (size_t)scanBlocks));
         RT_TRY(grow(bins, binCap, (size_t)(binSlots > 0 ? binSlots : 1) * 3 * nbins * kBinInts));
-        hipLaunchKernelGGL(initBoxKernel, dim3(blocksFor((long long)taskCount * 12)), dim3(kB), 0, s, box, taskCount * 12);
         RT_TRY(hipMemsetAsync(cursors, 0, (size_t)taskCount * 2 * sizeof(int), s));
-        hipLaunchKernelGGL(taskBoundsKernel, dim3(blocksFor(n, kPer)), dim3(kB), 0, s, order, pos, plo, phi, cen, box, n);
-        RT_TRY(hipGetLastError());
-        if (binSlots > 0) {
+        if (binSlots > 0) {   // large tasks: position-parallel bounds and bins
+            hipLaunchKernelGGL(initBoxKernel, dim3(blocksFor((long long)taskCount * 12)), dim3(kB), 0, s, box,
+                               taskCount * 12);
+            hipLaunchKernelGGL(bigBoundsKernel, dim3(blocksFor(n, kPer * kChunks)), dim3(kB), 0, s, pos, tasks, pa, pb, pc, box, n);
+            RT_TRY(hipGetLastError());
             hipLaunchKernelGGL(initBinsKernel, dim3(blocksFor((long long)binSlots * 3 * nbins)), dim3(kB), 0, s, bins,
                                (long long)binSlots * 3 * nbins);
-            hipLaunchKernelGGL(binKernel, dim3(blocksFor(n, kPer)), dim3(kB), 0, s, order, pos, tasks, plo, phi, cen,
-                               box, bins, nbins, n);
+            hipLaunchKernelGGL(bigBinKernel, dim3(blocksFor(n, kPer * kChunks)), dim3(kB), 0, s, pos, tasks, pa, pb, pc, box, bins,
+                               nbins, n);
             RT_TRY(hipGetLastError());
         }
-        {
-            RT_TRY(hipMemsetAsync(ctr + 1, 0, 2 * sizeof(int), s));
-            LevelArgs la;
-            la.tasks = tasks; la.taskCount = taskCount; la.box = box; la.bins = bins; la.nbins = nbins;
-            la.maxLeaf = prm.maxLeaf; la.nodes = nodes; la.counters = ctr; la.nextTasks = next; la.split = split;
-            hipLaunchKernelGGL(sahKernel, dim3(blocksFor(taskCount)), dim3(kB), 0, s, la);
-            RT_TRY(hipGetLastError());
-        }
-        hipLaunchKernelGGL(partitionKernel, dim3(blocksFor(n)), dim3(kB), 0, s, order, order2, pos, pos2, cen, tasks,
-                           split, box, cursors, nbins, n, ctr + 4);
+        RT_TRY(hipMemsetAsync(ctr + 2, 0, sizeof(int), s));
+        RT_TRY(hipMemsetAsync(ctr + 5, 0, 2 * sizeof(int), s));
+        LevelArgs la;
+        la.tasks = tasks; la.taskCount = taskCount; la.box = box; la.bins = bins; la.nbins = nbins;
+        la.maxLeaf = prm.maxLeaf; la.pa = pa; la.pb = pb; la.nodes = nodes; la.counters = ctr; la.nextTasks = next;
+        la.split = split; la.splitFlag = flags; la.childBase = childBase; la.blockPrefix = blockPrefix;
+        la.nodeBase = nodeCount; la.smallList = small; la.smallCount = smallCount; la.nextSmallList = small2;
+        hipLaunchKernelGGL(decideKernel, dim3(blocksFor(taskCount)), dim3(kB), 0, s, la);
         RT_TRY(hipGetLastError());
-        std::swap(order, order2);
-        std::swap(pos, pos2);
-        RT_TRY(hipMemcpyAsync(h, ctr, sizeof(h), hipMemcpyDeviceToHost, s));
+        const int smallBlocks = (smallCount + kWavesPerBlock - 1) / kWavesPerBlock;
+        if (smallCount > 0) {
+            hipLaunchKernelGGL(smallTaskKernel, dim3(smallBlocks), dim3(kB), 0, s, la, pc);
+            RT_TRY(hipGetLastError());
+        }
+        hipLaunchKernelGGL(scanBlocksKernel, dim3(scanBlocks), dim3(kScanBlock), 0, s, flags, childBase, blockSums, taskCount);
+        hipLaunchKernelGGL(scanTopKernel, dim3(1), dim3(kScanBlock), 0, s, blockSums, blockPrefix, scanBlocks, ctr + 6);
+        hipLaunchKernelGGL(emitKernel, dim3(blocksFor(taskCount)), dim3(kB), 0, s, la);
+        RT_TRY(hipGetLastError());
+        if (smallCount > 0) {
+            MoveArgs m{pa, pb, pc, pa2, pb2, pc2, pos, pos2, tasks, split, box, small, smallCount, nbins, ctr + 4};
+            hipLaunchKernelGGL(smallMoveKernel, dim3(smallBlocks), dim3(kB), 0, s, m);
+            RT_TRY(hipGetLastError());
+        }
+        hipLaunchKernelGGL(partitionKernel, dim3(blocksFor(n, kPer)), dim3(kB), 0, s, pa, pb, pc, pos, pa2, pb2, pc2, pos2,
+                           tasks, split, box, cursors, nbins, n, ctr + 4);
+        RT_TRY(hipGetLastError());
+        std::swap(pa, pa2); std::swap(pb, pb2); std::swap(pc, pc2); std::swap(pos, pos2);
+        RT_TRY(hipMemcpyAsync(h, ctr, 7 * sizeof(int), hipMemcpyDeviceToHost, s));
         RT_TRY(hipStreamSynchronize(s));
         if (h[4]) { err = hipErrorUnknown; goto done; }
-        taskCount = h[1];
+        lastLevel = level++;
+        nodeCount += 2 * h[6];
+        taskCount = 2 * h[6];
         binSlots = h[2];
+        smallCount = h[5];
         std::swap(tasks, next);
         std::swap(taskCap, nextCap);
+        std::swap(small, small2);
+        std::swap(smallCap, small2Cap);
     }
-    res.binNodeCount = h[0];
-    res.depth = h[3];
+    RT_TRY(hipMalloc(&order, (size_t)n * 4));
+    hipLaunchKernelGGL(orderKernel, dim3(blocksFor(n)), dim3(kB), 0, s, pc, order, n);
+    RT_TRY(hipGetLastError());
+    res.binNodeCount = nodeCount;
+    res.depth = lastLevel;
     // leaf-ordered triangle records with last-in-leaf flags
     RT_TRY(hipMalloc(&lastFlag, (size_t)n * 4));
     RT_TRY(hipMemsetAsync(lastFlag, 0, (size_t)n * 4, s));
@@ -576,10 +1042,12 @@ hipError_t buildBvhGpu(const Triangle* dTris, int n, const GpuBuildParams& prm, 
     res.binNodes = nodes; nodes = nullptr;
     res.order = order; order = nullptr;
 done:
-    for (void* p : {(void*)plo, (void*)phi, (void*)cen, (void*)order, (void*)order2, (void*)pos, (void*)pos2,
-                    (void*)lastFlag, (void*)tasks, (void*)next, (void*)split, (void*)box, (void*)bins, (void*)cursors,
-                    (void*)ctr, (void*)nodes})
+    for (void* p : {(void*)pa, (void*)pb, (void*)pc, (void*)pa2, (void*)pb2, (void*)pc2, (void*)order, (void*)pos,
+                    (void*)pos2, (void*)lastFlag, (void*)tasks, (void*)next, (void*)split, (void*)box, (void*)bins,
+                    (void*)cursors, (void*)ctr, (void*)nodes, (void*)small, (void*)small2, (void*)flags, (void*)childBase,
+                    (void*)blockSums, (void*)blockPrefix})
         if (p) (void)hipFree(p);
+    if (h) (void)hipHostFree(h);
     return err;
 }
 

@@ -5,7 +5,7 @@
 namespace rt {
 
 struct GpuBuildParams {
-    int bins = 32;      // bins per axis (2..32)
+    int bins = 16;      // bins per axis (2..32)
     int maxLeaf = 4;    // leaf at <= maxLeaf primitives (or at depth 32)
     int width = 4;      // wide layout: 2, 4 or 8 children per node
     int quantized = 0;  // 1: QNode<width> (8-bit child bounds), 0: WideNode<width>

@@ -130,7 +130,7 @@ int RtReadAccumulation(void* session, float* out, long long floatCount);
 int RtRayCounts(void* session, unsigned long long* closest, unsigned long long* shadow);
 /* Tuning knobs: "bvh_width" (2, 4 or 8), "bvh_quantized" (0/1: 8-bit child bounds, rounded
  * outwards, so hits are unchanged), "build_bins" (SAH bins per axis of the GPU build, 2..32,
- * default 32) and "build_max_leaf" (leaf size limit, 1..16, default 4 as the reference
+ * default 16) and "build_max_leaf" (leaf size limit, 1..16, default 4 as the reference
  * builder) take effect at the next RtInitialize; "blocks_per_cu" (persistent traversal
  * workgroups per compute unit and pipeline, 1..32, default 3), "short_stack" (LDS stack entries per traversal lane,
  * 8, 12 or 16, default 12; layouts other than 4-wide full-precision nodes use 16 for 12), "while_while" (0/1: speculative while-while traversal loop, default 1),

@@ -87,6 +87,8 @@ def lib():
         L.or_scene_free.argtypes = [P]
         L.or_scene_trace_rays.argtypes = [P, P, c.c_int, c.c_int, P]
         L.or_scene_bvh_nodes.argtypes = [P]
+        L.or_scene_sah_cost.argtypes = [P]
+        L.or_scene_sah_cost.restype = c.c_double
         L.or_scene_bvh_depth.argtypes = [P]
         L.or_scene_light_area.argtypes = [P]
         L.or_scene_light_area.restype = c.c_float
@@ -254,6 +256,11 @@ class OracleScene:
     @property
     def light_area(self):
         return float(lib().or_scene_light_area(self.h))
+
+    @property
+    def sah_cost(self):
+        """SAH cost of the scene's 12-bin reference build (sah_cost)."""
+        return float(lib().or_scene_sah_cost(self.h))
 
     def trace(self, rays, any_hit=False):
         """Hit records of ray queries against the scene's own BVH (the one render uses)."""

@@ -583,6 +583,11 @@ void or_scene_trace_rays(void* h, const RtRay* rays, int count, int any, RtHit* 
     or_trace_rays((void*)&sc.bvh, sc.tris.data(), rays, count, any, hits, nullptr, nullptr);
 }
 int or_scene_bvh_nodes(void* h) { return (int)static_cast<Scene*>(h)->bvh.nodes.size(); }
+// SAH cost (or_sah_cost) of the scene's own 12-bin reference build.
+double or_scene_sah_cost(void* h) {
+    const Scene& sc = *static_cast<Scene*>(h);
+    return or_sah_cost(sc.bvh.nodes.data(), (int)sc.bvh.nodes.size());
+}
 int or_scene_bvh_depth(void* h) { return static_cast<Scene*>(h)->bvh.depth; }
 float or_scene_light_area(void* h) { return static_cast<Scene*>(h)->lightArea; }
 
