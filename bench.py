@@ -305,6 +305,34 @@ def psnr_summary(study):
             "equal_psnr_plain_spp_4spp": study["equal_psnr_plain"]["spp"] if study["equal_psnr_plain"] else None}
 
 
+def build_timing(args):
+    """--build-timing BUILDS: the GPU BVH build of a workload's scene (its bench launch options),
+    BUILDS times in fresh sessions, printed as one JSON object: the workload for profiling the
+    build kernels (rocprofv3 --kernel-trace, --pmc) apart from rendering."""
+    import native_api
+    name = args.workload if args.workload != "batch" else "geodesic"
+    desc, tris, cam, st, w, h = workload(name, args.tris)
+    cfg = dict(TUNED[name])
+    for k in ("bvh_width", "build_max_leaf", "build_bins"):
+        if getattr(args, k, None) is not None:
+            cfg[k] = getattr(args, k)
+    times = []
+    for _ in range(args.build_timing):
+        with native_api.Session() as s:
+            for k, v in cfg.items():
+                s.set_option(k, v)
+            s.initialize(tris, w, h)
+            nodes, depth, ms = s.bvh_info()
+            times.append(ms)
+        log(f"build {len(times)}: {ms:.1f} ms, {nodes} binary nodes, depth {depth}")
+    print(json.dumps({"workload": desc, "triangles": len(tris), "builds": len(times),
+                      "build_ms_min": round(min(times), 2), "build_ms_median": round(float(np.median(times)), 2),
+                      "binary_nodes": nodes, "depth": depth,
+                      "options": {k: cfg[k] for k in ("bvh_width", "build_max_leaf", "build_bins") if k in cfg}}),
+          flush=True)
+    return 0
+
+
 def psnr_study(args):
     """--psnr-study: the full study on a C3 workload, printed as one JSON object."""
     import native_api
@@ -345,6 +373,9 @@ def main():
     ap.add_argument("--pipelines", type=int, choices=(1, 2, 3, 4),
                     help="path pipelines per frame (the library's default is 4)")
     ap.add_argument("--build-max-leaf", type=int)
+    ap.add_argument("--build-bins", type=int)
+    ap.add_argument("--build-timing", type=int, default=0, metavar="BUILDS",
+                    help="only build the workload's BVH BUILDS times (for profiling the build) and exit")
     ap.add_argument("--bvh-quantized", type=int, choices=(0, 1))
     ap.add_argument("--while-while", type=int, choices=(0, 1))
     ap.add_argument("--xcd-segments", type=int, choices=(0, 1),
@@ -382,14 +413,16 @@ def main():
         return psnr_study(args)
     if args.denoise_timing:
         return denoise_timing(args)
+    if args.build_timing:
+        return build_timing(args)
     if args.workload == "batch":
         return bench_batch(args)
     import native_api
     desc, tris, cam, st, w, h = workload(args.workload, args.tris)
     log(f"workload: {desc}")
     cfg = dict(TUNED[args.workload])
-    for k in ("bvh_width", "bvh_quantized", "build_max_leaf", "blocks_per_cu", "short_stack", "refill", "refill_below",
-              "while_while", "pipelines", "tail_bounce", "xcd_segments"):
+    for k in ("bvh_width", "bvh_quantized", "build_max_leaf", "build_bins", "blocks_per_cu", "short_stack", "refill",
+              "refill_below", "while_while", "pipelines", "tail_bounce", "xcd_segments"):
         if getattr(args, k) is not None:
             cfg[k] = getattr(args, k)
     if args.autotune:
@@ -399,8 +432,8 @@ def main():
         cfg = {k: v for k, v in best.items() if k != "mrays_per_s"}
         log(f"autotune (library search): {trials} configurations in {time.perf_counter() - t0:.1f} s, best {best}")
     s = native_api.Session()
-    for k in ("bvh_width", "bvh_quantized", "build_max_leaf", "blocks_per_cu", "short_stack", "refill", "refill_below",
-              "while_while", "pipelines", "tail_bounce", "xcd_segments"):
+    for k in ("bvh_width", "bvh_quantized", "build_max_leaf", "build_bins", "blocks_per_cu", "short_stack", "refill",
+              "refill_below", "while_while", "pipelines", "tail_bounce", "xcd_segments"):
         if k in cfg:
             s.set_option(k, cfg[k])
     t0 = time.perf_counter()

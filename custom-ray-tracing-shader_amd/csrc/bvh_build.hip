@@ -9,14 +9,24 @@
 // reference's float expression, so with the same bin count the GPU tree has the same
 // partitions as the reference's; only node numbering and the order inside a leaf differ.
 //
-// Construction is level-synchronous over "tasks" (nodes still to split), each owning a
-// contiguous range of the primitive order:
-//   bounds   per-task node box and centroid box (ordered-int atomics, LDS pre-reduction
-//            when a workgroup's positions all belong to one task)
-//   bin      per-task 3 x bins bins of counts and boxes (same pre-reduction)
-//   sah      one lane per task: leaf decision or best split; allocates the two children
-//   partition scatter to the left/right child range (wave-aggregated cursors)
-// then a top-down collapse writes WideNode<W> records and the leaf-ordered triangle records.
+// Construction is level-synchronous over "tasks" (nodes still to decide), each owning a
+// contiguous range of the primitive order; primitive records (box, centroid, index) move with
+// their positions, so every pass streams them.  Per level:
+//   large tasks (> 2048 primitives): position-parallel bounds and bins.  Threads fold their
+//            positions in registers (or, at task boundaries, runs of lanes fold across the
+//            wavefront) and each workgroup adds its task's box / bins with a few atomics.
+//   small tasks: a group of lanes per task (a wavefront, or 16 / 32 lanes for tasks of at most
+//            64 primitives) reduces the bounds, bins in LDS and runs the SAH sweep across the
+//            lanes; its records move by ballot compaction.
+//   decide   one lane per large task: the SAH sweep over its bins; leaf boxes.
+//   emit     an exclusive scan of the split flags places the children (nodes and tasks), then
+//            one lane per task writes its node and classifies its children.
+//   partition the large tasks' records to their child ranges (workgroup-aggregated cursors).
+// Integer min / max / add over ordered-int boxes are exact, so no result depends on the order
+// of the atomics.  A top-down collapse then writes the W-wide nodes, and the leaf-ordered
+// triangle records are gathered.
+#include <algorithm>
+#include <mutex>
 #include <vector>
 #include "bvh_gpu.h"
 #include "rt_core.h"
@@ -45,6 +55,9 @@ constexpr int kLeafTask = -1, kSmallTask = -2;
 // Tasks up to this many primitives are binned by one wavefront; larger ones by the
 // position-parallel passes (at most n / kSmallMax of them per level).
 constexpr int kSmallMax = 2048;
+// Small tasks up to this many primitives are "tiny": a group of 16 or 32 lanes decides each,
+// several to a wavefront.
+constexpr int kTinyMax = 64;
 
 struct SplitInfo { int axis, bin, leftCount, childTask; };
 
@@ -88,62 +101,123 @@ __global__ void initBoxKernel(int* box, int count12) {
     box[i] = boxInit(i % 12);   // 0..2 node lo, 3..5 node hi, 6..8 centroid lo, 9..11 centroid hi
 }
 
+// Runs of lanes: maximal blocks of consecutive lanes of a wavefront holding one key.  A run's
+// values fold into its first lane in log2(64) shuffle steps, so a run costs one set of atomics
+// instead of one per lane.  Every lane of the wave takes part in these calls.
+__device__ __forceinline__ int runLast(long long key, int lane) {
+    const long long next = __shfl_down(key, 1, 64);
+    const unsigned long long tails = __ballot(lane == 63 || next != key);
+    return lane + (int)__builtin_ctzll(tails >> lane);
+}
+__device__ __forceinline__ bool runFirst(long long key, int lane) {
+    const long long prev = __shfl_up(key, 1, 64);
+    return lane == 0 || prev != key;
+}
+
+// Box fields as ordered ints: 0..2 node lo, 3..5 node hi, 6..8 centroid lo, 9..11 centroid hi.
+__device__ __forceinline__ bool boxFieldIsMin(int f) { return f < 3 || (f >= 6 && f < 9); }
+__device__ __forceinline__ void boxFold(int* b, float4 l, float4 h, float cz) {
+    b[0] = min(b[0], ordKey(l.x)); b[1] = min(b[1], ordKey(l.y)); b[2] = min(b[2], ordKey(l.z));
+    b[3] = max(b[3], ordKey(h.x)); b[4] = max(b[4], ordKey(h.y)); b[5] = max(b[5], ordKey(h.z));
+    b[6] = min(b[6], ordKey(l.w)); b[7] = min(b[7], ordKey(h.w)); b[8] = min(b[8], ordKey(cz));
+    b[9] = max(b[9], ordKey(l.w)); b[10] = max(b[10], ordKey(h.w)); b[11] = max(b[11], ordKey(cz));
+}
+// The box over the whole wavefront, in every lane.
+__device__ __forceinline__ void waveBox(int* b) {
+    for (int f = 0; f < 12; f++) {
+        const bool isMin = boxFieldIsMin(f);
+        for (int o = 32; o > 0; o >>= 1) {
+            const int v = __shfl_xor(b[f], o, 64);
+            b[f] = isMin ? min(b[f], v) : max(b[f], v);
+        }
+    }
+}
+// The box of each run, in the run's first lane.
+__device__ __forceinline__ void runBox(int* b, int lastLane, int lane) {
+    for (int o = 1; o < 64; o <<= 1) {
+        const bool take = lane + o <= lastLane;
+        for (int f = 0; f < 12; f++) {
+            const int v = __shfl_down(b[f], o, 64);
+            if (take) b[f] = boxFieldIsMin(f) ? min(b[f], v) : max(b[f], v);
+        }
+    }
+}
+__device__ __forceinline__ void boxAtomics(int* dst, const int* b) {
+    for (int f = 0; f < 12; f++) {
+        if (boxFieldIsMin(f)) atomicMin(&dst[f], b[f]);
+        else atomicMax(&dst[f], b[f]);
+    }
+}
+
 // Node and centroid boxes of the large tasks.  A workgroup walks kChunks consecutive chunks of
-// 1024 positions; chunks that belong to one task accumulate in LDS, flushed to the task's
-// global box (12 atomics) when the task changes or the walk ends, so a task's box takes a few
-// atomics per workgroup instead of per chunk.  Chunks holding a task boundary go per position.
-constexpr int kChunks = 16;
+// 1024 positions.  While its chunks belong to one task, each thread folds its positions into
+// registers; when the task changes (or the walk ends) the registers reduce across the
+// wavefront, then the workgroup (LDS), and the task's global box takes 12 atomics.  A chunk
+// holding a task boundary reduces each wavefront's runs of one task and issues 12 global
+// atomics per run.  Min / max over ordered ints is exact, so the boxes do not depend on order.
+constexpr int kChunks = 4;
 __device__ __forceinline__ int chunkTask(const int* posTask, const Task* tasks, int base, int last) {
     const int t0 = posTask[base], t1 = posTask[last];
     return (t0 >= 0 && t0 == t1) ? (tasks[t0].binSlot >= 0 ? t0 : -2) : -1;   // -2: one small/leaf task
 }
-__device__ __forceinline__ void flushBox(int* sBox, int* box, int task) {
-    if (task >= 0 && threadIdx.x < 12) {
+__device__ __forceinline__ void flushBoxRun(int* acc, int* sBox, int* box, int task, int lane) {
+    waveBox(acc);
+    if (lane == 0)
+        for (int f = 0; f < 12; f++) {
+            if (boxFieldIsMin(f)) atomicMin(&sBox[f], acc[f]);
+            else atomicMax(&sBox[f], acc[f]);
+        }
+    __syncthreads();
+    if (threadIdx.x < 12) {
         int* g = box + (size_t)task * 12 + threadIdx.x;
-        const int f = threadIdx.x;
-        if (f < 3 || (f >= 6 && f < 9)) atomicMin(g, sBox[f]);
-        else atomicMax(g, sBox[f]);
+        if (boxFieldIsMin(threadIdx.x)) atomicMin(g, sBox[threadIdx.x]);
+        else atomicMax(g, sBox[threadIdx.x]);
+        sBox[threadIdx.x] = boxInit(threadIdx.x);
     }
+    __syncthreads();
+    for (int f = 0; f < 12; f++) acc[f] = boxInit(f);
 }
 __global__ __launch_bounds__(kB) void bigBoundsKernel(const int* posTask, const Task* tasks, const float4* pa,
                                                       const float4* pb, const float2* pc, int* box, int n) {
     __shared__ int sBox[12];
-    __shared__ int sUniform, sRun;
+    __shared__ int sUniform;
+    const int lane = threadIdx.x & 63;
+    int acc[12];
+    for (int f = 0; f < 12; f++) acc[f] = boxInit(f);
     if (threadIdx.x < 12) sBox[threadIdx.x] = boxInit(threadIdx.x);
-    if (threadIdx.x == 0) sRun = -1;
-    __syncthreads();
+    int run = -1;   // the task the registers accumulate (the same in every thread)
     for (int ch = 0; ch < kChunks; ch++) {
         const int base = (blockIdx.x * kChunks + ch) * kB * kPer;
         if (base >= n) break;
         const int last = min(base + kB * kPer, n) - 1;
+        __syncthreads();   // the previous chunk's reads of sUniform are done
         if (threadIdx.x == 0) sUniform = chunkTask(posTask, tasks, base, last);
         __syncthreads();
-        const int uni = sUniform, run = sRun;
-        if (uni != run && run >= 0) {   // the accumulated task ends here
-            flushBox(sBox, box, run);
-            __syncthreads();
-            if (threadIdx.x < 12) sBox[threadIdx.x] = boxInit(threadIdx.x);
+        const int uni = sUniform;
+        if (uni != run) {
+            if (run >= 0) flushBoxRun(acc, sBox, box, run, lane);
+            run = uni >= 0 ? uni : -1;
         }
-        __syncthreads();
-        if (threadIdx.x == 0) sRun = uni >= 0 ? uni : -1;
-        if (uni != -2) {
+        if (uni >= 0) {
             for (int k = 0; k < kPer; k++) {
                 const int j = base + k * kB + threadIdx.x;
-                if (j > last) break;
-                const int t = uni >= 0 ? uni : posTask[j];
-                if (t < 0 || (uni < 0 && tasks[t].binSlot < 0)) continue;
-                const float4 l = pa[j], h = pb[j];
-                const float2 c = pc[j];
-                int* dst = uni >= 0 ? sBox : box + (size_t)t * 12;
-                atomicMin(&dst[0], ordKey(l.x)); atomicMin(&dst[1], ordKey(l.y)); atomicMin(&dst[2], ordKey(l.z));
-                atomicMax(&dst[3], ordKey(h.x)); atomicMax(&dst[4], ordKey(h.y)); atomicMax(&dst[5], ordKey(h.z));
-                atomicMin(&dst[6], ordKey(l.w)); atomicMin(&dst[7], ordKey(h.w)); atomicMin(&dst[8], ordKey(c.x));
-                atomicMax(&dst[9], ordKey(l.w)); atomicMax(&dst[10], ordKey(h.w)); atomicMax(&dst[11], ordKey(c.x));
+                if (j <= last) boxFold(acc, pa[j], pb[j], pc[j].x);
+            }
+        } else if (uni == -1) {
+            for (int k = 0; k < kPer; k++) {
+                const int j = base + k * kB + threadIdx.x;
+                int t = j <= last ? posTask[j] : -1;
+                if (t >= 0 && tasks[t].binSlot < 0) t = -1;
+                int b[12];
+                for (int f = 0; f < 12; f++) b[f] = boxInit(f);
+                if (t >= 0) boxFold(b, pa[j], pb[j], pc[j].x);
+                runBox(b, runLast(t, lane), lane);
+                const bool first = runFirst(t, lane);
+                if (t >= 0 && first) boxAtomics(box + (size_t)t * 12, b);
             }
         }
-        __syncthreads();
     }
-    flushBox(sBox, box, sRun);
+    if (run >= 0) flushBoxRun(acc, sBox, box, run, lane);
 }
 
 __device__ __forceinline__ void binInit(int* b) {
@@ -185,69 +259,100 @@ __device__ __forceinline__ void binAdd(int* binsOfTask, int nbins, const float* 
     }
 }
 
-// Bins of the large tasks, accumulated in LDS over runs of chunks of one task as the bounds
-// pass, flushed to the task's global bins when the task changes.
-__device__ __forceinline__ void flushBins(const int* sBins, int* bins, const Task* tasks, int task, int nbins) {
-    if (task < 0) return;
-    int* g = bins + (size_t)tasks[task].binSlot * 3 * nbins * kBinInts;
-    for (int k = threadIdx.x; k < 3 * nbins; k += kB) {
-        const int* sb = sBins + k * kBinInts;
-        if (sb[0] == 0) continue;
-        int* d = g + k * kBinInts;
-        atomicAdd(&d[0], sb[0]);
-        atomicMin(&d[1], sb[1]); atomicMin(&d[2], sb[2]); atomicMin(&d[3], sb[3]);
-        atomicMax(&d[4], sb[4]); atomicMax(&d[5], sb[5]); atomicMax(&d[6], sb[6]);
+// Adds each live lane's primitive to its task's 3 x nbins bins (counts and boxes as ordered
+// ints).  `group` names the lane's bin set (lanes of one group and one bin form runs), and each
+// run adds its folded count and box with one set of atomics.  Every lane of the wave calls it.
+__device__ __forceinline__ void binAddRuns(int* binsOfTask, int nbins, const float* lo3, const float* sc3, float4 l,
+                                           float4 h, float cz, bool live, long long group, int lane) {
+    const float cc[3] = {l.w, h.w, cz};
+    for (int a = 0; a < 3; a++) {
+        const long long key = (live && sc3[a] != 0.0f) ? (group << 6) | binIndex(cc[a], lo3[a], sc3[a], nbins) : -1;
+        int v[kBinInts] = {live ? 1 : 0, ordKey(l.x), ordKey(l.y), ordKey(l.z), ordKey(h.x), ordKey(h.y), ordKey(h.z)};
+        const int lastLane = runLast(key, lane);
+        for (int o = 1; o < 64; o <<= 1) {
+            const bool take = lane + o <= lastLane;
+            int u[kBinInts];
+            for (int q = 0; q < kBinInts; q++) u[q] = __shfl_down(v[q], o, 64);
+            if (take) {
+                v[0] += u[0];
+                for (int q = 1; q < 4; q++) v[q] = min(v[q], u[q]);
+                for (int q = 4; q < 7; q++) v[q] = max(v[q], u[q]);
+            }
+        }
+        const bool first = runFirst(key, lane);
+        if (key >= 0 && first) {
+            int* b = binsOfTask + (a * nbins + (int)(key & 63)) * kBinInts;
+            atomicAdd(&b[0], v[0]);
+            atomicMin(&b[1], v[1]); atomicMin(&b[2], v[2]); atomicMin(&b[3], v[3]);
+            atomicMax(&b[4], v[4]); atomicMax(&b[5], v[5]); atomicMax(&b[6], v[6]);
+        }
     }
+}
+
+// Bins of the large tasks.  A workgroup takes kChunks consecutive chunks of 1024 positions
+// (4096 positions, so parts of at most 3 large tasks, each longer than 2048) and bins them into
+// LDS bin sets, one per large task it meets, claimed (compare-and-swap on a tag) by the first
+// run of lanes that meets the task; runs of lanes of one task and bin add to a set with one
+// atomic set.  At the end each set is added to its task's global bins.  A chunk inside one
+// small or leaf task is skipped.
+constexpr int kBinSets = 4;
+__device__ __forceinline__ int claimBinSet(int* sTag, int task) {
+    for (int k = 0; k < kBinSets; k++) {
+        const int old = atomicCAS(&sTag[k], -1, task);
+        if (old == -1 || old == task) return k;
+    }
+    return -1;   // more tasks than sets (not possible by the sizes above): global atomics
 }
 __global__ __launch_bounds__(kB) void bigBinKernel(const int* posTask, const Task* tasks, const float4* pa,
                                                    const float4* pb, const float2* pc, const int* box, int* bins,
                                                    int nbins, int n) {
-    __shared__ int sBins[3 * kMaxBins * kBinInts];
-    __shared__ int sUniform, sRun;
-    __shared__ float sLo[3], sScale[3];
-    for (int k = threadIdx.x; k < 3 * nbins; k += kB) binInit(sBins + k * kBinInts);
-    if (threadIdx.x == 0) sRun = -1;
-    __syncthreads();
+    __shared__ int sBins[kBinSets][3 * kMaxBins * kBinInts];
+    __shared__ int sTag[kBinSets];
+    __shared__ int sSkip;
+    const int lane = threadIdx.x & 63, setInts = 3 * nbins * kBinInts;
+    for (int k = threadIdx.x; k < kBinSets * 3 * nbins; k += kB) binInit(&sBins[k / (3 * nbins)][(k % (3 * nbins)) * kBinInts]);
+    if (threadIdx.x < kBinSets) sTag[threadIdx.x] = -1;
     for (int ch = 0; ch < kChunks; ch++) {
         const int base = (blockIdx.x * kChunks + ch) * kB * kPer;
         if (base >= n) break;
         const int last = min(base + kB * kPer, n) - 1;
-        if (threadIdx.x == 0) {
-            sUniform = chunkTask(posTask, tasks, base, last);
-            if (sUniform >= 0 && sUniform != sRun) binFrame(box + (size_t)sUniform * 12, nbins, sLo, sScale);
-        }
+        __syncthreads();   // the previous chunk's reads of sSkip are done (and the sets initialised)
+        if (threadIdx.x == 0) sSkip = chunkTask(posTask, tasks, base, last) == -2;
         __syncthreads();
-        const int uni = sUniform, run = sRun;
-        if (uni != run && run >= 0) {
-            flushBins(sBins, bins, tasks, run, nbins);
-            __syncthreads();
-            for (int k = threadIdx.x; k < 3 * nbins; k += kB) binInit(sBins + k * kBinInts);
-        }
-        __syncthreads();
-        if (threadIdx.x == 0) sRun = uni >= 0 ? uni : -1;
-        if (uni != -2) {
-            for (int k = 0; k < kPer; k++) {
-                const int j = base + k * kB + threadIdx.x;
-                if (j > last) break;
-                const int t = uni >= 0 ? uni : posTask[j];
-                if (t < 0) continue;
-                int* dst;
-                float lo3[3], sc3[3];
-                if (uni >= 0) {
-                    dst = sBins;
-                    for (int a = 0; a < 3; a++) { lo3[a] = sLo[a]; sc3[a] = sScale[a]; }
-                } else {
-                    const int slot = tasks[t].binSlot;
-                    if (slot < 0) continue;
-                    dst = bins + (size_t)slot * 3 * nbins * kBinInts;
-                    binFrame(box + (size_t)t * 12, nbins, lo3, sc3);
-                }
-                binAdd(dst, nbins, lo3, sc3, pa[j], pb[j], pc[j].x);
+        if (sSkip) continue;
+        for (int k = 0; k < kPer; k++) {
+            const int j = base + k * kB + threadIdx.x;
+            int t = j <= last ? posTask[j] : -1;
+            if (t >= 0 && tasks[t].binSlot < 0) t = -1;
+            // the bin set of each run of one task, claimed by the run's first lane
+            const bool head = runFirst(t, lane);
+            int set = (t >= 0 && head) ? claimBinSet(sTag, t) : -1;
+            const unsigned long long heads = __ballot(head);
+            const int headLane = 63 - __clzll(heads & (~0ull >> (63 - lane)));
+            set = __shfl(set, headLane, 64);
+            const bool live = t >= 0;
+            float lo3[3] = {0, 0, 0}, sc3[3] = {0, 0, 0};
+            float4 l = make_float4(0, 0, 0, 0), hh = l;
+            float cz = 0;
+            int* dst = &sBins[0][0];
+            if (live) {
+                binFrame(box + (size_t)t * 12, nbins, lo3, sc3);
+                l = pa[j]; hh = pb[j]; cz = pc[j].x;
+                dst = set >= 0 ? &sBins[set][0] : bins + (size_t)tasks[t].binSlot * setInts;
             }
+            binAddRuns(dst, nbins, lo3, sc3, l, hh, cz, live, live ? t : 0, lane);
         }
-        __syncthreads();
     }
-    flushBins(sBins, bins, tasks, sRun, nbins);
+    __syncthreads();
+    for (int k = threadIdx.x; k < kBinSets * 3 * nbins; k += kB) {
+        const int set = k / (3 * nbins), task = sTag[set];
+        const int* sb = &sBins[set][(k % (3 * nbins)) * kBinInts];
+        if (task < 0 || sb[0] == 0) continue;
+        int* d = bins + (size_t)tasks[task].binSlot * setInts + (k % (3 * nbins)) * kBinInts;
+        atomicAdd(&d[0], sb[0]);
+        atomicMin(&d[1], sb[1]); atomicMin(&d[2], sb[2]); atomicMin(&d[3], sb[3]);
+        atomicMax(&d[4], sb[4]); atomicMax(&d[5], sb[5]); atomicMax(&d[6], sb[6]);
+    }
 }
 
 struct LevelArgs {
@@ -255,14 +360,14 @@ struct LevelArgs {
     int* box; const int* bins; int nbins; int maxLeaf;
     const float4 *pa, *pb;
     RtBvhNode* nodes;
-    int* counters;         // [2] next bin slots, [4] error, [5] next small tasks
+    int* counters;         // [2] next bin slots, [4] error, [5] next small, [6] splits, [7] next tiny
     Task* nextTasks; SplitInfo* split;
     int* splitFlag;        // 1 per task that splits (scanned into its child slots)
     const int* childBase;  // exclusive scan of splitFlag: split k of the level -> slots 2k, 2k + 1
     const int* blockPrefix;   // scan of the per-1024-task block sums (added to childBase)
     int nodeBase;          // nodes allocated before this level's children
-    const int* smallList; int smallCount;   // this level's small tasks
-    int* nextSmallList;
+    int* nextSmallList;    // next level's small tasks above kTinyMax primitives
+    int* nextTinyList;     // next level's small tasks of at most kTinyMax primitives
 };
 
 // The reference's SAH sweep (Native/bvh.cuh:56-100) over one task's bins: the first strictly
@@ -391,9 +496,22 @@ __global__ __launch_bounds__(kScanBlock) void scanTopKernel(const int* blockSums
     if (threadIdx.x == 0) *total = sCarry;
 }
 
+// Distinct slots from `counter` for the lanes with `want`, one atomic per wavefront (the
+// lanes' ranks in the ballot); -1 for the others.  Every active lane of the wave calls it.
+__device__ __forceinline__ int waveAlloc(int* counter, bool want) {
+    const unsigned long long m = __ballot(want);
+    if (!m) return -1;
+    const int leader = __ffsll((unsigned long long)m) - 1;
+    const int rank = (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+    int base = 0;
+    if ((int)(threadIdx.x & 63) == leader) base = atomicAdd(counter, __popcll(m));
+    base = __shfl(base, leader, 64);
+    return want ? base + rank : -1;
+}
+
 // One lane per task: the task's node and, for a split, its two children in the slots the scan
-// gave it (nodes nodeBase + 2k, 2k + 1), classified for the next level.  The atomics (next
-// small list, large-task bin slots) are wave-aggregated by the compiler.
+// gave it (nodes nodeBase + 2k, 2k + 1), classified for the next level: large children take
+// bin slots, small ones join the small or tiny list.
 __global__ void emitKernel(LevelArgs a) {
     const int i = blockIdx.x * kB + threadIdx.x;
     if (i >= a.taskCount) return;
@@ -404,59 +522,70 @@ __global__ void emitKernel(LevelArgs a) {
     nd.low[0] = ordVal(bx[0]); nd.low[1] = ordVal(bx[1]); nd.low[2] = ordVal(bx[2]);
     nd.high[0] = ordVal(bx[3]); nd.high[1] = ordVal(bx[4]); nd.high[2] = ordVal(bx[5]);
     nd.first = t.first; nd.count = t.count;
-    if (si.axis >= 0) {
-        const int k = a.childBase[i] + a.blockPrefix[i / kScanBlock];
-        const int child = a.nodeBase + 2 * k;
+    const bool isSplit = si.axis >= 0;
+    const int k = isSplit ? a.childBase[i] + a.blockPrefix[i / kScanBlock] : 0;
+    const int child = a.nodeBase + 2 * k;
+    const int counts[2] = {si.leftCount, t.count - si.leftCount};
+    const int firsts[2] = {t.first, t.first + si.leftCount};
+    for (int c = 0; c < 2; c++) {   // convergent: the slot allocations are per wavefront
+        Task ct;
+        ct.node = child + c; ct.first = firsts[c]; ct.count = counts[c]; ct.depth = t.depth + 1;
+        const bool leaf = ct.count <= a.maxLeaf || ct.depth == kBvhDepthLimit;
+        const bool large = isSplit && !leaf && ct.count > kSmallMax;
+        const bool tiny = isSplit && !leaf && !large && ct.count <= kTinyMax;
+        const bool small = isSplit && !leaf && !large && !tiny;
+        const int slot = waveAlloc(&a.counters[2], large);
+        const int tinyAt = waveAlloc(&a.counters[7], tiny);
+        const int smallAt = waveAlloc(&a.counters[5], small);
+        if (isSplit) {
+            ct.binSlot = leaf ? kLeafTask : (large ? slot : kSmallTask);
+            a.nextTasks[2 * k + c] = ct;
+            if (tiny) a.nextTinyList[tinyAt] = 2 * k + c;
+            if (small) a.nextSmallList[smallAt] = 2 * k + c;
+        }
+    }
+    if (isSplit) {
         nd.first = child; nd.count = 0;
         si.childTask = 2 * k;
-        const int counts[2] = {si.leftCount, t.count - si.leftCount};
-        const int firsts[2] = {t.first, t.first + si.leftCount};
-        for (int c = 0; c < 2; c++) {
-            Task ct;
-            ct.node = child + c; ct.first = firsts[c]; ct.count = counts[c]; ct.depth = t.depth + 1;
-            if (ct.count <= a.maxLeaf || ct.depth == kBvhDepthLimit) ct.binSlot = kLeafTask;
-            else if (ct.count <= kSmallMax) ct.binSlot = kSmallTask;
-            else ct.binSlot = atomicAdd(&a.counters[2], 1);
-            a.nextTasks[2 * k + c] = ct;
-            if (ct.binSlot == kSmallTask) a.nextSmallList[atomicAdd(&a.counters[5], 1)] = 2 * k + c;
-        }
         a.split[i] = si;
     }
     a.nodes[t.node] = nd;
 }
 
-// Inclusive scans across the first `lanes` lanes (lanes above take no part in the result).
-__device__ __forceinline__ int scanMinUp(int v, int lane) {
-    for (int o = 1; o < 32; o <<= 1) { const int u = __shfl_up(v, o, 64); if (lane >= o) v = min(v, u); }
+// Inclusive scans inside groups of G lanes (lane = the lane's index in its group), over the
+// first min(G, 32) lanes: the up scans combine lanes 0..lane, the down scans lane..n-1.
+template <int G> __device__ __forceinline__ int scanMinUp(int v, int lane) {
+    for (int o = 1; o < (G < 32 ? G : 32); o <<= 1) { const int u = __shfl_up(v, o, G); if (lane >= o) v = min(v, u); }
     return v;
 }
-__device__ __forceinline__ int scanMaxUp(int v, int lane) {
-    for (int o = 1; o < 32; o <<= 1) { const int u = __shfl_up(v, o, 64); if (lane >= o) v = max(v, u); }
+template <int G> __device__ __forceinline__ int scanMaxUp(int v, int lane) {
+    for (int o = 1; o < (G < 32 ? G : 32); o <<= 1) { const int u = __shfl_up(v, o, G); if (lane >= o) v = max(v, u); }
     return v;
 }
-__device__ __forceinline__ int scanAddUp(int v, int lane) {
-    for (int o = 1; o < 32; o <<= 1) { const int u = __shfl_up(v, o, 64); if (lane >= o) v += u; }
+template <int G> __device__ __forceinline__ int scanAddUp(int v, int lane) {
+    for (int o = 1; o < (G < 32 ? G : 32); o <<= 1) { const int u = __shfl_up(v, o, G); if (lane >= o) v += u; }
     return v;
 }
-__device__ __forceinline__ int scanMinDown(int v, int lane, int n) {
-    for (int o = 1; o < 32; o <<= 1) { const int u = __shfl_down(v, o, 64); if (lane + o < n) v = min(v, u); }
+template <int G> __device__ __forceinline__ int scanMinDown(int v, int lane, int n) {
+    for (int o = 1; o < (G < 32 ? G : 32); o <<= 1) { const int u = __shfl_down(v, o, G); if (lane + o < n) v = min(v, u); }
     return v;
 }
-__device__ __forceinline__ int scanMaxDown(int v, int lane, int n) {
-    for (int o = 1; o < 32; o <<= 1) { const int u = __shfl_down(v, o, 64); if (lane + o < n) v = max(v, u); }
+template <int G> __device__ __forceinline__ int scanMaxDown(int v, int lane, int n) {
+    for (int o = 1; o < (G < 32 ? G : 32); o <<= 1) { const int u = __shfl_down(v, o, G); if (lane + o < n) v = max(v, u); }
     return v;
 }
-__device__ __forceinline__ int scanAddDown(int v, int lane, int n) {
-    for (int o = 1; o < 32; o <<= 1) { const int u = __shfl_down(v, o, 64); if (lane + o < n) v += u; }
+template <int G> __device__ __forceinline__ int scanAddDown(int v, int lane, int n) {
+    for (int o = 1; o < (G < 32 ? G : 32); o <<= 1) { const int u = __shfl_down(v, o, G); if (lane + o < n) v += u; }
     return v;
 }
 
-// sahSweep across a wavefront, lane b = bin b: the left boxes and counts of bins 0..b and the
-// right ones of b+1..nbins-1 by prefix / suffix scans (min, max and sums are exact, so every
-// cost is the sequential sweep's float), then per axis the smallest cost at the lowest bin,
-// and across axes the first strictly better than the best so far: the sequential sweep's
-// choice.  The result is the same in every lane.
-__device__ void sahSweepWave(const int* tb, const int* bx, int count, int nbins, int lane, SplitInfo& si) {
+// sahSweep across a group of G >= nbins lanes, lane b = bin b: the left boxes and counts of
+// bins 0..b and the right ones of b+1..nbins-1 by prefix / suffix scans (min, max and sums are
+// exact, so every cost is the sequential sweep's float), then per axis the smallest cost at
+// the lowest bin, and across axes the first strictly better than the best so far: the
+// sequential sweep's choice.  The result is the same in every lane of the group.
+template <int G>
+__device__ void sahSweepGroup(const int* tb, const int* bx, int count, int nbins, int lane, SplitInfo& si) {
     const float area = halfAreaOf(ordVal(bx[0]), ordVal(bx[1]), ordVal(bx[2]), ordVal(bx[3]), ordVal(bx[4]), ordVal(bx[5]));
     float best = area * count;
     for (int axis = 0; axis < 3; axis++) {
@@ -470,16 +599,16 @@ __device__ void sahSweepWave(const int* tb, const int* bx, int count, int nbins,
             lo[q] = in ? e[1 + q] : 0x7F800000;
             hi[q] = in ? e[4 + q] : (int)(0xFF800000u ^ 0x7FFFFFFFu);
         }
-        const int lc = scanAddUp(c, lane), rc = scanAddDown(c, lane, nbins);
+        const int lc = scanAddUp<G>(c, lane), rc = scanAddDown<G>(c, lane, nbins);
         int llo[3], lhi[3], rlo[3], rhi[3];
         for (int q = 0; q < 3; q++) {
-            llo[q] = scanMinUp(lo[q], lane); lhi[q] = scanMaxUp(hi[q], lane);
-            rlo[q] = scanMinDown(lo[q], lane, nbins); rhi[q] = scanMaxDown(hi[q], lane, nbins);
+            llo[q] = scanMinUp<G>(lo[q], lane); lhi[q] = scanMaxUp<G>(hi[q], lane);
+            rlo[q] = scanMinDown<G>(lo[q], lane, nbins); rhi[q] = scanMaxDown<G>(hi[q], lane, nbins);
         }
         // the right side of split b is the suffix from bin b + 1
-        const int rcNext = __shfl_down(rc, 1, 64);
+        const int rcNext = __shfl_down(rc, 1, G);
         int rloN[3], rhiN[3];
-        for (int q = 0; q < 3; q++) { rloN[q] = __shfl_down(rlo[q], 1, 64); rhiN[q] = __shfl_down(rhi[q], 1, 64); }
+        for (int q = 0; q < 3; q++) { rloN[q] = __shfl_down(rlo[q], 1, G); rhiN[q] = __shfl_down(rhi[q], 1, G); }
         const bool valid = lane < nbins - 1 && lc > 0 && rcNext > 0;
         float cost = INFINITY;
         if (valid) {
@@ -491,55 +620,57 @@ __device__ void sahSweepWave(const int* tb, const int* bx, int count, int nbins,
         }
         // smallest cost, lowest bin on ties
         float mc = cost;
-        int mb = valid ? lane : 64;
-        for (int o = 32; o > 0; o >>= 1) {
-            const float oc = __shfl_xor(mc, o, 64);
-            const int ob = __shfl_xor(mb, o, 64);
+        int mb = valid ? lane : G;
+        for (int o = G / 2; o > 0; o >>= 1) {
+            const float oc = __shfl_xor(mc, o, G);
+            const int ob = __shfl_xor(mb, o, G);
             if (oc < mc || (oc == mc && ob < mb)) { mc = oc; mb = ob; }
         }
-        if (mb < 64 && mc < best) {
+        if (mb < G && mc < best) {
             best = mc;
-            si.axis = axis; si.bin = mb; si.leftCount = __shfl(lc, mb, 64);
+            si.axis = axis; si.bin = mb; si.leftCount = __shfl(lc, mb, G);
         }
     }
 }
 
-// One wavefront per small task: bounds (wave reductions), bins (LDS atomics) and the SAH
-// sweep (across the wave), with no global atomics; the task's records move in smallMoveKernel.
-constexpr int kWavesPerBlock = kB / 64;
-__global__ __launch_bounds__(kB) void smallTaskKernel(LevelArgs a, const float2* pc) {
-    __shared__ int sBins[kWavesPerBlock][3 * kMaxBins * kBinInts];
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int w = blockIdx.x * kWavesPerBlock + wave;
-    if (w >= a.smallCount) return;   // the whole wave leaves; no workgroup barrier below
-    const int i = a.smallList[w];
+// Small tasks are decided without global atomics by a group of G lanes each: a wavefront
+// (G = 64) for tasks above kTinyMax primitives, 16 or 32 lanes (at least the bin count) for
+// tiny ones, several groups to a wavefront.  The group reduces the task's bounds, bins its
+// primitives in its own LDS bins (one atomic set per run of lanes in one bin when a whole
+// wavefront bins a large range) and runs the SAH sweep with lane b holding bin b; the task's
+// records move in smallMoveKernel.
+template <int G> __device__ __forceinline__ void groupBox(int* b) {
+    for (int f = 0; f < 12; f++) {
+        const bool isMin = boxFieldIsMin(f);
+        for (int o = G / 2; o > 0; o >>= 1) {
+            const int v = __shfl_xor(b[f], o, G);
+            b[f] = isMin ? min(b[f], v) : max(b[f], v);
+        }
+    }
+}
+constexpr int kRunBinMin = 256;   // a wavefront bins tasks this large by runs of lanes
+template <int G>
+__global__ __launch_bounds__(kB) void smallTaskKernel(LevelArgs a, const float2* pc, const int* list, int count) {
+    constexpr int kGroups = kB / G, kGroupBins = G < kMaxBins ? G : kMaxBins;
+    __shared__ int sBins[kGroups][3 * kGroupBins * kBinInts];
+    const int group = threadIdx.x / G, lane = threadIdx.x % G;
+    const int w = blockIdx.x * kGroups + group;
+    if (w >= count) return;   // the whole group leaves; no workgroup barrier below
+    const int i = list[w];
     const Task t = a.tasks[i];
     int b[12];
     for (int f = 0; f < 12; f++) b[f] = boxInit(f);
     const int end = t.first + t.count;
-    for (int j = t.first + lane; j < end; j += 64) {
-        const float4 l = a.pa[j], h = a.pb[j];
-        const float cz = pc[j].x;
-        b[0] = min(b[0], ordKey(l.x)); b[1] = min(b[1], ordKey(l.y)); b[2] = min(b[2], ordKey(l.z));
-        b[3] = max(b[3], ordKey(h.x)); b[4] = max(b[4], ordKey(h.y)); b[5] = max(b[5], ordKey(h.z));
-        b[6] = min(b[6], ordKey(l.w)); b[7] = min(b[7], ordKey(h.w)); b[8] = min(b[8], ordKey(cz));
-        b[9] = max(b[9], ordKey(l.w)); b[10] = max(b[10], ordKey(h.w)); b[11] = max(b[11], ordKey(cz));
-    }
-    for (int f = 0; f < 12; f++) {
-        const bool isMin = f < 3 || (f >= 6 && f < 9);
-        for (int o = 32; o > 0; o >>= 1) {
-            const int v = __shfl_xor(b[f], o, 64);
-            b[f] = isMin ? min(b[f], v) : max(b[f], v);
-        }
-    }
+    for (int j = t.first + lane; j < end; j += G) boxFold(b, a.pa[j], a.pb[j], pc[j].x);
+    groupBox<G>(b);
     int* bx = a.box + (size_t)i * 12;
     if (lane < 12) {
         int v = b[0];
         for (int f = 1; f < 12; f++) if (lane == f) v = b[f];
         bx[lane] = v;
     }
-    int* bins = sBins[wave];
-    for (int k = lane; k < 3 * a.nbins; k += 64) binInit(bins + k * kBinInts);
+    int* bins = sBins[group];
+    for (int k = lane; k < 3 * a.nbins; k += G) binInit(bins + k * kBinInts);
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     float lo3[3], sc3[3];
@@ -548,11 +679,22 @@ __global__ __launch_bounds__(kB) void smallTaskKernel(LevelArgs a, const float2*
         lo3[q] = lo;
         sc3[q] = ext > 1e-12f ? a.nbins / ext : 0.0f;
     }
-    for (int j = t.first + lane; j < end; j += 64) binAdd(bins, a.nbins, lo3, sc3, a.pa[j], a.pb[j], pc[j].x);
+    if (G == 64 && t.count >= kRunBinMin) {
+        for (int j0 = t.first; j0 < end; j0 += 64) {
+            const int j = j0 + lane;
+            const bool live = j < end;
+            float4 l = make_float4(0, 0, 0, 0), h = l;
+            float cz = 0;
+            if (live) { l = a.pa[j]; h = a.pb[j]; cz = pc[j].x; }
+            binAddRuns(bins, a.nbins, lo3, sc3, l, h, cz, live, 0, lane);
+        }
+    } else {
+        for (int j = t.first + lane; j < end; j += G) binAdd(bins, a.nbins, lo3, sc3, a.pa[j], a.pb[j], pc[j].x);
+    }
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
     __builtin_amdgcn_wave_barrier();
     SplitInfo si; si.axis = -1; si.bin = -1; si.leftCount = 0; si.childTask = -1;
-    sahSweepWave(bins, b, t.count, a.nbins, lane, si);
+    sahSweepGroup<G>(bins, b, t.count, a.nbins, lane, si);
     if (!splits(si, t.count)) si.axis = -1;
     if (lane == 0) {
         a.split[i] = si;
@@ -560,9 +702,9 @@ __global__ __launch_bounds__(kB) void smallTaskKernel(LevelArgs a, const float2*
     }
 }
 
-// One wavefront per small task, after emitKernel: its records to the left / right child ranges
-// of the other buffer (ballot compaction in rounds of 64, order kept), or, for a leaf, copied
-// once and marked finished in both buffers.
+// One group of G lanes per small task (as smallTaskKernel), after emitKernel: its records to
+// the left / right child ranges of the other buffer (ballot compaction in rounds of G, order
+// kept), or, for a leaf, copied once and marked finished in both buffers.
 struct MoveArgs {
     const float4 *pa, *pb;
     const float2* pc;
@@ -570,14 +712,18 @@ struct MoveArgs {
     float2* pcOut;
     int *posTask, *posTaskOut;
     const Task* tasks; const SplitInfo* split; const int* box;
-    const int* smallList; int smallCount; int nbins;
+    const int* list; int count; int nbins;
     int* error;
 };
+template <int G>
 __global__ __launch_bounds__(kB) void smallMoveKernel(MoveArgs m) {
-    const int lane = threadIdx.x & 63;
-    const int w = blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6);
-    if (w >= m.smallCount) return;
-    const int i = m.smallList[w];
+    const int group = threadIdx.x / G, lane = threadIdx.x % G;
+    const int w = blockIdx.x * (kB / G) + group;
+    if (w >= m.count) return;
+    // this group's lanes in the wavefront's ballots
+    const unsigned long long groupMask =
+        G == 64 ? ~0ull : (((1ull << (G & 63)) - 1) << (((threadIdx.x & 63) / G) * G));
+    const int i = m.list[w];
     const Task t = m.tasks[i];
     const SplitInfo si = m.split[i];
     const bool split = si.axis >= 0;
@@ -589,7 +735,7 @@ __global__ __launch_bounds__(kB) void smallMoveKernel(MoveArgs m) {
         scale = m.nbins / (ordVal(bx[9 + si.axis]) - lo);
     }
     int nl = 0, nr = 0;
-    for (int j0 = t.first; j0 < end; j0 += 64) {
+    for (int j0 = t.first; j0 < end; j0 += G) {
         const int j = j0 + lane;
         const bool live = j < end;
         float4 l = make_float4(0, 0, 0, 0), h = l;
@@ -600,8 +746,8 @@ __global__ __launch_bounds__(kB) void smallMoveKernel(MoveArgs m) {
             const float cc = si.axis == 0 ? l.w : (si.axis == 1 ? h.w : c.x);
             right = binIndex(cc, lo, scale, m.nbins) <= si.bin ? 0 : 1;
         }
-        const unsigned long long liveMask = __ballot(live);
-        const unsigned long long mr = __ballot(live && right), ml = liveMask & ~mr;
+        const unsigned long long liveMask = __ballot(live) & groupMask;
+        const unsigned long long mr = __ballot(live && right) & groupMask, ml = liveMask & ~mr;
         const unsigned long long mine = right ? mr : ml;
         const int rank = (int)__builtin_amdgcn_mbcnt_hi((unsigned)(mine >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mine, 0u));
         if (live) {
@@ -848,38 +994,65 @@ __global__ void collapseKernel(const WideTask* tasks, int taskCount, const RtBvh
 
 inline int blocksFor(long long n, int per = 1) { return (int)((n + (long long)kB * per - 1) / ((long long)kB * per)); }
 
-template <class T> hipError_t grow(T*& p, size_t& cap, size_t need) {
-    if (need <= cap) return hipSuccess;
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    cap = need + need / 2 + 64;
-    return hipMalloc(reinterpret_cast<void**>(&p), cap * sizeof(T));
+// A build's temporaries come from one device buffer per device, kept for later builds (grown
+// when a larger scene comes, never shrunk, held until the process ends) so that a build does
+// not allocate and free about 230 bytes per triangle each time; builds on one device take
+// turns on it.
+struct BuildWorkspace { char* dev = nullptr; size_t bytes = 0; int* pinned = nullptr; };
+constexpr int kMaxBuildDevices = 64;
+std::mutex gWorkspaceMutex[kMaxBuildDevices];
+BuildWorkspace gWorkspace[kMaxBuildDevices];
+constexpr int kPinnedInts = 16;   // [0..7] level counters, [8..10] collapse counters
+
+hipError_t reserveWorkspace(BuildWorkspace& ws, size_t bytes) {
+    if (!ws.pinned) {
+        const hipError_t e = hipHostMalloc(&ws.pinned, kPinnedInts * sizeof(int), hipHostMallocDefault);
+        if (e != hipSuccess) { ws.pinned = nullptr; return e; }
+    }
+    if (bytes <= ws.bytes) return hipSuccess;
+    if (ws.dev) (void)hipFree(ws.dev);
+    ws.dev = nullptr;
+    ws.bytes = 0;
+    const hipError_t e = hipMalloc(&ws.dev, bytes);
+    if (e == hipSuccess) ws.bytes = bytes;
+    return e;
 }
+
+// Takes 256-byte aligned arrays from a buffer in order (with a null base it only adds up the
+// bytes they need).
+struct Carver {
+    char* base;
+    size_t off = 0;
+    template <class T> T* take(size_t count) {
+        T* p = base ? reinterpret_cast<T*>(base + off) : nullptr;
+        off += (count * sizeof(T) + 255) & ~(size_t)255;
+        return p;
+    }
+};
 
 #define RT_TRY(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) { err = e_; goto done; } } while (0)
 
 }  // namespace
 
+// Top-down collapse, one level of wide nodes per launch; its task lists come from the build
+// workspace `ws` (free by now), the wide nodes are the caller's.
 template <class N>
-static hipError_t collapseW(const RtBvhNode* dBin, int binCount, void** wideOut, int* wideCount, int* wideDepth,
-                            hipStream_t s) {
+static hipError_t collapseW(const RtBvhNode* dBin, int binCount, char* ws, int* h, void** wideOut, int* wideCount,
+                            int* wideDepth, hipStream_t s) {
     hipError_t err = hipSuccess;
     N* out = nullptr;
-    WideTask *ta = nullptr, *tb = nullptr;
-    int* ctr = nullptr;
-    int* h = nullptr;   // pinned: wide node count, next task count, max depth
+    Carver c{ws};
+    // At most one wide node per binary inner node, plus the root.
+    WideTask* ta = c.take<WideTask>((size_t)binCount + 1);
+    WideTask* tb = c.take<WideTask>((size_t)binCount + 1);
+    int* ctr = c.take<int>(3);
     int taskCount = 1;
     const WideTask root{0, 0, 0};
-    // At most one wide node per binary inner node, plus the root.
     RT_TRY(hipMalloc(&out, (size_t)(binCount + 1) * sizeof(N)));
-    RT_TRY(hipMalloc(&ta, (size_t)(binCount + 1) * sizeof(WideTask)));
-    RT_TRY(hipMalloc(&tb, (size_t)(binCount + 1) * sizeof(WideTask)));
-    RT_TRY(hipMalloc(&ctr, 3 * sizeof(int)));
-    RT_TRY(hipHostMalloc(&h, 3 * sizeof(int), hipHostMallocDefault));
-    h[0] = 1; h[1] = 1; h[2] = 0;
+    h[0] = 1; h[1] = 1; h[2] = 0;   // wide node count, next task count, max depth
+    RT_TRY(hipMemcpyAsync(ta, &root, sizeof(root), hipMemcpyHostToDevice, s));
+    RT_TRY(hipMemcpyAsync(ctr, h, 3 * sizeof(int), hipMemcpyHostToDevice, s));
     RT_TRY(hipStreamSynchronize(s));
-    RT_TRY(hipMemcpy(ta, &root, sizeof(root), hipMemcpyHostToDevice));
-    RT_TRY(hipMemcpy(ctr, h, 3 * sizeof(int), hipMemcpyHostToDevice));
     while (taskCount > 0) {
         RT_TRY(hipMemsetAsync(ctr + 1, 0, sizeof(int), s));
         hipLaunchKernelGGL((collapseKernel<N>), dim3(blocksFor(taskCount)), dim3(kB), 0, s, ta, taskCount, dBin, out,
@@ -895,110 +1068,147 @@ static hipError_t collapseW(const RtBvhNode* dBin, int binCount, void** wideOut,
     *wideDepth = h[2];
 done:
     if (out) (void)hipFree(out);
-    if (ta) (void)hipFree(ta);
-    if (tb) (void)hipFree(tb);
-    if (ctr) (void)hipFree(ctr);
-    if (h) (void)hipHostFree(h);
     return err;
 }
 
+// Temporaries of one build, carved from the workspace.  A level holds at most n tasks (each
+// owns at least one primitive), at most n / (kSmallMax + 1) of them large.
+struct BuildBuffers {
+    float4 *pa, *pb, *pa2, *pb2;
+    float2 *pc, *pc2;
+    int *pos, *pos2, *lastFlag;
+    Task *tasks, *next;
+    SplitInfo* split;
+    int *box, *bins, *cursors, *ctr, *small, *small2, *tiny, *tiny2, *flags, *childBase, *blockSums, *blockPrefix;
+    size_t maxTasks, maxSlots;
+    void carve(Carver& c, int n, int nbins) {
+        maxTasks = (size_t)n;
+        maxSlots = (size_t)n / (kSmallMax + 1) + 1;
+        const size_t maxScan = (maxTasks + kScanBlock - 1) / kScanBlock;
+        pa = c.take<float4>(n); pb = c.take<float4>(n); pa2 = c.take<float4>(n); pb2 = c.take<float4>(n);
+        pc = c.take<float2>(n); pc2 = c.take<float2>(n);
+        pos = c.take<int>(n); pos2 = c.take<int>(n); lastFlag = c.take<int>(n);
+        tasks = c.take<Task>(maxTasks); next = c.take<Task>(maxTasks); split = c.take<SplitInfo>(maxTasks);
+        box = c.take<int>(maxTasks * 12); bins = c.take<int>(maxSlots * 3 * nbins * kBinInts);
+        cursors = c.take<int>(maxTasks * 2); ctr = c.take<int>(8);
+        small = c.take<int>(maxTasks); small2 = c.take<int>(maxTasks);
+        tiny = c.take<int>(maxTasks); tiny2 = c.take<int>(maxTasks);
+        flags = c.take<int>(maxTasks); childBase = c.take<int>(maxTasks);
+        blockSums = c.take<int>(maxScan); blockPrefix = c.take<int>(maxScan);
+    }
+};
+
+template <int G> static void launchSmall(const LevelArgs& la, const float2* pc, const int* list, int count,
+                                         const MoveArgs& mv, bool move, hipStream_t s) {
+    const int blocks = (count + kB / G - 1) / (kB / G);
+    if (!move) {
+        hipLaunchKernelGGL(smallTaskKernel<G>, dim3(blocks), dim3(kB), 0, s, la, pc, list, count);
+    } else {
+        MoveArgs m = mv;
+        m.list = list; m.count = count;
+        hipLaunchKernelGGL(smallMoveKernel<G>, dim3(blocks), dim3(kB), 0, s, m);
+    }
+}
+
 hipError_t buildBvhGpu(const Triangle* dTris, int n, const GpuBuildParams& prm, GpuBuildResult& res, hipStream_t s) {
-    hipError_t err = hipSuccess;
     const int nbins = prm.bins;
-    float4 *pa = nullptr, *pb = nullptr, *pa2 = nullptr, *pb2 = nullptr;
-    float2 *pc = nullptr, *pc2 = nullptr;
-    int *order = nullptr, *pos = nullptr, *pos2 = nullptr, *lastFlag = nullptr;
-    Task *tasks = nullptr, *next = nullptr;
-    SplitInfo* split = nullptr;
-    int *box = nullptr, *bins = nullptr, *cursors = nullptr, *ctr = nullptr, *small = nullptr, *small2 = nullptr;
-    int *flags = nullptr, *childBase = nullptr, *blockSums = nullptr, *blockPrefix = nullptr;
-    RtBvhNode* nodes = nullptr;
-    size_t taskCap = 0, nextCap = 0, splitCap = 0, boxCap = 0, binCap = 0, curCap = 0, smallCap = 0, small2Cap = 0,
-           flagCap = 0, baseCap = 0, sumCap = 0, prefixCap = 0;
-    int taskCount = 1, binSlots = 0, smallCount = 0, nodeCount = 1, level = 0, lastLevel = 0;
-    int* h = nullptr;   // pinned: [2] next bin slots, [4] partition error, [5] next small tasks, [6] splits
     if (nbins < 2 || nbins > kMaxBins || n < 1 || n >= kLeafFirstLimit) return hipErrorInvalidValue;
-    RT_TRY(hipHostMalloc(&h, 7 * sizeof(int), hipHostMallocDefault));
-    for (int k = 0; k < 7; k++) h[k] = 0;
-    RT_TRY(hipMalloc(&pa, (size_t)n * 16));
-    RT_TRY(hipMalloc(&pb, (size_t)n * 16));
-    RT_TRY(hipMalloc(&pc, (size_t)n * 8));
-    RT_TRY(hipMalloc(&pa2, (size_t)n * 16));
-    RT_TRY(hipMalloc(&pb2, (size_t)n * 16));
-    RT_TRY(hipMalloc(&pc2, (size_t)n * 8));
-    RT_TRY(hipMalloc(&pos, (size_t)n * 4));
-    RT_TRY(hipMalloc(&pos2, (size_t)n * 4));
+    int device = 0;
+    hipError_t err = hipGetDevice(&device);
+    if (err != hipSuccess) return err;
+    if (device < 0 || device >= kMaxBuildDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(gWorkspaceMutex[device]);
+    BuildWorkspace& ws = gWorkspace[device];
+    BuildBuffers bb;
+    int *order = nullptr, *h = nullptr;
+    RtBvhNode* nodes = nullptr;
+    int taskCount = 1, binSlots = 0, smallCount = 0, tinyCount = 0, nodeCount = 1, level = 0, lastLevel = 0;
+    const int tinyG = nbins <= 16 ? 16 : 32;   // lanes per tiny task: at least one per bin
+    res.recs = nullptr;
+    {
+        Carver sizing{nullptr};
+        bb.carve(sizing, n, nbins);
+        Carver collapseSizing{nullptr};
+        collapseSizing.take<WideTask>(2 * (size_t)n); collapseSizing.take<WideTask>(2 * (size_t)n); collapseSizing.take<int>(3);
+        RT_TRY(reserveWorkspace(ws, std::max(sizing.off, collapseSizing.off)));
+        Carver c{ws.dev};
+        bb.carve(c, n, nbins);
+        h = ws.pinned;   // [2] next bin slots, [4] partition error, [5] next small, [6] splits, [7] next tiny
+    }
     RT_TRY(hipMalloc(&nodes, (size_t)(2 * n) * sizeof(RtBvhNode)));
-    RT_TRY(hipMalloc(&ctr, 7 * sizeof(int)));
-    hipLaunchKernelGGL(primPrepKernel, dim3(blocksFor(n)), dim3(kB), 0, s, dTris, n, pa, pb, pc, pos);
+    hipLaunchKernelGGL(primPrepKernel, dim3(blocksFor(n)), dim3(kB), 0, s, dTris, n, bb.pa, bb.pb, bb.pc, bb.pos);
     RT_TRY(hipGetLastError());
     {
-        RT_TRY(grow(tasks, taskCap, 1));
         Task root{0, 0, n, 0, n <= prm.maxLeaf ? kLeafTask : (n <= kSmallMax ? kSmallTask : 0)};
         binSlots = root.binSlot >= 0 ? 1 : 0;
-        smallCount = root.binSlot == kSmallTask ? 1 : 0;
-        RT_TRY(grow(small, smallCap, 1));
-        const int zero = 0;
-        RT_TRY(hipStreamSynchronize(s));
-        RT_TRY(hipMemcpy(tasks, &root, sizeof(root), hipMemcpyHostToDevice));
-        RT_TRY(hipMemcpy(small, &zero, sizeof(int), hipMemcpyHostToDevice));
-        RT_TRY(hipMemcpy(ctr, h, 7 * sizeof(int), hipMemcpyHostToDevice));
+        const bool rootTiny = root.binSlot == kSmallTask && n <= kTinyMax;
+        smallCount = root.binSlot == kSmallTask && !rootTiny ? 1 : 0;
+        tinyCount = rootTiny ? 1 : 0;
+        for (int k = 0; k < 8; k++) h[k] = 0;
+        static const int zero = 0;
+        RT_TRY(hipMemcpyAsync(bb.tasks, &root, sizeof(root), hipMemcpyHostToDevice, s));
+        RT_TRY(hipMemcpyAsync(bb.small, &zero, sizeof(int), hipMemcpyHostToDevice, s));
+        RT_TRY(hipMemcpyAsync(bb.tiny, &zero, sizeof(int), hipMemcpyHostToDevice, s));
+        RT_TRY(hipMemcpyAsync(bb.ctr, h, 8 * sizeof(int), hipMemcpyHostToDevice, s));
+        RT_TRY(hipStreamSynchronize(s));   // the root task is a stack value
     }
     // One level of tasks per round: bounds and bins of the large tasks (position-parallel),
     // every task's decision (leaf box, SAH split), an exclusive scan of the splits that places
     // the children (nodes nodeCount + 2k, tasks 2k, 2k + 1), then the records' move.
     while (taskCount > 0) {
         const int scanBlocks = (taskCount + kScanBlock - 1) / kScanBlock;
-        RT_TRY(grow(box, boxCap, (size_t)taskCount * 12));
-        RT_TRY(grow(split, splitCap, (size_t)taskCount));
-        RT_TRY(grow(next, nextCap, (size_t)taskCount * 2));
-        RT_TRY(grow(small2, small2Cap, (size_t)taskCount * 2));
-        RT_TRY(grow(cursors, curCap, (size_t)taskCount * 2));
-        RT_TRY(grow(flags, flagCap, (size_t)taskCount));
-        RT_TRY(grow(childBase, baseCap, (size_t)taskCount));
-        RT_TRY(grow(blockSums, sumCap, (size_t)scanBlocks));
-        RT_TRY(grow(blockPrefix, prefixCap, (size_t)scanBlocks));
-        RT_TRY(grow(bins, binCap, (size_t)(binSlots > 0 ? binSlots : 1) * 3 * nbins * kBinInts));
-        RT_TRY(hipMemsetAsync(cursors, 0, (size_t)taskCount * 2 * sizeof(int), s));
+        if ((size_t)taskCount > bb.maxTasks || (size_t)binSlots > bb.maxSlots) {   // the bounds of BuildBuffers
+            err = hipErrorUnknown;
+            goto done;
+        }
+        RT_TRY(hipMemsetAsync(bb.cursors, 0, (size_t)taskCount * 2 * sizeof(int), s));
         if (binSlots > 0) {   // large tasks: position-parallel bounds and bins
-            hipLaunchKernelGGL(initBoxKernel, dim3(blocksFor((long long)taskCount * 12)), dim3(kB), 0, s, box,
+            hipLaunchKernelGGL(initBoxKernel, dim3(blocksFor((long long)taskCount * 12)), dim3(kB), 0, s, bb.box,
                                taskCount * 12);
-            hipLaunchKernelGGL(bigBoundsKernel, dim3(blocksFor(n, kPer * kChunks)), dim3(kB), 0, s, pos, tasks, pa, pb, pc, box, n);
+            hipLaunchKernelGGL(bigBoundsKernel, dim3(blocksFor(n, kPer * kChunks)), dim3(kB), 0, s, bb.pos, bb.tasks,
+                               bb.pa, bb.pb, bb.pc, bb.box, n);
             RT_TRY(hipGetLastError());
-            hipLaunchKernelGGL(initBinsKernel, dim3(blocksFor((long long)binSlots * 3 * nbins)), dim3(kB), 0, s, bins,
+            hipLaunchKernelGGL(initBinsKernel, dim3(blocksFor((long long)binSlots * 3 * nbins)), dim3(kB), 0, s, bb.bins,
                                (long long)binSlots * 3 * nbins);
-            hipLaunchKernelGGL(bigBinKernel, dim3(blocksFor(n, kPer * kChunks)), dim3(kB), 0, s, pos, tasks, pa, pb, pc, box, bins,
-                               nbins, n);
+            hipLaunchKernelGGL(bigBinKernel, dim3(blocksFor(n, kPer * kChunks)), dim3(kB), 0, s, bb.pos, bb.tasks, bb.pa,
+                               bb.pb, bb.pc, bb.box, bb.bins, nbins, n);
             RT_TRY(hipGetLastError());
         }
-        RT_TRY(hipMemsetAsync(ctr + 2, 0, sizeof(int), s));
-        RT_TRY(hipMemsetAsync(ctr + 5, 0, 2 * sizeof(int), s));
+        RT_TRY(hipMemsetAsync(bb.ctr + 2, 0, sizeof(int), s));
+        RT_TRY(hipMemsetAsync(bb.ctr + 5, 0, 3 * sizeof(int), s));
         LevelArgs la;
-        la.tasks = tasks; la.taskCount = taskCount; la.box = box; la.bins = bins; la.nbins = nbins;
-        la.maxLeaf = prm.maxLeaf; la.pa = pa; la.pb = pb; la.nodes = nodes; la.counters = ctr; la.nextTasks = next;
-        la.split = split; la.splitFlag = flags; la.childBase = childBase; la.blockPrefix = blockPrefix;
-        la.nodeBase = nodeCount; la.smallList = small; la.smallCount = smallCount; la.nextSmallList = small2;
+        la.tasks = bb.tasks; la.taskCount = taskCount; la.box = bb.box; la.bins = bb.bins; la.nbins = nbins;
+        la.maxLeaf = prm.maxLeaf; la.pa = bb.pa; la.pb = bb.pb; la.nodes = nodes; la.counters = bb.ctr;
+        la.nextTasks = bb.next; la.split = bb.split; la.splitFlag = bb.flags; la.childBase = bb.childBase;
+        la.blockPrefix = bb.blockPrefix; la.nodeBase = nodeCount; la.nextSmallList = bb.small2;
+        la.nextTinyList = bb.tiny2;
+        MoveArgs mv{bb.pa, bb.pb, bb.pc, bb.pa2, bb.pb2, bb.pc2, bb.pos, bb.pos2, bb.tasks, bb.split, bb.box,
+                    nullptr, 0, nbins, bb.ctr + 4};
         hipLaunchKernelGGL(decideKernel, dim3(blocksFor(taskCount)), dim3(kB), 0, s, la);
         RT_TRY(hipGetLastError());
-        const int smallBlocks = (smallCount + kWavesPerBlock - 1) / kWavesPerBlock;
-        if (smallCount > 0) {
-            hipLaunchKernelGGL(smallTaskKernel, dim3(smallBlocks), dim3(kB), 0, s, la, pc);
-            RT_TRY(hipGetLastError());
+        if (smallCount > 0) launchSmall<64>(la, bb.pc, bb.small, smallCount, mv, false, s);
+        if (tinyCount > 0) {
+            if (tinyG == 16) launchSmall<16>(la, bb.pc, bb.tiny, tinyCount, mv, false, s);
+            else launchSmall<32>(la, bb.pc, bb.tiny, tinyCount, mv, false, s);
         }
-        hipLaunchKernelGGL(scanBlocksKernel, dim3(scanBlocks), dim3(kScanBlock), 0, s, flags, childBase, blockSums, taskCount);
-        hipLaunchKernelGGL(scanTopKernel, dim3(1), dim3(kScanBlock), 0, s, blockSums, blockPrefix, scanBlocks, ctr + 6);
+        RT_TRY(hipGetLastError());
+        hipLaunchKernelGGL(scanBlocksKernel, dim3(scanBlocks), dim3(kScanBlock), 0, s, bb.flags, bb.childBase,
+                           bb.blockSums, taskCount);
+        hipLaunchKernelGGL(scanTopKernel, dim3(1), dim3(kScanBlock), 0, s, bb.blockSums, bb.blockPrefix, scanBlocks,
+                           bb.ctr + 6);
         hipLaunchKernelGGL(emitKernel, dim3(blocksFor(taskCount)), dim3(kB), 0, s, la);
         RT_TRY(hipGetLastError());
-        if (smallCount > 0) {
-            MoveArgs m{pa, pb, pc, pa2, pb2, pc2, pos, pos2, tasks, split, box, small, smallCount, nbins, ctr + 4};
-            hipLaunchKernelGGL(smallMoveKernel, dim3(smallBlocks), dim3(kB), 0, s, m);
-            RT_TRY(hipGetLastError());
+        if (smallCount > 0) launchSmall<64>(la, bb.pc, bb.small, smallCount, mv, true, s);
+        if (tinyCount > 0) {
+            if (tinyG == 16) launchSmall<16>(la, bb.pc, bb.tiny, tinyCount, mv, true, s);
+            else launchSmall<32>(la, bb.pc, bb.tiny, tinyCount, mv, true, s);
         }
-        hipLaunchKernelGGL(partitionKernel, dim3(blocksFor(n, kPer)), dim3(kB), 0, s, pa, pb, pc, pos, pa2, pb2, pc2, pos2,
-                           tasks, split, box, cursors, nbins, n, ctr + 4);
         RT_TRY(hipGetLastError());
-        std::swap(pa, pa2); std::swap(pb, pb2); std::swap(pc, pc2); std::swap(pos, pos2);
-        RT_TRY(hipMemcpyAsync(h, ctr, 7 * sizeof(int), hipMemcpyDeviceToHost, s));
+        hipLaunchKernelGGL(partitionKernel, dim3(blocksFor(n, kPer)), dim3(kB), 0, s, bb.pa, bb.pb, bb.pc, bb.pos, bb.pa2,
+                           bb.pb2, bb.pc2, bb.pos2, bb.tasks, bb.split, bb.box, bb.cursors, nbins, n, bb.ctr + 4);
+        RT_TRY(hipGetLastError());
+        std::swap(bb.pa, bb.pa2); std::swap(bb.pb, bb.pb2); std::swap(bb.pc, bb.pc2); std::swap(bb.pos, bb.pos2);
+        RT_TRY(hipMemcpyAsync(h, bb.ctr, 8 * sizeof(int), hipMemcpyDeviceToHost, s));
         RT_TRY(hipStreamSynchronize(s));
         if (h[4]) { err = hipErrorUnknown; goto done; }
         lastLevel = level++;
@@ -1006,48 +1216,53 @@ hipError_t buildBvhGpu(const Triangle* dTris, int n, const GpuBuildParams& prm, 
         taskCount = 2 * h[6];
         binSlots = h[2];
         smallCount = h[5];
-        std::swap(tasks, next);
-        std::swap(taskCap, nextCap);
-        std::swap(small, small2);
-        std::swap(smallCap, small2Cap);
+        tinyCount = h[7];
+        std::swap(bb.tasks, bb.next);
+        std::swap(bb.small, bb.small2);
+        std::swap(bb.tiny, bb.tiny2);
     }
     RT_TRY(hipMalloc(&order, (size_t)n * 4));
-    hipLaunchKernelGGL(orderKernel, dim3(blocksFor(n)), dim3(kB), 0, s, pc, order, n);
+    hipLaunchKernelGGL(orderKernel, dim3(blocksFor(n)), dim3(kB), 0, s, bb.pc, order, n);
     RT_TRY(hipGetLastError());
     res.binNodeCount = nodeCount;
     res.depth = lastLevel;
     // leaf-ordered triangle records with last-in-leaf flags
-    RT_TRY(hipMalloc(&lastFlag, (size_t)n * 4));
-    RT_TRY(hipMemsetAsync(lastFlag, 0, (size_t)n * 4, s));
-    hipLaunchKernelGGL(leafMarkKernel, dim3(blocksFor(res.binNodeCount)), dim3(kB), 0, s, nodes, res.binNodeCount, lastFlag);
+    RT_TRY(hipMemsetAsync(bb.lastFlag, 0, (size_t)n * 4, s));
+    hipLaunchKernelGGL(leafMarkKernel, dim3(blocksFor(res.binNodeCount)), dim3(kB), 0, s, nodes, res.binNodeCount,
+                       bb.lastFlag);
     RT_TRY(hipGetLastError());
     RT_TRY(hipMalloc(&res.recs, (size_t)(n + kRecordPad) * sizeof(TriRecord)));
     RT_TRY(hipMemsetAsync(res.recs + n, 0, kRecordPad * sizeof(TriRecord), s));
-    hipLaunchKernelGGL(recordKernel, dim3(blocksFor(n)), dim3(kB), 0, s, dTris, order, lastFlag, res.recs, n);
+    hipLaunchKernelGGL(recordKernel, dim3(blocksFor(n)), dim3(kB), 0, s, dTris, order, bb.lastFlag, res.recs, n);
     RT_TRY(hipGetLastError());
-    if (prm.quantized) {
-        switch (prm.width) {
-            case 2: RT_TRY(collapseW<QNode<2>>(nodes, res.binNodeCount, &res.wideNodes, &res.wideCount, &res.wideDepth, s)); break;
-            case 8: RT_TRY(collapseW<QNode<8>>(nodes, res.binNodeCount, &res.wideNodes, &res.wideCount, &res.wideDepth, s)); break;
-            default: RT_TRY(collapseW<QNode<4>>(nodes, res.binNodeCount, &res.wideNodes, &res.wideCount, &res.wideDepth, s)); break;
-        }
-    } else {
-        switch (prm.width) {
-            case 2: RT_TRY(collapseW<WideNode<2>>(nodes, res.binNodeCount, &res.wideNodes, &res.wideCount, &res.wideDepth, s)); break;
-            case 8: RT_TRY(collapseW<WideNode<8>>(nodes, res.binNodeCount, &res.wideNodes, &res.wideCount, &res.wideDepth, s)); break;
-            default: RT_TRY(collapseW<WideNode<4>>(nodes, res.binNodeCount, &res.wideNodes, &res.wideCount, &res.wideDepth, s)); break;
+    // the collapse reuses the workspace from its start (stream order: the kernels above are done
+    // with it before the collapse's first copy)
+    {
+        int* hc = h + 8;
+        if (prm.quantized) {
+            switch (prm.width) {
+                case 2: RT_TRY(collapseW<QNode<2>>(nodes, res.binNodeCount, ws.dev, hc, &res.wideNodes, &res.wideCount, &res.wideDepth, s)); break;
+                case 8: RT_TRY(collapseW<QNode<8>>(nodes, res.binNodeCount, ws.dev, hc, &res.wideNodes, &res.wideCount, &res.wideDepth, s)); break;
+                default: RT_TRY(collapseW<QNode<4>>(nodes, res.binNodeCount, ws.dev, hc, &res.wideNodes, &res.wideCount, &res.wideDepth, s)); break;
+            }
+        } else {
+            switch (prm.width) {
+                case 2: RT_TRY(collapseW<WideNode<2>>(nodes, res.binNodeCount, ws.dev, hc, &res.wideNodes, &res.wideCount, &res.wideDepth, s)); break;
+                case 8: RT_TRY(collapseW<WideNode<8>>(nodes, res.binNodeCount, ws.dev, hc, &res.wideNodes, &res.wideCount, &res.wideDepth, s)); break;
+                default: RT_TRY(collapseW<WideNode<4>>(nodes, res.binNodeCount, ws.dev, hc, &res.wideNodes, &res.wideCount, &res.wideDepth, s)); break;
+            }
         }
     }
     RT_TRY(hipStreamSynchronize(s));
     res.binNodes = nodes; nodes = nullptr;
     res.order = order; order = nullptr;
 done:
-    for (void* p : {(void*)pa, (void*)pb, (void*)pc, (void*)pa2, (void*)pb2, (void*)pc2, (void*)order, (void*)pos,
-                    (void*)pos2, (void*)lastFlag, (void*)tasks, (void*)next, (void*)split, (void*)box, (void*)bins,
-                    (void*)cursors, (void*)ctr, (void*)nodes, (void*)small, (void*)small2, (void*)flags, (void*)childBase,
-                    (void*)blockSums, (void*)blockPrefix})
-        if (p) (void)hipFree(p);
-    if (h) (void)hipHostFree(h);
+    if (err != hipSuccess) {
+        (void)hipStreamSynchronize(s);   // nothing in flight uses the workspace or the buffers below
+        if (res.recs) { (void)hipFree(res.recs); res.recs = nullptr; }
+    }
+    if (order) (void)hipFree(order);
+    if (nodes) (void)hipFree(nodes);
     return err;
 }
 
