@@ -22,6 +22,7 @@
 //                the taps mirrored
 //   weight grad  M = output channels, N = input channels, K = pixels; operands are staged in
 //                LDS transposed so each lane reads 8 consecutive pixels
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -134,113 +135,179 @@ __global__ __launch_bounds__(kB) void convFwdKernel(ConvArgs a) {
             }
 }
 
-// The same convolution for layers with 32 or 64 output channels on large images: a block
-// computes a 4-row x 64-column output tile with all output channels (one row per wave).
-// Each 32-channel slice of the input tile plus its one-pixel halo (6 x 66 pixels) is staged
-// in LDS once and read there by all nine taps, so each input pixel leaves L2 once per
-// block instead of once per tap.  (Staging the weights in LDS as well measured no faster:
-// the larger LDS footprint halves the resident blocks.)
-constexpr int kTileW = 64, kTileH = 4, kHaloW = kTileW + 2, kHaloH = kTileH + 2;
+// The same convolution for layers with 32 or 64 output channels on large images, as a
+// persistent kernel: a block computes kTileH-row x 64-column output tiles with all output
+// channels (one row per wave), looping over tiles and 32-channel input slices ("steps").
+// Each step's input slice plus its one-pixel halo ((kTileH + 2) x 66 pixels) and the weights of that
+// slice are staged in LDS and read there by all nine taps.  The next step's halo and weights
+// are loaded into registers while this step's MFMAs run and written to LDS after it, so the
+// loads' latency hides behind the MFMAs.  Loads are raw buffer loads whose out-of-image
+// offsets lie past the buffer's end, so the hardware returns the zero padding without
+// branches.  The MFMA takes the weights as its A operand, so a lane's results are 4
+// consecutive output channels of one pixel, stored as 8 bytes straight from registers.
+constexpr int kTileW = 64, kHaloW = kTileW + 2;
+// Output rows per tile (one per wavefront of the block).
+#ifndef RT_CONV_ROWS
+#define RT_CONV_ROWS 8
+#endif
+constexpr int kTileH = RT_CONV_ROWS, kHaloH = kTileH + 2, kTileThreads = 64 * kTileH;
+constexpr int kHaloChunks = kHaloH * kHaloW * 4;                  // 16-byte chunks of one slice
+constexpr int kHaloIters = (kHaloChunks + kTileThreads - 1) / kTileThreads;
+constexpr unsigned kOutside = 0x7FFFFFF0u;                        // a buffer offset past any view
 
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t bufferOf(const void* base, unsigned bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ bf16x8 bufferLoad16(__amdgpu_buffer_rsrc_t r, unsigned offset) {
+    return __builtin_bit_cast(bf16x8, __builtin_amdgcn_raw_buffer_load_b128(r, (int)offset, 0, 0));
+}
+
+// Bias, ReLU and the stores of a finished tile row: lane (g, r) holds output channels
+// 16 j + 4 g .. + 3 of pixel 16 q + r of its wave's row (biasOf[j]: the bias of those channels).
 template <int NT>
-__global__ __launch_bounds__(kB) void convFwdTileKernel(ConvArgs a) {
+__device__ __forceinline__ void tileEpilogue(const ConvArgs& a, const f32x4 (&acc)[4][NT], const float4 (&biasOf)[NT], int x0,
+                                             int y, int nn, int g, int r) {
+    if (y >= a.h) return;
+#pragma unroll
+    for (int j = 0; j < NT; j++) {
+        const int c0 = 16 * j + 4 * g;
+        const float4 bias = biasOf[j];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int x = x0 + 16 * q + r;
+            if (x >= a.w) continue;
+            const size_t p = (size_t)(nn * a.h + y) * a.w + x;
+            float v[4] = {acc[q][j][0] + bias.x, acc[q][j][1] + bias.y, acc[q][j][2] + bias.z, acc[q][j][3] + bias.w};
+            if (a.relu)
+#pragma unroll
+                for (int i = 0; i < 4; i++) v[i] = fmaxf(v[i], 0.0f);
+            if (a.out.p && c0 < a.out.c) {
+                typedef bf16 bf16x4 __attribute__((ext_vector_type(4)));
+                bf16x4* dst = reinterpret_cast<bf16x4*>(a.out.p + p * a.out.cs + a.out.co + c0);
+                bf16x4 o;
+                if (a.accumulate) {
+                    const bf16x4 old = *dst;
+#pragma unroll
+                    for (int i = 0; i < 4; i++) v[i] += (float)old[i];
+                }
+#pragma unroll
+                for (int i = 0; i < 4; i++) o[i] = (bf16)v[i];
+                *dst = o;
+            }
+            if (a.outF && c0 < a.outFStride) {
+#pragma unroll
+                for (int i = 0; i < 4; i++)
+                    if (c0 + i < a.outFStride) a.outF[p * a.outFStride + c0 + i] = v[i];
+            }
+        }
+    }
+}
+
+// RT_CONV_WAVES_PER_EU asks the compiler for that many resident waves per SIMD (it then caps
+// the registers); RT_CONV_TAP_UNROLL limits how many taps' LDS reads it schedules together.
+#ifndef RT_CONV_TAP_UNROLL
+#define RT_CONV_TAP_UNROLL 9
+#endif
+#ifdef RT_CONV_WAVES_PER_EU
+#define RT_CONV_OCCUPANCY __attribute__((amdgpu_waves_per_eu(RT_CONV_WAVES_PER_EU)))
+#else
+#define RT_CONV_OCCUPANCY
+#endif
+template <int NT>
+__global__ __launch_bounds__(kTileThreads) RT_CONV_OCCUPANCY void convFwdTileKernel(ConvArgs a, int tilesX, int tilesY, int tileCount) {
+    constexpr int kCout = 16 * NT, kWChunks = 9 * kCout * 4, kWIters = (kWChunks + kTileThreads - 1) / kTileThreads;
     __shared__ __attribute__((aligned(16))) bf16 tile[kHaloH * kHaloW * 32];
+    __shared__ __attribute__((aligned(16))) bf16 sW[9 * kCout * 32];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int g = lane >> 4, r = lane & 15;
-    const int x0 = blockIdx.x * kTileW, y0 = blockIdx.y * kTileH, nn = blockIdx.z;
+    const int slices = a.cinPad / 32;
+    const int steps = ((tileCount - (int)blockIdx.x + (int)gridDim.x - 1) / (int)gridDim.x) * slices;
+    if (steps <= 0) return;
+    const __amdgpu_buffer_rsrc_t inBuf = bufferOf(a.in.p, (unsigned)((size_t)a.n * a.h * a.w * a.in.cs * 2));
+    const __amdgpu_buffer_rsrc_t wBuf = bufferOf(a.wt, (unsigned)((size_t)9 * a.coutPad * a.cinPad * 2));
+    // this thread's halo chunks: pixel row and column in the halo tile and channel group
+    int hRow[kHaloIters], hCol[kHaloIters];
+#pragma unroll
+    for (int it = 0; it < kHaloIters; it++) {
+        const int v = threadIdx.x + it * kTileThreads, px = v >> 2;
+        hRow[it] = v < kHaloChunks ? px / kHaloW : -1000;   // -1000: no chunk (always outside)
+        hCol[it] = px % kHaloW;
+    }
+    const int part = threadIdx.x & 3;   // every chunk index of a thread has the same low two bits
+    bf16x8 rin[kHaloIters], rw[kWIters];
+    // loads the halo and weights of step k into rin / rw
+    auto fetch = [&](int k) {
+        const int t = blockIdx.x + (k / slices) * gridDim.x, ci0 = (k % slices) * 32;
+        const int x0 = (t % tilesX) * kTileW, y0 = ((t / tilesX) % tilesY) * kTileH, nn = t / (tilesX * tilesY);
+        const unsigned chan = (unsigned)(a.in.co + ci0 + 8 * part) * 2;
+#pragma unroll
+        for (int it = 0; it < kHaloIters; it++) {
+            const int yy = y0 + hRow[it] - 1, xx = x0 + hCol[it] - 1;
+            const bool in = (unsigned)yy < (unsigned)a.h && (unsigned)xx < (unsigned)a.w;
+            const unsigned off = in ? (unsigned)(((nn * a.h + yy) * a.w + xx) * a.in.cs) * 2 + chan : kOutside;
+            rin[it] = bufferLoad16(inBuf, off);
+        }
+        if (slices > 1 || k == 0) {
+#pragma unroll
+            for (int it = 0; it < kWIters; it++) {
+                const int v = threadIdx.x + it * kTileThreads;
+                const unsigned off = v < kWChunks ? (unsigned)((v >> 2) * a.cinPad + ci0 + 8 * part) * 2 : kOutside;
+                rw[it] = bufferLoad16(wBuf, off);
+            }
+        }
+    };
+    fetch(0);
+    float4 biasOf[NT];
+#pragma unroll
+    for (int j = 0; j < NT; j++)
+        biasOf[j] = a.bias ? *reinterpret_cast<const float4*>(a.bias + 16 * j + 4 * g) : make_float4(0, 0, 0, 0);
     f32x4 acc[4][NT];
+    int px0 = 0, py0 = 0, pnn = 0;   // the tile whose results acc holds
+    for (int k = 0; k < steps; k++) {
+        const int t = blockIdx.x + (k / slices) * gridDim.x, slice = k % slices;
+        const int x0 = (t % tilesX) * kTileW, y0 = ((t / tilesX) % tilesY) * kTileH, nn = t / (tilesX * tilesY);
+        __syncthreads();   // the previous step's reads of tile / sW are done
 #pragma unroll
-    for (int t = 0; t < 4; t++)
+        for (int it = 0; it < kHaloIters; it++) {
+            const int v = threadIdx.x + it * kTileThreads;
+            if (v < kHaloChunks) *reinterpret_cast<bf16x8*>(tile + v * 8) = rin[it];
+        }
+        if (slices > 1 || k == 0) {
 #pragma unroll
-        for (int j = 0; j < NT; j++) acc[t][j] = f32x4{0, 0, 0, 0};
-    for (int ci0 = 0; ci0 < a.cinPad; ci0 += 32) {
-        __syncthreads();   // the previous slice's reads are done
-        for (int v = threadIdx.x; v < kHaloH * kHaloW * 4; v += kB) {
-            const int px = v >> 2, part = v & 3;
-            const int yy = y0 + px / kHaloW - 1, xx = x0 + px % kHaloW - 1;
-            bf16x8 val = zero8();
-            if (yy >= 0 && yy < a.h && xx >= 0 && xx < a.w)
-                val = *reinterpret_cast<const bf16x8*>(a.in.p + ((size_t)(nn * a.h + yy) * a.w + xx) * a.in.cs + a.in.co +
-                                                       ci0 + 8 * part);
-            *reinterpret_cast<bf16x8*>(tile + px * 32 + 8 * part) = val;
+            for (int it = 0; it < kWIters; it++) {
+                const int v = threadIdx.x + it * kTileThreads;
+                if (v < kWChunks) *reinterpret_cast<bf16x8*>(sW + v * 8) = rw[it];
+            }
         }
         __syncthreads();
-// Taps unrolled: the next taps' weight loads issue while this tap's MFMAs run (RtDenoise at
-// 1080p 2.93 ms against 3.01 ms with the loop kept rolled).
-#ifndef RT_TAP_UNROLL
-#define RT_TAP_UNROLL 9
-#endif
-#pragma unroll RT_TAP_UNROLL
+        // The previous tile's stores go out here, after this step's wait for its loads and
+        // before the next step's loads: the wait at the next step (which counts stores too)
+        // then finds both done behind this step's MFMAs.
+        if (slice == 0 && k > 0) tileEpilogue<NT>(a, acc, biasOf, px0, py0 + wave, pnn, g, r);
+        if (k + 1 < steps) fetch(k + 1);
+        if (slice == 0) {
+            px0 = x0; py0 = y0; pnn = nn;
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+#pragma unroll
+                for (int j = 0; j < NT; j++) acc[q][j] = f32x4{0, 0, 0, 0};
+        }
+#pragma unroll RT_CONV_TAP_UNROLL
         for (int tap = 0; tap < 9; tap++) {
             const int dy = tap / 3, dx = tap % 3;
             bf16x8 av[4];
 #pragma unroll
-            for (int t = 0; t < 4; t++)
-                av[t] = *reinterpret_cast<const bf16x8*>(tile + ((wave + dy) * kHaloW + 16 * t + r + dx) * 32 + 8 * g);
-            const bf16* wrow = a.wt + ((size_t)tap * a.coutPad + r) * a.cinPad + ci0 + 8 * g;
+            for (int q = 0; q < 4; q++)
+                av[q] = *reinterpret_cast<const bf16x8*>(tile + ((wave + dy) * kHaloW + 16 * q + r + dx) * 32 + 8 * g);
 #pragma unroll
             for (int j = 0; j < NT; j++) {
-                const bf16x8 bv = *reinterpret_cast<const bf16x8*>(wrow + (size_t)16 * j * a.cinPad);
+                const bf16x8 wv = *reinterpret_cast<const bf16x8*>(sW + (tap * kCout + 16 * j + r) * 32 + 8 * g);
 #pragma unroll
-                for (int t = 0; t < 4; t++) acc[t][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[t], bv, acc[t][j], 0, 0, 0);
+                for (int q = 0; q < 4; q++) acc[q][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wv, av[q], acc[q][j], 0, 0, 0);
             }
         }
     }
-    const int y = y0 + wave;
-    // Plain BF16 output with every channel kept: the results go through LDS, 32 channels at a
-    // time, so each lane stores 16 contiguous bytes of a pixel instead of scattered 2-byte
-    // values (the MFMA result layout gives a lane one channel of four pixels).
-    if (a.out.p && !a.accumulate && !a.outF && a.out.c == 16 * NT && a.out.cs % 8 == 0 && a.out.co % 8 == 0) {
-        bf16* stage = tile + wave * kTileW * 32;   // [x][32 channels] of this wave's row
-#pragma unroll
-        for (int jh = 0; jh < NT / 2; jh++) {
-            __syncthreads();   // tile reads (or the previous half's stores) are done
-#pragma unroll
-            for (int t = 0; t < 4; t++)
-#pragma unroll
-                for (int jj = 0; jj < 2; jj++)
-#pragma unroll
-                    for (int i = 0; i < 4; i++) {
-                        const int c = 16 * (2 * jh + jj) + r;
-                        float v = acc[t][2 * jh + jj][i] + (a.bias ? a.bias[c] : 0.0f);
-                        if (a.relu) v = fmaxf(v, 0.0f);
-                        stage[(16 * t + 4 * g + i) * 32 + 16 * jj + r] = (bf16)v;
-                    }
-            __syncthreads();
-            if (y < a.h) {
-#pragma unroll
-                for (int it = 0; it < 4; it++) {
-                    const int q = it * 64 + lane, x = q >> 2, part = q & 3;
-                    if (x0 + x < a.w) {
-                        const bf16x8 v = *reinterpret_cast<const bf16x8*>(stage + x * 32 + 8 * part);
-                        *reinterpret_cast<bf16x8*>(a.out.p + ((size_t)(nn * a.h + y) * a.w + x0 + x) * a.out.cs + a.out.co +
-                                                   32 * jh + 8 * part) = v;
-                    }
-                }
-            }
-        }
-        return;
-    }
-    if (y >= a.h) return;
-#pragma unroll
-    for (int t = 0; t < 4; t++)
-#pragma unroll
-        for (int j = 0; j < NT; j++)
-#pragma unroll
-            for (int i = 0; i < 4; i++) {
-                const int x = x0 + 16 * t + 4 * g + i;
-                if (x >= a.w) continue;
-                const size_t p = (size_t)(nn * a.h + y) * a.w + x;
-                const int c = 16 * j + r;
-                float v = acc[t][j][i] + (a.bias ? a.bias[c] : 0.0f);
-                if (a.relu) v = fmaxf(v, 0.0f);
-                if (a.out.p && c < a.out.c) {
-                    bf16* dst = a.out.p + p * a.out.cs + a.out.co + c;
-                    if (a.accumulate) v += (float)*dst;
-                    *dst = (bf16)v;
-                }
-                if (a.outF && c < a.outFStride) a.outF[p * a.outFStride + c] = v;
-            }
+    tileEpilogue<NT>(a, acc, biasOf, px0, py0 + wave, pnn, g, r);
 }
 
 struct BwdWArgs {
@@ -610,10 +677,27 @@ hipError_t launchConvFwd(const ConvArgs& a, hipStream_t s) {
     const int npix = a.n * a.h * a.w;
     const int tiles = a.coutPad / 16;
     if (a.cinPad % 32 || a.coutPad % 32 || a.in.c != a.cinPad) return hipErrorInvalidValue;
-    if ((a.coutPad == 32 || a.coutPad == 64) && (long long)a.w * a.h >= 16384) {
-        const dim3 grid((a.w + kTileW - 1) / kTileW, (a.h + kTileH - 1) / kTileH, a.n);
-        if (a.coutPad == 32) hipLaunchKernelGGL(convFwdTileKernel<2>, grid, dim3(kB), 0, s, a);
-        else hipLaunchKernelGGL(convFwdTileKernel<4>, grid, dim3(kB), 0, s, a);
+    if ((a.coutPad == 32 || a.coutPad == 64) && (long long)a.w * a.h >= 16384 &&
+        (long long)a.n * a.h * a.w * a.in.cs * 2 < (long long)kOutside && (long long)9 * a.coutPad * a.cinPad * 2 < kOutside &&
+        (a.in.cs | a.in.co) % 8 == 0 && (!a.out.p || (a.out.cs | a.out.co) % 4 == 0)) {
+        const int tilesX = (a.w + kTileW - 1) / kTileW, tilesY = (a.h + kTileH - 1) / kTileH;
+        const int tileCount = tilesX * tilesY * a.n;
+        // persistent: as many blocks as the GPU holds at once (fewer for small images)
+        static int resident[2] = {0, 0};
+        const int k = a.coutPad == 32 ? 0 : 1;
+        if (!resident[k]) {
+            int dev = 0, cus = 0, perCu = 0;
+            hipError_t e = hipGetDevice(&dev);
+            if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+            if (e == hipSuccess)
+                e = k == 0 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, convFwdTileKernel<2>, kTileThreads, 0)
+                           : hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, convFwdTileKernel<4>, kTileThreads, 0);
+            if (e != hipSuccess) return e;
+            resident[k] = std::max(1, cus * std::max(1, perCu));
+        }
+        const dim3 grid(std::min(tileCount, resident[k]));
+        if (a.coutPad == 32) hipLaunchKernelGGL(convFwdTileKernel<2>, grid, dim3(kTileThreads), 0, s, a, tilesX, tilesY, tileCount);
+        else hipLaunchKernelGGL(convFwdTileKernel<4>, grid, dim3(kTileThreads), 0, s, a, tilesX, tilesY, tileCount);
         return hipGetLastError();
     }
     if (tiles % 4 == 0) hipLaunchKernelGGL(convFwdKernel<4>, dim3(blocks(npix), tiles / 4), dim3(kB), 0, s, a);
