@@ -64,6 +64,11 @@ struct ConvArgs {
     int n, h, w;
     int cinPad, coutPad;
     int relu, accumulate;
+    // Optional: input channels [0, lowC) are the nearest-neighbour upsampling of `low`, a
+    // (h / 2) x (w / 2) buffer (read in place, so the upsampled tensor is never written);
+    // channels lowC.. come from `in` as usual.  Tile kernel only.
+    View low;
+    int lowC;
 };
 
 template <int NT>
@@ -135,9 +140,10 @@ __global__ __launch_bounds__(kB) void convFwdKernel(ConvArgs a) {
             }
 }
 
-// The same convolution for layers with 32 or 64 output channels on large images, as a
-// persistent kernel: a block computes kTileH-row x 64-column output tiles with all output
-// channels (one row per wave), looping over tiles and 32-channel input slices ("steps").
+// The same convolution for images at least 48 pixels wide (or 16384 pixels), as a
+// persistent kernel: a block computes kTileH-row x 64-column output tiles of one group of
+// 16 NT output channels (blockIdx.y; one row per wave), looping over tiles and 32-channel
+// input slices ("steps").
 // Each step's input slice plus its one-pixel halo ((kTileH + 2) x 66 pixels) and the weights of that
 // slice are staged in LDS and read there by all nine taps.  The next step's halo and weights
 // are loaded into registers while this step's MFMAs run and written to LDS after it, so the
@@ -163,14 +169,14 @@ __device__ __forceinline__ bf16x8 bufferLoad16(__amdgpu_buffer_rsrc_t r, unsigne
 }
 
 // Bias, ReLU and the stores of a finished tile row: lane (g, r) holds output channels
-// 16 j + 4 g .. + 3 of pixel 16 q + r of its wave's row (biasOf[j]: the bias of those channels).
+// cBase + 16 j + 4 g .. + 3 of pixel 16 q + r of its wave's row (biasOf[j]: their bias).
 template <int NT>
-__device__ __forceinline__ void tileEpilogue(const ConvArgs& a, const f32x4 (&acc)[4][NT], const float4 (&biasOf)[NT], int x0,
-                                             int y, int nn, int g, int r) {
+__device__ __forceinline__ void tileEpilogue(const ConvArgs& a, const f32x4 (&acc)[4][NT], const float4 (&biasOf)[NT],
+                                             int cBase, int x0, int y, int nn, int g, int r) {
     if (y >= a.h) return;
 #pragma unroll
     for (int j = 0; j < NT; j++) {
-        const int c0 = 16 * j + 4 * g;
+        const int c0 = cBase + 16 * j + 4 * g;
         const float4 bias = biasOf[j];
 #pragma unroll
         for (int q = 0; q < 4; q++) {
@@ -220,11 +226,13 @@ __global__ __launch_bounds__(kTileThreads) RT_CONV_OCCUPANCY void convFwdTileKer
     __shared__ __attribute__((aligned(16))) bf16 sW[9 * kCout * 32];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int g = lane >> 4, r = lane & 15;
-    const int slices = a.cinPad / 32;
+    const int slices = a.cinPad / 32, cBase = blockIdx.y * kCout;   // this block's output channels
     const int steps = ((tileCount - (int)blockIdx.x + (int)gridDim.x - 1) / (int)gridDim.x) * slices;
     if (steps <= 0) return;
     const __amdgpu_buffer_rsrc_t inBuf = bufferOf(a.in.p, (unsigned)((size_t)a.n * a.h * a.w * a.in.cs * 2));
     const __amdgpu_buffer_rsrc_t wBuf = bufferOf(a.wt, (unsigned)((size_t)9 * a.coutPad * a.cinPad * 2));
+    const __amdgpu_buffer_rsrc_t lowBuf =
+        bufferOf(a.low.p ? a.low.p : a.in.p, (unsigned)((size_t)a.n * (a.h / 2) * (a.w / 2) * a.low.cs * 2));
     // this thread's halo chunks: pixel row and column in the halo tile and channel group
     int hRow[kHaloIters], hCol[kHaloIters];
 #pragma unroll
@@ -239,19 +247,34 @@ __global__ __launch_bounds__(kTileThreads) RT_CONV_OCCUPANCY void convFwdTileKer
     auto fetch = [&](int k) {
         const int t = blockIdx.x + (k / slices) * gridDim.x, ci0 = (k % slices) * 32;
         const int x0 = (t % tilesX) * kTileW, y0 = ((t / tilesX) % tilesY) * kTileH, nn = t / (tilesX * tilesY);
-        const unsigned chan = (unsigned)(a.in.co + ci0 + 8 * part) * 2;
+        if (ci0 < a.lowC) {   // upsampled channels: the half-resolution pixel under each halo pixel
+            const unsigned chan = (unsigned)(a.low.co + ci0 + 8 * part) * 2;
+            const int hl = a.h / 2, wl = a.w / 2;
 #pragma unroll
-        for (int it = 0; it < kHaloIters; it++) {
-            const int yy = y0 + hRow[it] - 1, xx = x0 + hCol[it] - 1;
-            const bool in = (unsigned)yy < (unsigned)a.h && (unsigned)xx < (unsigned)a.w;
-            const unsigned off = in ? (unsigned)(((nn * a.h + yy) * a.w + xx) * a.in.cs) * 2 + chan : kOutside;
-            rin[it] = bufferLoad16(inBuf, off);
+            for (int it = 0; it < kHaloIters; it++) {
+                const int yy = y0 + hRow[it] - 1, xx = x0 + hCol[it] - 1;
+                const bool in = (unsigned)yy < (unsigned)a.h && (unsigned)xx < (unsigned)a.w;
+                const unsigned off =
+                    in ? (unsigned)(((nn * hl + (yy >> 1)) * wl + (xx >> 1)) * a.low.cs) * 2 + chan : kOutside;
+                rin[it] = bufferLoad16(lowBuf, off);
+            }
+        } else {
+            const unsigned chan = (unsigned)(a.in.co + ci0 + 8 * part) * 2;
+#pragma unroll
+            for (int it = 0; it < kHaloIters; it++) {
+                const int yy = y0 + hRow[it] - 1, xx = x0 + hCol[it] - 1;
+                const bool in = (unsigned)yy < (unsigned)a.h && (unsigned)xx < (unsigned)a.w;
+                const unsigned off = in ? (unsigned)(((nn * a.h + yy) * a.w + xx) * a.in.cs) * 2 + chan : kOutside;
+                rin[it] = bufferLoad16(inBuf, off);
+            }
         }
         if (slices > 1 || k == 0) {
 #pragma unroll
             for (int it = 0; it < kWIters; it++) {
                 const int v = threadIdx.x + it * kTileThreads;
-                const unsigned off = v < kWChunks ? (unsigned)((v >> 2) * a.cinPad + ci0 + 8 * part) * 2 : kOutside;
+                const int row = v >> 2, tap = row / kCout, co = row % kCout;
+                const unsigned off =
+                    v < kWChunks ? (unsigned)(((tap * a.coutPad + cBase + co) * a.cinPad) + ci0 + 8 * part) * 2 : kOutside;
                 rw[it] = bufferLoad16(wBuf, off);
             }
         }
@@ -260,7 +283,7 @@ __global__ __launch_bounds__(kTileThreads) RT_CONV_OCCUPANCY void convFwdTileKer
     float4 biasOf[NT];
 #pragma unroll
     for (int j = 0; j < NT; j++)
-        biasOf[j] = a.bias ? *reinterpret_cast<const float4*>(a.bias + 16 * j + 4 * g) : make_float4(0, 0, 0, 0);
+        biasOf[j] = a.bias ? *reinterpret_cast<const float4*>(a.bias + cBase + 16 * j + 4 * g) : make_float4(0, 0, 0, 0);
     f32x4 acc[4][NT];
     int px0 = 0, py0 = 0, pnn = 0;   // the tile whose results acc holds
     for (int k = 0; k < steps; k++) {
@@ -283,7 +306,7 @@ __global__ __launch_bounds__(kTileThreads) RT_CONV_OCCUPANCY void convFwdTileKer
         // The previous tile's stores go out here, after this step's wait for its loads and
         // before the next step's loads: the wait at the next step (which counts stores too)
         // then finds both done behind this step's MFMAs.
-        if (slice == 0 && k > 0) tileEpilogue<NT>(a, acc, biasOf, px0, py0 + wave, pnn, g, r);
+        if (slice == 0 && k > 0) tileEpilogue<NT>(a, acc, biasOf, cBase, px0, py0 + wave, pnn, g, r);
         if (k + 1 < steps) fetch(k + 1);
         if (slice == 0) {
             px0 = x0; py0 = y0; pnn = nn;
@@ -307,7 +330,7 @@ __global__ __launch_bounds__(kTileThreads) RT_CONV_OCCUPANCY void convFwdTileKer
             }
         }
     }
-    tileEpilogue<NT>(a, acc, biasOf, px0, py0 + wave, pnn, g, r);
+    tileEpilogue<NT>(a, acc, biasOf, cBase, px0, py0 + wave, pnn, g, r);
 }
 
 struct BwdWArgs {
@@ -673,31 +696,42 @@ bf16* wbOf(DenoiserNet* net, int l) {
     return net->wbf + L.bfOff + 9LL * L.coutPad * L.cinPad;
 }
 
+// Whether launchConvFwd runs a convolution on the tile kernel.
+bool tileConv(const ConvArgs& a) {
+    return a.coutPad % 32 == 0 && (a.w >= 48 || (long long)a.w * a.h >= 16384) &&
+           (long long)a.n * a.h * a.w * a.in.cs * 2 < (long long)kOutside &&
+           (long long)9 * a.coutPad * a.cinPad * 2 < kOutside && (a.in.cs | a.in.co) % 8 == 0 &&
+           (!a.out.p || (a.out.cs | a.out.co) % 4 == 0) &&
+           (!a.low.p || (a.lowC % 32 == 0 && a.h % 2 == 0 && a.w % 2 == 0 && (a.low.cs | a.low.co) % 8 == 0));
+}
+
 hipError_t launchConvFwd(const ConvArgs& a, hipStream_t s) {
     const int npix = a.n * a.h * a.w;
     const int tiles = a.coutPad / 16;
     if (a.cinPad % 32 || a.coutPad % 32 || a.in.c != a.cinPad) return hipErrorInvalidValue;
-    if ((a.coutPad == 32 || a.coutPad == 64) && (long long)a.w * a.h >= 16384 &&
-        (long long)a.n * a.h * a.w * a.in.cs * 2 < (long long)kOutside && (long long)9 * a.coutPad * a.cinPad * 2 < kOutside &&
-        (a.in.cs | a.in.co) % 8 == 0 && (!a.out.p || (a.out.cs | a.out.co) % 4 == 0)) {
+    if (a.low.p && !tileConv(a)) return hipErrorInvalidValue;   // only the tile kernel reads upsampled channels
+    if (tileConv(a)) {
         const int tilesX = (a.w + kTileW - 1) / kTileW, tilesY = (a.h + kTileH - 1) / kTileH;
         const int tileCount = tilesX * tilesY * a.n;
         // persistent: as many blocks as the GPU holds at once (fewer for small images)
-        static int resident[2] = {0, 0};
-        const int k = a.coutPad == 32 ? 0 : 1;
-        if (!resident[k]) {
-            int dev = 0, cus = 0, perCu = 0;
+        static int resident[2] = {0, 0}, cus = 0;
+        if (!resident[0]) {
+            int dev = 0, perCu2 = 0, perCu4 = 0;
             hipError_t e = hipGetDevice(&dev);
             if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-            if (e == hipSuccess)
-                e = k == 0 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, convFwdTileKernel<2>, kTileThreads, 0)
-                           : hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, convFwdTileKernel<4>, kTileThreads, 0);
+            if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu2, convFwdTileKernel<2>, kTileThreads, 0);
+            if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu4, convFwdTileKernel<4>, kTileThreads, 0);
             if (e != hipSuccess) return e;
-            resident[k] = std::max(1, cus * std::max(1, perCu));
+            resident[0] = std::max(1, cus * std::max(1, perCu2));
+            resident[1] = std::max(1, cus * std::max(1, perCu4));
         }
-        const dim3 grid(std::min(tileCount, resident[k]));
-        if (a.coutPad == 32) hipLaunchKernelGGL(convFwdTileKernel<2>, grid, dim3(kTileThreads), 0, s, a, tilesX, tilesY, tileCount);
-        else hipLaunchKernelGGL(convFwdTileKernel<4>, grid, dim3(kTileThreads), 0, s, a, tilesX, tilesY, tileCount);
+        // 64-channel groups read each halo once for twice the channels; 32-channel groups make
+        // twice the blocks, for small images that would leave compute units idle
+        const bool wide = a.coutPad % 64 == 0 && (long long)tileCount * (a.coutPad / 64) >= 2LL * cus;
+        const int groups = a.coutPad / (wide ? 64 : 32);
+        const dim3 grid(std::min(tileCount, std::max(1, resident[wide ? 1 : 0] / groups)), groups);
+        if (wide) hipLaunchKernelGGL(convFwdTileKernel<4>, grid, dim3(kTileThreads), 0, s, a, tilesX, tilesY, tileCount);
+        else hipLaunchKernelGGL(convFwdTileKernel<2>, grid, dim3(kTileThreads), 0, s, a, tilesX, tilesY, tileCount);
         return hipGetLastError();
     }
     if (tiles % 4 == 0) hipLaunchKernelGGL(convFwdKernel<4>, dim3(blocks(npix), tiles / 4), dim3(kB), 0, s, a);
@@ -782,14 +816,18 @@ hipError_t ensureActivations(DenoiserNet* net, int n, int h, int w, bool trainin
     return hipSuccess;
 }
 
-hipError_t conv(DenoiserNet* net, int l, const View& in, const View& out, int n, int h, int w, hipStream_t s,
-                float* outF = nullptr) {
+ConvArgs convArgs(DenoiserNet* net, int l, const View& in, const View& out, int n, int h, int w, float* outF = nullptr) {
     const DnLayer& L = net->layers[l];
     ConvArgs a{};
     a.in = in; a.wt = wfOf(net, l); a.bias = net->params + L.bOff; a.out = out;
     a.outF = outF; a.outFStride = 4;
     a.n = n; a.h = h; a.w = w; a.cinPad = L.cinPad; a.coutPad = L.coutPad; a.relu = L.relu; a.accumulate = 0;
-    return launchConvFwd(a, s);
+    return a;
+}
+
+hipError_t conv(DenoiserNet* net, int l, const View& in, const View& out, int n, int h, int w, hipStream_t s,
+                float* outF = nullptr) {
+    return launchConvFwd(convArgs(net, l, in, out, n, h, w, outF), s);
 }
 
 hipError_t pool(const View& in, const View& out, int n, int h, int w, int c, hipStream_t s) {
@@ -803,8 +841,23 @@ hipError_t upsample(const View& in, const View& out, int n, int h, int w, int c,
 
 #define DN_TRY(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) return e_; } while (0)
 
-// Forward pass from net->x0 to net->outF.
-hipError_t forward(DenoiserNet* net, hipStream_t s) {
+// The first decoder convolution of a level: upsample(up) joins the skip channels already in
+// the concatenation buffer (in); with `fuse` (inference) the convolution reads the
+// upsampled channels from `up` in place when the tile kernel runs it.
+hipError_t upConv(DenoiserNet* net, int l, const View& up, int upC, const View& in, const View& out, int n, int h, int w,
+                  bool fuse, hipStream_t s) {
+    ConvArgs a = convArgs(net, l, in, out, n, h, w);
+    a.low = up; a.lowC = upC;
+    if (fuse && tileConv(a)) return launchConvFwd(a, s);
+    DN_TRY(upsample(up, View{in.p, in.cs, in.co, upC}, n, h, w, upC, s));
+    a.low = View{nullptr, 0, 0, 0}; a.lowC = 0;
+    return launchConvFwd(a, s);
+}
+
+// Forward pass from net->x0 to net->outF.  fuseUp (inference): decoder convolutions read
+// their upsampled inputs in place; training keeps the concatenation buffers whole for the
+// weight gradients.
+hipError_t forward(DenoiserNet* net, hipStream_t s, bool fuseUp = false) {
     const int n = net->n, h0 = net->h, w0 = net->w;
     const int h1 = h0 / 2, w1 = w0 / 2, h2 = h0 / 4, w2 = w0 / 4, h3 = h0 / 8, w3 = w0 / 8, h4 = h0 / 16, w4 = w0 / 16;
     DN_TRY(conv(net, E0A, V(net->x0, 32), V(net->e0a, 32), n, h0, w0, s));
@@ -824,14 +877,11 @@ hipError_t forward(DenoiserNet* net, hipStream_t s) {
     DN_TRY(upsample(V(net->bb, 128), V(net->D3, 224, 0, 128), n, h3, w3, 128, s));
     DN_TRY(conv(net, D3A, V(net->D3, 224), V(net->d3a, 96), n, h3, w3, s));
     DN_TRY(conv(net, D3B, V(net->d3a, 96), V(net->d3b, 96), n, h3, w3, s));
-    DN_TRY(upsample(V(net->d3b, 96), V(net->D2, 160, 0, 96), n, h2, w2, 96, s));
-    DN_TRY(conv(net, D2A, V(net->D2, 160), V(net->d2a, 64), n, h2, w2, s));
+    DN_TRY(upConv(net, D2A, V(net->d3b, 96), 96, V(net->D2, 160), V(net->d2a, 64), n, h2, w2, fuseUp, s));
     DN_TRY(conv(net, D2B, V(net->d2a, 64), V(net->d2b, 64), n, h2, w2, s));
-    DN_TRY(upsample(V(net->d2b, 64), V(net->D1, 128, 0, 64), n, h1, w1, 64, s));
-    DN_TRY(conv(net, D1A, V(net->D1, 128), V(net->d1a, 64), n, h1, w1, s));
+    DN_TRY(upConv(net, D1A, V(net->d2b, 64), 64, V(net->D1, 128), V(net->d1a, 64), n, h1, w1, fuseUp, s));
     DN_TRY(conv(net, D1B, V(net->d1a, 64), V(net->d1b, 64), n, h1, w1, s));
-    DN_TRY(upsample(V(net->d1b, 64), V(net->D0, 96, 0, 64), n, h0, w0, 64, s));
-    DN_TRY(conv(net, D0A, V(net->D0, 96), V(net->d0a, 32), n, h0, w0, s));
+    DN_TRY(upConv(net, D0A, V(net->d1b, 64), 64, V(net->D0, 96), V(net->d0a, 32), n, h0, w0, fuseUp, s));
     DN_TRY(conv(net, D0B, V(net->d0a, 32), V(net->d0b, 32), n, h0, w0, s));
     DN_TRY(conv(net, OUT, V(net->d0b, 32), View{nullptr, 32, 0, 32}, n, h0, w0, s, net->outF));
     return hipSuccess;
@@ -1020,7 +1070,7 @@ hipError_t denoiserInferDevice(DenoiserNet* net, const float* dInput, int n, int
                        net->demod);
     DN_TRY(hipGetLastError());
     DN_TRY(hipEventRecord(net->ev0, s));
-    DN_TRY(forward(net, s));
+    DN_TRY(forward(net, s, true));
     DN_TRY(hipEventRecord(net->ev1, s));
     hipLaunchKernelGGL(finishOutputKernel, dim3(blocks(npix)), dim3(kB), 0, s, net->outF, net->noisyLog, net->demod, npix,
                        dOutput);

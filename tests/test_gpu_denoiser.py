@@ -23,11 +23,12 @@ def conv_forward(lib, x, w, b, relu):
     return out
 
 
-# The last three shapes are large enough for the LDS-tiled kernel (32/64 output channels,
-# >= 16384 pixels per image), with partial 4 x 64 tiles and one or several input slices.
+# The last five shapes run on the LDS-tiled kernel (images at least 48 pixels wide), with
+# partial 8 x 64 tiles, one or several input slices and one or several output-channel groups.
 @pytest.mark.parametrize("cin,cout,h,w,relu", [(10, 32, 16, 16, 1), (32, 64, 5, 7, 0), (224, 96, 8, 12, 1),
                                                (32, 3, 9, 4, 0), (64, 128, 6, 6, 1), (160, 64, 4, 8, 0),
-                                               (10, 32, 130, 150, 1), (96, 64, 136, 128, 0), (32, 3, 129, 131, 0)])
+                                               (10, 32, 130, 150, 1), (96, 64, 136, 128, 0), (32, 3, 129, 131, 0),
+                                               (224, 96, 21, 70, 1), (96, 128, 13, 50, 0)])
 def test_conv_forward_vs_fp64(lib, cin, cout, h, w, relu):
     rng = np.random.default_rng(cin * 1000 + cout)
     x = odn.bf16_round(rng.normal(size=(2, h, w, cin)).astype(np.float32))
@@ -43,7 +44,7 @@ def test_conv_forward_vs_fp64(lib, cin, cout, h, w, relu):
 
 
 @pytest.mark.parametrize("cin,cout,h,w", [(32, 32, 16, 16), (224, 96, 8, 8), (64, 128, 12, 10), (32, 3, 16, 16),
-                                          (64, 32, 130, 132)])
+                                          (64, 32, 130, 132), (96, 128, 10, 52)])
 def test_conv_backward_vs_fp64(lib, cin, cout, h, w):
     rng = np.random.default_rng(cin + cout)
     n = 2
@@ -89,6 +90,20 @@ def test_inference_vs_fp64_oracle(lib):
     rng = np.random.default_rng(9)
     raw = np.abs(rng.normal(size=(1, 48, 32, 10))).astype(np.float32)
     with Denoiser(lib, seed=5) as net:
+        params = net.params
+        out = net.infer(raw)
+    ref = odn.infer(odn.bf16_round(params).astype(np.float64), raw)
+    assert np.abs(out - ref).max() <= 0.03 * np.abs(ref).max()
+
+
+def test_inference_with_upsampling_read_in_place_vs_fp64_oracle(lib):
+    """At 512 x 528 the decoder's first convolutions at levels 0-2 run on the tile kernel, which
+    reads their upsampled channels from the half-resolution tensor in place (partial 64-wide
+    tiles included); the result is the oracle's network."""
+    rng = np.random.default_rng(19)
+    raw = np.abs(rng.normal(size=(1, 512, 528, 10))).astype(np.float32)
+    raw[..., 6:9] = rng.normal(size=(1, 512, 528, 3))
+    with Denoiser(lib, seed=7) as net:
         params = net.params
         out = net.infer(raw)
     ref = odn.infer(odn.bf16_round(params).astype(np.float64), raw)
