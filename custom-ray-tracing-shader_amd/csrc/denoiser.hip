@@ -65,8 +65,8 @@ struct ConvArgs {
     int cinPad, coutPad;
     int relu, accumulate;
     // Optional: input channels [0, lowC) are the nearest-neighbour upsampling of `low`, a
-    // (h / 2) x (w / 2) buffer (read in place, so the upsampled tensor is never written);
-    // channels lowC.. come from `in` as usual.  Tile kernel only.
+    // (h / 2) x (w / 2) buffer (read in place, so the upsampled tensor is never written), and
+    // `in` holds channels lowC.. (in channel ci - lowC of the view).  Tile kernel only.
     View low;
     int lowC;
 };
@@ -259,7 +259,7 @@ __global__ __launch_bounds__(kTileThreads) RT_CONV_OCCUPANCY void convFwdTileKer
                 rin[it] = bufferLoad16(lowBuf, off);
             }
         } else {
-            const unsigned chan = (unsigned)(a.in.co + ci0 + 8 * part) * 2;
+            const unsigned chan = (unsigned)(a.in.co + ci0 - a.lowC + 8 * part) * 2;
 #pragma unroll
             for (int it = 0; it < kHaloIters; it++) {
                 const int yy = y0 + hRow[it] - 1, xx = x0 + hCol[it] - 1;
@@ -708,13 +708,13 @@ bool tileConv(const ConvArgs& a) {
 hipError_t launchConvFwd(const ConvArgs& a, hipStream_t s) {
     const int npix = a.n * a.h * a.w;
     const int tiles = a.coutPad / 16;
-    if (a.cinPad % 32 || a.coutPad % 32 || a.in.c != a.cinPad) return hipErrorInvalidValue;
+    if (a.cinPad % 32 || a.coutPad % 32 || a.in.c != a.cinPad - a.lowC) return hipErrorInvalidValue;
     if (a.low.p && !tileConv(a)) return hipErrorInvalidValue;   // only the tile kernel reads upsampled channels
     if (tileConv(a)) {
         const int tilesX = (a.w + kTileW - 1) / kTileW, tilesY = (a.h + kTileH - 1) / kTileH;
         const int tileCount = tilesX * tilesY * a.n;
         // persistent: as many blocks as the GPU holds at once (fewer for small images)
-        static int resident[2] = {0, 0}, cus = 0;
+        static int resident[3] = {0, 0, 0}, cus = 0;
         if (!resident[0]) {
             int dev = 0, perCu2 = 0, perCu4 = 0;
             hipError_t e = hipGetDevice(&dev);
@@ -724,6 +724,19 @@ hipError_t launchConvFwd(const ConvArgs& a, hipStream_t s) {
             if (e != hipSuccess) return e;
             resident[0] = std::max(1, cus * std::max(1, perCu2));
             resident[1] = std::max(1, cus * std::max(1, perCu4));
+        }
+        // Only an FP32 output of at most 16 channels (the network's last layer): the first
+        // 16-channel group suffices.
+        if (!a.out.p && a.outF && a.outFStride <= 16) {
+            if (!resident[2]) {
+                int perCu1 = 0;
+                const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu1, convFwdTileKernel<1>, kTileThreads, 0);
+                if (e != hipSuccess) return e;
+                resident[2] = std::max(1, cus * std::max(1, perCu1));
+            }
+            hipLaunchKernelGGL(convFwdTileKernel<1>, dim3(std::min(tileCount, resident[2]), 1), dim3(kTileThreads), 0, s, a,
+                               tilesX, tilesY, tileCount);
+            return hipGetLastError();
         }
         // 64-channel groups read each halo once for twice the channels; 32-channel groups make
         // twice the blocks, for small images that would leave compute units idle
@@ -841,49 +854,61 @@ hipError_t upsample(const View& in, const View& out, int n, int h, int w, int c,
 
 #define DN_TRY(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) return e_; } while (0)
 
-// The first decoder convolution of a level: upsample(up) joins the skip channels already in
-// the concatenation buffer (in); with `fuse` (inference) the convolution reads the
-// upsampled channels from `up` in place when the tile kernel runs it.
-hipError_t upConv(DenoiserNet* net, int l, const View& up, int upC, const View& in, const View& out, int n, int h, int w,
-                  bool fuse, hipStream_t s) {
-    ConvArgs a = convArgs(net, l, in, out, n, h, w);
-    a.low = up; a.lowC = upC;
-    if (fuse && tileConv(a)) return launchConvFwd(a, s);
-    DN_TRY(upsample(up, View{in.p, in.cs, in.co, upC}, n, h, w, upC, s));
-    a.low = View{nullptr, 0, 0, 0}; a.lowC = 0;
-    return launchConvFwd(a, s);
-}
-
-// Forward pass from net->x0 to net->outF.  fuseUp (inference): decoder convolutions read
-// their upsampled inputs in place; training keeps the concatenation buffers whole for the
-// weight gradients.
+// Forward pass from net->x0 to net->outF.
+//
+// Training (fuseUp false) keeps each level's concatenation buffer D = [upsampled | skip] whole,
+// since the weight gradients read it: the encoder writes its skip output into D, pooling reads
+// it there and the upsampling writes the other channels.  At inference (fuseUp), on levels whose
+// first decoder convolution runs on the tile kernel, nothing is concatenated: the encoder writes
+// its skip output to the buffer of the decoder's second convolution (free until that
+// convolution overwrites it), pooling reads it there, and the decoder convolution reads the
+// upsampled channels from the half-resolution tensor in place and the skip channels from that
+// buffer, each contiguous.
 hipError_t forward(DenoiserNet* net, hipStream_t s, bool fuseUp = false) {
-    const int n = net->n, h0 = net->h, w0 = net->w;
-    const int h1 = h0 / 2, w1 = w0 / 2, h2 = h0 / 4, w2 = w0 / 4, h3 = h0 / 8, w3 = w0 / 8, h4 = h0 / 16, w4 = w0 / 16;
-    DN_TRY(conv(net, E0A, V(net->x0, 32), V(net->e0a, 32), n, h0, w0, s));
-    DN_TRY(conv(net, E0B, V(net->e0a, 32), V(net->D0, 96, 64, 32), n, h0, w0, s));
-    DN_TRY(pool(V(net->D0, 96, 64, 32), V(net->p0, 32), n, h0, w0, 32, s));
-    DN_TRY(conv(net, E1A, V(net->p0, 32), V(net->e1a, 64), n, h1, w1, s));
-    DN_TRY(conv(net, E1B, V(net->e1a, 64), V(net->D1, 128, 64, 64), n, h1, w1, s));
-    DN_TRY(pool(V(net->D1, 128, 64, 64), V(net->p1, 64), n, h1, w1, 64, s));
-    DN_TRY(conv(net, E2A, V(net->p1, 64), V(net->e2a, 64), n, h2, w2, s));
-    DN_TRY(conv(net, E2B, V(net->e2a, 64), V(net->D2, 160, 96, 64), n, h2, w2, s));
-    DN_TRY(pool(V(net->D2, 160, 96, 64), V(net->p2, 64), n, h2, w2, 64, s));
-    DN_TRY(conv(net, E3A, V(net->p2, 64), V(net->e3a, 96), n, h3, w3, s));
-    DN_TRY(conv(net, E3B, V(net->e3a, 96), V(net->D3, 224, 128, 96), n, h3, w3, s));
-    DN_TRY(pool(V(net->D3, 224, 128, 96), V(net->p3, 96), n, h3, w3, 96, s));
-    DN_TRY(conv(net, BA, V(net->p3, 96), V(net->ba, 128), n, h4, w4, s));
-    DN_TRY(conv(net, BB, V(net->ba, 128), V(net->bb, 128), n, h4, w4, s));
-    DN_TRY(upsample(V(net->bb, 128), V(net->D3, 224, 0, 128), n, h3, w3, 128, s));
-    DN_TRY(conv(net, D3A, V(net->D3, 224), V(net->d3a, 96), n, h3, w3, s));
-    DN_TRY(conv(net, D3B, V(net->d3a, 96), V(net->d3b, 96), n, h3, w3, s));
-    DN_TRY(upConv(net, D2A, V(net->d3b, 96), 96, V(net->D2, 160), V(net->d2a, 64), n, h2, w2, fuseUp, s));
-    DN_TRY(conv(net, D2B, V(net->d2a, 64), V(net->d2b, 64), n, h2, w2, s));
-    DN_TRY(upConv(net, D1A, V(net->d2b, 64), 64, V(net->D1, 128), V(net->d1a, 64), n, h1, w1, fuseUp, s));
-    DN_TRY(conv(net, D1B, V(net->d1a, 64), V(net->d1b, 64), n, h1, w1, s));
-    DN_TRY(upConv(net, D0A, V(net->d1b, 64), 64, V(net->D0, 96), V(net->d0a, 32), n, h0, w0, fuseUp, s));
-    DN_TRY(conv(net, D0B, V(net->d0a, 32), V(net->d0b, 32), n, h0, w0, s));
-    DN_TRY(conv(net, OUT, V(net->d0b, 32), View{nullptr, 32, 0, 32}, n, h0, w0, s, net->outF));
+    const int n = net->n;
+    int H[5], W[5];
+    for (int l = 0; l < 5; l++) { H[l] = net->h >> l; W[l] = net->w >> l; }
+    // per level: concatenation buffer, its width, the upsampled channels, skip channels, the
+    // encoder / decoder buffers, decoder layers
+    struct Level { bf16* D; int dc; int upC; int skipC; bf16* enc; int encC; bf16* skipBuf; int ea, eb, da; };
+    const Level L[4] = {
+        {net->D0, 96, 64, 32, net->e0a, 32, net->d0b, E0A, E0B, D0A},
+        {net->D1, 128, 64, 64, net->e1a, 64, net->d1b, E1A, E1B, D1A},
+        {net->D2, 160, 96, 64, net->e2a, 64, net->d2b, E2A, E2B, D2A},
+        {net->D3, 224, 128, 96, net->e3a, 96, net->d3b, E3A, E3B, D3A}};
+    const View up[4] = {V(net->d1b, 64), V(net->d2b, 64), V(net->d3b, 96), V(net->bb, 128)};
+    const View dOut[4] = {V(net->d0a, 32), V(net->d1a, 64), V(net->d2a, 64), V(net->d3a, 96)};
+    bool fused[4];
+    View skip[4];
+    for (int l = 0; l < 4; l++) {
+        ConvArgs a = convArgs(net, L[l].da, V(L[l].skipBuf, L[l].skipC), dOut[l], n, H[l], W[l]);
+        a.low = up[l]; a.lowC = L[l].upC;
+        fused[l] = fuseUp && tileConv(a);
+        skip[l] = fused[l] ? V(L[l].skipBuf, L[l].skipC) : V(L[l].D, L[l].dc, L[l].upC, L[l].skipC);
+    }
+    const View pooled[4] = {V(net->p0, 32), V(net->p1, 64), V(net->p2, 64), V(net->p3, 96)};
+    View x = V(net->x0, 32);
+    for (int l = 0; l < 4; l++) {   // encoder
+        DN_TRY(conv(net, L[l].ea, x, V(L[l].enc, L[l].encC), n, H[l], W[l], s));
+        DN_TRY(conv(net, L[l].eb, V(L[l].enc, L[l].encC), skip[l], n, H[l], W[l], s));
+        DN_TRY(pool(skip[l], pooled[l], n, H[l], W[l], L[l].skipC, s));
+        x = pooled[l];
+    }
+    DN_TRY(conv(net, BA, V(net->p3, 96), V(net->ba, 128), n, H[4], W[4], s));
+    DN_TRY(conv(net, BB, V(net->ba, 128), V(net->bb, 128), n, H[4], W[4], s));
+    const int db[4] = {D0B, D1B, D2B, D3B};
+    for (int l = 3; l >= 0; l--) {   // decoder
+        if (fused[l]) {
+            ConvArgs a = convArgs(net, L[l].da, skip[l], dOut[l], n, H[l], W[l]);
+            a.low = up[l]; a.lowC = L[l].upC;
+            DN_TRY(launchConvFwd(a, s));
+        } else {
+            DN_TRY(upsample(up[l], V(L[l].D, L[l].dc, 0, L[l].upC), n, H[l], W[l], L[l].upC, s));
+            DN_TRY(conv(net, L[l].da, V(L[l].D, L[l].dc), dOut[l], n, H[l], W[l], s));
+        }
+        DN_TRY(conv(net, db[l], dOut[l], V(L[l].skipBuf, L[l].skipC), n, H[l], W[l], s));
+    }
+    DN_TRY(conv(net, OUT, V(net->d0b, 32), View{nullptr, 32, 0, 32}, n, H[0], W[0], s, net->outF));
     return hipSuccess;
 }
 
