@@ -140,7 +140,7 @@ __global__ __launch_bounds__(kB) void convFwdKernel(ConvArgs a) {
             }
 }
 
-// The same convolution for images at least 48 pixels wide (or 16384 pixels), as a
+// The same convolution for images at least 16 pixels wide (or 16384 pixels), as a
 // persistent kernel: a block computes kTileH-row x 64-column output tiles of one group of
 // 16 NT output channels (blockIdx.y; one row per wave), looping over tiles and 32-channel
 // input slices ("steps").
@@ -409,35 +409,7 @@ __global__ __launch_bounds__(kB) void convBwdWKernel(BwdWArgs a) {
             }
 }
 
-// dBias[c] += sum over a part of the pixels of dy[p][c] (channels <= kB); the block's lanes
-// are combined in LDS so each block adds once per channel.
-__global__ __launch_bounds__(kB) void biasGradKernel(View dy, int npix, int channels, int pixPerPart, float* db) {
-    __shared__ float red[kB];
-    const int c = threadIdx.x % channels;
-    const int lanesPer = kB / channels;
-    const int k = threadIdx.x / channels;
-    const int begin = blockIdx.x * pixPerPart, end = min(npix, begin + pixPerPart);
-    float s = 0;
-    if (k < lanesPer)
-        for (int p = begin + k; p < end; p += lanesPer) s += (float)dy.p[(size_t)p * dy.cs + dy.co + c];
-    red[threadIdx.x] = s;
-    __syncthreads();
-    if (threadIdx.x < channels) {
-        float t = 0;
-        for (int j = 0; j < lanesPer; j++) t += red[j * channels + threadIdx.x];
-        atomicAdd(&db[threadIdx.x], t);
-    }
-}
-
 // ------------------------------------------------------------------ elementwise kernels
-__global__ void reluMaskKernel(View g, View act, bf16* out, int npix, int channels) {
-    const long long i = blockIdx.x * (long long)kB + threadIdx.x;
-    if (i >= (long long)npix * channels) return;
-    const int p = (int)(i / channels), c = (int)(i % channels);
-    const float a = (float)act.p[(size_t)p * act.cs + act.co + c];
-    out[i] = a > 0.0f ? g.p[(size_t)p * g.cs + g.co + c] : (bf16)0.0f;
-}
-
 // Pooling and upsampling move 8 channels (16 bytes) per thread: every view's channel
 // stride, offset and width are multiples of 32, so each 8-channel group is 16-byte aligned.
 __device__ __forceinline__ bf16x8 load8(const View& v, size_t pixel, int c) {
@@ -445,6 +417,47 @@ __device__ __forceinline__ bf16x8 load8(const View& v, size_t pixel, int c) {
 }
 __device__ __forceinline__ void store8(const View& v, size_t pixel, int c, bf16x8 x) {
     *reinterpret_cast<bf16x8*>(v.p + pixel * v.cs + v.co + c) = x;
+}
+
+// The ReLU's backward: the gradient where the layer's output is positive, else zero, into a
+// dense [pixel][channels] buffer; 8 channels per thread.
+__global__ void reluMaskKernel(View g, View act, bf16* out, int npix, int channels) {
+    const int groups = channels / 8;
+    const long long i = blockIdx.x * (long long)kB + threadIdx.x;
+    if (i >= (long long)npix * groups) return;
+    const size_t p = (size_t)(i / groups);
+    const int c = 8 * (int)(i % groups);
+    const bf16x8 a = load8(act, p, c), v = load8(g, p, c);
+    bf16x8 o;
+#pragma unroll
+    for (int e = 0; e < 8; e++) o[e] = (float)a[e] > 0.0f ? v[e] : (bf16)0.0f;
+    *reinterpret_cast<bf16x8*>(out + p * channels + c) = o;
+}
+
+// dBias[c] += sum over a part of the pixels of dy[p][c] (channels a multiple of 8, at most
+// 8 kB): each thread sums 8 channels over a strided share of the part's pixels, the block's
+// threads are combined in LDS, and each block adds once per channel.
+__global__ __launch_bounds__(kB) void biasGradKernel(View dy, int npix, int channels, int pixPerPart, float* db) {
+    __shared__ float red[kB * 8];
+    const int groups = channels / 8, lanesPer = kB / groups;
+    const int cg = threadIdx.x % groups, k = threadIdx.x / groups;
+    const int begin = blockIdx.x * pixPerPart, end = min(npix, begin + pixPerPart);
+    float sum[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (k < lanesPer)
+        for (int p = begin + k; p < end; p += lanesPer) {
+            const bf16x8 v = load8(dy, (size_t)p, 8 * cg);
+#pragma unroll
+            for (int e = 0; e < 8; e++) sum[e] += (float)v[e];
+        }
+#pragma unroll
+    for (int e = 0; e < 8; e++) red[threadIdx.x * 8 + e] = sum[e];
+    __syncthreads();
+    if ((int)threadIdx.x < channels) {
+        const int c = threadIdx.x;
+        float t = 0;
+        for (int j = 0; j < lanesPer; j++) t += red[(j * groups + c / 8) * 8 + c % 8];
+        atomicAdd(&db[c], t);
+    }
 }
 
 __global__ void avgPoolKernel(View in, View out, int n, int h, int w, int channels) {
@@ -698,7 +711,7 @@ bf16* wbOf(DenoiserNet* net, int l) {
 
 // Whether launchConvFwd runs a convolution on the tile kernel.
 bool tileConv(const ConvArgs& a) {
-    return a.coutPad % 32 == 0 && (a.w >= 48 || (long long)a.w * a.h >= 16384) &&
+    return a.coutPad % 32 == 0 && (a.w >= 16 || (long long)a.w * a.h >= 16384) &&
            (long long)a.n * a.h * a.w * a.in.cs * 2 < (long long)kOutside &&
            (long long)9 * a.coutPad * a.cinPad * 2 < kOutside && (a.in.cs | a.in.co) % 8 == 0 &&
            (!a.out.p || (a.out.cs | a.out.co) % 4 == 0) &&
@@ -920,7 +933,7 @@ hipError_t convBack(DenoiserNet* net, int l, const View& g, const View& act, con
     const int npix = n * h * w;
     View dy = g;
     if (L.relu) {
-        hipLaunchKernelGGL(reluMaskKernel, dim3(blocks((long long)npix * L.coutPad)), dim3(kB), 0, s, g, act, net->dpre,
+        hipLaunchKernelGGL(reluMaskKernel, dim3(blocks((long long)npix * L.coutPad / 8)), dim3(kB), 0, s, g, act, net->dpre,
                            npix, L.coutPad);
         DN_TRY(hipGetLastError());
         dy = V(net->dpre, L.coutPad);

@@ -1,7 +1,8 @@
 """Train the learned denoiser on images this path tracer renders from seeded scenes.
 
-Dataset: for each seeded scene (displaced icosphere in a lit room, triangle soup under sky
-light, or a Cornell-box variant), a 4-spp render supplies the network input planes
+Dataset: for each seeded scene (displaced icosphere in a lit room or triangle soup under sky
+light, the C3 and C2 generators; optionally Cornell-box variants), a 4-spp render supplies the
+network input planes
 (RtReadDenoiserInputs: radiance, albedo, normal, depth) and an independently seeded
 high-sample render supplies the target radiance.  With --fine, part of the scenes are fine
 views instead: full-tessellation icosphere rooms seen through the field of view that a
@@ -13,6 +14,10 @@ written with a versioned header to denoiser_weights.bin next to the library, whe
 finds them.
 
     python train_denoiser.py --scenes 96 --target-spp 1024 --steps 3000 --out denoiser_weights.bin
+
+A long run can be split into shards that each render their own scenes (--first-seed) and
+continue from the previous shard's weights (--init-weights), with --lr-from / --lr-to placing
+each shard on one cosine learning-rate schedule over the whole run.
 """
 import argparse
 import json
@@ -59,16 +64,21 @@ def render_mean(session, camera, settings, spp, seed):
     settings.seed = seed
     settings.raysPerPixel = rays_per_call(spp)
     for _ in range(spp // settings.raysPerPixel):
-        session.render(camera, settings)
+        session.render_resident(camera, settings)
     assert lib.RtSampleCount(session.handle) == spp
     acc = session.accumulation()
     return acc[..., 0:3] / spp
 
 
-def random_scene(seed):
-    """(triangles, camera factory, settings) of one seeded training scene."""
+KINDS = ("sphere", "soup", "cornell")
+
+
+def random_scene(seed, kinds=("sphere", "soup")):
+    """(triangles, camera factory, settings) of one seeded training scene: the kind cycles
+    with the seed over `kinds` (sphere: displaced icosphere room, soup: triangle soup,
+    cornell: Cornell-box variant)."""
     rng = np.random.default_rng(seed)
-    kind = seed % 3
+    kind = KINDS.index(kinds[seed % len(kinds)])
     if kind == 0:
         tris = scenes.displaced_icosphere(subdiv=int(rng.integers(5, 9)), seed=int(seed), amplitude=float(rng.uniform(0.02, 0.1)),
                                           metal_probability=float(rng.uniform(0, 0.5)))
@@ -129,11 +139,12 @@ def fine_view(seed, size, full_height=1080):
 
 
 def heldout_scene(seed):
-    tris, cam, st = random_scene(seed)
+    """A 256 x 256 view of a seeded scene of any of the three kinds (for tests)."""
+    tris, cam, st = random_scene(seed, KINDS)
     return cam(256, 256), st, tris
 
 
-def build_dataset(lib, count, size, input_spp, target_spp, first_seed, log, fine=0.0, pool=()):
+def build_dataset(lib, count, size, input_spp, target_spp, first_seed, log, fine=0.0, pool=(), kinds=("sphere", "soup")):
     """input_spp: one sample count, or a sequence cycled over the scenes (mixed noise levels).
     A fraction `fine` of the scenes (chosen by seed) are fine views of the meshes in pool."""
     spps = tuple(input_spp) if isinstance(input_spp, (tuple, list)) else (input_spp,)
@@ -152,7 +163,7 @@ def build_dataset(lib, count, size, input_spp, target_spp, first_seed, log, fine
                 s, own = sessions[k], False
                 cam, st = fine_view(seed, size)
             else:
-                tris, camf, st = random_scene(seed)
+                tris, camf, st = random_scene(seed, kinds)
                 cam = camf(size, size)
                 s, own = Session(lib), True
                 s.initialize(tris, size, size)
@@ -200,6 +211,13 @@ def main():
                          "through the field of view of a crop of a 1080-line frame")
     ap.add_argument("--fine-meshes", type=int, default=4)
     ap.add_argument("--fine-subdiv", type=int, default=9)
+    ap.add_argument("--kinds", default="sphere,soup",
+                    help="scene generators cycled over the seeds: sphere (C3), soup (C2), cornell")
+    ap.add_argument("--first-seed", type=int, default=100, help="seed of the first training scene")
+    ap.add_argument("--init-weights", default=None, help="continue from these weights instead of a fresh network")
+    ap.add_argument("--lr-from", type=float, default=0.0,
+                    help="where this run starts on the cosine learning-rate schedule (0 = lr, 1 = 0)")
+    ap.add_argument("--lr-to", type=float, default=1.0, help="where this run ends on that schedule")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--out", default=str(HERE / "denoiser_weights.bin"))
     ap.add_argument("--report", default=None, help="write a JSON summary here")
@@ -208,13 +226,17 @@ def main():
     lib = load_library()
     t0 = time.perf_counter()
     mix = tuple(int(x) for x in args.input_spp_mix.split(",")) if args.input_spp_mix else args.input_spp
+    kinds = tuple(k for k in args.kinds.split(",") if k)
+    if not kinds or any(k not in KINDS for k in kinds):
+        raise SystemExit(f"--kinds: choose from {KINDS}")
     pool = fine_pool(args.fine_meshes, args.fine_subdiv, log) if args.fine > 0 else ()
-    inputs, targets = build_dataset(lib, args.scenes, args.size, mix, args.target_spp, 100, log, args.fine, pool)
+    inputs, targets = build_dataset(lib, args.scenes, args.size, mix, args.target_spp, args.first_seed, log, args.fine,
+                                    pool, kinds)
     hin, htg = build_dataset(lib, args.heldout, args.size, args.input_spp, args.target_spp, 100000, log, args.fine,
-                             pool)
+                             pool, kinds)
     del pool
     data_s = time.perf_counter() - t0
-    net = Denoiser(lib, seed=args.seed)
+    net = Denoiser(lib, weights=args.init_weights) if args.init_weights else Denoiser(lib, seed=args.seed)
     net.set_display_loss(args.display_weight)
     net.set_dataset(inputs, targets)
     before = evaluate(net, hin, htg)
@@ -223,7 +245,8 @@ def main():
     t0 = time.perf_counter()
     chunks = 10
     for k in range(chunks):
-        lr = args.lr * 0.5 * (1 + math.cos(math.pi * k / chunks))   # cosine decay in chunks
+        at = args.lr_from + (args.lr_to - args.lr_from) * k / chunks   # cosine decay in chunks
+        lr = args.lr * 0.5 * (1 + math.cos(math.pi * at))
         losses.append(net.train(args.steps // chunks, args.batch, args.crop, lr, args.seed + k))
         log(f"steps {(k + 1) * (args.steps // chunks)}: lr {lr:.2e}, loss {losses[-1][-50:].mean():.4f}")
     train_s = time.perf_counter() - t0
@@ -232,7 +255,9 @@ def main():
     log(f"held-out PSNR after training: plain {after[0]:.2f} dB, network {after[1]:.2f} dB")
     net.save(args.out)
     flops = 3 * lib.RtDenoiserForwardFlops(args.batch, args.crop, args.crop)
-    report = {"scenes": args.scenes, "heldout": args.heldout, "size": args.size, "input_spp": args.input_spp,
+    report = {"scenes": args.scenes, "first_seed": args.first_seed, "kinds": args.kinds,
+              "init_weights": args.init_weights, "lr_from": args.lr_from, "lr_to": args.lr_to,
+              "heldout": args.heldout, "size": args.size, "input_spp": args.input_spp,
               "input_spp_mix": args.input_spp_mix, "fine": args.fine, "fine_meshes": args.fine_meshes,
               "fine_subdiv": args.fine_subdiv,
               "target_spp": args.target_spp, "steps": args.steps, "batch": args.batch, "crop": args.crop,

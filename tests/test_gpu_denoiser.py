@@ -23,8 +23,9 @@ def conv_forward(lib, x, w, b, relu):
     return out
 
 
-# The last five shapes run on the LDS-tiled kernel (images at least 48 pixels wide), with
-# partial 8 x 64 tiles, one or several input slices and one or several output-channel groups.
+# Images at least 16 pixels wide (the first shape and the last five) run on the LDS-tiled
+# kernel, with partial 8 x 64 tiles, one or several input slices and one or several
+# output-channel groups; the others on the small-image kernel.
 @pytest.mark.parametrize("cin,cout,h,w,relu", [(10, 32, 16, 16, 1), (32, 64, 5, 7, 0), (224, 96, 8, 12, 1),
                                                (32, 3, 9, 4, 0), (64, 128, 6, 6, 1), (160, 64, 4, 8, 0),
                                                (10, 32, 130, 150, 1), (96, 64, 136, 128, 0), (32, 3, 129, 131, 0),
