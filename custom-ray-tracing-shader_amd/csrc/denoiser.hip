@@ -360,32 +360,50 @@ __global__ __launch_bounds__(kB) void convBwdWKernel(BwdWArgs a) {
     for (int i = 0; i < TCO; i++)
 #pragma unroll
         for (int j = 0; j < TCI; j++) acc[i][j] = f32x4{0, 0, 0, 0};
-    for (int s = 0; s < steps; s++) {
+    // A wave's 32 pixels per step: this lane's 16-byte chunks of dy (TCO of them) and of the
+    // shifted input (TCI), loaded one step ahead into registers.
+    auto fetch = [&](int s, bf16x8 (&va)[TCO], bf16x8 (&vb)[TCI]) {
         const int p0 = begin + (s * 4 + wave) * 32;
-        for (int idx = lane; idx < 32 * 2 * TCO; idx += 64) {
-            const int pxl = idx / (2 * TCO), cg = idx % (2 * TCO);
-            const int p = p0 + pxl;
-            bf16x8 v = zero8();
-            if (p < end) v = *reinterpret_cast<const bf16x8*>(a.dy.p + (size_t)p * a.dy.cs + a.dy.co + cob + 8 * cg);
 #pragma unroll
-            for (int e = 0; e < 8; e++) sA[wave][(8 * cg + e) * 32 + pxl] = v[e];
+        for (int k = 0; k < TCO; k++) {
+            const int idx = lane + 64 * k, pxl = idx / (2 * TCO), cg = idx % (2 * TCO), p = p0 + pxl;
+            va[k] = p < end ? *reinterpret_cast<const bf16x8*>(a.dy.p + (size_t)p * a.dy.cs + a.dy.co + cob + 8 * cg)
+                            : zero8();
         }
-        for (int idx = lane; idx < 32 * 2 * TCI; idx += 64) {
-            const int pxl = idx / (2 * TCI), cg = idx % (2 * TCI);
-            const int p = p0 + pxl;
+#pragma unroll
+        for (int k = 0; k < TCI; k++) {
+            const int idx = lane + 64 * k, pxl = idx / (2 * TCI), cg = idx % (2 * TCI), p = p0 + pxl;
             bf16x8 v = zero8();
             if (p < end) {
                 const int x = p % a.w, q = p / a.w, y = q % a.h, nn = q / a.h;
                 const int yy = y + dy, xx = x + dx;
-                if (yy >= 0 && yy < a.h && xx >= 0 && xx < a.w) {
-                    const int qq = (nn * a.h + yy) * a.w + xx;
-                    v = *reinterpret_cast<const bf16x8*>(a.in.p + (size_t)qq * a.in.cs + a.in.co + cib + 8 * cg);
-                }
+                if (yy >= 0 && yy < a.h && xx >= 0 && xx < a.w)
+                    v = *reinterpret_cast<const bf16x8*>(a.in.p + (size_t)((nn * a.h + yy) * a.w + xx) * a.in.cs + a.in.co +
+                                                         cib + 8 * cg);
             }
-#pragma unroll
-            for (int e = 0; e < 8; e++) sB[wave][(8 * cg + e) * 32 + pxl] = v[e];
+            vb[k] = v;
         }
-        __syncthreads();
+    };
+    bf16x8 va[TCO], vb[TCI];
+    fetch(0, va, vb);
+    // Each wave stages and reads only its own LDS region, so the waves synchronise with
+    // themselves (ordering their LDS writes and reads), not with the workgroup.
+    for (int s = 0; s < steps; s++) {
+#pragma unroll
+        for (int k = 0; k < TCO; k++) {
+            const int idx = lane + 64 * k, pxl = idx / (2 * TCO), cg = idx % (2 * TCO);
+#pragma unroll
+            for (int e = 0; e < 8; e++) sA[wave][(8 * cg + e) * 32 + pxl] = va[k][e];
+        }
+#pragma unroll
+        for (int k = 0; k < TCI; k++) {
+            const int idx = lane + 64 * k, pxl = idx / (2 * TCI), cg = idx % (2 * TCI);
+#pragma unroll
+            for (int e = 0; e < 8; e++) sB[wave][(8 * cg + e) * 32 + pxl] = vb[k][e];
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        if (s + 1 < steps) fetch(s + 1, va, vb);
 #pragma unroll
         for (int i = 0; i < TCO; i++) {
             const bf16x8 av = *reinterpret_cast<const bf16x8*>(&sA[wave][(16 * i + r) * 32 + 8 * g]);
@@ -395,7 +413,8 @@ __global__ __launch_bounds__(kB) void convBwdWKernel(BwdWArgs a) {
                 acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc[i][j], 0, 0, 0);
             }
         }
-        __syncthreads();
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        __builtin_amdgcn_wave_barrier();
     }
     float* part = a.part + (size_t)tap * a.coutPad * a.cinPad;   // the layer's gradient, zeroed before backward
 #pragma unroll
