@@ -200,6 +200,32 @@ def denoise_timing(args):
     return 0
 
 
+def denoise_train_timing(args):
+    """--denoise-train-timing STEPS: RtDenoiserTrain steps (forward, backward, Adam) on random
+    data of --train-batch crops of --train-crop pixels, after 20 warm-up steps: the workload
+    for profiling the training kernels.  MFMA utilisation counts 3 x the forward FLOP."""
+    import native_api
+    lib = native_api.load_library()
+    b, c = args.train_batch, args.train_crop
+    rng = np.random.default_rng(0)
+    n = max(b, 8)
+    inputs = rng.uniform(0, 1, (n, c, c, 10)).astype(np.float32)
+    targets = rng.uniform(0, 1, (n, c, c, 3)).astype(np.float32)
+    with native_api.Denoiser(lib, seed=1) as net:
+        net.set_display_loss(4.0)
+        net.set_dataset(inputs, targets)
+        net.train(20, b, c, 1e-4, 1)
+        t0 = time.perf_counter()
+        net.train(args.denoise_train_timing, b, c, 1e-4, 2)
+        wall = (time.perf_counter() - t0) / args.denoise_train_timing * 1e3
+        step_ms = net.timings()[1]
+    flops = 3 * lib.RtDenoiserForwardFlops(b, c, c)
+    print(json.dumps({"batch": b, "crop": c, "steps": args.denoise_train_timing, "step_ms": round(step_ms, 3),
+                      "wall_ms_per_step": round(wall, 3), "train_flop": flops,
+                      "mfma_utilization": round(flops / (step_ms / 1e3) / 2.5e15, 4)}), flush=True)
+    return 0
+
+
 def run_psnr_study(s, cam, st, w, h, reference_spp, denoised_ladder=(8, 12, 16, 20, 24, 32, 64), repeats=3):
     """Time to equal PSNR on an initialized session: learned denoise vs plain accumulation,
     against a reference_spp render (seed 20261023), 8-bit display PSNR as
@@ -393,6 +419,10 @@ def main():
     ap.add_argument("--reference-spp", type=int, default=4096)
     ap.add_argument("--denoise-timing", type=int, default=0, metavar="CALLS",
                     help="only time RtDenoise CALLS times on a 4-spp icosphere frame (for profiling) and exit")
+    ap.add_argument("--denoise-train-timing", type=int, default=0, metavar="STEPS",
+                    help="only time STEPS denoiser training steps on random data (for profiling) and exit")
+    ap.add_argument("--train-batch", type=int, default=8, help="crops per training step (--denoise-train-timing)")
+    ap.add_argument("--train-crop", type=int, default=256, help="crop size of --denoise-train-timing")
     ap.add_argument("--denoise-ensemble", type=int, choices=(1, 2, 4),
                     help="network passes per denoise (mirrored inputs averaged); default: the library's")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
@@ -415,6 +445,8 @@ def main():
         return psnr_study(args)
     if args.denoise_timing:
         return denoise_timing(args)
+    if args.denoise_train_timing:
+        return denoise_train_timing(args)
     if args.build_timing:
         return build_timing(args)
     if args.workload == "batch":

@@ -428,6 +428,253 @@ __global__ __launch_bounds__(kB) void convBwdWKernel(BwdWArgs a) {
             }
 }
 
+// Weight and bias gradients as one persistent kernel over tiles of pixels, all nine taps per
+// block (the per-tap kernel above stages every operand nine times).
+//
+// A block of 4 waves handles one pair of 32-channel groups (32 output x 32 input channels;
+// blockIdx.y) and loops over tiles of TH x TW dy pixels (8 x 64 for S = 2 strips of 32
+// columns, 16 x 32 for S = 1), accumulating in registers.  Each tile's dy pixels and the input
+// pixels under them with a one-pixel halo are staged in LDS in NHWC order (64 bytes per pixel,
+// chunk order swizzled by wgByte).  The MFMA operands need 8 consecutive pixels of one channel
+// per lane; ds_read_b64_tr_b16 gathers them from that pixel-major layout (lane i of a 16-lane
+// group receives element i % 4 of the 8-byte words that lanes 4e + i / 4 name, e = 0..3), so
+// a tap's pixel shift is only an address offset.
+// A wave owns a 32-column strip of 4 dy rows: it walks the 6 input rows under them, loads each
+// input row once per column shift and uses it for the (up to) three dy rows that reach it,
+// so one dy row's operands serve all nine taps.  The bias gradient is one more MFMA per dy row
+// against ones.  At the end the 4 waves' sums are added through LDS and the block writes them
+// to its slot of `part`; wgReduceKernel sums the slots in runs and adds each run's sum to the
+// gradient with one atomic.
+constexpr int kWgThreads = 256, kWgSlot = 9 * 32 * 32 + 32;   // floats per block slot: weights + bias
+
+struct BwdWTileArgs {
+    View dy, in;
+    float* part;   // [pairs][gridDim.x][kWgSlot]
+    int n, h, w;
+    int nci;       // input-channel groups of 32
+    int tilesX, tilesY, tileCount;
+};
+
+typedef short s16x4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(3))) s16x4 LdsS16x4;
+
+// Position of 16-byte chunk c of pixel p in its LDS tile: pixels 2 and 3 of every 4 swap the
+// 32-byte halves of their 64 bytes, so four consecutive pixels put a 16-channel range in four
+// different 32-byte bank groups of a 128-byte window.  Tile rows are padded to a multiple of 4
+// pixels, so p & 3 is the column & 3.
+__device__ __forceinline__ int wgByte(int p, int chunk) { return p * 64 + ((chunk ^ (p & 2)) << 4); }
+
+typedef __attribute__((address_space(3))) char LdsChar;
+// A transposing read of 4 pixels x 16 channels: lane jj of a 16-lane group names pixel
+// col0 + shift + (jj >> 2) and channels n0 + 4 (jj & 3) .. + 3, and gets channel n0 + jj of
+// the four pixels.  Returns the byte offset of the lane's 8 bytes from the start of the tile
+// row (col0 a multiple of 4; the next four pixels are 256 bytes further).
+__device__ __forceinline__ int wgLaneOffset(int col0, int shift, int n0, int jj) {
+    const int col = col0 + shift + (jj >> 2);
+    return wgByte(col, (n0 >> 3) + ((jj >> 1) & 1)) + ((jj & 1) << 3);
+}
+__device__ __forceinline__ s16x4 wgTr(const char* p) {
+    return __builtin_amdgcn_ds_read_tr16_b64_v4i16((LdsS16x4*)(LdsChar*)p);
+}
+__device__ __forceinline__ bf16x8 wgPack(s16x4 lo, s16x4 hi) {
+    typedef short s16x8 __attribute__((ext_vector_type(8)));
+    return __builtin_bit_cast(bf16x8, (s16x8)__builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7));
+}
+
+// WG_IN_PARTS splits the input tile's loads into that many batches (fewer registers in
+// flight while staging).
+#ifndef WG_IN_PARTS
+#define WG_IN_PARTS 1
+#endif
+template <int S>
+__global__ __launch_bounds__(kWgThreads, 2) void convBwdWTileKernel(BwdWTileArgs a) {
+    constexpr int RG = 4 / S, TH = 4 * RG, TW = 32 * S, HH = TH + 2, HW = TW + 2, HWP = (HW + 3) / 4 * 4;
+    constexpr int kDyBytes = TH * TW * 64, kInPix = HH * HWP;
+    constexpr int kDyIters = TH * TW * 4 / kWgThreads, kInIters = (kInPix * 4 + kWgThreads - 1) / kWgThreads;
+    constexpr int kInParts = WG_IN_PARTS;   // input-tile loads in flight at once: kInIters / kInParts
+    static_assert(kDyBytes + kInPix * 64 >= kWgSlot * 4, "the block's sums are added in the tile buffer");
+    __shared__ __attribute__((aligned(16))) char lds[kDyBytes + kInPix * 64];
+    char* const sDy = lds;
+    char* const sIn = lds + kDyBytes;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int g = lane >> 4, jj = lane & 15;
+    const int strip = wave % S, rg = wave / S;
+    const int pair = blockIdx.y, cob = (pair / a.nci) * 32, cib = (pair % a.nci) * 32;
+    const bool bias = pair % a.nci == 0;
+    const __amdgpu_buffer_rsrc_t dyBuf = bufferOf(a.dy.p, (unsigned)((size_t)a.n * a.h * a.w * a.dy.cs * 2));
+    const __amdgpu_buffer_rsrc_t inBuf = bufferOf(a.in.p, (unsigned)((size_t)a.n * a.h * a.w * a.in.cs * 2));
+    // This lane's operand offsets within a tile row: dy (channel group i, pixels +0 / +4) and
+    // input (column shift dx, channel group j, pixels +0 / +4).
+    const int col0 = 32 * strip + 8 * g;
+    const char* baseA = sDy + 4 * rg * (TW * 64);
+    const char* baseB = sIn + 4 * rg * (HWP * 64);
+    const char* opA[2];
+    const char* opB[3][2];
+#pragma unroll
+    for (int i = 0; i < 2; i++) opA[i] = baseA + wgLaneOffset(col0, 0, 16 * i, jj);
+#pragma unroll
+    for (int dx = 0; dx < 3; dx++)
+#pragma unroll
+        for (int j = 0; j < 2; j++) opB[dx][j] = baseB + wgLaneOffset(col0, dx, 16 * j, jj);
+    f32x4 acc[9][2][2], accB[2];
+#pragma unroll
+    for (int t = 0; t < 9; t++)
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+            for (int j = 0; j < 2; j++) acc[t][i][j] = f32x4{0, 0, 0, 0};
+    accB[0] = accB[1] = f32x4{0, 0, 0, 0};
+    bf16x8 ones;
+#pragma unroll
+    for (int e = 0; e < 8; e++) ones[e] = (bf16)1.0f;
+    for (int t = blockIdx.x; t < a.tileCount; t += gridDim.x) {
+        const int x0 = (t % a.tilesX) * TW, y0 = ((t / a.tilesX) % a.tilesY) * TH, nn = t / (a.tilesX * a.tilesY);
+        __syncthreads();   // the previous tile's reads are done
+        {
+            bf16x8 v[kDyIters];
+#pragma unroll
+            for (int k = 0; k < kDyIters; k++) {
+                const int q = threadIdx.x + k * kWgThreads, p = q >> 2, c = q & 3;
+                const int y = y0 + p / TW, x = x0 + p % TW;
+                const bool in = y < a.h && x < a.w;
+                v[k] = bufferLoad16(dyBuf, in ? (unsigned)(((nn * a.h + y) * a.w + x) * a.dy.cs + a.dy.co + cob + 8 * c) * 2
+                                              : kOutside);
+            }
+#pragma unroll
+            for (int k = 0; k < kDyIters; k++) {
+                const int q = threadIdx.x + k * kWgThreads;
+                *reinterpret_cast<bf16x8*>(sDy + wgByte(q >> 2, q & 3)) = v[k];
+            }
+        }
+#pragma unroll
+        for (int half = 0; half < kInParts; half++) {
+            constexpr int kHalf = (kInIters + kInParts - 1) / kInParts;
+            bf16x8 v[kHalf];
+#pragma unroll
+            for (int k = 0; k < kHalf; k++) {
+                const int q = threadIdx.x + (half * kHalf + k) * kWgThreads, p = q >> 2, c = q & 3;
+                const int row = p / HWP, col = p % HWP, y = y0 - 1 + row, x = x0 - 1 + col;
+                const bool in = q < kInPix * 4 && col < HW && (unsigned)y < (unsigned)a.h && (unsigned)x < (unsigned)a.w;
+                v[k] = bufferLoad16(inBuf, in ? (unsigned)(((nn * a.h + y) * a.w + x) * a.in.cs + a.in.co + cib + 8 * c) * 2
+                                              : kOutside);
+            }
+#pragma unroll
+            for (int k = 0; k < kHalf; k++) {
+                const int q = threadIdx.x + (half * kHalf + k) * kWgThreads;
+                if (q < kInPix * 4) *reinterpret_cast<bf16x8*>(sIn + wgByte(q >> 2, q & 3)) = v[k];
+            }
+        }
+        __syncthreads();
+        // dy row 4 rg + delta of the tile meets its tap row ddy (0: the row above) in input
+        // (halo) row 4 rg + delta + ddy = 4 rg + rho.
+        bf16x8 A[4][2];
+#pragma unroll
+        for (int rho = 0; rho < 6; rho++) {
+            if (rho < 4) {
+#pragma unroll
+                for (int i = 0; i < 2; i++)
+                    A[rho][i] = wgPack(wgTr(opA[i] + rho * (TW * 64)), wgTr(opA[i] + rho * (TW * 64) + 256));
+                if (bias) {
+#pragma unroll
+                    for (int i = 0; i < 2; i++) accB[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[rho][i], ones, accB[i], 0, 0, 0);
+                }
+            }
+#pragma unroll
+            for (int dx = 0; dx < 3; dx++) {
+                bf16x8 B[2];
+#pragma unroll
+                for (int j = 0; j < 2; j++)
+                    B[j] = wgPack(wgTr(opB[dx][j] + rho * (HWP * 64)), wgTr(opB[dx][j] + rho * (HWP * 64) + 256));
+#pragma unroll
+                for (int ddy = 0; ddy < 3; ddy++) {
+                    const int delta = rho - ddy;
+                    if (delta < 0 || delta > 3) continue;
+#pragma unroll
+                    for (int i = 0; i < 2; i++)
+#pragma unroll
+                        for (int j = 0; j < 2; j++)
+                            acc[ddy * 3 + dx][i][j] =
+                                __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[delta][i], B[j], acc[ddy * 3 + dx][i][j], 0, 0, 0);
+                }
+            }
+        }
+    }
+    // Sum the four waves' partial gradients: waves 2 and 3 park theirs in LDS for waves 0 and 1,
+    // then wave 1 parks the pair's sum for wave 0, which writes the block's slot.  (Lane-major
+    // parking keeps the LDS stores and loads free of bank conflicts.)
+    __syncthreads();
+    float* park = reinterpret_cast<float*>(lds);          // [2 regions][144 values][64 lanes]
+    float* biasPark = park + 2 * 144 * 64;                // [4 waves][32 channels]
+    static_assert((2 * 144 * 64 + 4 * 32) * 4 <= kDyBytes + kInPix * 64, "parking fits in the tile buffer");
+    auto parkAcc = [&](int region) {
+#pragma unroll
+        for (int t = 0; t < 9; t++)
+#pragma unroll
+            for (int i = 0; i < 2; i++)
+#pragma unroll
+                for (int j = 0; j < 2; j++)
+#pragma unroll
+                    for (int e = 0; e < 4; e++) park[(region * 144 + ((t * 2 + i) * 2 + j) * 4 + e) * 64 + lane] = acc[t][i][j][e];
+    };
+    auto takeAcc = [&](int region) {
+#pragma unroll
+        for (int t = 0; t < 9; t++)
+#pragma unroll
+            for (int i = 0; i < 2; i++)
+#pragma unroll
+                for (int j = 0; j < 2; j++)
+#pragma unroll
+                    for (int e = 0; e < 4; e++) acc[t][i][j][e] += park[(region * 144 + ((t * 2 + i) * 2 + j) * 4 + e) * 64 + lane];
+    };
+    if (jj == 0)
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+            for (int e = 0; e < 4; e++) biasPark[wave * 32 + 16 * i + 4 * g + e] = accB[i][e];
+    if (wave >= 2) parkAcc(wave - 2);
+    __syncthreads();
+    if (wave < 2) takeAcc(wave);
+    __syncthreads();
+    if (wave == 1) parkAcc(0);
+    __syncthreads();
+    if (wave != 0) return;
+    takeAcc(0);
+    float* slot = a.part + ((size_t)pair * gridDim.x + blockIdx.x) * kWgSlot;
+#pragma unroll
+    for (int t = 0; t < 9; t++)
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+            for (int j = 0; j < 2; j++)
+#pragma unroll
+                for (int e = 0; e < 4; e++) slot[(t * 32 + 16 * i + 4 * g + e) * 32 + 16 * j + jj] = acc[t][i][j][e];
+    if (bias && lane < 32)
+        slot[9 * 1024 + lane] = biasPark[lane] + biasPark[32 + lane] + biasPark[64 + lane] + biasPark[96 + lane];
+}
+
+// dW[tap][co][ci] += the sum over a pair's blocks of their slots (and dB for the pairs of
+// input-channel group 0).  blockIdx.y takes kWgReduceRun consecutive blocks of every pair, and
+// their sum is added with one atomic.
+constexpr int kWgReduceRun = 16;
+__global__ void wgReduceKernel(const float* part, int blocksPerPair, int pairs, int nci, int cinPad, int coutPad,
+                               float* dW, float* dB) {
+    const long long idx = blockIdx.x * (long long)kB + threadIdx.x;
+    if (idx >= (long long)pairs * kWgSlot) return;
+    const int pair = (int)(idx / kWgSlot), k = (int)(idx % kWgSlot);
+    const int cob = (pair / nci) * 32, cib = (pair % nci) * 32;
+    if (k >= 9 * 1024 && cib != 0) return;
+    const int b0 = blockIdx.y * kWgReduceRun, b1 = min(blocksPerPair, b0 + kWgReduceRun);
+    const float* src = part + (size_t)pair * blocksPerPair * kWgSlot + k;
+    float sum = 0.0f;
+    for (int b = b0; b < b1; b++) sum += src[(size_t)b * kWgSlot];
+    if (k < 9 * 1024) {
+        const int tap = k >> 10, co = (k >> 5) & 31, ci = k & 31;
+        atomicAdd(&dW[((size_t)tap * coutPad + cob + co) * cinPad + cib + ci], sum);
+    } else {
+        atomicAdd(&dB[cob + k - 9 * 1024], sum);
+    }
+}
+
 // ------------------------------------------------------------------ elementwise kernels
 // Pooling and upsampling move 8 channels (16 bytes) per thread: every view's channel
 // stride, offset and width are multiples of 32, so each 8-channel group is 16-byte aligned.
@@ -660,14 +907,28 @@ __global__ void adamKernel(float* w, const float* g, float* m, float* v, long lo
 }
 
 // FP32 master [9][co][ci] -> BF16 forward copy (same layout) and BF16 input-gradient copy
-// [9][ci][co] with mirrored taps.
-__global__ void packWeightsKernel(const float* w, int coPad, int ciPad, bf16* wf, bf16* wb) {
-    const long long i = blockIdx.x * (long long)kB + threadIdx.x;
-    if (i >= 9LL * coPad * ciPad) return;
-    const int ci = (int)(i % ciPad), co = (int)((i / ciPad) % coPad), tap = (int)(i / ((long long)ciPad * coPad));
-    const bf16 v = (bf16)w[i];
-    wf[i] = v;
-    wb[((size_t)(8 - tap) * ciPad + ci) * coPad + co] = v;
+// [9][ci][co] with mirrored taps, for every layer in one launch (blocks firstBlock.. of the
+// grid belong to a layer).
+constexpr int kMaxPackLayers = 24;
+struct PackLayer {
+    const float* w;
+    bf16 *wf, *wb;
+    int coPad, ciPad, firstBlock;
+};
+struct PackArgs {
+    PackLayer layer[kMaxPackLayers];
+    int count;
+};
+__global__ void packWeightsKernel(PackArgs a) {
+    int l = 0;
+    while (l + 1 < a.count && (int)blockIdx.x >= a.layer[l + 1].firstBlock) l++;
+    const PackLayer& L = a.layer[l];
+    const long long i = (blockIdx.x - L.firstBlock) * (long long)kB + threadIdx.x;
+    if (i >= 9LL * L.coPad * L.ciPad) return;
+    const int ci = (int)(i % L.ciPad), co = (int)((i / L.ciPad) % L.coPad), tap = (int)(i / ((long long)L.ciPad * L.coPad));
+    const bf16 v = (bf16)L.w[i];
+    L.wf[i] = v;
+    L.wb[((size_t)(8 - tap) * L.ciPad + ci) * L.coPad + co] = v;
 }
 
 inline int blocks(long long n) { return (int)((n + kB - 1) / kB); }
@@ -701,6 +962,9 @@ struct DenoiserNet {
     int dsN = 0, dsH = 0, dsW = 0;
     Crop* dCrops = nullptr;
     int cropCap = 0;
+    // per-block slots of the weight-gradient kernel (summed by wgReduceKernel)
+    float* wgPart = nullptr;
+    size_t wgPartFloats = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     float lastFwdMs = 0, lastTrainMs = 0;
@@ -788,7 +1052,36 @@ hipError_t launchConvFwd(const ConvArgs& a, hipStream_t s) {
 // atomics into dW and dB (zeroed once per backward pass).
 hipError_t convWeightGrad(DenoiserNet* net, const View& dy, const View& in, int n, int h, int w, int cinPad,
                           int coutPad, float* dW, float* dB, hipStream_t s) {
-    (void)net;
+    if ((dy.cs | dy.co | in.cs | in.co) % 8 == 0 && (long long)n * h * w * std::max(dy.cs, in.cs) * 2 < (long long)kOutside) {
+        static int cus = 0;
+        if (!cus) {
+            int dev = 0;
+            hipError_t e = hipGetDevice(&dev);
+            if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+            if (e != hipSuccess) return e;
+        }
+        const int S = w > 32 ? 2 : 1, TH = 16 / S, TW = 32 * S;
+        const int tilesX = (w + TW - 1) / TW, tilesY = (h + TH - 1) / TH, tileCount = tilesX * tilesY * n;
+        const int nci = cinPad / 32, pairs = (coutPad / 32) * nci;
+        const int per = std::max(1, std::min(tileCount, 2 * cus / pairs));   // two blocks per compute unit
+        const size_t need = (size_t)pairs * per * kWgSlot;
+        if (need > net->wgPartFloats) {
+            if (net->wgPart) (void)hipFree(net->wgPart);
+            net->wgPart = nullptr;
+            net->wgPartFloats = 0;
+            const hipError_t e = hipMalloc(&net->wgPart, need * sizeof(float));
+            if (e != hipSuccess) return e;
+            net->wgPartFloats = need;
+        }
+        BwdWTileArgs a{dy, in, net->wgPart, n, h, w, nci, tilesX, tilesY, tileCount};
+        if (S == 2) hipLaunchKernelGGL(convBwdWTileKernel<2>, dim3(per, pairs), dim3(kWgThreads), 0, s, a);
+        else hipLaunchKernelGGL(convBwdWTileKernel<1>, dim3(per, pairs), dim3(kWgThreads), 0, s, a);
+        hipError_t e;
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        hipLaunchKernelGGL(wgReduceKernel, dim3(blocks((long long)pairs * kWgSlot), (per + kWgReduceRun - 1) / kWgReduceRun),
+                           dim3(kB), 0, s, net->wgPart, per, pairs, nci, cinPad, coutPad, dW, dB);
+        return hipGetLastError();
+    }
     const int npix = n * h * w;
     int parts = 64;
     int per = (npix + parts - 1) / parts;
@@ -1011,13 +1304,17 @@ hipError_t backward(DenoiserNet* net, hipStream_t s) {
 }
 
 hipError_t packAll(DenoiserNet* net, hipStream_t s) {
+    static_assert(kLayers <= kMaxPackLayers, "one packing launch covers every layer");
+    PackArgs a{};
+    int total = 0;
     for (int l = 0; l < kLayers; l++) {
         const DnLayer& L = net->layers[l];
-        hipLaunchKernelGGL(packWeightsKernel, dim3(blocks(9LL * L.coutPad * L.cinPad)), dim3(kB), 0, s, net->params + L.wOff,
-                           L.coutPad, L.cinPad, wfOf(net, l), wbOf(net, l));
-        DN_TRY(hipGetLastError());
+        a.layer[l] = PackLayer{net->params + L.wOff, wfOf(net, l), wbOf(net, l), L.coutPad, L.cinPad, total};
+        total += blocks(9LL * L.coutPad * L.cinPad);
     }
-    return hipSuccess;
+    a.count = kLayers;
+    hipLaunchKernelGGL(packWeightsKernel, dim3(total), dim3(kB), 0, s, a);
+    return hipGetLastError();
 }
 
 }  // namespace
@@ -1050,7 +1347,7 @@ void denoiserDestroy(DenoiserNet* net) {
     if (!net) return;
     freeActivations(net);
     for (void* p : {(void*)net->params, (void*)net->grads, (void*)net->adamM, (void*)net->adamV, (void*)net->wbf,
-                    (void*)net->dsIn, (void*)net->dsTgt, (void*)net->dCrops})
+                    (void*)net->dsIn, (void*)net->dsTgt, (void*)net->dCrops, (void*)net->wgPart})
         if (p) (void)hipFree(p);
     if (net->ev0) (void)hipEventDestroy(net->ev0);
     if (net->ev1) (void)hipEventDestroy(net->ev1);

@@ -109,14 +109,20 @@ def random_scene(seed, kinds=("sphere", "soup")):
     return tris, cam, st
 
 
-def fine_pool(count, subdiv, log=None):
-    """Displaced-icosphere rooms at full tessellation (other seeds than the benchmark scene),
-    shared by the fine views."""
+def fine_pool(count, subdiv, log=None, first=0, geodesic=False):
+    """Displaced-sphere rooms at full tessellation (other seeds than the benchmark scene),
+    shared by the fine views: mesh k has seed 5000 + first + k and is the subdivision-`subdiv`
+    icosphere, or with `geodesic` every other mesh (odd first + k) the 7,009,280-triangle
+    frequency-592 geodesic sphere."""
     pool = []
     for k in range(count):
-        rng = np.random.default_rng(5000 + k)
-        tris = scenes.displaced_icosphere(subdiv=subdiv, seed=5000 + k, amplitude=float(rng.uniform(0.03, 0.08)),
-                                          metal_probability=float(rng.uniform(0.1, 0.45)))
+        seed = 5000 + first + k
+        rng = np.random.default_rng(seed)
+        amp, metal = float(rng.uniform(0.03, 0.08)), float(rng.uniform(0.1, 0.45))
+        if geodesic and (first + k) % 2 == 1:
+            tris = scenes.displaced_geodesic(frequency=592, seed=seed, amplitude=amp, metal_probability=metal)
+        else:
+            tris = scenes.displaced_icosphere(subdiv=subdiv, seed=seed, amplitude=amp, metal_probability=metal)
         tris["emission"] = np.where(tris["emission"] > 0, rng.uniform(8, 20), 0).astype(np.float32)
         pool.append(tris)
         if log:
@@ -211,6 +217,9 @@ def main():
                          "through the field of view of a crop of a 1080-line frame")
     ap.add_argument("--fine-meshes", type=int, default=4)
     ap.add_argument("--fine-subdiv", type=int, default=9)
+    ap.add_argument("--fine-first", type=int, default=0, help="seed offset of the first fine mesh")
+    ap.add_argument("--fine-geodesic", action="store_true",
+                    help="every other fine mesh is the frequency-592 geodesic sphere instead of the icosphere")
     ap.add_argument("--kinds", default="sphere,soup",
                     help="scene generators cycled over the seeds: sphere (C3), soup (C2), cornell")
     ap.add_argument("--first-seed", type=int, default=100, help="seed of the first training scene")
@@ -229,7 +238,7 @@ def main():
     kinds = tuple(k for k in args.kinds.split(",") if k)
     if not kinds or any(k not in KINDS for k in kinds):
         raise SystemExit(f"--kinds: choose from {KINDS}")
-    pool = fine_pool(args.fine_meshes, args.fine_subdiv, log) if args.fine > 0 else ()
+    pool = fine_pool(args.fine_meshes, args.fine_subdiv, log, args.fine_first, args.fine_geodesic) if args.fine > 0 else ()
     inputs, targets = build_dataset(lib, args.scenes, args.size, mix, args.target_spp, args.first_seed, log, args.fine,
                                     pool, kinds)
     hin, htg = build_dataset(lib, args.heldout, args.size, args.input_spp, args.target_spp, 100000, log, args.fine,
@@ -259,7 +268,7 @@ def main():
               "init_weights": args.init_weights, "lr_from": args.lr_from, "lr_to": args.lr_to,
               "heldout": args.heldout, "size": args.size, "input_spp": args.input_spp,
               "input_spp_mix": args.input_spp_mix, "fine": args.fine, "fine_meshes": args.fine_meshes,
-              "fine_subdiv": args.fine_subdiv,
+              "fine_subdiv": args.fine_subdiv, "fine_first": args.fine_first, "fine_geodesic": args.fine_geodesic,
               "target_spp": args.target_spp, "steps": args.steps, "batch": args.batch, "crop": args.crop,
               "display_weight": args.display_weight,
               "dataset_seconds": round(data_s, 1), "train_seconds": round(train_s, 1),
