@@ -486,6 +486,7 @@ __device__ __forceinline__ bf16x8 wgPack(s16x4 lo, s16x4 hi) {
 #ifndef WG_IN_PARTS
 #define WG_IN_PARTS 1
 #endif
+
 template <int S>
 __global__ __launch_bounds__(kWgThreads, 2) void convBwdWTileKernel(BwdWTileArgs a) {
     constexpr int RG = 4 / S, TH = 4 * RG, TW = 32 * S, HH = TH + 2, HW = TW + 2, HWP = (HW + 3) / 4 * 4;
@@ -653,17 +654,16 @@ __global__ __launch_bounds__(kWgThreads, 2) void convBwdWTileKernel(BwdWTileArgs
 }
 
 // dW[tap][co][ci] += the sum over a pair's blocks of their slots (and dB for the pairs of
-// input-channel group 0).  blockIdx.y takes kWgReduceRun consecutive blocks of every pair, and
+// input-channel group 0).  Run `run` takes kWgReduceRun consecutive blocks of every pair, and
 // their sum is added with one atomic.
 constexpr int kWgReduceRun = 16;
-__global__ void wgReduceKernel(const float* part, int blocksPerPair, int pairs, int nci, int cinPad, int coutPad,
-                               float* dW, float* dB) {
-    const long long idx = blockIdx.x * (long long)kB + threadIdx.x;
+__device__ __forceinline__ void wgReduceRun(const float* part, int blocksPerPair, int pairs, int nci, int cinPad,
+                                            int coutPad, float* dW, float* dB, long long idx, int run) {
     if (idx >= (long long)pairs * kWgSlot) return;
     const int pair = (int)(idx / kWgSlot), k = (int)(idx % kWgSlot);
     const int cob = (pair / nci) * 32, cib = (pair % nci) * 32;
     if (k >= 9 * 1024 && cib != 0) return;
-    const int b0 = blockIdx.y * kWgReduceRun, b1 = min(blocksPerPair, b0 + kWgReduceRun);
+    const int b0 = run * kWgReduceRun, b1 = min(blocksPerPair, b0 + kWgReduceRun);
     const float* src = part + (size_t)pair * blocksPerPair * kWgSlot + k;
     float sum = 0.0f;
     for (int b = b0; b < b1; b++) sum += src[(size_t)b * kWgSlot];
@@ -673,6 +673,31 @@ __global__ void wgReduceKernel(const float* part, int blocksPerPair, int pairs, 
     } else {
         atomicAdd(&dB[cob + k - 9 * 1024], sum);
     }
+}
+__global__ void wgReduceKernel(const float* part, int blocksPerPair, int pairs, int nci, int cinPad, int coutPad,
+                               float* dW, float* dB) {
+    wgReduceRun(part, blocksPerPair, pairs, nci, cinPad, coutPad, dW, dB, blockIdx.x * (long long)kB + threadIdx.x,
+                blockIdx.y);
+}
+// The reductions of every layer of a backward pass in one launch: blocks firstBlock.. of the
+// grid belong to a layer, xBlocks of them per run.
+constexpr int kMaxWgLayers = 24;
+struct WgReduceLayer {
+    const float* part;
+    float *dW, *dB;
+    int per, pairs, nci, cinPad, coutPad, xBlocks, firstBlock;
+};
+struct WgReduceArgs {
+    WgReduceLayer layer[kMaxWgLayers];
+    int count;
+};
+__global__ void wgReduceAllKernel(WgReduceArgs a) {
+    int l = 0;
+    while (l + 1 < a.count && (int)blockIdx.x >= a.layer[l + 1].firstBlock) l++;
+    const WgReduceLayer& L = a.layer[l];
+    const int local = blockIdx.x - L.firstBlock;
+    wgReduceRun(L.part, L.per, L.pairs, L.nci, L.cinPad, L.coutPad, L.dW, L.dB,
+                (local % L.xBlocks) * (long long)kB + threadIdx.x, local / L.xBlocks);
 }
 
 // ------------------------------------------------------------------ elementwise kernels
@@ -796,20 +821,28 @@ constexpr float kDemodEps = 0.01f;
 
 // Raw planes [p][10] = radiance, albedo, normal, depth -> BF16 features [p][32], the
 // demodulated log radiance the residual is added to, and the modulation factors.
-__device__ __forceinline__ void writeFeatures(const float* raw, bf16* x0, float* noisyLog, float* demod, size_t p) {
-    float f[kDenoiseInputChannels];
+// Each pixel's outputs go out as whole 16-byte vectors.
+__device__ __forceinline__ float4 writeFeatures(const float* raw, bf16* x0, float* noisyLog, float* demod, size_t p) {
+    float f[kDenoiseInputChannels], m[3];
+#pragma unroll
     for (int c = 0; c < 3; c++) {
-        const float m = fmaxf(raw[3 + c], 0.0f) + kDemodEps;
-        f[c] = log1pf(fmaxf(raw[c], 0.0f) / m);
-        demod[p * 4 + c] = m;
+        m[c] = fmaxf(raw[3 + c], 0.0f) + kDemodEps;
+        f[c] = log1pf(fmaxf(raw[c], 0.0f) / m[c]);
     }
-    demod[p * 4 + 3] = 0.0f;
+#pragma unroll
     for (int c = 3; c < 9; c++) f[c] = raw[c];
     f[9] = log1pf(fmaxf(raw[9], 0.0f));
-    bf16* dst = x0 + p * 32;
-    for (int c = 0; c < 32; c++) dst[c] = (bf16)(c < kDenoiseInputChannels ? f[c] : 0.0f);
-    for (int c = 0; c < 3; c++) noisyLog[p * 4 + c] = f[c];
-    noisyLog[p * 4 + 3] = 0.0f;
+    bf16x8 lo, hi = zero8();
+#pragma unroll
+    for (int c = 0; c < 8; c++) lo[c] = (bf16)f[c];
+    hi[0] = (bf16)f[8];
+    hi[1] = (bf16)f[9];
+    bf16x8* dst = reinterpret_cast<bf16x8*>(x0 + p * 32);
+    dst[0] = lo; dst[1] = hi; dst[2] = zero8(); dst[3] = zero8();
+    const float4 dm = make_float4(m[0], m[1], m[2], 0.0f);
+    *reinterpret_cast<float4*>(demod + p * 4) = dm;
+    *reinterpret_cast<float4*>(noisyLog + p * 4) = make_float4(f[0], f[1], f[2], 0.0f);
+    return dm;
 }
 
 __global__ void featuresKernel(const float* raw, int npix, bf16* x0, float* noisyLog, float* demod) {
@@ -829,9 +862,10 @@ __global__ void cropKernel(const float* dsIn, const float* dsTgt, int dsH, int d
     const int sx = cr.ox + ((cr.flip & 1) ? crop - 1 - x : x);
     const int sy = cr.oy + ((cr.flip & 2) ? crop - 1 - y : y);
     const size_t sp = ((size_t)cr.image * dsH + sy) * dsW + sx;
-    writeFeatures(dsIn + sp * kDenoiseInputChannels, x0, noisyLog, demod, (size_t)i);
-    for (int c = 0; c < 3; c++) targetLog[i * 4 + c] = log1pf(fmaxf(dsTgt[sp * 3 + c], 0.0f) / demod[i * 4 + c]);
-    targetLog[i * 4 + 3] = 0.0f;
+    const float4 m = writeFeatures(dsIn + sp * kDenoiseInputChannels, x0, noisyLog, demod, (size_t)i);
+    const float* t = dsTgt + sp * 3;
+    *reinterpret_cast<float4*>(targetLog + i * 4) = make_float4(
+        log1pf(fmaxf(t[0], 0.0f) / m.x), log1pf(fmaxf(t[1], 0.0f) / m.y), log1pf(fmaxf(t[2], 0.0f) / m.z), 0.0f);
 }
 
 // Display transform of Native/renderer.cu:86-90 at exposure 1 without the 8-bit rounding:
@@ -855,27 +889,34 @@ __global__ void lossKernel(const float* out, const float* noisyLog, const float*
     float l = 0;
     if (p < npix) {
         const float scale = 1.0f / (3.0f * npix);
-        for (int c = 0; c < 32; c++) {
-            float gr = 0;
-            if (c < 3) {
-                const float y = noisyLog[(size_t)p * 4 + c] + out[(size_t)p * 4 + c];
-                const float tl = targetLog[(size_t)p * 4 + c];
-                const float d = y - tl;
-                l += fabsf(d);
-                gr = d > 0 ? scale : (d < 0 ? -scale : 0.0f);
-                if (displayWeight > 0) {
-                    const float m = demod[(size_t)p * 4 + c];
-                    const float em = expm1f(y);
-                    float sp, st;
-                    const float dp = displayValue(fmaxf(em, 0.0f) * m, sp);
-                    const float dt = displayValue(fmaxf(expm1f(tl), 0.0f) * m, st);
-                    const float dd = dp - dt;
-                    l += displayWeight * fabsf(dd);
-                    if (em > 0 && dd != 0) gr += (dd > 0 ? scale : -scale) * displayWeight * sp * (em + 1.0f) * m;
-                }
+        const float4 n4 = *reinterpret_cast<const float4*>(noisyLog + (size_t)p * 4);
+        const float4 o4 = *reinterpret_cast<const float4*>(out + (size_t)p * 4);
+        const float4 t4 = *reinterpret_cast<const float4*>(targetLog + (size_t)p * 4);
+        const float4 m4 = *reinterpret_cast<const float4*>(demod + (size_t)p * 4);
+        const float ny[3] = {n4.x, n4.y, n4.z}, oy[3] = {o4.x, o4.y, o4.z}, ty[3] = {t4.x, t4.y, t4.z};
+        const float my[3] = {m4.x, m4.y, m4.z};
+        bf16x8 g = zero8();
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const float y = ny[c] + oy[c];
+            const float tl = ty[c];
+            const float d = y - tl;
+            l += fabsf(d);
+            float gr = d > 0 ? scale : (d < 0 ? -scale : 0.0f);
+            if (displayWeight > 0) {
+                const float m = my[c];
+                const float em = expm1f(y);
+                float sp, st;
+                const float dp = displayValue(fmaxf(em, 0.0f) * m, sp);
+                const float dt = displayValue(fmaxf(expm1f(tl), 0.0f) * m, st);
+                const float dd = dp - dt;
+                l += displayWeight * fabsf(dd);
+                if (em > 0 && dd != 0) gr += (dd > 0 ? scale : -scale) * displayWeight * sp * (em + 1.0f) * m;
             }
-            gout[(size_t)p * 32 + c] = (bf16)gr;
+            g[c] = (bf16)gr;
         }
+        bf16x8* dst = reinterpret_cast<bf16x8*>(gout + (size_t)p * 32);
+        dst[0] = g; dst[1] = zero8(); dst[2] = zero8(); dst[3] = zero8();
     }
     red[threadIdx.x] = l;
     __syncthreads();
@@ -962,9 +1003,18 @@ struct DenoiserNet {
     int dsN = 0, dsH = 0, dsW = 0;
     Crop* dCrops = nullptr;
     int cropCap = 0;
-    // per-block slots of the weight-gradient kernel (summed by wgReduceKernel)
+    // per-block slots of the weight-gradient kernel (summed by wgReduceKernel); a backward
+    // pass gives every layer its own slots and sums them all in one launch at the end
     float* wgPart = nullptr;
     size_t wgPartFloats = 0;
+    struct WgPending {
+        size_t off;
+        int per, pairs, nci, cinPad, coutPad;
+        float *dW, *dB;
+    };
+    std::vector<WgPending> wgPending;
+    bool wgDefer = false;
+    size_t wgUsed = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     float lastFwdMs = 0, lastTrainMs = 0;
@@ -1001,48 +1051,53 @@ bool tileConv(const ConvArgs& a) {
            (!a.low.p || (a.lowC % 32 == 0 && a.h % 2 == 0 && a.w % 2 == 0 && (a.low.cs | a.low.co) % 8 == 0));
 }
 
+// The persistent tile kernel for a convolution that tileConv accepts.
+hipError_t launchConvTile(const ConvArgs& a, hipStream_t s) {
+    const int tilesX = (a.w + kTileW - 1) / kTileW, tilesY = (a.h + kTileH - 1) / kTileH;
+    const int tileCount = tilesX * tilesY * a.n;
+    // persistent: as many blocks as the GPU holds at once (fewer for small images)
+    static int resident[3] = {0, 0, 0}, cus = 0;
+    if (!resident[0]) {
+        int dev = 0, perCu2 = 0, perCu4 = 0;
+        hipError_t e = hipGetDevice(&dev);
+        if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        if (e == hipSuccess)
+            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu2, convFwdTileKernel<2>, kTileThreads, 0);
+        if (e == hipSuccess)
+            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu4, convFwdTileKernel<4>, kTileThreads, 0);
+        if (e != hipSuccess) return e;
+        resident[0] = std::max(1, cus * std::max(1, perCu2));
+        resident[1] = std::max(1, cus * std::max(1, perCu4));
+    }
+    // Only an FP32 output of at most 16 channels (the network's last layer): the first
+    // 16-channel group suffices.
+    if (!a.out.p && a.outF && a.outFStride <= 16) {
+        if (!resident[2]) {
+            int perCu1 = 0;
+            const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu1, convFwdTileKernel<1>, kTileThreads, 0);
+            if (e != hipSuccess) return e;
+            resident[2] = std::max(1, cus * std::max(1, perCu1));
+        }
+        hipLaunchKernelGGL(convFwdTileKernel<1>, dim3(std::min(tileCount, resident[2]), 1), dim3(kTileThreads), 0, s, a,
+                           tilesX, tilesY, tileCount);
+        return hipGetLastError();
+    }
+    // 64-channel groups read each halo once for twice the channels; 32-channel groups make
+    // twice the blocks, for small images that would leave compute units idle
+    const bool wide = a.coutPad % 64 == 0 && (long long)tileCount * (a.coutPad / 64) >= 2LL * cus;
+    const int groups = a.coutPad / (wide ? 64 : 32);
+    const dim3 grid(std::min(tileCount, std::max(1, resident[wide ? 1 : 0] / groups)), groups);
+    if (wide) hipLaunchKernelGGL(convFwdTileKernel<4>, grid, dim3(kTileThreads), 0, s, a, tilesX, tilesY, tileCount);
+    else hipLaunchKernelGGL(convFwdTileKernel<2>, grid, dim3(kTileThreads), 0, s, a, tilesX, tilesY, tileCount);
+    return hipGetLastError();
+}
+
 hipError_t launchConvFwd(const ConvArgs& a, hipStream_t s) {
     const int npix = a.n * a.h * a.w;
     const int tiles = a.coutPad / 16;
     if (a.cinPad % 32 || a.coutPad % 32 || a.in.c != a.cinPad - a.lowC) return hipErrorInvalidValue;
     if (a.low.p && !tileConv(a)) return hipErrorInvalidValue;   // only the tile kernel reads upsampled channels
-    if (tileConv(a)) {
-        const int tilesX = (a.w + kTileW - 1) / kTileW, tilesY = (a.h + kTileH - 1) / kTileH;
-        const int tileCount = tilesX * tilesY * a.n;
-        // persistent: as many blocks as the GPU holds at once (fewer for small images)
-        static int resident[3] = {0, 0, 0}, cus = 0;
-        if (!resident[0]) {
-            int dev = 0, perCu2 = 0, perCu4 = 0;
-            hipError_t e = hipGetDevice(&dev);
-            if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-            if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu2, convFwdTileKernel<2>, kTileThreads, 0);
-            if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu4, convFwdTileKernel<4>, kTileThreads, 0);
-            if (e != hipSuccess) return e;
-            resident[0] = std::max(1, cus * std::max(1, perCu2));
-            resident[1] = std::max(1, cus * std::max(1, perCu4));
-        }
-        // Only an FP32 output of at most 16 channels (the network's last layer): the first
-        // 16-channel group suffices.
-        if (!a.out.p && a.outF && a.outFStride <= 16) {
-            if (!resident[2]) {
-                int perCu1 = 0;
-                const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu1, convFwdTileKernel<1>, kTileThreads, 0);
-                if (e != hipSuccess) return e;
-                resident[2] = std::max(1, cus * std::max(1, perCu1));
-            }
-            hipLaunchKernelGGL(convFwdTileKernel<1>, dim3(std::min(tileCount, resident[2]), 1), dim3(kTileThreads), 0, s, a,
-                               tilesX, tilesY, tileCount);
-            return hipGetLastError();
-        }
-        // 64-channel groups read each halo once for twice the channels; 32-channel groups make
-        // twice the blocks, for small images that would leave compute units idle
-        const bool wide = a.coutPad % 64 == 0 && (long long)tileCount * (a.coutPad / 64) >= 2LL * cus;
-        const int groups = a.coutPad / (wide ? 64 : 32);
-        const dim3 grid(std::min(tileCount, std::max(1, resident[wide ? 1 : 0] / groups)), groups);
-        if (wide) hipLaunchKernelGGL(convFwdTileKernel<4>, grid, dim3(kTileThreads), 0, s, a, tilesX, tilesY, tileCount);
-        else hipLaunchKernelGGL(convFwdTileKernel<2>, grid, dim3(kTileThreads), 0, s, a, tilesX, tilesY, tileCount);
-        return hipGetLastError();
-    }
+    if (tileConv(a)) return launchConvTile(a, s);
     if (tiles % 4 == 0) hipLaunchKernelGGL(convFwdKernel<4>, dim3(blocks(npix), tiles / 4), dim3(kB), 0, s, a);
     else hipLaunchKernelGGL(convFwdKernel<2>, dim3(blocks(npix), tiles / 2), dim3(kB), 0, s, a);
     return hipGetLastError();
@@ -1050,36 +1105,65 @@ hipError_t launchConvFwd(const ConvArgs& a, hipStream_t s) {
 
 // Weight and bias gradients of one layer from the pre-activation gradient dy, added with
 // atomics into dW and dB (zeroed once per backward pass).
+#define DN_TRY(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) return e_; } while (0)
+
+// Launch shape of the tiled weight-gradient kernel for one layer.
+struct WgPlan {
+    int S, tilesX, tilesY, tileCount, nci, pairs, per;
+    size_t slots() const { return (size_t)pairs * per * kWgSlot; }
+};
+hipError_t wgPlan(int n, int h, int w, int cinPad, int coutPad, WgPlan& p) {
+    static int cus = 0;
+    if (!cus) {
+        int dev = 0;
+        hipError_t e = hipGetDevice(&dev);
+        if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        if (e != hipSuccess) return e;
+    }
+    p.S = w > 32 ? 2 : 1;
+    const int TH = 16 / p.S, TW = 32 * p.S;
+    p.tilesX = (w + TW - 1) / TW;
+    p.tilesY = (h + TH - 1) / TH;
+    p.tileCount = p.tilesX * p.tilesY * n;
+    p.nci = cinPad / 32;
+    p.pairs = (coutPad / 32) * p.nci;
+    p.per = std::max(1, std::min(p.tileCount, 2 * cus / p.pairs));   // two blocks per compute unit
+    return hipSuccess;
+}
+
+hipError_t ensureWgSlots(DenoiserNet* net, size_t need) {
+    if (need <= net->wgPartFloats) return hipSuccess;
+    if (net->wgPart) (void)hipFree(net->wgPart);
+    net->wgPart = nullptr;
+    net->wgPartFloats = 0;
+    const hipError_t e = hipMalloc(&net->wgPart, need * sizeof(float));
+    if (e == hipSuccess) net->wgPartFloats = need;
+    return e;
+}
+
 hipError_t convWeightGrad(DenoiserNet* net, const View& dy, const View& in, int n, int h, int w, int cinPad,
                           int coutPad, float* dW, float* dB, hipStream_t s) {
     if ((dy.cs | dy.co | in.cs | in.co) % 8 == 0 && (long long)n * h * w * std::max(dy.cs, in.cs) * 2 < (long long)kOutside) {
-        static int cus = 0;
-        if (!cus) {
-            int dev = 0;
-            hipError_t e = hipGetDevice(&dev);
-            if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-            if (e != hipSuccess) return e;
+        WgPlan p;
+        DN_TRY(wgPlan(n, h, w, cinPad, coutPad, p));
+        size_t off = 0;
+        if (net->wgDefer) {   // the backward pass sized the slots for every layer
+            off = net->wgUsed;
+            if (off + p.slots() > net->wgPartFloats) return hipErrorInvalidValue;
+            net->wgUsed += p.slots();
+        } else {
+            DN_TRY(ensureWgSlots(net, p.slots()));
         }
-        const int S = w > 32 ? 2 : 1, TH = 16 / S, TW = 32 * S;
-        const int tilesX = (w + TW - 1) / TW, tilesY = (h + TH - 1) / TH, tileCount = tilesX * tilesY * n;
-        const int nci = cinPad / 32, pairs = (coutPad / 32) * nci;
-        const int per = std::max(1, std::min(tileCount, 2 * cus / pairs));   // two blocks per compute unit
-        const size_t need = (size_t)pairs * per * kWgSlot;
-        if (need > net->wgPartFloats) {
-            if (net->wgPart) (void)hipFree(net->wgPart);
-            net->wgPart = nullptr;
-            net->wgPartFloats = 0;
-            const hipError_t e = hipMalloc(&net->wgPart, need * sizeof(float));
-            if (e != hipSuccess) return e;
-            net->wgPartFloats = need;
+        BwdWTileArgs a{dy, in, net->wgPart + off, n, h, w, p.nci, p.tilesX, p.tilesY, p.tileCount};
+        if (p.S == 2) hipLaunchKernelGGL(convBwdWTileKernel<2>, dim3(p.per, p.pairs), dim3(kWgThreads), 0, s, a);
+        else hipLaunchKernelGGL(convBwdWTileKernel<1>, dim3(p.per, p.pairs), dim3(kWgThreads), 0, s, a);
+        DN_TRY(hipGetLastError());
+        if (net->wgDefer) {
+            net->wgPending.push_back(DenoiserNet::WgPending{off, p.per, p.pairs, p.nci, cinPad, coutPad, dW, dB});
+            return hipSuccess;
         }
-        BwdWTileArgs a{dy, in, net->wgPart, n, h, w, nci, tilesX, tilesY, tileCount};
-        if (S == 2) hipLaunchKernelGGL(convBwdWTileKernel<2>, dim3(per, pairs), dim3(kWgThreads), 0, s, a);
-        else hipLaunchKernelGGL(convBwdWTileKernel<1>, dim3(per, pairs), dim3(kWgThreads), 0, s, a);
-        hipError_t e;
-        if ((e = hipGetLastError()) != hipSuccess) return e;
-        hipLaunchKernelGGL(wgReduceKernel, dim3(blocks((long long)pairs * kWgSlot), (per + kWgReduceRun - 1) / kWgReduceRun),
-                           dim3(kB), 0, s, net->wgPart, per, pairs, nci, cinPad, coutPad, dW, dB);
+        hipLaunchKernelGGL(wgReduceKernel, dim3(blocks((long long)p.pairs * kWgSlot), (p.per + kWgReduceRun - 1) / kWgReduceRun),
+                           dim3(kB), 0, s, net->wgPart, p.per, p.pairs, p.nci, cinPad, coutPad, dW, dB);
         return hipGetLastError();
     }
     const int npix = n * h * w;
@@ -1177,7 +1261,6 @@ hipError_t upsample(const View& in, const View& out, int n, int h, int w, int c,
     return hipGetLastError();
 }
 
-#define DN_TRY(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) return e_; } while (0)
 
 // Forward pass from net->x0 to net->outF.
 //
@@ -1260,9 +1343,8 @@ hipError_t convBack(DenoiserNet* net, int l, const View& g, const View& act, con
     return hipSuccess;
 }
 
-hipError_t backward(DenoiserNet* net, hipStream_t s) {
+hipError_t backwardLayers(DenoiserNet* net, hipStream_t s) {
     const int n = net->n, h0 = net->h, w0 = net->w;
-    DN_TRY(hipMemsetAsync(net->grads, 0, net->nParams * sizeof(float), s));
     const int h1 = h0 / 2, w1 = w0 / 2, h2 = h0 / 4, w2 = w0 / 4, h3 = h0 / 8, w3 = w0 / 8, h4 = h0 / 16, w4 = w0 / 16;
     const View none{nullptr, 0, 0, 0};
     DN_TRY(convBack(net, OUT, V(net->gout, 32), none, V(net->d0b, 32), V(net->gd0b, 32), false, n, h0, w0, s));
@@ -1300,6 +1382,43 @@ hipError_t backward(DenoiserNet* net, hipStream_t s) {
                        V(net->gD0, 96, 64, 32), n, h0, w0, 32);
     DN_TRY(convBack(net, E0B, V(net->gD0, 96, 64, 32), V(net->D0, 96, 64, 32), V(net->e0a, 32), V(net->ge0a, 32), false, n, h0, w0, s));
     DN_TRY(convBack(net, E0A, V(net->ge0a, 32), V(net->e0a, 32), V(net->x0, 32), none, false, n, h0, w0, s));
+    return hipGetLastError();
+}
+
+// Level (pooling depth) of each layer, in kSpec order.
+constexpr int kLayerLevel[] = {0, 0, 1, 1, 2, 2, 3, 3, 4, 4, 3, 3, 2, 2, 1, 1, 0, 0, 0};
+static_assert(sizeof(kLayerLevel) / sizeof(kLayerLevel[0]) == kLayers, "one level per layer");
+
+// Gradients of every parameter: the layers' weight-gradient kernels write their block slots
+// into one workspace, and a single launch at the end sums them into the gradient.
+hipError_t backward(DenoiserNet* net, hipStream_t s) {
+    DN_TRY(hipMemsetAsync(net->grads, 0, net->nParams * sizeof(float), s));
+    size_t total = 0;
+    for (int l = 0; l < kLayers; l++) {
+        const DnLayer& L = net->layers[l];
+        WgPlan p;
+        DN_TRY(wgPlan(net->n, net->h >> kLayerLevel[l], net->w >> kLayerLevel[l], L.cinPad, L.coutPad, p));
+        total += p.slots();
+    }
+    DN_TRY(ensureWgSlots(net, total));
+    net->wgPending.clear();
+    net->wgUsed = 0;
+    net->wgDefer = true;
+    const hipError_t e = backwardLayers(net, s);
+    net->wgDefer = false;
+    if (e != hipSuccess) return e;
+    if (net->wgPending.empty()) return hipSuccess;
+    if (net->wgPending.size() > (size_t)kMaxWgLayers) return hipErrorInvalidValue;
+    WgReduceArgs a{};
+    int first = 0;
+    for (size_t i = 0; i < net->wgPending.size(); i++) {
+        const DenoiserNet::WgPending& q = net->wgPending[i];
+        const int xBlocks = blocks((long long)q.pairs * kWgSlot), runs = (q.per + kWgReduceRun - 1) / kWgReduceRun;
+        a.layer[i] = WgReduceLayer{net->wgPart + q.off, q.dW, q.dB, q.per, q.pairs, q.nci, q.cinPad, q.coutPad, xBlocks, first};
+        first += xBlocks * runs;
+    }
+    a.count = (int)net->wgPending.size();
+    hipLaunchKernelGGL(wgReduceAllKernel, dim3(first), dim3(kB), 0, s, a);
     return hipGetLastError();
 }
 
