@@ -241,6 +241,7 @@ def run_psnr_study(s, cam, st, w, h, reference_spp, denoised_ladder=(8, 12, 16, 
     ref = td.render_mean(s, cam, st, reference_spp, seed=20261023)
     ref_s = time.perf_counter() - t0
     ref_img = td.display(ref, st.exposure)
+    ref_albedo = s.denoiser_inputs()[..., 3:6]
 
     def plain(spp, seed=42):
         lib.RtReset(s.handle)
@@ -277,6 +278,11 @@ def run_psnr_study(s, cam, st, w, h, reference_spp, denoised_ladder=(8, 12, 16, 
     sec, img, plain_img = denoised(4)
     tiles, rejected, infer_ms = s.denoise_stats()
     dn_psnr = td.psnr(img, ref_img)
+    # The ceiling of any denoiser that re-modulates its irradiance estimate with the frame's own
+    # 4-sample albedo (as this one does): the reference's exact irradiance times that albedo.
+    albedo4 = s.denoiser_inputs()[..., 3:6]
+    eps = 0.01   # the network's demodulation offset (denoiser.hip kDemodEps)
+    ceiling = td.psnr(td.display(ref / (ref_albedo + eps) * (albedo4 + eps), st.exposure), ref_img)
     log(f"denoised   4 spp: {sec * 1e3:8.1f} ms, PSNR {dn_psnr:.2f} dB")
     dn_ladder = []
     for spp in denoised_ladder:
@@ -302,6 +308,7 @@ def run_psnr_study(s, cam, st, w, h, reference_spp, denoised_ladder=(8, 12, 16, 
             "plain": ladder,
             "denoised_4spp": {"seconds": round(sec, 4), "psnr_db": round(dn_psnr, 3),
                               "plain_4spp_psnr_db": round(td.psnr(plain_img, ref_img), 3),
+                              "ceiling_with_4spp_albedo_db": round(ceiling, 3),
                               "tiles": tiles, "tiles_fallback": rejected, "network_ms": round(infer_ms, 3),
                               "network_gflop": round(flops / 1e9, 1),
                               "mfma_utilization": round(flops / (infer_ms / 1e3) / 2.5e15, 4)},
@@ -324,7 +331,8 @@ def psnr_summary(study):
             "reference_spp": study["reference_spp"],
             "plain_256spp": {"psnr_db": p256["psnr_db"], "seconds": p256["seconds"]},
             "denoised_4spp": {"psnr_db": d4["psnr_db"], "seconds": d4["seconds"], "network_ms": d4["network_ms"],
-                              "tiles_fallback": d4["tiles_fallback"], "tiles": d4["tiles"]},
+                              "tiles_fallback": d4["tiles_fallback"], "tiles": d4["tiles"],
+                              "ceiling_with_4spp_albedo_db": d4["ceiling_with_4spp_albedo_db"]},
             "denoised_4spp_gap_to_plain_256spp_db": study["vs_256spp"]["psnr_gap_db"],
             "bar_met_at_4spp": bool(study["vs_256spp"]["psnr_gap_db"] >= 0 and study["vs_256spp"]["time_ratio"] >= 10),
             "match_256spp": m,

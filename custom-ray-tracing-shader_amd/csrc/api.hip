@@ -144,7 +144,7 @@ struct Session {
     float* dDenoiseAuxIn = nullptr;    // [hp][wp][10] mirrored input (self-ensemble)
     float* dDenoiseAuxOut = nullptr;   // [hp][wp][4] its network output
     int denoiseEnsemble = 1;           // network passes per RtDenoise: 1, 2 (x mirror) or 4 (x, y, xy)
-    int* dTileStats = nullptr;         // [0] tiles, [1] rejected
+    int* dTileStats = nullptr;         // [kTileStatSlots] rejected-tile counters
     int lastTiles = 0, lastRejected = 0;
     float lastInferMs = 0;
 };
@@ -979,7 +979,7 @@ static bool denoiseBuffers(Session* s, int& hp, int& wp) {
     wp = (s->width + 15) / 16 * 16;
     if (!s->dDenoiseIn && !devAlloc(s, s->dDenoiseIn, (size_t)hp * wp * kDenoiseInputChannels)) return false;
     if (!s->dDenoiseOut && !devAlloc(s, s->dDenoiseOut, (size_t)hp * wp * 4)) return false;
-    if (!s->dTileStats && !devAlloc(s, s->dTileStats, 2)) return false;
+    if (!s->dTileStats && !devAlloc(s, s->dTileStats, kTileStatSlots)) return false;
     if (s->denoiseEnsemble > 1) {
         if (!s->dDenoiseAuxIn && !devAlloc(s, s->dDenoiseAuxIn, (size_t)hp * wp * kDenoiseInputChannels)) return false;
         if (!s->dDenoiseAuxOut && !devAlloc(s, s->dDenoiseAuxOut, (size_t)hp * wp * 4)) return false;
@@ -1022,12 +1022,14 @@ int RtDenoise(void* p, unsigned char* output, int outputBytes, float* millisecon
             return 0;
         s->lastInferMs += denoiserLastForwardMs(s->denoiser);
     }
-    if ((s->error = hipMemsetAsync(s->dTileStats, 0, 2 * sizeof(int), st)) != hipSuccess) return 0;
+    if ((s->error = hipMemsetAsync(s->dTileStats, 0, kTileStatSlots * sizeof(int), st)) != hipSuccess) return 0;
     if ((s->error = launchDenoisePresent(a, st)) != hipSuccess) return 0;
     if ((s->error = hipStreamSynchronize(st)) != hipSuccess) return 0;
-    int stats[2] = {0, 0};
+    int stats[kTileStatSlots];
     if ((s->error = hipMemcpy(stats, s->dTileStats, sizeof(stats), hipMemcpyDeviceToHost)) != hipSuccess) return 0;
-    s->lastTiles = stats[0]; s->lastRejected = stats[1];
+    s->lastTiles = ((s->width + 15) / 16) * ((s->height + 15) / 16);   // one workgroup per 16x16 tile
+    s->lastRejected = 0;
+    for (int k = 0; k < kTileStatSlots; k++) s->lastRejected += stats[k];
     s->error = hipMemcpy(output, s->dPixels, (size_t)s->width * s->height * 4, hipMemcpyDeviceToHost);
     if (s->error != hipSuccess) return 0;
     *milliseconds = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();

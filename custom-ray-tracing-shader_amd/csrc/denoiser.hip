@@ -141,24 +141,25 @@ __global__ __launch_bounds__(kB) void convFwdKernel(ConvArgs a) {
 }
 
 // The same convolution for images at least 16 pixels wide (or 16384 pixels), as a
-// persistent kernel: a block computes kTileH-row x 64-column output tiles of one group of
+// persistent kernel: a block computes ROWS-row x 64-column output tiles of one group of
 // 16 NT output channels (blockIdx.y; one row per wave), looping over tiles and 32-channel
 // input slices ("steps").
-// Each step's input slice plus its one-pixel halo ((kTileH + 2) x 66 pixels) and the weights of that
+// Each step's input slice plus its one-pixel halo ((ROWS + 2) x 66 pixels) and the weights of that
 // slice are staged in LDS and read there by all nine taps.  The next step's halo and weights
 // are loaded into registers while this step's MFMAs run and written to LDS after it, so the
 // loads' latency hides behind the MFMAs.  Loads are raw buffer loads whose out-of-image
 // offsets lie past the buffer's end, so the hardware returns the zero padding without
 // branches.  The MFMA takes the weights as its A operand, so a lane's results are 4
 // consecutive output channels of one pixel, stored as 8 bytes straight from registers.
+// Output rows per tile (one per wavefront of the block): 8, or 4 for layers with too few
+// 8-row tiles to keep several blocks on every compute unit (the 4-row block's smaller LDS
+// lets three share one, so their steps' load latencies overlap).
 constexpr int kTileW = 64, kHaloW = kTileW + 2;
-// Output rows per tile (one per wavefront of the block).
-#ifndef RT_CONV_ROWS
-#define RT_CONV_ROWS 8
-#endif
-constexpr int kTileH = RT_CONV_ROWS, kHaloH = kTileH + 2, kTileThreads = 64 * kTileH;
-constexpr int kHaloChunks = kHaloH * kHaloW * 4;                  // 16-byte chunks of one slice
-constexpr int kHaloIters = (kHaloChunks + kTileThreads - 1) / kTileThreads;
+template <int ROWS> struct TileShape {
+    static constexpr int kH = ROWS, kHaloH = ROWS + 2, kThreads = 64 * ROWS;
+    static constexpr int kHaloChunks = kHaloH * kHaloW * 4;   // 16-byte chunks of one slice
+    static constexpr int kHaloIters = (kHaloChunks + kThreads - 1) / kThreads;
+};
 constexpr unsigned kOutside = 0x7FFFFFF0u;                        // a buffer offset past any view
 
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t bufferOf(const void* base, unsigned bytes) {
@@ -219,8 +220,14 @@ __device__ __forceinline__ void tileEpilogue(const ConvArgs& a, const f32x4 (&ac
 #else
 #define RT_CONV_OCCUPANCY
 #endif
-template <int NT>
-__global__ __launch_bounds__(kTileThreads) RT_CONV_OCCUPANCY void convFwdTileKernel(ConvArgs a, int tilesX, int tilesY, int tileCount) {
+// Resident blocks per compute unit a tile kernel is built for: 4-row blocks of up to 32
+// output channels fit three (their LDS), 64-channel ones two; 8-row blocks one.
+template <int NT, int ROWS> constexpr int tileBlocksPerCu() { return ROWS == 4 ? (NT == 4 ? 2 : 3) : 1; }
+template <int NT, int ROWS>
+__global__ __launch_bounds__(64 * ROWS, (tileBlocksPerCu<NT, ROWS>())) RT_CONV_OCCUPANCY void convFwdTileKernel(ConvArgs a, int tilesX, int tilesY, int tileCount) {
+    using T = TileShape<ROWS>;
+    constexpr int kTileH = T::kH, kHaloH = T::kHaloH, kTileThreads = T::kThreads;
+    constexpr int kHaloChunks = T::kHaloChunks, kHaloIters = T::kHaloIters;
     constexpr int kCout = 16 * NT, kWChunks = 9 * kCout * 4, kWIters = (kWChunks + kTileThreads - 1) / kTileThreads;
     __shared__ __attribute__((aligned(16))) bf16 tile[kHaloH * kHaloW * 32];
     __shared__ __attribute__((aligned(16))) bf16 sW[9 * kCout * 32];
@@ -822,8 +829,15 @@ constexpr float kDemodEps = 0.01f;
 // Raw planes [p][10] = radiance, albedo, normal, depth -> BF16 features [p][32], the
 // demodulated log radiance the residual is added to, and the modulation factors.
 // Each pixel's outputs go out as whole 16-byte vectors.
-__device__ __forceinline__ float4 writeFeatures(const float* raw, bf16* x0, float* noisyLog, float* demod, size_t p) {
-    float f[kDenoiseInputChannels], m[3];
+__device__ __forceinline__ float4 writeFeatures(const float* rawp, bf16* x0, float* noisyLog, float* demod, size_t p) {
+    float raw[kDenoiseInputChannels], f[kDenoiseInputChannels], m[3];
+    const float2* r2 = reinterpret_cast<const float2*>(rawp);   // 40 bytes per pixel: five 8-byte loads
+#pragma unroll
+    for (int k = 0; k < kDenoiseInputChannels / 2; k++) {
+        const float2 v = r2[k];
+        raw[2 * k] = v.x;
+        raw[2 * k + 1] = v.y;
+    }
 #pragma unroll
     for (int c = 0; c < 3; c++) {
         m[c] = fmaxf(raw[3 + c], 0.0f) + kDemodEps;
@@ -1051,9 +1065,16 @@ bool tileConv(const ConvArgs& a) {
            (!a.low.p || (a.lowC % 32 == 0 && a.h % 2 == 0 && a.w % 2 == 0 && (a.low.cs | a.low.co) % 8 == 0));
 }
 
-// The persistent tile kernel for a convolution that tileConv accepts.
-hipError_t launchConvTile(const ConvArgs& a, hipStream_t s) {
-    const int tilesX = (a.w + kTileW - 1) / kTileW, tilesY = (a.h + kTileH - 1) / kTileH;
+// Work items (tiles x channel groups) per compute unit below which a layer runs 4-row tiles.
+#ifndef RT_CONV_SMALL
+#define RT_CONV_SMALL 16
+#endif
+
+// The persistent tile kernel for a convolution that tileConv accepts, with ROWS-row tiles.
+template <int ROWS>
+hipError_t launchConvTileRows(const ConvArgs& a, hipStream_t s) {
+    constexpr int threads = 64 * ROWS;
+    const int tilesX = (a.w + kTileW - 1) / kTileW, tilesY = (a.h + ROWS - 1) / ROWS;
     const int tileCount = tilesX * tilesY * a.n;
     // persistent: as many blocks as the GPU holds at once (fewer for small images)
     static int resident[3] = {0, 0, 0}, cus = 0;
@@ -1062,9 +1083,9 @@ hipError_t launchConvTile(const ConvArgs& a, hipStream_t s) {
         hipError_t e = hipGetDevice(&dev);
         if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
         if (e == hipSuccess)
-            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu2, convFwdTileKernel<2>, kTileThreads, 0);
+            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu2, convFwdTileKernel<2, ROWS>, threads, 0);
         if (e == hipSuccess)
-            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu4, convFwdTileKernel<4>, kTileThreads, 0);
+            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu4, convFwdTileKernel<4, ROWS>, threads, 0);
         if (e != hipSuccess) return e;
         resident[0] = std::max(1, cus * std::max(1, perCu2));
         resident[1] = std::max(1, cus * std::max(1, perCu4));
@@ -1074,11 +1095,11 @@ hipError_t launchConvTile(const ConvArgs& a, hipStream_t s) {
     if (!a.out.p && a.outF && a.outFStride <= 16) {
         if (!resident[2]) {
             int perCu1 = 0;
-            const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu1, convFwdTileKernel<1>, kTileThreads, 0);
+            const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu1, convFwdTileKernel<1, ROWS>, threads, 0);
             if (e != hipSuccess) return e;
             resident[2] = std::max(1, cus * std::max(1, perCu1));
         }
-        hipLaunchKernelGGL(convFwdTileKernel<1>, dim3(std::min(tileCount, resident[2]), 1), dim3(kTileThreads), 0, s, a,
+        hipLaunchKernelGGL((convFwdTileKernel<1, ROWS>), dim3(std::min(tileCount, resident[2]), 1), dim3(threads), 0, s, a,
                            tilesX, tilesY, tileCount);
         return hipGetLastError();
     }
@@ -1087,9 +1108,22 @@ hipError_t launchConvTile(const ConvArgs& a, hipStream_t s) {
     const bool wide = a.coutPad % 64 == 0 && (long long)tileCount * (a.coutPad / 64) >= 2LL * cus;
     const int groups = a.coutPad / (wide ? 64 : 32);
     const dim3 grid(std::min(tileCount, std::max(1, resident[wide ? 1 : 0] / groups)), groups);
-    if (wide) hipLaunchKernelGGL(convFwdTileKernel<4>, grid, dim3(kTileThreads), 0, s, a, tilesX, tilesY, tileCount);
-    else hipLaunchKernelGGL(convFwdTileKernel<2>, grid, dim3(kTileThreads), 0, s, a, tilesX, tilesY, tileCount);
+    if (wide) hipLaunchKernelGGL((convFwdTileKernel<4, ROWS>), grid, dim3(threads), 0, s, a, tilesX, tilesY, tileCount);
+    else hipLaunchKernelGGL((convFwdTileKernel<2, ROWS>), grid, dim3(threads), 0, s, a, tilesX, tilesY, tileCount);
     return hipGetLastError();
+}
+
+hipError_t launchConvTile(const ConvArgs& a, hipStream_t s) {
+    static int cus = 0;
+    if (!cus) {
+        int dev = 0;
+        hipError_t e = hipGetDevice(&dev);
+        if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        if (e != hipSuccess) return e;
+    }
+    const long long tiles8 = (long long)((a.w + kTileW - 1) / kTileW) * ((a.h + 7) / 8) * a.n;
+    const long long items8 = tiles8 * (a.coutPad / 32);
+    return items8 < (long long)RT_CONV_SMALL * cus ? launchConvTileRows<4>(a, s) : launchConvTileRows<8>(a, s);
 }
 
 hipError_t launchConvFwd(const ConvArgs& a, hipStream_t s) {

@@ -17,6 +17,7 @@ constexpr int kCntRays = 0, kCntShadow = kMaxBounces + 1, kWorkRays = 2 * (kMaxB
 // 4-wide full-precision nodes are addressed by 32-bit byte offsets in the traversal.
 constexpr long long kMaxWide4Nodes = 1LL << 25;
 constexpr int kXcds = 8, kXcdCursorStride = 32, kXcdCursorInts = kXcds * kXcdCursorStride;
+constexpr int kTileStatSlots = 64;   // counters of the denoiser's rejected tiles
 constexpr int kXcdSlotInts = (kMaxBounces + 1) * kXcdCursorInts;   // per pipeline
 
 // One traversal launch covers two queues: closest-hit rays, then any-hit rays (ray indices
@@ -82,7 +83,7 @@ struct DenoiseArgs {
     float* planes;                             // [hp][wp][10] network input
     const float* denoised;                     // [hp][wp][4] network output radiance
     uchar4* pixels;
-    int* tileStats;                            // [0] tiles, [1] tiles that fell back
+    int* tileStats;                            // [kTileStatSlots] counts of tiles that fell back
     float exposure;
     int fallback;                              // 0 per-tile check, 1 always fall back, 2 never
 };

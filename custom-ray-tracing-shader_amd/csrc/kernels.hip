@@ -1229,11 +1229,12 @@ __global__ __launch_bounds__(kBlock) void denoisePrepareKernel(DenoiseArgs a) {
     const float inv = 1.0f / a.samples;
     const float4 r = a.sumRad[src], al = a.sumAlb[src], n = a.sumNrm[src];
     const f3 nu = unitv(F3(n.x, n.y, n.z));
-    float* o = a.planes + (size_t)i * 10;
-    o[0] = r.x * inv; o[1] = r.y * inv; o[2] = r.z * inv;
-    o[3] = al.x * inv; o[4] = al.y * inv; o[5] = al.z * inv;
-    o[6] = nu.x; o[7] = nu.y; o[8] = nu.z;
-    o[9] = r.w > 0 ? n.w / r.w : 0.0f;
+    float2* o = reinterpret_cast<float2*>(a.planes + (size_t)i * 10);   // 40 bytes: five 8-byte stores
+    o[0] = make_float2(r.x * inv, r.y * inv);
+    o[1] = make_float2(r.z * inv, al.x * inv);
+    o[2] = make_float2(al.y * inv, al.z * inv);
+    o[3] = make_float2(nu.x, nu.y);
+    o[4] = make_float2(nu.z, r.w > 0 ? n.w / r.w : 0.0f);
 }
 __global__ void mirrorPlanesKernel(const float* in, float* out, int hp, int wp, int channels, int flip) {
     const long long i = blockIdx.x * (long long)kBlock + threadIdx.x;
@@ -1276,52 +1277,60 @@ hipError_t launchDenoisePrepare(const DenoiseArgs& a, hipStream_t s) {
 // within 4 standard errors (plus 10%) of the accumulated mean, else show plain
 // accumulation; then mask pixels with no primary hit and tone-map
 // (Native/renderer.cu:429-435).
+__device__ __forceinline__ float waveSum(float v) {
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+    return v;
+}
 __global__ __launch_bounds__(kBlock) void denoisePresentKernel(DenoiseArgs a) {
-    __shared__ float sNoisy[kBlock], sVar[kBlock], sDen[kBlock];
-    __shared__ int sBad[kBlock];
+    constexpr int kWaves = kBlock / 64;
+    __shared__ float sNoisy[kWaves], sVar[kWaves], sDen[kWaves];
+    __shared__ int sBad[kWaves];
     __shared__ int sReject;
     const int lx = threadIdx.x & 15, ly = threadIdx.x >> 4;
     const int x = blockIdx.x * 16 + lx, y = blockIdx.y * 16 + ly;
     const bool valid = x < a.w && y < a.h;
     const float inv = 1.0f / a.samples;
     f3 noisy = F3(0, 0, 0), den = F3(0, 0, 0);
-    float covered = 0;
+    float covered = 0, mu = 0, var = 0, dl = 0;
+    bool bad = false;
     if (valid) {
         const int i = y * a.w + x;
         const float4 r = a.sumRad[i], al = a.sumAlb[i];
         noisy = F3(r.x * inv, r.y * inv, r.z * inv);
         covered = r.w;
-        const float* d = a.denoised + ((size_t)y * a.wp + x) * 4;
-        den = F3(d[0], d[1], d[2]);
-        const float mu = 0.2126f * noisy.x + 0.7152f * noisy.y + 0.0722f * noisy.z;
-        sNoisy[threadIdx.x] = mu;
-        sVar[threadIdx.x] = fmaxf(al.w * inv - mu * mu, 0.0f) * inv;
-        sDen[threadIdx.x] = 0.2126f * den.x + 0.7152f * den.y + 0.0722f * den.z;
-        sBad[threadIdx.x] = !(isfinite(den.x) && isfinite(den.y) && isfinite(den.z));
-    } else {
-        sNoisy[threadIdx.x] = sVar[threadIdx.x] = sDen[threadIdx.x] = 0;
-        sBad[threadIdx.x] = 0;
+        const float4 d = *reinterpret_cast<const float4*>(a.denoised + ((size_t)y * a.wp + x) * 4);
+        den = F3(d.x, d.y, d.z);
+        mu = 0.2126f * noisy.x + 0.7152f * noisy.y + 0.0722f * noisy.z;
+        var = fmaxf(al.w * inv - mu * mu, 0.0f) * inv;
+        dl = 0.2126f * den.x + 0.7152f * den.y + 0.0722f * den.z;
+        bad = !(isfinite(den.x) && isfinite(den.y) && isfinite(den.z));
+    }
+    // Tile sums: within each wave by lane exchanges, then the four waves' sums in LDS.
+    const int wave = threadIdx.x >> 6;
+    mu = waveSum(mu);
+    var = waveSum(var);
+    dl = waveSum(dl);
+    const bool anyBad = __any(bad);
+    if ((threadIdx.x & 63) == 0) {
+        sNoisy[wave] = mu; sVar[wave] = var; sDen[wave] = dl; sBad[wave] = anyBad;
     }
     __syncthreads();
-    for (int st = kBlock / 2; st > 0; st >>= 1) {
-        if (threadIdx.x < st) {
-            sNoisy[threadIdx.x] += sNoisy[threadIdx.x + st];
-            sVar[threadIdx.x] += sVar[threadIdx.x + st];
-            sDen[threadIdx.x] += sDen[threadIdx.x + st];
-            sBad[threadIdx.x] |= sBad[threadIdx.x + st];
-        }
-        __syncthreads();
-    }
     if (threadIdx.x == 0) {
+        float tn = 0, tv = 0, td = 0;
+        int tb = 0;
+#pragma unroll
+        for (int k = 0; k < kWaves; k++) { tn += sNoisy[k]; tv += sVar[k]; td += sDen[k]; tb |= sBad[k]; }
         const int nx = min(16, a.w - blockIdx.x * 16), ny = min(16, a.h - blockIdx.y * 16);
         const float cnt = (float)(nx * ny);
-        const float mn = sNoisy[0] / cnt, md = sDen[0] / cnt, se = sqrtf(sVar[0]) / cnt;
+        const float mn = tn / cnt, md = td / cnt, se = sqrtf(tv) / cnt;
         int reject = 0;
         if (a.fallback == 1) reject = 1;
-        else if (a.fallback == 0) reject = sBad[0] || fabsf(md - mn) > 4.0f * se + 0.1f * mn + 1e-4f;
+        else if (a.fallback == 0) reject = tb || fabsf(md - mn) > 4.0f * se + 0.1f * mn + 1e-4f;
         sReject = reject;
-        atomicAdd(&a.tileStats[0], 1);
-        if (reject) atomicAdd(&a.tileStats[1], 1);
+        // rejected tiles, spread over kTileStatSlots counters (one address for every tile
+        // would serialise the atomics of thousands of workgroups)
+        if (reject) atomicAdd(&a.tileStats[(blockIdx.y * gridDim.x + blockIdx.x) % kTileStatSlots], 1);
     }
     __syncthreads();
     if (!valid) return;
