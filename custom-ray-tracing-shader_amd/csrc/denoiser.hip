@@ -450,8 +450,9 @@ __global__ __launch_bounds__(kB) void convBwdWKernel(BwdWArgs a) {
 // input row once per column shift and uses it for the (up to) three dy rows that reach it,
 // so one dy row's operands serve all nine taps.  The bias gradient is one more MFMA per dy row
 // against ones.  At the end the 4 waves' sums are added through LDS and the block writes them
-// to its slot of `part`; wgReduceKernel sums the slots in runs and adds each run's sum to the
-// gradient with one atomic.
+// to its slot of `part`; wgReduceKernel (one layer) or wgReduceAllKernel (every layer of a
+// backward pass) sums the slots in runs and adds each run's sum to the gradient with one
+// atomic.
 constexpr int kWgThreads = 256, kWgSlot = 9 * 32 * 32 + 32;   // floats per block slot: weights + bias
 
 struct BwdWTileArgs {
