@@ -69,6 +69,10 @@ struct ConvArgs {
     // `in` holds channels lowC.. (in channel ci - lowC of the view).  Tile kernel only.
     View low;
     int lowC;
+    // Optional: the output is the input gradient of a ReLU layer's output; it is kept where
+    // `omask` (that output: the same pixels and channels as `out`) is positive and zero
+    // elsewhere, so the layer's backward reads it as its pre-activation gradient.
+    View omask;
 };
 
 template <int NT>
@@ -134,6 +138,7 @@ __global__ __launch_bounds__(kB) void convFwdKernel(ConvArgs a) {
                 if (a.out.p && c < a.out.c) {
                     bf16* dst = a.out.p + (size_t)p * a.out.cs + a.out.co + c;
                     if (a.accumulate) v += (float)*dst;
+                    if (a.omask.p && !((float)a.omask.p[(size_t)p * a.omask.cs + a.omask.co + c] > 0.0f)) v = 0.0f;
                     *dst = (bf16)v;
                 }
                 if (a.outF && c < a.outFStride) a.outF[(size_t)p * a.outFStride + c] = v;
@@ -196,6 +201,12 @@ __device__ __forceinline__ void tileEpilogue(const ConvArgs& a, const f32x4 (&ac
                     const bf16x4 old = *dst;
 #pragma unroll
                     for (int i = 0; i < 4; i++) v[i] += (float)old[i];
+                }
+                if (a.omask.p) {
+                    const bf16x4 m = *reinterpret_cast<const bf16x4*>(a.omask.p + p * a.omask.cs + a.omask.co + c0);
+#pragma unroll
+                    for (int i = 0; i < 4; i++)
+                        if (!((float)m[i] > 0.0f)) v[i] = 0.0f;
                 }
 #pragma unroll
                 for (int i = 0; i < 4; i++) o[i] = (bf16)v[i];
@@ -718,21 +729,6 @@ __device__ __forceinline__ void store8(const View& v, size_t pixel, int c, bf16x
     *reinterpret_cast<bf16x8*>(v.p + pixel * v.cs + v.co + c) = x;
 }
 
-// The ReLU's backward: the gradient where the layer's output is positive, else zero, into a
-// dense [pixel][channels] buffer; 8 channels per thread.
-__global__ void reluMaskKernel(View g, View act, bf16* out, int npix, int channels) {
-    const int groups = channels / 8;
-    const long long i = blockIdx.x * (long long)kB + threadIdx.x;
-    if (i >= (long long)npix * groups) return;
-    const size_t p = (size_t)(i / groups);
-    const int c = 8 * (int)(i % groups);
-    const bf16x8 a = load8(act, p, c), v = load8(g, p, c);
-    bf16x8 o;
-#pragma unroll
-    for (int e = 0; e < 8; e++) o[e] = (float)a[e] > 0.0f ? v[e] : (bf16)0.0f;
-    *reinterpret_cast<bf16x8*>(out + p * channels + c) = o;
-}
-
 // dBias[c] += sum over a part of the pixels of dy[p][c] (channels a multiple of 8, at most
 // 8 kB): each thread sums 8 channels over a strided share of the part's pixels, the block's
 // threads are combined in LDS, and each block adds once per channel.
@@ -780,7 +776,9 @@ __global__ void avgPoolKernel(View in, View out, int n, int h, int w, int channe
 }
 
 // Adds the pooling gradient to the input-side gradient view.
-__global__ void avgPoolBackKernel(View gout, View gin, int n, int h, int w, int channels) {
+// Adds the pooling gradient to the input-side gradient view where the pooled layer's output
+// (`mask`: a ReLU output, the same pixels and channels as gin) is positive.
+__global__ void avgPoolBackKernel(View gout, View gin, View mask, int n, int h, int w, int channels) {
     const int groups = channels / 8;
     const long long i = blockIdx.x * (long long)kB + threadIdx.x;
     if (i >= (long long)n * h * w * groups) return;
@@ -788,10 +786,10 @@ __global__ void avgPoolBackKernel(View gout, View gin, int n, int h, int w, int 
     const long long p = i / groups;
     const int x = (int)(p % w), y = (int)((p / w) % h), nn = (int)(p / ((long long)w * h));
     const size_t po = ((size_t)nn * (h / 2) + y / 2) * (w / 2) + x / 2;
-    const bf16x8 g = load8(gout, po, c);
+    const bf16x8 g = load8(gout, po, c), m = load8(mask, (size_t)p, c);
     bf16x8 d = load8(gin, (size_t)p, c);
 #pragma unroll
-    for (int e = 0; e < 8; e++) d[e] = (bf16)((float)d[e] + 0.25f * (float)g[e]);
+    for (int e = 0; e < 8; e++) d[e] = (float)m[e] > 0.0f ? (bf16)((float)d[e] + 0.25f * (float)g[e]) : d[e];
     store8(gin, (size_t)p, c, d);
 }
 
@@ -1011,7 +1009,7 @@ struct DenoiserNet {
     bf16 *x0 = nullptr, *e0a, *D0, *p0, *e1a, *D1, *p1, *e2a, *D2, *p2, *e3a, *D3, *p3, *ba, *bb, *d3a, *d3b, *d2a, *d2b,
          *d1a, *d1b, *d0a, *d0b;
     bf16 *gD0, *gD1, *gD2, *gD3, *ge0a, *gp0, *ge1a, *gp1, *ge2a, *gp2, *ge3a, *gp3, *gba, *gbb, *gd3a, *gd3b, *gd2a,
-         *gd2b, *gd1a, *gd1b, *gd0a, *gd0b, *gout, *dpre;
+         *gd2b, *gd1a, *gd1b, *gd0a, *gd0b, *gout;
     float *outF = nullptr, *noisyLog = nullptr, *demod = nullptr, *targetLog = nullptr, *loss = nullptr;
     // dataset
     float *dsIn = nullptr, *dsTgt = nullptr;
@@ -1063,7 +1061,8 @@ bool tileConv(const ConvArgs& a) {
            (long long)a.n * a.h * a.w * a.in.cs * 2 < (long long)kOutside &&
            (long long)9 * a.coutPad * a.cinPad * 2 < kOutside && (a.in.cs | a.in.co) % 8 == 0 &&
            (!a.out.p || (a.out.cs | a.out.co) % 4 == 0) &&
-           (!a.low.p || (a.lowC % 32 == 0 && a.h % 2 == 0 && a.w % 2 == 0 && (a.low.cs | a.low.co) % 8 == 0));
+           (!a.low.p || (a.lowC % 32 == 0 && a.h % 2 == 0 && a.w % 2 == 0 && (a.low.cs | a.low.co) % 8 == 0)) &&
+           (!a.omask.p || (a.omask.cs | a.omask.co) % 4 == 0);
 }
 
 // Work items (tiles x channel groups) per compute unit below which a layer runs 4-row tiles.
@@ -1263,7 +1262,7 @@ hipError_t ensureActivations(DenoiserNet* net, int n, int h, int w, bool trainin
         alloc(net->ge2a, 2, 64); alloc(net->gp2, 3, 64); alloc(net->ge3a, 3, 96); alloc(net->gp3, 4, 96);
         alloc(net->gba, 4, 128); alloc(net->gbb, 4, 128); alloc(net->gd3a, 3, 96); alloc(net->gd3b, 3, 96);
         alloc(net->gd2a, 2, 64); alloc(net->gd2b, 2, 64); alloc(net->gd1a, 1, 64); alloc(net->gd1b, 1, 64);
-        alloc(net->gd0a, 0, 32); alloc(net->gd0b, 0, 32); alloc(net->dpre, 0, 32);
+        alloc(net->gd0a, 0, 32); alloc(net->gd0b, 0, 32);
         bf16* go = nullptr;
         alloc(go, 0, 32);
         if (err == hipSuccess) net->gout = go;
@@ -1356,23 +1355,20 @@ hipError_t forward(DenoiserNet* net, hipStream_t s, bool fuseUp = false) {
 }
 
 // Backward through one convolution: weight/bias gradients and (if gin.p) the input
-// gradient, set or accumulated.  g is the gradient w.r.t. the layer's (post-ReLU) output.
-hipError_t convBack(DenoiserNet* net, int l, const View& g, const View& act, const View& in, const View& gin,
+// gradient, set or accumulated.  g is the gradient w.r.t. the layer's pre-activation output:
+// every producer of a ReLU layer's output gradient zeroes it where that output is not
+// positive (the input-gradient convolution's `omask`, the pooling backward's mask; nearest
+// upsampling's backward sums pixels that share one mask value).  gmask: the ReLU output under
+// gin, or none when gin is a pooling output's gradient (no activation).
+hipError_t convBack(DenoiserNet* net, int l, const View& g, const View& in, const View& gin, const View& gmask,
                     bool accumulate, int n, int h, int w, hipStream_t s) {
     const DnLayer& L = net->layers[l];
-    const int npix = n * h * w;
-    View dy = g;
-    if (L.relu) {
-        hipLaunchKernelGGL(reluMaskKernel, dim3(blocks((long long)npix * L.coutPad / 8)), dim3(kB), 0, s, g, act, net->dpre,
-                           npix, L.coutPad);
-        DN_TRY(hipGetLastError());
-        dy = V(net->dpre, L.coutPad);
-    }
-    DN_TRY(convWeightGrad(net, dy, in, n, h, w, L.cinPad, L.coutPad, net->grads + L.wOff, net->grads + L.bOff, s));
+    DN_TRY(convWeightGrad(net, g, in, n, h, w, L.cinPad, L.coutPad, net->grads + L.wOff, net->grads + L.bOff, s));
     if (gin.p) {
         ConvArgs a{};
-        a.in = dy; a.wt = wbOf(net, l); a.bias = nullptr; a.out = gin; a.outF = nullptr; a.outFStride = 0;
+        a.in = g; a.wt = wbOf(net, l); a.bias = nullptr; a.out = gin; a.outF = nullptr; a.outFStride = 0;
         a.n = n; a.h = h; a.w = w; a.cinPad = L.coutPad; a.coutPad = L.cinPad; a.relu = 0; a.accumulate = accumulate;
+        a.omask = gmask;
         DN_TRY(launchConvFwd(a, s));
     }
     return hipSuccess;
@@ -1382,41 +1378,42 @@ hipError_t backwardLayers(DenoiserNet* net, hipStream_t s) {
     const int n = net->n, h0 = net->h, w0 = net->w;
     const int h1 = h0 / 2, w1 = w0 / 2, h2 = h0 / 4, w2 = w0 / 4, h3 = h0 / 8, w3 = w0 / 8, h4 = h0 / 16, w4 = w0 / 16;
     const View none{nullptr, 0, 0, 0};
-    DN_TRY(convBack(net, OUT, V(net->gout, 32), none, V(net->d0b, 32), V(net->gd0b, 32), false, n, h0, w0, s));
-    DN_TRY(convBack(net, D0B, V(net->gd0b, 32), V(net->d0b, 32), V(net->d0a, 32), V(net->gd0a, 32), false, n, h0, w0, s));
-    DN_TRY(convBack(net, D0A, V(net->gd0a, 32), V(net->d0a, 32), V(net->D0, 96), V(net->gD0, 96), false, n, h0, w0, s));
+    // convBack(layer, output gradient, input, input gradient, ReLU output under it, ...)
+    DN_TRY(convBack(net, OUT, V(net->gout, 32), V(net->d0b, 32), V(net->gd0b, 32), V(net->d0b, 32), false, n, h0, w0, s));
+    DN_TRY(convBack(net, D0B, V(net->gd0b, 32), V(net->d0a, 32), V(net->gd0a, 32), V(net->d0a, 32), false, n, h0, w0, s));
+    DN_TRY(convBack(net, D0A, V(net->gd0a, 32), V(net->D0, 96), V(net->gD0, 96), V(net->D0, 96), false, n, h0, w0, s));
     hipLaunchKernelGGL(upsampleBackKernel, dim3(blocks((long long)n * h1 * w1 * 8)), dim3(kB), 0, s, V(net->gD0, 96, 0, 64),
                        V(net->gd1b, 64), n, h1, w1, 64);
-    DN_TRY(convBack(net, D1B, V(net->gd1b, 64), V(net->d1b, 64), V(net->d1a, 64), V(net->gd1a, 64), false, n, h1, w1, s));
-    DN_TRY(convBack(net, D1A, V(net->gd1a, 64), V(net->d1a, 64), V(net->D1, 128), V(net->gD1, 128), false, n, h1, w1, s));
+    DN_TRY(convBack(net, D1B, V(net->gd1b, 64), V(net->d1a, 64), V(net->gd1a, 64), V(net->d1a, 64), false, n, h1, w1, s));
+    DN_TRY(convBack(net, D1A, V(net->gd1a, 64), V(net->D1, 128), V(net->gD1, 128), V(net->D1, 128), false, n, h1, w1, s));
     hipLaunchKernelGGL(upsampleBackKernel, dim3(blocks((long long)n * h2 * w2 * 8)), dim3(kB), 0, s, V(net->gD1, 128, 0, 64),
                        V(net->gd2b, 64), n, h2, w2, 64);
-    DN_TRY(convBack(net, D2B, V(net->gd2b, 64), V(net->d2b, 64), V(net->d2a, 64), V(net->gd2a, 64), false, n, h2, w2, s));
-    DN_TRY(convBack(net, D2A, V(net->gd2a, 64), V(net->d2a, 64), V(net->D2, 160), V(net->gD2, 160), false, n, h2, w2, s));
+    DN_TRY(convBack(net, D2B, V(net->gd2b, 64), V(net->d2a, 64), V(net->gd2a, 64), V(net->d2a, 64), false, n, h2, w2, s));
+    DN_TRY(convBack(net, D2A, V(net->gd2a, 64), V(net->D2, 160), V(net->gD2, 160), V(net->D2, 160), false, n, h2, w2, s));
     hipLaunchKernelGGL(upsampleBackKernel, dim3(blocks((long long)n * h3 * w3 * 12)), dim3(kB), 0, s, V(net->gD2, 160, 0, 96),
                        V(net->gd3b, 96), n, h3, w3, 96);
-    DN_TRY(convBack(net, D3B, V(net->gd3b, 96), V(net->d3b, 96), V(net->d3a, 96), V(net->gd3a, 96), false, n, h3, w3, s));
-    DN_TRY(convBack(net, D3A, V(net->gd3a, 96), V(net->d3a, 96), V(net->D3, 224), V(net->gD3, 224), false, n, h3, w3, s));
+    DN_TRY(convBack(net, D3B, V(net->gd3b, 96), V(net->d3a, 96), V(net->gd3a, 96), V(net->d3a, 96), false, n, h3, w3, s));
+    DN_TRY(convBack(net, D3A, V(net->gd3a, 96), V(net->D3, 224), V(net->gD3, 224), V(net->D3, 224), false, n, h3, w3, s));
     hipLaunchKernelGGL(upsampleBackKernel, dim3(blocks((long long)n * h4 * w4 * 16)), dim3(kB), 0, s, V(net->gD3, 224, 0, 128),
                        V(net->gbb, 128), n, h4, w4, 128);
-    DN_TRY(convBack(net, BB, V(net->gbb, 128), V(net->bb, 128), V(net->ba, 128), V(net->gba, 128), false, n, h4, w4, s));
-    DN_TRY(convBack(net, BA, V(net->gba, 128), V(net->ba, 128), V(net->p3, 96), V(net->gp3, 96), false, n, h4, w4, s));
+    DN_TRY(convBack(net, BB, V(net->gbb, 128), V(net->ba, 128), V(net->gba, 128), V(net->ba, 128), false, n, h4, w4, s));
+    DN_TRY(convBack(net, BA, V(net->gba, 128), V(net->p3, 96), V(net->gp3, 96), none, false, n, h4, w4, s));
     hipLaunchKernelGGL(avgPoolBackKernel, dim3(blocks((long long)n * h3 * w3 * 12)), dim3(kB), 0, s, V(net->gp3, 96),
-                       V(net->gD3, 224, 128, 96), n, h3, w3, 96);
-    DN_TRY(convBack(net, E3B, V(net->gD3, 224, 128, 96), V(net->D3, 224, 128, 96), V(net->e3a, 96), V(net->ge3a, 96), false, n, h3, w3, s));
-    DN_TRY(convBack(net, E3A, V(net->ge3a, 96), V(net->e3a, 96), V(net->p2, 64), V(net->gp2, 64), false, n, h3, w3, s));
+                       V(net->gD3, 224, 128, 96), V(net->D3, 224, 128, 96), n, h3, w3, 96);
+    DN_TRY(convBack(net, E3B, V(net->gD3, 224, 128, 96), V(net->e3a, 96), V(net->ge3a, 96), V(net->e3a, 96), false, n, h3, w3, s));
+    DN_TRY(convBack(net, E3A, V(net->ge3a, 96), V(net->p2, 64), V(net->gp2, 64), none, false, n, h3, w3, s));
     hipLaunchKernelGGL(avgPoolBackKernel, dim3(blocks((long long)n * h2 * w2 * 8)), dim3(kB), 0, s, V(net->gp2, 64),
-                       V(net->gD2, 160, 96, 64), n, h2, w2, 64);
-    DN_TRY(convBack(net, E2B, V(net->gD2, 160, 96, 64), V(net->D2, 160, 96, 64), V(net->e2a, 64), V(net->ge2a, 64), false, n, h2, w2, s));
-    DN_TRY(convBack(net, E2A, V(net->ge2a, 64), V(net->e2a, 64), V(net->p1, 64), V(net->gp1, 64), false, n, h2, w2, s));
+                       V(net->gD2, 160, 96, 64), V(net->D2, 160, 96, 64), n, h2, w2, 64);
+    DN_TRY(convBack(net, E2B, V(net->gD2, 160, 96, 64), V(net->e2a, 64), V(net->ge2a, 64), V(net->e2a, 64), false, n, h2, w2, s));
+    DN_TRY(convBack(net, E2A, V(net->ge2a, 64), V(net->p1, 64), V(net->gp1, 64), none, false, n, h2, w2, s));
     hipLaunchKernelGGL(avgPoolBackKernel, dim3(blocks((long long)n * h1 * w1 * 8)), dim3(kB), 0, s, V(net->gp1, 64),
-                       V(net->gD1, 128, 64, 64), n, h1, w1, 64);
-    DN_TRY(convBack(net, E1B, V(net->gD1, 128, 64, 64), V(net->D1, 128, 64, 64), V(net->e1a, 64), V(net->ge1a, 64), false, n, h1, w1, s));
-    DN_TRY(convBack(net, E1A, V(net->ge1a, 64), V(net->e1a, 64), V(net->p0, 32), V(net->gp0, 32), false, n, h1, w1, s));
+                       V(net->gD1, 128, 64, 64), V(net->D1, 128, 64, 64), n, h1, w1, 64);
+    DN_TRY(convBack(net, E1B, V(net->gD1, 128, 64, 64), V(net->e1a, 64), V(net->ge1a, 64), V(net->e1a, 64), false, n, h1, w1, s));
+    DN_TRY(convBack(net, E1A, V(net->ge1a, 64), V(net->p0, 32), V(net->gp0, 32), none, false, n, h1, w1, s));
     hipLaunchKernelGGL(avgPoolBackKernel, dim3(blocks((long long)n * h0 * w0 * 4)), dim3(kB), 0, s, V(net->gp0, 32),
-                       V(net->gD0, 96, 64, 32), n, h0, w0, 32);
-    DN_TRY(convBack(net, E0B, V(net->gD0, 96, 64, 32), V(net->D0, 96, 64, 32), V(net->e0a, 32), V(net->ge0a, 32), false, n, h0, w0, s));
-    DN_TRY(convBack(net, E0A, V(net->ge0a, 32), V(net->e0a, 32), V(net->x0, 32), none, false, n, h0, w0, s));
+                       V(net->gD0, 96, 64, 32), V(net->D0, 96, 64, 32), n, h0, w0, 32);
+    DN_TRY(convBack(net, E0B, V(net->gD0, 96, 64, 32), V(net->e0a, 32), V(net->ge0a, 32), V(net->e0a, 32), false, n, h0, w0, s));
+    DN_TRY(convBack(net, E0A, V(net->ge0a, 32), V(net->x0, 32), none, none, false, n, h0, w0, s));
     return hipGetLastError();
 }
 
