@@ -63,7 +63,7 @@ struct ConvArgs {
     int outFStride;
     int n, h, w;
     int cinPad, coutPad;
-    int relu, accumulate;
+    int relu;
     // Optional: input channels [0, lowC) are the nearest-neighbour upsampling of `low`, a
     // (h / 2) x (w / 2) buffer (read in place, so the upsampled tensor is never written), and
     // `in` holds channels lowC.. (in channel ci - lowC of the view).  Tile kernel only.
@@ -137,7 +137,6 @@ __global__ __launch_bounds__(kB) void convFwdKernel(ConvArgs a) {
                 if (a.relu) v = fmaxf(v, 0.0f);
                 if (a.out.p && c < a.out.c) {
                     bf16* dst = a.out.p + (size_t)p * a.out.cs + a.out.co + c;
-                    if (a.accumulate) v += (float)*dst;
                     if (a.omask.p && !((float)a.omask.p[(size_t)p * a.omask.cs + a.omask.co + c] > 0.0f)) v = 0.0f;
                     *dst = (bf16)v;
                 }
@@ -149,16 +148,17 @@ __global__ __launch_bounds__(kB) void convFwdKernel(ConvArgs a) {
 // persistent kernel: a block computes ROWS-row x 64-column output tiles of one group of
 // 16 NT output channels (blockIdx.y; one row per wave), looping over tiles and 32-channel
 // input slices ("steps").
-// Each step's input slice plus its one-pixel halo ((ROWS + 2) x 66 pixels) and the weights of that
-// slice are staged in LDS and read there by all nine taps.  The next step's halo and weights
-// are loaded into registers while this step's MFMAs run and written to LDS after it, so the
-// loads' latency hides behind the MFMAs.  Loads are raw buffer loads whose out-of-image
-// offsets lie past the buffer's end, so the hardware returns the zero padding without
-// branches.  The MFMA takes the weights as its A operand, so a lane's results are 4
+// Each step's input slice plus its one-pixel halo ((ROWS + 2) x 66 pixels) is staged in LDS
+// with the weights of that slice (for a layer of one slice, once before the first step) and
+// read there by all nine taps.  The next step's halo (and weights) are loaded into registers
+// while this step's MFMAs run and written to LDS after it, so the loads' latency hides
+// behind the MFMAs.  Loads and stores are raw buffer accesses whose out-of-image offsets lie
+// past the buffer's end, so the hardware returns the zero padding and drops the stores
+// without branches.  The MFMA takes the weights as its A operand, so a lane's results are 4
 // consecutive output channels of one pixel, stored as 8 bytes straight from registers.
 // Output rows per tile (one per wavefront of the block): 8, or 4 for layers with too few
-// 8-row tiles to keep several blocks on every compute unit (the 4-row block's smaller LDS
-// lets three share one, so their steps' load latencies overlap).
+// 8-row tiles to give every compute unit a block (the 4-row block's smaller LDS lets two or
+// three share one).
 constexpr int kTileW = 64, kHaloW = kTileW + 2;
 template <int ROWS> struct TileShape {
     static constexpr int kH = ROWS, kHaloH = ROWS + 2, kThreads = 64 * ROWS;
@@ -167,6 +167,14 @@ template <int ROWS> struct TileShape {
 };
 constexpr unsigned kOutside = 0x7FFFFFF0u;                        // a buffer offset past any view
 
+// LDS position (in bf16) of 16-byte chunk c of staged row `row` (a halo pixel, or one output
+// channel's weights): 64 bytes per row, the chunk XOR-swizzled by bits 1-2 of `key`, which
+// is the row's position modulo 8 in 128-byte LDS lines (the halo column; a halo row is 66
+// pixels, 33 lines, so the halo row does not change it).  A wave's 16-byte reads then hit
+// every bank once per eight lanes (eight consecutive pixels or channels), where the plain
+// layout put two lanes on each bank.
+__device__ __forceinline__ int swz(int row, int c, int key) { return row * 32 + 8 * (c ^ ((key >> 1) & 3)); }
+
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t bufferOf(const void* base, unsigned bytes) {
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes, 0x00020000);
 }
@@ -174,49 +182,58 @@ __device__ __forceinline__ bf16x8 bufferLoad16(__amdgpu_buffer_rsrc_t r, unsigne
     return __builtin_bit_cast(bf16x8, __builtin_amdgcn_raw_buffer_load_b128(r, (int)offset, 0, 0));
 }
 
+typedef bf16 bf16x4 __attribute__((ext_vector_type(4)));
+typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+
+// Buffer resources of a tile kernel's outputs (raw buffer stores drop offsets past the end,
+// which is how lanes outside the image skip their stores without branches).
+struct TileOut {
+    __amdgpu_buffer_rsrc_t out, mask, outF;
+};
+
 // Bias, ReLU and the stores of a finished tile row: lane (g, r) holds output channels
-// cBase + 16 j + 4 g .. + 3 of pixel 16 q + r of its wave's row (biasOf[j]: their bias).
+// cBase + 16 j + 4 g .. + 3 of pixel 16 q + r of its wave's row (sBias: the block's biases).
+// Each lane's byte offsets are formed once per row; pixel group q adds a fixed stride and
+// channel group j a constant.
 template <int NT>
-__device__ __forceinline__ void tileEpilogue(const ConvArgs& a, const f32x4 (&acc)[4][NT], const float4 (&biasOf)[NT],
-                                             int cBase, int x0, int y, int nn, int g, int r) {
+__device__ __forceinline__ void tileEpilogue(const ConvArgs& a, const TileOut& o, const f32x4 (&acc)[4][NT],
+                                             const float* sBias, int cBase, int x0, int y, int nn, int g, int r) {
     if (y >= a.h) return;
+    const int pix = (nn * a.h + y) * a.w + x0 + r;   // this lane's pixel in group q = 0
+    const int left = a.w - x0 - r;                    // group q holds an image pixel while 16 q < left
+    const int c0 = cBase + 4 * g;                     // this lane's first channel in group j = 0
+    const unsigned outBase = (unsigned)(pix * a.out.cs + a.out.co + c0) * 2;
+    const unsigned maskBase = (unsigned)(pix * a.omask.cs + a.omask.co + c0) * 2;
+    const unsigned fBase = (unsigned)(pix * a.outFStride + c0) * 4;
 #pragma unroll
-    for (int j = 0; j < NT; j++) {
-        const int c0 = cBase + 16 * j + 4 * g;
-        const float4 bias = biasOf[j];
+    for (int q = 0; q < 4; q++) {
+        const bool in = 16 * q < left;
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-            const int x = x0 + 16 * q + r;
-            if (x >= a.w) continue;
-            const size_t p = (size_t)(nn * a.h + y) * a.w + x;
+        for (int j = 0; j < NT; j++) {
+            const unsigned outOff = in && c0 + 16 * j < a.out.c ? outBase + (unsigned)(32 * q * a.out.cs + 32 * j) : kOutside;
+            const unsigned maskOff = maskBase + (unsigned)(32 * q * a.omask.cs + 32 * j);
+            const unsigned fOff = in && c0 + 16 * j < a.outFStride ? fBase + (unsigned)(64 * q * a.outFStride + 64 * j) : kOutside;
+            const float4 bias = *reinterpret_cast<const float4*>(sBias + 16 * j + 4 * g);
             float v[4] = {acc[q][j][0] + bias.x, acc[q][j][1] + bias.y, acc[q][j][2] + bias.z, acc[q][j][3] + bias.w};
             if (a.relu)
 #pragma unroll
                 for (int i = 0; i < 4; i++) v[i] = fmaxf(v[i], 0.0f);
-            if (a.out.p && c0 < a.out.c) {
-                typedef bf16 bf16x4 __attribute__((ext_vector_type(4)));
-                bf16x4* dst = reinterpret_cast<bf16x4*>(a.out.p + p * a.out.cs + a.out.co + c0);
-                bf16x4 o;
-                if (a.accumulate) {
-                    const bf16x4 old = *dst;
-#pragma unroll
-                    for (int i = 0; i < 4; i++) v[i] += (float)old[i];
-                }
+            if (a.out.p) {
                 if (a.omask.p) {
-                    const bf16x4 m = *reinterpret_cast<const bf16x4*>(a.omask.p + p * a.omask.cs + a.omask.co + c0);
+                    const bf16x4 m = __builtin_bit_cast(bf16x4, __builtin_amdgcn_raw_buffer_load_b64(o.mask, (int)maskOff, 0, 0));
 #pragma unroll
                     for (int i = 0; i < 4; i++)
                         if (!((float)m[i] > 0.0f)) v[i] = 0.0f;
                 }
+                bf16x4 h;
 #pragma unroll
-                for (int i = 0; i < 4; i++) o[i] = (bf16)v[i];
-                *dst = o;
+                for (int i = 0; i < 4; i++) h[i] = (bf16)v[i];
+                __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, h), o.out, (int)outOff, 0, 0);
             }
-            if (a.outF && c0 < a.outFStride) {
-#pragma unroll
-                for (int i = 0; i < 4; i++)
-                    if (c0 + i < a.outFStride) a.outF[p * a.outFStride + c0 + i] = v[i];
-            }
+            if (a.outF)
+                __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, f32x4{v[0], v[1], v[2], v[3]}), o.outF,
+                                                       (int)fOff, 0, 0);
         }
     }
 }
@@ -233,35 +250,68 @@ __device__ __forceinline__ void tileEpilogue(const ConvArgs& a, const f32x4 (&ac
 #endif
 // Resident blocks per compute unit a tile kernel is built for: 4-row blocks of up to 32
 // output channels fit three (their LDS), 64-channel ones two; 8-row blocks one.
-template <int NT, int ROWS> constexpr int tileBlocksPerCu() { return ROWS == 4 ? (NT == 4 ? 2 : 3) : 1; }
-template <int NT, int ROWS>
-__global__ __launch_bounds__(64 * ROWS, (tileBlocksPerCu<NT, ROWS>())) RT_CONV_OCCUPANCY void convFwdTileKernel(ConvArgs a, int tilesX, int tilesY, int tileCount) {
+// (The multi-slice 32-channel kernel holds a slice of weights in registers too: two blocks.)
+template <int NT, int ROWS, bool MULTI> constexpr int tileBlocksPerCu() {
+    return ROWS == 4 ? (NT == 4 || (NT == 2 && MULTI) ? 2 : 3) : 1;
+}
+// MULTI: the layer has several 32-channel input slices (each step stages a slice's weights as
+// well); without it the weights are staged once, before the first step.
+template <int NT, int ROWS, bool MULTI>
+__global__ __launch_bounds__(64 * ROWS, (tileBlocksPerCu<NT, ROWS, MULTI>())) RT_CONV_OCCUPANCY void convFwdTileKernel(ConvArgs a, int tilesX, int tilesY, int tileCount) {
     using T = TileShape<ROWS>;
     constexpr int kTileH = T::kH, kHaloH = T::kHaloH, kTileThreads = T::kThreads;
     constexpr int kHaloChunks = T::kHaloChunks, kHaloIters = T::kHaloIters;
     constexpr int kCout = 16 * NT, kWChunks = 9 * kCout * 4, kWIters = (kWChunks + kTileThreads - 1) / kTileThreads;
     __shared__ __attribute__((aligned(16))) bf16 tile[kHaloH * kHaloW * 32];
     __shared__ __attribute__((aligned(16))) bf16 sW[9 * kCout * 32];
+    __shared__ __attribute__((aligned(16))) float sBias[kCout];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int g = lane >> 4, r = lane & 15;
-    const int slices = a.cinPad / 32, cBase = blockIdx.y * kCout;   // this block's output channels
+    const int slices = MULTI ? a.cinPad / 32 : 1, cBase = blockIdx.y * kCout;   // this block's output channels
     const int steps = ((tileCount - (int)blockIdx.x + (int)gridDim.x - 1) / (int)gridDim.x) * slices;
     if (steps <= 0) return;
     const __amdgpu_buffer_rsrc_t inBuf = bufferOf(a.in.p, (unsigned)((size_t)a.n * a.h * a.w * a.in.cs * 2));
     const __amdgpu_buffer_rsrc_t wBuf = bufferOf(a.wt, (unsigned)((size_t)9 * a.coutPad * a.cinPad * 2));
     const __amdgpu_buffer_rsrc_t lowBuf =
         bufferOf(a.low.p ? a.low.p : a.in.p, (unsigned)((size_t)a.n * (a.h / 2) * (a.w / 2) * a.low.cs * 2));
-    // this thread's halo chunks: pixel row and column in the halo tile and channel group
-    int hRow[kHaloIters], hCol[kHaloIters];
-#pragma unroll
-    for (int it = 0; it < kHaloIters; it++) {
-        const int v = threadIdx.x + it * kTileThreads, px = v >> 2;
-        hRow[it] = v < kHaloChunks ? px / kHaloW : -1000;   // -1000: no chunk (always outside)
-        hCol[it] = px % kHaloW;
-    }
+    const size_t pixels = (size_t)a.n * a.h * a.w;
+    const TileOut o{bufferOf(a.out.p, (unsigned)(pixels * a.out.cs * 2)), bufferOf(a.omask.p, (unsigned)(pixels * a.omask.cs * 2)),
+                    bufferOf(a.outF, (unsigned)(pixels * a.outFStride * 4))};
+    // this thread's halo chunk `it`: pixel row and column in the halo tile
+    auto hRow = [&](int it) {
+        const int v = threadIdx.x + it * kTileThreads;
+        return v < kHaloChunks ? (v >> 2) / kHaloW : -1000;   // -1000: no chunk (always outside)
+    };
+    auto hCol = [&](int it) { return ((int)threadIdx.x + it * kTileThreads) / 4 % kHaloW; };
     const int part = threadIdx.x & 3;   // every chunk index of a thread has the same low two bits
+    // byte offset of chunk `it` from the tile's first pixel (channel 0 of the slice), or past any
+    // buffer for a thread without that chunk (24-bit products: a halo row is under 2^24 bytes)
+    const unsigned rowBytes = (unsigned)(a.w * a.in.cs * 2), pixBytes = (unsigned)(a.in.cs * 2);
+    auto rel = [&](int it) {
+        return threadIdx.x + it * kTileThreads < kHaloChunks
+                   ? __umul24((unsigned)hRow(it), rowBytes) + __umul24((unsigned)hCol(it), pixBytes) - rowBytes - pixBytes + 16u * part
+                   : kOutside;
+    };
     bf16x8 rin[kHaloIters], rw[kWIters];
-    // loads the halo and weights of step k into rin / rw
+    // loads the weights of input slice ci0 into rw
+    auto fetchW = [&](int ci0) {
+#pragma unroll
+        for (int it = 0; it < kWIters; it++) {
+            const int v = threadIdx.x + it * kTileThreads;
+            const int row = v >> 2, tap = row / kCout, co = row % kCout;
+            const unsigned off =
+                v < kWChunks ? (unsigned)(((tap * a.coutPad + cBase + co) * a.cinPad) + ci0 + 8 * part) * 2 : kOutside;
+            rw[it] = bufferLoad16(wBuf, off);
+        }
+    };
+    auto storeW = [&]() {
+#pragma unroll
+        for (int it = 0; it < kWIters; it++) {
+            const int v = threadIdx.x + it * kTileThreads;
+            if (v < kWChunks) *reinterpret_cast<bf16x8*>(sW + swz(v >> 2, v & 3, v >> 2)) = rw[it];
+        }
+    };
+    // loads the halo (and for MULTI the weights) of step k into rin / rw
     auto fetch = [&](int k) {
         const int t = blockIdx.x + (k / slices) * gridDim.x, ci0 = (k % slices) * 32;
         const int x0 = (t % tilesX) * kTileW, y0 = ((t / tilesX) % tilesY) * kTileH, nn = t / (tilesX * tilesY);
@@ -270,38 +320,32 @@ __global__ __launch_bounds__(64 * ROWS, (tileBlocksPerCu<NT, ROWS>())) RT_CONV_O
             const int hl = a.h / 2, wl = a.w / 2;
 #pragma unroll
             for (int it = 0; it < kHaloIters; it++) {
-                const int yy = y0 + hRow[it] - 1, xx = x0 + hCol[it] - 1;
+                const int yy = y0 + hRow(it) - 1, xx = x0 + hCol(it) - 1;
                 const bool in = (unsigned)yy < (unsigned)a.h && (unsigned)xx < (unsigned)a.w;
                 const unsigned off =
                     in ? (unsigned)(((nn * hl + (yy >> 1)) * wl + (xx >> 1)) * a.low.cs) * 2 + chan : kOutside;
                 rin[it] = bufferLoad16(lowBuf, off);
             }
         } else {
-            const unsigned chan = (unsigned)(a.in.co + ci0 - a.lowC + 8 * part) * 2;
+            // Tiles whose halo lies inside the image skip the per-chunk bounds test (their
+            // threads without a chunk still land past the buffer: base + kOutside).
+            const unsigned base = (unsigned)(((nn * a.h + y0) * a.w + x0) * a.in.cs + a.in.co + ci0 - a.lowC) * 2;
+            const bool inside = x0 >= 1 && x0 + kTileW < a.w && y0 >= 1 && y0 + kTileH < a.h;
 #pragma unroll
             for (int it = 0; it < kHaloIters; it++) {
-                const int yy = y0 + hRow[it] - 1, xx = x0 + hCol[it] - 1;
-                const bool in = (unsigned)yy < (unsigned)a.h && (unsigned)xx < (unsigned)a.w;
-                const unsigned off = in ? (unsigned)(((nn * a.h + yy) * a.w + xx) * a.in.cs) * 2 + chan : kOutside;
-                rin[it] = bufferLoad16(inBuf, off);
+                const int yy = y0 + hRow(it) - 1, xx = x0 + hCol(it) - 1;
+                const bool in = inside || ((unsigned)yy < (unsigned)a.h && (unsigned)xx < (unsigned)a.w);
+                rin[it] = bufferLoad16(inBuf, in ? base + rel(it) : kOutside);
             }
         }
-        if (slices > 1 || k == 0) {
-#pragma unroll
-            for (int it = 0; it < kWIters; it++) {
-                const int v = threadIdx.x + it * kTileThreads;
-                const int row = v >> 2, tap = row / kCout, co = row % kCout;
-                const unsigned off =
-                    v < kWChunks ? (unsigned)(((tap * a.coutPad + cBase + co) * a.cinPad) + ci0 + 8 * part) * 2 : kOutside;
-                rw[it] = bufferLoad16(wBuf, off);
-            }
-        }
+        if (MULTI) fetchW(ci0);
     };
+    if (!MULTI) {   // the only slice's weights, staged once (published by the first step's barrier)
+        fetchW(0);
+        storeW();
+    }
     fetch(0);
-    float4 biasOf[NT];
-#pragma unroll
-    for (int j = 0; j < NT; j++)
-        biasOf[j] = a.bias ? *reinterpret_cast<const float4*>(a.bias + cBase + 16 * j + 4 * g) : make_float4(0, 0, 0, 0);
+    if ((int)threadIdx.x < kCout) sBias[threadIdx.x] = a.bias ? a.bias[cBase + threadIdx.x] : 0.0f;   // read after a barrier
     f32x4 acc[4][NT];
     int px0 = 0, py0 = 0, pnn = 0;   // the tile whose results acc holds
     for (int k = 0; k < steps; k++) {
@@ -311,20 +355,14 @@ __global__ __launch_bounds__(64 * ROWS, (tileBlocksPerCu<NT, ROWS>())) RT_CONV_O
 #pragma unroll
         for (int it = 0; it < kHaloIters; it++) {
             const int v = threadIdx.x + it * kTileThreads;
-            if (v < kHaloChunks) *reinterpret_cast<bf16x8*>(tile + v * 8) = rin[it];
+            if (v < kHaloChunks) *reinterpret_cast<bf16x8*>(tile + swz(v >> 2, v & 3, hCol(it))) = rin[it];
         }
-        if (slices > 1 || k == 0) {
-#pragma unroll
-            for (int it = 0; it < kWIters; it++) {
-                const int v = threadIdx.x + it * kTileThreads;
-                if (v < kWChunks) *reinterpret_cast<bf16x8*>(sW + v * 8) = rw[it];
-            }
-        }
+        if (MULTI) storeW();
         __syncthreads();
         // The previous tile's stores go out here, after this step's wait for its loads and
         // before the next step's loads: the wait at the next step (which counts stores too)
         // then finds both done behind this step's MFMAs.
-        if (slice == 0 && k > 0) tileEpilogue<NT>(a, acc, biasOf, cBase, px0, py0 + wave, pnn, g, r);
+        if (slice == 0 && k > 0) tileEpilogue<NT>(a, o, acc, sBias, cBase, px0, py0 + wave, pnn, g, r);
         if (k + 1 < steps) fetch(k + 1);
         if (slice == 0) {
             px0 = x0; py0 = y0; pnn = nn;
@@ -339,16 +377,16 @@ __global__ __launch_bounds__(64 * ROWS, (tileBlocksPerCu<NT, ROWS>())) RT_CONV_O
             bf16x8 av[4];
 #pragma unroll
             for (int q = 0; q < 4; q++)
-                av[q] = *reinterpret_cast<const bf16x8*>(tile + ((wave + dy) * kHaloW + 16 * q + r + dx) * 32 + 8 * g);
+                av[q] = *reinterpret_cast<const bf16x8*>(tile + swz((wave + dy) * kHaloW + 16 * q + r + dx, g, r + dx));
 #pragma unroll
             for (int j = 0; j < NT; j++) {
-                const bf16x8 wv = *reinterpret_cast<const bf16x8*>(sW + (tap * kCout + 16 * j + r) * 32 + 8 * g);
+                const bf16x8 wv = *reinterpret_cast<const bf16x8*>(sW + swz(tap * kCout + 16 * j + r, g, r));
 #pragma unroll
                 for (int q = 0; q < 4; q++) acc[q][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wv, av[q], acc[q][j], 0, 0, 0);
             }
         }
     }
-    tileEpilogue<NT>(a, acc, biasOf, cBase, px0, py0 + wave, pnn, g, r);
+    tileEpilogue<NT>(a, o, acc, sBias, cBase, px0, py0 + wave, pnn, g, r);
 }
 
 struct BwdWArgs {
@@ -1058,59 +1096,59 @@ bf16* wbOf(DenoiserNet* net, int l) {
 // Whether launchConvFwd runs a convolution on the tile kernel.
 bool tileConv(const ConvArgs& a) {
     return a.coutPad % 32 == 0 && (a.w >= 16 || (long long)a.w * a.h >= 16384) &&
-           (long long)a.n * a.h * a.w * a.in.cs * 2 < (long long)kOutside &&
+           (long long)a.n * a.h * a.w * a.in.cs * 2 < (long long)kOutside && (long long)a.w * a.in.cs * 2 < (1LL << 24) &&
            (long long)9 * a.coutPad * a.cinPad * 2 < kOutside && (a.in.cs | a.in.co) % 8 == 0 &&
-           (!a.out.p || (a.out.cs | a.out.co) % 4 == 0) &&
+           (!a.out.p || ((a.out.cs | a.out.co) % 4 == 0 && (long long)a.n * a.h * a.w * a.out.cs * 2 < (long long)kOutside)) &&
+           (!a.outF || (a.outFStride % 4 == 0 && (long long)a.n * a.h * a.w * a.outFStride * 4 < (long long)kOutside)) &&
            (!a.low.p || (a.lowC % 32 == 0 && a.h % 2 == 0 && a.w % 2 == 0 && (a.low.cs | a.low.co) % 8 == 0)) &&
-           (!a.omask.p || (a.omask.cs | a.omask.co) % 4 == 0);
+           (!a.omask.p || ((a.omask.cs | a.omask.co) % 4 == 0 && (long long)a.n * a.h * a.w * a.omask.cs * 2 < (long long)kOutside));
 }
 
-// Work items (tiles x channel groups) per compute unit below which a layer runs 4-row tiles.
+// A layer runs 4-row tiles when its 8-row work items (tiles x 32-channel groups) number fewer
+// than RT_CONV_SMALL percent of the compute units (the 1/16- and 1/8-resolution layers of a
+// 1080p frame), 8-row tiles otherwise (their halo rows and weights serve twice the pixels).
 #ifndef RT_CONV_SMALL
-#define RT_CONV_SMALL 16
+#define RT_CONV_SMALL 50
 #endif
+
+// Launches one tile kernel as a persistent grid: as many blocks of each channel group as the
+// GPU holds at once (its own occupancy, found once), fewer for small images.
+template <int NT, int ROWS, bool MULTI>
+hipError_t launchTileKernel(const ConvArgs& a, int groups, int tilesX, int tilesY, int tileCount, hipStream_t s) {
+    constexpr int threads = 64 * ROWS;
+    static const int resident = [] {
+        int dev = 0, cus = 0, perCu = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, convFwdTileKernel<NT, ROWS, MULTI>, threads, 0) != hipSuccess)
+            return 0;
+        return std::max(1, cus) * std::max(1, perCu);
+    }();
+    if (!resident) return hipErrorInvalidDevice;
+    const dim3 grid(std::min(tileCount, std::max(1, resident / groups)), groups);
+    hipLaunchKernelGGL((convFwdTileKernel<NT, ROWS, MULTI>), grid, dim3(threads), 0, s, a, tilesX, tilesY, tileCount);
+    return hipGetLastError();
+}
 
 // The persistent tile kernel for a convolution that tileConv accepts, with ROWS-row tiles.
 template <int ROWS>
-hipError_t launchConvTileRows(const ConvArgs& a, hipStream_t s) {
-    constexpr int threads = 64 * ROWS;
+hipError_t launchConvTileRows(const ConvArgs& a, int cus, hipStream_t s) {
     const int tilesX = (a.w + kTileW - 1) / kTileW, tilesY = (a.h + ROWS - 1) / ROWS;
     const int tileCount = tilesX * tilesY * a.n;
-    // persistent: as many blocks as the GPU holds at once (fewer for small images)
-    static int resident[3] = {0, 0, 0}, cus = 0;
-    if (!resident[0]) {
-        int dev = 0, perCu2 = 0, perCu4 = 0;
-        hipError_t e = hipGetDevice(&dev);
-        if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-        if (e == hipSuccess)
-            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu2, convFwdTileKernel<2, ROWS>, threads, 0);
-        if (e == hipSuccess)
-            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu4, convFwdTileKernel<4, ROWS>, threads, 0);
-        if (e != hipSuccess) return e;
-        resident[0] = std::max(1, cus * std::max(1, perCu2));
-        resident[1] = std::max(1, cus * std::max(1, perCu4));
-    }
+    const bool multi = a.cinPad > 32;
     // Only an FP32 output of at most 16 channels (the network's last layer): the first
     // 16-channel group suffices.
-    if (!a.out.p && a.outF && a.outFStride <= 16) {
-        if (!resident[2]) {
-            int perCu1 = 0;
-            const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu1, convFwdTileKernel<1, ROWS>, threads, 0);
-            if (e != hipSuccess) return e;
-            resident[2] = std::max(1, cus * std::max(1, perCu1));
-        }
-        hipLaunchKernelGGL((convFwdTileKernel<1, ROWS>), dim3(std::min(tileCount, resident[2]), 1), dim3(threads), 0, s, a,
-                           tilesX, tilesY, tileCount);
-        return hipGetLastError();
-    }
+    if (!a.out.p && a.outF && a.outFStride <= 16)
+        return multi ? launchTileKernel<1, ROWS, true>(a, 1, tilesX, tilesY, tileCount, s)
+                     : launchTileKernel<1, ROWS, false>(a, 1, tilesX, tilesY, tileCount, s);
     // 64-channel groups read each halo once for twice the channels; 32-channel groups make
     // twice the blocks, for small images that would leave compute units idle
     const bool wide = a.coutPad % 64 == 0 && (long long)tileCount * (a.coutPad / 64) >= 2LL * cus;
     const int groups = a.coutPad / (wide ? 64 : 32);
-    const dim3 grid(std::min(tileCount, std::max(1, resident[wide ? 1 : 0] / groups)), groups);
-    if (wide) hipLaunchKernelGGL((convFwdTileKernel<4, ROWS>), grid, dim3(threads), 0, s, a, tilesX, tilesY, tileCount);
-    else hipLaunchKernelGGL((convFwdTileKernel<2, ROWS>), grid, dim3(threads), 0, s, a, tilesX, tilesY, tileCount);
-    return hipGetLastError();
+    if (wide) return multi ? launchTileKernel<4, ROWS, true>(a, groups, tilesX, tilesY, tileCount, s)
+                           : launchTileKernel<4, ROWS, false>(a, groups, tilesX, tilesY, tileCount, s);
+    return multi ? launchTileKernel<2, ROWS, true>(a, groups, tilesX, tilesY, tileCount, s)
+                 : launchTileKernel<2, ROWS, false>(a, groups, tilesX, tilesY, tileCount, s);
 }
 
 hipError_t launchConvTile(const ConvArgs& a, hipStream_t s) {
@@ -1123,7 +1161,7 @@ hipError_t launchConvTile(const ConvArgs& a, hipStream_t s) {
     }
     const long long tiles8 = (long long)((a.w + kTileW - 1) / kTileW) * ((a.h + 7) / 8) * a.n;
     const long long items8 = tiles8 * (a.coutPad / 32);
-    return items8 < (long long)RT_CONV_SMALL * cus ? launchConvTileRows<4>(a, s) : launchConvTileRows<8>(a, s);
+    return items8 * 100 < (long long)RT_CONV_SMALL * cus ? launchConvTileRows<4>(a, cus, s) : launchConvTileRows<8>(a, cus, s);
 }
 
 hipError_t launchConvFwd(const ConvArgs& a, hipStream_t s) {
@@ -1277,7 +1315,7 @@ ConvArgs convArgs(DenoiserNet* net, int l, const View& in, const View& out, int 
     ConvArgs a{};
     a.in = in; a.wt = wfOf(net, l); a.bias = net->params + L.bOff; a.out = out;
     a.outF = outF; a.outFStride = 4;
-    a.n = n; a.h = h; a.w = w; a.cinPad = L.cinPad; a.coutPad = L.coutPad; a.relu = L.relu; a.accumulate = 0;
+    a.n = n; a.h = h; a.w = w; a.cinPad = L.cinPad; a.coutPad = L.coutPad; a.relu = L.relu;
     return a;
 }
 
@@ -1355,19 +1393,19 @@ hipError_t forward(DenoiserNet* net, hipStream_t s, bool fuseUp = false) {
 }
 
 // Backward through one convolution: weight/bias gradients and (if gin.p) the input
-// gradient, set or accumulated.  g is the gradient w.r.t. the layer's pre-activation output:
+// gradient.  g is the gradient w.r.t. the layer's pre-activation output:
 // every producer of a ReLU layer's output gradient zeroes it where that output is not
 // positive (the input-gradient convolution's `omask`, the pooling backward's mask; nearest
 // upsampling's backward sums pixels that share one mask value).  gmask: the ReLU output under
 // gin, or none when gin is a pooling output's gradient (no activation).
 hipError_t convBack(DenoiserNet* net, int l, const View& g, const View& in, const View& gin, const View& gmask,
-                    bool accumulate, int n, int h, int w, hipStream_t s) {
+                    int n, int h, int w, hipStream_t s) {
     const DnLayer& L = net->layers[l];
     DN_TRY(convWeightGrad(net, g, in, n, h, w, L.cinPad, L.coutPad, net->grads + L.wOff, net->grads + L.bOff, s));
     if (gin.p) {
         ConvArgs a{};
         a.in = g; a.wt = wbOf(net, l); a.bias = nullptr; a.out = gin; a.outF = nullptr; a.outFStride = 0;
-        a.n = n; a.h = h; a.w = w; a.cinPad = L.coutPad; a.coutPad = L.cinPad; a.relu = 0; a.accumulate = accumulate;
+        a.n = n; a.h = h; a.w = w; a.cinPad = L.coutPad; a.coutPad = L.cinPad; a.relu = 0;
         a.omask = gmask;
         DN_TRY(launchConvFwd(a, s));
     }
@@ -1379,41 +1417,41 @@ hipError_t backwardLayers(DenoiserNet* net, hipStream_t s) {
     const int h1 = h0 / 2, w1 = w0 / 2, h2 = h0 / 4, w2 = w0 / 4, h3 = h0 / 8, w3 = w0 / 8, h4 = h0 / 16, w4 = w0 / 16;
     const View none{nullptr, 0, 0, 0};
     // convBack(layer, output gradient, input, input gradient, ReLU output under it, ...)
-    DN_TRY(convBack(net, OUT, V(net->gout, 32), V(net->d0b, 32), V(net->gd0b, 32), V(net->d0b, 32), false, n, h0, w0, s));
-    DN_TRY(convBack(net, D0B, V(net->gd0b, 32), V(net->d0a, 32), V(net->gd0a, 32), V(net->d0a, 32), false, n, h0, w0, s));
-    DN_TRY(convBack(net, D0A, V(net->gd0a, 32), V(net->D0, 96), V(net->gD0, 96), V(net->D0, 96), false, n, h0, w0, s));
+    DN_TRY(convBack(net, OUT, V(net->gout, 32), V(net->d0b, 32), V(net->gd0b, 32), V(net->d0b, 32), n, h0, w0, s));
+    DN_TRY(convBack(net, D0B, V(net->gd0b, 32), V(net->d0a, 32), V(net->gd0a, 32), V(net->d0a, 32), n, h0, w0, s));
+    DN_TRY(convBack(net, D0A, V(net->gd0a, 32), V(net->D0, 96), V(net->gD0, 96), V(net->D0, 96), n, h0, w0, s));
     hipLaunchKernelGGL(upsampleBackKernel, dim3(blocks((long long)n * h1 * w1 * 8)), dim3(kB), 0, s, V(net->gD0, 96, 0, 64),
                        V(net->gd1b, 64), n, h1, w1, 64);
-    DN_TRY(convBack(net, D1B, V(net->gd1b, 64), V(net->d1a, 64), V(net->gd1a, 64), V(net->d1a, 64), false, n, h1, w1, s));
-    DN_TRY(convBack(net, D1A, V(net->gd1a, 64), V(net->D1, 128), V(net->gD1, 128), V(net->D1, 128), false, n, h1, w1, s));
+    DN_TRY(convBack(net, D1B, V(net->gd1b, 64), V(net->d1a, 64), V(net->gd1a, 64), V(net->d1a, 64), n, h1, w1, s));
+    DN_TRY(convBack(net, D1A, V(net->gd1a, 64), V(net->D1, 128), V(net->gD1, 128), V(net->D1, 128), n, h1, w1, s));
     hipLaunchKernelGGL(upsampleBackKernel, dim3(blocks((long long)n * h2 * w2 * 8)), dim3(kB), 0, s, V(net->gD1, 128, 0, 64),
                        V(net->gd2b, 64), n, h2, w2, 64);
-    DN_TRY(convBack(net, D2B, V(net->gd2b, 64), V(net->d2a, 64), V(net->gd2a, 64), V(net->d2a, 64), false, n, h2, w2, s));
-    DN_TRY(convBack(net, D2A, V(net->gd2a, 64), V(net->D2, 160), V(net->gD2, 160), V(net->D2, 160), false, n, h2, w2, s));
+    DN_TRY(convBack(net, D2B, V(net->gd2b, 64), V(net->d2a, 64), V(net->gd2a, 64), V(net->d2a, 64), n, h2, w2, s));
+    DN_TRY(convBack(net, D2A, V(net->gd2a, 64), V(net->D2, 160), V(net->gD2, 160), V(net->D2, 160), n, h2, w2, s));
     hipLaunchKernelGGL(upsampleBackKernel, dim3(blocks((long long)n * h3 * w3 * 12)), dim3(kB), 0, s, V(net->gD2, 160, 0, 96),
                        V(net->gd3b, 96), n, h3, w3, 96);
-    DN_TRY(convBack(net, D3B, V(net->gd3b, 96), V(net->d3a, 96), V(net->gd3a, 96), V(net->d3a, 96), false, n, h3, w3, s));
-    DN_TRY(convBack(net, D3A, V(net->gd3a, 96), V(net->D3, 224), V(net->gD3, 224), V(net->D3, 224), false, n, h3, w3, s));
+    DN_TRY(convBack(net, D3B, V(net->gd3b, 96), V(net->d3a, 96), V(net->gd3a, 96), V(net->d3a, 96), n, h3, w3, s));
+    DN_TRY(convBack(net, D3A, V(net->gd3a, 96), V(net->D3, 224), V(net->gD3, 224), V(net->D3, 224), n, h3, w3, s));
     hipLaunchKernelGGL(upsampleBackKernel, dim3(blocks((long long)n * h4 * w4 * 16)), dim3(kB), 0, s, V(net->gD3, 224, 0, 128),
                        V(net->gbb, 128), n, h4, w4, 128);
-    DN_TRY(convBack(net, BB, V(net->gbb, 128), V(net->ba, 128), V(net->gba, 128), V(net->ba, 128), false, n, h4, w4, s));
-    DN_TRY(convBack(net, BA, V(net->gba, 128), V(net->p3, 96), V(net->gp3, 96), none, false, n, h4, w4, s));
+    DN_TRY(convBack(net, BB, V(net->gbb, 128), V(net->ba, 128), V(net->gba, 128), V(net->ba, 128), n, h4, w4, s));
+    DN_TRY(convBack(net, BA, V(net->gba, 128), V(net->p3, 96), V(net->gp3, 96), none, n, h4, w4, s));
     hipLaunchKernelGGL(avgPoolBackKernel, dim3(blocks((long long)n * h3 * w3 * 12)), dim3(kB), 0, s, V(net->gp3, 96),
                        V(net->gD3, 224, 128, 96), V(net->D3, 224, 128, 96), n, h3, w3, 96);
-    DN_TRY(convBack(net, E3B, V(net->gD3, 224, 128, 96), V(net->e3a, 96), V(net->ge3a, 96), V(net->e3a, 96), false, n, h3, w3, s));
-    DN_TRY(convBack(net, E3A, V(net->ge3a, 96), V(net->p2, 64), V(net->gp2, 64), none, false, n, h3, w3, s));
+    DN_TRY(convBack(net, E3B, V(net->gD3, 224, 128, 96), V(net->e3a, 96), V(net->ge3a, 96), V(net->e3a, 96), n, h3, w3, s));
+    DN_TRY(convBack(net, E3A, V(net->ge3a, 96), V(net->p2, 64), V(net->gp2, 64), none, n, h3, w3, s));
     hipLaunchKernelGGL(avgPoolBackKernel, dim3(blocks((long long)n * h2 * w2 * 8)), dim3(kB), 0, s, V(net->gp2, 64),
                        V(net->gD2, 160, 96, 64), V(net->D2, 160, 96, 64), n, h2, w2, 64);
-    DN_TRY(convBack(net, E2B, V(net->gD2, 160, 96, 64), V(net->e2a, 64), V(net->ge2a, 64), V(net->e2a, 64), false, n, h2, w2, s));
-    DN_TRY(convBack(net, E2A, V(net->ge2a, 64), V(net->p1, 64), V(net->gp1, 64), none, false, n, h2, w2, s));
+    DN_TRY(convBack(net, E2B, V(net->gD2, 160, 96, 64), V(net->e2a, 64), V(net->ge2a, 64), V(net->e2a, 64), n, h2, w2, s));
+    DN_TRY(convBack(net, E2A, V(net->ge2a, 64), V(net->p1, 64), V(net->gp1, 64), none, n, h2, w2, s));
     hipLaunchKernelGGL(avgPoolBackKernel, dim3(blocks((long long)n * h1 * w1 * 8)), dim3(kB), 0, s, V(net->gp1, 64),
                        V(net->gD1, 128, 64, 64), V(net->D1, 128, 64, 64), n, h1, w1, 64);
-    DN_TRY(convBack(net, E1B, V(net->gD1, 128, 64, 64), V(net->e1a, 64), V(net->ge1a, 64), V(net->e1a, 64), false, n, h1, w1, s));
-    DN_TRY(convBack(net, E1A, V(net->ge1a, 64), V(net->p0, 32), V(net->gp0, 32), none, false, n, h1, w1, s));
+    DN_TRY(convBack(net, E1B, V(net->gD1, 128, 64, 64), V(net->e1a, 64), V(net->ge1a, 64), V(net->e1a, 64), n, h1, w1, s));
+    DN_TRY(convBack(net, E1A, V(net->ge1a, 64), V(net->p0, 32), V(net->gp0, 32), none, n, h1, w1, s));
     hipLaunchKernelGGL(avgPoolBackKernel, dim3(blocks((long long)n * h0 * w0 * 4)), dim3(kB), 0, s, V(net->gp0, 32),
                        V(net->gD0, 96, 64, 32), V(net->D0, 96, 64, 32), n, h0, w0, 32);
-    DN_TRY(convBack(net, E0B, V(net->gD0, 96, 64, 32), V(net->e0a, 32), V(net->ge0a, 32), V(net->e0a, 32), false, n, h0, w0, s));
-    DN_TRY(convBack(net, E0A, V(net->ge0a, 32), V(net->x0, 32), none, none, false, n, h0, w0, s));
+    DN_TRY(convBack(net, E0B, V(net->gD0, 96, 64, 32), V(net->e0a, 32), V(net->ge0a, 32), V(net->e0a, 32), n, h0, w0, s));
+    DN_TRY(convBack(net, E0A, V(net->ge0a, 32), V(net->x0, 32), none, none, n, h0, w0, s));
     return hipGetLastError();
 }
 
@@ -1739,7 +1777,7 @@ hipError_t convForwardHost(const float* in, int n, int h, int w, int cin, const 
         ConvArgs a{};
         a.in = View{din, ciPad, 0, ciPad}; a.wt = dw; a.bias = db; a.out = View{nullptr, coPad, 0, coPad};
         a.outF = dout; a.outFStride = coPad; a.n = n; a.h = h; a.w = w; a.cinPad = ciPad; a.coutPad = coPad;
-        a.relu = relu; a.accumulate = 0;
+        a.relu = relu;
         e = launchConvFwd(a, nullptr);
     }
     if (e == hipSuccess) e = hipDeviceSynchronize();
@@ -1784,7 +1822,7 @@ hipError_t convBackwardHost(const float* in, int n, int h, int w, int cin, const
         ConvArgs a{};
         a.in = View{dg, coPad, 0, coPad}; a.wt = dwb; a.bias = nullptr; a.out = View{nullptr, ciPad, 0, ciPad};
         a.outF = dgi; a.outFStride = ciPad; a.n = n; a.h = h; a.w = w; a.cinPad = coPad; a.coutPad = ciPad;
-        a.relu = 0; a.accumulate = 0;
+        a.relu = 0;
         e = launchConvFwd(a, nullptr);
     }
     if (e == hipSuccess) e = hipMemset(dgw, 0, 9ULL * coPad * ciPad * 4);
