@@ -26,6 +26,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <random>
 #include "denoiser.h"
 
@@ -254,8 +255,9 @@ __device__ __forceinline__ void tileEpilogue(const ConvArgs& a, const TileOut& o
 template <int NT, int ROWS, bool MULTI> constexpr int tileBlocksPerCu() {
     return ROWS == 4 ? (NT == 4 || (NT == 2 && MULTI) ? 2 : 3) : 1;
 }
-// MULTI: the layer has several 32-channel input slices (each step stages a slice's weights as
-// well); without it the weights are staged once, before the first step.
+// MULTI: each step stages its input slice's weights as well (layers whose weights do not fit
+// in LDS beside the halo); without it all of the block's weights (every slice: dynamic LDS of
+// 9 x 16 NT x cinPad BF16) are staged once, before the first step.
 template <int NT, int ROWS, bool MULTI>
 __global__ __launch_bounds__(64 * ROWS, (tileBlocksPerCu<NT, ROWS, MULTI>())) RT_CONV_OCCUPANCY void convFwdTileKernel(ConvArgs a, int tilesX, int tilesY, int tileCount) {
     using T = TileShape<ROWS>;
@@ -263,11 +265,13 @@ __global__ __launch_bounds__(64 * ROWS, (tileBlocksPerCu<NT, ROWS, MULTI>())) RT
     constexpr int kHaloChunks = T::kHaloChunks, kHaloIters = T::kHaloIters;
     constexpr int kCout = 16 * NT, kWChunks = 9 * kCout * 4, kWIters = (kWChunks + kTileThreads - 1) / kTileThreads;
     __shared__ __attribute__((aligned(16))) bf16 tile[kHaloH * kHaloW * 32];
-    __shared__ __attribute__((aligned(16))) bf16 sW[9 * kCout * 32];
+    __shared__ __attribute__((aligned(16))) bf16 sWStep[MULTI ? 9 * kCout * 32 : 8];
+    extern __shared__ __attribute__((aligned(16))) bf16 sWAll[];
     __shared__ __attribute__((aligned(16))) float sBias[kCout];
+    bf16* const sW = MULTI ? sWStep : sWAll;   // [slice][tap][output channel][32 input channels]
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int g = lane >> 4, r = lane & 15;
-    const int slices = MULTI ? a.cinPad / 32 : 1, cBase = blockIdx.y * kCout;   // this block's output channels
+    const int slices = a.cinPad / 32, cBase = blockIdx.y * kCout;   // this block's output channels
     const int steps = ((tileCount - (int)blockIdx.x + (int)gridDim.x - 1) / (int)gridDim.x) * slices;
     if (steps <= 0) return;
     const __amdgpu_buffer_rsrc_t inBuf = bufferOf(a.in.p, (unsigned)((size_t)a.n * a.h * a.w * a.in.cs * 2));
@@ -293,22 +297,24 @@ __global__ __launch_bounds__(64 * ROWS, (tileBlocksPerCu<NT, ROWS, MULTI>())) RT
                    : kOutside;
     };
     bf16x8 rin[kHaloIters], rw[kWIters];
-    // loads the weights of input slice ci0 into rw
-    auto fetchW = [&](int ci0) {
+    // loads the weights of input slice ci0 into dst
+    auto fetchW = [&](int ci0, bf16x8 (&dst)[kWIters]) {
 #pragma unroll
         for (int it = 0; it < kWIters; it++) {
             const int v = threadIdx.x + it * kTileThreads;
             const int row = v >> 2, tap = row / kCout, co = row % kCout;
             const unsigned off =
                 v < kWChunks ? (unsigned)(((tap * a.coutPad + cBase + co) * a.cinPad) + ci0 + 8 * part) * 2 : kOutside;
-            rw[it] = bufferLoad16(wBuf, off);
+            dst[it] = bufferLoad16(wBuf, off);
         }
     };
-    auto storeW = [&]() {
+    // stores src as the weights of LDS slice `slice`
+    auto storeW = [&](int slice, const bf16x8 (&src)[kWIters]) {
+        bf16* const dst = sW + slice * (9 * kCout * 32);
 #pragma unroll
         for (int it = 0; it < kWIters; it++) {
             const int v = threadIdx.x + it * kTileThreads;
-            if (v < kWChunks) *reinterpret_cast<bf16x8*>(sW + swz(v >> 2, v & 3, v >> 2)) = rw[it];
+            if (v < kWChunks) *reinterpret_cast<bf16x8*>(dst + swz(v >> 2, v & 3, v >> 2)) = src[it];
         }
     };
     // loads the halo (and for MULTI the weights) of step k into rin / rw
@@ -338,11 +344,19 @@ __global__ __launch_bounds__(64 * ROWS, (tileBlocksPerCu<NT, ROWS, MULTI>())) RT
                 rin[it] = bufferLoad16(inBuf, in ? base + rel(it) : kOutside);
             }
         }
-        if (MULTI) fetchW(ci0);
+        if (MULTI) fetchW(ci0, rw);
     };
-    if (!MULTI) {   // the only slice's weights, staged once (published by the first step's barrier)
-        fetchW(0);
-        storeW();
+    if (!MULTI) {   // every slice's weights, staged once (published by the first step's barrier)
+        bf16x8 wa[kWIters], wb[kWIters];
+        fetchW(0, wa);
+        for (int slice = 0; slice < slices; slice += 2) {   // the next slice's loads overlap this slice's stores
+            if (slice + 1 < slices) fetchW(32 * (slice + 1), wb);
+            storeW(slice, wa);
+            if (slice + 1 < slices) {
+                if (slice + 2 < slices) fetchW(32 * (slice + 2), wa);
+                storeW(slice + 1, wb);
+            }
+        }
     }
     fetch(0);
     if ((int)threadIdx.x < kCout) sBias[threadIdx.x] = a.bias ? a.bias[cBase + threadIdx.x] : 0.0f;   // read after a barrier
@@ -357,7 +371,7 @@ __global__ __launch_bounds__(64 * ROWS, (tileBlocksPerCu<NT, ROWS, MULTI>())) RT
             const int v = threadIdx.x + it * kTileThreads;
             if (v < kHaloChunks) *reinterpret_cast<bf16x8*>(tile + swz(v >> 2, v & 3, hCol(it))) = rin[it];
         }
-        if (MULTI) storeW();
+        if (MULTI) storeW(0, rw);
         __syncthreads();
         // The previous tile's stores go out here, after this step's wait for its loads and
         // before the next step's loads: the wait at the next step (which counts stores too)
@@ -371,6 +385,7 @@ __global__ __launch_bounds__(64 * ROWS, (tileBlocksPerCu<NT, ROWS, MULTI>())) RT
 #pragma unroll
                 for (int j = 0; j < NT; j++) acc[q][j] = f32x4{0, 0, 0, 0};
         }
+        const bf16* const wSlice = sW + (MULTI ? 0 : slice * (9 * kCout * 32));
 #pragma unroll RT_CONV_TAP_UNROLL
         for (int tap = 0; tap < 9; tap++) {
             const int dy = tap / 3, dx = tap % 3;
@@ -380,7 +395,7 @@ __global__ __launch_bounds__(64 * ROWS, (tileBlocksPerCu<NT, ROWS, MULTI>())) RT
                 av[q] = *reinterpret_cast<const bf16x8*>(tile + swz((wave + dy) * kHaloW + 16 * q + r + dx, g, r + dx));
 #pragma unroll
             for (int j = 0; j < NT; j++) {
-                const bf16x8 wv = *reinterpret_cast<const bf16x8*>(sW + swz(tap * kCout + 16 * j + r, g, r));
+                const bf16x8 wv = *reinterpret_cast<const bf16x8*>(wSlice + swz(tap * kCout + 16 * j + r, g, r));
 #pragma unroll
                 for (int q = 0; q < 4; q++) acc[q][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wv, av[q], acc[q][j], 0, 0, 0);
             }
@@ -1112,56 +1127,94 @@ bool tileConv(const ConvArgs& a) {
 #endif
 
 // Launches one tile kernel as a persistent grid: as many blocks of each channel group as the
-// GPU holds at once (its own occupancy, found once), fewer for small images.
+// GPU holds at once (its occupancy with `dynLds` bytes of weights, found once per size),
+// fewer for small images.
 template <int NT, int ROWS, bool MULTI>
-hipError_t launchTileKernel(const ConvArgs& a, int groups, int tilesX, int tilesY, int tileCount, hipStream_t s) {
+hipError_t launchTileKernel(const ConvArgs& a, int groups, int tilesX, int tilesY, int tileCount, int dynLds, hipStream_t s) {
     constexpr int threads = 64 * ROWS;
-    static const int resident = [] {
-        int dev = 0, cus = 0, perCu = 0;
-        if (hipGetDevice(&dev) != hipSuccess ||
-            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-            hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, convFwdTileKernel<NT, ROWS, MULTI>, threads, 0) != hipSuccess)
-            return 0;
-        return std::max(1, cus) * std::max(1, perCu);
-    }();
-    if (!resident) return hipErrorInvalidDevice;
-    const dim3 grid(std::min(tileCount, std::max(1, resident / groups)), groups);
-    hipLaunchKernelGGL((convFwdTileKernel<NT, ROWS, MULTI>), grid, dim3(threads), 0, s, a, tilesX, tilesY, tileCount);
+    static std::mutex lock;
+    static int sizes[8], resident[8], known = 0;
+    int blocks = 0;
+    {
+        std::lock_guard<std::mutex> guard(lock);
+        for (int i = 0; i < known && !blocks; i++)
+            if (sizes[i] == dynLds) blocks = resident[i];
+        if (!blocks) {
+            int dev = 0, cus = 0, perCu = 0;
+            hipError_t e = hipGetDevice(&dev);
+            if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+            // a request to allow that much dynamic LDS (the launch itself reports a size the
+            // device cannot give; a runtime that needs no such request may refuse it)
+            if (e == hipSuccess && dynLds > 0 &&
+                hipFuncSetAttribute(reinterpret_cast<const void*>(convFwdTileKernel<NT, ROWS, MULTI>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, dynLds) != hipSuccess)
+                (void)hipGetLastError();
+            if (e == hipSuccess)
+                e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, convFwdTileKernel<NT, ROWS, MULTI>, threads, dynLds);
+            if (e != hipSuccess) return e;
+            blocks = std::max(1, cus) * std::max(1, perCu);
+            if (known < 8) {
+                sizes[known] = dynLds;
+                resident[known++] = blocks;
+            }
+        }
+    }
+    const dim3 grid(std::min(tileCount, std::max(1, blocks / groups)), groups);
+    hipLaunchKernelGGL((convFwdTileKernel<NT, ROWS, MULTI>), grid, dim3(threads), dynLds, s, a, tilesX, tilesY, tileCount);
     return hipGetLastError();
 }
 
-// The persistent tile kernel for a convolution that tileConv accepts, with ROWS-row tiles.
+// LDS of a tile block besides its weights: the halo slice and the biases.
+template <int NT, int ROWS> constexpr int tileStaticLds() { return TileShape<ROWS>::kHaloH * kHaloW * 64 + 64 * NT; }
+
+// Whether a block's weights (all input slices) fit in LDS beside its halo at the resident-weight
+// kernel's blocks per compute unit.
+template <int NT, int ROWS> bool weightsResident(const ConvArgs& a, int ldsPerCu) {
+    return a.cinPad == 32 ||
+           (tileStaticLds<NT, ROWS>() + 9 * 16 * NT * a.cinPad * 2) * tileBlocksPerCu<NT, ROWS, false>() <= ldsPerCu;
+}
+
+// The persistent tile kernel for a convolution that tileConv accepts, with ROWS-row tiles: all
+// weights of a block resident in LDS when they fit, else one input slice's weights per step.
+template <int NT, int ROWS>
+hipError_t launchTileWeights(const ConvArgs& a, int groups, int tilesX, int tilesY, int tileCount, int ldsPerCu, hipStream_t s) {
+    if (weightsResident<NT, ROWS>(a, ldsPerCu))
+        return launchTileKernel<NT, ROWS, false>(a, groups, tilesX, tilesY, tileCount, 9 * 16 * NT * a.cinPad * 2, s);
+    return launchTileKernel<NT, ROWS, true>(a, groups, tilesX, tilesY, tileCount, 0, s);
+}
+
 template <int ROWS>
-hipError_t launchConvTileRows(const ConvArgs& a, int cus, hipStream_t s) {
+hipError_t launchConvTileRows(const ConvArgs& a, int cus, int ldsPerCu, hipStream_t s) {
     const int tilesX = (a.w + kTileW - 1) / kTileW, tilesY = (a.h + ROWS - 1) / ROWS;
     const int tileCount = tilesX * tilesY * a.n;
-    const bool multi = a.cinPad > 32;
     // Only an FP32 output of at most 16 channels (the network's last layer): the first
     // 16-channel group suffices.
-    if (!a.out.p && a.outF && a.outFStride <= 16)
-        return multi ? launchTileKernel<1, ROWS, true>(a, 1, tilesX, tilesY, tileCount, s)
-                     : launchTileKernel<1, ROWS, false>(a, 1, tilesX, tilesY, tileCount, s);
+    if (!a.out.p && a.outF && a.outFStride <= 16) return launchTileWeights<1, ROWS>(a, 1, tilesX, tilesY, tileCount, ldsPerCu, s);
     // 64-channel groups read each halo once for twice the channels; 32-channel groups make
-    // twice the blocks, for small images that would leave compute units idle
-    const bool wide = a.coutPad % 64 == 0 && (long long)tileCount * (a.coutPad / 64) >= 2LL * cus;
+    // twice the blocks, for small images that would leave compute units idle, and keep their
+    // weights in LDS where 64-channel groups could not
+    const bool wide = a.coutPad % 64 == 0 && (long long)tileCount * (a.coutPad / 64) >= 2LL * cus &&
+                      (weightsResident<4, ROWS>(a, ldsPerCu) || !weightsResident<2, ROWS>(a, ldsPerCu));
     const int groups = a.coutPad / (wide ? 64 : 32);
-    if (wide) return multi ? launchTileKernel<4, ROWS, true>(a, groups, tilesX, tilesY, tileCount, s)
-                           : launchTileKernel<4, ROWS, false>(a, groups, tilesX, tilesY, tileCount, s);
-    return multi ? launchTileKernel<2, ROWS, true>(a, groups, tilesX, tilesY, tileCount, s)
-                 : launchTileKernel<2, ROWS, false>(a, groups, tilesX, tilesY, tileCount, s);
+    return wide ? launchTileWeights<4, ROWS>(a, groups, tilesX, tilesY, tileCount, ldsPerCu, s)
+                : launchTileWeights<2, ROWS>(a, groups, tilesX, tilesY, tileCount, ldsPerCu, s);
 }
 
 hipError_t launchConvTile(const ConvArgs& a, hipStream_t s) {
-    static int cus = 0;
+    static int cus = 0, ldsPerCu = 0;
     if (!cus) {
-        int dev = 0;
+        int dev = 0, c = 0, l = 0;
         hipError_t e = hipGetDevice(&dev);
-        if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        if (e == hipSuccess) e = hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev);
+        if (e == hipSuccess) e = hipDeviceGetAttribute(&l, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev);
         if (e != hipSuccess) return e;
+        ldsPerCu = l;
+        cus = c;
     }
     const long long tiles8 = (long long)((a.w + kTileW - 1) / kTileW) * ((a.h + 7) / 8) * a.n;
     const long long items8 = tiles8 * (a.coutPad / 32);
-    return items8 * 100 < (long long)RT_CONV_SMALL * cus ? launchConvTileRows<4>(a, cus, s) : launchConvTileRows<8>(a, cus, s);
+    return items8 * 100 < (long long)RT_CONV_SMALL * cus ? launchConvTileRows<4>(a, cus, ldsPerCu, s)
+                                                          : launchConvTileRows<8>(a, cus, ldsPerCu, s);
 }
 
 hipError_t launchConvFwd(const ConvArgs& a, hipStream_t s) {
