@@ -23,13 +23,17 @@ def conv_forward(lib, x, w, b, relu):
     return out
 
 
-# Images at least 16 pixels wide (the first shape and the last five) run on the LDS-tiled
+# Images at least 16 pixels wide (the first shape and the last nine) run on the LDS-tiled
 # kernel, with partial 8 x 64 tiles, one or several input slices and one or several
-# output-channel groups; the others on the small-image kernel.
+# output-channel groups; the others on the small-image kernel.  The last four run 8-row tiles
+# with every input slice's weights resident in LDS: 32-channel groups over 2 and 4 slices,
+# 64-channel groups over 2 slices, and 32-channel groups chosen because 64-channel groups'
+# weights (128 inputs) would not fit.
 @pytest.mark.parametrize("cin,cout,h,w,relu", [(10, 32, 16, 16, 1), (32, 64, 5, 7, 0), (224, 96, 8, 12, 1),
                                                (32, 3, 9, 4, 0), (64, 128, 6, 6, 1), (160, 64, 4, 8, 0),
                                                (10, 32, 130, 150, 1), (96, 64, 136, 128, 0), (32, 3, 129, 131, 0),
-                                               (224, 96, 21, 70, 1), (96, 128, 13, 50, 0)])
+                                               (224, 96, 21, 70, 1), (96, 128, 13, 50, 0), (64, 64, 130, 260, 1),
+                                               (128, 64, 130, 200, 0), (64, 64, 256, 512, 1), (128, 64, 256, 512, 0)])
 def test_conv_forward_vs_fp64(lib, cin, cout, h, w, relu):
     rng = np.random.default_rng(cin * 1000 + cout)
     x = odn.bf16_round(rng.normal(size=(2, h, w, cin)).astype(np.float32))
