@@ -194,12 +194,12 @@ struct TileOut {
 };
 
 // Bias, ReLU and the stores of a finished tile row: lane (g, r) holds output channels
-// cBase + 16 j + 4 g .. + 3 of pixel 16 q + r of its wave's row (sBias: the block's biases).
+// cBase + 16 j + 4 g .. + 3 of pixel 16 q + r of its wave's row (biasOf[j]: their biases).
 // Each lane's byte offsets are formed once per row; pixel group q adds a fixed stride and
 // channel group j a constant.
 template <int NT>
 __device__ __forceinline__ void tileEpilogue(const ConvArgs& a, const TileOut& o, const f32x4 (&acc)[4][NT],
-                                             const float* sBias, int cBase, int x0, int y, int nn, int g, int r) {
+                                             const float4 (&biasOf)[NT], int cBase, int x0, int y, int nn, int g, int r) {
     if (y >= a.h) return;
     const int pix = (nn * a.h + y) * a.w + x0 + r;   // this lane's pixel in group q = 0
     const int left = a.w - x0 - r;                    // group q holds an image pixel while 16 q < left
@@ -215,7 +215,7 @@ __device__ __forceinline__ void tileEpilogue(const ConvArgs& a, const TileOut& o
             const unsigned outOff = in && c0 + 16 * j < a.out.c ? outBase + (unsigned)(32 * q * a.out.cs + 32 * j) : kOutside;
             const unsigned maskOff = maskBase + (unsigned)(32 * q * a.omask.cs + 32 * j);
             const unsigned fOff = in && c0 + 16 * j < a.outFStride ? fBase + (unsigned)(64 * q * a.outFStride + 64 * j) : kOutside;
-            const float4 bias = *reinterpret_cast<const float4*>(sBias + 16 * j + 4 * g);
+            const float4 bias = biasOf[j];
             float v[4] = {acc[q][j][0] + bias.x, acc[q][j][1] + bias.y, acc[q][j][2] + bias.z, acc[q][j][3] + bias.w};
             if (a.relu)
 #pragma unroll
@@ -252,6 +252,10 @@ __device__ __forceinline__ void tileEpilogue(const ConvArgs& a, const TileOut& o
 // Resident blocks per compute unit a tile kernel is built for: 4-row blocks of up to 32
 // output channels fit three (their LDS), 64-channel ones two; 8-row blocks one.
 // (The multi-slice 32-channel kernel holds a slice of weights in registers too: two blocks.)
+// Whether a tile kernel loads its halo straight into two LDS buffers (the 8-row kernels with
+// resident weights and 32 or 64 output channels; the 16-channel output layer is faster
+// staging its halo through registers).
+template <int NT, int ROWS, bool MULTI> constexpr bool tileDmaHalo() { return !MULTI && ROWS == 8 && NT >= 2; }
 template <int NT, int ROWS, bool MULTI> constexpr int tileBlocksPerCu() {
     return ROWS == 4 ? (NT == 4 || (NT == 2 && MULTI) ? 2 : 3) : 1;
 }
@@ -264,7 +268,13 @@ __global__ __launch_bounds__(64 * ROWS, (tileBlocksPerCu<NT, ROWS, MULTI>())) RT
     constexpr int kTileH = T::kH, kHaloH = T::kHaloH, kTileThreads = T::kThreads;
     constexpr int kHaloChunks = T::kHaloChunks, kHaloIters = T::kHaloIters;
     constexpr int kCout = 16 * NT, kWChunks = 9 * kCout * 4, kWIters = (kWChunks + kTileThreads - 1) / kTileThreads;
-    __shared__ __attribute__((aligned(16))) bf16 tile[kHaloH * kHaloW * 32];
+    // 8-row kernels with resident weights and 32 or 64 output channels load each step's halo
+    // straight into LDS (64 lanes x 16 bytes per instruction) into one of two buffers, the
+    // next step's while this step's taps read the other.
+    constexpr bool kDma = tileDmaHalo<NT, ROWS, MULTI>();
+    constexpr int kDmaInstr = (kHaloChunks + 63) / 64, kDmaPerWave = (kDmaInstr + ROWS - 1) / ROWS;
+    __shared__ __attribute__((aligned(16))) bf16 tile[kDma ? kDmaInstr * 512 : kHaloH * kHaloW * 32];
+    __shared__ __attribute__((aligned(16))) bf16 tile2[kDma ? kDmaInstr * 512 : 8];
     __shared__ __attribute__((aligned(16))) bf16 sWStep[MULTI ? 9 * kCout * 32 : 8];
     extern __shared__ __attribute__((aligned(16))) bf16 sWAll[];
     __shared__ __attribute__((aligned(16))) float sBias[kCout];
@@ -282,16 +292,16 @@ __global__ __launch_bounds__(64 * ROWS, (tileBlocksPerCu<NT, ROWS, MULTI>())) RT
     const TileOut o{bufferOf(a.out.p, (unsigned)(pixels * a.out.cs * 2)), bufferOf(a.omask.p, (unsigned)(pixels * a.omask.cs * 2)),
                     bufferOf(a.outF, (unsigned)(pixels * a.outFStride * 4))};
     // this thread's halo chunk `it`: pixel row and column in the halo tile
-    auto hRow = [&](int it) {
+    auto hRow = [&](int it) __attribute__((always_inline)) {
         const int v = threadIdx.x + it * kTileThreads;
         return v < kHaloChunks ? (v >> 2) / kHaloW : -1000;   // -1000: no chunk (always outside)
     };
-    auto hCol = [&](int it) { return ((int)threadIdx.x + it * kTileThreads) / 4 % kHaloW; };
+    auto hCol = [&](int it) __attribute__((always_inline)) { return ((int)threadIdx.x + it * kTileThreads) / 4 % kHaloW; };
     const int part = threadIdx.x & 3;   // every chunk index of a thread has the same low two bits
     // byte offset of chunk `it` from the tile's first pixel (channel 0 of the slice), or past any
     // buffer for a thread without that chunk (24-bit products: a halo row is under 2^24 bytes)
     const unsigned rowBytes = (unsigned)(a.w * a.in.cs * 2), pixBytes = (unsigned)(a.in.cs * 2);
-    auto rel = [&](int it) {
+    auto rel = [&](int it) __attribute__((always_inline)) {
         return threadIdx.x + it * kTileThreads < kHaloChunks
                    ? __umul24((unsigned)hRow(it), rowBytes) + __umul24((unsigned)hCol(it), pixBytes) - rowBytes - pixBytes + 16u * part
                    : kOutside;
@@ -318,7 +328,7 @@ __global__ __launch_bounds__(64 * ROWS, (tileBlocksPerCu<NT, ROWS, MULTI>())) RT
         }
     };
     // loads the halo (and for MULTI the weights) of step k into rin / rw
-    auto fetch = [&](int k) {
+    auto fetch = [&](int k) __attribute__((always_inline)) {
         const int t = blockIdx.x + (k / slices) * gridDim.x, ci0 = (k % slices) * 32;
         const int x0 = (t % tilesX) * kTileW, y0 = ((t / tilesX) % tilesY) * kTileH, nn = t / (tilesX * tilesY);
         if (ci0 < a.lowC) {   // upsampled channels: the half-resolution pixel under each halo pixel
@@ -346,6 +356,45 @@ __global__ __launch_bounds__(64 * ROWS, (tileBlocksPerCu<NT, ROWS, MULTI>())) RT
         }
         if (MULTI) fetchW(ci0, rw);
     };
+    // kDma: the halo of step k into LDS buffer dst.  Lane l of the wave's instruction i fills
+    // 16-byte slot s = 64 i + l, i.e. chunk slot s & 3 of halo pixel s / 4, so it loads the
+    // channel chunk that the swizzled layout keeps there; out-of-image and surplus lanes read
+    // past the buffer's end, which writes zeros.
+    auto dma = [&](int k, bf16* dst) __attribute__((always_inline)) {
+        const int t = blockIdx.x + (k / slices) * gridDim.x, ci0 = (k % slices) * 32;
+        const int x0 = (t % tilesX) * kTileW, y0 = ((t / tilesX) % tilesY) * kTileH, nn = t / (tilesX * tilesY);
+        // Byte offset of the tile's first pixel and the strides of one halo row and pixel; the
+        // upsampled channels step through the half-resolution tensor (tiles start on even
+        // rows and columns, so halo offset d maps to d >> 1 there).
+        const bool low = ci0 < a.lowC;
+        unsigned base, rowB, pixB;
+        __amdgpu_buffer_rsrc_t src;
+        if (low) {
+            const int wl = a.w / 2, cs = a.low.cs;
+            base = (unsigned)(((nn * (a.h / 2) + y0 / 2) * wl + x0 / 2) * cs + a.low.co + ci0) * 2;
+            rowB = (unsigned)(wl * cs * 2); pixB = (unsigned)(cs * 2);
+            src = lowBuf;
+        } else {
+            const int cs = a.in.cs;
+            base = (unsigned)(((nn * a.h + y0) * a.w + x0) * cs + a.in.co + ci0 - a.lowC) * 2;
+            rowB = (unsigned)(a.w * cs * 2); pixB = (unsigned)(cs * 2);
+            src = inBuf;
+        }
+        const int shift = low ? 1 : 0;
+#pragma unroll
+        for (int n = 0; n < kDmaPerWave; n++) {
+            const int i = wave + n * ROWS;
+            if (i < kDmaInstr) {
+                const int sl = 64 * i + lane, px = sl >> 2, hr = px / kHaloW, hc = px - hr * kHaloW;
+                const int c = (sl & 3) ^ ((hc >> 1) & 3);
+                const int yy = y0 + hr - 1, xx = x0 + hc - 1;
+                const bool in = sl < kHaloChunks && (unsigned)yy < (unsigned)a.h && (unsigned)xx < (unsigned)a.w;
+                const unsigned off = base + (unsigned)((hr - 1) >> shift) * rowB + (unsigned)((hc - 1) >> shift) * pixB + 16u * c;
+                __builtin_amdgcn_raw_ptr_buffer_load_lds(src, (__attribute__((address_space(3))) void*)(dst + i * 512), 16,
+                                                         (int)(in ? off : kOutside), 0, 0, 0);
+            }
+        }
+    };
     if (!MULTI) {   // every slice's weights, staged once (published by the first step's barrier)
         bf16x8 wa[kWIters], wb[kWIters];
         fetchW(0, wa);
@@ -358,33 +407,35 @@ __global__ __launch_bounds__(64 * ROWS, (tileBlocksPerCu<NT, ROWS, MULTI>())) RT
             }
         }
     }
-    fetch(0);
     if ((int)threadIdx.x < kCout) sBias[threadIdx.x] = a.bias ? a.bias[cBase + threadIdx.x] : 0.0f;   // read after a barrier
     f32x4 acc[4][NT];
     int px0 = 0, py0 = 0, pnn = 0;   // the tile whose results acc holds
-    for (int k = 0; k < steps; k++) {
-        const int t = blockIdx.x + (k / slices) * gridDim.x, slice = k % slices;
-        const int x0 = (t % tilesX) * kTileW, y0 = ((t / tilesX) % tilesY) * kTileH, nn = t / (tilesX * tilesY);
-        __syncthreads();   // the previous step's reads of tile / sW are done
+    // The biases of this lane's output channels: registers for the DMA kernels (an LDS read
+    // there would wait for the loads in flight into the other buffer), else read from LDS.
+    float4 biasReg[NT];
+    if (kDma)
 #pragma unroll
-        for (int it = 0; it < kHaloIters; it++) {
-            const int v = threadIdx.x + it * kTileThreads;
-            if (v < kHaloChunks) *reinterpret_cast<bf16x8*>(tile + swz(v >> 2, v & 3, hCol(it))) = rin[it];
-        }
-        if (MULTI) storeW(0, rw);
-        __syncthreads();
-        // The previous tile's stores go out here, after this step's wait for its loads and
-        // before the next step's loads: the wait at the next step (which counts stores too)
-        // then finds both done behind this step's MFMAs.
-        if (slice == 0 && k > 0) tileEpilogue<NT>(a, o, acc, sBias, cBase, px0, py0 + wave, pnn, g, r);
-        if (k + 1 < steps) fetch(k + 1);
-        if (slice == 0) {
-            px0 = x0; py0 = y0; pnn = nn;
+        for (int j = 0; j < NT; j++)
+            biasReg[j] = a.bias ? *reinterpret_cast<const float4*>(a.bias + cBase + 16 * j + 4 * g) : make_float4(0, 0, 0, 0);
+    auto epilogue = [&]() __attribute__((always_inline)) {
+        float4 biasOf[NT];
 #pragma unroll
-            for (int q = 0; q < 4; q++)
+        for (int j = 0; j < NT; j++) biasOf[j] = kDma ? biasReg[j] : *reinterpret_cast<const float4*>(sBias + 16 * j + 4 * g);
+        tileEpilogue<NT>(a, o, acc, biasOf, cBase, px0, py0 + wave, pnn, g, r);
+    };
+    // at a tile's first slice: the previous tile's results out, the accumulators cleared
+    auto beginStep = [&](int k) __attribute__((always_inline)) {
+        if (k % slices) return;
+        if (k > 0) epilogue();
+        const int t = blockIdx.x + (k / slices) * gridDim.x;
+        px0 = (t % tilesX) * kTileW; py0 = ((t / tilesX) % tilesY) * kTileH; pnn = t / (tilesX * tilesY);
 #pragma unroll
-                for (int j = 0; j < NT; j++) acc[q][j] = f32x4{0, 0, 0, 0};
-        }
+        for (int q = 0; q < 4; q++)
+#pragma unroll
+            for (int j = 0; j < NT; j++) acc[q][j] = f32x4{0, 0, 0, 0};
+    };
+    // the nine taps of one step over its halo in LDS
+    auto taps = [&](const bf16* halo, int slice) __attribute__((always_inline)) {
         const bf16* const wSlice = sW + (MULTI ? 0 : slice * (9 * kCout * 32));
 #pragma unroll RT_CONV_TAP_UNROLL
         for (int tap = 0; tap < 9; tap++) {
@@ -392,7 +443,7 @@ __global__ __launch_bounds__(64 * ROWS, (tileBlocksPerCu<NT, ROWS, MULTI>())) RT
             bf16x8 av[4];
 #pragma unroll
             for (int q = 0; q < 4; q++)
-                av[q] = *reinterpret_cast<const bf16x8*>(tile + swz((wave + dy) * kHaloW + 16 * q + r + dx, g, r + dx));
+                av[q] = *reinterpret_cast<const bf16x8*>(halo + swz((wave + dy) * kHaloW + 16 * q + r + dx, g, r + dx));
 #pragma unroll
             for (int j = 0; j < NT; j++) {
                 const bf16x8 wv = *reinterpret_cast<const bf16x8*>(wSlice + swz(tap * kCout + 16 * j + r, g, r));
@@ -400,8 +451,47 @@ __global__ __launch_bounds__(64 * ROWS, (tileBlocksPerCu<NT, ROWS, MULTI>())) RT
                 for (int q = 0; q < 4; q++) acc[q][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wv, av[q], acc[q][j], 0, 0, 0);
             }
         }
+    };
+    if constexpr (kDma) {
+        // vmcnt(0) with the other counters unconstrained: this wave's loads into LDS (and its
+        // stores) are complete
+        constexpr int kWaitLoads = 0x0F70;
+        dma(0, tile);
+        __builtin_amdgcn_s_waitcnt(kWaitLoads);
+        __syncthreads();
+        // one barrier per step: the next step's halo goes into the other buffer, whose last
+        // readers finished before the previous step's barrier
+        auto step = [&](int k, const bf16* cur, bf16* nxt) __attribute__((always_inline)) {
+            beginStep(k);
+            if (k + 1 < steps) dma(k + 1, nxt);
+            taps(cur, k % slices);
+            __builtin_amdgcn_s_waitcnt(kWaitLoads);
+            __syncthreads();
+        };
+        for (int k = 0; k < steps; k += 2) {
+            step(k, tile, tile2);
+            if (k + 1 < steps) step(k + 1, tile2, tile);
+        }
+    } else {
+        fetch(0);
+        for (int k = 0; k < steps; k++) {
+            __syncthreads();   // the previous step's reads of tile / sW are done
+#pragma unroll
+            for (int it = 0; it < kHaloIters; it++) {
+                const int v = threadIdx.x + it * kTileThreads;
+                if (v < kHaloChunks) *reinterpret_cast<bf16x8*>(tile + swz(v >> 2, v & 3, hCol(it))) = rin[it];
+            }
+            if (MULTI) storeW(0, rw);
+            __syncthreads();
+            // The previous tile's stores go out here, after this step's wait for its loads and
+            // before the next step's loads: the wait at the next step (which counts stores too)
+            // then finds both done behind this step's MFMAs.
+            beginStep(k);
+            if (k + 1 < steps) fetch(k + 1);
+            taps(tile, k % slices);
+        }
     }
-    tileEpilogue<NT>(a, o, acc, sBias, cBase, px0, py0 + wave, pnn, g, r);
+    epilogue();
 }
 
 struct BwdWArgs {
@@ -1164,8 +1254,13 @@ hipError_t launchTileKernel(const ConvArgs& a, int groups, int tilesX, int tiles
     return hipGetLastError();
 }
 
-// LDS of a tile block besides its weights: the halo slice and the biases.
-template <int NT, int ROWS> constexpr int tileStaticLds() { return TileShape<ROWS>::kHaloH * kHaloW * 64 + 64 * NT; }
+// LDS of a resident-weight tile block besides its weights: the halo slice (two buffers of
+// whole 1 KiB load instructions where the halo is loaded straight into LDS) and the biases.
+template <int NT, int ROWS> constexpr int tileStaticLds() {
+    return (tileDmaHalo<NT, ROWS, false>() ? 2 * ((TileShape<ROWS>::kHaloChunks + 63) / 64) * 1024
+                                           : TileShape<ROWS>::kHaloH * kHaloW * 64) +
+           64 * NT + 16;
+}
 
 // Whether a block's weights (all input slices) fit in LDS beside its halo at the resident-weight
 // kernel's blocks per compute unit.
