@@ -115,6 +115,23 @@ def test_inference_with_upsampling_read_in_place_vs_fp64_oracle(lib):
     assert np.abs(out - ref).max() <= 0.03 * np.abs(ref).max()
 
 
+def test_batched_inference_vs_fp64_oracle(lib):
+    """Two different images in one call (RtDenoiserInfer with n = 2): at 144 x 272 the level-0
+    layers run the 8-row kernels that load their halos straight into LDS (D0A's upsampled
+    channels included) and level 1 the 4-row kernels, so both find image 1's pixels behind
+    image 0's."""
+    rng = np.random.default_rng(23)
+    raw = np.abs(rng.normal(size=(2, 144, 272, 10))).astype(np.float32)
+    raw[1] *= 3.0
+    raw[..., 6:9] = rng.normal(size=(2, 144, 272, 3))
+    with Denoiser(lib, seed=11) as net:
+        params = net.params
+        out = net.infer(raw)
+    ref = odn.infer(odn.bf16_round(params).astype(np.float64), raw)
+    for i in range(2):
+        assert np.abs(out[i] - ref[i]).max() <= 0.03 * np.abs(ref[i]).max(), i
+
+
 def test_training_reduces_loss(lib):
     rng = np.random.default_rng(3)
     n, h, w = 6, 64, 64
