@@ -226,12 +226,16 @@ def denoise_train_timing(args):
     return 0
 
 
-def run_psnr_study(s, cam, st, w, h, reference_spp, denoised_ladder=(8, 12, 16, 20, 24, 32, 64), repeats=3):
+def run_psnr_study(s, cam, st, w, h, reference_spp, denoised_ladder=(8, 12, 16, 20, 24, 32, 64), repeats=3,
+                    ensemble_ladder=(12, 16, 20)):
     """Time to equal PSNR on an initialized session: learned denoise vs plain accumulation,
     against a reference_spp render (seed 20261023), 8-bit display PSNR as
     Tools/verify_quality.py:53-54.  Wall times include each RtRender's readback
-    (Tools/verify_quality.py:72-78) and the denoiser's transfers.  Restores nothing: the
-    session's accumulation is left at the last denoised frame."""
+    (Tools/verify_quality.py:72-78) and the denoiser's transfers.  The denoised ladder runs
+    with the session's denoise_ensemble, then ensemble_ladder with two network passes
+    (denoise_ensemble 2: the input mirrored, the outputs averaged); the 256-spp match is the
+    fastest denoised configuration that reaches plain 256-spp PSNR.  Restores the session's
+    ensemble setting; its accumulation is left at the last denoised frame."""
     import copy
     import train_denoiser as td
     lib = s.lib
@@ -285,11 +289,15 @@ def run_psnr_study(s, cam, st, w, h, reference_spp, denoised_ladder=(8, 12, 16, 
     ceiling = td.psnr(td.display(ref / (ref_albedo + eps) * (albedo4 + eps), st.exposure), ref_img)
     log(f"denoised   4 spp: {sec * 1e3:8.1f} ms, PSNR {dn_psnr:.2f} dB")
     dn_ladder = []
-    for spp in denoised_ladder:
-        el, im, _ = denoised(spp)
-        dn_ladder.append({"spp": spp, "seconds": round(el, 4), "psnr_db": round(td.psnr(im, ref_img), 3),
-                          "tiles_fallback": s.denoise_stats()[1]})
-        log(f"denoised {spp:3d} spp: {el * 1e3:8.1f} ms, PSNR {dn_ladder[-1]['psnr_db']:.2f} dB")
+    for passes, spps in ((ensemble, denoised_ladder), (2, ensemble_ladder if ensemble != 2 else ())):
+        lib.RtSetOption(s.handle, b"denoise_ensemble", passes)
+        s.denoise()   # the new setting's buffers before timing
+        for spp in spps:
+            el, im, _ = denoised(spp)
+            dn_ladder.append({"spp": spp, "ensemble": passes, "seconds": round(el, 4),
+                              "psnr_db": round(td.psnr(im, ref_img), 3), "tiles_fallback": s.denoise_stats()[1]})
+            log(f"denoised {spp:3d} spp x{passes}: {el * 1e3:8.1f} ms, PSNR {dn_ladder[-1]['psnr_db']:.2f} dB")
+    lib.RtSetOption(s.handle, b"denoise_ensemble", ensemble)
     hp, wp = (h + 15) // 16 * 16, (w + 15) // 16 * 16
     flops = lib.RtDenoiserForwardFlops(1, hp, wp)
     # equal-PSNR point on the plain ladder (linear in log2 spp, times interpolated alike)
@@ -303,7 +311,7 @@ def run_psnr_study(s, cam, st, w, h, reference_spp, denoised_ladder=(8, 12, 16, 
     if eq is None and dn_psnr > ladder[-1]["psnr_db"]:
         eq = {"spp": f">{ladder[-1]['spp']}", "seconds": ladder[-1]["seconds"]}
     p256 = next(x for x in ladder if x["spp"] == 256)
-    match = next((d for d in dn_ladder if d["psnr_db"] >= p256["psnr_db"]), None)
+    match = min((d for d in dn_ladder if d["psnr_db"] >= p256["psnr_db"]), key=lambda d: d["seconds"], default=None)
     return {"reference_spp": reference_spp, "reference_seconds": round(ref_s, 2), "denoise_ensemble": ensemble,
             "plain": ladder,
             "denoised_4spp": {"seconds": round(sec, 4), "psnr_db": round(dn_psnr, 3),
@@ -317,7 +325,7 @@ def run_psnr_study(s, cam, st, w, h, reference_spp, denoised_ladder=(8, 12, 16, 
             "vs_256spp": {"psnr_gap_db": round(dn_psnr - p256["psnr_db"], 3),
                           "time_ratio": round(p256["seconds"] / sec, 2)},
             "denoised_ladder": dn_ladder,
-            "match_256spp": ({"denoised_spp": match["spp"], "seconds": match["seconds"],
+            "match_256spp": ({"denoised_spp": match["spp"], "ensemble": match["ensemble"], "seconds": match["seconds"],
                               "time_ratio": round(p256["seconds"] / match["seconds"], 2)} if match else None)}
 
 
